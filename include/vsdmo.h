@@ -1,0 +1,162 @@
+/*
+ * vsdmo.h -- C ABI of libvsdmo, the MI355X-native discrete mesh optimisation
+ * (DMO) library.  Plain C types only: host pointers, sizes and opaque handles.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository, VSDMO/src/...).  Errors are returned as
+ * vsdmo_status; a thread-local message is available from vsdmo_last_error().
+ * No call exits the process or throws.
+ *
+ * Ownership: a vsdmo_mesh owns its device buffers; a vsdmo_opt references a
+ * mesh and mutates its vertex positions in place.  A mesh cannot be destroyed
+ * while an optimiser references it (VSDMO_INVALID_ARG).  Handles are not
+ * thread-safe; use one handle per host thread or lock externally.  All calls
+ * are blocking unless stated otherwise.
+ */
+#ifndef VSDMO_H
+#define VSDMO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VSDMO_ABI_VERSION 1
+
+typedef enum vsdmo_status {
+    VSDMO_OK = 0,
+    VSDMO_INVALID_ARG = 1,
+    VSDMO_HIP_ERROR = 2,
+    VSDMO_TOPOLOGY = 3,          /* non-manifold input, CCW walk failure */
+    VSDMO_QUALITY_DECREASED = 4, /* monotone-quality guard fired */
+    VSDMO_LDS_OVERFLOW = 5,      /* one-ring larger than the kernels support */
+    VSDMO_NOT_SUPPORTED = 6
+} vsdmo_status;
+
+typedef enum vsdmo_elem { VSDMO_TRI = 0, VSDMO_QUAD = 1, VSDMO_TET = 2, VSDMO_HEX = 3 } vsdmo_elem;
+
+/* reference QualityCriterium (dmo/DMOQualityUtil.h:10-18); MEAN_RATIO is the
+   supported objective (for hexes the reference's shapeMetricHex) */
+typedef enum vsdmo_criterion { VSDMO_MEAN_RATIO = 0 } vsdmo_criterion;
+
+typedef enum vsdmo_coloring {
+    VSDMO_COLOR_GREEDY = 0,  /* first-fit in vertex order via Jones-Plassmann rounds, priority n-i */
+    VSDMO_COLOR_JPL = 1      /* reference color_jpl (mesh/VertexColoring.cu:76-125): colour = round */
+} vsdmo_coloring;
+
+typedef enum vsdmo_extent {
+    VSDMO_EXTENT_AUTO = -1,  /* reference choice: max for tet/tri, min for hex */
+    VSDMO_EXTENT_MAX = 0,
+    VSDMO_EXTENT_MIN = 1
+} vsdmo_extent;
+
+typedef struct vsdmo_mesh vsdmo_mesh;
+typedef struct vsdmo_opt vsdmo_opt;
+
+typedef struct vsdmo_mesh_params {
+    int32_t coloring;          /* vsdmo_coloring */
+    float feature_angle_deg;   /* 20 (reference mesh/MeshTriGPU.cu:506) */
+} vsdmo_mesh_params;
+
+typedef struct vsdmo_params {
+    int32_t nq;                /* candidates per axis, 2..8 (reference DMO_NQ = 8) */
+    int32_t depth;             /* refinement levels (reference DMO_DEPTH = 3) */
+    float shrink;              /* per-level window factor; 0 = 2/(nq-1) as the reference */
+    int32_t extent_mode;       /* vsdmo_extent */
+    int32_t fixed_boundary;    /* 1: surface and feature vertices are not moved */
+    int32_t quality_guard;     /* 1: fail with VSDMO_QUALITY_DECREASED if min quality drops */
+    float guard_tolerance;     /* relative slack of the guard (FP32 frame differences) */
+} vsdmo_params;
+
+typedef struct vsdmo_counts {
+    int32_t n_vertices, n_elements, n_surface, n_free, n_feature, n_inner;
+    int32_t n_colors_inner, n_colors_free, n_colors_feature;
+    int32_t n_faces, n_halfedges, max_ring, max_halfedges, flat;
+} vsdmo_counts;
+
+typedef struct vsdmo_stats {
+    int64_t searches_inner, searches_surface, searches_feature;
+    int32_t iterations;
+    float min_quality;
+    double kernel_ms;          /* summed device time of the search kernels (timing on) */
+    int64_t kernel_launches;
+} vsdmo_stats;
+
+/* defaults: greedy colouring, 20 degrees / nq 8, depth 3, reference shrink,
+   auto extent, boundary moved (reference behaviour), guard on */
+void vsdmo_mesh_params_default(vsdmo_mesh_params* p);
+void vsdmo_params_default(vsdmo_params* p);
+
+/* Replaces DMOMesh{Tri,Tet,Hex}Factory::create (mesh/MeshFactory.h:17-41,
+ * mesh/MeshFactory.cu:3-111) together with Mesh{Tri,Tet,Hex}GPU::init
+ * (mesh/MeshTriGPU.cu:37-52, mesh/MeshTetGPU.cu:35-42, mesh/MeshHexGPU.cu:35-43).
+ * xyz: 3*nV floats; cells: npe*nCells int32 (tri 3, tet 4, hex 8; hexes in the
+ * reference's internal order, top face first).  Host arrays are copied in and
+ * the mesh structures are built on the GPU.  boundary_or_null must be NULL
+ * (boundary detection from the elements, reference findBoundary2D=true). */
+vsdmo_status vsdmo_mesh_create(vsdmo_elem type, const float* xyz, int32_t nV, const int32_t* cells,
+                               int32_t nCells, const uint8_t* boundary_or_null,
+                               const vsdmo_mesh_params* params, vsdmo_mesh** out);
+/* Replaces the DMOMesh* destructors (mesh/DMOMeshBase.h:132-142) */
+vsdmo_status vsdmo_mesh_destroy(vsdmo_mesh* mesh);
+
+/* Replaces the public counters of DMOMeshBase / DMOMeshTet (mesh/DMOMeshBase.h:80-172) */
+vsdmo_status vsdmo_mesh_counts(const vsdmo_mesh* mesh, vsdmo_counts* out);
+/* positions: 3*nV floats; original_order=1 returns them indexed by input id
+   (the reference writes files in the renumbered order, io/FileWriter.cu:64-102) */
+vsdmo_status vsdmo_mesh_get_positions(const vsdmo_mesh* mesh, float* out, int32_t original_order);
+vsdmo_status vsdmo_mesh_set_positions(vsdmo_mesh* mesh, const float* in, int32_t original_order);
+/* new_of_old[nV]: renumbering applied by the build (mesh/SortUtil.cu:103-107) */
+vsdmo_status vsdmo_mesh_get_permutation(const vsdmo_mesh* mesh, int32_t* new_of_old);
+/* elements in the renumbered ids, npe*nE */
+vsdmo_status vsdmo_mesh_get_cells(const vsdmo_mesh* mesh, int32_t* out);
+/* colour class offsets (absolute ids); which: 0 inner, 1 free surface, 2 feature
+   (reference colorOffsets*, mesh/DMOMeshBase.h:80-172); out has n_colors+1 */
+vsdmo_status vsdmo_mesh_get_coloring(const vsdmo_mesh* mesh, int32_t which, int32_t* offsets);
+/* per-vertex colour within its class family, nV */
+vsdmo_status vsdmo_mesh_get_colors(const vsdmo_mesh* mesh, int32_t* colors);
+/* boundary faces of a volume mesh (tri/quad, renumbered ids), fpe*n_faces */
+vsdmo_status vsdmo_mesh_get_faces(const vsdmo_mesh* mesh, int32_t* out);
+/* volume one-ring CSR (halffaces / halfhexes, mesh/MeshTetGPU.cu:113-142,
+   mesh/MeshHexGPU.cu:116-139): row_ptr nV+1, values stride*row_ptr[nV] */
+vsdmo_status vsdmo_mesh_get_onering(const vsdmo_mesh* mesh, int32_t* row_ptr, int32_t* values);
+/* surface halfedge CSR over the surface vertices (mesh/MeshTypes.h:6-11,
+   mesh/MeshTriGPU.cu:128-214): row_ptr n_surface+1; target/opposite/face/next n_halfedges */
+vsdmo_status vsdmo_mesh_get_halfedges(const vsdmo_mesh* mesh, int32_t* row_ptr, int32_t* target,
+                                      int32_t* opposite, int32_t* face, int32_t* next);
+/* per surface vertex flags (boundary of an open surface, feature) */
+vsdmo_status vsdmo_mesh_get_flags(const vsdmo_mesh* mesh, uint8_t* boundary1d, uint8_t* feature);
+
+/* Replaces DMO::DMO{Tet,Hex,TriFlat}Class(mesh, criterion, gridScale, n_iter)
+ * (dmo/DMOTet.h:24, dmo/DMOHex.h:18, dmo/DMOTriFlat.h:25) and their init()
+ * (dmo/DMOTet.cu:92-125).  params may be NULL (defaults). */
+vsdmo_status vsdmo_opt_create(vsdmo_mesh* mesh, int32_t criterion, float grid_scale, int32_t n_iter,
+                              const vsdmo_params* params, vsdmo_opt** out);
+/* Replaces DMOBaseClass::doIteration (dmo/DMOTet.cu:128-170, dmo/DMOHex.cu:131-182,
+ * dmo/DMOTriFlat.cu:129-164), n times */
+vsdmo_status vsdmo_opt_iterate(vsdmo_opt* opt, int32_t n);
+/* Replaces DMOBaseClass::optimize (dmo/DMOBase.h:44-52): iterate until n_iter */
+vsdmo_status vsdmo_opt_optimize(vsdmo_opt* opt);
+/* Replaces DMOBaseClass::isDone */
+int32_t vsdmo_opt_is_done(const vsdmo_opt* opt);
+/* Replaces DMOBaseClass::getQuality (last minimum element quality, FP32) */
+vsdmo_status vsdmo_opt_min_quality(const vsdmo_opt* opt, float* out);
+/* mean element quality (FP64 sum of FP32 qualities, fixed reduction order) */
+vsdmo_status vsdmo_opt_mean_quality(const vsdmo_opt* opt, double* out);
+/* Replaces getQualityHistogram (dmo/DMOTet.cu:202-212): n_cols bins on [0,1) */
+vsdmo_status vsdmo_opt_histogram(const vsdmo_opt* opt, int32_t n_cols, int32_t* out);
+/* Replaces getElementQualities (dmo/DMOTet.cu:235-240), into host memory */
+vsdmo_status vsdmo_opt_element_qualities(const vsdmo_opt* opt, float* host_out);
+/* counters and timing; set_timing(1) records device time of every search launch */
+vsdmo_status vsdmo_opt_stats(const vsdmo_opt* opt, vsdmo_stats* out);
+vsdmo_status vsdmo_opt_set_timing(vsdmo_opt* opt, int32_t on);
+vsdmo_status vsdmo_opt_destroy(vsdmo_opt* opt);
+
+/* thread-local message of the last failing call on this thread */
+const char* vsdmo_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

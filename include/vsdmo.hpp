@@ -1,0 +1,156 @@
+// vsdmo.hpp -- header-only C++ drop-in over the C ABI (include/vsdmo.h) that
+// keeps the reference's mesh / optimiser API shape:
+//   DMOMeshTetFactory::create(...) -> std::shared_ptr<DMOMeshTet>
+//       (reference VSDMO/src/mesh/MeshFactory.h:17-41)
+//   DMO::DMOTetClass(mesh, QualityCriterium, gridScale, n_iter) with
+//       optimize(), doIteration(), isDone(), getQuality(),
+//       getQualityHistogram(), getElementQualities(), displayQualityGPU()
+//       (reference VSDMO/src/dmo/DMOBase.h:23-88, DMOTet.h:24)
+// The file-name overloads of the reference factories read OFF / TetGen / MEDIT
+// files; this build takes host arrays (file I/O is not part of it).
+// Errors become std::runtime_error carrying vsdmo_last_error().
+#pragma once
+
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vsdmo.h"
+
+namespace vsdmo_cpp {
+
+inline void check(vsdmo_status s, const char* where)
+{
+    if (s != VSDMO_OK) throw std::runtime_error(std::string(where) + ": " + vsdmo_last_error());
+}
+
+class DMOMesh {
+public:
+    DMOMesh(vsdmo_elem type, const std::vector<float>& xyz, const std::vector<int32_t>& cells, int npe,
+            const vsdmo_mesh_params* prm = nullptr)
+    {
+        check(vsdmo_mesh_create(type, xyz.data(), (int32_t)(xyz.size() / 3), cells.data(),
+                                (int32_t)(cells.size() / npe), nullptr, prm, &h_),
+              "vsdmo_mesh_create");
+    }
+    ~DMOMesh() { if (h_) vsdmo_mesh_destroy(h_); }
+    DMOMesh(const DMOMesh&) = delete;
+    DMOMesh& operator=(const DMOMesh&) = delete;
+    vsdmo_mesh* handle() const { return h_; }
+    vsdmo_counts counts() const
+    {
+        vsdmo_counts c{};
+        check(vsdmo_mesh_counts(h_, &c), "vsdmo_mesh_counts");
+        return c;
+    }
+    int nVertices() const { return counts().n_vertices; }
+    std::vector<float> positions(bool original_order = true) const
+    {
+        std::vector<float> out(3 * (size_t)nVertices());
+        check(vsdmo_mesh_get_positions(h_, out.data(), original_order), "vsdmo_mesh_get_positions");
+        return out;
+    }
+
+private:
+    vsdmo_mesh* h_ = nullptr;
+};
+
+using DMOMeshTri = DMOMesh;
+using DMOMeshTet = DMOMesh;
+using DMOMeshHex = DMOMesh;
+
+struct DMOMeshTriFactory {
+    static std::shared_ptr<DMOMeshTri> create(const std::vector<float>& xyz, const std::vector<int32_t>& tris)
+    {
+        return std::make_shared<DMOMeshTri>(VSDMO_TRI, xyz, tris, 3);
+    }
+};
+struct DMOMeshTetFactory {
+    static std::shared_ptr<DMOMeshTet> create(const std::vector<float>& xyz, const std::vector<int32_t>& tets)
+    {
+        return std::make_shared<DMOMeshTet>(VSDMO_TET, xyz, tets, 4);
+    }
+};
+struct DMOMeshHexFactory {
+    static std::shared_ptr<DMOMeshHex> create(const std::vector<float>& xyz, const std::vector<int32_t>& hexes)
+    {
+        return std::make_shared<DMOMeshHex>(VSDMO_HEX, xyz, hexes, 8);
+    }
+};
+
+}  // namespace vsdmo_cpp
+
+namespace DMO {
+
+enum class QualityCriterium { MEAN_RATIO = 0 };
+
+class DMOBaseClass {
+public:
+    DMOBaseClass(vsdmo_cpp::DMOMesh& mesh, QualityCriterium q = QualityCriterium::MEAN_RATIO, float gridScale = 0.5f,
+                 int n_iter = 100, const vsdmo_params* prm = nullptr)
+        : mesh_(mesh)
+    {
+        vsdmo_cpp::check(vsdmo_opt_create(mesh.handle(), (int32_t)q, gridScale, n_iter, prm, &h_), "vsdmo_opt_create");
+    }
+    virtual ~DMOBaseClass() { if (h_) vsdmo_opt_destroy(h_); }
+    DMOBaseClass(const DMOBaseClass&) = delete;
+    DMOBaseClass& operator=(const DMOBaseClass&) = delete;
+
+    void optimize() { vsdmo_cpp::check(vsdmo_opt_optimize(h_), "vsdmo_opt_optimize"); }
+    virtual void doIteration() { vsdmo_cpp::check(vsdmo_opt_iterate(h_, 1), "vsdmo_opt_iterate"); }
+    bool isDone() const { return vsdmo_opt_is_done(h_) != 0; }
+    float getQuality() const
+    {
+        float q = 0.0f;
+        vsdmo_cpp::check(vsdmo_opt_min_quality(h_, &q), "vsdmo_opt_min_quality");
+        return q;
+    }
+    void getQualityHistogram(std::vector<int>& vec, int n_cols)
+    {
+        vec.assign(n_cols, 0);
+        vsdmo_cpp::check(vsdmo_opt_histogram(h_, n_cols, vec.data()), "vsdmo_opt_histogram");
+    }
+    // host buffer of n_elements floats (the reference fills a device buffer)
+    void getElementQualities(void* outFloatBuffer)
+    {
+        vsdmo_cpp::check(vsdmo_opt_element_qualities(h_, (float*)outFloatBuffer), "vsdmo_opt_element_qualities");
+    }
+    void displayQualityGPU(int n_cols = 10)
+    {
+        std::vector<int> h;
+        getQualityHistogram(h, n_cols);
+        std::printf("interval ");
+        for (int i = 0; i < n_cols; ++i) std::printf("%.2f ", (float)i / n_cols);
+        std::printf("\namount   ");
+        for (int v : h) std::printf("%d ", v);
+        std::printf("\nq_min = %g\n", getQuality());
+    }
+    vsdmo_opt* handle() const { return h_; }
+
+protected:
+    vsdmo_cpp::DMOMesh& mesh_;
+    vsdmo_opt* h_ = nullptr;
+};
+
+class DMOTetClass : public DMOBaseClass {
+public:
+    DMOTetClass(vsdmo_cpp::DMOMeshTet& m, QualityCriterium q = QualityCriterium::MEAN_RATIO, float gridScale = 0.5f,
+                int n_iter = 30, const vsdmo_params* prm = nullptr)
+        : DMOBaseClass(m, q, gridScale, n_iter, prm) {}
+};
+class DMOHexClass : public DMOBaseClass {
+public:
+    DMOHexClass(vsdmo_cpp::DMOMeshHex& m, QualityCriterium q = QualityCriterium::MEAN_RATIO, float gridScale = 0.5f,
+                int n_iter = 30, const vsdmo_params* prm = nullptr)
+        : DMOBaseClass(m, q, gridScale, n_iter, prm) {}
+};
+class DMOTriFlatClass : public DMOBaseClass {
+public:
+    DMOTriFlatClass(vsdmo_cpp::DMOMeshTri& m, QualityCriterium q = QualityCriterium::MEAN_RATIO,
+                    float gridScale = 0.5f, int n_iter = 30, const vsdmo_params* prm = nullptr)
+        : DMOBaseClass(m, q, gridScale, n_iter, prm) {}
+};
+
+}  // namespace DMO

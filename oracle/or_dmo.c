@@ -1,0 +1,261 @@
+/*
+ * or_dmo.c -- oracle restatement of the DMO vertex search (TEST
+ * INFRASTRUCTURE).
+ *
+ * Volume inner search: reference k_optimizeHierarchicalInnerTet / Hex
+ *   (DMOTetImplCub.h:20-137, DMOHexImplCub.h:20-137):
+ *   - window half-extent per axis from the one-ring: max |ring - x| (tet) or
+ *     min |ring - x| (hex, DMOHexImplCub.h:87-88), taken once at x0;
+ *   - level d: grid p_i = aff*(i*gmin + (nq-1-i)*gmax), gmax/gmin =
+ *     curr +- s_d*extent, s_0 = grid_scale, s_{d+1} = s_d*shrink (2/(nq-1));
+ *   - candidate index n = i + nq*j + nq^2*k, the lowest index wins ties
+ *     (cub::ArgMax keeps the lower key; each thread owns a contiguous range);
+ *   - the vertex moves to the level's best candidate iff it is strictly better
+ *     than the current position; the position is written iff it moved (the
+ *     reference's cumulative per-thread qMax with "qLast < qMax" and the final
+ *     "qOld < qMax" reduce to exactly this).
+ * Flat 2D search: reference k_optimizeHierarchical2DFlat
+ *   (DMOTriFlatImplCub.h:23-152): thread t of nq^2/2 owns candidates
+ *   A = (t/nq, t%nq) and B = (t/nq + nq/2, t%nq); per-level max resets; the
+ *   thread picks A iff q_A > q_B (B on an exact tie); the lowest thread with the
+ *   block maximum wins; the final write compares q(old) with the LAST level's
+ *   maximum.
+ */
+#include "vsdmo_oracle.h"
+#include "or_internal.h"
+#include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXR 512
+
+static inline float invpow(float sigma, int k)
+{
+    float is = 1.0f / sigma, r = 1.0f;
+    for (int i = 0; i < k; ++i) r = r * is;
+    return r;
+}
+
+static inline float shrink_of(const or_dmo_params* p)
+{
+    return (p->shrink > 0.0f) ? p->shrink : 2.0f / (float)(p->nq - 1);
+}
+
+/* grid coordinate (reference DMOTetImplCub.h:101-103) */
+static inline float gridc(float aff, int i, int nq, float gmin, float gmax)
+{
+    return aff * ((float)i * gmin + (float)(nq - 1 - i) * gmax);
+}
+
+static int vertex_volume(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz)
+{
+    int tet = (m->elem == OR_TET);
+    int st = m->ringStride;
+    int r0 = m->ringRow[vid], n = m->ringRow[vid + 1] - r0;
+    if (n <= 0 || n > (tet ? MAXR : 64)) return 0;
+    v3 x0 = ld3(xyz + 3 * (size_t)vid);
+    float ext[3] = { p->extent_mode == OR_EXTENT_MIN ? FLT_MAX : -FLT_MAX,
+                     p->extent_mode == OR_EXTENT_MIN ? FLT_MAX : -FLT_MAX,
+                     p->extent_mode == OR_EXTENT_MIN ? FLT_MAX : -FLT_MAX };
+    float mabs = 0.0f;
+    for (int e = 0; e < n; ++e)
+        for (int k = 0; k < st; ++k) {
+            const float* q = xyz + 3 * (size_t)m->ringVal[(size_t)(r0 + e) * st + k];
+            float d[3] = { fabsf(q[0] - x0.x), fabsf(q[1] - x0.y), fabsf(q[2] - x0.z) };
+            for (int a = 0; a < 3; ++a) {
+                ext[a] = (p->extent_mode == OR_EXTENT_MIN) ? fminf(ext[a], d[a]) : fmaxf(ext[a], d[a]);
+                mabs = fmaxf(mabs, d[a]);
+            }
+        }
+    float sigma = or_pow2_scale(mabs);
+    float s2 = sigma * sigma, s3 = s2 * sigma;
+    float is3 = invpow(sigma, 3);
+
+    tetc tcs[MAXR];
+    float cd[4 * 64], cF[4 * 64];
+    hexp hp[5 * 64];
+    hexacc hacc = hex_acc_init();
+    if (tet) {
+        for (int e = 0; e < n; ++e) {
+            const int* h = m->ringVal + (size_t)(r0 + e) * 3;
+            tcs[e] = tet_consts(loc3(xyz + 3 * (size_t)h[0], x0, sigma), loc3(xyz + 3 * (size_t)h[1], x0, sigma),
+                                loc3(xyz + 3 * (size_t)h[2], x0, sigma));
+        }
+    } else {
+        for (int e = 0; e < n; ++e) {
+            const int* h = m->ringVal + (size_t)(r0 + e) * 7;
+            v3 r[8];
+            r[0] = x0;
+            for (int k = 1; k < 8; ++k) r[k] = loc3(xyz + 3 * (size_t)h[k - 1], x0, sigma);
+            hex_consts(r, cd + 4 * e, cF + 4 * e, hp + 5 * e);
+        }
+        hacc = hex_const_acc(n, cd, cF);
+    }
+    const float epsD = HEX_EPS * s3, epsF = HEX_EPS * s2;
+#define SCORE(px, py, pz) (tet ? tet_score(n, tcs, (px), (py), (pz), is3) : hex_score(n, hacc, hp, (px), (py), (pz), epsD, epsF))
+
+    const int nq = p->nq;
+    const float aff = 1.0f / (float)(nq - 1);
+    const float shr = shrink_of(p);
+    float cur[3] = { x0.x, x0.y, x0.z };
+    float qcur = SCORE(0.0f, 0.0f, 0.0f);
+    float ds = p->grid_scale;
+    int moved = 0;
+    for (int d = 0; d < p->depth; ++d) {
+        float gmax[3], gmin[3];
+        for (int a = 0; a < 3; ++a) { gmax[a] = cur[a] + ds * ext[a]; gmin[a] = cur[a] - ds * ext[a]; }
+        float best = -FLT_MAX, bp[3] = { 0, 0, 0 };
+        for (int k = 0; k < nq; ++k) {
+            float pzg = gridc(aff, k, nq, gmin[2], gmax[2]);
+            float pz = (pzg - x0.z) * sigma;
+            for (int j = 0; j < nq; ++j) {
+                float pyg = gridc(aff, j, nq, gmin[1], gmax[1]);
+                float py = (pyg - x0.y) * sigma;
+                for (int i = 0; i < nq; ++i) {
+                    float pxg = gridc(aff, i, nq, gmin[0], gmax[0]);
+                    float px = (pxg - x0.x) * sigma;
+                    float q = SCORE(px, py, pz);
+                    if (q > best) { best = q; bp[0] = pxg; bp[1] = pyg; bp[2] = pzg; }
+                }
+            }
+        }
+        if (qcur < best) { cur[0] = bp[0]; cur[1] = bp[1]; cur[2] = bp[2]; qcur = best; moved = 1; }
+        ds = ds * shr;
+    }
+#undef SCORE
+    if (moved) { xyz[3 * (size_t)vid] = cur[0]; xyz[3 * (size_t)vid + 1] = cur[1]; xyz[3 * (size_t)vid + 2] = cur[2]; }
+    return moved;
+}
+
+static int vertex_flat(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz)
+{
+    int h0 = m->heRow[vid], n = m->heRow[vid + 1] - h0;
+    if (n <= 0 || n >= MAXR) return 0;
+    float x0[3] = { xyz[3 * (size_t)vid], xyz[3 * (size_t)vid + 1], xyz[3 * (size_t)vid + 2] };
+    float ext[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+    float mabs = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        const float* q = xyz + 3 * (size_t)m->heTarget[h0 + i];
+        for (int a = 0; a < 3; ++a) ext[a] = fmaxf(ext[a], fabsf(q[a] - x0[a]));
+        mabs = fmaxf(mabs, fmaxf(fabsf(q[0] - x0[0]), fabsf(q[1] - x0[1])));
+    }
+    float sigma = or_pow2_scale(mabs);
+    float is2 = invpow(sigma, 2);
+    /* closed fan: elements (ring[k], ring[k+1]), ring[n] = ring[0] */
+    tric cs[MAXR];
+    for (int e = 0; e < n; ++e) {
+        const float* a = xyz + 3 * (size_t)m->heTarget[h0 + e];
+        const float* b = xyz + 3 * (size_t)m->heTarget[h0 + (e + 1) % n];
+        cs[e] = tri_consts((a[0] - x0[0]) * sigma, (a[1] - x0[1]) * sigma, (b[0] - x0[0]) * sigma, (b[1] - x0[1]) * sigma);
+    }
+    const int nq = p->nq, half = nq / 2, T = nq * nq / 2;
+    const float aff = 1.0f / (float)(nq - 1);
+    const float shr = shrink_of(p);
+    float cur[2] = { x0[0], x0[1] };
+    float qcur = tri_score(n, cs, 0.0f, 0.0f, is2);
+    const float qold = qcur;
+    float qmaxLast = -FLT_MAX;
+    float ds = p->grid_scale;
+    for (int d = 0; d < p->depth; ++d) {
+        float gmax[2], gmin[2];
+        for (int a = 0; a < 2; ++a) { gmax[a] = cur[a] + ds * ext[a]; gmin[a] = cur[a] - ds * ext[a]; }
+        float best = -FLT_MAX, bp[2] = { 0, 0 };
+        for (int t = 0; t < T; ++t) {
+            int i1 = t / nq, j1 = t % nq, i2 = i1 + half;
+            float ax = gridc(aff, i1, nq, gmin[0], gmax[0]), ay = gridc(aff, j1, nq, gmin[1], gmax[1]);
+            float bx = gridc(aff, i2, nq, gmin[0], gmax[0]), by = ay;
+            float q1 = tri_score(n, cs, (ax - x0[0]) * sigma, (ay - x0[1]) * sigma, is2);
+            float q2 = tri_score(n, cs, (bx - x0[0]) * sigma, (by - x0[1]) * sigma, is2);
+            float qm = (q2 > q1) ? q2 : q1;
+            if (qm > best) {
+                best = qm;
+                if (q1 > q2) { bp[0] = ax; bp[1] = ay; } else { bp[0] = bx; bp[1] = by; }
+            }
+        }
+        if (qcur < best) { cur[0] = bp[0]; cur[1] = bp[1]; qcur = best; }
+        qmaxLast = best;
+        ds = ds * shr;
+    }
+    if (qold < qmaxLast) { xyz[3 * (size_t)vid] = cur[0]; xyz[3 * (size_t)vid + 1] = cur[1]; return 1; }
+    return 0;
+}
+
+int or_dmo_vertex(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz)
+{
+    if (m->elem == OR_TET || m->elem == OR_HEX) return vertex_volume(m, p, vid, xyz);
+    if (m->elem == OR_TRI) return vertex_flat(m, p, vid, xyz);
+    return 0;
+}
+
+long long or_dmo_iteration(or_mesh* m, const or_dmo_params* p)
+{
+    long long cnt = 0;
+#ifdef _OPENMP
+    int nt = p->nthreads > 0 ? p->nthreads : omp_get_max_threads();
+#endif
+    if (m->elem == OR_TET || m->elem == OR_HEX) {
+        for (int c = 0; c < m->nColorsInner; ++c) {
+            int lo = m->colOffInner[c], hi = m->colOffInner[c + 1];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+            for (int v = lo; v < hi; ++v) vertex_volume(m, p, v, m->xyz);
+            cnt += hi - lo;
+        }
+    } else if (m->elem == OR_TRI && m->flat) {
+        for (int c = 0; c < m->nColorsFree; ++c) {
+            int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+            for (int v = lo; v < hi; ++v) vertex_flat(m, p, v, m->xyz);
+            cnt += hi - lo;
+        }
+    }
+    return cnt;
+}
+
+/* ------------------------------------------------------------- quality */
+static void elem_pts(const or_mesh* m, int e, float* pts)
+{
+    for (int i = 0; i < m->npe; ++i) {
+        const float* q = m->xyz + 3 * (size_t)m->cells[(size_t)e * m->npe + i];
+        pts[3 * i] = q[0]; pts[3 * i + 1] = q[1]; pts[3 * i + 2] = q[2];
+    }
+}
+
+void or_element_qualities(const or_mesh* m, float* out)
+{
+    for (int e = 0; e < m->nE; ++e) {
+        float pts[24];
+        elem_pts(m, e, pts);
+        out[e] = (m->elem == OR_TET) ? or_q_tet(pts) : (m->elem == OR_HEX) ? or_q_hex(pts) : or_q_tri2d(pts);
+    }
+}
+
+float or_min_quality(const or_mesh* m)
+{
+    float q = FLT_MAX;
+    for (int e = 0; e < m->nE; ++e) {
+        float pts[24];
+        elem_pts(m, e, pts);
+        float v = (m->elem == OR_TET) ? or_q_tet(pts) : (m->elem == OR_HEX) ? or_q_hex(pts) : or_q_tri2d(pts);
+        q = fminf(q, v);
+    }
+    return q;
+}
+
+void or_quality64(const or_mesh* m, double* qmin, double* qmean)
+{
+    double mn = 1e300, sum = 0.0;
+    for (int e = 0; e < m->nE; ++e) {
+        float pts[24];
+        elem_pts(m, e, pts);
+        double v = (m->elem == OR_TET) ? or_q_tet64(pts) : (m->elem == OR_HEX) ? or_q_hex64(pts) : or_q_tri2d64(pts);
+        if (v < mn) mn = v;
+        sum += v;
+    }
+    *qmin = mn;
+    *qmean = m->nE > 0 ? sum / m->nE : 0.0;
+}

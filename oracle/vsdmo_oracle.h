@@ -1,0 +1,137 @@
+/*
+ * vsdmo_oracle.h -- CPU oracle for the DMO hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is a plain-C restatement of the reference's discrete mesh optimisation
+ * (DMO) algorithm, used exclusively by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  Nothing in the product links,
+ * loads or calls it.
+ *
+ * Reference files restated (paths relative to the reference repository):
+ *   mesh build  : VSDMO/src/mesh/MeshTetGPU.cu, MeshHexGPU.cu, MeshTriGPU.cu,
+ *                 MeshGPUCommon.cu, SortUtil.cu, VertexColoring.cu
+ *   search      : VSDMO/src/dmo/DMOTetImplCub.h, DMOHexImplCub.h,
+ *                 DMOTriFlatImplCub.h
+ *   metrics     : VSDMO/src/dmo/DMOQualityTet.h, DMOQualityHex.h,
+ *                 DMOQualityTri.h, DMOQualityUtil.h
+ *   drivers     : VSDMO/src/dmo/DMOTet.cu, DMOHex.cu, DMOTriFlat.cu
+ *
+ * Canonical FP32 evaluation order.  The reference's own FP32 bits are not
+ * reproducible (nvcc contracts mul+add into FMA at its discretion and the
+ * reference calls CUDA's cbrtf).  The oracle therefore fixes ONE evaluation
+ * order -- the same mathematical quantities, with every rounding step written
+ * out (contraction off, explicit fmaf) -- and the HIP kernels reproduce it bit
+ * for bit.  The order is documented next to each function below.  Every
+ * function is additionally checked against closed forms and FP64 re-evaluation
+ * in tests/.
+ */
+#ifndef VSDMO_ORACLE_H
+#define VSDMO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_TRI = 0, OR_QUAD = 1, OR_TET = 2, OR_HEX = 3 };
+enum { OR_COLOR_GREEDY = 0, OR_COLOR_JPL = 1 };
+enum { OR_EXTENT_MAX = 0, OR_EXTENT_MIN = 1 };
+
+/* ---- canonical scalar primitives ---- */
+float or_cbrt(float x);                 /* own cube root (Newton on x^-1/3) */
+float or_pow2_scale(float m);           /* 2^-e with m = f*2^e, f in [1,2)   */
+
+/* ---- element qualities (canonical FP32, vertex-0 frame) ---- */
+float or_q_tet(const float* p4x3);      /* mean ratio, signed volume if inverted */
+float or_q_hex(const float* p8x3);      /* 9-sub-tet shape metric, internal order */
+float or_q_tri2d(const float* p3x3);    /* 2D mean ratio (x,y), signed if inverted */
+/* ---- FP64 literal re-evaluation (reference formulas in double) ---- */
+double or_q_tet64(const float* p4x3);
+double or_q_hex64(const float* p8x3);
+double or_q_tri2d64(const float* p3x3);
+
+/* ---- ring scores (what the search maximises) ---- */
+/* ring: nElem elements of 3 (tet), 7 (hex) or 2 (tri fan pair) points, each
+   point xyz; x0: frame origin (vertex position at search start); p: candidate */
+float or_ring_score_tet(int nElem, const float* ring, const float* x0, float sigma, const float* p);
+float or_ring_score_hex(int nElem, const float* ring, const float* x0, float sigma, const float* p);
+float or_ring_score_tri2d(int nElem, const float* ring, const float* x0, float sigma, const float* p);
+
+/* ---- mesh build ---- */
+typedef struct or_mesh {
+    int elem;            /* OR_TRI / OR_QUAD / OR_TET / OR_HEX */
+    int npe;             /* vertices per element */
+    int nV, nE;
+    int flat;            /* tri meshes: 1 if all z == 0 */
+    float* xyz;          /* 3*nV, final (renumbered) order */
+    int* cells;          /* npe*nE, final ids */
+    int* new_of_old;     /* nV */
+    int* old_of_new;     /* nV */
+    /* vertex ranges: [0,nFree) free surface | [nFree,nSurf) feature | [nSurf,nV) inner */
+    int nSurf, nFeature, nFree, nInner;
+    int nColorsInner, nColorsFree, nColorsFeature;
+    int* colOffInner;    /* nColorsInner+1, absolute ids */
+    int* colOffFree;     /* nColorsFree+1 */
+    int* colOffFeature;  /* nColorsFeature+1, absolute ids */
+    int* colorOf;        /* nV: colour within its class family */
+    /* boundary faces of a volume mesh (tri for tet, quad for hex); for surface
+       meshes these are the elements themselves */
+    int nFaces, fpe;
+    int* faces;          /* fpe*nFaces, final ids */
+    /* volume one-ring CSR over all nV vertices: halffaces (3) / halfhexes (7) */
+    int ringStride;
+    int* ringRow;        /* nV+1 */
+    int* ringVal;        /* ringStride*ringRow[nV] */
+    int maxRing;
+    /* surface halfedges CSR over [0,nSurf) */
+    int nHE;
+    int* heRow;          /* nSurf+1 */
+    int* heTarget; int* heOpp; int* heFace; int* heNext;
+    int maxHE;
+    unsigned char* isBoundary1D;  /* nSurf */
+    unsigned char* isFeature;     /* nSurf */
+} or_mesh;
+
+typedef struct or_build_params {
+    int coloring;        /* OR_COLOR_GREEDY (default) or OR_COLOR_JPL */
+    float feature_angle_deg; /* 20 in the reference */
+} or_build_params;
+
+/* returns 0 on success, nonzero error code otherwise */
+int or_mesh_build(int elem, const float* xyz, int nV, const int* cells, int nE,
+                  const or_build_params* prm, or_mesh** out);
+void or_mesh_free(or_mesh* m);
+
+/* individual build steps (exported for step-level tests) */
+int or_boundary_faces(int elem, int nE, const int* cells, int* outFaces); /* returns nFaces */
+int or_color(int mode, int n, const int* rowPtr, const int* colInd, int* colors); /* returns nColors */
+int or_adjacency_range(int elem, int nE, const int* cells, int lo, int hi,
+                       int** rowPtr, int** colInd);   /* CSR over [lo,hi), local ids */
+
+/* ---- DMO ---- */
+typedef struct or_dmo_params {
+    int nq;              /* candidates per axis (8) */
+    int depth;           /* refinement levels (3) */
+    float grid_scale;    /* initial window scale (0.5) */
+    float shrink;        /* per-level window factor; 0 => 2/(nq-1) as the reference */
+    int extent_mode;     /* OR_EXTENT_MAX (tet/tri) or OR_EXTENT_MIN (hex) */
+    int fixed_boundary;  /* 1: surface/feature passes skipped */
+    int nthreads;        /* OpenMP threads, 0 = default */
+} or_dmo_params;
+
+/* one DMO iteration over the inner colour classes (volume meshes) or the free
+   colour classes (flat tri meshes); positions updated in place in m->xyz.
+   Returns the number of vertex searches executed. */
+long long or_dmo_iteration(or_mesh* m, const or_dmo_params* p);
+/* search one vertex; exported for brute-force tests.  Returns 1 if moved. */
+int or_dmo_vertex(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz_inout);
+
+/* element quality statistics in FP32 canonical (min) and FP64 (min, mean) */
+float or_min_quality(const or_mesh* m);
+void or_quality64(const or_mesh* m, double* qmin, double* qmean);
+void or_element_qualities(const or_mesh* m, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

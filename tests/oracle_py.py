@@ -1,0 +1,236 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker: tests compare the HIP path against it.  It is
+never the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+OR_TRI, OR_QUAD, OR_TET, OR_HEX = 0, 1, 2, 3
+OR_COLOR_GREEDY, OR_COLOR_JPL = 0, 1
+OR_EXTENT_MAX, OR_EXTENT_MIN = 0, 1
+
+_ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+_bp = C.POINTER(C.c_ubyte)
+
+
+class OrMesh(C.Structure):
+    _fields_ = [
+        ("elem", C.c_int), ("npe", C.c_int), ("nV", C.c_int), ("nE", C.c_int), ("flat", C.c_int),
+        ("xyz", _fp), ("cells", _ip), ("new_of_old", _ip), ("old_of_new", _ip),
+        ("nSurf", C.c_int), ("nFeature", C.c_int), ("nFree", C.c_int), ("nInner", C.c_int),
+        ("nColorsInner", C.c_int), ("nColorsFree", C.c_int), ("nColorsFeature", C.c_int),
+        ("colOffInner", _ip), ("colOffFree", _ip), ("colOffFeature", _ip), ("colorOf", _ip),
+        ("nFaces", C.c_int), ("fpe", C.c_int), ("faces", _ip),
+        ("ringStride", C.c_int), ("ringRow", _ip), ("ringVal", _ip), ("maxRing", C.c_int),
+        ("nHE", C.c_int), ("heRow", _ip), ("heTarget", _ip), ("heOpp", _ip), ("heFace", _ip), ("heNext", _ip),
+        ("maxHE", C.c_int), ("isBoundary1D", _bp), ("isFeature", _bp),
+    ]
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("coloring", C.c_int), ("feature_angle_deg", C.c_float)]
+
+
+class DmoParams(C.Structure):
+    _fields_ = [("nq", C.c_int), ("depth", C.c_int), ("grid_scale", C.c_float), ("shrink", C.c_float),
+                ("extent_mode", C.c_int), ("fixed_boundary", C.c_int), ("nthreads", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.or_cbrt.restype = C.c_float; L.or_cbrt.argtypes = [C.c_float]
+        L.or_pow2_scale.restype = C.c_float; L.or_pow2_scale.argtypes = [C.c_float]
+        for n in ("or_q_tet", "or_q_hex", "or_q_tri2d"):
+            getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp]
+        for n in ("or_q_tet64", "or_q_hex64", "or_q_tri2d64"):
+            getattr(L, n).restype = C.c_double; getattr(L, n).argtypes = [_fp]
+        for n in ("or_ring_score_tet", "or_ring_score_hex", "or_ring_score_tri2d"):
+            getattr(L, n).restype = C.c_float
+            getattr(L, n).argtypes = [C.c_int, _fp, _fp, C.c_float, _fp]
+        L.or_mesh_build.restype = C.c_int
+        L.or_mesh_build.argtypes = [C.c_int, _fp, C.c_int, _ip, C.c_int, C.POINTER(BuildParams), C.POINTER(C.POINTER(OrMesh))]
+        L.or_mesh_free.argtypes = [C.POINTER(OrMesh)]
+        L.or_boundary_faces.restype = C.c_int
+        L.or_boundary_faces.argtypes = [C.c_int, C.c_int, _ip, _ip]
+        L.or_color.restype = C.c_int
+        L.or_color.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip]
+        L.or_adjacency_range.restype = C.c_int
+        L.or_adjacency_range.argtypes = [C.c_int, C.c_int, _ip, C.c_int, C.c_int, C.POINTER(_ip), C.POINTER(_ip)]
+        L.or_dmo_iteration.restype = C.c_longlong
+        L.or_dmo_iteration.argtypes = [C.POINTER(OrMesh), C.POINTER(DmoParams)]
+        L.or_dmo_vertex.restype = C.c_int
+        L.or_dmo_vertex.argtypes = [C.POINTER(OrMesh), C.POINTER(DmoParams), C.c_int, _fp]
+        L.or_min_quality.restype = C.c_float
+        L.or_min_quality.argtypes = [C.POINTER(OrMesh)]
+        L.or_quality64.argtypes = [C.POINTER(OrMesh), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_element_qualities.argtypes = [C.POINTER(OrMesh), _fp]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def cbrt(x: float) -> float:
+    return lib().or_cbrt(float(x))
+
+
+def q_elem(kind: str, pts, fp64: bool = False) -> float:
+    a, p = _f(pts)
+    name = {"tet": "or_q_tet", "hex": "or_q_hex", "tri": "or_q_tri2d"}[kind] + ("64" if fp64 else "")
+    return getattr(lib(), name)(p)
+
+
+def ring_score(kind: str, ring, x0, sigma, p) -> float:
+    r, rp = _f(ring)
+    x, xp = _f(x0)
+    q, qp = _f(p)
+    per = {"tet": 9, "hex": 21, "tri": 6}[kind]
+    n = r.size // per
+    return getattr(lib(), {"tet": "or_ring_score_tet", "hex": "or_ring_score_hex", "tri": "or_ring_score_tri2d"}[kind])(
+        n, rp, xp, C.c_float(sigma), qp)
+
+
+def boundary_faces(kind: str, cells) -> np.ndarray:
+    c, cp = _i(cells)
+    k = 3 if kind == "tet" else 4
+    per = 4 if kind == "tet" else 6
+    out = np.empty(len(c) * per * k, dtype=np.int32)
+    n = lib().or_boundary_faces(OR_TET if kind == "tet" else OR_HEX, len(c), cp, out.ctypes.data_as(_ip))
+    return out[: n * k].reshape(n, k)
+
+
+def color(mode: int, row_ptr, col_ind) -> np.ndarray:
+    r, rp = _i(row_ptr)
+    c, cp = _i(col_ind)
+    n = len(r) - 1
+    out = np.empty(max(n, 1), dtype=np.int32)
+    lib().or_color(mode, n, rp, cp, out.ctypes.data_as(_ip))
+    return out[:n]
+
+
+_ELEM = {"tri": OR_TRI, "quad": OR_QUAD, "tet": OR_TET, "hex": OR_HEX}
+
+
+def _arr(ptr, n, dtype=np.int32):
+    if n <= 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class OracleMesh:
+    """Owns an or_mesh built from host arrays; exposes numpy copies."""
+
+    def __init__(self, kind: str, xyz, cells, coloring: int = OR_COLOR_GREEDY, feature_angle: float = 20.0):
+        self.kind = kind
+        x, xp = _f(xyz)
+        c, cp = _i(cells)
+        prm = BuildParams(coloring, feature_angle)
+        out = C.POINTER(OrMesh)()
+        rc = lib().or_mesh_build(_ELEM[kind], xp, len(x), cp, len(c), C.byref(prm), C.byref(out))
+        if not out:
+            raise ValueError(f"oracle mesh build failed: {rc}")
+        self.ptr = out
+        self.rc = rc
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().or_mesh_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def m(self) -> OrMesh:
+        return self.ptr.contents
+
+    def positions(self) -> np.ndarray:
+        m = self.m
+        return _arr(m.xyz, 3 * m.nV, np.float32).reshape(-1, 3)
+
+    def set_positions(self, xyz):
+        m = self.m
+        a = np.ascontiguousarray(xyz, dtype=np.float32).ravel()
+        C.memmove(m.xyz, a.ctypes.data, a.nbytes)
+
+    def cells(self):
+        m = self.m
+        return _arr(m.cells, m.npe * m.nE).reshape(-1, m.npe)
+
+    def new_of_old(self):
+        return _arr(self.m.new_of_old, self.m.nV)
+
+    def offsets(self, which: str):
+        m = self.m
+        if which == "inner":
+            return _arr(m.colOffInner, m.nColorsInner + 1)
+        if which == "free":
+            return _arr(m.colOffFree, m.nColorsFree + 1)
+        return _arr(m.colOffFeature, m.nColorsFeature + 1)
+
+    def faces(self):
+        m = self.m
+        return _arr(m.faces, m.fpe * m.nFaces).reshape(-1, m.fpe)
+
+    def ring(self):
+        m = self.m
+        row = _arr(m.ringRow, m.nV + 1)
+        return row, _arr(m.ringVal, m.ringStride * int(row[-1])).reshape(-1, m.ringStride)
+
+    def halfedges(self):
+        m = self.m
+        row = _arr(m.heRow, m.nSurf + 1)
+        n = m.nHE
+        return row, np.stack([_arr(m.heTarget, n), _arr(m.heOpp, n), _arr(m.heFace, n), _arr(m.heNext, n)], axis=1)
+
+    def flags(self):
+        m = self.m
+        return _arr(m.isBoundary1D, m.nSurf, np.uint8), _arr(m.isFeature, m.nSurf, np.uint8)
+
+    def iterate(self, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None, fixed_boundary=1, nthreads=0) -> int:
+        if extent is None:
+            extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, fixed_boundary, nthreads)
+        return lib().or_dmo_iteration(self.ptr, C.byref(p))
+
+    def search_vertex(self, vid, xyz, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None):
+        if extent is None:
+            extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, 1, 1)
+        a = np.ascontiguousarray(xyz, dtype=np.float32).copy()
+        moved = lib().or_dmo_vertex(self.ptr, C.byref(p), vid, a.ctypes.data_as(_fp))
+        return moved, a
+
+    def min_quality(self) -> float:
+        return lib().or_min_quality(self.ptr)
+
+    def quality64(self):
+        a, b = C.c_double(), C.c_double()
+        lib().or_quality64(self.ptr, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def element_qualities(self) -> np.ndarray:
+        out = np.empty(self.m.nE, dtype=np.float32)
+        lib().or_element_qualities(self.ptr, out.ctypes.data_as(_fp))
+        return out

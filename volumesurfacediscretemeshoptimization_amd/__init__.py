@@ -1,0 +1,264 @@
+"""MI355X-native discrete mesh optimisation (DMO).
+
+Python mirror of the reference's mesh/optimiser interface over the C ABI of
+libvsdmo.so (include/vsdmo.h):
+
+* ``DMOMeshTriFactory.create`` / ``DMOMeshTetFactory.create`` /
+  ``DMOMeshHexFactory.create`` (reference VSDMO/src/mesh/MeshFactory.h:17-41)
+  build the device mesh: boundary faces, renumbering, colouring, one-rings.
+* ``DMOTetClass`` / ``DMOHexClass`` / ``DMOTriFlatClass``
+  (VSDMO/src/dmo/DMOTet.h:24, DMOHex.h:18, DMOTriFlat.h:25) with the virtual
+  interface of ``DMOBaseClass`` (VSDMO/src/dmo/DMOBase.h:23-88):
+  ``optimize``, ``doIteration``, ``isDone``, ``getQuality``,
+  ``getQualityHistogram``, ``getElementQualities``, ``displayQualityGPU``.
+
+All arithmetic runs in the HIP kernels of libvsdmo.so; this module only moves
+host arrays across the ABI.  A missing library raises; there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import VsdmoError, check, load  # noqa: F401
+
+MEAN_RATIO = 0
+
+__all__ = ["DMOMeshTriFactory", "DMOMeshTetFactory", "DMOMeshHexFactory", "DMOMesh",
+           "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "VsdmoError", "MEAN_RATIO", "load"]
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_lib._fp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_lib._ip)
+
+
+class DMOMesh:
+    """Device mesh (owns the vsdmo_mesh handle)."""
+
+    NPE = {_lib.VSDMO_TRI: 3, _lib.VSDMO_TET: 4, _lib.VSDMO_HEX: 8}
+
+    def __init__(self, elem: int, xyz, cells, coloring: int = _lib.VSDMO_COLOR_GREEDY, feature_angle: float = 20.0):
+        lib = load()
+        self._lib = lib
+        self.elem = elem
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        if xyz.ndim != 2 or xyz.shape[1] != 3:
+            raise ValueError("xyz must have shape (nV, 3)")
+        npe = self.NPE[elem]
+        if cells.ndim != 2 or cells.shape[1] != npe:
+            raise ValueError(f"cells must have shape (nE, {npe})")
+        prm = _lib.MeshParams(coloring, feature_angle)
+        h = C.c_void_p()
+        check(lib.vsdmo_mesh_create(elem, _fptr(xyz), xyz.shape[0], _iptr(cells), cells.shape[0], None,
+                                    C.byref(prm), C.byref(h)), "vsdmo_mesh_create")
+        self._h = h
+        self.npe = npe
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            check(self._lib.vsdmo_mesh_destroy(self._h), "vsdmo_mesh_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def counts(self) -> dict:
+        c = _lib.Counts()
+        check(self._lib.vsdmo_mesh_counts(self._h, C.byref(c)), "vsdmo_mesh_counts")
+        return {name: getattr(c, name) for name, _ in _lib.Counts._fields_}
+
+    @property
+    def nVertices(self) -> int:
+        return self.counts()["n_vertices"]
+
+    def positions(self, original_order: bool = True) -> np.ndarray:
+        n = self.nVertices
+        out = np.empty((n, 3), dtype=np.float32)
+        check(self._lib.vsdmo_mesh_get_positions(self._h, _fptr(out), int(original_order)), "get_positions")
+        return out
+
+    def set_positions(self, xyz, original_order: bool = True):
+        a = np.ascontiguousarray(xyz, dtype=np.float32)
+        check(self._lib.vsdmo_mesh_set_positions(self._h, _fptr(a), int(original_order)), "set_positions")
+
+    def permutation(self) -> np.ndarray:
+        out = np.empty(self.nVertices, dtype=np.int32)
+        check(self._lib.vsdmo_mesh_get_permutation(self._h, _iptr(out)), "get_permutation")
+        return out
+
+    def cells(self) -> np.ndarray:
+        c = self.counts()
+        out = np.empty((c["n_elements"], self.npe), dtype=np.int32)
+        check(self._lib.vsdmo_mesh_get_cells(self._h, _iptr(out)), "get_cells")
+        return out
+
+    def color_offsets(self, which: str) -> np.ndarray:
+        c = self.counts()
+        idx = {"inner": 0, "free": 1, "feature": 2}[which]
+        n = c[{"inner": "n_colors_inner", "free": "n_colors_free", "feature": "n_colors_feature"}[which]]
+        out = np.empty(n + 1, dtype=np.int32)
+        check(self._lib.vsdmo_mesh_get_coloring(self._h, idx, _iptr(out)), "get_coloring")
+        return out
+
+    def colors(self) -> np.ndarray:
+        out = np.empty(self.nVertices, dtype=np.int32)
+        check(self._lib.vsdmo_mesh_get_colors(self._h, _iptr(out)), "get_colors")
+        return out
+
+    def faces(self) -> np.ndarray:
+        c = self.counts()
+        k = 3 if self.elem in (_lib.VSDMO_TET, _lib.VSDMO_TRI) else 4
+        out = np.empty((max(c["n_faces"], 0), k), dtype=np.int32)
+        if c["n_faces"]:
+            check(self._lib.vsdmo_mesh_get_faces(self._h, _iptr(out)), "get_faces")
+        return out
+
+    def onering(self):
+        n = self.nVertices
+        row = np.empty(n + 1, dtype=np.int32)
+        check(self._lib.vsdmo_mesh_get_onering(self._h, _iptr(row), None), "get_onering")
+        st = 7 if self.elem == _lib.VSDMO_HEX else 3
+        vals = np.empty((max(int(row[-1]), 1), st), dtype=np.int32)
+        check(self._lib.vsdmo_mesh_get_onering(self._h, _iptr(row), _iptr(vals)), "get_onering")
+        return row, vals[: int(row[-1])]
+
+    def halfedges(self):
+        c = self.counts()
+        row = np.empty(c["n_surface"] + 1, dtype=np.int32)
+        n = max(c["n_halfedges"], 1)
+        t, o, f, nx = (np.empty(n, dtype=np.int32) for _ in range(4))
+        check(self._lib.vsdmo_mesh_get_halfedges(self._h, _iptr(row), _iptr(t), _iptr(o), _iptr(f), _iptr(nx)),
+              "get_halfedges")
+        k = c["n_halfedges"]
+        return row, np.stack([t[:k], o[:k], f[:k], nx[:k]], axis=1)
+
+    def flags(self):
+        c = self.counts()
+        n = max(c["n_surface"], 1)
+        b = np.zeros(n, dtype=np.uint8)
+        f = np.zeros(n, dtype=np.uint8)
+        check(self._lib.vsdmo_mesh_get_flags(self._h, b.ctypes.data_as(_lib._u8p), f.ctypes.data_as(_lib._u8p)),
+              "get_flags")
+        return b[: c["n_surface"]], f[: c["n_surface"]]
+
+
+class DMOMeshTriFactory:
+    @staticmethod
+    def create(xyz, tris, **kw) -> DMOMesh:
+        return DMOMesh(_lib.VSDMO_TRI, xyz, tris, **kw)
+
+
+class DMOMeshTetFactory:
+    @staticmethod
+    def create(xyz, tets, **kw) -> DMOMesh:
+        return DMOMesh(_lib.VSDMO_TET, xyz, tets, **kw)
+
+
+class DMOMeshHexFactory:
+    @staticmethod
+    def create(xyz, hexes, **kw) -> DMOMesh:
+        return DMOMesh(_lib.VSDMO_HEX, xyz, hexes, **kw)
+
+
+class _DMOBase:
+    """Reference DMOBaseClass (VSDMO/src/dmo/DMOBase.h:23-88)."""
+
+    def __init__(self, mesh: DMOMesh, qualityCriterium: int = MEAN_RATIO, gridScale: float = 0.5,
+                 n_iter: int = 30, nq: int = 8, depth: int = 3, shrink: float = 0.0,
+                 extent_mode: int = _lib.VSDMO_EXTENT_AUTO, fixed_boundary: bool = True,
+                 quality_guard: bool = True, guard_tolerance: float = 1e-5):
+        lib = load()
+        self._lib = lib
+        self.mesh = mesh
+        p = _lib.Params()
+        lib.vsdmo_params_default(C.byref(p))
+        p.nq, p.depth, p.shrink, p.extent_mode = nq, depth, shrink, extent_mode
+        p.fixed_boundary, p.quality_guard, p.guard_tolerance = int(fixed_boundary), int(quality_guard), guard_tolerance
+        h = C.c_void_p()
+        check(lib.vsdmo_opt_create(mesh.handle, qualityCriterium, gridScale, n_iter, C.byref(p), C.byref(h)),
+              "vsdmo_opt_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            check(self._lib.vsdmo_opt_destroy(self._h), "vsdmo_opt_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def doIteration(self):
+        check(self._lib.vsdmo_opt_iterate(self._h, 1), "vsdmo_opt_iterate")
+
+    def iterate(self, n: int):
+        check(self._lib.vsdmo_opt_iterate(self._h, int(n)), "vsdmo_opt_iterate")
+
+    def optimize(self):
+        check(self._lib.vsdmo_opt_optimize(self._h), "vsdmo_opt_optimize")
+
+    def isDone(self) -> bool:
+        return bool(self._lib.vsdmo_opt_is_done(self._h))
+
+    def getQuality(self) -> float:
+        q = C.c_float()
+        check(self._lib.vsdmo_opt_min_quality(self._h, C.byref(q)), "vsdmo_opt_min_quality")
+        return q.value
+
+    def getMeanQuality(self) -> float:
+        q = C.c_double()
+        check(self._lib.vsdmo_opt_mean_quality(self._h, C.byref(q)), "vsdmo_opt_mean_quality")
+        return q.value
+
+    def getQualityHistogram(self, n_cols: int = 10) -> np.ndarray:
+        out = np.zeros(n_cols, dtype=np.int32)
+        check(self._lib.vsdmo_opt_histogram(self._h, n_cols, _iptr(out)), "vsdmo_opt_histogram")
+        return out
+
+    def getElementQualities(self) -> np.ndarray:
+        n = self.mesh.counts()["n_elements"]
+        out = np.empty(n, dtype=np.float32)
+        check(self._lib.vsdmo_opt_element_qualities(self._h, _fptr(out)), "vsdmo_opt_element_qualities")
+        return out
+
+    def displayQualityGPU(self, n_cols: int = 10):
+        h = self.getQualityHistogram(n_cols)
+        print("interval " + " ".join(f"{i / n_cols:.2f}" for i in range(n_cols)))
+        print("amount   " + " ".join(str(int(v)) for v in h))
+        print(f"q_min = {self.getQuality()}")
+
+    def set_timing(self, on: bool):
+        check(self._lib.vsdmo_opt_set_timing(self._h, int(on)), "vsdmo_opt_set_timing")
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        check(self._lib.vsdmo_opt_stats(self._h, C.byref(s)), "vsdmo_opt_stats")
+        return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
+
+
+class DMOTetClass(_DMOBase):
+    pass
+
+
+class DMOHexClass(_DMOBase):
+    pass
+
+
+class DMOTriFlatClass(_DMOBase):
+    pass

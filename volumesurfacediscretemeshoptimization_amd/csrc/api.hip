@@ -1,0 +1,539 @@
+// api.hip -- C ABI of libvsdmo (declarations and reference citations in
+// include/vsdmo.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "dmo_kernels.h"
+#include "vsdmo_internal.h"
+
+namespace vsdmo {
+void build_volume(vsdmo_mesh& m, const vsdmo_mesh_params& prm);
+void build_surface_mesh(vsdmo_mesh& m, const vsdmo_mesh_params& prm);
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+template <class F>
+static vsdmo_status guarded(F&& f)
+{
+    try {
+        g_err.clear();
+        return f();
+    } catch (const HipError& e) {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return VSDMO_HIP_ERROR;
+    } catch (const StatusError& e) {
+        set_error(e.msg);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return VSDMO_HIP_ERROR;
+    } catch (...) {
+        set_error("unknown internal error");
+        return VSDMO_HIP_ERROR;
+    }
+}
+
+static vsdmo_status fail(vsdmo_status s, const std::string& msg)
+{
+    set_error(msg);
+    return s;
+}
+
+// largest one-ring each search kernel is built for (LDS per 64-thread block)
+static int max_ring_supported(int elem)
+{
+    return elem == VSDMO_HEX ? 256 : 1024;   // <= 48 KB of LDS per block
+}
+
+template <class T>
+static vsdmo_status copy_dev(const DevBuf<T>& b, size_t n, T* out, hipStream_t s)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!out) return fail(VSDMO_INVALID_ARG, "NULL output");
+        b.download(out, n, s);
+        VSDMO_HIP(hipStreamSynchronize(s));
+        return VSDMO_OK;
+    });
+}
+
+static void destroy_mesh(vsdmo_mesh* m)
+{
+    hipStream_t s = m->stream;
+    delete m;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+}  // namespace vsdmo
+
+using namespace vsdmo;
+
+struct vsdmo_opt {
+    vsdmo_mesh* m = nullptr;
+    int criterion = 0;
+    float grid_scale = 0.5f;
+    int n_iter = 30;
+    vsdmo_params p{};
+    float shrink = 0.0f, aff = 0.0f;
+    int extent_min = 0;
+    int curr_it = 0;
+    float last_min = 0.0f;
+    double last_sum = 0.0;
+    int64_t s_inner = 0, s_surf = 0, s_feat = 0;
+    bool timing = false;
+    double kernel_ms = 0.0;
+    int64_t launches = 0;
+    DevBuf<float> q, pmin, omin;
+    DevBuf<double> psum, osum;
+    std::vector<hipEvent_t> ev;   // pairs around timed launches
+};
+
+extern "C" {
+
+void vsdmo_mesh_params_default(vsdmo_mesh_params* p)
+{
+    if (!p) return;
+    p->coloring = VSDMO_COLOR_GREEDY;
+    p->feature_angle_deg = 20.0f;
+}
+
+void vsdmo_params_default(vsdmo_params* p)
+{
+    if (!p) return;
+    p->nq = 8;
+    p->depth = 3;
+    p->shrink = 0.0f;
+    p->extent_mode = VSDMO_EXTENT_AUTO;
+    p->fixed_boundary = 1;
+    p->quality_guard = 1;
+    p->guard_tolerance = 1e-5f;
+}
+
+const char* vsdmo_last_error(void) { return g_err.c_str(); }
+
+vsdmo_status vsdmo_mesh_create(vsdmo_elem type, const float* xyz, int32_t nV, const int32_t* cells, int32_t nCells,
+                               const uint8_t* boundary_or_null, const vsdmo_mesh_params* params, vsdmo_mesh** out)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!out) return fail(VSDMO_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        if (!xyz || !cells || nV <= 0 || nCells <= 0) return fail(VSDMO_INVALID_ARG, "empty or NULL mesh arrays");
+        if (boundary_or_null) return fail(VSDMO_NOT_SUPPORTED, "explicit boundary flags are not supported; pass NULL");
+        if (type != VSDMO_TET && type != VSDMO_HEX && type != VSDMO_TRI)
+            return fail(VSDMO_NOT_SUPPORTED, "element type not supported (tri, tet, hex)");
+        int npe = (type == VSDMO_TRI) ? 3 : (type == VSDMO_TET) ? 4 : 8;
+        long long nci = (long long)nCells * npe;
+        if (nci > (long long)0x7fffffff / 2) return fail(VSDMO_INVALID_ARG, "mesh too large for int32 indexing");
+        for (long long i = 0; i < nci; ++i)
+            if (cells[i] < 0 || cells[i] >= nV) return fail(VSDMO_INVALID_ARG, "element vertex index out of range");
+        for (long long e = 0; e < nCells; ++e)
+            for (int a = 0; a < npe; ++a)
+                for (int b = a + 1; b < npe; ++b)
+                    if (cells[e * npe + a] == cells[e * npe + b])
+                        return fail(VSDMO_INVALID_ARG, "element with repeated vertex " + std::to_string(e));
+        vsdmo_mesh_params prm;
+        vsdmo_mesh_params_default(&prm);
+        if (params) prm = *params;
+        if (prm.coloring != VSDMO_COLOR_GREEDY && prm.coloring != VSDMO_COLOR_JPL)
+            return fail(VSDMO_INVALID_ARG, "unknown colouring mode");
+
+        auto* m = new vsdmo_mesh();
+        try {
+            m->elem = type; m->npe = npe; m->nV = nV; m->nE = nCells;
+            VSDMO_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+            hipStream_t s = m->stream;
+            std::vector<float> hx(nV), hy(nV), hz(nV);
+            int flat = 1;
+            for (int i = 0; i < nV; ++i) {
+                hx[i] = xyz[3 * i]; hy[i] = xyz[3 * i + 1]; hz[i] = xyz[3 * i + 2];
+                if (hz[i] != 0.0f) flat = 0;
+            }
+            m->flat = (type == VSDMO_TRI) ? flat : 0;
+            if (type == VSDMO_TRI && !flat) {
+                destroy_mesh(m);
+                return fail(VSDMO_NOT_SUPPORTED, "non-flat triangle surfaces are not supported (flat 2D meshes only)");
+            }
+            m->x.alloc(nV); m->y.alloc(nV); m->z.alloc(nV);
+            m->x.upload(hx.data(), nV, s); m->y.upload(hy.data(), nV, s); m->z.upload(hz.data(), nV, s);
+            m->cells.alloc(nci);
+            m->cells.upload(cells, nci, s);
+            m->colorOf.alloc(nV);
+            VSDMO_HIP(hipMemsetAsync(m->colorOf.p, 0, sizeof(int) * nV, s));
+            m->newOfOld.alloc(nV);
+            {
+                std::vector<int> id(nV);
+                for (int i = 0; i < nV; ++i) id[i] = i;
+                m->newOfOld.upload(id.data(), nV, s);
+                VSDMO_HIP(hipStreamSynchronize(s));
+            }
+            if (type == VSDMO_TRI) build_surface_mesh(*m, prm);
+            else build_volume(*m, prm);
+            m->new_of_old = m->newOfOld.to_host(s);
+            if (m->maxRing > max_ring_supported(type) || m->maxHE > 1024) {
+                int mr = m->maxRing;
+                destroy_mesh(m);
+                return fail(VSDMO_LDS_OVERFLOW, "one-ring of " + std::to_string(mr) + " elements exceeds the kernel limit");
+            }
+        } catch (...) {
+            destroy_mesh(m);
+            throw;
+        }
+        *out = m;
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_mesh_destroy(vsdmo_mesh* m)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!m) return VSDMO_OK;
+        if (m->n_opts > 0) return fail(VSDMO_INVALID_ARG, "mesh still referenced by an optimiser");
+        destroy_mesh(m);
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_mesh_counts(const vsdmo_mesh* m, vsdmo_counts* o)
+{
+    if (!m || !o) return fail(VSDMO_INVALID_ARG, "NULL argument");
+    o->n_vertices = m->nV; o->n_elements = m->nE;
+    o->n_surface = m->nSurf; o->n_free = m->nFree; o->n_feature = m->nFeature; o->n_inner = m->nInner;
+    o->n_colors_inner = (int)m->colOffInner.size() - 1;
+    o->n_colors_free = (int)m->colOffFree.size() - 1;
+    o->n_colors_feature = (int)m->colOffFeature.size() - 1;
+    o->n_faces = m->nFaces; o->n_halfedges = m->nHE;
+    o->max_ring = m->maxRing; o->max_halfedges = m->maxHE; o->flat = m->flat;
+    return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_mesh_get_positions(const vsdmo_mesh* m, float* out, int32_t original_order)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!m || !out) return fail(VSDMO_INVALID_ARG, "NULL argument");
+        std::vector<float> hx = m->x.to_host(m->stream), hy = m->y.to_host(m->stream), hz = m->z.to_host(m->stream);
+        for (int o = 0; o < m->nV; ++o) {
+            int v = original_order ? m->new_of_old[o] : o;
+            out[3 * o] = hx[v]; out[3 * o + 1] = hy[v]; out[3 * o + 2] = hz[v];
+        }
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_mesh_set_positions(vsdmo_mesh* m, const float* in, int32_t original_order)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!m || !in) return fail(VSDMO_INVALID_ARG, "NULL argument");
+        std::vector<float> hx(m->nV), hy(m->nV), hz(m->nV);
+        for (int o = 0; o < m->nV; ++o) {
+            int v = original_order ? m->new_of_old[o] : o;
+            hx[v] = in[3 * o]; hy[v] = in[3 * o + 1]; hz[v] = in[3 * o + 2];
+        }
+        m->x.upload(hx.data(), m->nV, m->stream);
+        m->y.upload(hy.data(), m->nV, m->stream);
+        m->z.upload(hz.data(), m->nV, m->stream);
+        VSDMO_HIP(hipStreamSynchronize(m->stream));
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_mesh_get_permutation(const vsdmo_mesh* m, int32_t* new_of_old)
+{
+    if (!m || !new_of_old) return fail(VSDMO_INVALID_ARG, "NULL argument");
+    std::memcpy(new_of_old, m->new_of_old.data(), sizeof(int) * m->nV);
+    return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_mesh_get_cells(const vsdmo_mesh* m, int32_t* out)
+{
+    if (!m) return fail(VSDMO_INVALID_ARG, "NULL mesh");
+    return copy_dev(m->cells, (size_t)m->nE * m->npe, out, m->stream);
+}
+
+vsdmo_status vsdmo_mesh_get_coloring(const vsdmo_mesh* m, int32_t which, int32_t* offsets)
+{
+    if (!m || !offsets) return fail(VSDMO_INVALID_ARG, "NULL argument");
+    const std::vector<int>* v = which == 0 ? &m->colOffInner : which == 1 ? &m->colOffFree : which == 2 ? &m->colOffFeature : nullptr;
+    if (!v) return fail(VSDMO_INVALID_ARG, "which must be 0 (inner), 1 (free) or 2 (feature)");
+    std::memcpy(offsets, v->data(), sizeof(int) * v->size());
+    return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_mesh_get_colors(const vsdmo_mesh* m, int32_t* colors)
+{
+    if (!m) return fail(VSDMO_INVALID_ARG, "NULL mesh");
+    return copy_dev(m->colorOf, (size_t)m->nV, colors, m->stream);
+}
+
+vsdmo_status vsdmo_mesh_get_faces(const vsdmo_mesh* m, int32_t* out)
+{
+    if (!m) return fail(VSDMO_INVALID_ARG, "NULL mesh");
+    return copy_dev(m->faces, (size_t)m->nFaces * m->fpe, out, m->stream);
+}
+
+vsdmo_status vsdmo_mesh_get_onering(const vsdmo_mesh* m, int32_t* row_ptr, int32_t* values)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!m || !row_ptr) return fail(VSDMO_INVALID_ARG, "NULL argument");
+        m->ringRow.download(row_ptr, (size_t)m->nV + 1, m->stream);
+        VSDMO_HIP(hipStreamSynchronize(m->stream));
+        int st = m->elem == VSDMO_HEX ? 7 : 3;
+        size_t nnz = (size_t)row_ptr[m->nV] * st;
+        if (values && nnz) {
+            m->ringVal.download(values, nnz, m->stream);
+            VSDMO_HIP(hipStreamSynchronize(m->stream));
+        }
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_mesh_get_halfedges(const vsdmo_mesh* m, int32_t* row_ptr, int32_t* target, int32_t* opposite,
+                                      int32_t* face, int32_t* next)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!m || !row_ptr) return fail(VSDMO_INVALID_ARG, "NULL argument");
+        m->heRow.download(row_ptr, (size_t)m->nSurf + 1, m->stream);
+        size_t n = (size_t)m->nHE;
+        if (n) {
+            if (target) m->heTarget.download(target, n, m->stream);
+            if (opposite) m->heOpp.download(opposite, n, m->stream);
+            if (face) m->heFace.download(face, n, m->stream);
+            if (next) m->heNext.download(next, n, m->stream);
+        }
+        VSDMO_HIP(hipStreamSynchronize(m->stream));
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_mesh_get_flags(const vsdmo_mesh* m, uint8_t* b1d, uint8_t* feat)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!m) return fail(VSDMO_INVALID_ARG, "NULL mesh");
+        if (m->nSurf > 0) {
+            if (b1d) m->isB1D.download(b1d, m->nSurf, m->stream);
+            if (feat) m->isFeature.download(feat, m->nSurf, m->stream);
+        }
+        VSDMO_HIP(hipStreamSynchronize(m->stream));
+        return VSDMO_OK;
+    });
+}
+
+// ---------------------------------------------------------------- optimiser
+static void opt_quality(vsdmo_opt* o)
+{
+    vsdmo_mesh* m = o->m;
+    QualityArgs qa{ m->x.p, m->y.p, m->z.p, m->cells.p, m->nE, m->npe, m->elem, o->q.p };
+    launch_quality(qa, o->pmin.p, o->psum.p, o->omin.p, o->osum.p, m->stream);
+    VSDMO_HIP(hipGetLastError());
+    float mn = 0.0f;
+    double sm = 0.0;
+    VSDMO_HIP(hipMemcpyAsync(&mn, o->omin.p, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    VSDMO_HIP(hipMemcpyAsync(&sm, o->osum.p, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    VSDMO_HIP(hipStreamSynchronize(m->stream));
+    o->last_min = mn;
+    o->last_sum = sm;
+}
+
+vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale, int32_t n_iter,
+                              const vsdmo_params* params, vsdmo_opt** out)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!out) return fail(VSDMO_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        if (!m) return fail(VSDMO_INVALID_ARG, "mesh is NULL");
+        if (criterion != VSDMO_MEAN_RATIO) return fail(VSDMO_NOT_SUPPORTED, "only the mean-ratio criterion is supported");
+        if (!(grid_scale > 0.0f) || n_iter < 0) return fail(VSDMO_INVALID_ARG, "grid_scale must be > 0 and n_iter >= 0");
+        vsdmo_params p;
+        vsdmo_params_default(&p);
+        if (params) p = *params;
+        if (!(p.nq == 2 || p.nq == 4 || p.nq == 6 || p.nq == 8))
+            return fail(VSDMO_INVALID_ARG, "nq must be 2, 4, 6 or 8");
+        if (p.depth < 1 || p.depth > 16) return fail(VSDMO_INVALID_ARG, "depth must be in [1,16]");
+        if (!p.fixed_boundary)
+            return fail(VSDMO_NOT_SUPPORTED, "surface/feature vertex optimisation is not available in this build; set fixed_boundary=1");
+        auto* o = new vsdmo_opt();
+        o->m = m;
+        o->criterion = criterion;
+        o->grid_scale = grid_scale;
+        o->n_iter = n_iter;
+        o->p = p;
+        o->aff = 1.0f / (float)(p.nq - 1);
+        o->shrink = p.shrink > 0.0f ? p.shrink : 2.0f / (float)(p.nq - 1);
+        int em = p.extent_mode;
+        if (em == VSDMO_EXTENT_AUTO) em = (m->elem == VSDMO_HEX) ? VSDMO_EXTENT_MIN : VSDMO_EXTENT_MAX;
+        o->extent_min = (em == VSDMO_EXTENT_MIN);
+        try {
+            int nb = (m->nE + 255) / 256;
+            o->q.alloc(m->nE);
+            o->pmin.alloc(nb); o->psum.alloc(nb);
+            o->omin.alloc(1); o->osum.alloc(1);
+            opt_quality(o);
+        } catch (...) {
+            delete o;
+            throw;
+        }
+        m->n_opts++;
+        *out = o;
+        return VSDMO_OK;
+    });
+}
+
+static void timed_launch(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds)
+{
+    hipStream_t s = o->m->stream;
+    if (o->timing) {
+        hipEvent_t e0, e1;
+        VSDMO_HIP(hipEventCreate(&e0));
+        VSDMO_HIP(hipEventCreate(&e1));
+        VSDMO_HIP(hipEventRecord(e0, s));
+        launch_search(kind, o->p.nq, a, lds, s);
+        VSDMO_HIP(hipEventRecord(e1, s));
+        o->ev.push_back(e0);
+        o->ev.push_back(e1);
+    } else {
+        launch_search(kind, o->p.nq, a, lds, s);
+    }
+    o->launches++;
+}
+
+static void collect_timing(vsdmo_opt* o)
+{
+    if (o->ev.empty()) return;
+    VSDMO_HIP(hipStreamSynchronize(o->m->stream));
+    for (size_t i = 0; i + 1 < o->ev.size(); i += 2) {
+        float ms = 0.0f;
+        VSDMO_HIP(hipEventElapsedTime(&ms, o->ev[i], o->ev[i + 1]));
+        o->kernel_ms += ms;
+        (void)hipEventDestroy(o->ev[i]);
+        (void)hipEventDestroy(o->ev[i + 1]);
+    }
+    o->ev.clear();
+}
+
+vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
+        if (n < 0) return fail(VSDMO_INVALID_ARG, "negative iteration count");
+        vsdmo_mesh* m = o->m;
+        SearchArgs a{};
+        a.x = m->x.p; a.y = m->y.p; a.z = m->z.p;
+        a.xw = m->x.p; a.yw = m->y.p; a.zw = m->z.p;
+        a.ringRow = m->ringRow.p; a.ringVal = m->ringVal.p;
+        a.heRow = m->heRow.p; a.heTarget = m->heTarget.p;
+        a.gridScale = o->grid_scale; a.shrink = o->shrink; a.aff = o->aff;
+        a.depth = o->p.depth; a.extentMin = o->extent_min;
+        for (int it = 0; it < n; ++it) {
+            if (m->elem == VSDMO_TET || m->elem == VSDMO_HEX) {
+                size_t lds = search_lds_bytes(m->elem, m->maxRing);
+                for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c) {
+                    a.cStart = m->colOffInner[c]; a.cEnd = m->colOffInner[c + 1];
+                    timed_launch(o, m->elem, a, lds);
+                    o->s_inner += a.cEnd - a.cStart;
+                }
+            } else {
+                size_t lds = search_lds_bytes(VSDMO_TRI, m->maxHE);
+                for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {
+                    a.cStart = m->colOffFree[c]; a.cEnd = m->colOffFree[c + 1];
+                    timed_launch(o, VSDMO_TRI, a, lds);
+                    o->s_surf += a.cEnd - a.cStart;
+                }
+            }
+            VSDMO_HIP(hipGetLastError());
+            float prev = o->last_min;
+            opt_quality(o);
+            o->curr_it++;
+            if (o->p.quality_guard) {
+                float tol = o->p.guard_tolerance * std::fabs(prev);
+                if (o->last_min < prev - tol) {
+                    collect_timing(o);
+                    return fail(VSDMO_QUALITY_DECREASED, "minimum quality decreased from " + std::to_string(prev) +
+                                                             " to " + std::to_string(o->last_min));
+                }
+            }
+        }
+        collect_timing(o);
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_opt_optimize(vsdmo_opt* o)
+{
+    if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
+    int left = o->n_iter - o->curr_it;
+    return left > 0 ? vsdmo_opt_iterate(o, left) : VSDMO_OK;
+}
+
+int32_t vsdmo_opt_is_done(const vsdmo_opt* o) { return o ? (o->curr_it >= o->n_iter) : 1; }
+
+vsdmo_status vsdmo_opt_min_quality(const vsdmo_opt* o, float* out)
+{
+    if (!o || !out) return fail(VSDMO_INVALID_ARG, "NULL argument");
+    *out = o->last_min;
+    return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_opt_mean_quality(const vsdmo_opt* o, double* out)
+{
+    if (!o || !out) return fail(VSDMO_INVALID_ARG, "NULL argument");
+    *out = o->m->nE > 0 ? o->last_sum / o->m->nE : 0.0;
+    return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_opt_histogram(const vsdmo_opt* o, int32_t n_cols, int32_t* out)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!o || !out || n_cols <= 0) return fail(VSDMO_INVALID_ARG, "bad argument");
+        DevBuf<int> bins(n_cols);
+        VSDMO_HIP(hipMemsetAsync(bins.p, 0, sizeof(int) * n_cols, o->m->stream));
+        launch_histogram(o->q.p, o->m->nE, n_cols, bins.p, o->m->stream);
+        VSDMO_HIP(hipGetLastError());
+        bins.download(out, n_cols, o->m->stream);
+        VSDMO_HIP(hipStreamSynchronize(o->m->stream));
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_opt_element_qualities(const vsdmo_opt* o, float* host_out)
+{
+    if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
+    return copy_dev(o->q, (size_t)o->m->nE, host_out, o->m->stream);
+}
+
+vsdmo_status vsdmo_opt_stats(const vsdmo_opt* o, vsdmo_stats* out)
+{
+    if (!o || !out) return fail(VSDMO_INVALID_ARG, "NULL argument");
+    out->searches_inner = o->s_inner;
+    out->searches_surface = o->s_surf;
+    out->searches_feature = o->s_feat;
+    out->iterations = o->curr_it;
+    out->min_quality = o->last_min;
+    out->kernel_ms = o->kernel_ms;
+    out->kernel_launches = o->launches;
+    return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_opt_set_timing(vsdmo_opt* o, int32_t on)
+{
+    if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
+    o->timing = on != 0;
+    o->kernel_ms = 0.0;
+    o->launches = 0;
+    return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_opt_destroy(vsdmo_opt* o)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!o) return VSDMO_OK;
+        collect_timing(o);
+        o->m->n_opts--;
+        delete o;
+        return VSDMO_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,526 @@
+// dmo_kernels.hip -- DMO vertex-search and quality kernels for gfx950.
+//
+// One wave64 per vertex.  For the 3D searches the nq^3 candidates of a level
+// are split so that lane l = j + nq*k owns the nq candidates (i, j, k),
+// i = 0..nq-1, i.e. the contiguous candidate indices [nq*l, nq*l + nq): a
+// lowest-lane-then-lowest-i argmax is the reference's lowest-index tie rule.
+// The one-ring is staged in LDS as per-element constants in a vertex-local
+// frame; per element and lane the (y, z) part of det and L is formed once and
+// reused for the nq candidates along x.
+//
+// Reference kernels: k_optimizeHierarchicalInnerTet / Hex
+// (VSDMO/src/dmo/DMOTetImplCub.h:20-137, DMOHexImplCub.h:20-137) and
+// k_optimizeHierarchical2DFlat (DMOTriFlatImplCub.h:23-152).
+#include <float.h>
+
+#include "vsdmo_internal.h"
+#include "vsdmo_numerics.h"
+#include "dmo_kernels.h"
+
+namespace vsdmo {
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m));
+    return v;
+}
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m));
+    return v;
+}
+// argmax over the wave: larger score wins, equal scores -> smaller index
+__device__ __forceinline__ void wave_argmax(float& best, int& idx)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        float ob = __shfl_xor(best, m);
+        int oi = __shfl_xor(idx, m);
+        bool take = (ob > best) || (ob == best && oi < idx);
+        best = take ? ob : best;
+        idx = take ? oi : idx;
+    }
+}
+
+// ------------------------------------------------------------------ tets
+template <int NQ>
+__global__ __launch_bounds__(64) void k_search_tet(SearchArgs a)
+{
+    extern __shared__ TetC s_tc[];
+    const int lane = threadIdx.x;
+    const int vid = a.cStart + blockIdx.x;
+    const int r0 = a.ringRow[vid];
+    const int n = a.ringRow[vid + 1] - r0;
+    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
+    const bool emin = a.extentMin != 0;
+
+    // window half-extent per axis and the frame scale
+    float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
+    for (int e = lane; e < n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 3;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            int v = h[t];
+            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
+            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+        }
+    }
+    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
+    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
+    mabs = wave_max(mabs);
+    const float sg = pow2_scale(mabs);
+    const float isg = 1.0f / sg;
+    const float is3 = ((1.0f * isg) * isg) * isg;
+
+    for (int e = lane; e < n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 3;
+        f3 p[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            int v = h[t];
+            p[t] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+        }
+        s_tc[e] = tet_consts(p[0], p[1], p[2]);
+    }
+    __syncthreads();
+
+    // score of the start position (p' = 0)
+    float qcur;
+    {
+        const float px = 0.0f, py = 0.0f, pz = 0.0f;
+        const float Qyz = 3.0f * fmaf(pz, pz, py * py);
+        const float t3 = 3.0f * px;
+        TetAcc acc = tet_acc0();
+        for (int e = 0; e < n; ++e) {
+            const TetC c = s_tc[e];
+            float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+            float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+            tet_acc_add(acc, fmaf(c.nnx, px, Tyz), fmaf(px, t3 + c.m2sx, Ryz));
+        }
+        qcur = tet_acc_score(acc, is3);
+    }
+
+    const int j = lane % NQ, k = lane / NQ;
+    const bool active = lane < NQ * NQ;
+    float cx = x0x, cy = x0y, cz = x0z;
+    float ds = a.gridScale;
+    bool moved = false;
+    for (int d = 0; d < a.depth; ++d) {
+        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
+        const float gyM = cy + ds * ey, gym = cy - ds * ey;
+        const float gzM = cz + ds * ez, gzm = cz - ds * ez;
+        const float py = (grid_coord(a.aff, j, NQ, gym, gyM) - x0y) * sg;
+        const float pz = (grid_coord(a.aff, k, NQ, gzm, gzM) - x0z) * sg;
+        float px[NQ], t3[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            px[i] = (grid_coord(a.aff, i, NQ, gxm, gxM) - x0x) * sg;
+            t3[i] = 3.0f * px[i];
+        }
+        const float Qyz = 3.0f * fmaf(pz, pz, py * py);
+        TetAcc acc[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) acc[i] = tet_acc0();
+        for (int e = 0; e < n; ++e) {
+            const TetC c = s_tc[e];
+            const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+            const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)
+                tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
+        }
+        float best = -FLT_MAX;
+        int bi = 0;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            float q = tet_acc_score(acc[i], is3);
+            if (q > best) { best = q; bi = i; }
+        }
+        if (!active) best = -FLT_MAX;
+        int cand = lane * NQ + bi;
+        wave_argmax(best, cand);
+        if (qcur < best) {
+            const int wi = cand % NQ, wj = (cand / NQ) % NQ, wk = cand / (NQ * NQ);
+            cx = grid_coord(a.aff, wi, NQ, gxm, gxM);
+            cy = grid_coord(a.aff, wj, NQ, gym, gyM);
+            cz = grid_coord(a.aff, wk, NQ, gzm, gzM);
+            qcur = best;
+            moved = true;
+        }
+        ds = ds * a.shrink;
+    }
+    if (moved && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+}
+
+// ------------------------------------------------------------------ hexes
+struct HexLds { float cd[4], cF[4]; HexP hp[5]; };
+
+template <int NQ>
+__global__ __launch_bounds__(64) void k_search_hex(SearchArgs a)
+{
+    extern __shared__ HexLds s_hx[];
+    const int lane = threadIdx.x;
+    const int vid = a.cStart + blockIdx.x;
+    const int r0 = a.ringRow[vid];
+    const int n = a.ringRow[vid + 1] - r0;
+    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
+    const bool emin = a.extentMin != 0;
+
+    float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
+    for (int e = lane; e < n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 7;
+        for (int t = 0; t < 7; ++t) {
+            int v = h[t];
+            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
+            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+        }
+    }
+    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
+    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
+    mabs = wave_max(mabs);
+    const float sg = pow2_scale(mabs);
+    const float s2 = sg * sg, s3 = s2 * sg;
+    const float epsD = kHexEps * s3, epsF = kHexEps * s2;
+
+    for (int e = lane; e < n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 7;
+        f3 r[8];
+        r[0] = { 0.0f, 0.0f, 0.0f };
+        for (int t = 0; t < 7; ++t) {
+            int v = h[t];
+            r[t + 1] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+        }
+        HexLds L;
+        hex_consts(r, L.cd, L.cF, L.hp);
+        s_hx[e] = L;
+    }
+    __syncthreads();
+
+    // p-free sub-tets of the whole ring, in ring order
+    HexAcc acc0 = hex_acc0();
+    for (int e = 0; e < n; ++e)
+#pragma unroll
+        for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(acc0, s_hx[e].cd[sIdx], s_hx[e].cF[sIdx]);
+
+    float qcur;
+    {
+        const float px = 0.0f, py = 0.0f, pz = 0.0f;
+        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
+        HexAcc acc = acc0;
+        for (int e = 0; e < n; ++e)
+#pragma unroll
+            for (int sIdx = 0; sIdx < 5; ++sIdx) {
+                const HexP h = s_hx[e].hp[sIdx];
+                const bool a3 = (sIdx == 1 || sIdx == 4);
+                float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+                float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+                hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
+            }
+        qcur = hex_acc_score(acc, epsD, epsF);
+    }
+
+    const int j = lane % NQ, k = lane / NQ;
+    const bool active = lane < NQ * NQ;
+    float cx = x0x, cy = x0y, cz = x0z;
+    float ds = a.gridScale;
+    bool moved = false;
+    for (int d = 0; d < a.depth; ++d) {
+        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
+        const float gyM = cy + ds * ey, gym = cy - ds * ey;
+        const float gzM = cz + ds * ez, gzm = cz - ds * ez;
+        const float py = (grid_coord(a.aff, j, NQ, gym, gyM) - x0y) * sg;
+        const float pz = (grid_coord(a.aff, k, NQ, gzm, gzM) - x0z) * sg;
+        float px[NQ], t3[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            px[i] = (grid_coord(a.aff, i, NQ, gxm, gxM) - x0x) * sg;
+            t3[i] = 3.0f * px[i];
+        }
+        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
+        HexAcc acc[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) acc[i] = acc0;
+        for (int e = 0; e < n; ++e) {
+#pragma unroll
+            for (int sIdx = 0; sIdx < 5; ++sIdx) {
+                const HexP h = s_hx[e].hp[sIdx];
+                const bool a3 = (sIdx == 1 || sIdx == 4);
+                const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+                const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+#pragma unroll
+                for (int i = 0; i < NQ; ++i)
+                    hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
+            }
+        }
+        float best = -FLT_MAX;
+        int bi = 0;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            float q = hex_acc_score(acc[i], epsD, epsF);
+            if (q > best) { best = q; bi = i; }
+        }
+        if (!active) best = -FLT_MAX;
+        int cand = lane * NQ + bi;
+        wave_argmax(best, cand);
+        if (qcur < best) {
+            const int wi = cand % NQ, wj = (cand / NQ) % NQ, wk = cand / (NQ * NQ);
+            cx = grid_coord(a.aff, wi, NQ, gxm, gxM);
+            cy = grid_coord(a.aff, wj, NQ, gym, gyM);
+            cz = grid_coord(a.aff, wk, NQ, gzm, gzM);
+            qcur = best;
+            moved = true;
+        }
+        ds = ds * a.shrink;
+    }
+    if (moved && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+}
+
+// ------------------------------------------------------------- flat tris
+// Lane l holds the reference thread t = l/2's candidate B (l even) or A
+// (l odd): A = (t/nq, t%nq), B = (t/nq + nq/2, t%nq).  The lowest lane among
+// the maxima is the reference's winner (lowest thread; B on an exact tie).
+template <int NQ>
+__global__ __launch_bounds__(64) void k_search_flat(SearchArgs a)
+{
+    extern __shared__ TriC s_tr[];
+    const int lane = threadIdx.x;
+    const int vid = a.cStart + blockIdx.x;
+    const int h0 = a.heRow[vid];
+    const int n = a.heRow[vid + 1] - h0;
+    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
+
+    float ex = -FLT_MAX, ey = -FLT_MAX, mabs = 0.0f;
+    for (int e = lane; e < n; e += 64) {
+        int v = a.heTarget[h0 + e];
+        float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y);
+        ex = fmaxf(ex, dx);
+        ey = fmaxf(ey, dy);
+        mabs = fmaxf(mabs, fmaxf(dx, dy));
+    }
+    ex = wave_max(ex); ey = wave_max(ey);
+    mabs = wave_max(mabs);
+    const float sg = pow2_scale(mabs);
+    const float isg = 1.0f / sg;
+    const float is2 = (1.0f * isg) * isg;
+    for (int e = lane; e < n; e += 64) {
+        int va = a.heTarget[h0 + e];
+        int vb = a.heTarget[h0 + (e + 1 == n ? 0 : e + 1)];
+        s_tr[e] = tri_consts((a.x[va] - x0x) * sg, (a.y[va] - x0y) * sg, (a.x[vb] - x0x) * sg, (a.y[vb] - x0y) * sg);
+    }
+    __syncthreads();
+
+    auto score = [&](float px, float py) {
+        const float Q2 = 2.0f * (py * py);
+        const float t2 = 2.0f * px;
+        TriAcc acc = tri_acc0();
+        for (int e = 0; e < n; ++e) {
+            const TriC c = s_tr[e];
+            float Ty = fmaf(c.gy, py, c.C);
+            float Ry = fmaf(c.sy, py, c.K + Q2);
+            tri_acc_add(acc, fmaf(c.gx, px, Ty), fmaf(px, t2 + c.sx, Ry));
+        }
+        return tri_acc_score(acc, is2);
+    };
+
+    float qcur = score(0.0f, 0.0f);
+    const float qold = qcur;
+    const int t = lane >> 1;
+    const bool isA = (lane & 1) != 0;
+    const int ci = t / NQ + (isA ? 0 : NQ / 2), cj = t % NQ;
+    const bool active = t < NQ * NQ / 2;
+    float cx = x0x, cy = x0y;
+    float ds = a.gridScale;
+    float qlast = -FLT_MAX;
+    for (int d = 0; d < a.depth; ++d) {
+        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
+        const float gyM = cy + ds * ey, gym = cy - ds * ey;
+        float best = active ? score((grid_coord(a.aff, ci, NQ, gxm, gxM) - x0x) * sg,
+                                    (grid_coord(a.aff, cj, NQ, gym, gyM) - x0y) * sg)
+                            : -FLT_MAX;
+        int wl = lane;
+        wave_argmax(best, wl);
+        if (qcur < best) {
+            const int wt = wl >> 1;
+            const int wi = wt / NQ + ((wl & 1) ? 0 : NQ / 2), wj = wt % NQ;
+            cx = grid_coord(a.aff, wi, NQ, gxm, gxM);
+            cy = grid_coord(a.aff, wj, NQ, gym, gyM);
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * a.shrink;
+    }
+    (void)x0z;
+    if (qold < qlast && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; }
+}
+
+// ------------------------------------------------------------ qualities
+// element quality = ring score of the element seen from its vertex 0 at p'=0
+__global__ void k_elem_quality(QualityArgs a)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nE) return;
+    const int* c = a.cells + (size_t)e * a.npe;
+    const float x0x = a.x[c[0]], x0y = a.y[c[0]], x0z = a.z[c[0]];
+    float q;
+    if (a.elem == VSDMO_TET) {
+        float m = 0.0f;
+        for (int i = 1; i < 4; ++i) {
+            m = fmaxf(m, fabsf(a.x[c[i]] - x0x));
+            m = fmaxf(m, fabsf(a.y[c[i]] - x0y));
+            m = fmaxf(m, fabsf(a.z[c[i]] - x0z));
+        }
+        const float sg = pow2_scale(m);
+        const float isg = 1.0f / sg;
+        f3 p[3];
+        for (int i = 0; i < 3; ++i) {
+            int v = c[i + 1];
+            p[i] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+        }
+        TetC tc = tet_consts(p[0], p[1], p[2]);
+        const float px = 0.0f, py = 0.0f, pz = 0.0f;
+        const float Qyz = 3.0f * fmaf(pz, pz, py * py), t3 = 3.0f * px;
+        TetAcc acc = tet_acc0();
+        float Tyz = fmaf(tc.nny, py, fmaf(tc.nnz, pz, tc.D));
+        float Ryz = fmaf(tc.m2sy, py, fmaf(tc.m2sz, pz, tc.K + Qyz));
+        tet_acc_add(acc, fmaf(tc.nnx, px, Tyz), fmaf(px, t3 + tc.m2sx, Ryz));
+        q = tet_acc_score(acc, ((1.0f * isg) * isg) * isg);
+    } else if (a.elem == VSDMO_HEX) {
+        float m = 0.0f;
+        for (int i = 1; i < 8; ++i) {
+            m = fmaxf(m, fabsf(a.x[c[i]] - x0x));
+            m = fmaxf(m, fabsf(a.y[c[i]] - x0y));
+            m = fmaxf(m, fabsf(a.z[c[i]] - x0z));
+        }
+        const float sg = pow2_scale(m);
+        const float s2 = sg * sg, s3 = s2 * sg;
+        f3 r[8];
+        r[0] = { 0.0f, 0.0f, 0.0f };
+        for (int i = 1; i < 8; ++i) {
+            int v = c[i];
+            r[i] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+        }
+        float cd[4], cF[4];
+        HexP hp[5];
+        hex_consts(r, cd, cF, hp);
+        HexAcc acc = hex_acc0();
+        for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(acc, cd[sIdx], cF[sIdx]);
+        const float px = 0.0f, py = 0.0f, pz = 0.0f;
+        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
+        for (int sIdx = 0; sIdx < 5; ++sIdx) {
+            const bool a3 = (sIdx == 1 || sIdx == 4);
+            float Tyz = fmaf(hp[sIdx].gy, py, fmaf(hp[sIdx].gz, pz, hp[sIdx].D));
+            float Ryz = fmaf(hp[sIdx].sy, py, fmaf(hp[sIdx].sz, pz, hp[sIdx].K + (a3 ? Q3 : Q1)));
+            hex_acc_add(acc, fmaf(hp[sIdx].gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + hp[sIdx].sx, Ryz));
+        }
+        q = hex_acc_score(acc, kHexEps * s3, kHexEps * s2);
+    } else {
+        float m = 0.0f;
+        for (int i = 1; i < 3; ++i) {
+            m = fmaxf(m, fabsf(a.x[c[i]] - x0x));
+            m = fmaxf(m, fabsf(a.y[c[i]] - x0y));
+        }
+        const float sg = pow2_scale(m);
+        const float isg = 1.0f / sg;
+        TriC tc = tri_consts((a.x[c[1]] - x0x) * sg, (a.y[c[1]] - x0y) * sg, (a.x[c[2]] - x0x) * sg, (a.y[c[2]] - x0y) * sg);
+        const float px = 0.0f, py = 0.0f;
+        const float Q2 = 2.0f * (py * py), t2 = 2.0f * px;
+        TriAcc acc = tri_acc0();
+        float Ty = fmaf(tc.gy, py, tc.C);
+        float Ry = fmaf(tc.sy, py, tc.K + Q2);
+        tri_acc_add(acc, fmaf(tc.gx, px, Ty), fmaf(px, t2 + tc.sx, Ry));
+        q = tri_acc_score(acc, (1.0f * isg) * isg);
+    }
+    a.q[e] = q;
+}
+
+// per-block min and FP64 sum in a fixed tree order
+__global__ void k_reduce_q(const float* __restrict__ q, int n, float* __restrict__ pmin, double* __restrict__ psum)
+{
+    __shared__ float smin[256];
+    __shared__ double ssum[256];
+    int i = blockIdx.x * 256 + threadIdx.x;
+    float v = (i < n) ? q[i] : FLT_MAX;
+    smin[threadIdx.x] = v;
+    ssum[threadIdx.x] = (i < n) ? (double)v : 0.0;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + s]);
+            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { pmin[blockIdx.x] = smin[0]; psum[blockIdx.x] = ssum[0]; }
+}
+
+__global__ void k_reduce_final(const float* __restrict__ pmin, const double* __restrict__ psum, int n,
+                               float* __restrict__ omin, double* __restrict__ osum)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float m = FLT_MAX;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) { m = fminf(m, pmin[i]); s = s + psum[i]; }
+    *omin = m;
+    *osum = s;
+}
+
+__global__ void k_histogram(const float* __restrict__ q, int n, int ncols, int* __restrict__ bins)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = fminf(0.9999f, q[i]);
+    long idx = (long)(v * (float)ncols);
+    if (idx < 0) idx = 0;
+    atomicAdd(&bins[idx], 1);
+}
+
+// ------------------------------------------------------------- launchers
+template <int NQ>
+static void launch_nq(int kind, const SearchArgs& a, int nBlocks, size_t lds, hipStream_t s)
+{
+    if (kind == VSDMO_TET) k_search_tet<NQ><<<nBlocks, 64, lds, s>>>(a);
+    else if (kind == VSDMO_HEX) k_search_hex<NQ><<<nBlocks, 64, lds, s>>>(a);
+    else k_search_flat<NQ><<<nBlocks, 64, lds, s>>>(a);
+}
+
+size_t search_lds_bytes(int kind, int maxRing)
+{
+    size_t per = (kind == VSDMO_TET) ? sizeof(TetC) : (kind == VSDMO_HEX) ? sizeof(HexLds) : sizeof(TriC);
+    return per * (size_t)std::max(maxRing, 1);
+}
+
+void launch_search(int kind, int nq, const SearchArgs& a, size_t lds, hipStream_t s)
+{
+    int nBlocks = a.cEnd - a.cStart;
+    if (nBlocks <= 0) return;
+    switch (nq) {
+    case 2: launch_nq<2>(kind, a, nBlocks, lds, s); break;
+    case 4: launch_nq<4>(kind, a, nBlocks, lds, s); break;
+    case 6: launch_nq<6>(kind, a, nBlocks, lds, s); break;
+    case 8: launch_nq<8>(kind, a, nBlocks, lds, s); break;
+    default: break;
+    }
+}
+
+void launch_quality(const QualityArgs& a, float* pmin, double* psum, float* omin, double* osum, hipStream_t s)
+{
+    int nb = (a.nE + 255) / 256;
+    k_elem_quality<<<nb, 256, 0, s>>>(a);
+    k_reduce_q<<<nb, 256, 0, s>>>(a.q, a.nE, pmin, psum);
+    k_reduce_final<<<1, 64, 0, s>>>(pmin, psum, nb, omin, osum);
+}
+
+void launch_histogram(const float* q, int n, int ncols, int* bins, hipStream_t s)
+{
+    k_histogram<<<(n + 255) / 256, 256, 0, s>>>(q, n, ncols, bins);
+}
+
+}  // namespace vsdmo
