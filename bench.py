@@ -20,8 +20,6 @@ import os
 import sys
 import time
 
-import numpy as np
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -32,10 +30,11 @@ METRIC = "vertex updates/s at equal final min mean-ratio quality; learned vs pla
 FLOP_PER_UPDATE_TET = 2_698_080
 CBRT_PER_UPDATE_TET = 36_960
 PEAK_FP32_TFLOPS = 157.3
-# HBM bytes per launch of the tet search from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on this workload, scaled by the copy-kernel calibration; None
-# until measured
-TRAFFIC_BYTES_PER_LAUNCH = None
+# HBM bytes per launch of the tet search on this workload: rocprofv3 --pmc
+# FETCH_SIZE 5,007.5 KiB x 2.0 (copy-kernel calibration) + WRITE_SIZE 694.9 KiB
+# x 1.0, mean over the 24 dispatches of 3 iterations
+# (profiles/k_search_tet_tet40.md)
+TRAFFIC_BYTES_PER_LAUNCH = 10_966_938
 
 
 def cpu_baseline(xyz, tets, iters: int, threads: int):
@@ -63,8 +62,8 @@ def main(argv=None):
     ap.add_argument("--cpu-iters", type=int, default=2)
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: DMO here runs on a single MI355X; multi-GPU runs are not part "
-              "of this build (see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: this DMO build runs on a single MI355X and has no multi-GPU "
+              "path; run it with --gpus 1.", file=sys.stderr)
         return 2
 
     from volumesurfacediscretemeshoptimization_amd import DMOMeshTetFactory, DMOTetClass, synth

@@ -461,15 +461,27 @@ __global__ void k_reduce_q(const float* __restrict__ q, int n, float* __restrict
     if (threadIdx.x == 0) { pmin[blockIdx.x] = smin[0]; psum[blockIdx.x] = ssum[0]; }
 }
 
+// one block of 256 threads: strided partial sums then a fixed tree (the order
+// of additions depends only on n, so the FP64 sum is reproducible)
 __global__ void k_reduce_final(const float* __restrict__ pmin, const double* __restrict__ psum, int n,
                                float* __restrict__ omin, double* __restrict__ osum)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ float smin[256];
+    __shared__ double ssum[256];
     float m = FLT_MAX;
     double s = 0.0;
-    for (int i = 0; i < n; ++i) { m = fminf(m, pmin[i]); s = s + psum[i]; }
-    *omin = m;
-    *osum = s;
+    for (int i = threadIdx.x; i < n; i += 256) { m = fminf(m, pmin[i]); s = s + psum[i]; }
+    smin[threadIdx.x] = m;
+    ssum[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + k]);
+            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { *omin = smin[0]; *osum = ssum[0]; }
 }
 
 __global__ void k_histogram(const float* __restrict__ q, int n, int ncols, int* __restrict__ bins)
@@ -515,7 +527,7 @@ void launch_quality(const QualityArgs& a, float* pmin, double* psum, float* omin
     int nb = (a.nE + 255) / 256;
     k_elem_quality<<<nb, 256, 0, s>>>(a);
     k_reduce_q<<<nb, 256, 0, s>>>(a.q, a.nE, pmin, psum);
-    k_reduce_final<<<1, 64, 0, s>>>(pmin, psum, nb, omin, osum);
+    k_reduce_final<<<1, 256, 0, s>>>(pmin, psum, nb, omin, osum);
 }
 
 void launch_histogram(const float* q, int n, int ncols, int* bins, hipStream_t s)
