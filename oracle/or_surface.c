@@ -1,0 +1,623 @@
+/*
+ * or_surface.c -- oracle restatement of surface-vertex DMO (TEST
+ * INFRASTRUCTURE): local quadric fits, feature-sector fits, 1D feature-curve
+ * fits, the projection of (u,v) candidates onto the blended local surfaces,
+ * the nearest-neighbour walk and the 2D / 1D surface searches.
+ *
+ * Reference files (VSDMO/src/dmo/...):
+ *   surfaceP/Surface.h:18-93, surfaceP/Surface.cu:11-121   quadric fit + frame
+ *   surfaceP/SurfaceFeature.h:79-106, .cu:12-223             feature sectors
+ *   Surface1D.h:13-114, Surface1D.cu:7-103                    1D curve fit
+ *   SurfaceEstimation.h:122-197 (localSurfaceEstimationWithFeature), Surface1D.h:54-93
+ *   DMOCommon.cu:11-43 (k_updateNearestNeighbor), DMOCommon.h:130-165 (k_calcOptFeatureVec)
+ *   DMOHexImplCub.h:141-318 / DMOTetImplCub.h:141-322 (2D), :320-489 / :326-494 (1D)
+ *   DMOHex.cu:131-182, DMOTet.cu:128-170 (iteration order)
+ * Arithmetic follows the reference expressions literally with contraction
+ * off: dot = (x*x' + y*y') + z*z', norm = sqrtf(dot), normalize divides each
+ * component, Vec3 sums left to right.  3x3 systems are solved by an LU with
+ * partial pivoting (first maximal |pivot|, division by the pivot, then
+ * forward and back substitution), the textbook getrf/getrs recipe.
+ */
+#include "vsdmo_oracle.h"
+#include "or_internal.h"
+#include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAX_NEIGH 40
+#define TABLE_SIZE 20
+#define SEPS 1.19209e-07f
+
+typedef struct { float x, y, z; } w3;
+static inline w3 W(const float* p) { w3 r = { p[0], p[1], p[2] }; return r; }
+static inline w3 wsub(w3 a, w3 b) { w3 r = { a.x - b.x, a.y - b.y, a.z - b.z }; return r; }
+static inline w3 wadd(w3 a, w3 b) { w3 r = { a.x + b.x, a.y + b.y, a.z + b.z }; return r; }
+static inline w3 wmul(float s, w3 a) { w3 r = { s * a.x, s * a.y, s * a.z }; return r; }
+static inline float wdot(w3 a, w3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline w3 wcross(w3 a, w3 b) { w3 r = { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; return r; }
+static inline float wnorm(w3 a) { return sqrtf((a.x * a.x + a.y * a.y) + a.z * a.z); }
+static inline w3 wnormalize(w3 a) { float l = wnorm(a); w3 r = { a.x / l, a.y / l, a.z / l }; return r; }
+
+/* localSurface: p0, e0, e1, e2, a, b, c (15 floats) + numPoints */
+typedef struct { w3 p0, e[3]; float a, b, c; int np; } lsurf;
+typedef struct { w3 p0, e[2]; float a; } lsurf1;
+
+static inline float poly(const lsurf* s, float u, float v) { return (s->a * u * u + s->b * u * v) + s->c * v * v; }
+static inline w3 l2w(const lsurf* s, float u, float v, float w)
+{
+    return wadd(wadd(wadd(s->p0, wmul(u, s->e[0])), wmul(v, s->e[1])), wmul(w, s->e[2]));
+}
+
+/* 3x3 LU with partial pivoting + solve; A column-major as in the reference */
+static void lu_solve3(const float* Acm, const float* rhs, float* x)
+{
+    float A[3][3], b[3];
+    for (int i = 0; i < 3; ++i) { for (int j = 0; j < 3; ++j) A[i][j] = Acm[i + 3 * j]; b[i] = rhs[i]; }
+    int piv[3] = { 0, 1, 2 };
+    for (int j = 0; j < 3; ++j) {
+        int p = j;
+        float mx = fabsf(A[j][j]);
+        for (int i = j + 1; i < 3; ++i) if (fabsf(A[i][j]) > mx) { mx = fabsf(A[i][j]); p = i; }
+        if (p != j) {
+            for (int k = 0; k < 3; ++k) { float t = A[j][k]; A[j][k] = A[p][k]; A[p][k] = t; }
+            int t = piv[j]; piv[j] = piv[p]; piv[p] = t;
+        }
+        for (int i = j + 1; i < 3; ++i) {
+            A[i][j] = A[i][j] / A[j][j];
+            for (int k = j + 1; k < 3; ++k) A[i][k] = A[i][k] - A[i][j] * A[j][k];
+        }
+    }
+    float y[3];
+    for (int i = 0; i < 3; ++i) y[i] = b[piv[i]];
+    for (int i = 1; i < 3; ++i) for (int k = 0; k < i; ++k) y[i] = y[i] - A[i][k] * y[k];
+    for (int i = 2; i >= 0; --i) {
+        for (int k = i + 1; k < 3; ++k) y[i] = y[i] - A[i][k] * y[k];
+        y[i] = y[i] / A[i][i];
+    }
+    x[0] = y[0]; x[1] = y[1]; x[2] = y[2];
+}
+
+/* frame from a normal and a neighbour (computeLocalCoordinates, Surface.h:66-93) */
+static void frame3(w3 n, w3 r, w3* e0, w3* e1, w3* e2)
+{
+    if (wnorm(wcross(n, r)) < 10.0f * SEPS) { w3 rr = { n.x, n.z, -n.y }; r = rr; }
+    w3 t1 = wcross(n, r);
+    w3 t2 = wcross(n, t1);
+    *e2 = wnormalize(n);
+    *e0 = wnormalize(t1);
+    *e1 = wnormalize(t2);
+}
+
+/* accumulate the normal equations of w = a u^2 + b uv + c v^2 and solve */
+static void fit_quadric(lsurf* s, const w3* pts, int np)
+{
+    float M[9] = { 0 }, R[3] = { 0 };
+    for (int i = 0; i < np; ++i) {
+        w3 p = wsub(pts[i], s->p0);
+        float u = wdot(s->e[0], p), v = wdot(s->e[1], p), w = wdot(s->e[2], p);
+        R[0] += u * u * w; R[1] += u * v * w; R[2] += v * v * w;
+        M[0] += u * u * u * u; M[1] += u * u * u * v; M[4] += u * u * v * v;
+        M[2] += u * u * v * v; M[5] += u * v * v * v; M[8] += v * v * v * v;
+    }
+    M[3] = M[1]; M[6] = M[2]; M[7] = M[5];
+    float abc[3];
+    lu_solve3(M, R, abc);
+    s->a = abc[0]; s->b = abc[1]; s->c = abc[2];
+    s->np = np;
+    if (np <= 4) { s->a = 0.0f; s->b = 0.0f; s->c = 0.0f; }
+    if (s->a != s->a) s->a = 0.0f;
+    if (s->b != s->b) s->b = 0.0f;
+    if (s->c != s->c) s->c = 0.0f;
+}
+
+struct or_surf {
+    int nS, nFree;
+    float* pinit;         /* 3*nS: surface positions at optimiser creation */
+    float* vnorm;         /* 3*nS */
+    float* fnorm;         /* 3*nFaces */
+    lsurf* ls;            /* nS */
+    lsurf1* ls1;          /* nS (feature vertices) */
+    int* frow;            /* nFeature+1 */
+    int* table;           /* nFeature*TABLE_SIZE */
+    lsurf* fs;            /* feature sector surfaces */
+    int* nn;              /* nS */
+    unsigned char* opt1d; /* nS */
+};
+
+static const float* P(const or_mesh* m, int v) { return m->xyz + 3 * (size_t)v; }
+
+/* face normals (k_calcFaceNormals) and vertex normals (k_calcVertexNormals):
+   normalized sum of incident face normals in CCW order */
+static void normals(const or_mesh* m, float* fn, float* vn)
+{
+    int k = m->fpe;
+    for (int f = 0; f < m->nFaces; ++f) {
+        const int* F = m->faces + (size_t)f * k;
+        w3 p0 = W(P(m, F[0])), p1 = W(P(m, F[1])), p2 = W(P(m, F[k == 3 ? 2 : 3]));
+        w3 n = wnormalize(wcross(wsub(p1, p0), wsub(p2, p0)));
+        fn[3 * f] = n.x; fn[3 * f + 1] = n.y; fn[3 * f + 2] = n.z;
+    }
+    for (int v = 0; v < m->nSurf; ++v) {
+        w3 n = { 0.0f, 0.0f, 0.0f };
+        for (int h = m->heRow[v]; h < m->heRow[v + 1]; ++h)
+            if (m->heFace[h] != -1) n = wadd(n, W(fn + 3 * m->heFace[h]));
+        n = wnormalize(n);
+        vn[3 * v] = n.x; vn[3 * v + 1] = n.y; vn[3 * v + 2] = n.z;
+    }
+}
+
+or_surf* or_surface_init(const or_mesh* m)
+{
+    or_surf* s = (or_surf*)calloc(1, sizeof(or_surf));
+    int nS = m->nSurf;
+    s->nS = nS; s->nFree = m->nFree;
+    int nF = m->nFeature;
+    s->pinit = (float*)malloc(sizeof(float) * 3 * (size_t)(nS > 0 ? nS : 1));
+    memcpy(s->pinit, m->xyz, sizeof(float) * 3 * (size_t)nS);
+    s->vnorm = (float*)malloc(sizeof(float) * 3 * (size_t)(nS > 0 ? nS : 1));
+    s->fnorm = (float*)malloc(sizeof(float) * 3 * (size_t)(m->nFaces > 0 ? m->nFaces : 1));
+    normals(m, s->fnorm, s->vnorm);
+    s->ls = (lsurf*)calloc((size_t)(nS > 0 ? nS : 1), sizeof(lsurf));
+    s->ls1 = (lsurf1*)calloc((size_t)(nS > 0 ? nS : 1), sizeof(lsurf1));
+    s->nn = (int*)malloc(sizeof(int) * (size_t)(nS > 0 ? nS : 1));
+    s->opt1d = (unsigned char*)calloc((size_t)(nS > 0 ? nS : 1), 1);
+    for (int v = 0; v < nS; ++v) s->nn[v] = v;
+
+    /* 1D curve fits of feature vertices (Surface1D.cu:7-91) */
+    for (int v = 0; v < nS; ++v) {
+        if (!m->isFeature[v]) continue;
+        int nb[MAX_NEIGH + 1], np = 0, vf = -1;
+        nb[np++] = v;
+        for (int h = m->heRow[v]; h < m->heRow[v + 1]; ++h) {
+            int d = m->heTarget[h];
+            if (m->isFeature[d] && np < MAX_NEIGH) { nb[np++] = d; vf = d; }
+        }
+        if (vf < 0) continue;
+        w3 n = m->flat ? (w3){ 0.0f, 0.0f, 1.0f } : W(s->vnorm + 3 * v);
+        w3 r = wsub(W(P(m, vf)), W(P(m, v)));
+        if (wnorm(wcross(n, r)) < 10.0f * SEPS) { w3 rr = { n.x, n.z, -n.y }; r = rr; }
+        w3 t1 = wcross(n, r), t2 = wcross(n, t1);
+        n = wnormalize(n); t2 = wnormalize(t2);
+        lsurf1* q = &s->ls1[v];
+        q->p0 = W(P(m, v)); q->e[0] = t2; q->e[1] = n;
+        float ATA = 0.0f, ATb = 0.0f;
+        for (int i = 0; i < np; ++i) {
+            w3 p = wsub(W(P(m, nb[i])), q->p0);
+            float u = wdot(q->e[0], p), vv = wdot(q->e[1], p);
+            ATb += u * u * vv;
+            ATA += u * u * u * u;
+        }
+        q->a = ATb / ATA;
+    }
+    /* quadric fits of all surface vertices (Surface.cu:11-105) */
+    for (int v = 0; v < nS; ++v) {
+        w3 pts[MAX_NEIGH + 1];
+        int np = 0;
+        pts[np++] = W(P(m, v));
+        for (int h = m->heRow[v]; h < m->heRow[v + 1] && np < MAX_NEIGH; ++h) pts[np++] = W(P(m, m->heTarget[h]));
+        if (np == 5) pts[np++] = wmul(0.5f, wadd(pts[2], pts[3]));
+        lsurf* q = &s->ls[v];
+        q->p0 = pts[0];
+        int first = m->heTarget[m->heRow[v]];
+        frame3(W(s->vnorm + 3 * v), wsub(W(P(m, first)), W(P(m, v))), &q->e[0], &q->e[1], &q->e[2]);
+        fit_quadric(q, pts, np);
+    }
+    /* feature sectors (SurfaceFeature.cu:134-223) */
+    s->frow = (int*)calloc((size_t)nF + 1, sizeof(int));
+    s->table = (int*)malloc(sizeof(int) * (size_t)(nF > 0 ? nF : 1) * TABLE_SIZE);
+    for (int i = 0; i < nF * TABLE_SIZE; ++i) s->table[i] = -1;
+    for (int i = 0; i < nF; ++i) {
+        int v = m->nFree + i, cnt = 0;
+        for (int h = m->heRow[v] + 1; h < m->heRow[v + 1]; ++h) if (m->isFeature[m->heTarget[h]]) ++cnt;
+        if (!m->isBoundary1D[v]) ++cnt;
+        s->frow[i + 1] = s->frow[i] + cnt;
+    }
+    int nSec = s->frow[nF];
+    s->fs = (lsurf*)calloc((size_t)(nSec > 0 ? nSec : 1), sizeof(lsurf));
+    int* secN = (int*)malloc(sizeof(int) * (size_t)(nSec > 0 ? nSec : 1) * MAX_NEIGH);
+    int* secCnt = (int*)calloc((size_t)(nSec > 0 ? nSec : 1), sizeof(int));
+    for (int i = 0; i < nF; ++i) {
+        int v = m->nFree + i;
+        int h0 = m->heRow[v], h1 = m->heRow[v + 1];
+        int sid = 0, nneigh = 0, lastWasFeature = 1, last = h0, vindex = 1;
+        s->table[TABLE_SIZE * i] = 0;
+        const int nsec = s->frow[i + 1] - s->frow[i];
+        /* non-feature neighbours after the last feature neighbour of an open
+           boundary vertex belong to no sector: they are dropped (sid == nsec) */
+        for (int h = h0 + 1; h < h1; ++h) {
+            int d = m->heTarget[h];
+            int* L = secN + (size_t)MAX_NEIGH * (s->frow[i] + (sid < nsec ? sid : 0));
+            const int own = sid < nsec;
+            if (lastWasFeature) { nneigh = 0; if (own) { L[nneigh++] = v; L[nneigh++] = m->heTarget[last]; } }
+            if (m->isFeature[d]) {
+                if (vindex < TABLE_SIZE) s->table[TABLE_SIZE * i + vindex] = sid;
+                if (own && nneigh < MAX_NEIGH) L[nneigh++] = d;
+                if (own) secCnt[s->frow[i] + sid] = nneigh;
+                ++sid; nneigh = 0; lastWasFeature = 1;
+            } else {
+                if (vindex < TABLE_SIZE) s->table[TABLE_SIZE * i + vindex] = sid;
+                if (own && nneigh < MAX_NEIGH) L[nneigh++] = d;
+                lastWasFeature = 0;
+            }
+            ++last; ++vindex;
+        }
+        if (!m->isBoundary1D[v] && sid < nsec) {
+            int* L = secN + (size_t)MAX_NEIGH * (s->frow[i] + sid);
+            if (lastWasFeature) { nneigh = 0; L[nneigh++] = v; L[nneigh++] = m->heTarget[last]; }
+            if (nneigh < MAX_NEIGH) L[nneigh++] = m->heTarget[h0];
+            secCnt[s->frow[i] + sid] = nneigh;
+        }
+    }
+    for (int q = 0; q < nSec; ++q) {
+        const int* L = secN + (size_t)MAX_NEIGH * q;
+        int np = secCnt[q];
+        if (np < 2) continue;            /* not reached for closed surfaces */
+        int v = L[0];
+        w3 pts[MAX_NEIGH + 1];
+        for (int i = 0; i < np; ++i) pts[i] = W(P(m, L[i]));
+        if (np == 5) { pts[np] = wmul(0.5f, wadd(pts[2], pts[3])); ++np; }
+        w3 n = { 0.0f, 0.0f, 0.0f };
+        for (int h = m->heRow[v]; h < m->heRow[v + 1]; ++h)
+            if (m->heTarget[h] == L[1] && m->heFace[h] >= 0) n = W(s->fnorm + 3 * m->heFace[h]);
+        lsurf* fq = &s->fs[q];
+        fq->p0 = pts[0];
+        frame3(n, wsub(W(P(m, L[1])), W(P(m, v))), &fq->e[0], &fq->e[1], &fq->e[2]);
+        fit_quadric(fq, pts, np);
+    }
+    free(secN); free(secCnt);
+    /* feature eligibility for the 1D search (DMOCommon.h:130-165): two
+       feature neighbours enclosing >= 160 degrees, written as cos <= cos 160 */
+    const float c160 = (float)cos(160.0 * 3.14159265358979323846 / 180.0);
+    for (int v = 0; v < nS; ++v) {
+        if (!m->isFeature[v]) continue;
+        int nn = 0; w3 pp[2];
+        for (int h = m->heRow[v]; h < m->heRow[v + 1]; ++h) {
+            int d = m->heTarget[h];
+            if (m->isFeature[d]) { if (nn < 2) pp[nn] = W(P(m, d)); ++nn; }
+        }
+        if (nn == 2) {
+            w3 a = wsub(pp[0], W(P(m, v))), b = wsub(pp[1], W(P(m, v)));
+            float c = wdot(a, b) / sqrtf(wdot(a, a) * wdot(b, b));
+            if (c <= c160) s->opt1d[v] = 1;
+        }
+    }
+    return s;
+}
+
+void or_surface_free(or_surf* s)
+{
+    if (!s) return;
+    free(s->pinit); free(s->vnorm); free(s->fnorm); free(s->ls); free(s->ls1);
+    free(s->frow); free(s->table); free(s->fs); free(s->nn); free(s->opt1d);
+    free(s);
+}
+
+/* projection of (u,v) (localSurfaceEstimationWithFeature, SurfaceEstimation.h:122-197) */
+static w3 project2d(const or_mesh* m, const or_surf* s, const float* snap, int vid, const float* vidPos, float u, float v)
+{
+    const lsurf* L = &s->ls[s->nn[vid]];
+    float w = poly(L, u, v);
+    w3 pw = l2w(L, u, v, w);
+    w3 pl[MAX_NEIGH + 1];
+    float inv[MAX_NEIGH + 1];
+    float isum = 0.0f;
+    pl[0] = pw;
+    {
+        w3 d = wsub(W(s->pinit + 3 * s->nn[vid]), pw);
+        inv[0] = 1.0f / (wdot(d, d) + SEPS);
+        isum += inv[0];
+    }
+    int h0 = m->heRow[vid], nh = m->heRow[vid + 1] - h0;
+    if (nh > MAX_NEIGH) nh = MAX_NEIGH;
+    for (int i = 0; i < nh; ++i) {
+        int vn = m->heTarget[h0 + i];
+        int nn = s->nn[vn];
+        w3 d = wsub(W(s->pinit + 3 * nn), pw);
+        inv[i + 1] = 1.0f / (wdot(d, d) + SEPS);
+        isum += inv[i + 1];
+        const lsurf* LN;
+        if (m->isFeature[nn]) {
+            int off = nn - s->nFree;
+            float md = 3.40282347e+38f;
+            int arg = 0, idx = 0;
+            w3 vp = W(vidPos);
+            for (int h = m->heRow[nn]; h < m->heRow[nn + 1]; ++h, ++idx) {
+                w3 t = wsub(W(snap + 3 * (size_t)m->heTarget[h]), vp);
+                float dd = wdot(t, t);
+                if (dd < md) { md = dd; arg = idx; }
+            }
+            int sid = (arg < TABLE_SIZE) ? s->table[TABLE_SIZE * off + arg] : -1;
+            const int nsec = s->frow[off + 1] - s->frow[off];
+            if (sid < 0 || sid >= nsec) sid = 0;
+            LN = nsec > 0 ? &s->fs[s->frow[off] + sid] : &s->ls[nn];
+        } else {
+            LN = &s->ls[nn];
+        }
+        w3 q = wsub(pw, LN->p0);
+        float lu = wdot(q, LN->e[0]), lv = wdot(q, LN->e[1]);
+        float lw = poly(LN, lu, lv);
+        pl[i + 1] = l2w(LN, lu, lv, lw);
+    }
+    w3 sum = { 0.0f, 0.0f, 0.0f };
+    float ii = 1.0f / isum;
+    for (int i = 0; i < nh + 1; ++i) sum = wadd(sum, wmul(ii, wmul(inv[i], pl[i])));
+    return sum;
+}
+
+/* projection onto the 1D curves (Surface1D.h:54-93) over the feature
+   neighbours nb[0..n) */
+static w3 project1d(const or_surf* s, int vid, const int* nb, int n, float u)
+{
+    const lsurf1* L = &s->ls1[s->nn[vid]];
+    float v = L->a * u * u;
+    w3 pw = wadd(wadd(L->p0, wmul(u, L->e[0])), wmul(v, L->e[1]));
+    w3 pl[MAX_NEIGH + 1];
+    float inv[MAX_NEIGH + 1];
+    float isum = 0.0f;
+    pl[0] = pw;
+    {
+        w3 d = wsub(W(s->pinit + 3 * s->nn[vid]), pw);
+        inv[0] = 1.0f / (wdot(d, d) + SEPS);
+        isum += inv[0];
+    }
+    for (int i = 0; i < n; ++i) {
+        int nn = s->nn[nb[i]];
+        w3 d = wsub(W(s->pinit + 3 * nn), pw);
+        inv[i + 1] = 1.0f / (wdot(d, d) + SEPS);
+        isum += inv[i + 1];
+        const lsurf1* LN = &s->ls1[nn];
+        w3 q = wsub(pw, LN->p0);
+        float lu = wdot(q, LN->e[0]);
+        float lv = LN->a * lu * lu;
+        pl[i + 1] = wadd(wadd(LN->p0, wmul(lu, LN->e[0])), wmul(lv, LN->e[1]));
+    }
+    w3 sum = { 0.0f, 0.0f, 0.0f };
+    float ii = 1.0f / isum;
+    for (int i = 0; i < n + 1; ++i) sum = wadd(sum, wmul(ii, wmul(inv[i], pl[i])));
+    return sum;
+}
+
+/* ring score of a volume vertex at an arbitrary point: canonical evaluation
+   (or_internal.h) with the frame of the vertex's start position */
+typedef struct {
+    int tet, n;
+    v3 x0;
+    float sigma, is3, epsD, epsF;
+    tetc tc[512];
+    float cd[4 * 64], cF[4 * 64];
+    hexp hp[5 * 64];
+    hexacc hacc;
+    float minDist, maxDist;
+} vring;
+
+static void vring_init(const or_mesh* m, int vid, const float* xyz, vring* r)
+{
+    r->tet = (m->elem == OR_TET);
+    int st = m->ringStride, r0 = m->ringRow[vid];
+    r->n = m->ringRow[vid + 1] - r0;
+    r->x0 = ld3(xyz + 3 * (size_t)vid);
+    float mabs = 0.0f, mn = FLT_MAX, mx = -FLT_MAX;
+    for (int e = 0; e < r->n; ++e)
+        for (int k = 0; k < st; ++k) {
+            w3 q = W(xyz + 3 * (size_t)m->ringVal[(size_t)(r0 + e) * st + k]);
+            w3 d = { q.x - r->x0.x, q.y - r->x0.y, q.z - r->x0.z };
+            mabs = fmaxf(mabs, fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z))));
+            float dist = wnorm(d);
+            mn = fminf(mn, dist); mx = fmaxf(mx, dist);
+        }
+    r->minDist = mn; r->maxDist = mx;
+    r->sigma = or_pow2_scale(mabs);
+    float is = 1.0f / r->sigma;
+    r->is3 = ((1.0f * is) * is) * is;
+    float s2 = r->sigma * r->sigma, s3 = s2 * r->sigma;
+    r->epsD = HEX_EPS * s3; r->epsF = HEX_EPS * s2;
+    for (int e = 0; e < r->n; ++e) {
+        const int* h = m->ringVal + (size_t)(r0 + e) * st;
+        if (r->tet) {
+            r->tc[e] = tet_consts(loc3(xyz + 3 * (size_t)h[0], r->x0, r->sigma), loc3(xyz + 3 * (size_t)h[1], r->x0, r->sigma),
+                                  loc3(xyz + 3 * (size_t)h[2], r->x0, r->sigma));
+        } else {
+            v3 q[8];
+            q[0] = r->x0;
+            for (int k = 1; k < 8; ++k) q[k] = loc3(xyz + 3 * (size_t)h[k - 1], r->x0, r->sigma);
+            hex_consts(q, r->cd + 4 * e, r->cF + 4 * e, r->hp + 5 * e);
+        }
+    }
+    if (!r->tet) r->hacc = hex_const_acc(r->n, r->cd, r->cF);
+}
+
+static float vring_score(const vring* r, w3 p)
+{
+    float px = (p.x - r->x0.x) * r->sigma, py = (p.y - r->x0.y) * r->sigma, pz = (p.z - r->x0.z) * r->sigma;
+    return r->tet ? tet_score(r->n, r->tc, px, py, pz, r->is3) : hex_score(r->n, r->hacc, r->hp, px, py, pz, r->epsD, r->epsF);
+}
+
+/* 2D surface search of a free surface vertex (k_optimizeHierarchical2DHex /
+   2DTet): (u,v) window around the current position in the nearest
+   neighbour's frame, half-width s_d * (min distance to the ring points for
+   hexes, max for tets); thread t holds (t/nq, t%nq) and (t/nq + nq/2, t%nq);
+   B on exact ties; lowest thread wins; write iff q(old) < last level max */
+static int search2d(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz)
+{
+    vring* r = (vring*)malloc(sizeof(vring));
+    vring_init(m, vid, xyz, r);
+    if (r->n <= 0) { free(r); return 0; }
+    const float dist = (m->elem == OR_HEX) ? r->minDist : r->maxDist;
+    const lsurf* L = &s->ls[s->nn[vid]];
+    float vidPos[3] = { xyz[3 * (size_t)vid], xyz[3 * (size_t)vid + 1], xyz[3 * (size_t)vid + 2] };
+    w3 cur = W(vidPos);
+    w3 cv = wsub(cur, L->p0);
+    float cu = wdot(L->e[0], cv), cvv = wdot(L->e[1], cv);
+    const int nq = p->nq, half = nq / 2, T = nq * nq / 2;
+    const float aff = 1.0f / (float)(nq - 1);
+    const float shr = (p->shrink > 0.0f) ? p->shrink : 2.0f / (float)(nq - 1);
+    float qcur = vring_score(r, cur);
+    const float qold = qcur;
+    float qlast = -FLT_MAX;
+    float ds = p->grid_scale;
+    for (int d = 0; d < p->depth; ++d) {
+        float uMax = ds * dist, uMin = -(ds * dist);
+        float best = -FLT_MAX, bu = 0.0f, bv = 0.0f;
+        for (int t = 0; t < T; ++t) {
+            int i1 = t / nq, j1 = t % nq, i2 = i1 + half;
+            float u1 = cu + aff * ((float)i1 * uMin + (float)(nq - 1 - i1) * uMax);
+            float v1 = cvv + aff * ((float)j1 * uMin + (float)(nq - 1 - j1) * uMax);
+            float u2 = cu + aff * ((float)i2 * uMin + (float)(nq - 1 - i2) * uMax);
+            float v2 = v1;
+            float q1 = vring_score(r, project2d(m, s, snap, vid, vidPos, u1, v1));
+            float q2 = vring_score(r, project2d(m, s, snap, vid, vidPos, u2, v2));
+            float qm = q1, uu = u1, vv = v1;
+            if (q2 > q1) { qm = q2; uu = u2; vv = v2; }
+            if (qm > best) { best = qm; bu = uu; bv = vv; }
+        }
+        if (qcur < best) {
+            cur = project2d(m, s, snap, vid, vidPos, bu, bv);
+            cu = bu; cvv = bv;
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * shr;
+    }
+    free(r);
+    if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
+    return 0;
+}
+
+/* 1D feature-curve search (k_optimizeHierarchical1DHex / 1DTet) */
+static int search1d(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, float* xyz)
+{
+    if (!s->opt1d[vid]) return 0;
+    vring* r = (vring*)malloc(sizeof(vring));
+    vring_init(m, vid, xyz, r);
+    if (r->n <= 0) { free(r); return 0; }
+    /* feature neighbours that themselves have exactly two feature neighbours */
+    int nb[MAX_NEIGH], nnb = 0;
+    for (int h = m->heRow[vid]; h < m->heRow[vid + 1]; ++h) {
+        int d = m->heTarget[h];
+        if (!m->isFeature[d]) continue;
+        int c = 0;
+        for (int h2 = m->heRow[d]; h2 < m->heRow[d + 1]; ++h2) if (m->isFeature[m->heTarget[h2]]) ++c;
+        if (c == 2 && nnb < MAX_NEIGH) nb[nnb++] = d;
+    }
+    const float dist = (m->elem == OR_HEX) ? r->minDist : r->maxDist;
+    const lsurf1* L = &s->ls1[s->nn[vid]];
+    w3 cur = W(xyz + 3 * (size_t)vid);
+    w3 cv = wsub(cur, L->p0);
+    float cu = wdot(L->e[0], cv);
+    const int nq = p->nq;
+    const float aff = 1.0f / (float)(nq - 1);
+    const float shr = (p->shrink > 0.0f) ? p->shrink : 2.0f / (float)(nq - 1);
+    float qcur = vring_score(r, cur);
+    const float qold = qcur;
+    float qlast = -FLT_MAX;
+    float ds = p->grid_scale;
+    for (int d = 0; d < p->depth; ++d) {
+        float uMax = ds * dist, uMin = -(ds * dist);
+        float best = -FLT_MAX, bu = 0.0f;
+        for (int i = 0; i < nq; ++i) {
+            float u = cu + aff * ((float)i * uMin + (float)(nq - 1 - i) * uMax);
+            float q = vring_score(r, project1d(s, vid, nb, nnb, u));
+            if (q > best) { best = q; bu = u; }
+        }
+        if (qcur < best) {
+            cur = project1d(s, vid, nb, nnb, bu);
+            cu = bu;
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * shr;
+    }
+    free(r);
+    if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
+    return 0;
+}
+
+/* nearest-neighbour walk over the initial surface positions, within the
+   vertex's feature class (k_updateNearestNeighbor, DMOCommon.cu:11-43) */
+static void update_nn(const or_mesh* m, or_surf* s)
+{
+    for (int v = 0; v < m->nSurf; ++v) {
+        w3 p = W(P(m, v));
+        int vh = s->nn[v];
+        w3 d0 = wsub(p, W(s->pinit + 3 * vh));
+        float sq = wdot(d0, d0);
+        int vnew;
+        do {
+            vnew = vh;
+            int c = vh;
+            for (int h = m->heRow[c]; h < m->heRow[c + 1]; ++h) {
+                int dst = m->heTarget[h];
+                if (m->isFeature[v] != m->isFeature[dst]) continue;
+                w3 dd = wsub(p, W(s->pinit + 3 * dst));
+                float sd = wdot(dd, dd);
+                if (sd < sq) { sq = sd; vh = dst; }
+            }
+        } while (vnew != vh);
+        s->nn[v] = vnew;
+    }
+}
+
+long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* p, float lastMin, long long* nSurf2d, long long* nFeat1d)
+{
+    long long cnt = 0;
+#ifdef _OPENMP
+    int nt = p->nthreads > 0 ? p->nthreads : omp_get_max_threads();
+#endif
+    *nSurf2d = 0; *nFeat1d = 0;
+    cnt += or_dmo_iteration(m, p);
+    if (!p->fixed_boundary && lastMin > 0.0f && s) {
+        /* the feature-sector choice of the projection reads positions of
+           vertices up to two hops away, which may belong to the colour class
+           being updated: it reads a snapshot taken at the start of the pass */
+        float* snap = (float*)malloc(sizeof(float) * 3 * (size_t)(m->nSurf > 0 ? m->nSurf : 1));
+        for (int c = 0; c < m->nColorsFree; ++c) {
+            int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
+            memcpy(snap, m->xyz, sizeof(float) * 3 * (size_t)m->nSurf);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+            for (int v = lo; v < hi; ++v) search2d(m, s, snap, p, v, m->xyz);
+            *nSurf2d += hi - lo;
+        }
+        free(snap);
+        for (int c = 0; c < m->nColorsFeature; ++c) {
+            int lo = m->colOffFeature[c], hi = m->colOffFeature[c + 1];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+            for (int v = lo; v < hi; ++v) search1d(m, s, p, v, m->xyz);
+            for (int v = lo; v < hi; ++v) *nFeat1d += s->opt1d[v];
+        }
+    }
+    if (s) update_nn(m, s);
+    return cnt + *nSurf2d + *nFeat1d;
+}
+
+/* accessors for tests */
+void or_surface_get(const or_surf* s, int what, void* out)
+{
+    switch (what) {
+    case 0: memcpy(out, s->nn, sizeof(int) * (size_t)s->nS); break;
+    case 1: for (int v = 0; v < s->nS; ++v) {
+                float* o = (float*)out + 16 * (size_t)v;
+                const lsurf* q = &s->ls[v];
+                o[0] = q->p0.x; o[1] = q->p0.y; o[2] = q->p0.z;
+                for (int k = 0; k < 3; ++k) { o[3 + 3 * k] = q->e[k].x; o[4 + 3 * k] = q->e[k].y; o[5 + 3 * k] = q->e[k].z; }
+                o[12] = q->a; o[13] = q->b; o[14] = q->c; o[15] = (float)q->np;
+            }
+            break;
+    case 2: for (int v = 0; v < s->nS; ++v) ((unsigned char*)out)[v] = s->opt1d[v]; break;
+    case 3: for (int v = 0; v < s->nS; ++v) {
+                float* o = (float*)out + 10 * (size_t)v;
+                const lsurf1* q = &s->ls1[v];
+                o[0] = q->p0.x; o[1] = q->p0.y; o[2] = q->p0.z;
+                o[3] = q->e[0].x; o[4] = q->e[0].y; o[5] = q->e[0].z;
+                o[6] = q->e[1].x; o[7] = q->e[1].y; o[8] = q->e[1].z; o[9] = q->a;
+            }
+            break;
+    default: break;
+    }
+}
+
+int or_surface_nsectors(const or_surf* s, int nFeature) { return s->frow[nFeature]; }

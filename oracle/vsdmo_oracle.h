@@ -126,6 +126,22 @@ long long or_dmo_iteration(or_mesh* m, const or_dmo_params* p);
 /* search one vertex; exported for brute-force tests.  Returns 1 if moved. */
 int or_dmo_vertex(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz_inout);
 
+/* ---- surface vertices ---- */
+typedef struct or_surf or_surf;
+/* local surfaces, feature sectors, 1D curves, NN table, 1D eligibility;
+   built from the mesh's current positions */
+or_surf* or_surface_init(const or_mesh* m);
+void or_surface_free(or_surf* s);
+/* inner pass, then (if !fixed_boundary and lastMin > 0) free-surface 2D and
+   feature 1D passes, then the nearest-neighbour update; returns searches */
+long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* p, float lastMin,
+                                   long long* nSurf2d, long long* nFeat1d);
+/* what: 0 NN table (int nS), 1 local surfaces (16 floats each: p0, e0, e1,
+   e2, a, b, c, numPoints), 2 1D eligibility (uint8 nS), 3 1D curves (10
+   floats: p0, e0, e1, a) */
+void or_surface_get(const or_surf* s, int what, void* out);
+int or_surface_nsectors(const or_surf* s, int nFeature);
+
 /* element quality statistics in FP32 canonical (min) and FP64 (min, mean) */
 float or_min_quality(const or_mesh* m);
 void or_quality64(const or_mesh* m, double* qmin, double* qmean);

@@ -80,6 +80,15 @@ def lib():
         L.or_min_quality.argtypes = [C.POINTER(OrMesh)]
         L.or_quality64.argtypes = [C.POINTER(OrMesh), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_element_qualities.argtypes = [C.POINTER(OrMesh), _fp]
+        L.or_surface_init.restype = C.c_void_p
+        L.or_surface_init.argtypes = [C.POINTER(OrMesh)]
+        L.or_surface_free.argtypes = [C.c_void_p]
+        L.or_dmo_iteration_surface.restype = C.c_longlong
+        L.or_dmo_iteration_surface.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams), C.c_float,
+                                               C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.or_surface_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.or_surface_nsectors.restype = C.c_int
+        L.or_surface_nsectors.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -157,6 +166,9 @@ class OracleMesh:
         self.rc = rc
 
     def __del__(self):
+        if getattr(self, "surf", None):
+            lib().or_surface_free(self.surf)
+            self.surf = None
         if getattr(self, "ptr", None):
             lib().or_mesh_free(self.ptr)
             self.ptr = None
@@ -221,6 +233,37 @@ class OracleMesh:
         a = np.ascontiguousarray(xyz, dtype=np.float32).copy()
         moved = lib().or_dmo_vertex(self.ptr, C.byref(p), vid, a.ctypes.data_as(_fp))
         return moved, a
+
+    def surface_init(self):
+        """Local surfaces etc. from the current positions (optimiser init)."""
+        if getattr(self, "surf", None):
+            lib().or_surface_free(self.surf)
+        self.surf = lib().or_surface_init(self.ptr)
+        self.last_min = self.min_quality()
+
+    def iterate_surface(self, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None, nthreads=0):
+        """Inner pass, surface 2D / feature 1D passes gated on the last minimum
+        quality, NN update (reference doIteration); returns (2D, 1D) counts."""
+        if extent is None:
+            extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, 0, nthreads)
+        a, b = C.c_longlong(), C.c_longlong()
+        lib().or_dmo_iteration_surface(self.ptr, self.surf, C.byref(p), C.c_float(self.last_min), C.byref(a), C.byref(b))
+        self.last_min = self.min_quality()
+        return a.value, b.value
+
+    def surface_get(self, what: int):
+        n = self.m.nSurf
+        if what == 0:
+            out = np.empty(n, np.int32)
+        elif what == 1:
+            out = np.empty((n, 16), np.float32)
+        elif what == 2:
+            out = np.empty(n, np.uint8)
+        else:
+            out = np.empty((n, 10), np.float32)
+        lib().or_surface_get(self.surf, what, out.ctypes.data_as(C.c_void_p))
+        return out
 
     def min_quality(self) -> float:
         return lib().or_min_quality(self.ptr)

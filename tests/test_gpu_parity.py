@@ -132,3 +132,28 @@ def test_c1_full_size(oracle):
     st = opt.stats()
     assert st["searches_inner"] == 10 * 59319
     opt.close()
+
+
+@pytest.mark.parametrize("kind,n,jit", [("tet", 10, 0.35), ("hex", 8, 0.15)])
+def test_surface_dmo_parity(oracle, kind, n, jit):
+    """Surface + feature passes (quadric fits, projection, NN walk) on the GPU
+    against the oracle: positions bit-exact after every iteration."""
+    gen = {"tet": lambda: synth.tet_cube(n, seed=51, jitter=jit), "hex": lambda: synth.hex_cube(n, seed=51, jitter=jit)}[kind]
+    xyz, cells = gen()
+    g = _build(kind, xyz, cells)
+    cls = DMOTetClass if kind == "tet" else DMOHexClass
+    opt = cls(g, n_iter=4, fixed_boundary=False)
+    o = oracle.OracleMesh(kind, xyz, cells)
+    o.surface_init()
+    assert opt.getQuality() == o.min_quality()
+    for it in range(4):
+        opt.doIteration()
+        n2d, n1d = o.iterate_surface()
+        gp = g.positions(original_order=False)
+        op = o.positions()
+        bad = (gp != op).any(axis=1)
+        assert not bad.any(), (it, int(bad.sum()), np.nonzero(bad)[0][:10])
+        assert opt.getQuality() == o.min_quality()
+    st = opt.stats()
+    assert st["searches_surface"] > 0
+    opt.close()

@@ -120,3 +120,38 @@ def test_flat_tri_improves(oracle):
     perm = M.new_of_old()
     b = M.m.nFree
     assert np.array_equal(pos[b:], xyz[np.argsort(perm)][b:])
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_surface_dmo_oracle(oracle, kind):
+    """Surface passes: vertices stay on the cube faces / edges, quality is
+    monotone, and the 1D eligibility matches the cube's edge vertices."""
+    n = 8
+    gen = {"tet": lambda: synth.tet_cube(n, seed=13), "hex": lambda: synth.hex_cube(n, seed=13, jitter=0.15)}[kind]
+    xyz, cells = gen()
+    M = oracle.OracleMesh(kind, xyz, cells)
+    M.surface_init()
+    m = M.m
+    opt1d = M.surface_get(2)
+    if kind == "hex":
+        assert opt1d.sum() == 12 * (n - 1)       # every non-corner edge vertex
+    ls = M.surface_get(1)
+    # frames are orthonormal
+    e = ls[:, 3:12].reshape(-1, 3, 3).astype(np.float64)
+    assert np.allclose(np.einsum("vij,vkj->vik", e, e), np.eye(3), atol=1e-5)
+    q = [M.quality64()[0]]
+    counts = []
+    for _ in range(4):
+        counts.append(M.iterate_surface())
+        q.append(M.quality64()[0])
+    assert all(b >= a - 1e-6 * abs(a) for a, b in zip(q, q[1:])), q
+    assert sum(c[0] for c in counts) > 0          # surface searches ran
+    p = M.positions()[: m.nSurf].astype(np.float64)
+    dev = np.abs(np.abs(p - 0.5).max(axis=1) - 0.5)
+    assert dev.max() < 1e-6                        # still on the cube surface
+    # feature vertices stay on the cube edges: two coordinates at 0 or 1
+    feat = p[m.nFree:]
+    onb = (np.abs(np.abs(feat - 0.5) - 0.5) < 1e-6).sum(axis=1)
+    assert np.all(onb >= 2)
+    nnt = M.surface_get(0)
+    assert np.all((nnt >= 0) & (nnt < m.nSurf))

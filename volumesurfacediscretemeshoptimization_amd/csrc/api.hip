@@ -90,6 +90,13 @@ struct vsdmo_opt {
     DevBuf<float> q, pmin, omin;
     DevBuf<double> psum, osum;
     std::vector<hipEvent_t> ev;   // pairs around timed launches
+    // surface-vertex state (fixed_boundary == 0)
+    bool surface = false;
+    int nSectors = 0;
+    DevBuf<float> pinit, fnorm, ls, ls1, fs, snapx, snapy, snapz;
+    DevBuf<int> nn, frow, table;
+    DevBuf<uint8_t> opt1d;
+    std::vector<uint8_t> opt1d_host;
 };
 
 extern "C" {
@@ -337,6 +344,63 @@ static void opt_quality(vsdmo_opt* o)
     o->last_sum = sm;
 }
 
+static SurfArgs surf_args(vsdmo_opt* o)
+{
+    vsdmo_mesh* m = o->m;
+    SurfArgs a{};
+    a.x = m->x.p; a.y = m->y.p; a.z = m->z.p;
+    a.xw = m->x.p; a.yw = m->y.p; a.zw = m->z.p;
+    a.sx = o->snapx.p; a.sy = o->snapy.p; a.sz = o->snapz.p;
+    a.ringRow = m->ringRow.p; a.ringVal = m->ringVal.p;
+    a.heRow = m->heRow.p; a.heTarget = m->heTarget.p; a.heFace = m->heFace.p;
+    a.faces = m->faces.p;
+    a.isFeature = m->isFeature.p; a.isB1D = m->isB1D.p;
+    a.nS = m->nSurf; a.nFree = m->nFree; a.nFeature = m->nFeature; a.nFaces = m->nFaces; a.fpe = m->fpe;
+    a.flat = m->flat;
+    a.cos160 = (float)std::cos(160.0 * 3.14159265358979323846 / 180.0);
+    a.pinit = o->pinit.p; a.fnorm = o->fnorm.p; a.ls = o->ls.p; a.ls1 = o->ls1.p; a.fs = o->fs.p;
+    a.nn = o->nn.p; a.frow = o->frow.p; a.table = o->table.p; a.opt1d = o->opt1d.p;
+    a.gridScale = o->grid_scale; a.shrink = o->shrink; a.aff = o->aff; a.depth = o->p.depth;
+    return a;
+}
+
+// local surfaces, feature sectors, 1D curves, NN table and 1D eligibility
+// from the current positions (reference DMOHexClass::init, dmo/DMOHex.cu:95-129)
+static void surface_init(vsdmo_opt* o)
+{
+    vsdmo_mesh* m = o->m;
+    hipStream_t s = m->stream;
+    const int nS = m->nSurf, nF = m->nFeature;
+    o->pinit.alloc((size_t)3 * nS); o->fnorm.alloc((size_t)3 * std::max(m->nFaces, 1));
+    o->ls.alloc((size_t)16 * nS); o->ls1.alloc((size_t)10 * nS);
+    o->nn.alloc(nS); o->opt1d.alloc(nS);
+    o->frow.alloc((size_t)nF + 1); o->table.alloc((size_t)20 * std::max(nF, 1));
+    o->snapx.alloc(nS); o->snapy.alloc(nS); o->snapz.alloc(nS);
+    VSDMO_HIP(hipMemsetAsync(o->ls1.p, 0, sizeof(float) * 10 * nS, s));
+    SurfArgs a = surf_args(o);
+    surf_init_launch(a, s);
+    std::vector<int> frow(nF + 1, 0);
+    if (nF > 0) {
+        DevBuf<int> cnt(nF);
+        surf_sector_count_launch(a, cnt.p, s);
+        std::vector<int> hc = cnt.to_host(s);
+        for (int i = 0; i < nF; ++i) frow[i + 1] = frow[i] + hc[i];
+    }
+    o->frow.upload(frow.data(), nF + 1, s);
+    o->nSectors = frow[nF];
+    o->fs.alloc((size_t)16 * std::max(o->nSectors, 1));
+    a = surf_args(o);
+    if (nF > 0) {
+        DevBuf<int> secN((size_t)40 * std::max(o->nSectors, 1)), secCnt(std::max(o->nSectors, 1));
+        VSDMO_HIP(hipMemsetAsync(secCnt.p, 0, sizeof(int) * std::max(o->nSectors, 1), s));
+        surf_sector_launch(a, secN.p, secCnt.p, o->nSectors, s);
+        VSDMO_HIP(hipStreamSynchronize(s));
+    }
+    VSDMO_HIP(hipGetLastError());
+    o->opt1d_host = o->opt1d.to_host(s);
+    o->surface = true;
+}
+
 vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale, int32_t n_iter,
                               const vsdmo_params* params, vsdmo_opt** out)
 {
@@ -352,8 +416,10 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
         if (!(p.nq == 2 || p.nq == 4 || p.nq == 6 || p.nq == 8))
             return fail(VSDMO_INVALID_ARG, "nq must be 2, 4, 6 or 8");
         if (p.depth < 1 || p.depth > 16) return fail(VSDMO_INVALID_ARG, "depth must be in [1,16]");
-        if (!p.fixed_boundary)
-            return fail(VSDMO_NOT_SUPPORTED, "surface/feature vertex optimisation is not available in this build; set fixed_boundary=1");
+        if (!p.fixed_boundary && m->elem == VSDMO_TRI)
+            return fail(VSDMO_NOT_SUPPORTED, "boundary-curve optimisation of flat triangle meshes is not available; set fixed_boundary=1");
+        if (!p.fixed_boundary && (m->maxHE > 40 || m->nSurf == 0))
+            return fail(VSDMO_NOT_SUPPORTED, "surface optimisation needs surface one-rings of at most 40 vertices");
         auto* o = new vsdmo_opt();
         o->m = m;
         o->criterion = criterion;
@@ -370,6 +436,7 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
             o->q.alloc(m->nE);
             o->pmin.alloc(nb); o->psum.alloc(nb);
             o->omin.alloc(1); o->osum.alloc(1);
+            if (!p.fixed_boundary) surface_init(o);
             opt_quality(o);
         } catch (...) {
             delete o;
@@ -434,6 +501,25 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                     timed_launch(o, m->elem, a, lds);
                     o->s_inner += a.cEnd - a.cStart;
                 }
+                if (o->surface && o->last_min > 0.0f) {
+                    SurfArgs sa = surf_args(o);
+                    size_t slds = surf_lds_bytes(m->elem, m->maxRing);
+                    const int tet = (m->elem == VSDMO_TET);
+                    for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {
+                        int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
+                        VSDMO_HIP(hipMemcpyAsync(o->snapx.p, m->x.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                        VSDMO_HIP(hipMemcpyAsync(o->snapy.p, m->y.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                        VSDMO_HIP(hipMemcpyAsync(o->snapz.p, m->z.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                        surf_search2d_launch(sa, lo, hi, o->p.nq, tet, slds, m->stream);
+                        o->s_surf += hi - lo;
+                    }
+                    for (size_t c = 0; c + 1 < m->colOffFeature.size(); ++c) {
+                        int lo = m->colOffFeature[c], hi = m->colOffFeature[c + 1];
+                        surf_search1d_launch(sa, lo, hi, o->p.nq, tet, slds, m->stream);
+                        for (int v = lo; v < hi; ++v) o->s_feat += o->opt1d_host[v];
+                    }
+                }
+                if (o->surface) surf_nn_launch(surf_args(o), m->stream);
             } else {
                 size_t lds = search_lds_bytes(VSDMO_TRI, m->maxHE);
                 for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {

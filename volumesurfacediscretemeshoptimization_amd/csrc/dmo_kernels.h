@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
 
 namespace vsdmo {
 
@@ -31,6 +32,49 @@ struct QualityArgs {
     int nE, npe, elem;
     float* q;
 };
+
+// surface-vertex state and mesh views for the surface kernels
+struct SurfArgs {
+    const float* x;
+    const float* y;
+    const float* z;
+    float* xw;
+    float* yw;
+    float* zw;
+    const float* sx;        // positions snapshot at the start of a surface colour pass
+    const float* sy;
+    const float* sz;
+    const int* ringRow;
+    const int* ringVal;
+    const int* heRow;
+    const int* heTarget;
+    const int* heFace;
+    const int* faces;
+    const uint8_t* isFeature;
+    const uint8_t* isB1D;
+    int nS, nFree, nFeature, nFaces, fpe, flat;
+    float cos160;
+    // owned by the optimiser
+    float* pinit;     // 3*nS
+    float* fnorm;     // 3*nFaces
+    float* ls;        // 16*nS
+    float* ls1;       // 10*nS
+    float* fs;        // 16*nSectors
+    int* nn;          // nS
+    int* frow;        // nFeature+1
+    int* table;       // 20*nFeature
+    uint8_t* opt1d;   // nS
+    float gridScale, shrink, aff;
+    int depth;
+};
+
+size_t surf_lds_bytes(int elem, int maxRing);
+void surf_init_launch(const SurfArgs& a, hipStream_t s);
+void surf_sector_count_launch(const SurfArgs& a, int* cnt, hipStream_t s);
+void surf_sector_launch(const SurfArgs& a, int* secN, int* secCnt, int nSec, hipStream_t s);
+void surf_nn_launch(const SurfArgs& a, hipStream_t s);
+void surf_search2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s);
+void surf_search1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s);
 
 size_t search_lds_bytes(int kind, int maxRing);
 void launch_search(int kind, int nq, const SearchArgs& a, size_t lds, hipStream_t s);

@@ -1,0 +1,649 @@
+// surface.hip -- surface-vertex DMO on the GPU: local quadric fits (own 3x3
+// LU), feature-sector fits, 1D feature-curve fits, nearest-neighbour walk,
+// and the 2D surface / 1D feature-curve searches with projection onto the
+// blended local surfaces.
+//
+// Reference (VSDMO/src/dmo/...): surfaceP/Surface.h:18-169, Surface.cu:11-121,
+// surfaceP/SurfaceFeature.h:79-106, SurfaceFeature.cu:12-223,
+// Surface1D.h:13-114, Surface1D.cu:7-103, SurfaceEstimation.h:122-197,
+// DMOCommon.cu:11-43, DMOCommon.h:130-165, DMOHexImplCub.h:141-489,
+// DMOTetImplCub.h:141-494.  Expressions follow the reference literally
+// (dot = (x*x' + y*y') + z*z', normalize divides by sqrtf(dot)) with
+// contraction off, so a host restatement of the same expressions reproduces
+// every value.
+#include <float.h>
+
+#include "dmo_kernels.h"
+#include "vsdmo_internal.h"
+#include "vsdmo_numerics.h"
+
+namespace vsdmo {
+
+constexpr int kMaxNeigh = 40;
+constexpr int kTableSize = 20;
+constexpr float kSEps = 1.19209e-07f;
+
+struct w3 { float x, y, z; };
+__device__ __forceinline__ w3 W(const float* p) { return { p[0], p[1], p[2] }; }
+__device__ __forceinline__ w3 WP(const float* x, const float* y, const float* z, int v) { return { x[v], y[v], z[v] }; }
+__device__ __forceinline__ w3 wsub(w3 a, w3 b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+__device__ __forceinline__ w3 wadd(w3 a, w3 b) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
+__device__ __forceinline__ w3 wmul(float s, w3 a) { return { s * a.x, s * a.y, s * a.z }; }
+__device__ __forceinline__ float wdot(w3 a, w3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ w3 wcross(w3 a, w3 b)
+{
+    return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x };
+}
+__device__ __forceinline__ float wnorm(w3 a) { return sqrtf((a.x * a.x + a.y * a.y) + a.z * a.z); }
+__device__ __forceinline__ w3 wnormalize(w3 a)
+{
+    float l = wnorm(a);
+    return { a.x / l, a.y / l, a.z / l };
+}
+
+// local surface record: p0 (3), e0 e1 e2 (9), a b c (3), numPoints (1)
+struct LSurf { w3 p0, e[3]; float a, b, c, np; };
+struct LSurf1 { w3 p0, e[2]; float a; };
+static_assert(sizeof(LSurf) == 16 * sizeof(float), "LSurf layout");
+static_assert(sizeof(LSurf1) == 10 * sizeof(float), "LSurf1 layout");
+
+__device__ __forceinline__ float poly(const LSurf& s, float u, float v) { return (s.a * u * u + s.b * u * v) + s.c * v * v; }
+__device__ __forceinline__ w3 l2w(const LSurf& s, float u, float v, float w)
+{
+    return wadd(wadd(wadd(s.p0, wmul(u, s.e[0])), wmul(v, s.e[1])), wmul(w, s.e[2]));
+}
+
+// 3x3 LU with partial pivoting (first maximal |pivot|) and solve; column-major A
+__device__ void lu_solve3(const float* Acm, const float* rhs, float* x)
+{
+    float A[3][3], b[3];
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) A[i][j] = Acm[i + 3 * j];
+        b[i] = rhs[i];
+    }
+    int piv[3] = { 0, 1, 2 };
+    for (int j = 0; j < 3; ++j) {
+        int p = j;
+        float mx = fabsf(A[j][j]);
+        for (int i = j + 1; i < 3; ++i)
+            if (fabsf(A[i][j]) > mx) { mx = fabsf(A[i][j]); p = i; }
+        if (p != j) {
+            for (int k = 0; k < 3; ++k) { float t = A[j][k]; A[j][k] = A[p][k]; A[p][k] = t; }
+            int t = piv[j]; piv[j] = piv[p]; piv[p] = t;
+        }
+        for (int i = j + 1; i < 3; ++i) {
+            A[i][j] = A[i][j] / A[j][j];
+            for (int k = j + 1; k < 3; ++k) A[i][k] = A[i][k] - A[i][j] * A[j][k];
+        }
+    }
+    float y[3];
+    for (int i = 0; i < 3; ++i) y[i] = b[piv[i]];
+    for (int i = 1; i < 3; ++i)
+        for (int k = 0; k < i; ++k) y[i] = y[i] - A[i][k] * y[k];
+    for (int i = 2; i >= 0; --i) {
+        for (int k = i + 1; k < 3; ++k) y[i] = y[i] - A[i][k] * y[k];
+        y[i] = y[i] / A[i][i];
+    }
+    x[0] = y[0]; x[1] = y[1]; x[2] = y[2];
+}
+
+__device__ void frame3(w3 n, w3 r, w3* e0, w3* e1, w3* e2)
+{
+    if (wnorm(wcross(n, r)) < 10.0f * kSEps) r = { n.x, n.z, -n.y };
+    w3 t1 = wcross(n, r);
+    w3 t2 = wcross(n, t1);
+    *e2 = wnormalize(n);
+    *e0 = wnormalize(t1);
+    *e1 = wnormalize(t2);
+}
+
+__device__ void fit_quadric(LSurf& s, const w3* pts, int np)
+{
+    float M[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 }, R[3] = { 0, 0, 0 };
+    for (int i = 0; i < np; ++i) {
+        w3 p = wsub(pts[i], s.p0);
+        float u = wdot(s.e[0], p), v = wdot(s.e[1], p), w = wdot(s.e[2], p);
+        R[0] += u * u * w; R[1] += u * v * w; R[2] += v * v * w;
+        M[0] += u * u * u * u; M[1] += u * u * u * v; M[4] += u * u * v * v;
+        M[2] += u * u * v * v; M[5] += u * v * v * v; M[8] += v * v * v * v;
+    }
+    M[3] = M[1]; M[6] = M[2]; M[7] = M[5];
+    float abc[3];
+    lu_solve3(M, R, abc);
+    s.a = abc[0]; s.b = abc[1]; s.c = abc[2];
+    s.np = (float)np;
+    if (np <= 4) { s.a = 0.0f; s.b = 0.0f; s.c = 0.0f; }
+    if (s.a != s.a) s.a = 0.0f;
+    if (s.b != s.b) s.b = 0.0f;
+    if (s.c != s.c) s.c = 0.0f;
+}
+
+// ------------------------------------------------------------- init kernels
+__global__ void k_surf_face_normals(SurfArgs a)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nFaces) return;
+    const int* F = a.faces + (size_t)f * a.fpe;
+    w3 p0 = WP(a.x, a.y, a.z, F[0]), p1 = WP(a.x, a.y, a.z, F[1]), p2 = WP(a.x, a.y, a.z, F[a.fpe == 3 ? 2 : 3]);
+    w3 n = wnormalize(wcross(wsub(p1, p0), wsub(p2, p0)));
+    a.fnorm[3 * f] = n.x; a.fnorm[3 * f + 1] = n.y; a.fnorm[3 * f + 2] = n.z;
+}
+
+__global__ void k_surf_vertex_init(SurfArgs a)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.nS) return;
+    // vertex normal: normalized sum of incident face normals (CCW order)
+    w3 n = { 0.0f, 0.0f, 0.0f };
+    for (int h = a.heRow[v]; h < a.heRow[v + 1]; ++h)
+        if (a.heFace[h] != -1) n = wadd(n, W(a.fnorm + 3 * a.heFace[h]));
+    n = wnormalize(n);
+    a.pinit[3 * v] = a.x[v]; a.pinit[3 * v + 1] = a.y[v]; a.pinit[3 * v + 2] = a.z[v];
+    a.nn[v] = v;
+    w3 pv = WP(a.x, a.y, a.z, v);
+    const bool feat = a.isFeature[v] != 0;
+    // 1D curve fit of feature vertices (Surface1D.cu:48-91)
+    if (feat) {
+        int nb[kMaxNeigh + 1], np = 0, vf = -1;
+        nb[np++] = v;
+        for (int h = a.heRow[v]; h < a.heRow[v + 1]; ++h) {
+            int d = a.heTarget[h];
+            if (a.isFeature[d] && np < kMaxNeigh) { nb[np++] = d; vf = d; }
+        }
+        if (vf >= 0) {
+            w3 nrm = a.flat ? w3{ 0.0f, 0.0f, 1.0f } : n;
+            w3 r = wsub(WP(a.x, a.y, a.z, vf), pv);
+            if (wnorm(wcross(nrm, r)) < 10.0f * kSEps) r = { nrm.x, nrm.z, -nrm.y };
+            w3 t1 = wcross(nrm, r), t2 = wcross(nrm, t1);
+            nrm = wnormalize(nrm);
+            t2 = wnormalize(t2);
+            LSurf1 q;
+            q.p0 = pv; q.e[0] = t2; q.e[1] = nrm;
+            float ATA = 0.0f, ATb = 0.0f;
+            for (int i = 0; i < np; ++i) {
+                w3 p = wsub(WP(a.x, a.y, a.z, nb[i]), q.p0);
+                float u = wdot(q.e[0], p), vv = wdot(q.e[1], p);
+                ATb += u * u * vv;
+                ATA += u * u * u * u;
+            }
+            q.a = ATb / ATA;
+            reinterpret_cast<LSurf1*>(a.ls1)[v] = q;
+        }
+        // 1D eligibility: two feature neighbours, cos <= cos(160 deg)
+        int nf = 0;
+        w3 pp[2] = { { 0, 0, 0 }, { 0, 0, 0 } };
+        for (int h = a.heRow[v]; h < a.heRow[v + 1]; ++h) {
+            int d = a.heTarget[h];
+            if (a.isFeature[d]) { if (nf < 2) pp[nf] = WP(a.x, a.y, a.z, d); ++nf; }
+        }
+        uint8_t ok = 0;
+        if (nf == 2) {
+            w3 ea = wsub(pp[0], pv), eb = wsub(pp[1], pv);
+            float c = wdot(ea, eb) / sqrtf(wdot(ea, ea) * wdot(eb, eb));
+            if (c <= a.cos160) ok = 1;
+        }
+        a.opt1d[v] = ok;
+    } else {
+        a.opt1d[v] = 0;
+    }
+    // quadric fit over {v} and its one-ring (Surface.cu:11-105)
+    w3 pts[kMaxNeigh + 1];
+    int np = 0;
+    pts[np++] = pv;
+    for (int h = a.heRow[v]; h < a.heRow[v + 1] && np < kMaxNeigh; ++h) pts[np++] = WP(a.x, a.y, a.z, a.heTarget[h]);
+    if (np == 5) { pts[np] = wmul(0.5f, wadd(pts[2], pts[3])); ++np; }
+    LSurf q;
+    q.p0 = pts[0];
+    int first = a.heTarget[a.heRow[v]];
+    frame3(n, wsub(WP(a.x, a.y, a.z, first), pv), &q.e[0], &q.e[1], &q.e[2]);
+    fit_quadric(q, pts, np);
+    reinterpret_cast<LSurf*>(a.ls)[v] = q;
+}
+
+// number of sector surfaces of each feature vertex (k_countLocalSurfacesFeature)
+__global__ void k_sector_count(SurfArgs a, int* cnt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nFeature) return;
+    int v = a.nFree + i, c = 0;
+    for (int h = a.heRow[v] + 1; h < a.heRow[v + 1]; ++h) if (a.isFeature[a.heTarget[h]]) ++c;
+    if (!a.isB1D[v]) ++c;
+    cnt[i] = c;
+}
+
+// sector neighbour lists and the halfedge -> sector table (k_setNeighborsFeature)
+__global__ void k_sector_lists(SurfArgs a, int* secN, int* secCnt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nFeature) return;
+    int v = a.nFree + i;
+    int h0 = a.heRow[v], h1 = a.heRow[v + 1];
+    int sid = 0, nneigh = 0, last = h0, vindex = 1;
+    bool lastWasFeature = true;
+    for (int t = 0; t < kTableSize; ++t) a.table[kTableSize * i + t] = -1;
+    a.table[kTableSize * i] = 0;
+    const int nsec = a.frow[i + 1] - a.frow[i];
+    // non-feature neighbours after the last feature neighbour of an open
+    // boundary vertex belong to no sector: they are dropped (sid == nsec)
+    for (int h = h0 + 1; h < h1; ++h) {
+        int d = a.heTarget[h];
+        int* L = secN + (size_t)kMaxNeigh * (a.frow[i] + (sid < nsec ? sid : 0));
+        const bool own = sid < nsec;
+        if (lastWasFeature) { nneigh = 0; if (own) { L[nneigh++] = v; L[nneigh++] = a.heTarget[last]; } }
+        if (a.isFeature[d]) {
+            if (vindex < kTableSize) a.table[kTableSize * i + vindex] = sid;
+            if (own && nneigh < kMaxNeigh) L[nneigh++] = d;
+            if (own) secCnt[a.frow[i] + sid] = nneigh;
+            ++sid; nneigh = 0; lastWasFeature = true;
+        } else {
+            if (vindex < kTableSize) a.table[kTableSize * i + vindex] = sid;
+            if (own && nneigh < kMaxNeigh) L[nneigh++] = d;
+            lastWasFeature = false;
+        }
+        ++last; ++vindex;
+    }
+    if (!a.isB1D[v] && sid < nsec) {
+        int* L = secN + (size_t)kMaxNeigh * (a.frow[i] + sid);
+        if (lastWasFeature) { nneigh = 0; L[nneigh++] = v; L[nneigh++] = a.heTarget[last]; }
+        if (nneigh < kMaxNeigh) L[nneigh++] = a.heTarget[h0];
+        secCnt[a.frow[i] + sid] = nneigh;
+    }
+}
+
+__global__ void k_sector_fit(SurfArgs a, const int* secN, const int* secCnt, int nSec)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nSec) return;
+    const int* L = secN + (size_t)kMaxNeigh * q;
+    int np = secCnt[q];
+    if (np < 2) return;   // not reached for closed surfaces
+    int v = L[0];
+    w3 pts[kMaxNeigh + 1];
+    for (int i = 0; i < np; ++i) pts[i] = WP(a.x, a.y, a.z, L[i]);
+    if (np == 5) { pts[np] = wmul(0.5f, wadd(pts[2], pts[3])); ++np; }
+    w3 n = { 0.0f, 0.0f, 0.0f };
+    for (int h = a.heRow[v]; h < a.heRow[v + 1]; ++h)
+        if (a.heTarget[h] == L[1] && a.heFace[h] >= 0) n = W(a.fnorm + 3 * a.heFace[h]);
+    LSurf s;
+    s.p0 = pts[0];
+    frame3(n, wsub(WP(a.x, a.y, a.z, L[1]), WP(a.x, a.y, a.z, v)), &s.e[0], &s.e[1], &s.e[2]);
+    fit_quadric(s, pts, np);
+    reinterpret_cast<LSurf*>(a.fs)[q] = s;
+}
+
+// nearest-neighbour walk over the initial surface positions within the
+// vertex's feature class (k_updateNearestNeighbor)
+__global__ void k_update_nn(SurfArgs a)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.nS) return;
+    w3 p = WP(a.x, a.y, a.z, v);
+    int vh = a.nn[v];
+    w3 d0 = wsub(p, W(a.pinit + 3 * vh));
+    float sq = wdot(d0, d0);
+    const uint8_t fv = a.isFeature[v];
+    int vnew;
+    do {
+        vnew = vh;
+        int c = vh;
+        for (int h = a.heRow[c]; h < a.heRow[c + 1]; ++h) {
+            int dst = a.heTarget[h];
+            if ((fv != 0) != (a.isFeature[dst] != 0)) continue;
+            w3 dd = wsub(p, W(a.pinit + 3 * dst));
+            float sd = wdot(dd, dd);
+            if (sd < sq) { sq = sd; vh = dst; }
+        }
+    } while (vnew != vh);
+    a.nn[v] = vnew;
+}
+
+// ------------------------------------------------------------- projection
+// localSurfaceEstimationWithFeature: evaluate the nearest neighbour's quadric
+// at (u,v), then blend with its projections onto each one-ring neighbour's
+// surface (sector surfaces for feature neighbours), weights 1/(d^2 + eps).
+// Two passes over the neighbours (weights, then the weighted sum) so no
+// per-neighbour arrays are kept; the arithmetic is the single-pass one.
+__device__ w3 project2d(const SurfArgs& a, int vid, w3 vidPos, float u, float v)
+{
+    const LSurf* ls = reinterpret_cast<const LSurf*>(a.ls);
+    const LSurf* fs = reinterpret_cast<const LSurf*>(a.fs);
+    const LSurf L = ls[a.nn[vid]];
+    const float w = poly(L, u, v);
+    const w3 pw = l2w(L, u, v, w);
+    const int h0 = a.heRow[vid];
+    int nh = a.heRow[vid + 1] - h0;
+    if (nh > kMaxNeigh) nh = kMaxNeigh;
+    float isum = 0.0f;
+    {
+        w3 d = wsub(W(a.pinit + 3 * a.nn[vid]), pw);
+        isum += 1.0f / (wdot(d, d) + kSEps);
+    }
+    for (int i = 0; i < nh; ++i) {
+        int nn = a.nn[a.heTarget[h0 + i]];
+        w3 d = wsub(W(a.pinit + 3 * nn), pw);
+        isum += 1.0f / (wdot(d, d) + kSEps);
+    }
+    const float ii = 1.0f / isum;
+    w3 sum = { 0.0f, 0.0f, 0.0f };
+    {
+        w3 d = wsub(W(a.pinit + 3 * a.nn[vid]), pw);
+        float inv = 1.0f / (wdot(d, d) + kSEps);
+        sum = wadd(sum, wmul(ii, wmul(inv, pw)));
+    }
+    for (int i = 0; i < nh; ++i) {
+        int nn = a.nn[a.heTarget[h0 + i]];
+        w3 d = wsub(W(a.pinit + 3 * nn), pw);
+        float inv = 1.0f / (wdot(d, d) + kSEps);
+        LSurf LN;
+        if (a.isFeature[nn]) {
+            int off = nn - a.nFree;
+            float md = 3.40282347e+38f;
+            int arg = 0, idx = 0;
+            for (int h = a.heRow[nn]; h < a.heRow[nn + 1]; ++h, ++idx) {
+                w3 t = wsub(WP(a.sx, a.sy, a.sz, a.heTarget[h]), vidPos);
+                float dd = wdot(t, t);
+                if (dd < md) { md = dd; arg = idx; }
+            }
+            int sid = (arg < kTableSize) ? a.table[kTableSize * off + arg] : -1;
+            const int nsec = a.frow[off + 1] - a.frow[off];
+            if (sid < 0 || sid >= nsec) sid = 0;
+            LN = nsec > 0 ? fs[a.frow[off] + sid] : ls[nn];
+        } else {
+            LN = ls[nn];
+        }
+        w3 q = wsub(pw, LN.p0);
+        float lu = wdot(q, LN.e[0]), lv = wdot(q, LN.e[1]);
+        float lw = poly(LN, lu, lv);
+        sum = wadd(sum, wmul(ii, wmul(inv, l2w(LN, lu, lv, lw))));
+    }
+    return sum;
+}
+
+__device__ w3 project1d(const SurfArgs& a, int vid, const int* nb, int n, float u)
+{
+    const LSurf1* ls1 = reinterpret_cast<const LSurf1*>(a.ls1);
+    const LSurf1 L = ls1[a.nn[vid]];
+    const float v = L.a * u * u;
+    const w3 pw = wadd(wadd(L.p0, wmul(u, L.e[0])), wmul(v, L.e[1]));
+    float isum = 0.0f;
+    {
+        w3 d = wsub(W(a.pinit + 3 * a.nn[vid]), pw);
+        isum += 1.0f / (wdot(d, d) + kSEps);
+    }
+    for (int i = 0; i < n; ++i) {
+        int nn = a.nn[nb[i]];
+        w3 d = wsub(W(a.pinit + 3 * nn), pw);
+        isum += 1.0f / (wdot(d, d) + kSEps);
+    }
+    const float ii = 1.0f / isum;
+    w3 sum = { 0.0f, 0.0f, 0.0f };
+    {
+        w3 d = wsub(W(a.pinit + 3 * a.nn[vid]), pw);
+        float inv = 1.0f / (wdot(d, d) + kSEps);
+        sum = wadd(sum, wmul(ii, wmul(inv, pw)));
+    }
+    for (int i = 0; i < n; ++i) {
+        int nn = a.nn[nb[i]];
+        w3 d = wsub(W(a.pinit + 3 * nn), pw);
+        float inv = 1.0f / (wdot(d, d) + kSEps);
+        const LSurf1 LN = ls1[nn];
+        w3 q = wsub(pw, LN.p0);
+        float lu = wdot(q, LN.e[0]);
+        float lv = LN.a * lu * lu;
+        sum = wadd(sum, wmul(ii, wmul(inv, wadd(wadd(LN.p0, wmul(lu, LN.e[0])), wmul(lv, LN.e[1])))));
+    }
+    return sum;
+}
+
+// ------------------------------------------------------------- ring scores
+struct HexRingLds { float cd[4], cF[4]; HexP hp[5]; };
+
+struct RingFrame {
+    float x0x, x0y, x0z, sg, is3, epsD, epsF, minDist, maxDist;
+    int n;
+};
+
+// stage the volume one-ring of vid in LDS (same constants as the inner
+// search) and return frame data; all lanes call it
+__device__ RingFrame stage_ring(const SurfArgs& a, int vid, bool tet, void* lds, HexAcc* hacc)
+{
+    const int lane = threadIdx.x;
+    RingFrame f;
+    const int r0 = a.ringRow[vid];
+    f.n = a.ringRow[vid + 1] - r0;
+    f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
+    const int st = tet ? 3 : 7;
+    float mabs = 0.0f, mn = FLT_MAX, mx = -FLT_MAX;
+    for (int e = lane; e < f.n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * st;
+        for (int t = 0; t < st; ++t) {
+            int v = h[t];
+            w3 d = { a.x[v] - f.x0x, a.y[v] - f.x0y, a.z[v] - f.x0z };
+            mabs = fmaxf(mabs, fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z))));
+            float dist = wnorm(d);
+            mn = fminf(mn, dist);
+            mx = fmaxf(mx, dist);
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        mabs = fmaxf(mabs, __shfl_xor(mabs, m));
+        mn = fminf(mn, __shfl_xor(mn, m));
+        mx = fmaxf(mx, __shfl_xor(mx, m));
+    }
+    f.minDist = mn; f.maxDist = mx;
+    f.sg = pow2_scale(mabs);
+    const float isg = 1.0f / f.sg;
+    f.is3 = ((1.0f * isg) * isg) * isg;
+    const float s2 = f.sg * f.sg, s3 = s2 * f.sg;
+    f.epsD = kHexEps * s3; f.epsF = kHexEps * s2;
+    for (int e = lane; e < f.n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * st;
+        if (tet) {
+            f3 p[3];
+            for (int t = 0; t < 3; ++t) {
+                int v = h[t];
+                p[t] = { (a.x[v] - f.x0x) * f.sg, (a.y[v] - f.x0y) * f.sg, (a.z[v] - f.x0z) * f.sg };
+            }
+            reinterpret_cast<TetC*>(lds)[e] = tet_consts(p[0], p[1], p[2]);
+        } else {
+            f3 r[8];
+            r[0] = { 0.0f, 0.0f, 0.0f };
+            for (int t = 0; t < 7; ++t) {
+                int v = h[t];
+                r[t + 1] = { (a.x[v] - f.x0x) * f.sg, (a.y[v] - f.x0y) * f.sg, (a.z[v] - f.x0z) * f.sg };
+            }
+            HexRingLds L;
+            hex_consts(r, L.cd, L.cF, L.hp);
+            reinterpret_cast<HexRingLds*>(lds)[e] = L;
+        }
+    }
+    __syncthreads();
+    if (!tet) {
+        HexAcc acc = hex_acc0();
+        const HexRingLds* L = reinterpret_cast<const HexRingLds*>(lds);
+        for (int e = 0; e < f.n; ++e)
+            for (int s = 0; s < 4; ++s) hex_acc_add(acc, L[e].cd[s], L[e].cF[s]);
+        *hacc = acc;
+    }
+    return f;
+}
+
+__device__ float ring_score(const RingFrame& f, bool tet, const void* lds, const HexAcc& hacc, w3 p)
+{
+    const float px = (p.x - f.x0x) * f.sg, py = (p.y - f.x0y) * f.sg, pz = (p.z - f.x0z) * f.sg;
+    if (tet) {
+        const TetC* tc = reinterpret_cast<const TetC*>(lds);
+        const float Qyz = 3.0f * fmaf(pz, pz, py * py);
+        const float t3 = 3.0f * px;
+        TetAcc acc = tet_acc0();
+        for (int e = 0; e < f.n; ++e) {
+            const TetC c = tc[e];
+            float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+            float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+            tet_acc_add(acc, fmaf(c.nnx, px, Tyz), fmaf(px, t3 + c.m2sx, Ryz));
+        }
+        return tet_acc_score(acc, f.is3);
+    }
+    const HexRingLds* L = reinterpret_cast<const HexRingLds*>(lds);
+    const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
+    HexAcc acc = hacc;
+    for (int e = 0; e < f.n; ++e)
+        for (int s = 0; s < 5; ++s) {
+            const HexP h = L[e].hp[s];
+            const bool a3 = (s == 1 || s == 4);
+            float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+            float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+            hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
+        }
+    return hex_acc_score(acc, f.epsD, f.epsF);
+}
+
+__device__ __forceinline__ void wave_argmax_s(float& best, int& idx)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        float ob = __shfl_xor(best, m);
+        int oi = __shfl_xor(idx, m);
+        bool take = (ob > best) || (ob == best && oi < idx);
+        best = take ? ob : best;
+        idx = take ? oi : idx;
+    }
+}
+
+// ------------------------------------------------------------ 2D surface search
+// Lane l holds reference thread t = l/2's candidate A (l even) or B (l odd):
+// A = (t/nq, t%nq), B = (t/nq + nq/2, t%nq); the surface kernels keep A on an
+// exact tie, so A ranks first.  Window half-width s_d * (min distance to the
+// ring points for hexes, max for tets) in the nearest neighbour's (u,v).
+__global__ __launch_bounds__(64) void k_search_surf2d(SurfArgs a, int cStart, int nq, int tet)
+{
+    extern __shared__ float s_lds[];
+    const int lane = threadIdx.x;
+    const int vid = cStart + blockIdx.x;
+    HexAcc hacc = hex_acc0();
+    const RingFrame f = stage_ring(a, vid, tet != 0, s_lds, &hacc);
+    if (f.n <= 0) return;
+    const float dist = tet ? f.maxDist : f.minDist;
+    const LSurf L = reinterpret_cast<const LSurf*>(a.ls)[a.nn[vid]];
+    const w3 vidPos = { f.x0x, f.x0y, f.x0z };
+    w3 cur = vidPos;
+    const w3 cvec = wsub(cur, L.p0);
+    float cu = wdot(L.e[0], cvec), cv = wdot(L.e[1], cvec);
+    float qcur = ring_score(f, tet != 0, s_lds, hacc, cur);
+    const float qold = qcur;
+    float qlast = -FLT_MAX;
+    const int t = lane >> 1;
+    const bool isB = (lane & 1) != 0;
+    const bool active = t < nq * nq / 2;
+    const int ci = t / nq + (isB ? nq / 2 : 0), cj = t % nq;
+    float ds = a.gridScale;
+    for (int d = 0; d < a.depth; ++d) {
+        const float uMax = ds * dist, uMin = -(ds * dist);
+        float best = -FLT_MAX;
+        float u = 0.0f, v = 0.0f;
+        if (active) {
+            u = cu + a.aff * ((float)ci * uMin + (float)(nq - 1 - ci) * uMax);
+            v = cv + a.aff * ((float)cj * uMin + (float)(nq - 1 - cj) * uMax);
+            best = ring_score(f, tet != 0, s_lds, hacc, project2d(a, vid, vidPos, u, v));
+        }
+        int wl = lane;
+        wave_argmax_s(best, wl);
+        if (qcur < best) {
+            const float bu = __shfl(u, wl), bv = __shfl(v, wl);
+            cur = project2d(a, vid, vidPos, bu, bv);
+            cu = bu; cv = bv;
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * a.shrink;
+    }
+    if (qold < qlast && lane == 0) { a.xw[vid] = cur.x; a.yw[vid] = cur.y; a.zw[vid] = cur.z; }
+}
+
+// ------------------------------------------------------------ 1D feature search
+__global__ __launch_bounds__(64) void k_search_feat1d(SurfArgs a, int cStart, int nq, int tet)
+{
+    extern __shared__ float s_lds[];
+    const int lane = threadIdx.x;
+    const int vid = cStart + blockIdx.x;
+    if (!a.opt1d[vid]) return;   // uniform over the block
+    HexAcc hacc = hex_acc0();
+    const RingFrame f = stage_ring(a, vid, tet != 0, s_lds, &hacc);
+    if (f.n <= 0) return;
+    int nb[kMaxNeigh], nnb = 0;
+    for (int h = a.heRow[vid]; h < a.heRow[vid + 1]; ++h) {
+        int d = a.heTarget[h];
+        if (!a.isFeature[d]) continue;
+        int c = 0;
+        for (int h2 = a.heRow[d]; h2 < a.heRow[d + 1]; ++h2) if (a.isFeature[a.heTarget[h2]]) ++c;
+        if (c == 2 && nnb < kMaxNeigh) nb[nnb++] = d;
+    }
+    const float dist = tet ? f.maxDist : f.minDist;
+    const LSurf1 L = reinterpret_cast<const LSurf1*>(a.ls1)[a.nn[vid]];
+    w3 cur = { f.x0x, f.x0y, f.x0z };
+    const w3 cvec = wsub(cur, L.p0);
+    float cu = wdot(L.e[0], cvec);
+    float qcur = ring_score(f, tet != 0, s_lds, hacc, cur);
+    const float qold = qcur;
+    float qlast = -FLT_MAX;
+    const bool active = lane < nq;
+    float ds = a.gridScale;
+    for (int d = 0; d < a.depth; ++d) {
+        const float uMax = ds * dist, uMin = -(ds * dist);
+        float best = -FLT_MAX, u = 0.0f;
+        if (active) {
+            u = cu + a.aff * ((float)lane * uMin + (float)(nq - 1 - lane) * uMax);
+            best = ring_score(f, tet != 0, s_lds, hacc, project1d(a, vid, nb, nnb, u));
+        }
+        int wl = lane;
+        wave_argmax_s(best, wl);
+        if (qcur < best) {
+            const float bu = __shfl(u, wl);
+            cur = project1d(a, vid, nb, nnb, bu);
+            cu = bu;
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * a.shrink;
+    }
+    if (qold < qlast && lane == 0) { a.xw[vid] = cur.x; a.yw[vid] = cur.y; a.zw[vid] = cur.z; }
+}
+
+// ------------------------------------------------------------- launchers
+static inline int nb256(long long n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+
+size_t surf_lds_bytes(int elem, int maxRing)
+{
+    size_t per = (elem == VSDMO_TET) ? sizeof(TetC) : sizeof(HexRingLds);
+    return per * (size_t)(maxRing > 0 ? maxRing : 1);
+}
+
+void surf_init_launch(const SurfArgs& a, hipStream_t s)
+{
+    k_surf_face_normals<<<nb256(a.nFaces), 256, 0, s>>>(a);
+    k_surf_vertex_init<<<nb256(a.nS), 256, 0, s>>>(a);
+}
+void surf_sector_count_launch(const SurfArgs& a, int* cnt, hipStream_t s)
+{
+    k_sector_count<<<nb256(a.nFeature), 256, 0, s>>>(a, cnt);
+}
+void surf_sector_launch(const SurfArgs& a, int* secN, int* secCnt, int nSec, hipStream_t s)
+{
+    k_sector_lists<<<nb256(a.nFeature), 256, 0, s>>>(a, secN, secCnt);
+    if (nSec > 0) k_sector_fit<<<nb256(nSec), 256, 0, s>>>(a, secN, secCnt, nSec);
+}
+void surf_nn_launch(const SurfArgs& a, hipStream_t s)
+{
+    k_update_nn<<<nb256(a.nS), 256, 0, s>>>(a);
+}
+void surf_search2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s)
+{
+    if (cEnd > cStart) k_search_surf2d<<<cEnd - cStart, 64, lds, s>>>(a, cStart, nq, tet);
+}
+void surf_search1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s)
+{
+    if (cEnd > cStart) k_search_feat1d<<<cEnd - cStart, 64, lds, s>>>(a, cStart, nq, tet);
+}
+
+}  // namespace vsdmo
