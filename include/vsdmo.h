@@ -64,7 +64,12 @@ typedef struct vsdmo_params {
     int32_t depth;             /* refinement levels (reference DMO_DEPTH = 3) */
     float shrink;              /* per-level window factor; 0 = 2/(nq-1) as the reference */
     int32_t extent_mode;       /* vsdmo_extent */
-    int32_t fixed_boundary;    /* 1: surface and feature vertices are not moved */
+    int32_t fixed_boundary;    /* 1: surface and feature vertices are not moved;
+                                  0: tet/hex surface vertices move on their local
+                                  quadric fits and feature vertices along their
+                                  feature curves, boundary vertices of flat
+                                  triangle meshes along the boundary (needs surface
+                                  one-rings of at most 40 vertices) */
     int32_t quality_guard;     /* 1: fail with VSDMO_QUALITY_DECREASED if min quality drops */
     float guard_tolerance;     /* relative slack of the guard (FP32 frame differences) */
 } vsdmo_params;
@@ -84,7 +89,8 @@ typedef struct vsdmo_stats {
 } vsdmo_stats;
 
 /* defaults: greedy colouring, 20 degrees / nq 8, depth 3, reference shrink,
-   auto extent, boundary moved (reference behaviour), guard on */
+   auto extent, boundary fixed (fixed_boundary = 1; the reference moves the
+   boundary: pass 0 for its behaviour), guard on */
 void vsdmo_mesh_params_default(vsdmo_mesh_params* p);
 void vsdmo_params_default(vsdmo_params* p);
 

@@ -533,6 +533,74 @@ static int search1d(const or_mesh* m, const or_surf* s, const or_dmo_params* p, 
     return 0;
 }
 
+/* 1D boundary-curve search of a flat triangle mesh
+   (k_optimizeHierarchical1DFlat, DMOTriFlatImplCub.h:158-333): the ring is
+   the halfedge fan of the vertex, closed only for a vertex that is not on a
+   surface boundary; window half-width s_d * (max Euclidean distance to the
+   fan); scored by the 2D mean ratio of the fan triangles (p, r_k, r_k+1) in
+   the canonical evaluation with the frame of the start position. */
+static int search1d_flat(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, float* xyz)
+{
+    if (!s->opt1d[vid]) return 0;
+    const int h0 = m->heRow[vid], n = m->heRow[vid + 1] - h0;
+    const int nOne = n + (m->isBoundary1D[vid] ? 0 : 1);
+    if (n <= 0 || nOne < 2 || n > MAX_NEIGH) return 0;
+    const w3 x0 = W(xyz + 3 * (size_t)vid);
+    float mabs = 0.0f, mx = -FLT_MAX;
+    for (int i = 0; i < n; ++i) {
+        w3 d = wsub(W(xyz + 3 * (size_t)m->heTarget[h0 + i]), x0);
+        mabs = fmaxf(mabs, fmaxf(fabsf(d.x), fabsf(d.y)));
+        mx = fmaxf(mx, wnorm(d));
+    }
+    const float sg = or_pow2_scale(mabs);
+    const float isg = 1.0f / sg;
+    const float is2 = (1.0f * isg) * isg;
+    tric cs[MAX_NEIGH];
+    const int ne = nOne - 1;
+    for (int e = 0; e < ne; ++e) {
+        const float* a = xyz + 3 * (size_t)m->heTarget[h0 + e];
+        const float* b = xyz + 3 * (size_t)m->heTarget[h0 + (e + 1 == n ? 0 : e + 1)];
+        cs[e] = tri_consts((a[0] - x0.x) * sg, (a[1] - x0.y) * sg, (b[0] - x0.x) * sg, (b[1] - x0.y) * sg);
+    }
+    int nb[MAX_NEIGH], nnb = 0;
+    for (int h = h0; h < h0 + n; ++h) {
+        int d = m->heTarget[h];
+        if (!m->isFeature[d]) continue;
+        int c = 0;
+        for (int h2 = m->heRow[d]; h2 < m->heRow[d + 1]; ++h2) if (m->isFeature[m->heTarget[h2]]) ++c;
+        if (c == 2 && nnb < MAX_NEIGH) nb[nnb++] = d;
+    }
+    const lsurf1* L = &s->ls1[s->nn[vid]];
+    w3 cur = x0;
+    float cu = wdot(L->e[0], wsub(cur, L->p0));
+    const int nq = p->nq;
+    const float aff = 1.0f / (float)(nq - 1);
+    const float shr = (p->shrink > 0.0f) ? p->shrink : 2.0f / (float)(nq - 1);
+    float qcur = tri_score(ne, cs, 0.0f, 0.0f, is2);
+    const float qold = qcur;
+    float qlast = -FLT_MAX;
+    float ds = p->grid_scale;
+    for (int d = 0; d < p->depth; ++d) {
+        float uMax = ds * mx, uMin = -(ds * mx);
+        float best = -FLT_MAX, bu = 0.0f;
+        for (int i = 0; i < nq; ++i) {
+            float u = cu + aff * ((float)i * uMin + (float)(nq - 1 - i) * uMax);
+            w3 q = project1d(s, vid, nb, nnb, u);
+            float sc = tri_score(ne, cs, (q.x - x0.x) * sg, (q.y - x0.y) * sg, is2);
+            if (sc > best) { best = sc; bu = u; }
+        }
+        if (qcur < best) {
+            cur = project1d(s, vid, nb, nnb, bu);
+            cu = bu;
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * shr;
+    }
+    if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
+    return 0;
+}
+
 /* nearest-neighbour walk over the initial surface positions, within the
    vertex's feature class (k_updateNearestNeighbor, DMOCommon.cu:11-43) */
 static void update_nn(const or_mesh* m, or_surf* s)
@@ -566,7 +634,18 @@ long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* 
 #endif
     *nSurf2d = 0; *nFeat1d = 0;
     cnt += or_dmo_iteration(m, p);
-    if (!p->fixed_boundary && lastMin > 0.0f && s) {
+    if (!p->fixed_boundary && lastMin > 0.0f && s && m->elem == OR_TRI) {
+        /* flat triangle mesh: the free vertices were searched above; the
+           boundary vertices follow their boundary curves */
+        for (int c = 0; c < m->nColorsFeature; ++c) {
+            int lo = m->colOffFeature[c], hi = m->colOffFeature[c + 1];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+            for (int v = lo; v < hi; ++v) search1d_flat(m, s, p, v, m->xyz);
+            for (int v = lo; v < hi; ++v) *nFeat1d += s->opt1d[v];
+        }
+    } else if (!p->fixed_boundary && lastMin > 0.0f && s) {
         /* the feature-sector choice of the projection reads positions of
            vertices up to two hops away, which may belong to the colour class
            being updated: it reads a snapshot taken at the start of the pass */

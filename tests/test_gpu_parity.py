@@ -157,3 +157,71 @@ def test_surface_dmo_parity(oracle, kind, n, jit):
     st = opt.stats()
     assert st["searches_surface"] > 0
     opt.close()
+
+
+def test_flat_boundary_1d_parity(oracle):
+    """Flat triangle mesh with the boundary free: free 2D pass, boundary 1D
+    pass and nearest-neighbour walk bit-exact against the oracle."""
+    xyz, tris = synth.tri_grid(32, seed=61, jitter=0.45)
+    g = _build("tri", xyz, tris)
+    opt = DMOTriFlatClass(g, n_iter=5, fixed_boundary=False)
+    o = oracle.OracleMesh("tri", xyz, tris)
+    o.surface_init()
+    gated = 0
+    for it in range(5):
+        gated += o.last_min > 0
+        opt.doIteration()
+        o.iterate_surface()
+        gp = g.positions(original_order=False)
+        op = o.positions()
+        bad = (gp != op).any(axis=1)
+        assert not bad.any(), (it, int(bad.sum()), np.nonzero(bad)[0][:10])
+        assert opt.getQuality() == o.min_quality()
+    st = opt.stats()
+    assert gated >= 2 and st["searches_feature"] == gated * (4 * 30 - 4)
+    opt.close()
+
+
+def test_c2_full_size(oracle):
+    """48^3 hex cube with the surface free, interior and tangential surface
+    jitter 0.3h: the jitter inverts some hexes, the shape metric is 0 there, so
+    the minimum quality stays 0 and the surface passes stay gated off exactly
+    as in the reference; two iterations bit-exact against the oracle."""
+    xyz, hexes = synth.hex_cube(48, seed=0)
+    g = _build("hex", xyz, hexes)
+    c = g.counts()
+    assert (c["n_vertices"], c["n_elements"], c["n_inner"]) == (117649, 110592, 103823)
+    assert (c["n_free"], c["n_feature"]) == (13254, 572)
+    opt = DMOHexClass(g, n_iter=10, fixed_boundary=False)
+    o = oracle.OracleMesh("hex", xyz, hexes)
+    o.surface_init()
+    assert opt.getQuality() == o.min_quality() == 0.0
+    for it in range(2):
+        opt.doIteration()
+        n2, n1 = o.iterate_surface()
+        assert (n2, n1) == (0, 0)
+    assert np.array_equal(g.positions(original_order=False), o.positions())
+    st = opt.stats()
+    assert st["searches_inner"] == 2 * 103823 and st["searches_surface"] == 0
+    m0 = opt.getMeanQuality()
+    opt.doIteration()
+    assert opt.getMeanQuality() >= m0
+    opt.close()
+
+
+def test_surface_hex_parity_at_size(oracle):
+    """24^3 hex cube with jitter 0.15h (valid mesh, surface passes run every
+    iteration): three iterations bit-exact."""
+    xyz, hexes = synth.hex_cube(24, seed=3, jitter=0.15)
+    g = _build("hex", xyz, hexes)
+    opt = DMOHexClass(g, n_iter=3, fixed_boundary=False)
+    o = oracle.OracleMesh("hex", xyz, hexes)
+    o.surface_init()
+    assert o.min_quality() > 0
+    for it in range(3):
+        opt.doIteration()
+        n2, n1 = o.iterate_surface()
+        assert n2 == g.counts()["n_free"]
+        bad = (g.positions(original_order=False) != o.positions()).any(axis=1)
+        assert not bad.any(), (it, int(bad.sum()))
+    opt.close()

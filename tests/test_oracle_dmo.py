@@ -155,3 +155,41 @@ def test_surface_dmo_oracle(oracle, kind):
     assert np.all(onb >= 2)
     nnt = M.surface_get(0)
     assert np.all((nnt >= 0) & (nnt < m.nSurf))
+
+
+def test_flat_boundary_1d_oracle(oracle):
+    """Flat triangle mesh with the boundary free (reference default): boundary
+    vertices slide along the square's sides, corners stay, and the minimum
+    quality never drops."""
+    n = 20
+    xyz, tris = synth.tri_grid(n, seed=17, jitter=0.45)
+    M = oracle.OracleMesh("tri", xyz, tris)
+    M.surface_init()
+    m = M.m
+    opt1d = M.surface_get(2)
+    assert opt1d[: m.nFree].sum() == 0
+    # boundary vertices except the corners and the four vertices joined by a
+    # cell diagonal across the (n-1, 0) and (0, n-1) corners (three feature
+    # neighbours each)
+    assert opt1d.sum() == 4 * (n - 2) - 4
+    p0 = M.positions().copy()
+    q = [M.quality64()[0]]
+    n1 = gated = 0
+    for _ in range(4):
+        gated += M.last_min > 0          # the boundary pass needs a valid mesh
+        _, b = M.iterate_surface()
+        n1 += b
+        q.append(M.quality64()[0])
+    assert gated >= 2
+    assert n1 == gated * (4 * (n - 2) - 4)
+    assert all(b >= a - 1e-6 * abs(a) for a, b in zip(q, q[1:])), q
+    p = M.positions()
+    bnd = p[m.nFree:m.nSurf].astype(np.float64)
+    b0 = p0[m.nFree:m.nSurf].astype(np.float64)
+    on_side = (np.abs(bnd[:, :2] - np.round(bnd[:, :2])) < 1e-6) & (np.abs(b0[:, :2] - np.round(b0[:, :2])) < 1e-6)
+    assert np.all(on_side.any(axis=1))           # one coordinate still exactly on a side
+    assert np.all(np.abs(bnd[:, 2]) == 0.0)      # stays in the plane
+    moved = np.abs(bnd - b0).max(axis=1) > 0
+    assert moved.sum() > 0
+    corners = (np.abs(b0[:, :2] - np.round(b0[:, :2])) < 1e-9).all(axis=1)
+    assert not moved[corners].any()

@@ -416,8 +416,6 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
         if (!(p.nq == 2 || p.nq == 4 || p.nq == 6 || p.nq == 8))
             return fail(VSDMO_INVALID_ARG, "nq must be 2, 4, 6 or 8");
         if (p.depth < 1 || p.depth > 16) return fail(VSDMO_INVALID_ARG, "depth must be in [1,16]");
-        if (!p.fixed_boundary && m->elem == VSDMO_TRI)
-            return fail(VSDMO_NOT_SUPPORTED, "boundary-curve optimisation of flat triangle meshes is not available; set fixed_boundary=1");
         if (!p.fixed_boundary && (m->maxHE > 40 || m->nSurf == 0))
             return fail(VSDMO_NOT_SUPPORTED, "surface optimisation needs surface one-rings of at most 40 vertices");
         auto* o = new vsdmo_opt();
@@ -527,6 +525,15 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                     timed_launch(o, VSDMO_TRI, a, lds);
                     o->s_surf += a.cEnd - a.cStart;
                 }
+                if (o->surface && o->last_min > 0.0f) {
+                    SurfArgs sa = surf_args(o);
+                    for (size_t c = 0; c + 1 < m->colOffFeature.size(); ++c) {
+                        int lo = m->colOffFeature[c], hi = m->colOffFeature[c + 1];
+                        surf_flat1d_launch(sa, lo, hi, o->p.nq, m->stream);
+                        for (int v = lo; v < hi; ++v) o->s_feat += o->opt1d_host[v];
+                    }
+                }
+                if (o->surface) surf_nn_launch(surf_args(o), m->stream);
             }
             VSDMO_HIP(hipGetLastError());
             float prev = o->last_min;

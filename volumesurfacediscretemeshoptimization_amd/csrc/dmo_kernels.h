@@ -75,6 +75,7 @@ void surf_sector_launch(const SurfArgs& a, int* secN, int* secCnt, int nSec, hip
 void surf_nn_launch(const SurfArgs& a, hipStream_t s);
 void surf_search2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s);
 void surf_search1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s);
+void surf_flat1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s);
 
 size_t search_lds_bytes(int kind, int maxRing);
 void launch_search(int kind, int nq, const SearchArgs& a, size_t lds, hipStream_t s);

@@ -610,6 +610,92 @@ __global__ __launch_bounds__(64) void k_search_feat1d(SurfArgs a, int cStart, in
     if (qold < qlast && lane == 0) { a.xw[vid] = cur.x; a.yw[vid] = cur.y; a.zw[vid] = cur.z; }
 }
 
+// ------------------------------------------------------ 1D flat boundary search
+// Boundary vertices of a flat triangle mesh follow their boundary curve
+// (k_optimizeHierarchical1DFlat): the ring is the halfedge fan (closed only
+// off a surface boundary), the window half-width is s_d times the largest
+// Euclidean distance to the fan, and candidates are scored by the 2D mean
+// ratio of the fan triangles in the frame of the start position.
+__global__ __launch_bounds__(64) void k_search_flat1d(SurfArgs a, int cStart, int nq)
+{
+    __shared__ TriC s_tr[kMaxNeigh];
+    const int lane = threadIdx.x;
+    const int vid = cStart + blockIdx.x;
+    if (!a.opt1d[vid]) return;   // uniform over the block
+    const int h0 = a.heRow[vid], n = a.heRow[vid + 1] - h0;
+    const int nOne = n + (a.isB1D[vid] ? 0 : 1);
+    if (n <= 0 || nOne < 2 || n > kMaxNeigh) return;
+    const w3 x0 = WP(a.x, a.y, a.z, vid);
+    float mabs = 0.0f, mx = -FLT_MAX;
+    if (lane < n) {
+        w3 d = wsub(WP(a.x, a.y, a.z, a.heTarget[h0 + lane]), x0);
+        mabs = fmaxf(fabsf(d.x), fabsf(d.y));
+        mx = wnorm(d);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        mabs = fmaxf(mabs, __shfl_xor(mabs, m));
+        mx = fmaxf(mx, __shfl_xor(mx, m));
+    }
+    const float sg = pow2_scale(mabs);
+    const float isg = 1.0f / sg;
+    const float is2 = (1.0f * isg) * isg;
+    const int ne = nOne - 1;
+    if (lane < ne) {
+        const int va = a.heTarget[h0 + lane], vb = a.heTarget[h0 + (lane + 1 == n ? 0 : lane + 1)];
+        s_tr[lane] = tri_consts((a.x[va] - x0.x) * sg, (a.y[va] - x0.y) * sg, (a.x[vb] - x0.x) * sg, (a.y[vb] - x0.y) * sg);
+    }
+    __syncthreads();
+    auto score = [&](w3 q) {
+        const float px = (q.x - x0.x) * sg, py = (q.y - x0.y) * sg;
+        const float Q2 = 2.0f * (py * py);
+        const float t2 = 2.0f * px;
+        TriAcc acc = tri_acc0();
+        for (int e = 0; e < ne; ++e) {
+            const TriC c = s_tr[e];
+            float Ty = fmaf(c.gy, py, c.C);
+            float Ry = fmaf(c.sy, py, c.K + Q2);
+            tri_acc_add(acc, fmaf(c.gx, px, Ty), fmaf(px, t2 + c.sx, Ry));
+        }
+        return tri_acc_score(acc, is2);
+    };
+    int nb[kMaxNeigh], nnb = 0;
+    for (int h = h0; h < h0 + n; ++h) {
+        int d = a.heTarget[h];
+        if (!a.isFeature[d]) continue;
+        int c = 0;
+        for (int h2 = a.heRow[d]; h2 < a.heRow[d + 1]; ++h2) if (a.isFeature[a.heTarget[h2]]) ++c;
+        if (c == 2 && nnb < kMaxNeigh) nb[nnb++] = d;
+    }
+    const LSurf1 L = reinterpret_cast<const LSurf1*>(a.ls1)[a.nn[vid]];
+    w3 cur = x0;
+    float cu = wdot(L.e[0], wsub(cur, L.p0));
+    float qcur = score(cur);
+    const float qold = qcur;
+    float qlast = -FLT_MAX;
+    const bool active = lane < nq;
+    float ds = a.gridScale;
+    for (int d = 0; d < a.depth; ++d) {
+        const float uMax = ds * mx, uMin = -(ds * mx);
+        float best = -FLT_MAX, u = 0.0f;
+        if (active) {
+            u = cu + a.aff * ((float)lane * uMin + (float)(nq - 1 - lane) * uMax);
+            best = score(project1d(a, vid, nb, nnb, u));
+        }
+        int wl = lane;
+        wave_argmax_s(best, wl);
+        if (qcur < best) {
+            const float bu = __shfl(u, wl);
+            cur = project1d(a, vid, nb, nnb, bu);
+            cu = bu;
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * a.shrink;
+    }
+    if (qold < qlast && lane == 0) { a.xw[vid] = cur.x; a.yw[vid] = cur.y; a.zw[vid] = cur.z; }
+}
+
 // ------------------------------------------------------------- launchers
 static inline int nb256(long long n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
 
@@ -644,6 +730,10 @@ void surf_search2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int t
 void surf_search1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s)
 {
     if (cEnd > cStart) k_search_feat1d<<<cEnd - cStart, 64, lds, s>>>(a, cStart, nq, tet);
+}
+void surf_flat1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s)
+{
+    if (cEnd > cStart) k_search_flat1d<<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
 }
 
 }  // namespace vsdmo
