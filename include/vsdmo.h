@@ -72,6 +72,8 @@ typedef struct vsdmo_params {
                                   one-rings of at most 40 vertices) */
     int32_t quality_guard;     /* 1: fail with VSDMO_QUALITY_DECREASED if min quality drops */
     float guard_tolerance;     /* relative slack of the guard (FP32 frame differences) */
+    int32_t waves_per_vertex;  /* 1, 2 or 4 wavefronts per vertex search (a multiple of
+                                  nq's divisor; results do not depend on it) */
 } vsdmo_params;
 
 typedef struct vsdmo_counts {
@@ -161,6 +163,66 @@ vsdmo_status vsdmo_opt_destroy(vsdmo_opt* opt);
 
 /* thread-local message of the last failing call on this thread */
 const char* vsdmo_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Batched DMO over many independent meshes.  No counterpart in the
+ * reference, which optimises one mesh per DMO*Class object
+ * (dmo/DMOBase.h:23-88); each mesh's searches are the reference's and give
+ * the same positions as one vsdmo_opt per mesh with fixed_boundary = 1.
+ * Colour round r sweeps colour r of every mesh (inner colours of tet / hex
+ * meshes, free colours of flat triangle meshes) in one persistent launch; an
+ * iteration (rounds + per-mesh quality) is captured once as a hipGraph and
+ * replayed.  The meshes must outlive the batch (vsdmo_mesh_destroy refuses
+ * while a batch holds them); the batch works on its own copy of the
+ * positions until vsdmo_batch_writeback.
+ * ------------------------------------------------------------------------ */
+typedef struct vsdmo_batch vsdmo_batch;
+
+typedef struct vsdmo_batch_params {
+    int32_t nq;                /* candidates per axis: 2, 4, 6, 8 */
+    int32_t depth;             /* refinement levels */
+    int32_t waves_per_vertex;  /* 1, 2, 4 (divides nq); does not change results */
+    float grid_scale;          /* first-level window scale (reference 0.5) */
+    int32_t use_graph;         /* 1: replay a captured hipGraph of one iteration */
+    uint32_t shuffle_seed;     /* 0: work items in mesh order; else a seeded permutation inside each round */
+    int32_t quality_guard;     /* 1: fail with VSDMO_QUALITY_DECREASED when a mesh's min quality drops */
+    float guard_tolerance;
+} vsdmo_batch_params;
+
+typedef struct vsdmo_batch_stats {
+    int64_t searches;              /* vertex searches executed */
+    int32_t iterations, rounds, meshes;
+    int64_t items_per_iteration;
+    double gpu_ms;                 /* device time of the iterations (events around them) */
+    int32_t max_grid;              /* blocks of the largest persistent launch */
+    int32_t total_vertices;
+} vsdmo_batch_stats;
+
+void vsdmo_batch_params_default(vsdmo_batch_params* p);
+vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, const vsdmo_batch_params* params,
+                                vsdmo_batch** out);
+vsdmo_status vsdmo_batch_iterate(vsdmo_batch* b, int32_t n);
+/* per-mesh minimum quality after the last iteration (n_meshes floats) */
+vsdmo_status vsdmo_batch_min_quality(vsdmo_batch* b, float* per_mesh);
+vsdmo_status vsdmo_batch_mean_quality(vsdmo_batch* b, double* per_mesh);
+/* positions of one mesh in the optimiser numbering (3*nV floats) */
+vsdmo_status vsdmo_batch_get_positions(vsdmo_batch* b, int32_t mesh, float* xyz);
+/* copy the batch positions into the meshes */
+vsdmo_status vsdmo_batch_writeback(vsdmo_batch* b);
+/* back to the positions at vsdmo_batch_create */
+vsdmo_status vsdmo_batch_restore(vsdmo_batch* b);
+vsdmo_status vsdmo_batch_get_params(vsdmo_batch* b, vsdmo_batch_params* p);
+vsdmo_status vsdmo_batch_set_params(vsdmo_batch* b, const vsdmo_batch_params* p);
+/* Autotuner over (nq, depth, waves_per_vertex): every trial starts from the
+ * current positions and runs `iters` iterations; waves_per_vertex is chosen
+ * by time at (8, 3), then (nq, depth) candidates are accepted when every
+ * mesh's final min quality is >= the (8, 3) run's - tol; the fastest
+ * accepted configuration is set and returned, the positions are left as
+ * they were, and each trial is written to log_path as one JSON line. */
+vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, const char* log_path,
+                                  vsdmo_batch_params* chosen);
+vsdmo_status vsdmo_batch_get_stats(vsdmo_batch* b, vsdmo_batch_stats* out);
+vsdmo_status vsdmo_batch_destroy(vsdmo_batch* b);
 
 #ifdef __cplusplus
 }

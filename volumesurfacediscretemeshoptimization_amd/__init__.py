@@ -27,7 +27,7 @@ from ._lib import VsdmoError, check, load  # noqa: F401
 MEAN_RATIO = 0
 
 __all__ = ["DMOMeshTriFactory", "DMOMeshTetFactory", "DMOMeshHexFactory", "DMOMesh",
-           "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "VsdmoError", "MEAN_RATIO", "load"]
+           "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "DMOBatch", "VsdmoError", "MEAN_RATIO", "load"]
 
 
 def _fptr(a):
@@ -180,7 +180,7 @@ class _DMOBase:
     def __init__(self, mesh: DMOMesh, qualityCriterium: int = MEAN_RATIO, gridScale: float = 0.5,
                  n_iter: int = 30, nq: int = 8, depth: int = 3, shrink: float = 0.0,
                  extent_mode: int = _lib.VSDMO_EXTENT_AUTO, fixed_boundary: bool = True,
-                 quality_guard: bool = True, guard_tolerance: float = 1e-5):
+                 quality_guard: bool = True, guard_tolerance: float = 1e-5, waves_per_vertex: int = 1):
         lib = load()
         self._lib = lib
         self.mesh = mesh
@@ -188,6 +188,7 @@ class _DMOBase:
         lib.vsdmo_params_default(C.byref(p))
         p.nq, p.depth, p.shrink, p.extent_mode = nq, depth, shrink, extent_mode
         p.fixed_boundary, p.quality_guard, p.guard_tolerance = int(fixed_boundary), int(quality_guard), guard_tolerance
+        p.waves_per_vertex = int(waves_per_vertex)
         h = C.c_void_p()
         check(lib.vsdmo_opt_create(mesh.handle, qualityCriterium, gridScale, n_iter, C.byref(p), C.byref(h)),
               "vsdmo_opt_create")
@@ -262,3 +263,87 @@ class DMOHexClass(_DMOBase):
 
 class DMOTriFlatClass(_DMOBase):
     pass
+
+
+class DMOBatch:
+    """Batched DMO over many meshes (no counterpart in the reference): colour
+    round r sweeps colour r of every mesh in one persistent launch, one
+    iteration is replayed as a hipGraph.  Each mesh gets the positions one
+    DMO*Class per mesh (boundary fixed) would give."""
+
+    def __init__(self, meshes, nq: int = 8, depth: int = 3, waves_per_vertex: int = 1, gridScale: float = 0.5,
+                 use_graph: bool = True, shuffle_seed: int = 0, quality_guard: bool = True,
+                 guard_tolerance: float = 1e-5):
+        lib = load()
+        self._lib = lib
+        self.meshes = list(meshes)
+        p = _lib.BatchParams()
+        lib.vsdmo_batch_params_default(C.byref(p))
+        p.nq, p.depth, p.waves_per_vertex, p.grid_scale = nq, depth, waves_per_vertex, gridScale
+        p.use_graph, p.shuffle_seed = int(use_graph), int(shuffle_seed)
+        p.quality_guard, p.guard_tolerance = int(quality_guard), guard_tolerance
+        arr = (C.c_void_p * len(self.meshes))(*[m.handle.value for m in self.meshes])
+        h = C.c_void_p()
+        check(lib.vsdmo_batch_create(arr, len(self.meshes), C.byref(p), C.byref(h)), "vsdmo_batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            check(self._lib.vsdmo_batch_destroy(self._h), "vsdmo_batch_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def iterate(self, n: int = 1):
+        check(self._lib.vsdmo_batch_iterate(self._h, int(n)), "vsdmo_batch_iterate")
+
+    def min_quality(self) -> np.ndarray:
+        out = np.empty(len(self.meshes), dtype=np.float32)
+        check(self._lib.vsdmo_batch_min_quality(self._h, _fptr(out)), "vsdmo_batch_min_quality")
+        return out
+
+    def mean_quality(self) -> np.ndarray:
+        out = np.empty(len(self.meshes), dtype=np.float64)
+        check(self._lib.vsdmo_batch_mean_quality(self._h, out.ctypes.data_as(C.POINTER(C.c_double))),
+              "vsdmo_batch_mean_quality")
+        return out
+
+    def positions(self, i: int) -> np.ndarray:
+        """Positions of mesh i in the optimiser numbering."""
+        n = self.meshes[i].nVertices
+        out = np.empty((n, 3), dtype=np.float32)
+        check(self._lib.vsdmo_batch_get_positions(self._h, int(i), _fptr(out)), "vsdmo_batch_get_positions")
+        return out
+
+    def writeback(self):
+        check(self._lib.vsdmo_batch_writeback(self._h), "vsdmo_batch_writeback")
+
+    def restore(self):
+        check(self._lib.vsdmo_batch_restore(self._h), "vsdmo_batch_restore")
+
+    def params(self) -> dict:
+        p = _lib.BatchParams()
+        check(self._lib.vsdmo_batch_get_params(self._h, C.byref(p)), "vsdmo_batch_get_params")
+        return {name: getattr(p, name) for name, _ in _lib.BatchParams._fields_}
+
+    def set_params(self, **kw):
+        p = _lib.BatchParams()
+        check(self._lib.vsdmo_batch_get_params(self._h, C.byref(p)), "vsdmo_batch_get_params")
+        for k, v in kw.items():
+            setattr(p, k, v)
+        check(self._lib.vsdmo_batch_set_params(self._h, C.byref(p)), "vsdmo_batch_set_params")
+
+    def autotune(self, iters: int = 5, tol: float = 1e-3, log_path: str = "") -> dict:
+        p = _lib.BatchParams()
+        check(self._lib.vsdmo_batch_autotune(self._h, int(iters), float(tol), log_path.encode(), C.byref(p)),
+              "vsdmo_batch_autotune")
+        return {name: getattr(p, name) for name, _ in _lib.BatchParams._fields_}
+
+    def stats(self) -> dict:
+        s = _lib.BatchStats()
+        check(self._lib.vsdmo_batch_get_stats(self._h, C.byref(s)), "vsdmo_batch_get_stats")
+        return {name: getattr(s, name) for name, _ in _lib.BatchStats._fields_}

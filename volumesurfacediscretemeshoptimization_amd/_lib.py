@@ -30,7 +30,8 @@ class MeshParams(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("nq", C.c_int32), ("depth", C.c_int32), ("shrink", C.c_float), ("extent_mode", C.c_int32),
-                ("fixed_boundary", C.c_int32), ("quality_guard", C.c_int32), ("guard_tolerance", C.c_float)]
+                ("fixed_boundary", C.c_int32), ("quality_guard", C.c_int32), ("guard_tolerance", C.c_float),
+                ("waves_per_vertex", C.c_int32)]
 
 
 class Counts(C.Structure):
@@ -44,6 +45,18 @@ class Stats(C.Structure):
     _fields_ = [("searches_inner", C.c_int64), ("searches_surface", C.c_int64), ("searches_feature", C.c_int64),
                 ("iterations", C.c_int32), ("min_quality", C.c_float), ("kernel_ms", C.c_double),
                 ("kernel_launches", C.c_int64)]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("nq", C.c_int32), ("depth", C.c_int32), ("waves_per_vertex", C.c_int32), ("grid_scale", C.c_float),
+                ("use_graph", C.c_int32), ("shuffle_seed", C.c_uint32), ("quality_guard", C.c_int32),
+                ("guard_tolerance", C.c_float)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("searches", C.c_int64), ("iterations", C.c_int32), ("rounds", C.c_int32), ("meshes", C.c_int32),
+                ("items_per_iteration", C.c_int64), ("gpu_ms", C.c_double), ("max_grid", C.c_int32),
+                ("total_vertices", C.c_int32)]
 
 
 EXPORTS = {
@@ -76,6 +89,20 @@ EXPORTS = {
     "vsdmo_opt_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsdmo_opt_destroy": (C.c_int, [C.c_void_p]),
     "vsdmo_last_error": (C.c_char_p, []),
+    "vsdmo_batch_params_default": (None, [C.POINTER(BatchParams)]),
+    "vsdmo_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(BatchParams),
+                                     C.POINTER(C.c_void_p)]),
+    "vsdmo_batch_iterate": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vsdmo_batch_min_quality": (C.c_int, [C.c_void_p, _fp]),
+    "vsdmo_batch_mean_quality": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "vsdmo_batch_get_positions": (C.c_int, [C.c_void_p, C.c_int32, _fp]),
+    "vsdmo_batch_writeback": (C.c_int, [C.c_void_p]),
+    "vsdmo_batch_restore": (C.c_int, [C.c_void_p]),
+    "vsdmo_batch_get_params": (C.c_int, [C.c_void_p, C.POINTER(BatchParams)]),
+    "vsdmo_batch_set_params": (C.c_int, [C.c_void_p, C.POINTER(BatchParams)]),
+    "vsdmo_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_char_p, C.POINTER(BatchParams)]),
+    "vsdmo_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "vsdmo_batch_destroy": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None

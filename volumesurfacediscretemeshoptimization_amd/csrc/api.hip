@@ -118,6 +118,7 @@ void vsdmo_params_default(vsdmo_params* p)
     p->fixed_boundary = 1;
     p->quality_guard = 1;
     p->guard_tolerance = 1e-5f;
+    p->waves_per_vertex = 1;
 }
 
 const char* vsdmo_last_error(void) { return g_err.c_str(); }
@@ -416,6 +417,8 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
         if (!(p.nq == 2 || p.nq == 4 || p.nq == 6 || p.nq == 8))
             return fail(VSDMO_INVALID_ARG, "nq must be 2, 4, 6 or 8");
         if (p.depth < 1 || p.depth > 16) return fail(VSDMO_INVALID_ARG, "depth must be in [1,16]");
+        if (!waves_supported(p.nq, p.waves_per_vertex))
+            return fail(VSDMO_INVALID_ARG, "waves_per_vertex must be 1, 2 or 4 and divide nq");
         if (!p.fixed_boundary && (m->maxHE > 40 || m->nSurf == 0))
             return fail(VSDMO_NOT_SUPPORTED, "surface optimisation needs surface one-rings of at most 40 vertices");
         auto* o = new vsdmo_opt();
@@ -454,12 +457,12 @@ static void timed_launch(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds
         VSDMO_HIP(hipEventCreate(&e0));
         VSDMO_HIP(hipEventCreate(&e1));
         VSDMO_HIP(hipEventRecord(e0, s));
-        launch_search(kind, o->p.nq, a, lds, s);
+        launch_search(kind, o->p.nq, o->p.waves_per_vertex, a, lds, s);
         VSDMO_HIP(hipEventRecord(e1, s));
         o->ev.push_back(e0);
         o->ev.push_back(e1);
     } else {
-        launch_search(kind, o->p.nq, a, lds, s);
+        launch_search(kind, o->p.nq, o->p.waves_per_vertex, a, lds, s);
     }
     o->launches++;
 }

@@ -77,8 +77,34 @@ void surf_search2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int t
 void surf_search1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s);
 void surf_flat1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s);
 
+// chunked per-mesh qualities of a batch
+struct BatchQArgs {
+    const float* x;
+    const float* y;
+    const float* z;
+    const int* cells;          // global vertex ids, npe per element
+    const int* chunkElem0;     // nChunks+1: first element of each chunk
+    const long long* chunkCell0;  // nChunks: first cell index (ints) of each chunk
+    const int* chunkType;      // nChunks: vsdmo_elem
+    const int* meshChunk0;     // nMeshes+1
+    float* cmin;
+    double* csum;
+    float* mmin;
+    double* msum;
+    float* hist;               // hcap rows of nMeshes minima
+    int* hctr;
+    int nChunks, nMeshes, hcap;
+};
+
 size_t search_lds_bytes(int kind, int maxRing);
-void launch_search(int kind, int nq, const SearchArgs& a, size_t lds, hipStream_t s);
+bool waves_supported(int nq, int w);
+void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hipStream_t s);
+void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
+                        int lo, int hi, int* ctr, int grid, size_t lds, hipStream_t s);
+int batch_round_blocks_per_cu(int nq, int w, size_t lds);
+void launch_fill_int(int* p, int n, int v, hipStream_t s);
+void launch_batch_quality(const BatchQArgs& a, hipStream_t s);
+size_t batch_lds_bytes(int maxRingTet, int maxRingHex, int maxHETri);
 void launch_quality(const QualityArgs& a, float* pmin, double* psum, float* omin, double* osum, hipStream_t s);
 void launch_histogram(const float* q, int n, int ncols, int* bins, hipStream_t s);
 

@@ -44,21 +44,82 @@ __device__ __forceinline__ void wave_argmax(float& best, int& idx)
     }
 }
 
-// ------------------------------------------------------------------ tets
-template <int NQ>
-__global__ __launch_bounds__(64) void k_search_tet(SearchArgs a)
+// Block-level reductions for W waves per vertex (blockDim = 64*W).  With
+// W = 1 they are the wave shuffles alone; otherwise the per-wave results are
+// combined through LDS.  max/min and the (score, index) argmax are order
+// independent, so every W gives the same result.
+template <int W>
+__device__ __forceinline__ float block_max(float v, float* s)
 {
-    extern __shared__ TetC s_tc[];
-    const int lane = threadIdx.x;
-    const int vid = a.cStart + blockIdx.x;
+    v = wave_max(v);
+    if constexpr (W > 1) {
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) s[w] = v;
+        __syncthreads();
+        v = s[0];
+#pragma unroll
+        for (int k = 1; k < W; ++k) v = fmaxf(v, s[k]);
+        __syncthreads();
+    }
+    return v;
+}
+template <int W>
+__device__ __forceinline__ float block_min(float v, float* s)
+{
+    v = wave_min(v);
+    if constexpr (W > 1) {
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) s[w] = v;
+        __syncthreads();
+        v = s[0];
+#pragma unroll
+        for (int k = 1; k < W; ++k) v = fminf(v, s[k]);
+        __syncthreads();
+    }
+    return v;
+}
+template <int W>
+__device__ __forceinline__ void block_argmax(float& best, int& idx, float* sf, int* si)
+{
+    wave_argmax(best, idx);
+    if constexpr (W > 1) {
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { sf[w] = best; si[w] = idx; }
+        __syncthreads();
+        best = sf[0];
+        idx = si[0];
+#pragma unroll
+        for (int k = 1; k < W; ++k) {
+            const float ob = sf[k];
+            const int oi = si[k];
+            if (ob > best || (ob == best && oi < idx)) { best = ob; idx = oi; }
+        }
+        __syncthreads();
+    }
+}
+
+// LDS scratch of the block reductions
+template <int W>
+struct RedLds { float f[W]; int i[W]; };
+
+// ------------------------------------------------------------------ tets
+// One vertex per block of W waves.  Lane l = j + nq*k of wave w owns the
+// candidates (i, j, k) with i in [w*nq/W, (w+1)*nq/W); candidate index
+// i + nq*l, so the lowest index wins ties as in the reference.
+template <int NQ, int W>
+__device__ void tet_vertex(const SearchArgs& a, int vid, bool emin, TetC* s_tc, RedLds<W>& rd)
+{
+    constexpr int T = 64 * W;
+    constexpr int CPL = NQ / W;
+    static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
+    const int tid = threadIdx.x;
     const int r0 = a.ringRow[vid];
     const int n = a.ringRow[vid + 1] - r0;
     const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
-    const bool emin = a.extentMin != 0;
 
     // window half-extent per axis and the frame scale
     float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    for (int e = lane; e < n; e += 64) {
+    for (int e = tid; e < n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 3;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
@@ -70,14 +131,14 @@ __global__ __launch_bounds__(64) void k_search_tet(SearchArgs a)
             mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
         }
     }
-    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
-    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
-    mabs = wave_max(mabs);
+    if (emin) { ex = block_min<W>(ex, rd.f); ey = block_min<W>(ey, rd.f); ez = block_min<W>(ez, rd.f); }
+    else { ex = block_max<W>(ex, rd.f); ey = block_max<W>(ey, rd.f); ez = block_max<W>(ez, rd.f); }
+    mabs = block_max<W>(mabs, rd.f);
     const float sg = pow2_scale(mabs);
     const float isg = 1.0f / sg;
     const float is3 = ((1.0f * isg) * isg) * isg;
 
-    for (int e = lane; e < n; e += 64) {
+    for (int e = tid; e < n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 3;
         f3 p[3];
 #pragma unroll
@@ -105,6 +166,7 @@ __global__ __launch_bounds__(64) void k_search_tet(SearchArgs a)
         qcur = tet_acc_score(acc, is3);
     }
 
+    const int lane = tid & 63, i0 = (tid >> 6) * CPL;
     const int j = lane % NQ, k = lane / NQ;
     const bool active = lane < NQ * NQ;
     float cx = x0x, cy = x0y, cz = x0z;
@@ -116,34 +178,34 @@ __global__ __launch_bounds__(64) void k_search_tet(SearchArgs a)
         const float gzM = cz + ds * ez, gzm = cz - ds * ez;
         const float py = (grid_coord(a.aff, j, NQ, gym, gyM) - x0y) * sg;
         const float pz = (grid_coord(a.aff, k, NQ, gzm, gzM) - x0z) * sg;
-        float px[NQ], t3[NQ];
+        float px[CPL], t3[CPL];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            px[i] = (grid_coord(a.aff, i, NQ, gxm, gxM) - x0x) * sg;
+        for (int i = 0; i < CPL; ++i) {
+            px[i] = (grid_coord(a.aff, i0 + i, NQ, gxm, gxM) - x0x) * sg;
             t3[i] = 3.0f * px[i];
         }
         const float Qyz = 3.0f * fmaf(pz, pz, py * py);
-        TetAcc acc[NQ];
+        TetAcc acc[CPL];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) acc[i] = tet_acc0();
+        for (int i = 0; i < CPL; ++i) acc[i] = tet_acc0();
         for (int e = 0; e < n; ++e) {
             const TetC c = s_tc[e];
             const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
             const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
 #pragma unroll
-            for (int i = 0; i < NQ; ++i)
+            for (int i = 0; i < CPL; ++i)
                 tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
         }
         float best = -FLT_MAX;
         int bi = 0;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
+        for (int i = 0; i < CPL; ++i) {
             float q = tet_acc_score(acc[i], is3);
             if (q > best) { best = q; bi = i; }
         }
         if (!active) best = -FLT_MAX;
-        int cand = lane * NQ + bi;
-        wave_argmax(best, cand);
+        int cand = lane * NQ + i0 + bi;
+        block_argmax<W>(best, cand, rd.f, rd.i);
         if (qcur < best) {
             const int wi = cand % NQ, wj = (cand / NQ) % NQ, wk = cand / (NQ * NQ);
             cx = grid_coord(a.aff, wi, NQ, gxm, gxM);
@@ -154,25 +216,25 @@ __global__ __launch_bounds__(64) void k_search_tet(SearchArgs a)
         }
         ds = ds * a.shrink;
     }
-    if (moved && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    if (moved && tid == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
 }
 
 // ------------------------------------------------------------------ hexes
 struct HexLds { float cd[4], cF[4]; HexP hp[5]; };
 
-template <int NQ>
-__global__ __launch_bounds__(64) void k_search_hex(SearchArgs a)
+template <int NQ, int W>
+__device__ void hex_vertex(const SearchArgs& a, int vid, bool emin, HexLds* s_hx, RedLds<W>& rd)
 {
-    extern __shared__ HexLds s_hx[];
-    const int lane = threadIdx.x;
-    const int vid = a.cStart + blockIdx.x;
+    constexpr int T = 64 * W;
+    constexpr int CPL = NQ / W;
+    static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
+    const int tid = threadIdx.x;
     const int r0 = a.ringRow[vid];
     const int n = a.ringRow[vid + 1] - r0;
     const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
-    const bool emin = a.extentMin != 0;
 
     float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    for (int e = lane; e < n; e += 64) {
+    for (int e = tid; e < n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 7;
         for (int t = 0; t < 7; ++t) {
             int v = h[t];
@@ -183,14 +245,14 @@ __global__ __launch_bounds__(64) void k_search_hex(SearchArgs a)
             mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
         }
     }
-    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
-    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
-    mabs = wave_max(mabs);
+    if (emin) { ex = block_min<W>(ex, rd.f); ey = block_min<W>(ey, rd.f); ez = block_min<W>(ez, rd.f); }
+    else { ex = block_max<W>(ex, rd.f); ey = block_max<W>(ey, rd.f); ez = block_max<W>(ez, rd.f); }
+    mabs = block_max<W>(mabs, rd.f);
     const float sg = pow2_scale(mabs);
     const float s2 = sg * sg, s3 = s2 * sg;
     const float epsD = kHexEps * s3, epsF = kHexEps * s2;
 
-    for (int e = lane; e < n; e += 64) {
+    for (int e = tid; e < n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 7;
         f3 r[8];
         r[0] = { 0.0f, 0.0f, 0.0f };
@@ -227,6 +289,7 @@ __global__ __launch_bounds__(64) void k_search_hex(SearchArgs a)
         qcur = hex_acc_score(acc, epsD, epsF);
     }
 
+    const int lane = tid & 63, i0 = (tid >> 6) * CPL;
     const int j = lane % NQ, k = lane / NQ;
     const bool active = lane < NQ * NQ;
     float cx = x0x, cy = x0y, cz = x0z;
@@ -238,16 +301,16 @@ __global__ __launch_bounds__(64) void k_search_hex(SearchArgs a)
         const float gzM = cz + ds * ez, gzm = cz - ds * ez;
         const float py = (grid_coord(a.aff, j, NQ, gym, gyM) - x0y) * sg;
         const float pz = (grid_coord(a.aff, k, NQ, gzm, gzM) - x0z) * sg;
-        float px[NQ], t3[NQ];
+        float px[CPL], t3[CPL];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            px[i] = (grid_coord(a.aff, i, NQ, gxm, gxM) - x0x) * sg;
+        for (int i = 0; i < CPL; ++i) {
+            px[i] = (grid_coord(a.aff, i0 + i, NQ, gxm, gxM) - x0x) * sg;
             t3[i] = 3.0f * px[i];
         }
         const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
-        HexAcc acc[NQ];
+        HexAcc acc[CPL];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) acc[i] = acc0;
+        for (int i = 0; i < CPL; ++i) acc[i] = acc0;
         for (int e = 0; e < n; ++e) {
 #pragma unroll
             for (int sIdx = 0; sIdx < 5; ++sIdx) {
@@ -256,20 +319,20 @@ __global__ __launch_bounds__(64) void k_search_hex(SearchArgs a)
                 const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
                 const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
 #pragma unroll
-                for (int i = 0; i < NQ; ++i)
+                for (int i = 0; i < CPL; ++i)
                     hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
             }
         }
         float best = -FLT_MAX;
         int bi = 0;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
+        for (int i = 0; i < CPL; ++i) {
             float q = hex_acc_score(acc[i], epsD, epsF);
             if (q > best) { best = q; bi = i; }
         }
         if (!active) best = -FLT_MAX;
-        int cand = lane * NQ + bi;
-        wave_argmax(best, cand);
+        int cand = lane * NQ + i0 + bi;
+        block_argmax<W>(best, cand, rd.f, rd.i);
         if (qcur < best) {
             const int wi = cand % NQ, wj = (cand / NQ) % NQ, wk = cand / (NQ * NQ);
             cx = grid_coord(a.aff, wi, NQ, gxm, gxM);
@@ -280,37 +343,38 @@ __global__ __launch_bounds__(64) void k_search_hex(SearchArgs a)
         }
         ds = ds * a.shrink;
     }
-    if (moved && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    if (moved && tid == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
 }
 
 // ------------------------------------------------------------- flat tris
-// Lane l holds the reference thread t = l/2's candidate B (l even) or A
-// (l odd): A = (t/nq, t%nq), B = (t/nq + nq/2, t%nq).  The lowest lane among
-// the maxima is the reference's winner (lowest thread; B on an exact tie).
-template <int NQ>
-__global__ __launch_bounds__(64) void k_search_flat(SearchArgs a)
+// Lane l of wave 0 holds the reference thread t = l/2's candidate B (l even)
+// or A (l odd): A = (t/nq, t%nq), B = (t/nq + nq/2, t%nq).  The lowest lane
+// among the maxima is the reference's winner (lowest thread; B on an exact
+// tie).  Further waves (W > 1) only help stage the ring.
+template <int NQ, int W>
+__device__ void flat_vertex(const SearchArgs& a, int vid, TriC* s_tr, RedLds<W>& rd)
 {
-    extern __shared__ TriC s_tr[];
-    const int lane = threadIdx.x;
-    const int vid = a.cStart + blockIdx.x;
+    constexpr int T = 64 * W;
+    const int tid = threadIdx.x;
     const int h0 = a.heRow[vid];
     const int n = a.heRow[vid + 1] - h0;
-    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
+    const float x0x = a.x[vid], x0y = a.y[vid];
 
     float ex = -FLT_MAX, ey = -FLT_MAX, mabs = 0.0f;
-    for (int e = lane; e < n; e += 64) {
+    for (int e = tid; e < n; e += T) {
         int v = a.heTarget[h0 + e];
         float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y);
         ex = fmaxf(ex, dx);
         ey = fmaxf(ey, dy);
         mabs = fmaxf(mabs, fmaxf(dx, dy));
     }
-    ex = wave_max(ex); ey = wave_max(ey);
-    mabs = wave_max(mabs);
+    ex = block_max<W>(ex, rd.f);
+    ey = block_max<W>(ey, rd.f);
+    mabs = block_max<W>(mabs, rd.f);
     const float sg = pow2_scale(mabs);
     const float isg = 1.0f / sg;
     const float is2 = (1.0f * isg) * isg;
-    for (int e = lane; e < n; e += 64) {
+    for (int e = tid; e < n; e += T) {
         int va = a.heTarget[h0 + e];
         int vb = a.heTarget[h0 + (e + 1 == n ? 0 : e + 1)];
         s_tr[e] = tri_consts((a.x[va] - x0x) * sg, (a.y[va] - x0y) * sg, (a.x[vb] - x0x) * sg, (a.y[vb] - x0y) * sg);
@@ -332,10 +396,10 @@ __global__ __launch_bounds__(64) void k_search_flat(SearchArgs a)
 
     float qcur = score(0.0f, 0.0f);
     const float qold = qcur;
-    const int t = lane >> 1;
-    const bool isA = (lane & 1) != 0;
+    const int t = tid >> 1;
+    const bool isA = (tid & 1) != 0;
     const int ci = t / NQ + (isA ? 0 : NQ / 2), cj = t % NQ;
-    const bool active = t < NQ * NQ / 2;
+    const bool active = tid < 64 && t < NQ * NQ / 2;
     float cx = x0x, cy = x0y;
     float ds = a.gridScale;
     float qlast = -FLT_MAX;
@@ -345,8 +409,8 @@ __global__ __launch_bounds__(64) void k_search_flat(SearchArgs a)
         float best = active ? score((grid_coord(a.aff, ci, NQ, gxm, gxM) - x0x) * sg,
                                     (grid_coord(a.aff, cj, NQ, gym, gyM) - x0y) * sg)
                             : -FLT_MAX;
-        int wl = lane;
-        wave_argmax(best, wl);
+        int wl = tid;
+        block_argmax<W>(best, wl, rd.f, rd.i);
         if (qcur < best) {
             const int wt = wl >> 1;
             const int wi = wt / NQ + ((wl & 1) ? 0 : NQ / 2), wj = wt % NQ;
@@ -357,32 +421,91 @@ __global__ __launch_bounds__(64) void k_search_flat(SearchArgs a)
         qlast = best;
         ds = ds * a.shrink;
     }
-    (void)x0z;
-    if (qold < qlast && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; }
+    if (qold < qlast && tid == 0) { a.xw[vid] = cx; a.yw[vid] = cy; }
+}
+
+// one vertex per block, vertices [cStart, cEnd) of one colour class
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_search_tet(SearchArgs a)
+{
+    extern __shared__ TetC s_tc[];
+    __shared__ RedLds<W> rd;
+    tet_vertex<NQ, W>(a, a.cStart + blockIdx.x, a.extentMin != 0, s_tc, rd);
+}
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_search_hex(SearchArgs a)
+{
+    extern __shared__ HexLds s_hx[];
+    __shared__ RedLds<W> rd;
+    hex_vertex<NQ, W>(a, a.cStart + blockIdx.x, a.extentMin != 0, s_hx, rd);
+}
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_search_flat(SearchArgs a)
+{
+    extern __shared__ TriC s_tr[];
+    __shared__ RedLds<W> rd;
+    flat_vertex<NQ, W>(a, a.cStart + blockIdx.x, s_tr, rd);
+}
+
+// ------------------------------------------------------- batched sweep
+// Persistent kernel of one colour round of a batch of meshes: blocks take
+// work items from a queue (one atomic counter per round) until it is empty.
+// An item is a global vertex id with its element type in bits 30-31
+// (0 tet, 1 hex, 2 flat tri).  Vertices of one round belong to one colour
+// class of their mesh, so the order in which they are taken does not change
+// any result.
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_batch_round(SearchArgs a, const int* __restrict__ rowHex,
+                                                       const int* __restrict__ valHex, const int* __restrict__ items,
+                                                       int lo, int hi, int* __restrict__ ctr)
+{
+    extern __shared__ float4 s_dyn[];
+    __shared__ RedLds<W> rd;
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();   // previous item's LDS reads are done
+        if (threadIdx.x == 0) s_item = atomicAdd(ctr, 1);
+        __syncthreads();
+        const int it = lo + s_item;
+        if (it >= hi) break;
+        const int code = items[it];
+        const int vid = code & 0x3fffffff, type = (code >> 30) & 3;
+        if (type == 0) tet_vertex<NQ, W>(a, vid, false, reinterpret_cast<TetC*>(s_dyn), rd);
+        else if (type == 1) {
+            SearchArgs ah = a;
+            ah.ringRow = rowHex;
+            ah.ringVal = valHex;
+            hex_vertex<NQ, W>(ah, vid, true, reinterpret_cast<HexLds*>(s_dyn), rd);
+        }
+        else flat_vertex<NQ, W>(a, vid, reinterpret_cast<TriC*>(s_dyn), rd);
+    }
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int n, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ------------------------------------------------------------ qualities
 // element quality = ring score of the element seen from its vertex 0 at p'=0
-__global__ void k_elem_quality(QualityArgs a)
+__device__ float elem_quality(const float* __restrict__ X, const float* __restrict__ Y, const float* __restrict__ Z,
+                              const int* c, int elem)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.nE) return;
-    const int* c = a.cells + (size_t)e * a.npe;
-    const float x0x = a.x[c[0]], x0y = a.y[c[0]], x0z = a.z[c[0]];
-    float q;
-    if (a.elem == VSDMO_TET) {
+    const float x0x = X[c[0]], x0y = Y[c[0]], x0z = Z[c[0]];
+    if (elem == VSDMO_TET) {
         float m = 0.0f;
         for (int i = 1; i < 4; ++i) {
-            m = fmaxf(m, fabsf(a.x[c[i]] - x0x));
-            m = fmaxf(m, fabsf(a.y[c[i]] - x0y));
-            m = fmaxf(m, fabsf(a.z[c[i]] - x0z));
+            m = fmaxf(m, fabsf(X[c[i]] - x0x));
+            m = fmaxf(m, fabsf(Y[c[i]] - x0y));
+            m = fmaxf(m, fabsf(Z[c[i]] - x0z));
         }
         const float sg = pow2_scale(m);
         const float isg = 1.0f / sg;
         f3 p[3];
         for (int i = 0; i < 3; ++i) {
             int v = c[i + 1];
-            p[i] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+            p[i] = { (X[v] - x0x) * sg, (Y[v] - x0y) * sg, (Z[v] - x0z) * sg };
         }
         TetC tc = tet_consts(p[0], p[1], p[2]);
         const float px = 0.0f, py = 0.0f, pz = 0.0f;
@@ -391,13 +514,14 @@ __global__ void k_elem_quality(QualityArgs a)
         float Tyz = fmaf(tc.nny, py, fmaf(tc.nnz, pz, tc.D));
         float Ryz = fmaf(tc.m2sy, py, fmaf(tc.m2sz, pz, tc.K + Qyz));
         tet_acc_add(acc, fmaf(tc.nnx, px, Tyz), fmaf(px, t3 + tc.m2sx, Ryz));
-        q = tet_acc_score(acc, ((1.0f * isg) * isg) * isg);
-    } else if (a.elem == VSDMO_HEX) {
+        return tet_acc_score(acc, ((1.0f * isg) * isg) * isg);
+    }
+    if (elem == VSDMO_HEX) {
         float m = 0.0f;
         for (int i = 1; i < 8; ++i) {
-            m = fmaxf(m, fabsf(a.x[c[i]] - x0x));
-            m = fmaxf(m, fabsf(a.y[c[i]] - x0y));
-            m = fmaxf(m, fabsf(a.z[c[i]] - x0z));
+            m = fmaxf(m, fabsf(X[c[i]] - x0x));
+            m = fmaxf(m, fabsf(Y[c[i]] - x0y));
+            m = fmaxf(m, fabsf(Z[c[i]] - x0z));
         }
         const float sg = pow2_scale(m);
         const float s2 = sg * sg, s3 = s2 * sg;
@@ -405,7 +529,7 @@ __global__ void k_elem_quality(QualityArgs a)
         r[0] = { 0.0f, 0.0f, 0.0f };
         for (int i = 1; i < 8; ++i) {
             int v = c[i];
-            r[i] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+            r[i] = { (X[v] - x0x) * sg, (Y[v] - x0y) * sg, (Z[v] - x0z) * sg };
         }
         float cd[4], cF[4];
         HexP hp[5];
@@ -420,25 +544,76 @@ __global__ void k_elem_quality(QualityArgs a)
             float Ryz = fmaf(hp[sIdx].sy, py, fmaf(hp[sIdx].sz, pz, hp[sIdx].K + (a3 ? Q3 : Q1)));
             hex_acc_add(acc, fmaf(hp[sIdx].gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + hp[sIdx].sx, Ryz));
         }
-        q = hex_acc_score(acc, kHexEps * s3, kHexEps * s2);
-    } else {
-        float m = 0.0f;
-        for (int i = 1; i < 3; ++i) {
-            m = fmaxf(m, fabsf(a.x[c[i]] - x0x));
-            m = fmaxf(m, fabsf(a.y[c[i]] - x0y));
-        }
-        const float sg = pow2_scale(m);
-        const float isg = 1.0f / sg;
-        TriC tc = tri_consts((a.x[c[1]] - x0x) * sg, (a.y[c[1]] - x0y) * sg, (a.x[c[2]] - x0x) * sg, (a.y[c[2]] - x0y) * sg);
-        const float px = 0.0f, py = 0.0f;
-        const float Q2 = 2.0f * (py * py), t2 = 2.0f * px;
-        TriAcc acc = tri_acc0();
-        float Ty = fmaf(tc.gy, py, tc.C);
-        float Ry = fmaf(tc.sy, py, tc.K + Q2);
-        tri_acc_add(acc, fmaf(tc.gx, px, Ty), fmaf(px, t2 + tc.sx, Ry));
-        q = tri_acc_score(acc, (1.0f * isg) * isg);
+        return hex_acc_score(acc, kHexEps * s3, kHexEps * s2);
     }
-    a.q[e] = q;
+    float m = 0.0f;
+    for (int i = 1; i < 3; ++i) {
+        m = fmaxf(m, fabsf(X[c[i]] - x0x));
+        m = fmaxf(m, fabsf(Y[c[i]] - x0y));
+    }
+    const float sg = pow2_scale(m);
+    const float isg = 1.0f / sg;
+    TriC tc = tri_consts((X[c[1]] - x0x) * sg, (Y[c[1]] - x0y) * sg, (X[c[2]] - x0x) * sg, (Y[c[2]] - x0y) * sg);
+    const float px = 0.0f, py = 0.0f;
+    const float Q2 = 2.0f * (py * py), t2 = 2.0f * px;
+    TriAcc acc = tri_acc0();
+    float Ty = fmaf(tc.gy, py, tc.C);
+    float Ry = fmaf(tc.sy, py, tc.K + Q2);
+    tri_acc_add(acc, fmaf(tc.gx, px, Ty), fmaf(px, t2 + tc.sx, Ry));
+    return tri_acc_score(acc, (1.0f * isg) * isg);
+}
+
+__global__ void k_elem_quality(QualityArgs a)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nE) return;
+    a.q[e] = elem_quality(a.x, a.y, a.z, a.cells + (size_t)e * a.npe, a.elem);
+}
+
+// batch qualities: one block of 256 per chunk of at most 256 elements of one
+// mesh; per-chunk min and FP64 sum in a fixed tree order
+__global__ void k_batch_quality(BatchQArgs a)
+{
+    __shared__ float smin[256];
+    __shared__ double ssum[256];
+    const int c = blockIdx.x;
+    const int e0 = a.chunkElem0[c], ne = a.chunkElem0[c + 1] - e0;
+    const int elem = a.chunkType[c];
+    const int npe = elem == VSDMO_TET ? 4 : elem == VSDMO_HEX ? 8 : 3;
+    float q = FLT_MAX;
+    double s = 0.0;
+    if ((int)threadIdx.x < ne) {
+        q = elem_quality(a.x, a.y, a.z, a.cells + a.chunkCell0[c] + (size_t)threadIdx.x * npe, elem);
+        s = (double)q;
+    }
+    smin[threadIdx.x] = q;
+    ssum[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + k]);
+            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { a.cmin[c] = smin[0]; a.csum[c] = ssum[0]; }
+}
+
+// per-mesh min / sum over the mesh's chunks in chunk order (one block); the
+// minima are also appended to the history row *hctr, then *hctr advances
+__global__ void k_batch_mesh_reduce(BatchQArgs a)
+{
+    for (int b = threadIdx.x; b < a.nMeshes; b += blockDim.x) {
+        float m = FLT_MAX;
+        double s = 0.0;
+        for (int c = a.meshChunk0[b]; c < a.meshChunk0[b + 1]; ++c) { m = fminf(m, a.cmin[c]); s = s + a.csum[c]; }
+        a.mmin[b] = m;
+        a.msum[b] = s;
+        const int row = *a.hctr;
+        if (row < a.hcap) a.hist[(size_t)row * a.nMeshes + b] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *a.hctr = *a.hctr + 1;
 }
 
 // per-block min and FP64 sum in a fixed tree order
@@ -495,12 +670,100 @@ __global__ void k_histogram(const float* __restrict__ q, int n, int ncols, int* 
 }
 
 // ------------------------------------------------------------- launchers
-template <int NQ>
-static void launch_nq(int kind, const SearchArgs& a, int nBlocks, size_t lds, hipStream_t s)
+template <int NQ, int W>
+static void launch_nqw(int kind, const SearchArgs& a, int nBlocks, size_t lds, hipStream_t s)
 {
-    if (kind == VSDMO_TET) k_search_tet<NQ><<<nBlocks, 64, lds, s>>>(a);
-    else if (kind == VSDMO_HEX) k_search_hex<NQ><<<nBlocks, 64, lds, s>>>(a);
-    else k_search_flat<NQ><<<nBlocks, 64, lds, s>>>(a);
+    if (kind == VSDMO_TET) k_search_tet<NQ, W><<<nBlocks, 64 * W, lds, s>>>(a);
+    else if (kind == VSDMO_HEX) k_search_hex<NQ, W><<<nBlocks, 64 * W, lds, s>>>(a);
+    else k_search_flat<NQ, W><<<nBlocks, 64 * W, lds, s>>>(a);
+}
+template <int NQ>
+static void launch_nq(int kind, int w, const SearchArgs& a, int nBlocks, size_t lds, hipStream_t s)
+{
+    if (w == 2 && NQ % 2 == 0) launch_nqw<NQ, (NQ % 2 == 0 ? 2 : 1)>(kind, a, nBlocks, lds, s);
+    else if (w == 4 && NQ % 4 == 0) launch_nqw<NQ, (NQ % 4 == 0 ? 4 : 1)>(kind, a, nBlocks, lds, s);
+    else launch_nqw<NQ, 1>(kind, a, nBlocks, lds, s);
+}
+struct BatchLaunch {
+    SearchArgs a;
+    const int* rowHex;
+    const int* valHex;
+    const int* items;
+    int lo, hi;
+    int* ctr;
+    int grid;
+    size_t lds;
+    hipStream_t s;
+};
+template <int NQ, int W>
+static void batch_nqw(const BatchLaunch& L)
+{
+    k_batch_round<NQ, W><<<L.grid, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo, L.hi, L.ctr);
+}
+template <int NQ, int W>
+static int batch_occ(size_t lds)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_round<NQ, W>, 64 * W, lds) != hipSuccess) nb = 1;
+    return nb;
+}
+template <int NQ>
+static void batch_nq(int w, const BatchLaunch& L)
+{
+    if (w == 2 && NQ % 2 == 0) batch_nqw<NQ, (NQ % 2 == 0 ? 2 : 1)>(L);
+    else if (w == 4 && NQ % 4 == 0) batch_nqw<NQ, (NQ % 4 == 0 ? 4 : 1)>(L);
+    else batch_nqw<NQ, 1>(L);
+}
+template <int NQ>
+static int batch_occ_nq(int w, size_t lds)
+{
+    if (w == 2 && NQ % 2 == 0) return batch_occ<NQ, (NQ % 2 == 0 ? 2 : 1)>(lds);
+    if (w == 4 && NQ % 4 == 0) return batch_occ<NQ, (NQ % 4 == 0 ? 4 : 1)>(lds);
+    return batch_occ<NQ, 1>(lds);
+}
+
+bool waves_supported(int nq, int w) { return w == 1 || (w == 2 && nq % 2 == 0) || (w == 4 && nq % 4 == 0); }
+
+int batch_round_blocks_per_cu(int nq, int w, size_t lds)
+{
+    switch (nq) {
+    case 2: return batch_occ_nq<2>(w, lds);
+    case 4: return batch_occ_nq<4>(w, lds);
+    case 6: return batch_occ_nq<6>(w, lds);
+    default: return batch_occ_nq<8>(w, lds);
+    }
+}
+
+void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
+                        int lo, int hi, int* ctr, int grid, size_t lds, hipStream_t s)
+{
+    if (hi <= lo || grid <= 0) return;
+    BatchLaunch L{ a, rowHex, valHex, items, lo, hi, ctr, grid, lds, s };
+    switch (nq) {
+    case 2: batch_nq<2>(w, L); break;
+    case 4: batch_nq<4>(w, L); break;
+    case 6: batch_nq<6>(w, L); break;
+    case 8: batch_nq<8>(w, L); break;
+    default: break;
+    }
+}
+
+void launch_fill_int(int* p, int n, int v, hipStream_t s)
+{
+    if (n > 0) k_fill_int<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+}
+
+void launch_batch_quality(const BatchQArgs& a, hipStream_t s)
+{
+    if (a.nChunks > 0) k_batch_quality<<<a.nChunks, 256, 0, s>>>(a);
+    k_batch_mesh_reduce<<<1, 256, 0, s>>>(a);
+}
+
+size_t batch_lds_bytes(int maxRingTet, int maxRingHex, int maxHETri)
+{
+    size_t b = std::max(sizeof(TetC) * (size_t)maxRingTet, sizeof(HexLds) * (size_t)maxRingHex);
+    b = std::max(b, sizeof(TriC) * (size_t)maxHETri);
+    return std::max(b, (size_t)16);
 }
 
 size_t search_lds_bytes(int kind, int maxRing)
@@ -509,15 +772,15 @@ size_t search_lds_bytes(int kind, int maxRing)
     return per * (size_t)std::max(maxRing, 1);
 }
 
-void launch_search(int kind, int nq, const SearchArgs& a, size_t lds, hipStream_t s)
+void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hipStream_t s)
 {
     int nBlocks = a.cEnd - a.cStart;
     if (nBlocks <= 0) return;
     switch (nq) {
-    case 2: launch_nq<2>(kind, a, nBlocks, lds, s); break;
-    case 4: launch_nq<4>(kind, a, nBlocks, lds, s); break;
-    case 6: launch_nq<6>(kind, a, nBlocks, lds, s); break;
-    case 8: launch_nq<8>(kind, a, nBlocks, lds, s); break;
+    case 2: launch_nq<2>(kind, w, a, nBlocks, lds, s); break;
+    case 4: launch_nq<4>(kind, w, a, nBlocks, lds, s); break;
+    case 6: launch_nq<6>(kind, w, a, nBlocks, lds, s); break;
+    case 8: launch_nq<8>(kind, w, a, nBlocks, lds, s); break;
     default: break;
     }
 }
