@@ -88,6 +88,8 @@ typedef struct vsdmo_stats {
     float min_quality;
     double kernel_ms;          /* summed device time of the search kernels (timing on) */
     int64_t kernel_launches;
+    int64_t searches_learned;  /* inner searches that started from the learned proposal */
+    int64_t fallbacks;         /* of those, the ones that fell back to the full search */
 } vsdmo_stats;
 
 /* defaults: greedy colouring, 20 degrees / nq 8, depth 3, reference shrink,
@@ -163,6 +165,46 @@ vsdmo_status vsdmo_opt_destroy(vsdmo_opt* opt);
 
 /* thread-local message of the last failing call on this thread */
 const char* vsdmo_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Learned proposal.  No counterpart in the reference.  A 96-128-128-3 GELU
+ * MLP (BF16 weights and activations on MFMA, FP32 accumulation) maps a
+ * vertex's one-ring -- the distinct one-ring vertices in order of first
+ * appearance in its ring elements, relative to the vertex, divided by their
+ * mean distance L, zero padded to 32 x 3 -- to the displacement / L that
+ * the full DMO search gives it.  The optimiser then searches a 4x4x4 grid on
+ * a window `window_frac` times the plain first-level window around the
+ * proposal for `depth` levels and falls back to the plain full search when
+ * no candidate beats the vertex's current quality.
+ * ------------------------------------------------------------------------ */
+typedef struct vsdmo_mlp vsdmo_mlp;
+typedef struct vsdmo_dataset vsdmo_dataset;
+
+int32_t vsdmo_mlp_num_params(void);   /* 29,315 FP32 parameters: W1[128x96] b1 W2[128x128] b2 W3[3x128] b3 */
+vsdmo_status vsdmo_mlp_create(uint64_t seed, vsdmo_mlp** out);
+vsdmo_status vsdmo_mlp_destroy(vsdmo_mlp* m);
+vsdmo_status vsdmo_mlp_get_params(vsdmo_mlp* m, float* params);
+vsdmo_status vsdmo_mlp_set_params(vsdmo_mlp* m, const float* params);   /* also resets Adam */
+/* host rows: X n x 96 (rounded to BF16 on upload), Y n x 3 */
+vsdmo_status vsdmo_mlp_forward(vsdmo_mlp* m, const float* X, int32_t n, float* Y);
+/* mean squared error over n rows and its gradient w.r.t. the packed parameters */
+vsdmo_status vsdmo_mlp_gradients(vsdmo_mlp* m, const float* X, const float* T, int32_t n, double* loss, float* grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8), seeded shuffles per epoch, the ragged
+ * tail of each epoch dropped; epoch_loss[epochs] = mean batch loss */
+vsdmo_status vsdmo_mlp_train(vsdmo_mlp* m, vsdmo_dataset* ds, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                             double* epoch_loss);
+
+vsdmo_status vsdmo_dataset_create(int64_t capacity, vsdmo_dataset** out);
+vsdmo_status vsdmo_dataset_size(vsdmo_dataset* d, int64_t* n);
+vsdmo_status vsdmo_dataset_get(vsdmo_dataset* d, int64_t start, int32_t n, float* X, float* T);
+vsdmo_status vsdmo_dataset_destroy(vsdmo_dataset* d);
+
+/* run n plain iterations recording one (one-ring features, displacement / L)
+ * pair per inner vertex search until the dataset is full */
+vsdmo_status vsdmo_opt_record(vsdmo_opt* o, vsdmo_dataset* ds, int32_t n);
+/* mode 0: plain search; 1: network proposal; 2: the proposal is the vertex
+ * itself (the same windowed search and fallback without the network) */
+vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac);
 
 /* ------------------------------------------------------------------------
  * Batched DMO over many independent meshes.  No counterpart in the

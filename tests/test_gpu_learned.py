@@ -1,0 +1,196 @@
+"""GPU tests of the learned proposal: the BF16 MFMA MLP against the FP64
+restatement (forward and gradients), training (loss decreases, seeds
+reproduce), the recorded training pairs against the feature encoding, the
+fallback rule, and what the trained network gains over the same windowed
+search without it.
+
+Tolerances: forward max |dY| <= 2e-3 + 5e-3 |Y| (BF16 activations rounded on
+both sides; an FP32-vs-FP64 accumulation difference can flip one rounding);
+gradients per parameter block ||g - g_ref|| <= 3e-2 ||g_ref|| (the backward
+pass rounds dY, dZ2, dZ1 to BF16 for the MFMA products).
+"""
+import numpy as np
+import pytest
+
+import mlp_ref
+from volumesurfacediscretemeshoptimization_amd import (DMODataset, DMOHexClass, DMOMeshHexFactory, DMOMeshTetFactory,
+                                                       DMOMLP, DMOTetClass)
+from volumesurfacediscretemeshoptimization_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _xy(n, seed):
+    rng = np.random.default_rng(seed)
+    X = (rng.normal(size=(n, 96)) * 0.6).astype(np.float32)
+    T = (rng.normal(size=(n, 3)) * 0.1).astype(np.float32)
+    return X, T
+
+
+def test_mlp_forward_matches_reference():
+    mlp = DMOMLP(seed=3)
+    p = mlp.params().astype(np.float64)
+    X, _ = _xy(333, 1)
+    Y = mlp.forward(X).astype(np.float64)
+    Yr, _ = mlp_ref.forward(p, X, emulate=True)
+    err = np.abs(Y - Yr)
+    assert np.all(err <= 2e-3 + 5e-3 * np.abs(Yr)), err.max()
+    # without emulation the BF16 network is still close to the FP64 one
+    Y64, _ = mlp_ref.forward(p, X, emulate=False)
+    assert np.abs(Y - Y64).max() < 0.05
+    mlp.close()
+
+
+def test_mlp_gradients_match_fp64():
+    mlp = DMOMLP(seed=4)
+    p = mlp.params().astype(np.float64)
+    X, T = _xy(257, 2)
+    loss, g = mlp.gradients(X, T)
+    lr_, gr = mlp_ref.loss_and_grad(p, X, T, emulate=True)
+    assert abs(loss - lr_) <= 1e-3 * abs(lr_), (loss, lr_)
+    for name, sl in mlp_ref.block_slices().items():
+        a, b = g[sl].astype(np.float64), gr[sl]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert rel < 3e-2, (name, rel)
+    mlp.close()
+
+
+def _record(kind, sizes_seeds, iters, cap):
+    ds = DMODataset(cap)
+    for n, seed in sizes_seeds:
+        if kind == "tet":
+            xyz, cells = synth.tet_cube(n, seed=seed)
+            m = DMOMeshTetFactory.create(xyz, cells)
+            o = DMOTetClass(m, n_iter=iters)
+        else:
+            xyz, cells = synth.hex_cube(n, seed=seed, jitter=0.2)
+            m = DMOMeshHexFactory.create(xyz, cells)
+            o = DMOHexClass(m, n_iter=iters)
+        o.record(ds, iters)
+        o.close()
+        m.close()
+    return ds
+
+
+def test_dataset_pairs_bit_exact(oracle):
+    xyz, tets = synth.tet_cube(8, seed=5)
+    m = DMOMeshTetFactory.create(xyz, tets)
+    o = DMOTetClass(m, n_iter=1)
+    p0 = m.positions(original_order=False)
+    ds = DMODataset(100000)
+    o.record(ds, 1)
+    p1 = m.positions(original_order=False)
+    c = m.counts()
+    assert len(ds) == c["n_inner"]
+    # recording does not change the trajectory
+    m2 = DMOMeshTetFactory.create(xyz, tets)
+    o2 = DMOTetClass(m2, n_iter=1)
+    o2.iterate(1)
+    assert np.array_equal(m2.positions(original_order=False), p1)
+    # first colour class: features from the start positions, targets from the moves
+    off = m.color_offsets("inner")
+    lo, hi = int(off[0]), int(off[1])
+    X, T = ds.get(0, hi - lo)
+    om = oracle.OracleMesh("tet", xyz, tets)
+    row, val = om.ring()
+    assert np.array_equal(om.positions(), p0)
+    for i, v in enumerate(range(lo, hi)):
+        nb = mlp_ref.canonical_neighbours(row, val, 3, v)
+        f, L = mlp_ref.features(p0, nb, v)
+        assert np.array_equal(X[i], f), v
+        t = ((p1[v] - p0[v]) / L).astype(np.float32)
+        assert np.array_equal(T[i], t), v
+
+
+def test_training_reduces_loss_and_is_deterministic():
+    ds = _record("tet", [(10, 100), (10, 101)], 4, 1 << 20)
+    n = len(ds)
+    assert n == 2 * 4 * 9 ** 3
+    a, b, c = DMOMLP(seed=7), DMOMLP(seed=7), DMOMLP(seed=8)
+    la = a.train(ds, epochs=6, batch=256, lr=1e-3, seed=11)
+    lb = b.train(ds, epochs=6, batch=256, lr=1e-3, seed=11)
+    c.train(ds, epochs=1, batch=256, lr=1e-3, seed=11)
+    assert la[-1] < 0.7 * la[0], la
+    assert np.array_equal(la, lb)
+    assert np.array_equal(a.params(), b.params())
+    assert not np.array_equal(a.params(), c.params())
+
+
+def test_fallback_rule_bad_proposal():
+    """A network whose proposal lies far outside every one-ring: no window
+    candidate beats q(v), so every vertex falls back to the plain search and
+    the positions equal the plain run bit for bit."""
+    xyz, tets = synth.tet_cube(10, seed=12, jitter=0.2)
+    mlp = DMOMLP(seed=1)
+    p = np.zeros(mlp.n_params, dtype=np.float32)
+    p[-3:] = 50.0                     # b3: displacement 50 L in every axis
+    mlp.set_params(p)
+    m1 = DMOMeshTetFactory.create(xyz, tets)
+    m2 = DMOMeshTetFactory.create(xyz, tets)
+    o1 = DMOTetClass(m1, n_iter=2)
+    o2 = DMOTetClass(m2, n_iter=2)
+    assert o1.getQuality() > 0
+    o2.set_proposal(mlp, mode=1, depth=2, window_frac=0.25)
+    o1.iterate(2)
+    o2.iterate(2)
+    assert np.array_equal(m1.positions(original_order=False), m2.positions(original_order=False))
+    st = o2.stats()
+    assert st["searches_learned"] == 2 * m1.counts()["n_inner"]
+    assert st["fallbacks"] == st["searches_learned"]
+
+
+def test_identity_proposal_monotone():
+    xyz, tets = synth.tet_cube(12, seed=13)
+    m = DMOMeshTetFactory.create(xyz, tets)
+    o = DMOTetClass(m, n_iter=5)
+    o.set_proposal(None, mode=2, depth=2, window_frac=0.25)
+    q = [o.getQuality()]
+    for _ in range(5):
+        o.doIteration()
+        q.append(o.getQuality())
+    assert all(b >= a for a, b in zip(q, q[1:])), q
+    st = o.stats()
+    assert 0 < st["fallbacks"] < st["searches_learned"]
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_learned_reaches_plain_quality_and_ablation(kind):
+    """Trained on seeded jittered meshes, the proposal reaches the plain
+    run's final minimum quality within 1e-3 and falls back less often than
+    the same windowed search around the vertex itself (mode 2)."""
+    n = 14 if kind == "tet" else 12
+    train = [(n, 200 + s) for s in range(8)]
+    ds = _record(kind, train, 8, 1 << 21)
+    mlp = DMOMLP(seed=5)
+    loss = mlp.train(ds, epochs=8, batch=1024, lr=1e-3, seed=6)
+    assert loss[-1] < loss[0]
+    if kind == "tet":
+        xyz, cells = synth.tet_cube(n, seed=999)
+        fac, cls = DMOMeshTetFactory, DMOTetClass
+    else:
+        xyz, cells = synth.hex_cube(n, seed=999, jitter=0.2)
+        fac, cls = DMOMeshHexFactory, DMOHexClass
+    iters = 10
+    mp = fac.create(xyz, cells)
+    op = cls(mp, n_iter=iters)
+    op.iterate(iters)
+    q_plain = op.getQuality()
+    res = {}
+    for mode in (1, 2):
+        m = fac.create(xyz, cells)
+        o = cls(m, n_iter=3 * iters)
+        o.set_proposal(mlp if mode == 1 else None, mode=mode, depth=2, window_frac=0.25)
+        q = [o.getQuality()]
+        it = 0
+        while it < 2 * iters and q[-1] < q_plain - 1e-3:
+            o.doIteration()
+            it += 1
+            q.append(o.getQuality())
+        st = o.stats()
+        res[mode] = (it, q[-1], st["fallbacks"] / max(1, st["searches_learned"]))
+        assert all(b >= a for a, b in zip(q, q[1:]))
+        o.close()
+        m.close()
+    print(kind, "plain", q_plain, "learned", res[1], "identity", res[2])
+    assert res[1][1] >= q_plain - 1e-3, res
+    assert res[1][2] < res[2][2], res

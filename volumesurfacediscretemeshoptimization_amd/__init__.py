@@ -27,7 +27,8 @@ from ._lib import VsdmoError, check, load  # noqa: F401
 MEAN_RATIO = 0
 
 __all__ = ["DMOMeshTriFactory", "DMOMeshTetFactory", "DMOMeshHexFactory", "DMOMesh",
-           "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "DMOBatch", "VsdmoError", "MEAN_RATIO", "load"]
+           "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "DMOBatch", "DMOMLP", "DMODataset", "VsdmoError",
+           "MEAN_RATIO", "load"]
 
 
 def _fptr(a):
@@ -251,6 +252,114 @@ class _DMOBase:
         s = _lib.Stats()
         check(self._lib.vsdmo_opt_stats(self._h, C.byref(s)), "vsdmo_opt_stats")
         return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
+
+
+    def set_proposal(self, mlp=None, mode: int = 1, depth: int = 2, window_frac: float = 0.25):
+        """Learned proposal for the inner searches: mode 1 = network, 2 = the
+        vertex itself (same windowed search and fallback, no network), 0 = off."""
+        h = mlp.handle if mlp is not None else None
+        check(self._lib.vsdmo_opt_set_proposal(self._h, h, int(mode), int(depth), float(window_frac)),
+              "vsdmo_opt_set_proposal")
+
+    def record(self, dataset, n: int):
+        """n plain iterations, recording one training pair per inner search."""
+        check(self._lib.vsdmo_opt_record(self._h, dataset.handle, int(n)), "vsdmo_opt_record")
+
+
+class DMOMLP:
+    """96-128-128-3 GELU MLP with BF16 MFMA kernels (the learned proposal)."""
+
+    def __init__(self, seed: int = 0):
+        lib = load()
+        self._lib = lib
+        h = C.c_void_p()
+        check(lib.vsdmo_mlp_create(C.c_uint64(seed), C.byref(h)), "vsdmo_mlp_create")
+        self._h = h
+        self.n_params = lib.vsdmo_mlp_num_params()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            check(self._lib.vsdmo_mlp_destroy(self._h), "vsdmo_mlp_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        check(self._lib.vsdmo_mlp_get_params(self._h, _fptr(out)), "vsdmo_mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == self.n_params
+        check(self._lib.vsdmo_mlp_set_params(self._h, _fptr(a)), "vsdmo_mlp_set_params")
+
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n = X.shape[0]
+        Y = np.empty((n, 3), dtype=np.float32)
+        check(self._lib.vsdmo_mlp_forward(self._h, _fptr(X), n, _fptr(Y)), "vsdmo_mlp_forward")
+        return Y
+
+    def gradients(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = C.c_double()
+        check(self._lib.vsdmo_mlp_gradients(self._h, _fptr(X), _fptr(T), X.shape[0], C.byref(loss), _fptr(g)),
+              "vsdmo_mlp_gradients")
+        return loss.value, g
+
+    def train(self, dataset, epochs: int = 20, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        out = np.zeros(epochs, dtype=np.float64)
+        check(self._lib.vsdmo_mlp_train(self._h, dataset.handle, int(epochs), int(batch), float(lr), C.c_uint64(seed),
+                                        out.ctypes.data_as(C.POINTER(C.c_double))), "vsdmo_mlp_train")
+        return out
+
+
+class DMODataset:
+    """Device-resident training pairs (one-ring features, displacement / L)."""
+
+    def __init__(self, capacity: int):
+        lib = load()
+        self._lib = lib
+        h = C.c_void_p()
+        check(lib.vsdmo_dataset_create(C.c_int64(capacity), C.byref(h)), "vsdmo_dataset_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            check(self._lib.vsdmo_dataset_destroy(self._h), "vsdmo_dataset_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        n = C.c_int64()
+        check(self._lib.vsdmo_dataset_size(self._h, C.byref(n)), "vsdmo_dataset_size")
+        return n.value
+
+    def get(self, start: int, n: int):
+        X = np.empty((n, 96), dtype=np.float32)
+        T = np.empty((n, 3), dtype=np.float32)
+        check(self._lib.vsdmo_dataset_get(self._h, C.c_int64(start), int(n), _fptr(X), _fptr(T)), "vsdmo_dataset_get")
+        return X, T
 
 
 class DMOTetClass(_DMOBase):

@@ -44,7 +44,7 @@ class Counts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("searches_inner", C.c_int64), ("searches_surface", C.c_int64), ("searches_feature", C.c_int64),
                 ("iterations", C.c_int32), ("min_quality", C.c_float), ("kernel_ms", C.c_double),
-                ("kernel_launches", C.c_int64)]
+                ("kernel_launches", C.c_int64), ("searches_learned", C.c_int64), ("fallbacks", C.c_int64)]
 
 
 class BatchParams(C.Structure):
@@ -89,6 +89,21 @@ EXPORTS = {
     "vsdmo_opt_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsdmo_opt_destroy": (C.c_int, [C.c_void_p]),
     "vsdmo_last_error": (C.c_char_p, []),
+    "vsdmo_mlp_num_params": (C.c_int32, []),
+    "vsdmo_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "vsdmo_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "vsdmo_mlp_get_params": (C.c_int, [C.c_void_p, _fp]),
+    "vsdmo_mlp_set_params": (C.c_int, [C.c_void_p, _fp]),
+    "vsdmo_mlp_forward": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
+    "vsdmo_mlp_gradients": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, C.POINTER(C.c_double), _fp]),
+    "vsdmo_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                  C.POINTER(C.c_double)]),
+    "vsdmo_dataset_create": (C.c_int, [C.c_int64, C.POINTER(C.c_void_p)]),
+    "vsdmo_dataset_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "vsdmo_dataset_get": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _fp, _fp]),
+    "vsdmo_dataset_destroy": (C.c_int, [C.c_void_p]),
+    "vsdmo_opt_record": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "vsdmo_opt_set_proposal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float]),
     "vsdmo_batch_params_default": (None, [C.POINTER(BatchParams)]),
     "vsdmo_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(BatchParams),
                                      C.POINTER(C.c_void_p)]),

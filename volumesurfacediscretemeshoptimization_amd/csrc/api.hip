@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "dmo_kernels.h"
+#include "mlp_internal.h"
 #include "vsdmo_internal.h"
 
 namespace vsdmo {
@@ -97,6 +98,15 @@ struct vsdmo_opt {
     DevBuf<int> nn, frow, table;
     DevBuf<uint8_t> opt1d;
     std::vector<uint8_t> opt1d_host;
+    // learned proposal (inner tet / hex searches)
+    vsdmo_mlp* mlp = nullptr;
+    int learn_mode = 0, learn_depth = 2;
+    float learn_frac = 0.25f;
+    DevBuf<uint16_t> lX;
+    DevBuf<float> lL, lV0, lY;
+    DevBuf<int> lFallback;
+    int64_t s_learned = 0, fallbacks = 0;
+    vsdmo_dataset* rec = nullptr;   // training pairs are recorded while set
 };
 
 extern "C" {
@@ -467,6 +477,25 @@ static void timed_launch(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds
     o->launches++;
 }
 
+static void timed_learned(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds)
+{
+    hipStream_t s = o->m->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (o->timing) {
+        VSDMO_HIP(hipEventCreate(&e0));
+        VSDMO_HIP(hipEventCreate(&e1));
+        VSDMO_HIP(hipEventRecord(e0, s));
+    }
+    launch_search_learned(kind, o->p.nq, a, o->lY.p, o->lL.p, o->learn_mode, o->learn_depth, o->learn_frac,
+                          o->lFallback.p, lds, s);
+    if (o->timing) {
+        VSDMO_HIP(hipEventRecord(e1, s));
+        o->ev.push_back(e0);
+        o->ev.push_back(e1);
+    }
+    o->launches++;
+}
+
 static void collect_timing(vsdmo_opt* o)
 {
     if (o->ev.empty()) return;
@@ -499,8 +528,29 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                 size_t lds = search_lds_bytes(m->elem, m->maxRing);
                 for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c) {
                     a.cStart = m->colOffInner[c]; a.cEnd = m->colOffInner[c + 1];
-                    timed_launch(o, m->elem, a, lds);
-                    o->s_inner += a.cEnd - a.cStart;
+                    const int nc = a.cEnd - a.cStart;
+                    vsdmo_dataset* ds = o->rec;
+                    const int take = ds ? (int)std::min<long long>(nc, ds->cap - ds->n) : 0;
+                    if (take > 0)
+                        launch_features(m->x.p, m->y.p, m->z.p, m->nbr.p, a.cStart, a.cStart + take,
+                                        ds->X.p + ds->n * kIn, ds->L.p, ds->v0.p, m->stream);
+                    if (o->learn_mode != 0 && !ds) {
+                        if (o->learn_mode == 1) {
+                            launch_features(m->x.p, m->y.p, m->z.p, m->nbr.p, a.cStart, a.cEnd, o->lX.p, o->lL.p,
+                                            o->lV0.p, m->stream);
+                            mlp_forward_dev(o->mlp, o->lX.p, nc, o->lY.p, m->stream);
+                        }
+                        timed_learned(o, m->elem, a, lds);
+                        o->s_learned += nc;
+                    } else {
+                        timed_launch(o, m->elem, a, lds);
+                    }
+                    if (take > 0) {
+                        launch_targets(m->x.p, m->y.p, m->z.p, a.cStart, a.cStart + take, ds->L.p, ds->v0.p,
+                                       ds->T.p + ds->n * kOut, m->stream);
+                        ds->n += take;
+                    }
+                    o->s_inner += nc;
                 }
                 if (o->surface && o->last_min > 0.0f) {
                     SurfArgs sa = surf_args(o);
@@ -542,6 +592,12 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
             float prev = o->last_min;
             opt_quality(o);
             o->curr_it++;
+            if (o->lFallback.p) {
+                int f = 0;
+                o->lFallback.download(&f, 1, m->stream);
+                VSDMO_HIP(hipStreamSynchronize(m->stream));
+                o->fallbacks = f;
+            }
             if (o->p.quality_guard) {
                 float tol = o->p.guard_tolerance * std::fabs(prev);
                 if (o->last_min < prev - tol) {
@@ -609,7 +665,67 @@ vsdmo_status vsdmo_opt_stats(const vsdmo_opt* o, vsdmo_stats* out)
     out->min_quality = o->last_min;
     out->kernel_ms = o->kernel_ms;
     out->kernel_launches = o->launches;
+    out->searches_learned = o->s_learned;
+    out->fallbacks = o->fallbacks;
     return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
+        vsdmo_mesh* m = o->m;
+        if (mode < 0 || mode > 2) return fail(VSDMO_INVALID_ARG, "mode must be 0 (off), 1 (network) or 2 (identity)");
+        if (mode == 1 && !mlp) return fail(VSDMO_INVALID_ARG, "mode 1 needs a network");
+        if (mode != 0 && m->elem != VSDMO_TET && m->elem != VSDMO_HEX)
+            return fail(VSDMO_NOT_SUPPORTED, "the learned proposal is for tet and hex inner vertices");
+        if (mode != 0 && (depth < 1 || depth > 8 || !(window_frac > 0.0f)))
+            return fail(VSDMO_INVALID_ARG, "depth must be in [1,8] and window_frac > 0");
+        o->learn_mode = mode;
+        o->mlp = mode == 1 ? mlp : nullptr;
+        o->learn_depth = depth;
+        o->learn_frac = window_frac;
+        if (mode == 0) return VSDMO_OK;
+        int maxc = 1;
+        for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c)
+            maxc = std::max(maxc, m->colOffInner[c + 1] - m->colOffInner[c]);
+        if (!o->lFallback.p) {
+            o->lFallback.alloc(1);
+            launch_fill_int(o->lFallback.p, 1, 0, m->stream);
+        }
+        if ((int)o->lL.n < maxc) {
+            o->lL.alloc(maxc); o->lV0.alloc((size_t)3 * maxc); o->lY.alloc((size_t)3 * maxc);
+            o->lX.alloc((size_t)kIn * maxc);
+        }
+        if (mode == 1) {
+            build_neighbour_table(*m);
+            VSDMO_HIP(hipStreamSynchronize(mlp->stream));
+            mlp_reserve(mlp, maxc);
+        }
+        VSDMO_HIP(hipStreamSynchronize(m->stream));
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_opt_record(vsdmo_opt* o, vsdmo_dataset* ds, int32_t n)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!o || !ds) return fail(VSDMO_INVALID_ARG, "NULL argument");
+        vsdmo_mesh* m = o->m;
+        if (m->elem != VSDMO_TET && m->elem != VSDMO_HEX)
+            return fail(VSDMO_NOT_SUPPORTED, "training pairs are recorded from tet and hex inner searches");
+        int maxc = 1;
+        for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c)
+            maxc = std::max(maxc, m->colOffInner[c + 1] - m->colOffInner[c]);
+        build_neighbour_table(*m);
+        VSDMO_HIP(hipStreamSynchronize(ds->stream));
+        if ((int)ds->L.n < maxc) { ds->L.alloc(maxc); ds->v0.alloc((size_t)3 * maxc); }
+        o->rec = ds;
+        vsdmo_status st = vsdmo_opt_iterate(o, n);
+        o->rec = nullptr;
+        VSDMO_HIP(hipStreamSynchronize(m->stream));
+        return st;
+    });
 }
 
 vsdmo_status vsdmo_opt_set_timing(vsdmo_opt* o, int32_t on)

@@ -447,6 +447,247 @@ __global__ __launch_bounds__(64 * W) void k_search_flat(SearchArgs a)
     flat_vertex<NQ, W>(a, a.cStart + blockIdx.x, s_tr, rd);
 }
 
+// ------------------------------------------------------ learned proposal
+// Vertex i of the colour class gets the proposal c = v + L_i * Y_i (network
+// output in units of its mean edge length L_i; mode 2: c = v).  A 4x4x4 grid
+// (one candidate per lane, candidate index = lane, lowest wins ties) is
+// searched around c on a window `frac` times the plain first-level window,
+// for `depthL` levels with the window shrunk by 2/3 per level; the centre
+// moves to a level's best candidate when it is strictly better than the best
+// quality so far, which starts at q(v).  Fallback rule: when no candidate of
+// any level beats q(v), the vertex gets the plain full search, unchanged.
+struct LearnArgs {
+    const float* Y;
+    const float* L;
+    int mode;
+    int depthL;
+    float frac;
+    int* fallbacks;
+};
+
+template <int NQ>
+__device__ void tet_vertex_learned(const SearchArgs& a, const LearnArgs& la, int vid, int ci, TetC* s_tc,
+                                   RedLds<1>& rd)
+{
+    const int lane = threadIdx.x;
+    const int r0 = a.ringRow[vid];
+    const int n = a.ringRow[vid + 1] - r0;
+    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
+    const bool emin = a.extentMin != 0;
+    float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
+    for (int e = lane; e < n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 3;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            int v = h[t];
+            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
+            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+        }
+    }
+    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
+    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
+    mabs = wave_max(mabs);
+    const float sg = pow2_scale(mabs);
+    const float isg = 1.0f / sg;
+    const float is3 = ((1.0f * isg) * isg) * isg;
+    for (int e = lane; e < n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 3;
+        f3 p[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            int v = h[t];
+            p[t] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+        }
+        s_tc[e] = tet_consts(p[0], p[1], p[2]);
+    }
+    __syncthreads();
+    auto score = [&](float px, float py, float pz) {
+        const float Qyz = 3.0f * fmaf(pz, pz, py * py);
+        const float t3 = 3.0f * px;
+        TetAcc acc = tet_acc0();
+        for (int e = 0; e < n; ++e) {
+            const TetC c = s_tc[e];
+            float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+            float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+            tet_acc_add(acc, fmaf(c.nnx, px, Tyz), fmaf(px, t3 + c.m2sx, Ryz));
+        }
+        return tet_acc_score(acc, is3);
+    };
+    float qbest = score(0.0f, 0.0f, 0.0f);
+    float cx = x0x, cy = x0y, cz = x0z;
+    if (la.mode == 1) {
+        const float l = la.L[ci];
+        cx = x0x + l * la.Y[3 * ci];
+        cy = x0y + l * la.Y[3 * ci + 1];
+        cz = x0z + l * la.Y[3 * ci + 2];
+    }
+    constexpr int NL = 4;
+    const float affL = 1.0f / (float)(NL - 1);
+    const int li = lane % NL, lj = (lane / NL) % NL, lk = lane / (NL * NL);
+    float ds = la.frac * a.gridScale;
+    bool moved = false;
+    for (int d = 0; d < la.depthL; ++d) {
+        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
+        const float gyM = cy + ds * ey, gym = cy - ds * ey;
+        const float gzM = cz + ds * ez, gzm = cz - ds * ez;
+        const float px = grid_coord(affL, li, NL, gxm, gxM);
+        const float py = grid_coord(affL, lj, NL, gym, gyM);
+        const float pz = grid_coord(affL, lk, NL, gzm, gzM);
+        float best = score((px - x0x) * sg, (py - x0y) * sg, (pz - x0z) * sg);
+        int cand = lane;
+        wave_argmax(best, cand);
+        if (qbest < best) {
+            cx = __shfl(px, cand); cy = __shfl(py, cand); cz = __shfl(pz, cand);
+            qbest = best;
+            moved = true;
+        }
+        ds = ds * (2.0f / (float)(NL - 1));
+    }
+    if (moved) {
+        if (lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+        return;
+    }
+    if (lane == 0) atomicAdd(la.fallbacks, 1);
+    __syncthreads();   // the plain search restages the ring
+    tet_vertex<NQ, 1>(a, vid, emin, s_tc, rd);
+}
+
+template <int NQ>
+__device__ void hex_vertex_learned(const SearchArgs& a, const LearnArgs& la, int vid, int ci, HexLds* s_hx,
+                                   RedLds<1>& rd)
+{
+    const int lane = threadIdx.x;
+    const int r0 = a.ringRow[vid];
+    const int n = a.ringRow[vid + 1] - r0;
+    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
+    const bool emin = a.extentMin != 0;
+    float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
+    for (int e = lane; e < n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 7;
+        for (int t = 0; t < 7; ++t) {
+            int v = h[t];
+            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
+            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+        }
+    }
+    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
+    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
+    mabs = wave_max(mabs);
+    const float sg = pow2_scale(mabs);
+    const float s2 = sg * sg, s3 = s2 * sg;
+    const float epsD = kHexEps * s3, epsF = kHexEps * s2;
+    for (int e = lane; e < n; e += 64) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 7;
+        f3 r[8];
+        r[0] = { 0.0f, 0.0f, 0.0f };
+        for (int t = 0; t < 7; ++t) {
+            int v = h[t];
+            r[t + 1] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+        }
+        HexLds Lh;
+        hex_consts(r, Lh.cd, Lh.cF, Lh.hp);
+        s_hx[e] = Lh;
+    }
+    __syncthreads();
+    HexAcc acc0 = hex_acc0();
+    for (int e = 0; e < n; ++e)
+#pragma unroll
+        for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(acc0, s_hx[e].cd[sIdx], s_hx[e].cF[sIdx]);
+    auto score = [&](float px, float py, float pz) {
+        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
+        HexAcc acc = acc0;
+        for (int e = 0; e < n; ++e)
+#pragma unroll
+            for (int sIdx = 0; sIdx < 5; ++sIdx) {
+                const HexP h = s_hx[e].hp[sIdx];
+                const bool a3 = (sIdx == 1 || sIdx == 4);
+                float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+                float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+                hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
+            }
+        return hex_acc_score(acc, epsD, epsF);
+    };
+    float qbest = score(0.0f, 0.0f, 0.0f);
+    float cx = x0x, cy = x0y, cz = x0z;
+    if (la.mode == 1) {
+        const float l = la.L[ci];
+        cx = x0x + l * la.Y[3 * ci];
+        cy = x0y + l * la.Y[3 * ci + 1];
+        cz = x0z + l * la.Y[3 * ci + 2];
+    }
+    constexpr int NL = 4;
+    const float affL = 1.0f / (float)(NL - 1);
+    const int li = lane % NL, lj = (lane / NL) % NL, lk = lane / (NL * NL);
+    float ds = la.frac * a.gridScale;
+    bool moved = false;
+    for (int d = 0; d < la.depthL; ++d) {
+        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
+        const float gyM = cy + ds * ey, gym = cy - ds * ey;
+        const float gzM = cz + ds * ez, gzm = cz - ds * ez;
+        const float px = grid_coord(affL, li, NL, gxm, gxM);
+        const float py = grid_coord(affL, lj, NL, gym, gyM);
+        const float pz = grid_coord(affL, lk, NL, gzm, gzM);
+        float best = score((px - x0x) * sg, (py - x0y) * sg, (pz - x0z) * sg);
+        int cand = lane;
+        wave_argmax(best, cand);
+        if (qbest < best) {
+            cx = __shfl(px, cand); cy = __shfl(py, cand); cz = __shfl(pz, cand);
+            qbest = best;
+            moved = true;
+        }
+        ds = ds * (2.0f / (float)(NL - 1));
+    }
+    if (moved) {
+        if (lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+        return;
+    }
+    if (lane == 0) atomicAdd(la.fallbacks, 1);
+    __syncthreads();
+    hex_vertex<NQ, 1>(a, vid, emin, s_hx, rd);
+}
+
+template <int NQ>
+__global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnArgs la)
+{
+    extern __shared__ TetC s_tcl[];
+    __shared__ RedLds<1> rd;
+    tet_vertex_learned<NQ>(a, la, a.cStart + blockIdx.x, blockIdx.x, s_tcl, rd);
+}
+template <int NQ>
+__global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnArgs la)
+{
+    extern __shared__ HexLds s_hxl[];
+    __shared__ RedLds<1> rd;
+    hex_vertex_learned<NQ>(a, la, a.cStart + blockIdx.x, blockIdx.x, s_hxl, rd);
+}
+
+void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y, const float* L, int mode, int depthL,
+                           float frac, int* fallbacks, size_t lds, hipStream_t s)
+{
+    const int nb = a.cEnd - a.cStart;
+    if (nb <= 0) return;
+    LearnArgs la{ Y, L, mode, depthL, frac, fallbacks };
+#define VSDMO_LEARN_CASE(Q)                                                                   \
+    case Q:                                                                                  \
+        if (kind == VSDMO_TET) k_search_tet_learned<Q><<<nb, 64, lds, s>>>(a, la);           \
+        else k_search_hex_learned<Q><<<nb, 64, lds, s>>>(a, la);                             \
+        break;
+    switch (nq) {
+        VSDMO_LEARN_CASE(2)
+        VSDMO_LEARN_CASE(4)
+        VSDMO_LEARN_CASE(6)
+        VSDMO_LEARN_CASE(8)
+    default: break;
+    }
+#undef VSDMO_LEARN_CASE
+}
+
 // ------------------------------------------------------- batched sweep
 // Persistent kernel of one colour round of a batch of meshes: blocks take
 // work items from a queue (one atomic counter per round) until it is empty.

@@ -98,6 +98,7 @@ struct vsdmo_mesh {
     vsdmo::DevBuf<int> newOfOld;     // nV
     vsdmo::DevBuf<int> heRow, heTarget, heOpp, heFace, heNext;
     vsdmo::DevBuf<uint8_t> isB1D, isFeature;
+    vsdmo::DevBuf<int> nbr;          // canonical neighbour table (learned proposal), built on demand
     hipStream_t stream = nullptr;
     int n_opts = 0;   // live optimisers referencing this mesh
 };
