@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -43,7 +45,109 @@ WORKLOADS = {
     "hex48": {"kind": "hex", "flop": 5_014_240, "cbrt": 110_880, "kernel": "k_search_hex<8>",
               "traffic": None},
     "tri64": {"kind": "tri", "flop": 27_048, "cbrt": 0, "kernel": "k_search_flat<8>", "traffic": None},
+    "batch256": {"kind": "batch", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_batch_round", "traffic": None},
 }
+# algorithmic FLOP of one flat-triangle vertex update (8x8 candidates: 196 ring
+# evaluations x 6 triangles x 23 FLOP)
+FLOP_PER_UPDATE_TRI = 27_048
+
+
+def bench_batch(args):
+    """256 meshes (half flat triangle grids, half tet cubes, vertex counts
+    log-uniform in [500, 50000]) swept colour round by colour round, one
+    iteration replayed as a hipGraph, parameters from the autotuner (run
+    before the timed region).  The unbatched baseline (one optimiser per
+    mesh, meshes one after another) is timed beside it."""
+    from volumesurfacediscretemeshoptimization_amd import (DMOBatch, DMOMeshTetFactory, DMOMeshTriFactory,
+                                                           DMOTetClass, DMOTriFlatClass, synth)
+    mix = synth.batch_mix(256, seed=args.seed)
+    fac = {"tri": DMOMeshTriFactory, "tet": DMOMeshTetFactory}
+    t0 = time.perf_counter()
+    meshes = [fac[k].create(xyz, cells) for k, xyz, cells in mix]
+    build_s = time.perf_counter() - t0
+    b = DMOBatch(meshes)
+    q0 = b.min_quality()
+    chosen = b.autotune(iters=args.tune_iters, tol=1e-3, log_path=args.tune_log)
+    # warm-up: one iteration, then back to the start positions
+    b.iterate(max(args.warmup, 1))
+    b.restore()
+    st0 = b.stats()
+    t0 = time.perf_counter()
+    b.iterate(args.steps)
+    dt = time.perf_counter() - t0
+    st = b.stats()
+    q1 = b.min_quality()
+    searches = st["searches"] - st0["searches"]
+    gpu_s = (st["gpu_ms"] - st0["gpu_ms"]) / 1e3
+    # algorithmic work per iteration: tet searches x tet FLOP + tri searches x tri FLOP
+    per_mesh = []
+    for i, (k, _, _) in enumerate(mix):
+        c = meshes[i].counts()
+        per_mesh.append(c["n_inner"] if k == "tet" else c["n_free"])
+    tet_s = sum(n for n, (k, _, _) in zip(per_mesh, mix) if k == "tet")
+    tri_s = sum(n for n, (k, _, _) in zip(per_mesh, mix) if k == "tri")
+    flops = args.steps * (tet_s * WORKLOADS["tet40"]["flop"] + tri_s * FLOP_PER_UPDATE_TRI)
+    achieved = flops / gpu_s / 1e12
+    # unbatched baseline with the reference parameters (8, 3)
+    for k, xyz, cells in mix[:2]:          # warm the single-mesh kernels on throwaway copies
+        wm = fac[k].create(xyz, cells)
+        wo = (DMOTriFlatClass if k == "tri" else DMOTetClass)(wm, n_iter=1)
+        wo.iterate(1)
+        wo.close()
+        wm.close()
+    plain_meshes = [fac[k].create(xyz, cells) for k, xyz, cells in mix]
+    opts = [(DMOTriFlatClass if k == "tri" else DMOTetClass)(m, n_iter=args.steps)
+            for (k, _, _), m in zip(mix, plain_meshes)]
+    t1 = time.perf_counter()
+    for o in opts:
+        o.iterate(args.steps)
+    dt_plain = time.perf_counter() - t1
+    plain_updates = sum(o.stats()["searches_inner"] + o.stats()["searches_surface"] for o in opts)
+    q_plain = np.array([o.getQuality() for o in opts], dtype=np.float32)
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_py
+        n = 0
+        tc = time.perf_counter()
+        sample = 0
+        for k, xyz, cells in mix:
+            om = oracle_py.OracleMesh(k, xyz, cells)
+            n += om.iterate(nthreads=threads)
+            sample += 1
+            if time.perf_counter() - tc > 15.0:
+                break
+        dtc = time.perf_counter() - tc
+        cpu = {"value": n / dtc, "unit": "vertex updates/s", "cores": threads, "kind": "port",
+               "sample": f"one DMO iteration of each of the first {sample} meshes of the mix ({n} vertex searches, "
+                         f"{dtc:.1f} s) by the C/OpenMP oracle with {threads} threads"}
+    out = {
+        "metric": METRIC, "value": searches / dt, "unit": "vertex updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched DMO, 256 meshes (even: flat triangle grids, odd: tet cubes; vertex counts "
+                               f"log-uniform in [500, 50000], seed {args.seed}; jitter +-0.35h), boundary fixed, one "
+                               "persistent launch per colour round, iteration replayed as a hipGraph, autotuned",
+                   "name": "batch256", "meshes": st["meshes"], "total_vertices": st["total_vertices"],
+                   "rounds": st["rounds"], "vertex_updates_per_step": st["items_per_iteration"],
+                   "autotuned": {k: chosen[k] for k in ("nq", "depth", "waves_per_vertex")},
+                   "mesh_build_s": build_s, "min_quality_start_worst": float(q0.min()),
+                   "min_quality_end_worst": float(q1.min()),
+                   "unbatched_value": plain_updates / dt_plain, "unbatched_ms_per_step": 1e3 * dt_plain / args.steps,
+                   "unbatched_min_quality_end_worst": float(q_plain.min()),
+                   "batched_over_unbatched": (searches / dt) / (plain_updates / dt_plain),
+                   "tune_log": args.tune_log},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_batch_round (all rounds + "
+                     "per-mesh quality, device time of the replayed iterations)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+    for o in opts:
+        o.close()
+    return 0
 
 
 def make_mesh(name: str, seed: int):
@@ -89,6 +193,119 @@ def cpu_baseline(kind, xyz, cells, iters: int, threads: int, surface: bool, name
                       f"by the C/OpenMP oracle with {threads} threads"}
 
 
+def bench_learned(args):
+    """Learned proposal vs plain DMO on the same mesh.  Untimed setup: training
+    pairs recorded from plain DMO on seeded meshes of the same family (seeds
+    1000+), then Adam training of the MLP.  Timed: the plain run of K
+    iterations, then the learned run until its minimum quality reaches the
+    plain run's final one (-1e-3).  value = plain-equivalent vertex updates
+    per second at equal final minimum quality (K x inner vertices / learned
+    wall time)."""
+    from volumesurfacediscretemeshoptimization_amd import (DMODataset, DMOHexClass, DMOMeshHexFactory,
+                                                           DMOMeshTetFactory, DMOMLP, DMOTetClass)
+    kind = WORKLOADS[args.workload]["kind"]
+    fac, cls = (DMOMeshTetFactory, DMOTetClass) if kind == "tet" else (DMOMeshHexFactory, DMOHexClass)
+    xyz, cells, desc, kw = make_mesh(args.workload, args.seed)
+    kw = {"fixed_boundary": kw["fixed_boundary"]}
+    # training pairs
+    t0 = time.perf_counter()
+    ds = DMODataset(args.pairs)
+    s = 0
+    while len(ds) < args.pairs:
+        txyz, tcells, _, _ = make_mesh(args.workload, 1000 + s)
+        m = fac.create(txyz, tcells)
+        o = cls(m, n_iter=args.steps, **kw)
+        o.record(ds, args.steps)
+        o.close()
+        m.close()
+        s += 1
+    rec_s = time.perf_counter() - t0
+    mlp = DMOMLP(seed=args.seed + 1)
+    t0 = time.perf_counter()
+    losses = mlp.train(ds, epochs=args.epochs, batch=4096, lr=1e-3, seed=args.seed + 2)
+    train_s = time.perf_counter() - t0
+    n_pairs = len(ds)
+    ds.close()
+    # warm-up of both paths on a throwaway copy
+    wm = fac.create(xyz, cells)
+    wo = cls(wm, n_iter=2, **kw)
+    wo.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac)
+    wo.iterate(1)
+    wo.set_proposal(None, mode=0)
+    wo.iterate(1)
+    wo.close()
+    wm.close()
+    # plain
+    mp = fac.create(xyz, cells)
+    op = cls(mp, n_iter=args.steps, **kw)
+    q0 = op.getQuality()
+    t0 = time.perf_counter()
+    op.iterate(args.steps)
+    dt_plain = time.perf_counter() - t0
+    stp = op.stats()
+    q_plain, qm_plain = op.getQuality(), op.getMeanQuality()
+    plain_updates = stp["searches_inner"] + stp["searches_surface"] + stp["searches_feature"]
+    # learned
+    ml = fac.create(xyz, cells)
+    ol = cls(ml, n_iter=3 * args.steps, **kw)
+    ol.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac)
+    ol.set_timing(True)
+    its = 0
+    t0 = time.perf_counter()
+    while its < 3 * args.steps:
+        ol.doIteration()
+        its += 1
+        # equal quality: minimum and mean within 1e-3 of the plain run's
+        if ol.getQuality() >= q_plain - 1e-3 and ol.getMeanQuality() >= qm_plain - 1e-3:
+            break
+    dt_learned = time.perf_counter() - t0
+    stl = ol.stats()
+    q_l, qm_l = ol.getQuality(), ol.getMeanQuality()
+    learned_updates = stl["searches_inner"] + stl["searches_surface"] + stl["searches_feature"]
+    reached = q_l >= q_plain - 1e-3 and qm_l >= qm_plain - 1e-3
+    value = plain_updates / dt_learned if reached else 0.0
+    wl = WORKLOADS[args.workload]
+    ring = 24 if kind == "tet" else 8
+    per_elem = 73 if kind == "tet" else 407
+    evals = stl["searches_learned"] * (64 * args.learn_depth + 1) + stl["fallbacks"] * 1540
+    flops = evals * ring * per_elem
+    achieved = flops / (stl["kernel_ms"] / 1e3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "vertex updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt_learned / its, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 search, bf16 MLP", "data": "synthetic",
+        "config": {"workload": desc + "; learned proposal (4x4x4 window 1/4 of the plain one, "
+                   f"{args.learn_depth} levels, fallback to the full search) vs plain", "name": args.workload,
+                   "mode": "learned", "value_definition": "plain run's vertex updates / learned wall time to reach "
+                   "the plain run's final min and mean quality (-1e-3)",
+                   "plain_value": plain_updates / dt_plain, "plain_ms_per_step": 1e3 * dt_plain / args.steps,
+                   "plain_iterations": args.steps, "learned_iterations": its,
+                   "learned_raw_value": learned_updates / dt_learned,
+                   "speedup_vs_plain": (value / (plain_updates / dt_plain)) if reached else 0.0,
+                   "reached_plain_quality": bool(reached), "min_quality_start": q0,
+                   "min_quality_plain": q_plain, "min_quality_learned": q_l,
+                   "mean_quality_plain": qm_plain, "mean_quality_learned": qm_l,
+                   "fallback_rate": stl["fallbacks"] / max(1, stl["searches_learned"]),
+                   "training_pairs": n_pairs, "training_meshes": s, "record_s": rec_s, "train_s": train_s,
+                   "epochs": args.epochs, "loss_first_epoch": float(losses[0]) if len(losses) else None,
+                   "loss_last_epoch": float(losses[-1]) if len(losses) else None},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": f"k_search_{kind}_learned<8>", "launches": stl["kernel_launches"],
+                     "mean_launch_us": 1e3 * stl["kernel_ms"] / max(1, stl["kernel_launches"])},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"],
+                                           args.workload)
+    print(json.dumps(out), flush=True)
+    for o in (op, ol):
+        o.close()
+    mlp.close()
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,11 +315,23 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--tune-iters", type=int, default=3, help="iterations per autotuner trial (batch256)")
+    ap.add_argument("--mode", choices=["plain", "learned"], default="plain")
+    ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs (learned mode)")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--learn-depth", type=int, default=4)
+    ap.add_argument("--learn-frac", type=float, default=0.25)
+    ap.add_argument("--tune-log", default=os.environ.get("VSDMO_TUNE_LOG", "autotune_batch256.jsonl"))
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this DMO build runs on a single MI355X and has no multi-GPU "
               "path; run it with --gpus 1.", file=sys.stderr)
         return 2
+
+    if args.workload == "batch256":
+        return bench_batch(args)
+    if args.mode == "learned":
+        return bench_learned(args)
 
     from volumesurfacediscretemeshoptimization_amd import (DMOHexClass, DMOMeshHexFactory, DMOMeshTetFactory,
                                                            DMOMeshTriFactory, DMOTetClass, DMOTriFlatClass)

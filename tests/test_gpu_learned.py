@@ -162,7 +162,7 @@ def test_learned_reaches_plain_quality_and_ablation(kind):
     train = [(n, 200 + s) for s in range(8)]
     ds = _record(kind, train, 8, 1 << 21)
     mlp = DMOMLP(seed=5)
-    loss = mlp.train(ds, epochs=8, batch=1024, lr=1e-3, seed=6)
+    loss = mlp.train(ds, epochs=20, batch=1024, lr=1e-3, seed=6)
     assert loss[-1] < loss[0]
     if kind == "tet":
         xyz, cells = synth.tet_cube(n, seed=999)
@@ -179,7 +179,7 @@ def test_learned_reaches_plain_quality_and_ablation(kind):
     for mode in (1, 2):
         m = fac.create(xyz, cells)
         o = cls(m, n_iter=3 * iters)
-        o.set_proposal(mlp if mode == 1 else None, mode=mode, depth=2, window_frac=0.25)
+        o.set_proposal(mlp if mode == 1 else None, mode=mode, depth=4, window_frac=0.25)
         q = [o.getQuality()]
         it = 0
         while it < 2 * iters and q[-1] < q_plain - 1e-3:

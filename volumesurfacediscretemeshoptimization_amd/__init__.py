@@ -254,7 +254,7 @@ class _DMOBase:
         return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
 
 
-    def set_proposal(self, mlp=None, mode: int = 1, depth: int = 2, window_frac: float = 0.25):
+    def set_proposal(self, mlp=None, mode: int = 1, depth: int = 4, window_frac: float = 0.25):
         """Learned proposal for the inner searches: mode 1 = network, 2 = the
         vertex itself (same windowed search and fallback, no network), 0 = off."""
         h = mlp.handle if mlp is not None else None

@@ -100,7 +100,7 @@ struct vsdmo_opt {
     std::vector<uint8_t> opt1d_host;
     // learned proposal (inner tet / hex searches)
     vsdmo_mlp* mlp = nullptr;
-    int learn_mode = 0, learn_depth = 2;
+    int learn_mode = 0, learn_depth = 4;
     float learn_frac = 0.25f;
     DevBuf<uint16_t> lX;
     DevBuf<float> lL, lV0, lY;
