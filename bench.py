@@ -206,7 +206,7 @@ def bench_learned(args):
     kind = WORKLOADS[args.workload]["kind"]
     fac, cls = (DMOMeshTetFactory, DMOTetClass) if kind == "tet" else (DMOMeshHexFactory, DMOHexClass)
     xyz, cells, desc, kw = make_mesh(args.workload, args.seed)
-    kw = {"fixed_boundary": kw["fixed_boundary"]}
+    kw = {"fixed_boundary": kw["fixed_boundary"], "waves_per_vertex": args.waves}
     # training pairs
     t0 = time.perf_counter()
     ds = DMODataset(args.pairs)
@@ -317,6 +317,7 @@ def main(argv=None):
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--tune-iters", type=int, default=3, help="iterations per autotuner trial (batch256)")
     ap.add_argument("--mode", choices=["plain", "learned"], default="plain")
+    ap.add_argument("--waves", type=int, default=1, help="wavefronts per vertex search (1, 2, 4)")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs (learned mode)")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--learn-depth", type=int, default=4)
@@ -341,6 +342,7 @@ def main(argv=None):
     fac = {"tet": DMOMeshTetFactory, "hex": DMOMeshHexFactory, "tri": DMOMeshTriFactory}[kind]
     cls = {"tet": DMOTetClass, "hex": DMOHexClass, "tri": DMOTriFlatClass}[kind]
     xyz, cells, desc, kw = make_mesh(args.workload, args.seed)
+    kw["waves_per_vertex"] = args.waves
     # warm-up on a throwaway copy
     if args.warmup > 0:
         wm = fac.create(xyz, cells)
@@ -378,6 +380,7 @@ def main(argv=None):
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"], args.workload)
+    desc += f"; {args.waves} wavefront(s) per vertex"
     out = {
         "metric": METRIC, "value": value, "unit": "vertex updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",

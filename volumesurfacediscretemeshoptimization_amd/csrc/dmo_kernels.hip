@@ -424,9 +424,15 @@ __device__ void flat_vertex(const SearchArgs& a, int vid, TriC* s_tr, RedLds<W>&
     if (qold < qlast && tid == 0) { a.xw[vid] = cx; a.yw[vid] = cy; }
 }
 
-// one vertex per block, vertices [cStart, cEnd) of one colour class
+// one vertex per block, vertices [cStart, cEnd) of one colour class.  The
+// tet search asks for 7 waves per SIMD (72 VGPRs, no spill): a colour class of
+// the 40^3 cube (~7.4k vertices) then fits the 256 CUs in about one round
+// instead of 1.45 rounds at the unconstrained 95 VGPRs.
+#ifndef VSDMO_TET_MIN_WAVES
+#define VSDMO_TET_MIN_WAVES 5
+#endif
 template <int NQ, int W>
-__global__ __launch_bounds__(64 * W) void k_search_tet(SearchArgs a)
+__global__ __launch_bounds__(64 * W, VSDMO_TET_MIN_WAVES) void k_search_tet(SearchArgs a)
 {
     extern __shared__ TetC s_tc[];
     __shared__ RedLds<W> rd;
@@ -447,6 +453,12 @@ __global__ __launch_bounds__(64 * W) void k_search_flat(SearchArgs a)
     flat_vertex<NQ, W>(a, a.cStart + blockIdx.x, s_tr, rd);
 }
 
+__global__ void k_fill_int(int* __restrict__ p, int n, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ------------------------------------------------------ learned proposal
 // Vertex i of the colour class gets the proposal c = v + L_i * Y_i (network
 // output in units of its mean edge length L_i; mode 2: c = v).  A 4x4x4 grid
@@ -462,7 +474,9 @@ struct LearnArgs {
     int mode;
     int depthL;
     float frac;
-    int* fallbacks;
+    int* fallbacks;   // running total
+    int* list;        // this class's fallback vertices (appended)
+    int* cnt;         // their number (zeroed before the class)
 };
 
 template <int NQ>
@@ -550,9 +564,11 @@ __device__ void tet_vertex_learned(const SearchArgs& a, const LearnArgs& la, int
         if (lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
         return;
     }
-    if (lane == 0) atomicAdd(la.fallbacks, 1);
-    __syncthreads();   // the plain search restages the ring
-    tet_vertex<NQ, 1>(a, vid, emin, s_tc, rd);
+    if (lane == 0) {
+        atomicAdd(la.fallbacks, 1);
+        la.list[atomicAdd(la.cnt, 1)] = vid;
+    }
+    (void)rd;
 }
 
 template <int NQ>
@@ -647,9 +663,11 @@ __device__ void hex_vertex_learned(const SearchArgs& a, const LearnArgs& la, int
         if (lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
         return;
     }
-    if (lane == 0) atomicAdd(la.fallbacks, 1);
-    __syncthreads();
-    hex_vertex<NQ, 1>(a, vid, emin, s_hx, rd);
+    if (lane == 0) {
+        atomicAdd(la.fallbacks, 1);
+        la.list[atomicAdd(la.cnt, 1)] = vid;
+    }
+    (void)rd;
 }
 
 template <int NQ>
@@ -667,12 +685,39 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
     hex_vertex_learned<NQ>(a, la, a.cStart + blockIdx.x, blockIdx.x, s_hxl, rd);
 }
 
+// plain search of the vertices list[0 .. *cnt) (the fallbacks of a class);
+// blocks beyond the count return at once
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_search_list_tet(SearchArgs a, const int* __restrict__ list,
+                                                           const int* __restrict__ cnt)
+{
+    extern __shared__ TetC s_tcf[];
+    __shared__ RedLds<W> rd;
+    if ((int)blockIdx.x >= *cnt) return;
+    tet_vertex<NQ, W>(a, list[blockIdx.x], a.extentMin != 0, s_tcf, rd);
+}
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_search_list_hex(SearchArgs a, const int* __restrict__ list,
+                                                           const int* __restrict__ cnt)
+{
+    extern __shared__ HexLds s_hxf[];
+    __shared__ RedLds<W> rd;
+    if ((int)blockIdx.x >= *cnt) return;
+    hex_vertex<NQ, W>(a, list[blockIdx.x], a.extentMin != 0, s_hxf, rd);
+}
+template <int NQ, int W>
+static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* cnt, int grid, size_t lds, hipStream_t s)
+{
+    if (kind == VSDMO_TET) k_search_list_tet<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
+    else k_search_list_hex<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
+}
+
 void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y, const float* L, int mode, int depthL,
-                           float frac, int* fallbacks, size_t lds, hipStream_t s)
+                           float frac, int* fallbacks, int* list, int* cnt, size_t lds, hipStream_t s)
 {
     const int nb = a.cEnd - a.cStart;
     if (nb <= 0) return;
-    LearnArgs la{ Y, L, mode, depthL, frac, fallbacks };
+    LearnArgs la{ Y, L, mode, depthL, frac, fallbacks, list, cnt };
 #define VSDMO_LEARN_CASE(Q)                                                                   \
     case Q:                                                                                  \
         if (kind == VSDMO_TET) k_search_tet_learned<Q><<<nb, 64, lds, s>>>(a, la);           \
@@ -686,6 +731,14 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y
     default: break;
     }
 #undef VSDMO_LEARN_CASE
+    // second pass: the plain full search of the fallbacks, W waves per vertex
+    switch (nq) {
+    case 2: list_nqw<2, 2>(kind, a, list, cnt, nb, lds, s); break;
+    case 4: list_nqw<4, 4>(kind, a, list, cnt, nb, lds, s); break;
+    case 6: list_nqw<6, 2>(kind, a, list, cnt, nb, lds, s); break;
+    case 8: list_nqw<8, 4>(kind, a, list, cnt, nb, lds, s); break;
+    default: break;
+    }
 }
 
 // ------------------------------------------------------- batched sweep
@@ -720,12 +773,6 @@ __global__ __launch_bounds__(64 * W) void k_batch_round(SearchArgs a, const int*
         }
         else flat_vertex<NQ, W>(a, vid, reinterpret_cast<TriC*>(s_dyn), rd);
     }
-}
-
-__global__ void k_fill_int(int* __restrict__ p, int n, int v)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
 }
 
 // ------------------------------------------------------------ qualities

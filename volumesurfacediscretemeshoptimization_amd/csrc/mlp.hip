@@ -136,22 +136,29 @@ __global__ void k_loss_final(const double* __restrict__ part, int nb, int n, dou
     *acc += s / (double)n;
 }
 
-// column sums (bias gradients), one thread per column, rows in order
-__global__ void k_colsum_f32(const float* __restrict__ a, int rows, int cols, float* __restrict__ out)
+// column sums (bias gradients): one block of 256 per column, strided FP64
+// partial sums then a fixed tree, so the order depends only on `rows`
+template <class T>
+__device__ __forceinline__ double as_f64(T v);
+template <>
+__device__ __forceinline__ double as_f64<float>(float v) { return (double)v; }
+template <>
+__device__ __forceinline__ double as_f64<uint16_t>(uint16_t v) { return (double)bf2f(v); }
+
+template <class T>
+__global__ __launch_bounds__(256) void k_colsum(const T* __restrict__ a, int rows, int cols, float* __restrict__ out)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    double s = 0.0;
-    for (int r = 0; r < rows; ++r) s += (double)a[(long long)r * cols + c];
-    out[c] = (float)s;
-}
-__global__ void k_colsum_bf16(const uint16_t* __restrict__ a, int rows, int cols, float* __restrict__ out)
-{
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    double s = 0.0;
-    for (int r = 0; r < rows; ++r) s += (double)bf2f(a[(long long)r * cols + c]);
-    out[c] = (float)s;
+    __shared__ double s[256];
+    const int c = blockIdx.x;
+    double acc = 0.0;
+    for (int r = threadIdx.x; r < rows; r += 256) acc += as_f64<T>(a[(long long)r * cols + c]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = (float)s[0];
 }
 
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
@@ -189,14 +196,11 @@ __global__ void k_gather_rows(const uint16_t* __restrict__ X, const float* __res
     if (c < 3) To[3 * r + c] = T[src * 3 + c];
 }
 
-// one-ring features: thread per vertex
-__global__ void k_features(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-                           const int* __restrict__ nbr, int lo, int hi, uint16_t* __restrict__ X, float* __restrict__ L,
-                           float* __restrict__ v0)
+// one-ring features of vertex v into o[0..95] (bf16); returns L
+__device__ __forceinline__ float vertex_features(const float* __restrict__ x, const float* __restrict__ y,
+                                                 const float* __restrict__ z, const int* __restrict__ nbr, int v,
+                                                 uint16_t* o)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int v = lo + i;
-    if (v >= hi) return;
     const int* nb = nbr + (size_t)v * kNbr;
     const float vx = x[v], vy = y[v], vz = z[v];
     float sum = 0.0f;
@@ -210,7 +214,6 @@ __global__ void k_features(const float* __restrict__ x, const float* __restrict_
     }
     float l = cnt > 0 ? sum / (float)cnt : 1.0f;
     if (!(l > 0.0f)) l = 1.0f;
-    uint16_t* o = X + (size_t)i * kIn;
     for (int k = 0; k < kNbr; ++k) {
         int u = nb[k];
         if (u < 0) {
@@ -221,8 +224,181 @@ __global__ void k_features(const float* __restrict__ x, const float* __restrict_
             o[3 * k + 2] = f2bf((z[u] - vz) / l);
         }
     }
-    L[i] = l;
-    v0[3 * i] = vx; v0[3 * i + 1] = vy; v0[3 * i + 2] = vz;
+    return l;
+}
+
+// one-ring features: thread per vertex
+__global__ void k_features(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+                           const int* __restrict__ nbr, int lo, int hi, uint16_t* __restrict__ X, float* __restrict__ L,
+                           float* __restrict__ v0)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = lo + i;
+    if (v >= hi) return;
+    L[i] = vertex_features(x, y, z, nbr, v, X + (size_t)i * kIn);
+    v0[3 * i] = x[v]; v0[3 * i + 1] = y[v]; v0[3 * i + 2] = z[v];
+}
+
+// Fused inference of 64 rows per block (4 waves, one 16-row tile each):
+// features (from the positions, or rows of Xin) -> LDS, layers 1 and 2 on
+// MFMA with the weights' 16-byte rows read straight from global memory,
+// bias + GELU to BF16 in LDS, layer 3 (128 -> 3) on the VALU.  Writes Y and
+// L of rows [0, n).
+constexpr int kLdx = kIn + 8;    // LDS row pitches (bf16), multiples of 8
+constexpr int kLdh = kHid + 8;
+struct InferArgs {
+    const float* x;
+    const float* y;
+    const float* z;
+    const int* nbr;
+    int lo, n;
+    const uint16_t* Xin;   // optional precomputed features (n x 96)
+    const uint16_t* wb;
+    const float* w;
+    float* Y;
+    float* L;
+    int* cnt;              // optional counter zeroed by block 0
+};
+
+__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a)
+{
+    __shared__ __align__(16) uint16_t sX[64 * kLdx];
+    __shared__ __align__(16) uint16_t sH[2 * 64 * kLdh];   // H1 | H2; the feature gathers use it first
+    uint16_t* sH1 = sH;
+    uint16_t* sH2 = sH + 64 * kLdh;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int row0 = blockIdx.x * 64;
+    // features
+    if (blockIdx.x == 0 && tid == 0 && a.cnt) *a.cnt = 0;   // fallback count of the search that follows
+    if (a.Xin) {
+        for (int i = tid; i < 64 * kIn; i += 256) {
+            int r = i / kIn, c = i - r * kIn;
+            sX[r * kLdx + c] = (row0 + r < a.n) ? a.Xin[(size_t)(row0 + r) * kIn + c] : (uint16_t)0;
+        }
+    } else {
+        // 4 threads per row gather 8 neighbours each; L is then summed in
+        // neighbour order by one thread (the order of k_features)
+        static_assert(sizeof(float) * 64 * kNbr * 4 <= sizeof(uint16_t) * 2 * 64 * kLdh, "feature scratch");
+        auto sRel = reinterpret_cast<float (*)[kNbr][3]>(sH);
+        auto sDist = reinterpret_cast<float (*)[kNbr]>(reinterpret_cast<float*>(sH) + 64 * kNbr * 3);
+        __shared__ float sL[64];
+        __shared__ int sCnt[64];
+        const int r = tid >> 2, q = tid & 3;
+        const int v = a.lo + row0 + r;
+        const bool live = row0 + r < a.n;
+        int cntq = 0;
+        if (live) {
+            const int* nb = a.nbr + (size_t)v * kNbr;
+            const float vx = a.x[v], vy = a.y[v], vz = a.z[v];
+            for (int k = q * 8; k < q * 8 + 8; ++k) {
+                const int u = nb[k];
+                if (u < 0) { sDist[r][k] = -1.0f; continue; }
+                const float dx = a.x[u] - vx, dy = a.y[u] - vy, dz = a.z[u] - vz;
+                sRel[r][k][0] = dx; sRel[r][k][1] = dy; sRel[r][k][2] = dz;
+                sDist[r][k] = sqrtf((dx * dx + dy * dy) + dz * dz);
+                ++cntq;
+            }
+        }
+        __syncthreads();
+        if (q == 0) {
+            float sum = 0.0f, l = 1.0f;
+            int cnt = 0;
+            if (live) {
+                for (int k = 0; k < kNbr; ++k) {
+                    if (sDist[r][k] < 0.0f) break;
+                    sum += sDist[r][k];
+                    ++cnt;
+                }
+                l = cnt > 0 ? sum / (float)cnt : 1.0f;
+                if (!(l > 0.0f)) l = 1.0f;
+                a.L[row0 + r] = l;
+            }
+            sL[r] = l;
+            sCnt[r] = live ? cnt : 0;
+        }
+        __syncthreads();
+        const float l = sL[r];
+        const int cnt = sCnt[r];
+        uint16_t* o = sX + r * kLdx;
+        for (int k = q * 8; k < q * 8 + 8; ++k) {
+            if (k < cnt) {
+                o[3 * k] = f2bf(sRel[r][k][0] / l);
+                o[3 * k + 1] = f2bf(sRel[r][k][1] / l);
+                o[3 * k + 2] = f2bf(sRel[r][k][2] / l);
+            } else {
+                o[3 * k] = 0; o[3 * k + 1] = 0; o[3 * k + 2] = 0;
+            }
+        }
+        (void)cntq;
+    }
+    __syncthreads();
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    // layer 1: 16 rows x 128 columns per wave
+    {
+        f32x4 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int k0 = 0; k0 < kIn; k0 += 32) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sX + (wave * 16 + r16) * kLdx + k0 + kq);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.wb + kOW1 + (size_t)(t * 16 + r16) * kIn + k0 + kq);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int n = t * 16 + r16;
+            const float b = a.w[kOB1 + n];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sH1[(wave * 16 + (lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
+        }
+    }
+    __syncthreads();
+    // layer 2
+    {
+        f32x4 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int k0 = 0; k0 < kHid; k0 += 32) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sH1 + (wave * 16 + r16) * kLdh + k0 + kq);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.wb + kOW2 + (size_t)(t * 16 + r16) * kHid + k0 + kq);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int n = t * 16 + r16;
+            const float b = a.w[kOB2 + n];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sH2[(wave * 16 + (lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
+        }
+    }
+    __syncthreads();
+    // layer 3 on the VALU: thread -> (row, output)
+    if (tid < 64 * kOut) {
+        const int r = tid / kOut, o = tid - r * kOut;
+        if (row0 + r < a.n) {
+            float acc = 0.0f;
+            const uint16_t* h = sH2 + r * kLdh;
+            const uint16_t* w3 = a.wb + kOW3 + o * kHid;
+            for (int k = 0; k < kHid; ++k) acc = fmaf(bf2f(h[k]), bf2f(w3[k]), acc);
+            a.Y[(size_t)(row0 + r) * kOut + o] = acc + a.w[kOB3 + o];
+        }
+    }
+}
+
+void launch_mlp_infer(vsdmo_mlp* m, const float* x, const float* y, const float* z, const int* nbr, int lo, int hi,
+                      const uint16_t* Xin, float* Y, float* L, int* cnt, hipStream_t s)
+{
+    const int n = hi - lo;
+    if (n <= 0) return;
+    InferArgs a{ x, y, z, nbr, lo, n, Xin, m->wb.p, m->w.p, Y, L, cnt };
+    k_mlp_infer<<<(n + 63) / 64, 256, 0, s>>>(a);
 }
 
 __global__ void k_targets(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, int lo,
@@ -332,7 +508,7 @@ static void mlp_backward_dev(vsdmo_mlp* m, int n, double* lossAcc, hipStream_t s
     GemmArgs g = gargs(kOut, kHid, n, m->dY16.p, 1, kOut, m->H2.p, kHid, 1);
     g.mode = EPI_F32; g.Cf = gr + kOW3; g.ldcf = kHid;
     gemm(g, s);
-    k_colsum_f32<<<1, 64, 0, s>>>(m->part.p, n, kOut, gr + kOB3);
+    k_colsum<float><<<kOut, 256, 0, s>>>(m->part.p, n, kOut, gr + kOB3);
     // dZ2 = (dY W3) * gelu'(Z2)
     g = gargs(n, kHid, kOut, m->dY16.p, kOut, 1, wb + kOW3, kHid, 1);
     g.mode = EPI_DGELU; g.Cb = m->dZ2.p; g.ldcb = kHid; g.Zin = m->Z2.p; g.ldz = kHid;
@@ -341,7 +517,7 @@ static void mlp_backward_dev(vsdmo_mlp* m, int n, double* lossAcc, hipStream_t s
     g = gargs(kHid, kHid, n, m->dZ2.p, 1, kHid, m->H1.p, kHid, 1);
     g.mode = EPI_F32; g.Cf = gr + kOW2; g.ldcf = kHid;
     gemm(g, s);
-    k_colsum_bf16<<<1, kHid, 0, s>>>(m->dZ2.p, n, kHid, gr + kOB2);
+    k_colsum<uint16_t><<<kHid, 256, 0, s>>>(m->dZ2.p, n, kHid, gr + kOB2);
     // dZ1 = (dZ2 W2) * gelu'(Z1)
     g = gargs(n, kHid, kHid, m->dZ2.p, kHid, 1, wb + kOW2, kHid, 1);
     g.mode = EPI_DGELU; g.Cb = m->dZ1.p; g.ldcb = kHid; g.Zin = m->Z1.p; g.ldz = kHid;
@@ -350,7 +526,7 @@ static void mlp_backward_dev(vsdmo_mlp* m, int n, double* lossAcc, hipStream_t s
     g = gargs(kHid, kIn, n, m->dZ1.p, 1, kHid, m->X.p, kIn, 1);
     g.mode = EPI_F32; g.Cf = gr + kOW1; g.ldcf = kIn;
     gemm(g, s);
-    k_colsum_bf16<<<1, kHid, 0, s>>>(m->dZ1.p, n, kHid, gr + kOB1);
+    k_colsum<uint16_t><<<kHid, 256, 0, s>>>(m->dZ1.p, n, kHid, gr + kOB1);
 }
 
 static void refresh_bf16(vsdmo_mlp* m, hipStream_t s)
@@ -478,7 +654,7 @@ vsdmo_status vsdmo_mlp_forward(vsdmo_mlp* m, const float* X, int32_t n, float* Y
         std::vector<uint16_t> xb((size_t)n * kIn);
         for (size_t i = 0; i < xb.size(); ++i) xb[i] = f2bf(X[i]);
         m->X.upload(xb.data(), xb.size(), m->stream);
-        mlp_forward_dev(m, m->X.p, n, m->Y.p, m->stream);
+        launch_mlp_infer(m, nullptr, nullptr, nullptr, nullptr, 0, n, m->X.p, m->Y.p, nullptr, nullptr, m->stream);
         m->Y.download(Y, (size_t)n * kOut, m->stream);
         VSDMO_HIP(hipStreamSynchronize(m->stream));
         return VSDMO_OK;
