@@ -91,12 +91,12 @@ def bench_batch(args):
     # unbatched baseline with the reference parameters (8, 3)
     for k, xyz, cells in mix[:2]:          # warm the single-mesh kernels on throwaway copies
         wm = fac[k].create(xyz, cells)
-        wo = (DMOTriFlatClass if k == "tri" else DMOTetClass)(wm, n_iter=1)
+        wo = (DMOTriFlatClass if k == "tri" else DMOTetClass)(wm, n_iter=1, fixed_boundary=True)
         wo.iterate(1)
         wo.close()
         wm.close()
     plain_meshes = [fac[k].create(xyz, cells) for k, xyz, cells in mix]
-    opts = [(DMOTriFlatClass if k == "tri" else DMOTetClass)(m, n_iter=args.steps)
+    opts = [(DMOTriFlatClass if k == "tri" else DMOTetClass)(m, n_iter=args.steps, fixed_boundary=True)
             for (k, _, _), m in zip(mix, plain_meshes)]
     t1 = time.perf_counter()
     for o in opts:

@@ -93,8 +93,9 @@ typedef struct vsdmo_stats {
 } vsdmo_stats;
 
 /* defaults: greedy colouring, 20 degrees / nq 8, depth 3, reference shrink,
-   auto extent, boundary fixed (fixed_boundary = 1; the reference moves the
-   boundary: pass 0 for its behaviour), guard on */
+   auto extent, surface and boundary vertices optimised as in the reference
+   (fixed_boundary = 0; pass 1 to keep them fixed), guard on, one wave per
+   vertex */
 void vsdmo_mesh_params_default(vsdmo_mesh_params* p);
 void vsdmo_params_default(vsdmo_params* p);
 

@@ -673,6 +673,26 @@ long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* 
     return cnt + *nSurf2d + *nFeat1d;
 }
 
+/* test hooks: the local quadric fit of a point set in the frame built from a
+   normal and a reference neighbour (computeLocalSurface1/2), and the 3x3 LU
+   solve on its own */
+void or_fit_local_surface(const float* pts, int np, const float* normal, const float* refp, float* out16)
+{
+    lsurf q;
+    q.p0 = W(pts);
+    w3 n = W(normal), r = wsub(W(refp), q.p0);
+    frame3(n, r, &q.e[0], &q.e[1], &q.e[2]);
+    w3 buf[MAX_NEIGH + 1];
+    if (np > MAX_NEIGH + 1) np = MAX_NEIGH + 1;
+    for (int i = 0; i < np; ++i) buf[i] = W(pts + 3 * i);
+    fit_quadric(&q, buf, np);
+    out16[0] = q.p0.x; out16[1] = q.p0.y; out16[2] = q.p0.z;
+    for (int k = 0; k < 3; ++k) { out16[3 + 3 * k] = q.e[k].x; out16[4 + 3 * k] = q.e[k].y; out16[5 + 3 * k] = q.e[k].z; }
+    out16[12] = q.a; out16[13] = q.b; out16[14] = q.c; out16[15] = (float)q.np;
+}
+
+void or_lu_solve3(const float* Acm, const float* rhs, float* x) { lu_solve3(Acm, rhs, x); }
+
 /* accessors for tests */
 void or_surface_get(const or_surf* s, int what, void* out)
 {

@@ -147,6 +147,12 @@ float or_min_quality(const or_mesh* m);
 void or_quality64(const or_mesh* m, double* qmin, double* qmean);
 void or_element_qualities(const or_mesh* m, float* out);
 
+/* test hooks of the surface fit: quadric of pts[0..np) (pts[0] = p0) in the
+   frame of (normal, refp - p0); out16 = p0, e0, e1, e2, a, b, c, np */
+void or_fit_local_surface(const float* pts, int np, const float* normal, const float* refp, float* out16);
+/* 3x3 LU with partial pivoting, A column-major */
+void or_lu_solve3(const float* Acm, const float* rhs, float* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,10 @@ def lib():
         for n in ("or_ring_score_tet", "or_ring_score_hex", "or_ring_score_tri2d"):
             getattr(L, n).restype = C.c_float
             getattr(L, n).argtypes = [C.c_int, _fp, _fp, C.c_float, _fp]
+        L.or_fit_local_surface.restype = None
+        L.or_fit_local_surface.argtypes = [_fp, C.c_int, _fp, _fp, _fp]
+        L.or_lu_solve3.restype = None
+        L.or_lu_solve3.argtypes = [_fp, _fp, _fp]
         L.or_mesh_build.restype = C.c_int
         L.or_mesh_build.argtypes = [C.c_int, _fp, C.c_int, _ip, C.c_int, C.POINTER(BuildParams), C.POINTER(C.POINTER(OrMesh))]
         L.or_mesh_free.argtypes = [C.POINTER(OrMesh)]
@@ -277,3 +281,23 @@ class OracleMesh:
         out = np.empty(self.m.nE, dtype=np.float32)
         lib().or_element_qualities(self.ptr, out.ctypes.data_as(_fp))
         return out
+
+
+def fit_local_surface(pts, normal, refp) -> np.ndarray:
+    """Quadric fit of pts (pts[0] = the centre) in the frame of (normal,
+    refp - centre): 16 floats p0, e0, e1, e2, a, b, c, numPoints."""
+    p = np.ascontiguousarray(pts, dtype=np.float32)
+    n = np.ascontiguousarray(normal, dtype=np.float32)
+    r = np.ascontiguousarray(refp, dtype=np.float32)
+    out = np.zeros(16, dtype=np.float32)
+    lib().or_fit_local_surface(p.ctypes.data_as(_fp), p.shape[0], n.ctypes.data_as(_fp), r.ctypes.data_as(_fp),
+                               out.ctypes.data_as(_fp))
+    return out
+
+
+def lu_solve3(A, b) -> np.ndarray:
+    Acm = np.ascontiguousarray(np.asarray(A, dtype=np.float32).T)   # column-major
+    bb = np.ascontiguousarray(b, dtype=np.float32)
+    x = np.zeros(3, dtype=np.float32)
+    lib().or_lu_solve3(Acm.ctypes.data_as(_fp), bb.ctypes.data_as(_fp), x.ctypes.data_as(_fp))
+    return x

@@ -45,7 +45,7 @@ def test_batch_matches_single_meshes():
     ref = []
     for k, xyz, cells in inputs:
         m = FAC[k].create(xyz, cells)
-        o = CLS[k](m, n_iter=iters)
+        o = CLS[k](m, n_iter=iters, fixed_boundary=True)
         o.iterate(iters)
         ref.append((m.positions(original_order=False), o.getQuality()))
         o.close()
@@ -151,8 +151,8 @@ def test_single_mesh_waves_per_vertex(kind, w):
     xyz, cells = gen()
     m1 = FAC[kind].create(xyz, cells)
     m2 = FAC[kind].create(xyz, cells)
-    o1 = CLS[kind](m1, n_iter=3)
-    o2 = CLS[kind](m2, n_iter=3, waves_per_vertex=w)
+    o1 = CLS[kind](m1, n_iter=3, fixed_boundary=True)
+    o2 = CLS[kind](m2, n_iter=3, waves_per_vertex=w, fixed_boundary=True)
     o1.iterate(3)
     o2.iterate(3)
     assert np.array_equal(m1.positions(original_order=False), m2.positions(original_order=False))

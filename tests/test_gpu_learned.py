@@ -61,11 +61,11 @@ def _record(kind, sizes_seeds, iters, cap):
         if kind == "tet":
             xyz, cells = synth.tet_cube(n, seed=seed)
             m = DMOMeshTetFactory.create(xyz, cells)
-            o = DMOTetClass(m, n_iter=iters)
+            o = DMOTetClass(m, n_iter=iters, fixed_boundary=True)
         else:
             xyz, cells = synth.hex_cube(n, seed=seed, jitter=0.2)
             m = DMOMeshHexFactory.create(xyz, cells)
-            o = DMOHexClass(m, n_iter=iters)
+            o = DMOHexClass(m, n_iter=iters, fixed_boundary=True)
         o.record(ds, iters)
         o.close()
         m.close()
@@ -75,7 +75,7 @@ def _record(kind, sizes_seeds, iters, cap):
 def test_dataset_pairs_bit_exact(oracle):
     xyz, tets = synth.tet_cube(8, seed=5)
     m = DMOMeshTetFactory.create(xyz, tets)
-    o = DMOTetClass(m, n_iter=1)
+    o = DMOTetClass(m, n_iter=1, fixed_boundary=True)
     p0 = m.positions(original_order=False)
     ds = DMODataset(100000)
     o.record(ds, 1)
@@ -84,7 +84,7 @@ def test_dataset_pairs_bit_exact(oracle):
     assert len(ds) == c["n_inner"]
     # recording does not change the trajectory
     m2 = DMOMeshTetFactory.create(xyz, tets)
-    o2 = DMOTetClass(m2, n_iter=1)
+    o2 = DMOTetClass(m2, n_iter=1, fixed_boundary=True)
     o2.iterate(1)
     assert np.array_equal(m2.positions(original_order=False), p1)
     # first colour class: features from the start positions, targets from the moves
@@ -127,8 +127,8 @@ def test_fallback_rule_bad_proposal():
     mlp.set_params(p)
     m1 = DMOMeshTetFactory.create(xyz, tets)
     m2 = DMOMeshTetFactory.create(xyz, tets)
-    o1 = DMOTetClass(m1, n_iter=2)
-    o2 = DMOTetClass(m2, n_iter=2)
+    o1 = DMOTetClass(m1, n_iter=2, fixed_boundary=True)
+    o2 = DMOTetClass(m2, n_iter=2, fixed_boundary=True)
     assert o1.getQuality() > 0
     o2.set_proposal(mlp, mode=1, depth=2, window_frac=0.25)
     o1.iterate(2)
@@ -142,7 +142,7 @@ def test_fallback_rule_bad_proposal():
 def test_identity_proposal_monotone():
     xyz, tets = synth.tet_cube(12, seed=13)
     m = DMOMeshTetFactory.create(xyz, tets)
-    o = DMOTetClass(m, n_iter=5)
+    o = DMOTetClass(m, n_iter=5, fixed_boundary=True)
     o.set_proposal(None, mode=2, depth=2, window_frac=0.25)
     q = [o.getQuality()]
     for _ in range(5):
@@ -172,13 +172,13 @@ def test_learned_reaches_plain_quality_and_ablation(kind):
         fac, cls = DMOMeshHexFactory, DMOHexClass
     iters = 10
     mp = fac.create(xyz, cells)
-    op = cls(mp, n_iter=iters)
+    op = cls(mp, n_iter=iters, fixed_boundary=True)
     op.iterate(iters)
     q_plain = op.getQuality()
     res = {}
     for mode in (1, 2):
         m = fac.create(xyz, cells)
-        o = cls(m, n_iter=3 * iters)
+        o = cls(m, n_iter=3 * iters, fixed_boundary=True)
         o.set_proposal(mlp if mode == 1 else None, mode=mode, depth=4, window_frac=0.25)
         q = [o.getQuality()]
         it = 0

@@ -74,7 +74,7 @@ def test_element_quality_bitexact(oracle, kind):
     xyz, cells = gen()
     g = _build(kind, xyz, cells)
     cls = {"tet": DMOTetClass, "hex": DMOHexClass, "tri": DMOTriFlatClass}[kind]
-    opt = cls(g, n_iter=1)
+    opt = cls(g, n_iter=1, fixed_boundary=True)
     o = oracle.OracleMesh(kind, xyz, cells)
     gq = opt.getElementQualities()
     oq = o.element_qualities()
@@ -91,7 +91,7 @@ def test_dmo_positions_parity(oracle, kind, n, iters):
     h = 1.0 / (n - 1 if kind == "tri" else n)
     g = _build(kind, xyz, cells)
     cls = {"tet": DMOTetClass, "hex": DMOHexClass, "tri": DMOTriFlatClass}[kind]
-    opt = cls(g, n_iter=iters)
+    opt = cls(g, n_iter=iters, fixed_boundary=True)
     o = oracle.OracleMesh(kind, xyz, cells)
     for it in range(iters):
         opt.doIteration()
@@ -115,7 +115,7 @@ def test_c1_full_size(oracle):
     g = _build("tet", xyz, tets)
     c = g.counts()
     assert (c["n_vertices"], c["n_elements"], c["n_inner"]) == (68921, 384000, 59319)
-    opt = DMOTetClass(g, n_iter=10)
+    opt = DMOTetClass(g, n_iter=10, fixed_boundary=True)
     o = oracle.OracleMesh("tet", xyz, tets)
     assert np.array_equal(g.color_offsets("inner"), o.offsets("inner"))
     q = [opt.getQuality()]

@@ -193,3 +193,45 @@ def test_flat_boundary_1d_oracle(oracle):
     assert moved.sum() > 0
     corners = (np.abs(b0[:, :2] - np.round(b0[:, :2])) < 1e-9).all(axis=1)
     assert not moved[corners].any()
+
+
+def test_quadric_fit_recovers_paraboloid(oracle):
+    """Points sampled on w = a u^2 + b uv + c v^2 in the fit's own frame give
+    back (a, b, c); a planar patch gives 0; four points or fewer give 0."""
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        p0 = rng.uniform(-1, 1, size=3)
+        ref = p0 + rng.normal(size=3) * 0.1
+        frame = oracle.fit_local_surface(np.vstack([p0] * 6), n, ref)
+        e = frame[3:12].reshape(3, 3).astype(np.float64)
+        assert np.allclose(e @ e.T, np.eye(3), atol=1e-5)
+        assert np.allclose(e[2], n, atol=1e-5)               # e2 is the normal
+        a, b, c = rng.uniform(-2, 2, size=3)
+        uv = rng.uniform(-0.1, 0.1, size=(8, 2))
+        uv[0] = 0.0
+        w = a * uv[:, 0] ** 2 + b * uv[:, 0] * uv[:, 1] + c * uv[:, 1] ** 2
+        pts = p0 + uv[:, :1] * e[0] + uv[:, 1:] * e[1] + w[:, None] * e[2]
+        out = oracle.fit_local_surface(pts, n, ref)
+        assert out[15] == 8
+        got = out[12:15].astype(np.float64)
+        assert np.allclose(got, [a, b, c], rtol=2e-2, atol=2e-2), (got, (a, b, c))
+        flat = p0 + uv[:, :1] * e[0] + uv[:, 1:] * e[1]
+        outf = oracle.fit_local_surface(flat, n, ref)
+        assert np.all(np.abs(outf[12:15]) < 1e-2 * max(1.0, abs(a) + abs(b) + abs(c))), outf[12:15]
+        few = oracle.fit_local_surface(pts[:4], n, ref)
+        assert np.all(few[12:15] == 0.0)
+
+
+def test_lu_solve3_partial_pivoting(oracle):
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        A = rng.normal(size=(3, 3)) + 3 * np.eye(3)
+        b = rng.normal(size=3)
+        x = oracle.lu_solve3(A, b)
+        assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-4, atol=1e-5)
+    # a zero leading pivot needs the row exchange
+    A = np.array([[0.0, 2.0, 1.0], [1.0, 1.0, 0.0], [2.0, 0.0, 3.0]])
+    b = np.array([3.0, 2.0, 5.0])
+    assert np.allclose(oracle.lu_solve3(A, b), np.linalg.solve(A, b), rtol=1e-6)

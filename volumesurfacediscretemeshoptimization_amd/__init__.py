@@ -180,7 +180,7 @@ class _DMOBase:
 
     def __init__(self, mesh: DMOMesh, qualityCriterium: int = MEAN_RATIO, gridScale: float = 0.5,
                  n_iter: int = 30, nq: int = 8, depth: int = 3, shrink: float = 0.0,
-                 extent_mode: int = _lib.VSDMO_EXTENT_AUTO, fixed_boundary: bool = True,
+                 extent_mode: int = _lib.VSDMO_EXTENT_AUTO, fixed_boundary: bool = False,
                  quality_guard: bool = True, guard_tolerance: float = 1e-5, waves_per_vertex: int = 1):
         lib = load()
         self._lib = lib

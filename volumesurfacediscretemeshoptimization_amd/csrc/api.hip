@@ -124,7 +124,7 @@ void vsdmo_params_default(vsdmo_params* p)
     p->depth = 3;
     p->shrink = 0.0f;
     p->extent_mode = VSDMO_EXTENT_AUTO;
-    p->fixed_boundary = 1;
+    p->fixed_boundary = 0;
     p->quality_guard = 1;
     p->guard_tolerance = 1e-5f;
     p->waves_per_vertex = 1;
