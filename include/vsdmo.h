@@ -167,6 +167,15 @@ vsdmo_status vsdmo_opt_destroy(vsdmo_opt* opt);
 /* thread-local message of the last failing call on this thread */
 const char* vsdmo_last_error(void);
 
+/* Replaces DMOBaseClass::getEstimateLocalSurfacePoints (mode 0) and
+ * getLocalSurfacePoints (mode 1, feature_sid) (dmo/DMOBase.h:38-39;
+ * kernels dmo/DMOCommon.h:197-276, dmo/DMOCommon.cu:73-150): nu x nv points
+ * (out: 3*nu*nv floats, host) sampled around surface vertex vid (optimiser
+ * numbering) on its blended projection or on its nearest neighbour's local
+ * fit.  Needs fixed_boundary = 0. */
+vsdmo_status vsdmo_opt_surface_points(vsdmo_opt* o, int32_t vid, int32_t nu, int32_t nv, int32_t mode,
+                                      int32_t feature_sid, float* out);
+
 /* ------------------------------------------------------------------------
  * Learned proposal.  No counterpart in the reference.  A 96-128-128-3 GELU
  * MLP (BF16 weights and activations on MFMA, FP32 accumulation) maps a

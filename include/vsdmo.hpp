@@ -127,6 +127,17 @@ public:
         for (int v : h) std::printf("%d ", v);
         std::printf("\nq_min = %g\n", getQuality());
     }
+    // host buffers of 3*nu*nv floats (the reference fills device buffers)
+    void getEstimateLocalSurfacePoints(int vid, int nu, int nv, void* outSurfacePoints)
+    {
+        vsdmo_cpp::check(vsdmo_opt_surface_points(h_, vid, nu, nv, 0, 0, (float*)outSurfacePoints),
+                         "vsdmo_opt_surface_points");
+    }
+    void getLocalSurfacePoints(int vid, int nu, int nv, void* outSurfacePoints, int featureSid)
+    {
+        vsdmo_cpp::check(vsdmo_opt_surface_points(h_, vid, nu, nv, 1, featureSid, (float*)outSurfacePoints),
+                         "vsdmo_opt_surface_points");
+    }
     vsdmo_opt* handle() const { return h_; }
 
 protected:

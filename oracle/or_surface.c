@@ -673,6 +673,42 @@ long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* 
     return cnt + *nSurf2d + *nFeat1d;
 }
 
+/* Sample points of the local surfaces of surface vertex vid on an nu x nv
+   grid (DMOCommon.h:197-276 k_fillEstimateLocalSurfacePoints with
+   USE_SURF_OF_NN = false, and DMOCommon.cu:73-150 k_fillLocalSurfacePoints):
+   point idx = i*nu + j, i = idx / nu, j = idx % nu.  mode 0: the blended
+   projection of (u, v) in the frame of the NN's fit, window s * maxDist;
+   mode 1: the NN's own fit (or its sector fit `sid` when the NN is a
+   feature vertex), window 2 s * maxDist.  maxDist = largest distance from
+   vid to its surface one-ring. */
+void or_surface_points(const or_mesh* m, const or_surf* s, int vid, int nu, int nv, int mode, int sid, float gridScale,
+                       float* out)
+{
+    w3 pv = W(P(m, vid));
+    float mx = -FLT_MAX;
+    for (int h = m->heRow[vid]; h < m->heRow[vid + 1]; ++h) mx = fmaxf(mx, wnorm(wsub(W(P(m, m->heTarget[h])), pv)));
+    const int nn = s->nn[vid];
+    const lsurf* L = &s->ls[nn];
+    if (mode == 1 && nn >= s->nFree) {
+        int off = nn - s->nFree, nsec = s->frow[off + 1] - s->frow[off];
+        if (sid >= nsec) sid = nsec - 1;
+        if (nsec > 0 && sid >= 0) L = &s->fs[s->frow[off] + sid];
+    }
+    w3 c = wsub(pv, L->p0);
+    float cu = wdot(L->e[0], c), cv = wdot(L->e[1], c);
+    float uMax = gridScale * mx, uMin = -(gridScale * mx);
+    if (mode == 1) { uMax *= 2.0f; uMin *= 2.0f; }
+    const float au = 1.0f / (float)(nu - 1), av = 1.0f / (float)(nv - 1);
+    float vidPos[3] = { pv.x, pv.y, pv.z };
+    for (int idx = 0; idx < nu * nv; ++idx) {
+        int j = idx % nu, i = idx / nu;
+        float u = cu + au * ((float)i * uMin + (float)(nu - 1 - i) * uMax);
+        float v = cv + av * ((float)j * uMin + (float)(nv - 1 - j) * uMax);
+        w3 q = (mode == 0) ? project2d(m, s, m->xyz, vid, vidPos, u, v) : l2w(L, u, v, poly(L, u, v));
+        out[3 * idx] = q.x; out[3 * idx + 1] = q.y; out[3 * idx + 2] = q.z;
+    }
+}
+
 /* test hooks: the local quadric fit of a point set in the frame built from a
    normal and a reference neighbour (computeLocalSurface1/2), and the 3x3 LU
    solve on its own */

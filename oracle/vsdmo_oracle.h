@@ -147,6 +147,10 @@ float or_min_quality(const or_mesh* m);
 void or_quality64(const or_mesh* m, double* qmin, double* qmean);
 void or_element_qualities(const or_mesh* m, float* out);
 
+/* sample points of vid's local surfaces on an nu x nv grid (mode 0: blended
+   projection, mode 1: the NN's fit / sector fit sid); out: 3*nu*nv floats */
+void or_surface_points(const or_mesh* m, const or_surf* s, int vid, int nu, int nv, int mode, int sid, float gridScale,
+                       float* out);
 /* test hooks of the surface fit: quadric of pts[0..np) (pts[0] = p0) in the
    frame of (normal, refp - p0); out16 = p0, e0, e1, e2, a, b, c, np */
 void or_fit_local_surface(const float* pts, int np, const float* normal, const float* refp, float* out16);

@@ -63,6 +63,9 @@ def lib():
         for n in ("or_ring_score_tet", "or_ring_score_hex", "or_ring_score_tri2d"):
             getattr(L, n).restype = C.c_float
             getattr(L, n).argtypes = [C.c_int, _fp, _fp, C.c_float, _fp]
+        L.or_surface_points.restype = None
+        L.or_surface_points.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_float, _fp]
         L.or_fit_local_surface.restype = None
         L.or_fit_local_surface.argtypes = [_fp, C.c_int, _fp, _fp, _fp]
         L.or_lu_solve3.restype = None
@@ -255,6 +258,12 @@ class OracleMesh:
         lib().or_dmo_iteration_surface(self.ptr, self.surf, C.byref(p), C.c_float(self.last_min), C.byref(a), C.byref(b))
         self.last_min = self.min_quality()
         return a.value, b.value
+
+    def surface_points(self, vid: int, nu: int, nv: int, mode: int, sid: int = 0, grid_scale: float = 0.5):
+        out = np.zeros((nu * nv, 3), dtype=np.float32)
+        lib().or_surface_points(self.ptr, self.surf, vid, nu, nv, mode, sid, C.c_float(grid_scale),
+                                out.ctypes.data_as(_fp))
+        return out
 
     def surface_get(self, what: int):
         n = self.m.nSurf

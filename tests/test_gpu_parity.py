@@ -225,3 +225,27 @@ def test_surface_hex_parity_at_size(oracle):
         bad = (g.positions(original_order=False) != o.positions()).any(axis=1)
         assert not bad.any(), (it, int(bad.sum()))
     opt.close()
+
+
+def test_surface_points_parity(oracle):
+    """getEstimateLocalSurfacePoints / getLocalSurfacePoints against the
+    oracle after two iterations (free, feature and corner vertices)."""
+    xyz, tets = synth.tet_cube(8, seed=71, jitter=0.3)
+    g = _build("tet", xyz, tets)
+    opt = DMOTetClass(g, n_iter=2)          # reference defaults: the surface moves
+    o = oracle.OracleMesh("tet", xyz, tets)
+    o.surface_init()
+    for _ in range(2):
+        opt.doIteration()
+        o.iterate_surface()
+    assert np.array_equal(g.positions(original_order=False), o.positions())
+    c = g.counts()
+    for vid in (0, c["n_free"] // 2, c["n_free"], c["n_surface"] - 1):
+        a = opt.getEstimateLocalSurfacePoints(vid, 5, 4)
+        b = o.surface_points(vid, 5, 4, 0)
+        assert np.array_equal(a, b), vid
+        for sid in (0, 1, 7):
+            a = opt.getLocalSurfacePoints(vid, 3, 6, sid)
+            b = o.surface_points(vid, 3, 6, 1, sid)
+            assert np.array_equal(a, b), (vid, sid)
+    opt.close()

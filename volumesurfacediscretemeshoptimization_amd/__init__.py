@@ -254,6 +254,22 @@ class _DMOBase:
         return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
 
 
+    def getEstimateLocalSurfacePoints(self, vid: int, nu: int, nv: int) -> np.ndarray:
+        """nu*nv points of the blended surface projection around surface
+        vertex vid (optimiser numbering)."""
+        out = np.empty((nu * nv, 3), dtype=np.float32)
+        check(self._lib.vsdmo_opt_surface_points(self._h, int(vid), int(nu), int(nv), 0, 0, _fptr(out)),
+              "vsdmo_opt_surface_points")
+        return out
+
+    def getLocalSurfacePoints(self, vid: int, nu: int, nv: int, featureSid: int = 0) -> np.ndarray:
+        """nu*nv points of the local fit of vid's nearest neighbour (its
+        sector fit featureSid for a feature vertex)."""
+        out = np.empty((nu * nv, 3), dtype=np.float32)
+        check(self._lib.vsdmo_opt_surface_points(self._h, int(vid), int(nu), int(nv), 1, int(featureSid), _fptr(out)),
+              "vsdmo_opt_surface_points")
+        return out
+
     def set_proposal(self, mlp=None, mode: int = 1, depth: int = 4, window_frac: float = 0.25):
         """Learned proposal for the inner searches: mode 1 = network, 2 = the
         vertex itself (same windowed search and fallback, no network), 0 = off."""

@@ -669,6 +669,27 @@ vsdmo_status vsdmo_opt_stats(const vsdmo_opt* o, vsdmo_stats* out)
     return VSDMO_OK;
 }
 
+vsdmo_status vsdmo_opt_surface_points(vsdmo_opt* o, int32_t vid, int32_t nu, int32_t nv, int32_t mode,
+                                      int32_t feature_sid, float* out)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!o || !out) return fail(VSDMO_INVALID_ARG, "NULL argument");
+        vsdmo_mesh* m = o->m;
+        if (!o->surface) return fail(VSDMO_NOT_SUPPORTED, "local surfaces exist only with fixed_boundary = 0");
+        if (vid < 0 || vid >= m->nSurf) return fail(VSDMO_INVALID_ARG, "vid is not a surface vertex");
+        if (nu < 2 || nv < 2 || (long long)nu * nv > (1 << 24)) return fail(VSDMO_INVALID_ARG, "nu, nv must be >= 2");
+        if (mode != 0 && mode != 1) return fail(VSDMO_INVALID_ARG, "mode must be 0 or 1");
+        SurfArgs a = surf_args(o);
+        a.sx = m->x.p; a.sy = m->y.p; a.sz = m->z.p;   // the current positions
+        DevBuf<float> d((size_t)3 * nu * nv);
+        surf_points_launch(a, vid, nu, nv, mode, feature_sid, d.p, m->stream);
+        VSDMO_HIP(hipGetLastError());
+        d.download(out, d.n, m->stream);
+        VSDMO_HIP(hipStreamSynchronize(m->stream));
+        return VSDMO_OK;
+    });
+}
+
 vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac)
 {
     return guarded([&]() -> vsdmo_status {

@@ -76,6 +76,7 @@ void surf_nn_launch(const SurfArgs& a, hipStream_t s);
 void surf_search2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s);
 void surf_search1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s);
 void surf_flat1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s);
+void surf_points_launch(const SurfArgs& a, int vid, int nu, int nv, int mode, int sid, float* out, hipStream_t s);
 
 // chunked per-mesh qualities of a batch
 struct BatchQArgs {

@@ -696,6 +696,38 @@ __global__ __launch_bounds__(64) void k_search_flat1d(SurfArgs a, int cStart, in
     if (qold < qlast && lane == 0) { a.xw[vid] = cur.x; a.yw[vid] = cur.y; a.zw[vid] = cur.z; }
 }
 
+// ------------------------------------------------------- surface sampling
+// Points of vid's local surfaces on an nu x nv grid (k_fillEstimateLocalSurfacePoints
+// with USE_SURF_OF_NN = false, DMOCommon.h:197-276; k_fillLocalSurfacePoints,
+// DMOCommon.cu:73-150): idx = i*nu + j.  mode 0 blends the projections
+// (window s * maxDist), mode 1 evaluates the NN's fit or its sector fit sid
+// (window 2 s * maxDist).  One thread per point.
+__global__ void k_surface_points(SurfArgs a, int vid, int nu, int nv, int mode, int sid, float* out)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nu * nv) return;
+    const w3 pv = WP(a.x, a.y, a.z, vid);
+    float mx = -FLT_MAX;
+    for (int h = a.heRow[vid]; h < a.heRow[vid + 1]; ++h) mx = fmaxf(mx, wnorm(wsub(WP(a.x, a.y, a.z, a.heTarget[h]), pv)));
+    const int nn = a.nn[vid];
+    LSurf L = reinterpret_cast<const LSurf*>(a.ls)[nn];
+    if (mode == 1 && nn >= a.nFree) {
+        int off = nn - a.nFree, nsec = a.frow[off + 1] - a.frow[off];
+        if (sid >= nsec) sid = nsec - 1;
+        if (nsec > 0 && sid >= 0) L = reinterpret_cast<const LSurf*>(a.fs)[a.frow[off] + sid];
+    }
+    const w3 c = wsub(pv, L.p0);
+    const float cu = wdot(L.e[0], c), cv = wdot(L.e[1], c);
+    float uMax = a.gridScale * mx, uMin = -(a.gridScale * mx);
+    if (mode == 1) { uMax *= 2.0f; uMin *= 2.0f; }
+    const float au = 1.0f / (float)(nu - 1), av = 1.0f / (float)(nv - 1);
+    const int j = idx % nu, i = idx / nu;
+    const float u = cu + au * ((float)i * uMin + (float)(nu - 1 - i) * uMax);
+    const float v = cv + av * ((float)j * uMin + (float)(nv - 1 - j) * uMax);
+    const w3 q = (mode == 0) ? project2d(a, vid, pv, u, v) : l2w(L, u, v, poly(L, u, v));
+    out[3 * idx] = q.x; out[3 * idx + 1] = q.y; out[3 * idx + 2] = q.z;
+}
+
 // ------------------------------------------------------------- launchers
 static inline int nb256(long long n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
 
@@ -730,6 +762,11 @@ void surf_search2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int t
 void surf_search1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s)
 {
     if (cEnd > cStart) k_search_feat1d<<<cEnd - cStart, 64, lds, s>>>(a, cStart, nq, tet);
+}
+void surf_points_launch(const SurfArgs& a, int vid, int nu, int nv, int mode, int sid, float* out, hipStream_t s)
+{
+    const int n = nu * nv;
+    if (n > 0) k_surface_points<<<(n + 63) / 64, 64, 0, s>>>(a, vid, nu, nv, mode, sid, out);
 }
 void surf_flat1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s)
 {
