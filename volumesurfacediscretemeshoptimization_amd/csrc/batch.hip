@@ -63,7 +63,7 @@ struct vsdmo_batch {
     DevBuf<int> rowTet, ringTet, rowHex, ringHex, heRow, heTarget;
     int maxRingTet = 0, maxRingHex = 0, maxHE = 0;
     // work items per round, in mesh order, and the order used (shuffled)
-    std::vector<int> itemsMeshOrder, roundOff;
+    std::vector<int> itemsMeshOrder, roundOff, roundMid;   // round r: heavy [off, mid), light [mid, off+1)
     DevBuf<int> items, ctr;
     std::vector<int> itemsPerMesh;               // searches per iteration per mesh
     // qualities
@@ -136,12 +136,15 @@ void upload_items(vsdmo_batch* b)
     std::vector<int> it = b->itemsMeshOrder;
     if (b->p.shuffle_seed != 0) {
         uint64_t s = b->p.shuffle_seed;
-        for (size_t r = 0; r + 1 < b->roundOff.size(); ++r) {
-            int lo = b->roundOff[r], hi = b->roundOff[r + 1];
+        auto shuffle = [&](int lo, int hi) {
             for (int i = hi - 1; i > lo; --i) {
                 int j = lo + (int)(splitmix(s) % (uint64_t)(i - lo + 1));
                 std::swap(it[i], it[j]);
             }
+        };
+        for (size_t r = 0; r + 1 < b->roundOff.size(); ++r) {
+            shuffle(b->roundOff[r], b->roundMid[r]);
+            shuffle(b->roundMid[r], b->roundOff[r + 1]);
         }
     }
     b->items.upload(it.data(), it.size(), b->stream);
@@ -168,7 +171,7 @@ void record_iteration(vsdmo_batch* b)
     SearchArgs a = search_args(b);
     for (int r = 0; r < nR; ++r)
         launch_batch_round(b->p.nq, b->p.waves_per_vertex, a, b->rowHex.p, b->ringHex.p, b->items.p, b->roundOff[r],
-                           b->roundOff[r + 1], b->ctr.p + r, b->grid[r], b->lds, s);
+                           b->roundMid[r], b->roundOff[r + 1], b->ctr.p + r, b->grid[r], b->lds, s);
     launch_batch_quality(quality_args(b), s);
 }
 
@@ -431,9 +434,11 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
                 nRounds = std::max(nRounds, (int)co.size() - 1);
             }
             b->roundOff.assign(nRounds + 1, 0);
+            b->roundMid.assign(nRounds, 0);
             b->itemsPerMesh.assign(nm, 0);
             for (int r = 0; r < nRounds; ++r) {
-                for (int pass = 0; pass < 2; ++pass)
+                for (int pass = 0; pass < 2; ++pass) {
+                    if (pass == 1) b->roundMid[r] = (int)b->itemsMeshOrder.size();
                     for (int i = 0; i < nm; ++i) {
                         vsdmo_mesh* m = b->meshes[i];
                         const bool tri = (m->elem == VSDMO_TRI);
@@ -445,6 +450,7 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
                             b->itemsMeshOrder.push_back((b->vOff[i] + v) | (type << 30));
                         b->itemsPerMesh[i] += co[r + 1] - co[r];
                     }
+                }
                 b->roundOff[r + 1] = (int)b->itemsMeshOrder.size();
             }
             b->items.alloc(std::max<size_t>(1, b->itemsMeshOrder.size()));

@@ -101,7 +101,7 @@ size_t search_lds_bytes(int kind, int maxRing);
 bool waves_supported(int nq, int w);
 void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hipStream_t s);
 void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
-                        int lo, int hi, int* ctr, int grid, size_t lds, hipStream_t s);
+                        int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s);
 int batch_round_blocks_per_cu(int nq, int w, size_t lds);
 void launch_fill_int(int* p, int n, int v, hipStream_t s);
 // learned-proposal search of one colour class [a.cStart, a.cEnd) (tet / hex);

@@ -748,10 +748,29 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y
 // (0 tet, 1 hex, 2 flat tri).  Vertices of one round belong to one colour
 // class of their mesh, so the order in which they are taken does not change
 // any result.
+// Items [lo, mid) are tet / hex searches, taken one at a time from the
+// round's counter (their cost varies with the ring); items [mid, hi) are flat
+// triangle searches, ~100x cheaper and uniform, statically strided over the
+// blocks so that they cost no atomics.
+template <int NQ, int W>
+__device__ __forceinline__ void batch_item(const SearchArgs& a, const int* rowHex, const int* valHex, int code,
+                                           float4* s_dyn, RedLds<W>& rd)
+{
+    const int vid = code & 0x3fffffff, type = (code >> 30) & 3;
+    if (type == 0) tet_vertex<NQ, W>(a, vid, false, reinterpret_cast<TetC*>(s_dyn), rd);
+    else if (type == 1) {
+        SearchArgs ah = a;
+        ah.ringRow = rowHex;
+        ah.ringVal = valHex;
+        hex_vertex<NQ, W>(ah, vid, true, reinterpret_cast<HexLds*>(s_dyn), rd);
+    }
+    else flat_vertex<NQ, W>(a, vid, reinterpret_cast<TriC*>(s_dyn), rd);
+}
+
 template <int NQ, int W>
 __global__ __launch_bounds__(64 * W) void k_batch_round(SearchArgs a, const int* __restrict__ rowHex,
                                                        const int* __restrict__ valHex, const int* __restrict__ items,
-                                                       int lo, int hi, int* __restrict__ ctr)
+                                                       int lo, int mid, int hi, int* __restrict__ ctr)
 {
     extern __shared__ float4 s_dyn[];
     __shared__ RedLds<W> rd;
@@ -761,17 +780,12 @@ __global__ __launch_bounds__(64 * W) void k_batch_round(SearchArgs a, const int*
         if (threadIdx.x == 0) s_item = atomicAdd(ctr, 1);
         __syncthreads();
         const int it = lo + s_item;
-        if (it >= hi) break;
-        const int code = items[it];
-        const int vid = code & 0x3fffffff, type = (code >> 30) & 3;
-        if (type == 0) tet_vertex<NQ, W>(a, vid, false, reinterpret_cast<TetC*>(s_dyn), rd);
-        else if (type == 1) {
-            SearchArgs ah = a;
-            ah.ringRow = rowHex;
-            ah.ringVal = valHex;
-            hex_vertex<NQ, W>(ah, vid, true, reinterpret_cast<HexLds*>(s_dyn), rd);
-        }
-        else flat_vertex<NQ, W>(a, vid, reinterpret_cast<TriC*>(s_dyn), rd);
+        if (it >= mid) break;
+        batch_item<NQ, W>(a, rowHex, valHex, items[it], s_dyn, rd);
+    }
+    for (int it = mid + (int)blockIdx.x; it < hi; it += (int)gridDim.x) {
+        __syncthreads();
+        batch_item<NQ, W>(a, rowHex, valHex, items[it], s_dyn, rd);
     }
 }
 
@@ -977,7 +991,7 @@ struct BatchLaunch {
     const int* rowHex;
     const int* valHex;
     const int* items;
-    int lo, hi;
+    int lo, mid, hi;
     int* ctr;
     int grid;
     size_t lds;
@@ -986,7 +1000,7 @@ struct BatchLaunch {
 template <int NQ, int W>
 static void batch_nqw(const BatchLaunch& L)
 {
-    k_batch_round<NQ, W><<<L.grid, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo, L.hi, L.ctr);
+    k_batch_round<NQ, W><<<L.grid, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo, L.mid, L.hi, L.ctr);
 }
 template <int NQ, int W>
 static int batch_occ(size_t lds)
@@ -1023,10 +1037,10 @@ int batch_round_blocks_per_cu(int nq, int w, size_t lds)
 }
 
 void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
-                        int lo, int hi, int* ctr, int grid, size_t lds, hipStream_t s)
+                        int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s)
 {
     if (hi <= lo || grid <= 0) return;
-    BatchLaunch L{ a, rowHex, valHex, items, lo, hi, ctr, grid, lds, s };
+    BatchLaunch L{ a, rowHex, valHex, items, lo, mid, hi, ctr, grid, lds, s };
     switch (nq) {
     case 2: batch_nq<2>(w, L); break;
     case 4: batch_nq<4>(w, L); break;
