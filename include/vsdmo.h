@@ -167,6 +167,25 @@ vsdmo_status vsdmo_opt_destroy(vsdmo_opt* opt);
 /* thread-local message of the last failing call on this thread */
 const char* vsdmo_last_error(void);
 
+/* Mesh files.  Replace the file overloads of DMOMesh{Tri,Tet,Hex}Factory::create
+ * (mesh/MeshFactory.cu:3-111) with the reference readers io/ParserOFF.cu:15-83
+ * (flat triangles, .off), io/ParserTetGen.cu:13-120 (<path>.node + <path>.ele,
+ * 1-based numbering detected from the first point index) and
+ * io/ParserHex.cu:115-240 (MEDIT .mesh hexahedra, file bottom face = internal
+ * top face), and the writers io/FileWriter.cu:9-147 (9 significant digits, so a
+ * written file reads back bit-identically).  vsdmo_mesh_file_read with NULL
+ * buffers returns the sizes only.  The reference returns nullptr on a parse
+ * failure; these calls return VSDMO_INVALID_ARG. */
+vsdmo_status vsdmo_mesh_file_read(vsdmo_elem type, const char* path, int32_t* nV, int32_t* nCells, float* xyz,
+                                  int32_t* cells);
+vsdmo_status vsdmo_mesh_file_write(vsdmo_elem type, const char* path, const float* xyz, int32_t nV,
+                                   const int32_t* cells, int32_t nCells);
+vsdmo_status vsdmo_mesh_create_from_file(vsdmo_elem type, const char* path, const vsdmo_mesh_params* params,
+                                         vsdmo_mesh** out);
+/* current positions and elements, in the input numbering (original_order = 1)
+ * or the optimiser numbering (0, as the reference's writers) */
+vsdmo_status vsdmo_mesh_write(vsdmo_mesh* m, const char* path, int32_t original_order);
+
 /* Replaces DMOBaseClass::getEstimateLocalSurfacePoints (mode 0) and
  * getLocalSurfacePoints (mode 1, feature_sid) (dmo/DMOBase.h:38-39;
  * kernels dmo/DMOCommon.h:197-276, dmo/DMOCommon.cu:73-150): nu x nv points

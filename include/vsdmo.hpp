@@ -6,8 +6,7 @@
 //       optimize(), doIteration(), isDone(), getQuality(),
 //       getQualityHistogram(), getElementQualities(), displayQualityGPU()
 //       (reference VSDMO/src/dmo/DMOBase.h:23-88, DMOTet.h:24)
-// The file-name overloads of the reference factories read OFF / TetGen / MEDIT
-// files; this build takes host arrays (file I/O is not part of it).
+// The file-name overloads read OFF / TetGen / MEDIT files (vsdmo_mesh_file_read).
 // Errors become std::runtime_error carrying vsdmo_last_error().
 #pragma once
 
@@ -61,16 +60,34 @@ using DMOMeshTri = DMOMesh;
 using DMOMeshTet = DMOMesh;
 using DMOMeshHex = DMOMesh;
 
+// the file overloads return nullptr when the file cannot be read, like the
+// reference factories
+inline std::shared_ptr<DMOMesh> create_from_file(vsdmo_elem type, int npe, const std::string& path)
+{
+    int32_t nV = 0, nC = 0;
+    if (vsdmo_mesh_file_read(type, path.c_str(), &nV, &nC, nullptr, nullptr) != VSDMO_OK) return nullptr;
+    std::vector<float> xyz(3 * (size_t)nV);
+    std::vector<int32_t> cells((size_t)npe * nC);
+    if (vsdmo_mesh_file_read(type, path.c_str(), &nV, &nC, xyz.data(), cells.data()) != VSDMO_OK) return nullptr;
+    return std::make_shared<DMOMesh>(type, xyz, cells, npe);
+}
+
 struct DMOMeshTriFactory {
     static std::shared_ptr<DMOMeshTri> create(const std::vector<float>& xyz, const std::vector<int32_t>& tris)
     {
         return std::make_shared<DMOMeshTri>(VSDMO_TRI, xyz, tris, 3);
     }
+    static std::shared_ptr<DMOMeshTri> create(const std::string& offFile) { return create_from_file(VSDMO_TRI, 3, offFile); }
 };
 struct DMOMeshTetFactory {
     static std::shared_ptr<DMOMeshTet> create(const std::vector<float>& xyz, const std::vector<int32_t>& tets)
     {
         return std::make_shared<DMOMeshTet>(VSDMO_TET, xyz, tets, 4);
+    }
+    // TetGen base name: <name>.node and <name>.ele
+    static std::shared_ptr<DMOMeshTet> create(const std::string& tetgenBase)
+    {
+        return create_from_file(VSDMO_TET, 4, tetgenBase);
     }
 };
 struct DMOMeshHexFactory {
@@ -78,6 +95,7 @@ struct DMOMeshHexFactory {
     {
         return std::make_shared<DMOMeshHex>(VSDMO_HEX, xyz, hexes, 8);
     }
+    static std::shared_ptr<DMOMeshHex> create(const std::string& meditFile) { return create_from_file(VSDMO_HEX, 8, meditFile); }
 };
 
 }  // namespace vsdmo_cpp

@@ -249,3 +249,23 @@ def test_surface_points_parity(oracle):
             b = o.surface_points(vid, 3, 6, 1, sid)
             assert np.array_equal(a, b), (vid, sid)
     opt.close()
+
+
+def test_mesh_files_through_the_factories(tmp_path):
+    """create(path) builds the same mesh as create(xyz, cells); write() after
+    optimisation reads back as the optimised positions."""
+    xyz, tets = synth.tet_cube(6, seed=81)
+    base = str(tmp_path / "c")
+    from volumesurfacediscretemeshoptimization_amd import read_mesh_file, write_mesh_file, _lib
+    write_mesh_file(_lib.VSDMO_TET, base, xyz, tets)
+    a = DMOMeshTetFactory.create(base)
+    b = _build("tet", xyz, tets)
+    assert a.counts() == b.counts()
+    assert np.array_equal(a.positions(), b.positions()) and np.array_equal(a.permutation(), b.permutation())
+    assert DMOMeshTetFactory.create(str(tmp_path / "missing")) is None
+    opt = DMOTetClass(a, n_iter=2)
+    opt.optimize()
+    a.write(str(tmp_path / "out"))
+    x2, t2 = read_mesh_file(_lib.VSDMO_TET, str(tmp_path / "out"))
+    assert np.array_equal(x2, a.positions()) and np.array_equal(t2, tets)
+    opt.close()

@@ -26,7 +26,7 @@ from ._lib import VsdmoError, check, load  # noqa: F401
 
 MEAN_RATIO = 0
 
-__all__ = ["DMOMeshTriFactory", "DMOMeshTetFactory", "DMOMeshHexFactory", "DMOMesh",
+__all__ = ["read_mesh_file", "write_mesh_file", "DMOMeshTriFactory", "DMOMeshTetFactory", "DMOMeshHexFactory", "DMOMesh",
            "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "DMOBatch", "DMOMLP", "DMODataset", "VsdmoError",
            "MEAN_RATIO", "load"]
 
@@ -76,6 +76,10 @@ class DMOMesh:
             self.close()
         except Exception:
             pass
+
+    def write(self, path: str, original_order: bool = True):
+        """Writes the current mesh (OFF / TetGen / MEDIT by element type)."""
+        check(self._lib.vsdmo_mesh_write(self._h, path.encode(), int(original_order)), "vsdmo_mesh_write")
 
     def counts(self) -> dict:
         c = _lib.Counts()
@@ -157,22 +161,55 @@ class DMOMesh:
         return b[: c["n_surface"]], f[: c["n_surface"]]
 
 
-class DMOMeshTriFactory:
-    @staticmethod
-    def create(xyz, tris, **kw) -> DMOMesh:
-        return DMOMesh(_lib.VSDMO_TRI, xyz, tris, **kw)
+def read_mesh_file(elem: int, path: str):
+    """(xyz, cells) of a mesh file: OFF (flat triangles), TetGen base path
+    (tets, <path>.node / <path>.ele) or MEDIT .mesh (hexes, internal order)."""
+    lib = load()
+    nv, nc = C.c_int32(), C.c_int32()
+    check(lib.vsdmo_mesh_file_read(elem, path.encode(), C.byref(nv), C.byref(nc), None, None), "vsdmo_mesh_file_read")
+    xyz = np.empty((nv.value, 3), dtype=np.float32)
+    cells = np.empty((nc.value, DMOMesh.NPE[elem]), dtype=np.int32)
+    check(lib.vsdmo_mesh_file_read(elem, path.encode(), C.byref(nv), C.byref(nc), _fptr(xyz), _iptr(cells)),
+          "vsdmo_mesh_file_read")
+    return xyz, cells
 
 
-class DMOMeshTetFactory:
-    @staticmethod
-    def create(xyz, tets, **kw) -> DMOMesh:
-        return DMOMesh(_lib.VSDMO_TET, xyz, tets, **kw)
+def write_mesh_file(elem: int, path: str, xyz, cells):
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    cells = np.ascontiguousarray(cells, dtype=np.int32)
+    check(lib.vsdmo_mesh_file_write(elem, path.encode(), _fptr(xyz), xyz.shape[0], _iptr(cells), cells.shape[0]),
+          "vsdmo_mesh_file_write")
 
 
-class DMOMeshHexFactory:
-    @staticmethod
-    def create(xyz, hexes, **kw) -> DMOMesh:
-        return DMOMesh(_lib.VSDMO_HEX, xyz, hexes, **kw)
+class _Factory:
+    ELEM = None
+
+    @classmethod
+    def create(cls, xyz_or_path, cells=None, **kw):
+        """create(xyz, cells) from host arrays, or create(path) from a mesh
+        file (reference DMOMesh*Factory::create(std::string); None when the
+        file cannot be read, like the reference's nullptr)."""
+        if isinstance(xyz_or_path, (str, bytes)):
+            path = xyz_or_path.decode() if isinstance(xyz_or_path, bytes) else xyz_or_path
+            try:
+                xyz, cells = read_mesh_file(cls.ELEM, path)
+            except VsdmoError:
+                return None
+            return DMOMesh(cls.ELEM, xyz, cells, **kw)
+        return DMOMesh(cls.ELEM, xyz_or_path, cells, **kw)
+
+
+class DMOMeshTriFactory(_Factory):
+    ELEM = _lib.VSDMO_TRI
+
+
+class DMOMeshTetFactory(_Factory):
+    ELEM = _lib.VSDMO_TET
+
+
+class DMOMeshHexFactory(_Factory):
+    ELEM = _lib.VSDMO_HEX
 
 
 class _DMOBase:

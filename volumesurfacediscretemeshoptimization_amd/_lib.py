@@ -89,6 +89,10 @@ EXPORTS = {
     "vsdmo_opt_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsdmo_opt_destroy": (C.c_int, [C.c_void_p]),
     "vsdmo_last_error": (C.c_char_p, []),
+    "vsdmo_mesh_file_read": (C.c_int, [C.c_int, C.c_char_p, _ip, _ip, _fp, _ip]),
+    "vsdmo_mesh_file_write": (C.c_int, [C.c_int, C.c_char_p, _fp, C.c_int32, _ip, C.c_int32]),
+    "vsdmo_mesh_create_from_file": (C.c_int, [C.c_int, C.c_char_p, C.POINTER(MeshParams), C.POINTER(C.c_void_p)]),
+    "vsdmo_mesh_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "vsdmo_mlp_num_params": (C.c_int32, []),
     "vsdmo_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "vsdmo_mlp_destroy": (C.c_int, [C.c_void_p]),

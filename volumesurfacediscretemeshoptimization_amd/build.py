@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvsdmo.so")
-SOURCES = ["api.hip", "mesh_build.hip", "dmo_kernels.hip", "surface.hip", "batch.hip", "mlp.hip"]
+SOURCES = ["api.hip", "mesh_build.hip", "dmo_kernels.hip", "surface.hip", "batch.hip", "mlp.hip", "mesh_io.cpp"]
 
 # -ffp-contract=off: no implicit FMA contraction (the kernels place fmaf by hand)
 # -fno-gpu-flush-denormals-to-zero: IEEE denormals, as on the host
