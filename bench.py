@@ -131,7 +131,7 @@ def bench_batch(args):
                                "persistent launch per colour round, iteration replayed as a hipGraph, autotuned",
                    "name": "batch256", "meshes": st["meshes"], "total_vertices": st["total_vertices"],
                    "rounds": st["rounds"], "vertex_updates_per_step": st["items_per_iteration"],
-                   "autotuned": {k: chosen[k] for k in ("nq", "depth", "waves_per_vertex")},
+                   "autotuned": {k: chosen[k] for k in ("nq", "depth", "waves_per_vertex", "persistent")},
                    "mesh_build_s": build_s, "min_quality_start_worst": float(q0.min()),
                    "min_quality_end_worst": float(q1.min()),
                    "unbatched_value": plain_updates / dt_plain, "unbatched_ms_per_step": 1e3 * dt_plain / args.steps,

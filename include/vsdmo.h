@@ -258,6 +258,8 @@ typedef struct vsdmo_batch_params {
     uint32_t shuffle_seed;     /* 0: work items in mesh order; else a seeded permutation inside each round */
     int32_t quality_guard;     /* 1: fail with VSDMO_QUALITY_DECREASED when a mesh's min quality drops */
     float guard_tolerance;
+    int32_t persistent;        /* 1: persistent blocks take a round's items from a queue; 0: one block per
+                                  item; does not change results */
 } vsdmo_batch_params;
 
 typedef struct vsdmo_batch_stats {
@@ -284,9 +286,9 @@ vsdmo_status vsdmo_batch_writeback(vsdmo_batch* b);
 vsdmo_status vsdmo_batch_restore(vsdmo_batch* b);
 vsdmo_status vsdmo_batch_get_params(vsdmo_batch* b, vsdmo_batch_params* p);
 vsdmo_status vsdmo_batch_set_params(vsdmo_batch* b, const vsdmo_batch_params* p);
-/* Autotuner over (nq, depth, waves_per_vertex): every trial starts from the
- * current positions and runs `iters` iterations; waves_per_vertex is chosen
- * by time at (8, 3), then (nq, depth) candidates are accepted when every
+/* Autotuner over (nq, depth, waves_per_vertex, persistent): every trial starts
+ * from the current positions and runs `iters` iterations; the schedule
+ * (waves_per_vertex, persistent) is chosen by time at (8, 3), then (nq, depth) candidates are accepted when every
  * mesh's final min quality is >= the (8, 3) run's - tol; the fastest
  * accepted configuration is set and returned, the positions are left as
  * they were, and each trial is written to log_path as one JSON line. */

@@ -69,7 +69,8 @@ def test_batch_matches_single_meshes():
 
 
 @pytest.mark.parametrize("kw", [{"shuffle_seed": 12345}, {"waves_per_vertex": 2}, {"waves_per_vertex": 4},
-                                {"waves_per_vertex": 2, "shuffle_seed": 99, "use_graph": False}])
+                                {"waves_per_vertex": 2, "shuffle_seed": 99, "use_graph": False},
+                                {"persistent": False}, {"persistent": False, "waves_per_vertex": 2}])
 def test_batch_schedule_independence(kw):
     inputs = _inputs()
     base = DMOBatch(_meshes(inputs))
@@ -136,7 +137,7 @@ def test_autotune_keeps_quality(tmp_path):
     lines = [json.loads(x) for x in log.read_text().splitlines()]
     assert lines[-1]["stage"] == "chosen"
     assert lines[-1]["nq"] == chosen["nq"] and lines[-1]["depth"] == chosen["depth"]
-    assert sum(1 for x in lines if x["stage"] == "waves") == 3
+    assert sum(1 for x in lines if x["stage"] == "schedule") == 6
     assert b.stats()["iterations"] == 0          # the tuner leaves no trace
     b.iterate(iters)
     q = b.min_quality()

@@ -435,7 +435,7 @@ class DMOBatch:
 
     def __init__(self, meshes, nq: int = 8, depth: int = 3, waves_per_vertex: int = 1, gridScale: float = 0.5,
                  use_graph: bool = True, shuffle_seed: int = 0, quality_guard: bool = True,
-                 guard_tolerance: float = 1e-5):
+                 guard_tolerance: float = 1e-5, persistent: bool = True):
         lib = load()
         self._lib = lib
         self.meshes = list(meshes)
@@ -444,6 +444,7 @@ class DMOBatch:
         p.nq, p.depth, p.waves_per_vertex, p.grid_scale = nq, depth, waves_per_vertex, gridScale
         p.use_graph, p.shuffle_seed = int(use_graph), int(shuffle_seed)
         p.quality_guard, p.guard_tolerance = int(quality_guard), guard_tolerance
+        p.persistent = int(persistent)
         arr = (C.c_void_p * len(self.meshes))(*[m.handle.value for m in self.meshes])
         h = C.c_void_p()
         check(lib.vsdmo_batch_create(arr, len(self.meshes), C.byref(p), C.byref(h)), "vsdmo_batch_create")

@@ -50,7 +50,7 @@ class Stats(C.Structure):
 class BatchParams(C.Structure):
     _fields_ = [("nq", C.c_int32), ("depth", C.c_int32), ("waves_per_vertex", C.c_int32), ("grid_scale", C.c_float),
                 ("use_graph", C.c_int32), ("shuffle_seed", C.c_uint32), ("quality_guard", C.c_int32),
-                ("guard_tolerance", C.c_float)]
+                ("guard_tolerance", C.c_float), ("persistent", C.c_int32)]
 
 
 class BatchStats(C.Structure):

@@ -158,7 +158,7 @@ void size_grids(vsdmo_batch* b)
     b->grid.assign(b->roundOff.size() > 0 ? b->roundOff.size() - 1 : 0, 0);
     for (size_t r = 0; r + 1 < b->roundOff.size(); ++r) {
         int n = b->roundOff[r + 1] - b->roundOff[r];
-        b->grid[r] = std::min(n, b->cus * per);
+        b->grid[r] = b->p.persistent ? std::min(n, b->cus * per) : 0;
     }
 }
 
@@ -327,6 +327,7 @@ void vsdmo_batch_params_default(vsdmo_batch_params* p)
     p->waves_per_vertex = 1;
     p->grid_scale = 0.5f;
     p->use_graph = 1;
+    p->persistent = 1;
     p->shuffle_seed = 0;
     p->quality_guard = 1;
     p->guard_tolerance = 1e-5f;
@@ -644,9 +645,10 @@ vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, cons
         auto log_trial = [&](const Trial& t, const char* stage, double dev, bool accepted) {
             if (!log) return;
             std::fprintf(log,
-                         "{\"stage\": \"%s\", \"nq\": %d, \"depth\": %d, \"waves_per_vertex\": %d, \"ms_per_iter\": %.6f, "
-                         "\"iters\": %d, \"min_quality_worst\": %.9g, \"max_drop_vs_ref\": %.9g, \"accepted\": %s}\n",
-                         stage, t.p.nq, t.p.depth, t.p.waves_per_vertex, t.ms, iters,
+                         "{\"stage\": \"%s\", \"nq\": %d, \"depth\": %d, \"waves_per_vertex\": %d, \"persistent\": %d, "
+                         "\"ms_per_iter\": %.6f, \"iters\": %d, \"min_quality_worst\": %.9g, \"max_drop_vs_ref\": %.9g, "
+                         "\"accepted\": %s}\n",
+                         stage, t.p.nq, t.p.depth, t.p.waves_per_vertex, t.p.persistent, t.ms, iters,
                          (double)*std::min_element(t.q.begin(), t.q.end()), dev, accepted ? "true" : "false");
         };
         // reference configuration (8, 3): waves per vertex by time
@@ -655,17 +657,20 @@ vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, cons
         Trial best{};
         std::vector<float> qref;
         int bestW = 1;
-        for (int w : { 1, 2, 4 }) {
-            vsdmo_batch_params p = ref;
-            p.waves_per_vertex = w;
-            (void)trial(p);            // warm-up of this shape
-            Trial t = trial(p);
-            if (qref.empty()) qref = t.q;
-            double dev = 0.0;
-            for (size_t m = 0; m < nm; ++m) dev = std::max(dev, (double)qref[m] - t.q[m]);
-            log_trial(t, "waves", dev, true);
-            if (best.q.empty() || t.ms < best.ms) { best = t; bestW = w; }
-        }
+        int bestP = 1;
+        for (int pers : { 1, 0 })
+            for (int w : { 1, 2, 4 }) {
+                vsdmo_batch_params p = ref;
+                p.waves_per_vertex = w;
+                p.persistent = pers;
+                (void)trial(p);            // warm-up of this shape
+                Trial t = trial(p);
+                if (qref.empty()) qref = t.q;
+                double dev = 0.0;
+                for (size_t m = 0; m < nm; ++m) dev = std::max(dev, (double)qref[m] - t.q[m]);
+                log_trial(t, "schedule", dev, true);
+                if (best.q.empty() || t.ms < best.ms) { best = t; bestW = w; bestP = pers; }
+            }
         // grid resolution and depth
         const int cand[][2] = { { 8, 2 }, { 6, 3 }, { 6, 2 }, { 4, 4 }, { 4, 3 }, { 6, 4 } };
         for (const auto& c : cand) {
@@ -673,6 +678,7 @@ vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, cons
             p.nq = c[0];
             p.depth = c[1];
             p.waves_per_vertex = waves_supported(p.nq, bestW) ? bestW : 1;
+            p.persistent = bestP;
             (void)trial(p);
             Trial t = trial(p);
             double dev = 0.0;
@@ -686,8 +692,8 @@ vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, cons
         }
         if (log) {
             std::fprintf(log, "{\"stage\": \"chosen\", \"nq\": %d, \"depth\": %d, \"waves_per_vertex\": %d, "
-                              "\"ms_per_iter\": %.6f, \"tol\": %g}\n",
-                         best.p.nq, best.p.depth, best.p.waves_per_vertex, best.ms, (double)tol);
+                              "\"persistent\": %d, \"ms_per_iter\": %.6f, \"tol\": %g}\n",
+                         best.p.nq, best.p.depth, best.p.waves_per_vertex, best.p.persistent, best.ms, (double)tol);
             std::fclose(log);
         }
         // back to the start state with the chosen parameters

@@ -100,6 +100,9 @@ struct BatchQArgs {
 size_t search_lds_bytes(int kind, int maxRing);
 bool waves_supported(int nq, int w);
 void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hipStream_t s);
+// one colour round of a batch: persistent (grid > 0 blocks, tet/hex items
+// [lo, mid) from the atomic counter *ctr, flat items [mid, hi) strided) or,
+// with grid <= 0, one block per item
 void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
                         int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s);
 int batch_round_blocks_per_cu(int nq, int w, size_t lds);

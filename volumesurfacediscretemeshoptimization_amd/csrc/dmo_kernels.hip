@@ -986,6 +986,18 @@ static void launch_nq(int kind, int w, const SearchArgs& a, int nBlocks, size_t 
     else if (w == 4 && NQ % 4 == 0) launch_nqw<NQ, (NQ % 4 == 0 ? 4 : 1)>(kind, a, nBlocks, lds, s);
     else launch_nqw<NQ, 1>(kind, a, nBlocks, lds, s);
 }
+// Non-persistent variant of a round: one block per item, scheduled by the
+// hardware dispatcher (no queue, no atomics).
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_batch_direct(SearchArgs a, const int* __restrict__ rowHex,
+                                                        const int* __restrict__ valHex, const int* __restrict__ items,
+                                                        int lo)
+{
+    extern __shared__ float4 s_dyn[];
+    __shared__ RedLds<W> rd;
+    batch_item<NQ, W>(a, rowHex, valHex, items[lo + blockIdx.x], s_dyn, rd);
+}
+
 struct BatchLaunch {
     SearchArgs a;
     const int* rowHex;
@@ -1000,7 +1012,10 @@ struct BatchLaunch {
 template <int NQ, int W>
 static void batch_nqw(const BatchLaunch& L)
 {
-    k_batch_round<NQ, W><<<L.grid, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo, L.mid, L.hi, L.ctr);
+    if (L.grid > 0)
+        k_batch_round<NQ, W><<<L.grid, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo, L.mid, L.hi, L.ctr);
+    else
+        k_batch_direct<NQ, W><<<L.hi - L.lo, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo);
 }
 template <int NQ, int W>
 static int batch_occ(size_t lds)
@@ -1039,7 +1054,7 @@ int batch_round_blocks_per_cu(int nq, int w, size_t lds)
 void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
                         int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s)
 {
-    if (hi <= lo || grid <= 0) return;
+    if (hi <= lo) return;   // grid <= 0: one block per item
     BatchLaunch L{ a, rowHex, valHex, items, lo, mid, hi, ctr, grid, lds, s };
     switch (nq) {
     case 2: batch_nq<2>(w, L); break;
