@@ -36,14 +36,14 @@ PEAK_FP32_TFLOPS = 157.3
 # x levels + the final check = 1,540 (3D) / 196 (2D), times the one-ring size
 # (24 tets / 8 hexes / 6 triangles) times the per-element FLOP of the metric
 # (tet 73, hex 407, tri 23), cube roots counted separately.
-# HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE x 2.0 copy-kernel
+# HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE x 1.99981 copy-kernel
 # calibration + WRITE_SIZE x 1.0, mean over the dispatches of 3 iterations,
 # profiles/k_search_tet_tet40.md and profiles/k_search_hex_hex48.md).
 WORKLOADS = {
     "tet40": {"kind": "tet", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_search_tet<8>",
-              "traffic": 10_966_938},
+              "traffic": 10_970_146},
     "hex48": {"kind": "hex", "flop": 5_014_240, "cbrt": 110_880, "kernel": "k_search_hex<8>",
-              "traffic": None},
+              "traffic": 17_570_841},
     "tri64": {"kind": "tri", "flop": 27_048, "cbrt": 0, "kernel": "k_search_flat<8>", "traffic": None},
     "batch256": {"kind": "batch", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_batch_round", "traffic": None},
 }
@@ -193,14 +193,46 @@ def cpu_baseline(kind, xyz, cells, iters: int, threads: int, surface: bool, name
                       f"by the C/OpenMP oracle with {threads} threads"}
 
 
+def learned_comparison(args):
+    """Learned proposal vs plain DMO on the bench mesh; returns a dict (see
+    bench_learned).  Never raises: an error is reported in the dict."""
+    try:
+        return _learned(args)
+    except Exception as e:          # the plain line must not depend on it
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def bench_learned(args):
     """Learned proposal vs plain DMO on the same mesh.  Untimed setup: training
     pairs recorded from plain DMO on seeded meshes of the same family (seeds
     1000+), then Adam training of the MLP.  Timed: the plain run of K
-    iterations, then the learned run until its minimum quality reaches the
-    plain run's final one (-1e-3).  value = plain-equivalent vertex updates
-    per second at equal final minimum quality (K x inner vertices / learned
+    iterations, then the learned run until its minimum and mean quality reach
+    the plain run's final ones (-1e-3).  value = plain-equivalent vertex
+    updates per second at equal final quality (K x inner vertices / learned
     wall time)."""
+    r = _learned(args)
+    xyz, cells, desc, kw = make_mesh(args.workload, args.seed)
+    wl = WORKLOADS[args.workload]
+    kind = wl["kind"]
+    out = {
+        "metric": METRIC, "value": r["value"], "unit": "vertex updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["learned_ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 search, bf16 MLP", "data": "synthetic",
+        "config": dict(workload=desc + "; learned proposal (4x4x4 window 1/4 of the plain one, "
+                       f"{args.learn_depth} levels, fallback to the full search) vs plain", name=args.workload,
+                       mode="learned", **{k: v for k, v in r.items() if k not in ("value", "roofline")}),
+        "roofline": r["roofline"],
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"],
+                                           args.workload)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def _learned(args):
     from volumesurfacediscretemeshoptimization_amd import (DMODataset, DMOHexClass, DMOMeshHexFactory,
                                                            DMOMeshTetFactory, DMOMLP, DMOTetClass)
     kind = WORKLOADS[args.workload]["kind"]
@@ -264,46 +296,36 @@ def bench_learned(args):
     learned_updates = stl["searches_inner"] + stl["searches_surface"] + stl["searches_feature"]
     reached = q_l >= q_plain - 1e-3 and qm_l >= qm_plain - 1e-3
     value = plain_updates / dt_learned if reached else 0.0
-    wl = WORKLOADS[args.workload]
     ring = 24 if kind == "tet" else 8
     per_elem = 73 if kind == "tet" else 407
     evals = stl["searches_learned"] * (64 * args.learn_depth + 1) + stl["fallbacks"] * 1540
     flops = evals * ring * per_elem
     achieved = flops / (stl["kernel_ms"] / 1e3) / 1e12
-    out = {
-        "metric": METRIC, "value": value, "unit": "vertex updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * dt_learned / its, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 search, bf16 MLP", "data": "synthetic",
-        "config": {"workload": desc + "; learned proposal (4x4x4 window 1/4 of the plain one, "
-                   f"{args.learn_depth} levels, fallback to the full search) vs plain", "name": args.workload,
-                   "mode": "learned", "value_definition": "plain run's vertex updates / learned wall time to reach "
-                   "the plain run's final min and mean quality (-1e-3)",
-                   "plain_value": plain_updates / dt_plain, "plain_ms_per_step": 1e3 * dt_plain / args.steps,
-                   "plain_iterations": args.steps, "learned_iterations": its,
-                   "learned_raw_value": learned_updates / dt_learned,
-                   "speedup_vs_plain": (value / (plain_updates / dt_plain)) if reached else 0.0,
-                   "reached_plain_quality": bool(reached), "min_quality_start": q0,
-                   "min_quality_plain": q_plain, "min_quality_learned": q_l,
-                   "mean_quality_plain": qm_plain, "mean_quality_learned": qm_l,
-                   "fallback_rate": stl["fallbacks"] / max(1, stl["searches_learned"]),
-                   "training_pairs": n_pairs, "training_meshes": s, "record_s": rec_s, "train_s": train_s,
-                   "epochs": args.epochs, "loss_first_epoch": float(losses[0]) if len(losses) else None,
-                   "loss_last_epoch": float(losses[-1]) if len(losses) else None},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": f"k_search_{kind}_learned<8>", "launches": stl["kernel_launches"],
-                     "mean_launch_us": 1e3 * stl["kernel_ms"] / max(1, stl["kernel_launches"])},
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"],
-                                           args.workload)
-    print(json.dumps(out), flush=True)
     for o in (op, ol):
         o.close()
     mlp.close()
-    return 0
+    return {
+        "value": value,
+        "value_definition": "plain run's vertex updates / learned wall time to reach the plain run's final min and "
+                            "mean quality (-1e-3)",
+        "plain_value": plain_updates / dt_plain, "plain_ms_per_step": 1e3 * dt_plain / args.steps,
+        "learned_ms_per_step": 1e3 * dt_learned / its,
+        "plain_iterations": args.steps, "learned_iterations": its,
+        "learned_raw_value": learned_updates / dt_learned,
+        "speedup_vs_plain": (value / (plain_updates / dt_plain)) if reached else 0.0,
+        "reached_plain_quality": bool(reached), "min_quality_start": q0,
+        "min_quality_plain": q_plain, "min_quality_learned": q_l,
+        "mean_quality_plain": qm_plain, "mean_quality_learned": qm_l,
+        "fallback_rate": stl["fallbacks"] / max(1, stl["searches_learned"]),
+        "training_pairs": n_pairs, "training_meshes": s, "record_s": rec_s, "train_s": train_s,
+        "epochs": args.epochs, "loss_first_epoch": float(losses[0]) if len(losses) else None,
+        "loss_last_epoch": float(losses[-1]) if len(losses) else None,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": f"k_search_{kind}_learned<8> + k_search_list_{kind}<8,4>",
+                     "launches": stl["kernel_launches"],
+                     "mean_launch_us": 1e3 * stl["kernel_ms"] / max(1, stl["kernel_launches"])},
+    }
 
 
 def main(argv=None):
@@ -318,6 +340,8 @@ def main(argv=None):
     ap.add_argument("--tune-iters", type=int, default=3, help="iterations per autotuner trial (batch256)")
     ap.add_argument("--mode", choices=["plain", "learned"], default="plain")
     ap.add_argument("--waves", type=int, default=1, help="wavefronts per vertex search (1, 2, 4)")
+    ap.add_argument("--compare-learned", type=int, default=None,
+                    help="attach the learned-vs-plain comparison to the plain line (default: on for tet40)")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs (learned mode)")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--learn-depth", type=int, default=4)
@@ -329,6 +353,8 @@ def main(argv=None):
               "path; run it with --gpus 1.", file=sys.stderr)
         return 2
 
+    if args.compare_learned is None:
+        args.compare_learned = int(args.workload == "tet40" and args.mode == "plain")
     if args.workload == "batch256":
         return bench_batch(args)
     if args.mode == "learned":
@@ -381,6 +407,10 @@ def main(argv=None):
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"], args.workload)
     desc += f"; {args.waves} wavefront(s) per vertex"
+    learned = None
+    if args.compare_learned and kind in ("tet", "hex"):
+        learned = learned_comparison(args)
+        learned.pop("roofline", None)
     out = {
         "metric": METRIC, "value": value, "unit": "vertex updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -395,6 +425,8 @@ def main(argv=None):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if learned is not None:
+        out["config"]["learned_vs_plain"] = learned
     print(json.dumps(out), flush=True)
     opt.close()
     mesh.close()

@@ -901,21 +901,42 @@ __global__ void k_batch_quality(BatchQArgs a)
     if (threadIdx.x == 0) { a.cmin[c] = smin[0]; a.csum[c] = ssum[0]; }
 }
 
-// per-mesh min / sum over the mesh's chunks in chunk order (one block); the
-// minima are also appended to the history row *hctr, then *hctr advances
+// per-mesh min / FP64 sum over the mesh's chunks: one block per mesh,
+// strided partials then a fixed tree (the order depends only on the chunk
+// count); the minimum is also written to history row *hctr
 __global__ void k_batch_mesh_reduce(BatchQArgs a)
 {
-    for (int b = threadIdx.x; b < a.nMeshes; b += blockDim.x) {
-        float m = FLT_MAX;
-        double s = 0.0;
-        for (int c = a.meshChunk0[b]; c < a.meshChunk0[b + 1]; ++c) { m = fminf(m, a.cmin[c]); s = s + a.csum[c]; }
-        a.mmin[b] = m;
-        a.msum[b] = s;
-        const int row = *a.hctr;
-        if (row < a.hcap) a.hist[(size_t)row * a.nMeshes + b] = m;
+    __shared__ float smin[256];
+    __shared__ double ssum[256];
+    const int b = blockIdx.x;
+    float m = FLT_MAX;
+    double s = 0.0;
+    for (int c = a.meshChunk0[b] + (int)threadIdx.x; c < a.meshChunk0[b + 1]; c += 256) {
+        m = fminf(m, a.cmin[c]);
+        s = s + a.csum[c];
     }
+    smin[threadIdx.x] = m;
+    ssum[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x == 0) *a.hctr = *a.hctr + 1;
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + k]);
+            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.mmin[b] = smin[0];
+        a.msum[b] = ssum[0];
+        const int row = *a.hctr;
+        if (row < a.hcap) a.hist[(size_t)row * a.nMeshes + b] = smin[0];
+    }
+}
+
+// advances the history row after every mesh of the iteration wrote it
+__global__ void k_batch_history_next(int* hctr)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *hctr = *hctr + 1;
 }
 
 // per-block min and FP64 sum in a fixed tree order
@@ -1073,7 +1094,8 @@ void launch_fill_int(int* p, int n, int v, hipStream_t s)
 void launch_batch_quality(const BatchQArgs& a, hipStream_t s)
 {
     if (a.nChunks > 0) k_batch_quality<<<a.nChunks, 256, 0, s>>>(a);
-    k_batch_mesh_reduce<<<1, 256, 0, s>>>(a);
+    if (a.nMeshes > 0) k_batch_mesh_reduce<<<a.nMeshes, 256, 0, s>>>(a);
+    k_batch_history_next<<<1, 64, 0, s>>>(a.hctr);
 }
 
 size_t batch_lds_bytes(int maxRingTet, int maxRingHex, int maxHETri)
