@@ -72,8 +72,10 @@ typedef struct vsdmo_params {
                                   one-rings of at most 40 vertices) */
     int32_t quality_guard;     /* 1: fail with VSDMO_QUALITY_DECREASED if min quality drops */
     float guard_tolerance;     /* relative slack of the guard (FP32 frame differences) */
-    int32_t waves_per_vertex;  /* 1, 2 or 4 wavefronts per vertex search (a multiple of
-                                  nq's divisor; results do not depend on it) */
+    int32_t waves_per_vertex;  /* 1, 2 or 4 wavefronts per vertex search (dividing nq), or 0:
+                                  one per vertex except the last 1024 vertices of a tet / hex
+                                  colour class, which get 4 (nq 4 or 8); results do not depend
+                                  on it */
 } vsdmo_params;
 
 typedef struct vsdmo_counts {

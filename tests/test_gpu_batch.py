@@ -159,3 +159,22 @@ def test_single_mesh_waves_per_vertex(kind, w):
     assert np.array_equal(m1.positions(original_order=False), m2.positions(original_order=False))
     o1.close()
     o2.close()
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_split_launch_bit_exact(kind):
+    """waves_per_vertex = 0: one vertex per wave with a 4-wave tail per class
+    (classes here have ~1,500 vertices: head and tail both used)."""
+    gen = {"tet": lambda: synth.tet_cube(24, seed=9), "hex": lambda: synth.hex_cube(24, seed=9, jitter=0.2)}[kind]
+    xyz, cells = gen()
+    m1 = FAC[kind].create(xyz, cells)
+    m2 = FAC[kind].create(xyz, cells)
+    off = m1.color_offsets("inner")
+    assert np.diff(off).max() > 1024
+    o1 = CLS[kind](m1, n_iter=2, fixed_boundary=True, waves_per_vertex=1)
+    o2 = CLS[kind](m2, n_iter=2, fixed_boundary=True, waves_per_vertex=0)
+    o1.iterate(2)
+    o2.iterate(2)
+    assert np.array_equal(m1.positions(original_order=False), m2.positions(original_order=False))
+    o1.close()
+    o2.close()

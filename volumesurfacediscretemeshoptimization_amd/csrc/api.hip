@@ -427,7 +427,7 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
             return fail(VSDMO_INVALID_ARG, "nq must be 2, 4, 6 or 8");
         if (p.depth < 1 || p.depth > 16) return fail(VSDMO_INVALID_ARG, "depth must be in [1,16]");
         if (!waves_supported(p.nq, p.waves_per_vertex))
-            return fail(VSDMO_INVALID_ARG, "waves_per_vertex must be 1, 2 or 4 and divide nq");
+            return fail(VSDMO_INVALID_ARG, "waves_per_vertex must be 0 (auto), 1, 2 or 4 and divide nq");
         if (!p.fixed_boundary && (m->maxHE > 40 || m->nSurf == 0))
             return fail(VSDMO_NOT_SUPPORTED, "surface optimisation needs surface one-rings of at most 40 vertices");
         auto* o = new vsdmo_opt();
@@ -458,6 +458,18 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
     });
 }
 
+// waves_per_vertex 0: one launch per class with one vertex per wave and the
+// last kSplitTail vertices at 4 waves each (tet / hex, nq 4 or 8)
+constexpr int kSplitTail = 1024;
+static void dispatch_search(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds, hipStream_t s)
+{
+    const int w = o->p.waves_per_vertex;
+    if (w == 0 && split_supported(kind, o->p.nq, o->m->maxRing))
+        launch_search_split(kind, o->p.nq, a, kSplitTail, o->m->maxRing, s);
+    else
+        launch_search(kind, o->p.nq, w == 0 ? 1 : w, a, lds, s);
+}
+
 static void timed_launch(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds)
 {
     hipStream_t s = o->m->stream;
@@ -466,12 +478,12 @@ static void timed_launch(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds
         VSDMO_HIP(hipEventCreate(&e0));
         VSDMO_HIP(hipEventCreate(&e1));
         VSDMO_HIP(hipEventRecord(e0, s));
-        launch_search(kind, o->p.nq, o->p.waves_per_vertex, a, lds, s);
+        dispatch_search(o, kind, a, lds, s);
         VSDMO_HIP(hipEventRecord(e1, s));
         o->ev.push_back(e0);
         o->ev.push_back(e1);
     } else {
-        launch_search(kind, o->p.nq, o->p.waves_per_vertex, a, lds, s);
+        dispatch_search(o, kind, a, lds, s);
     }
     o->launches++;
 }

@@ -283,7 +283,8 @@ std::string check_params(const vsdmo_batch_params& p)
 {
     if (!(p.nq == 2 || p.nq == 4 || p.nq == 6 || p.nq == 8)) return "nq must be 2, 4, 6 or 8";
     if (p.depth < 1 || p.depth > 16) return "depth must be in [1,16]";
-    if (!waves_supported(p.nq, p.waves_per_vertex)) return "waves_per_vertex must be 1, 2 or 4 and divide nq";
+    if (p.waves_per_vertex < 1 || !waves_supported(p.nq, p.waves_per_vertex))
+        return "waves_per_vertex must be 1, 2 or 4 and divide nq";
     if (!(p.grid_scale > 0.0f)) return "grid_scale must be > 0";
     return "";
 }

@@ -100,6 +100,10 @@ struct BatchQArgs {
 size_t search_lds_bytes(int kind, int maxRing);
 bool waves_supported(int nq, int w);
 void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hipStream_t s);
+// tet / hex class in one launch: one vertex per wave for all but the last
+// `tail` vertices, which get several waves each (waves_per_vertex = 0)
+bool split_supported(int kind, int nq, int ringCap);
+void launch_search_split(int kind, int nq, const SearchArgs& a, int tail, int ringCap, hipStream_t s);
 // one colour round of a batch: persistent (grid > 0 blocks, tet/hex items
 // [lo, mid) from the atomic counter *ctr, flat items [mid, hi) strided) or,
 // with grid <= 0, one block per item

@@ -112,7 +112,8 @@ __device__ void tet_vertex(const SearchArgs& a, int vid, bool emin, TetC* s_tc, 
     constexpr int T = 64 * W;
     constexpr int CPL = NQ / W;
     static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
-    const int tid = threadIdx.x;
+    // W = 1 bodies also run one per wave inside wider blocks (k_search_split)
+    const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
     const int r0 = a.ringRow[vid];
     const int n = a.ringRow[vid + 1] - r0;
     const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
@@ -228,7 +229,8 @@ __device__ void hex_vertex(const SearchArgs& a, int vid, bool emin, HexLds* s_hx
     constexpr int T = 64 * W;
     constexpr int CPL = NQ / W;
     static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
-    const int tid = threadIdx.x;
+    // W = 1 bodies also run one per wave inside wider blocks (k_search_split)
+    const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
     const int r0 = a.ringRow[vid];
     const int n = a.ringRow[vid + 1] - r0;
     const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
@@ -355,7 +357,8 @@ template <int NQ, int W>
 __device__ void flat_vertex(const SearchArgs& a, int vid, TriC* s_tr, RedLds<W>& rd)
 {
     constexpr int T = 64 * W;
-    const int tid = threadIdx.x;
+    // W = 1 bodies also run one per wave inside wider blocks (k_search_split)
+    const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
     const int h0 = a.heRow[vid];
     const int n = a.heRow[vid + 1] - h0;
     const float x0x = a.x[vid], x0y = a.y[vid];
@@ -741,6 +744,51 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y
     }
 }
 
+// ------------------------------------------------------- split launches
+// One colour class in one launch of 256-thread blocks: the first nHead
+// vertices one per wave (the W = 1 body, its ring staged in the wave's own
+// quarter of the LDS), the remaining ones one per block with 4 (or 2) waves
+// per vertex.  The end of a launch, where few waves are left on the GPU,
+// then drains in finer grains.  Every body gives the same result, so the
+// split changes no bits.  All 4 waves of a block pass the same barriers: a
+// head wave without a vertex passes the single barrier of a W = 1 search.
+template <int NQ, int WT>
+__global__ __launch_bounds__(256, 5) void k_search_tet_split(SearchArgs a, int nHead, int ringCap)
+{
+    extern __shared__ TetC s_tcs[];
+    __shared__ RedLds<1> rd1;
+    __shared__ RedLds<WT> rdt;
+    const bool emin = a.extentMin != 0;
+    const int headBlocks = (nHead + 3) >> 2;
+    if ((int)blockIdx.x < headBlocks) {
+        const int w = threadIdx.x >> 6;
+        const int i = 4 * (int)blockIdx.x + w;
+        if (i < nHead) tet_vertex<NQ, 1>(a, a.cStart + i, emin, s_tcs + (size_t)w * ringCap, rd1);
+        else __syncthreads();   // the one barrier of a W = 1 search
+    } else {
+        static_assert(WT == 4, "the tail blocks use all 4 waves");
+        tet_vertex<NQ, WT>(a, a.cStart + nHead + ((int)blockIdx.x - headBlocks), emin, s_tcs, rdt);
+    }
+}
+template <int NQ, int WT>
+__global__ __launch_bounds__(256) void k_search_hex_split(SearchArgs a, int nHead, int ringCap)
+{
+    extern __shared__ HexLds s_hxs[];
+    __shared__ RedLds<1> rd1;
+    __shared__ RedLds<WT> rdt;
+    const bool emin = a.extentMin != 0;
+    const int headBlocks = (nHead + 3) >> 2;
+    if ((int)blockIdx.x < headBlocks) {
+        const int w = threadIdx.x >> 6;
+        const int i = 4 * (int)blockIdx.x + w;
+        if (i < nHead) hex_vertex<NQ, 1>(a, a.cStart + i, emin, s_hxs + (size_t)w * ringCap, rd1);
+        else __syncthreads();
+    } else {
+        static_assert(WT == 4, "the tail blocks use all 4 waves");
+        hex_vertex<NQ, WT>(a, a.cStart + nHead + ((int)blockIdx.x - headBlocks), emin, s_hxs, rdt);
+    }
+}
+
 // ------------------------------------------------------- batched sweep
 // Persistent kernel of one colour round of a batch of meshes: blocks take
 // work items from a queue (one atomic counter per round) until it is empty.
@@ -1060,7 +1108,39 @@ static int batch_occ_nq(int w, size_t lds)
     return batch_occ<NQ, 1>(lds);
 }
 
-bool waves_supported(int nq, int w) { return w == 1 || (w == 2 && nq % 2 == 0) || (w == 4 && nq % 4 == 0); }
+bool waves_supported(int nq, int w)
+{
+    return w == 0 || w == 1 || (w == 2 && nq % 2 == 0) || (w == 4 && nq % 4 == 0);
+}
+
+template <int NQ, int WT>
+static void split_nq(int kind, const SearchArgs& a, int nHead, int nTail, int ringCap, hipStream_t s)
+{
+    const int grid = (nHead + 3) / 4 + nTail;
+    if (grid <= 0) return;
+    if (kind == VSDMO_TET)
+        k_search_tet_split<NQ, WT><<<grid, 256, sizeof(TetC) * 4 * (size_t)ringCap, s>>>(a, nHead, ringCap);
+    else
+        k_search_hex_split<NQ, WT><<<grid, 256, sizeof(HexLds) * 4 * (size_t)ringCap, s>>>(a, nHead, ringCap);
+}
+
+bool split_supported(int kind, int nq, int ringCap)
+{
+    if (kind != VSDMO_TET && kind != VSDMO_HEX) return false;
+    if (nq != 4 && nq != 8) return false;   // the tail needs 4 waves per vertex (nq / 4 candidates per lane)
+    const size_t per = (kind == VSDMO_TET) ? sizeof(TetC) : sizeof(HexLds);
+    return per * 4 * (size_t)std::max(ringCap, 1) <= 65536;
+}
+
+void launch_search_split(int kind, int nq, const SearchArgs& a, int tail, int ringCap, hipStream_t s)
+{
+    const int n = a.cEnd - a.cStart;
+    if (n <= 0) return;
+    const int nTail = std::min(n, std::max(tail, 0));
+    const int nHead = n - nTail;
+    if (nq == 4) split_nq<4, 4>(kind, a, nHead, nTail, ringCap, s);
+    else if (nq == 8) split_nq<8, 4>(kind, a, nHead, nTail, ringCap, s);
+}
 
 int batch_round_blocks_per_cu(int nq, int w, size_t lds)
 {
