@@ -45,6 +45,13 @@ typedef enum vsdmo_coloring {
     VSDMO_COLOR_JPL = 1      /* reference color_jpl (mesh/VertexColoring.cu:76-125): colour = round */
 } vsdmo_coloring;
 
+/* which optimiser a triangle / quad mesh gets */
+typedef enum vsdmo_surface_path {
+    VSDMO_SURFACE_PATH_AUTO = 0,     /* flat path for a flat triangle mesh (z == 0), surface path otherwise */
+    VSDMO_SURFACE_PATH_SURFACE = 1,  /* DMOTriClass / DMOQuadClass: vertices move on their local surfaces */
+    VSDMO_SURFACE_PATH_FLAT = 2      /* DMOTriFlatClass: 2D search in the plane (flat triangle meshes only) */
+} vsdmo_surface_path;
+
 typedef enum vsdmo_extent {
     VSDMO_EXTENT_AUTO = -1,  /* reference choice: max for tet/tri, min for hex */
     VSDMO_EXTENT_MAX = 0,
@@ -76,6 +83,8 @@ typedef struct vsdmo_params {
                                   one per vertex except the last 1024 vertices of a tet / hex
                                   colour class, which get 4 (nq 4 or 8); results do not depend
                                   on it */
+    int32_t surface_path;      /* vsdmo_surface_path, triangle and quad meshes only; on the
+                                  surface path fixed_boundary = 1 keeps the feature vertices */
 } vsdmo_params;
 
 typedef struct vsdmo_counts {
@@ -97,15 +106,16 @@ typedef struct vsdmo_stats {
 /* defaults: greedy colouring, 20 degrees / nq 8, depth 3, reference shrink,
    auto extent, surface and boundary vertices optimised as in the reference
    (fixed_boundary = 0; pass 1 to keep them fixed), guard on, one wave per
-   vertex */
+   vertex, automatic surface path */
 void vsdmo_mesh_params_default(vsdmo_mesh_params* p);
 void vsdmo_params_default(vsdmo_params* p);
 
-/* Replaces DMOMesh{Tri,Tet,Hex}Factory::create (mesh/MeshFactory.h:17-41,
- * mesh/MeshFactory.cu:3-111) together with Mesh{Tri,Tet,Hex}GPU::init
- * (mesh/MeshTriGPU.cu:37-52, mesh/MeshTetGPU.cu:35-42, mesh/MeshHexGPU.cu:35-43).
- * xyz: 3*nV floats; cells: npe*nCells int32 (tri 3, tet 4, hex 8; hexes in the
- * reference's internal order, top face first).  Host arrays are copied in and
+/* Replaces DMOMesh{Tri,Quad,Tet,Hex}Factory::create (mesh/MeshFactory.h:17-41,
+ * mesh/MeshFactory.cu:3-111) together with Mesh{Tri,Quad,Tet,Hex}GPU::init
+ * (mesh/MeshTriGPU.cu:37-52, mesh/MeshQuadGPU.cu:37-45, mesh/MeshTetGPU.cu:35-42,
+ * mesh/MeshHexGPU.cu:35-43).  xyz: 3*nV floats; cells: npe*nCells int32 (tri 3,
+ * quad 4, tet 4, hex 8; hexes in the reference's internal order, top face
+ * first).  A triangle or quad mesh is flat when every z is 0.  Host arrays are copied in and
  * the mesh structures are built on the GPU.  boundary_or_null must be NULL
  * (boundary detection from the elements, reference findBoundary2D=true). */
 vsdmo_status vsdmo_mesh_create(vsdmo_elem type, const float* xyz, int32_t nV, const int32_t* cells,
@@ -120,6 +130,11 @@ vsdmo_status vsdmo_mesh_counts(const vsdmo_mesh* mesh, vsdmo_counts* out);
    (the reference writes files in the renumbered order, io/FileWriter.cu:64-102) */
 vsdmo_status vsdmo_mesh_get_positions(const vsdmo_mesh* mesh, float* out, int32_t original_order);
 vsdmo_status vsdmo_mesh_set_positions(vsdmo_mesh* mesh, const float* in, int32_t original_order);
+/* Replaces laplaceSmoothing(DMOMeshTri&, n_iter) (dmo/Laplace.h:10, dmo/Laplace.cu:8-41):
+   n_iter sweeps over the free-vertex colour classes of a triangle or quad
+   mesh, p <- (p + sum of the one-ring) / (1 + n).  Optimisers created before
+   keep their local surfaces. */
+vsdmo_status vsdmo_mesh_laplace(vsdmo_mesh* mesh, int32_t n_iter);
 /* new_of_old[nV]: renumbering applied by the build (mesh/SortUtil.cu:103-107) */
 vsdmo_status vsdmo_mesh_get_permutation(const vsdmo_mesh* mesh, int32_t* new_of_old);
 /* elements in the renumbered ids, npe*nE */
@@ -141,9 +156,11 @@ vsdmo_status vsdmo_mesh_get_halfedges(const vsdmo_mesh* mesh, int32_t* row_ptr, 
 /* per surface vertex flags (boundary of an open surface, feature) */
 vsdmo_status vsdmo_mesh_get_flags(const vsdmo_mesh* mesh, uint8_t* boundary1d, uint8_t* feature);
 
-/* Replaces DMO::DMO{Tet,Hex,TriFlat}Class(mesh, criterion, gridScale, n_iter)
- * (dmo/DMOTet.h:24, dmo/DMOHex.h:18, dmo/DMOTriFlat.h:25) and their init()
- * (dmo/DMOTet.cu:92-125).  params may be NULL (defaults). */
+/* Replaces DMO::DMO{Tet,Hex,TriFlat,Tri,Quad}Class(mesh, criterion, gridScale, n_iter)
+ * (dmo/DMOTet.h:24, dmo/DMOHex.h:18, dmo/DMOTriFlat.h:25, dmo/DMOTri.h:24,
+ * dmo/DMOQuad.h:18) and their init() (dmo/DMOTet.cu:92-125, dmo/DMOTri.cu:93-125,
+ * dmo/DMOQuad.cu:91-123).  params may be NULL (defaults).  A triangle / quad
+ * mesh takes the path params->surface_path selects. */
 vsdmo_status vsdmo_opt_create(vsdmo_mesh* mesh, int32_t criterion, float grid_scale, int32_t n_iter,
                               const vsdmo_params* params, vsdmo_opt** out);
 /* Replaces DMOBaseClass::doIteration (dmo/DMOTet.cu:128-170, dmo/DMOHex.cu:131-182,
@@ -169,9 +186,9 @@ vsdmo_status vsdmo_opt_destroy(vsdmo_opt* opt);
 /* thread-local message of the last failing call on this thread */
 const char* vsdmo_last_error(void);
 
-/* Mesh files.  Replace the file overloads of DMOMesh{Tri,Tet,Hex}Factory::create
- * (mesh/MeshFactory.cu:3-111) with the reference readers io/ParserOFF.cu:15-83
- * (flat triangles, .off), io/ParserTetGen.cu:13-120 (<path>.node + <path>.ele,
+/* Mesh files.  Replace the file overloads of DMOMesh{Tri,Quad,Tet,Hex}Factory::create
+ * (mesh/MeshFactory.cu:3-111) with the reference readers io/ParserOFF.cu:15-160
+ * (triangle / quad .off), io/ParserTetGen.cu:13-120 (<path>.node + <path>.ele,
  * 1-based numbering detected from the first point index) and
  * io/ParserHex.cu:115-240 (MEDIT .mesh hexahedra, file bottom face = internal
  * top face), and the writers io/FileWriter.cu:9-147 (9 significant digits, so a

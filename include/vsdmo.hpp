@@ -5,7 +5,9 @@
 //   DMO::DMOTetClass(mesh, QualityCriterium, gridScale, n_iter) with
 //       optimize(), doIteration(), isDone(), getQuality(),
 //       getQualityHistogram(), getElementQualities(), displayQualityGPU()
-//       (reference VSDMO/src/dmo/DMOBase.h:23-88, DMOTet.h:24)
+//       (reference VSDMO/src/dmo/DMOBase.h:23-88, DMOTet.h:24), and the
+//       DMOHex / DMOTriFlat / DMOTri / DMOQuad classes
+//   laplaceSmoothing(mesh, n_iter) (reference VSDMO/src/dmo/Laplace.h:10)
 // The file-name overloads read OFF / TetGen / MEDIT files (vsdmo_mesh_file_read).
 // Errors become std::runtime_error carrying vsdmo_last_error().
 #pragma once
@@ -59,6 +61,7 @@ private:
 using DMOMeshTri = DMOMesh;
 using DMOMeshTet = DMOMesh;
 using DMOMeshHex = DMOMesh;
+using DMOMeshQuad = DMOMesh;
 
 // the file overloads return nullptr when the file cannot be read, like the
 // reference factories
@@ -78,6 +81,13 @@ struct DMOMeshTriFactory {
         return std::make_shared<DMOMeshTri>(VSDMO_TRI, xyz, tris, 3);
     }
     static std::shared_ptr<DMOMeshTri> create(const std::string& offFile) { return create_from_file(VSDMO_TRI, 3, offFile); }
+};
+struct DMOMeshQuadFactory {
+    static std::shared_ptr<DMOMeshQuad> create(const std::vector<float>& xyz, const std::vector<int32_t>& quads)
+    {
+        return std::make_shared<DMOMeshQuad>(VSDMO_QUAD, xyz, quads, 4);
+    }
+    static std::shared_ptr<DMOMeshQuad> create(const std::string& offFile) { return create_from_file(VSDMO_QUAD, 4, offFile); }
 };
 struct DMOMeshTetFactory {
     static std::shared_ptr<DMOMeshTet> create(const std::vector<float>& xyz, const std::vector<int32_t>& tets)
@@ -107,10 +117,14 @@ enum class QualityCriterium { MEAN_RATIO = 0 };
 class DMOBaseClass {
 public:
     DMOBaseClass(vsdmo_cpp::DMOMesh& mesh, QualityCriterium q = QualityCriterium::MEAN_RATIO, float gridScale = 0.5f,
-                 int n_iter = 100, const vsdmo_params* prm = nullptr)
+                 int n_iter = 100, const vsdmo_params* prm = nullptr, int32_t surfacePath = -1)
         : mesh_(mesh)
     {
-        vsdmo_cpp::check(vsdmo_opt_create(mesh.handle(), (int32_t)q, gridScale, n_iter, prm, &h_), "vsdmo_opt_create");
+        vsdmo_params p;
+        if (prm) p = *prm;
+        else vsdmo_params_default(&p);
+        if (surfacePath >= 0) p.surface_path = surfacePath;
+        vsdmo_cpp::check(vsdmo_opt_create(mesh.handle(), (int32_t)q, gridScale, n_iter, &p, &h_), "vsdmo_opt_create");
     }
     virtual ~DMOBaseClass() { if (h_) vsdmo_opt_destroy(h_); }
     DMOBaseClass(const DMOBaseClass&) = delete;
@@ -179,7 +193,27 @@ class DMOTriFlatClass : public DMOBaseClass {
 public:
     DMOTriFlatClass(vsdmo_cpp::DMOMeshTri& m, QualityCriterium q = QualityCriterium::MEAN_RATIO,
                     float gridScale = 0.5f, int n_iter = 30, const vsdmo_params* prm = nullptr)
-        : DMOBaseClass(m, q, gridScale, n_iter, prm) {}
+        : DMOBaseClass(m, q, gridScale, n_iter, prm, VSDMO_SURFACE_PATH_FLAT) {}
+};
+// triangle / quad surface meshes: vertices move on their local surfaces
+// (reference dmo/DMOTri.h:24, dmo/DMOQuad.h:18)
+class DMOTriClass : public DMOBaseClass {
+public:
+    DMOTriClass(vsdmo_cpp::DMOMeshTri& m, QualityCriterium q = QualityCriterium::MEAN_RATIO, float gridScale = 0.5f,
+                int n_iter = 30, const vsdmo_params* prm = nullptr)
+        : DMOBaseClass(m, q, gridScale, n_iter, prm, VSDMO_SURFACE_PATH_SURFACE) {}
+};
+class DMOQuadClass : public DMOBaseClass {
+public:
+    DMOQuadClass(vsdmo_cpp::DMOMeshQuad& m, QualityCriterium q = QualityCriterium::MEAN_RATIO, float gridScale = 0.5f,
+                 int n_iter = 30, const vsdmo_params* prm = nullptr)
+        : DMOBaseClass(m, q, gridScale, n_iter, prm, VSDMO_SURFACE_PATH_SURFACE) {}
 };
 
 }  // namespace DMO
+
+// reference dmo/Laplace.h:10 (free vertices of a triangle / quad mesh)
+inline void laplaceSmoothing(vsdmo_cpp::DMOMesh& mesh, int n_iter)
+{
+    vsdmo_cpp::check(vsdmo_mesh_laplace(mesh.handle(), n_iter), "vsdmo_mesh_laplace");
+}

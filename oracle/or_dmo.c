@@ -225,12 +225,35 @@ static void elem_pts(const or_mesh* m, int e, float* pts)
     }
 }
 
+/* the element metric of each mesh kind: tet / hex shape metrics, the 2D
+   mean ratio of flat triangles, the unsigned 3D mean ratio of surface
+   triangles and quads */
+static float elem_q(const or_mesh* m, const float* pts)
+{
+    switch (m->elem) {
+    case OR_TET: return or_q_tet(pts);
+    case OR_HEX: return or_q_hex(pts);
+    case OR_QUAD: return or_q_quad(pts);
+    default: return m->flat ? or_q_tri2d(pts) : or_q_tri3(pts);
+    }
+}
+
+static double elem_q64(const or_mesh* m, const float* pts)
+{
+    switch (m->elem) {
+    case OR_TET: return or_q_tet64(pts);
+    case OR_HEX: return or_q_hex64(pts);
+    case OR_QUAD: return or_q_quad64(pts);
+    default: return m->flat ? or_q_tri2d64(pts) : or_q_tri364(pts);
+    }
+}
+
 void or_element_qualities(const or_mesh* m, float* out)
 {
     for (int e = 0; e < m->nE; ++e) {
         float pts[24];
         elem_pts(m, e, pts);
-        out[e] = (m->elem == OR_TET) ? or_q_tet(pts) : (m->elem == OR_HEX) ? or_q_hex(pts) : or_q_tri2d(pts);
+        out[e] = elem_q(m, pts);
     }
 }
 
@@ -240,7 +263,7 @@ float or_min_quality(const or_mesh* m)
     for (int e = 0; e < m->nE; ++e) {
         float pts[24];
         elem_pts(m, e, pts);
-        float v = (m->elem == OR_TET) ? or_q_tet(pts) : (m->elem == OR_HEX) ? or_q_hex(pts) : or_q_tri2d(pts);
+        float v = elem_q(m, pts);
         q = fminf(q, v);
     }
     return q;
@@ -252,7 +275,7 @@ void or_quality64(const or_mesh* m, double* qmin, double* qmean)
     for (int e = 0; e < m->nE; ++e) {
         float pts[24];
         elem_pts(m, e, pts);
-        double v = (m->elem == OR_TET) ? or_q_tet64(pts) : (m->elem == OR_HEX) ? or_q_hex64(pts) : or_q_tri2d64(pts);
+        double v = elem_q64(m, pts);
         if (v < mn) mn = v;
         sum += v;
     }

@@ -673,6 +673,306 @@ long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* 
     return cnt + *nSurf2d + *nFeat1d;
 }
 
+/* ------------------------------------------------------------------------
+ * Surface meshes (triangles / quads): DMOTriClass / DMOQuadClass
+ * (DMOTri.cu:128-170, DMOQuad.cu:128-170, DMOTriImplCub.h:25-380,
+ * DMOQuadImplCub.h:23-404).  The free vertices search in 2D on the blended
+ * local surfaces, the feature vertices in 1D on their curves, as for volume
+ * meshes; the score is the mean ratio of the fan elements around the vertex,
+ * signed by the vertex normal of the nearest neighbour (DMOQualityTri.h:
+ * 257-275, DMOQualityQuad.h:30-50), and the final write test uses the
+ * unsigned metric.  Vertex normals follow the positions after every
+ * iteration (updateNormals).
+ * ---------------------------------------------------------------------- */
+static float sgnf(float a) { return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f); }
+static w3 wneg(w3 a) { w3 r = { -a.x, -a.y, -a.z }; return r; }
+static float two_sqrt3(void) { return 2.0f * sqrtf(3.0f); }
+
+/* meanRatioMetricTri (3D), signed by n when n != NULL: 2 sqrt(3) A / sum l^2,
+   e0 = p1 - p0, e1 = p2 - p0, e2 = p2 - p1, A = |e0 x e1| * sign(n . unit(e0 x e1)) */
+static float mr_tri3(w3 p0, w3 p1, w3 p2, const w3* n)
+{
+    w3 e0 = wsub(p1, p0), e1 = wsub(p2, p0), e2 = wsub(p2, p1);
+    float l = (wdot(e0, e0) + wdot(e1, e1)) + wdot(e2, e2);
+    w3 N = wcross(e0, e1);
+    float area = wnorm(N);
+    if (n) area = area * sgnf(wdot(wnormalize(N), *n));
+    return (two_sqrt3() * area) / l;
+}
+
+/* meanRatioMetricQuad: 2 (|e1 x -e0| + |e3 x -e2|) / sum l^2 with e_i =
+   p_(i+1) - p_i; with n: -1 when either corner triangle's unit normal has
+   n . u < 0.7 */
+static float mr_quad3(w3 p0, w3 p1, w3 p2, w3 p3, const w3* n)
+{
+    w3 e0 = wsub(p1, p0), e1 = wsub(p2, p1), e2 = wsub(p3, p2), e3 = wsub(p0, p3);
+    float l = ((wdot(e0, e0) + wdot(e1, e1)) + wdot(e2, e2)) + wdot(e3, e3);
+    w3 N1 = wcross(e1, wneg(e0)), N2 = wcross(e3, wneg(e2));
+    float a1 = wnorm(N1), a2 = wnorm(N2);
+    if (n) {
+        float d1 = wdot(wnormalize(N1), *n), d2 = wdot(wnormalize(N2), *n);
+        if (d1 < 0.7f || d2 < 0.7f) return -1.0f;
+    }
+    return (2.0f * (a1 + a2)) / l;
+}
+
+float or_q_tri3(const float* p) { return mr_tri3(W(p), W(p + 3), W(p + 6), NULL); }
+float or_q_tri3n(const float* p, const float* n) { w3 nn = W(n); return mr_tri3(W(p), W(p + 3), W(p + 6), &nn); }
+float or_q_quad(const float* p) { return mr_quad3(W(p), W(p + 3), W(p + 6), W(p + 9), NULL); }
+float or_q_quadn(const float* p, const float* n) { w3 nn = W(n); return mr_quad3(W(p), W(p + 3), W(p + 6), W(p + 9), &nn); }
+
+/* the same metrics in FP64 (literal formulas, for the tests' tolerance checks) */
+static void d3(const float* p, double* o) { o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; }
+static double dnorm3(const double* a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+static void dcross3(const double* a, const double* b, double* o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+}
+double or_q_tri364(const float* p)
+{
+    double a[3], b[3], c[3], e0[3], e1[3], e2[3], N[3];
+    d3(p, a); d3(p + 3, b); d3(p + 6, c);
+    for (int i = 0; i < 3; ++i) { e0[i] = b[i] - a[i]; e1[i] = c[i] - a[i]; e2[i] = c[i] - b[i]; }
+    double l = 0.0;
+    for (int i = 0; i < 3; ++i) l += e0[i] * e0[i] + e1[i] * e1[i] + e2[i] * e2[i];
+    dcross3(e0, e1, N);
+    return 2.0 * sqrt(3.0) * dnorm3(N) / l;
+}
+double or_q_quad64(const float* p)
+{
+    double v[4][3], e[4][3], m0[3], m2[3], N1[3], N2[3];
+    for (int k = 0; k < 4; ++k) d3(p + 3 * k, v[k]);
+    double l = 0.0;
+    for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < 3; ++i) { e[k][i] = v[(k + 1) % 4][i] - v[k][i]; l += e[k][i] * e[k][i]; }
+    for (int i = 0; i < 3; ++i) { m0[i] = -e[0][i]; m2[i] = -e[2][i]; }
+    dcross3(e[1], m0, N1);
+    dcross3(e[3], m2, N2);
+    return 2.0 * (dnorm3(N1) + dnorm3(N2)) / l;
+}
+
+/* element metric of a surface mesh on the surface path: unsigned 3D mean ratio */
+float or_q_surface_elem(const or_mesh* m, int e)
+{
+    const int* c = m->cells + (size_t)e * m->npe;
+    if (m->npe == 4) return mr_quad3(W(P(m, c[0])), W(P(m, c[1])), W(P(m, c[2])), W(P(m, c[3])), NULL);
+    return mr_tri3(W(P(m, c[0])), W(P(m, c[1])), W(P(m, c[2])), NULL);
+}
+
+void or_surface_mesh_qualities(const or_mesh* m, float* out)
+{
+    for (int e = 0; e < m->nE; ++e) out[e] = or_q_surface_elem(m, e);
+}
+
+float or_surface_mesh_min_quality(const or_mesh* m)
+{
+    float q = FLT_MAX;
+    for (int e = 0; e < m->nE; ++e) q = fminf(q, or_q_surface_elem(m, e));
+    return q;
+}
+
+/* fan of a surface vertex: triangles keep the halfedge targets r_i and score
+   the elements (p, r_k, r_k+1); quads keep (r_i, d_i), d_i the target of the
+   halfedge after vid -> r_i in its face, and score (p, r_k, d_k, r_k+1).
+   Closed (r_n = r_0, n elements) unless open (n - 1 elements).  maxDist is
+   the largest distance from vid to the r_i (and d_i).  The d_i are read from
+   the snapshot of the pass: on a surface boundary the "diagonal" of the
+   boundary halfedge is the boundary neighbour of the next boundary vertex,
+   which can belong to the colour class being updated; every other fan vertex
+   shares an element with vid, so the snapshot holds its current position. */
+typedef struct {
+    int quad, ne;
+    w3 pts[2 * MAX_NEIGH + 1];
+    w3 n;
+    float maxDist;
+} fan;
+
+static int fan_init(const or_mesh* m, const or_surf* s, const float* snap, int vid, int open, fan* f)
+{
+    const int h0 = m->heRow[vid], n = m->heRow[vid + 1] - h0;
+    if (n <= 0 || n > MAX_NEIGH) return 0;
+    f->quad = (m->fpe == 4);
+    const int K = f->quad ? 2 : 1;
+    const w3 x0 = W(P(m, vid));
+    float mx = -FLT_MAX;
+    for (int i = 0; i < n; ++i) {
+        const int h = h0 + i;
+        w3 r = W(P(m, m->heTarget[h]));
+        f->pts[K * i] = r;
+        mx = fmaxf(mx, wnorm(wsub(r, x0)));
+        if (f->quad) {
+            const int nx = m->heNext[h];
+            w3 d = W(snap + 3 * (size_t)m->heTarget[nx >= 0 ? nx : h]);
+            f->pts[K * i + 1] = d;
+            mx = fmaxf(mx, wnorm(wsub(d, x0)));
+        }
+    }
+    f->ne = open ? n - 1 : n;
+    if (!open) f->pts[K * n] = f->pts[0];
+    f->maxDist = mx;
+    f->n = W(s->vnorm + 3 * (size_t)s->nn[vid]);
+    return f->ne > 0;
+}
+
+static float fan_score(const fan* f, w3 p, int oriented)
+{
+    const w3* n = oriented ? &f->n : NULL;
+    float q = FLT_MAX;
+    for (int k = 0; k < f->ne; ++k) {
+        float e = f->quad ? mr_quad3(p, f->pts[2 * k], f->pts[2 * k + 1], f->pts[2 * k + 2], n)
+                          : mr_tri3(p, f->pts[k], f->pts[k + 1], n);
+        q = fminf(q, e);
+    }
+    return q;
+}
+
+/* 2D search of a free vertex of a surface mesh (k_optimizeHierarchical2D,
+   DMOTriImplCub.h:25-200, DMOQuadImplCub.h:23-212): window half-width
+   s_d * maxDist in the nearest neighbour's (u, v); candidates, ties and the
+   update rule as search2d; write iff q_unsigned(old) < last level max */
+static int search2d_fan(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz)
+{
+    fan f;
+    if (!fan_init(m, s, snap, vid, 0, &f)) return 0;
+    const lsurf* L = &s->ls[s->nn[vid]];
+    float vidPos[3] = { xyz[3 * (size_t)vid], xyz[3 * (size_t)vid + 1], xyz[3 * (size_t)vid + 2] };
+    w3 cur = W(vidPos);
+    w3 cv = wsub(cur, L->p0);
+    float cu = wdot(L->e[0], cv), cvv = wdot(L->e[1], cv);
+    const int nq = p->nq, half = nq / 2, T = nq * nq / 2;
+    const float aff = 1.0f / (float)(nq - 1);
+    const float shr = (p->shrink > 0.0f) ? p->shrink : 2.0f / (float)(nq - 1);
+    float qcur = fan_score(&f, cur, 1);
+    const float qold = fan_score(&f, cur, 0);
+    float qlast = -FLT_MAX;
+    float ds = p->grid_scale;
+    for (int d = 0; d < p->depth; ++d) {
+        float uMax = ds * f.maxDist, uMin = -(ds * f.maxDist);
+        float best = -FLT_MAX, bu = 0.0f, bv = 0.0f;
+        for (int t = 0; t < T; ++t) {
+            int i1 = t / nq, j1 = t % nq, i2 = i1 + half;
+            float u1 = cu + aff * ((float)i1 * uMin + (float)(nq - 1 - i1) * uMax);
+            float v1 = cvv + aff * ((float)j1 * uMin + (float)(nq - 1 - j1) * uMax);
+            float u2 = cu + aff * ((float)i2 * uMin + (float)(nq - 1 - i2) * uMax);
+            float q1 = fan_score(&f, project2d(m, s, snap, vid, vidPos, u1, v1), 1);
+            float q2 = fan_score(&f, project2d(m, s, snap, vid, vidPos, u2, v1), 1);
+            float qm = q1, uu = u1, vv = v1;
+            if (q2 > q1) { qm = q2; uu = u2; }
+            if (qm > best) { best = qm; bu = uu; bv = vv; }
+        }
+        if (qcur < best) {
+            cur = project2d(m, s, snap, vid, vidPos, bu, bv);
+            cu = bu; cvv = bv;
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * shr;
+    }
+    if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
+    return 0;
+}
+
+/* 1D search of a feature vertex of a surface mesh (k_optimizeHierarchical1D,
+   DMOTriImplCub.h:205-378, DMOQuadImplCub.h:216-402): the fan is open on a
+   surface boundary */
+static int search1d_fan(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz)
+{
+    if (!s->opt1d[vid]) return 0;
+    fan f;
+    if (!fan_init(m, s, snap, vid, m->isBoundary1D[vid] != 0, &f)) return 0;
+    int nb[MAX_NEIGH], nnb = 0;
+    for (int h = m->heRow[vid]; h < m->heRow[vid + 1]; ++h) {
+        int d = m->heTarget[h];
+        if (!m->isFeature[d]) continue;
+        int c = 0;
+        for (int h2 = m->heRow[d]; h2 < m->heRow[d + 1]; ++h2) if (m->isFeature[m->heTarget[h2]]) ++c;
+        if (c == 2 && nnb < MAX_NEIGH) nb[nnb++] = d;
+    }
+    const lsurf1* L = &s->ls1[s->nn[vid]];
+    w3 cur = W(xyz + 3 * (size_t)vid);
+    float cu = wdot(L->e[0], wsub(cur, L->p0));
+    const int nq = p->nq;
+    const float aff = 1.0f / (float)(nq - 1);
+    const float shr = (p->shrink > 0.0f) ? p->shrink : 2.0f / (float)(nq - 1);
+    float qcur = fan_score(&f, cur, 1);
+    const float qold = fan_score(&f, cur, 0);
+    float qlast = -FLT_MAX;
+    float ds = p->grid_scale;
+    for (int d = 0; d < p->depth; ++d) {
+        float uMax = ds * f.maxDist, uMin = -(ds * f.maxDist);
+        float best = -FLT_MAX, bu = 0.0f;
+        for (int i = 0; i < nq; ++i) {
+            float u = cu + aff * ((float)i * uMin + (float)(nq - 1 - i) * uMax);
+            float q = fan_score(&f, project1d(s, vid, nb, nnb, u), 1);
+            if (q > best) { best = q; bu = u; }
+        }
+        if (qcur < best) {
+            cur = project1d(s, vid, nb, nnb, bu);
+            cu = bu;
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * shr;
+    }
+    if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
+    return 0;
+}
+
+long long or_dmo_iteration_surface_mesh(or_mesh* m, or_surf* s, const or_dmo_params* p, long long* nSurf2d, long long* nFeat1d)
+{
+#ifdef _OPENMP
+    int nt = p->nthreads > 0 ? p->nthreads : omp_get_max_threads();
+#endif
+    *nSurf2d = 0; *nFeat1d = 0;
+    float* snap = (float*)malloc(sizeof(float) * 3 * (size_t)(m->nSurf > 0 ? m->nSurf : 1));
+    for (int c = 0; c < m->nColorsFree; ++c) {
+        int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
+        memcpy(snap, m->xyz, sizeof(float) * 3 * (size_t)m->nSurf);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+        for (int v = lo; v < hi; ++v) search2d_fan(m, s, snap, p, v, m->xyz);
+        *nSurf2d += hi - lo;
+    }
+    if (!p->fixed_boundary) {
+        for (int c = 0; c < m->nColorsFeature; ++c) {
+            int lo = m->colOffFeature[c], hi = m->colOffFeature[c + 1];
+            memcpy(snap, m->xyz, sizeof(float) * 3 * (size_t)m->nSurf);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+            for (int v = lo; v < hi; ++v) search1d_fan(m, s, snap, p, v, m->xyz);
+            for (int v = lo; v < hi; ++v) *nFeat1d += s->opt1d[v];
+        }
+    }
+    free(snap);
+    update_nn(m, s);
+    normals(m, s->fnorm, s->vnorm);
+    return *nSurf2d + *nFeat1d;
+}
+
+/* Laplace smoothing of the free vertices of a surface mesh, colour by colour
+   (laplaceSmoothing, Laplace.cu:8-41): p <- (p + sum of the halfedge
+   targets) * (1 / (1 + n)) */
+void or_laplace(or_mesh* m, int n_iter)
+{
+    for (int it = 0; it < n_iter; ++it)
+        for (int c = 0; c < m->nColorsFree; ++c) {
+            int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+            for (int v = lo; v < hi; ++v) {
+                const int h0 = m->heRow[v], n = m->heRow[v + 1] - h0;
+                w3 q = W(P(m, v));
+                for (int h = h0; h < h0 + n; ++h) q = wadd(q, W(P(m, m->heTarget[h])));
+                q = wmul(1.0f / (float)(1 + n), q);
+                float* o = m->xyz + 3 * (size_t)v;
+                o[0] = q.x; o[1] = q.y; o[2] = q.z;
+            }
+        }
+}
+
 /* Sample points of the local surfaces of surface vertex vid on an nu x nv
    grid (DMOCommon.h:197-276 k_fillEstimateLocalSurfacePoints with
    USE_SURF_OF_NN = false, and DMOCommon.cu:73-150 k_fillLocalSurfacePoints):
@@ -751,6 +1051,7 @@ void or_surface_get(const or_surf* s, int what, void* out)
                 o[6] = q->e[1].x; o[7] = q->e[1].y; o[8] = q->e[1].z; o[9] = q->a;
             }
             break;
+    case 4: memcpy(out, s->vnorm, sizeof(float) * 3 * (size_t)s->nS); break;
     default: break;
     }
 }

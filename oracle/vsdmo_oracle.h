@@ -13,7 +13,9 @@
  *                 DMOTriFlatImplCub.h
  *   metrics     : VSDMO/src/dmo/DMOQualityTet.h, DMOQualityHex.h,
  *                 DMOQualityTri.h, DMOQualityUtil.h
- *   drivers     : VSDMO/src/dmo/DMOTet.cu, DMOHex.cu, DMOTriFlat.cu
+ *   drivers     : VSDMO/src/dmo/DMOTet.cu, DMOHex.cu, DMOTriFlat.cu, DMOTri.cu,
+ *                 DMOQuad.cu, DMOTriImplCub.h, DMOQuadImplCub.h, Laplace.cu
+ *   surface metrics: DMOQualityTri.h, DMOQualityQuad.h
  *
  * Canonical FP32 evaluation order.  The reference's own FP32 bits are not
  * reproducible (nvcc contracts mul+add into FMA at its discretion and the
@@ -45,10 +47,19 @@ float or_pow2_scale(float m);           /* 2^-e with m = f*2^e, f in [1,2)   */
 float or_q_tet(const float* p4x3);      /* mean ratio, signed volume if inverted */
 float or_q_hex(const float* p8x3);      /* 9-sub-tet shape metric, internal order */
 float or_q_tri2d(const float* p3x3);    /* 2D mean ratio (x,y), signed if inverted */
+/* surface triangles / quads (literal FP32 with contraction off): unsigned
+   3D mean ratio, and the variant signed by a unit normal n (quads: -1 when
+   a corner triangle's normal has n . u < 0.7) */
+float or_q_tri3(const float* p3x3);
+float or_q_tri3n(const float* p3x3, const float* n);
+float or_q_quad(const float* p4x3);
+float or_q_quadn(const float* p4x3, const float* n);
 /* ---- FP64 literal re-evaluation (reference formulas in double) ---- */
 double or_q_tet64(const float* p4x3);
 double or_q_hex64(const float* p8x3);
 double or_q_tri2d64(const float* p3x3);
+double or_q_tri364(const float* p3x3);
+double or_q_quad64(const float* p4x3);
 
 /* ---- ring scores (what the search maximises) ---- */
 /* ring: nElem elements of 3 (tet), 7 (hex) or 2 (tri fan pair) points, each
@@ -136,14 +147,25 @@ void or_surface_free(or_surf* s);
    feature 1D passes, then the nearest-neighbour update; returns searches */
 long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* p, float lastMin,
                                    long long* nSurf2d, long long* nFeat1d);
+/* surface meshes (tri / quad) on the surface path: free-vertex 2D passes,
+   feature 1D passes (unless fixed_boundary), nearest-neighbour update and
+   vertex-normal update; no minimum-quality gate */
+long long or_dmo_iteration_surface_mesh(or_mesh* m, or_surf* s, const or_dmo_params* p,
+                                        long long* nSurf2d, long long* nFeat1d);
+/* the surface path's element metric (unsigned 3D mean ratio of triangles /
+   quads, also for flat triangle meshes) */
+void or_surface_mesh_qualities(const or_mesh* m, float* out);
+float or_surface_mesh_min_quality(const or_mesh* m);
+/* Laplace smoothing of the free vertices of a surface mesh */
+void or_laplace(or_mesh* m, int n_iter);
 /* what: 0 NN table (int nS), 1 local surfaces (16 floats each: p0, e0, e1,
    e2, a, b, c, numPoints), 2 1D eligibility (uint8 nS), 3 1D curves (10
-   floats: p0, e0, e1, a) */
+   floats: p0, e0, e1, a), 4 vertex normals (3 floats) */
 void or_surface_get(const or_surf* s, int what, void* out);
 int or_surface_nsectors(const or_surf* s, int nFeature);
 
 /* element quality statistics in FP32 canonical (min) and FP64 (min, mean) */
-float or_min_quality(const or_mesh* m);
+float or_min_quality(const or_mesh* m);   /* flat tri: 2D metric; surface tri / quad: unsigned 3D */
 void or_quality64(const or_mesh* m, double* qmin, double* qmean);
 void or_element_qualities(const or_mesh* m, float* out);
 

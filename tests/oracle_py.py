@@ -93,6 +93,19 @@ def lib():
         L.or_dmo_iteration_surface.restype = C.c_longlong
         L.or_dmo_iteration_surface.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams), C.c_float,
                                                C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.or_dmo_iteration_surface_mesh.restype = C.c_longlong
+        L.or_dmo_iteration_surface_mesh.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams),
+                                                    C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.or_surface_mesh_qualities.argtypes = [C.POINTER(OrMesh), _fp]
+        L.or_surface_mesh_min_quality.restype = C.c_float
+        L.or_surface_mesh_min_quality.argtypes = [C.POINTER(OrMesh)]
+        L.or_laplace.argtypes = [C.POINTER(OrMesh), C.c_int]
+        for n in ("or_q_tri3", "or_q_quad"):
+            getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp]
+        for n in ("or_q_tri3n", "or_q_quadn"):
+            getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp, _fp]
+        for n in ("or_q_tri364", "or_q_quad64"):
+            getattr(L, n).restype = C.c_double; getattr(L, n).argtypes = [_fp]
         L.or_surface_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.or_surface_nsectors.restype = C.c_int
         L.or_surface_nsectors.argtypes = [C.c_void_p, C.c_int]
@@ -114,10 +127,15 @@ def cbrt(x: float) -> float:
     return lib().or_cbrt(float(x))
 
 
-def q_elem(kind: str, pts, fp64: bool = False) -> float:
+def q_elem(kind: str, pts, fp64: bool = False, normal=None) -> float:
+    """kind: tet, hex, tri (2D), tri3 (surface triangle), quad (surface quad);
+    normal: signed surface variants (tri3 / quad)"""
     a, p = _f(pts)
-    name = {"tet": "or_q_tet", "hex": "or_q_hex", "tri": "or_q_tri2d"}[kind] + ("64" if fp64 else "")
-    return getattr(lib(), name)(p)
+    name = {"tet": "or_q_tet", "hex": "or_q_hex", "tri": "or_q_tri2d", "tri3": "or_q_tri3", "quad": "or_q_quad"}[kind]
+    if normal is not None:
+        n, npp = _f(normal)
+        return getattr(lib(), name + "n")(p, npp)
+    return getattr(lib(), name + ("64" if fp64 else ""))(p)
 
 
 def ring_score(kind: str, ring, x0, sigma, p) -> float:
@@ -259,6 +277,26 @@ class OracleMesh:
         self.last_min = self.min_quality()
         return a.value, b.value
 
+    def iterate_surface_mesh(self, nq=8, depth=3, grid_scale=0.5, shrink=0.0, fixed_boundary=0, nthreads=0):
+        """Surface mesh (tri / quad) on the surface path: 2D passes, 1D passes,
+        NN and normal updates (reference DMOTriClass / DMOQuadClass
+        doIteration); returns (2D, 1D) counts."""
+        p = DmoParams(nq, depth, grid_scale, shrink, OR_EXTENT_MAX, fixed_boundary, nthreads)
+        a, b = C.c_longlong(), C.c_longlong()
+        lib().or_dmo_iteration_surface_mesh(self.ptr, self.surf, C.byref(p), C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def surface_mesh_qualities(self) -> np.ndarray:
+        out = np.empty(self.m.nE, dtype=np.float32)
+        lib().or_surface_mesh_qualities(self.ptr, out.ctypes.data_as(_fp))
+        return out
+
+    def surface_mesh_min_quality(self) -> float:
+        return lib().or_surface_mesh_min_quality(self.ptr)
+
+    def laplace(self, n_iter: int):
+        lib().or_laplace(self.ptr, int(n_iter))
+
     def surface_points(self, vid: int, nu: int, nv: int, mode: int, sid: int = 0, grid_scale: float = 0.5):
         out = np.zeros((nu * nv, 3), dtype=np.float32)
         lib().or_surface_points(self.ptr, self.surf, vid, nu, nv, mode, sid, C.c_float(grid_scale),
@@ -273,6 +311,8 @@ class OracleMesh:
             out = np.empty((n, 16), np.float32)
         elif what == 2:
             out = np.empty(n, np.uint8)
+        elif what == 4:
+            out = np.empty((n, 3), np.float32)
         else:
             out = np.empty((n, 10), np.float32)
         lib().or_surface_get(self.surf, what, out.ctypes.data_as(C.c_void_p))

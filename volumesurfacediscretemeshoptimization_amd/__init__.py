@@ -3,14 +3,17 @@
 Python mirror of the reference's mesh/optimiser interface over the C ABI of
 libvsdmo.so (include/vsdmo.h):
 
-* ``DMOMeshTriFactory.create`` / ``DMOMeshTetFactory.create`` /
-  ``DMOMeshHexFactory.create`` (reference VSDMO/src/mesh/MeshFactory.h:17-41)
-  build the device mesh: boundary faces, renumbering, colouring, one-rings.
-* ``DMOTetClass`` / ``DMOHexClass`` / ``DMOTriFlatClass``
-  (VSDMO/src/dmo/DMOTet.h:24, DMOHex.h:18, DMOTriFlat.h:25) with the virtual
-  interface of ``DMOBaseClass`` (VSDMO/src/dmo/DMOBase.h:23-88):
-  ``optimize``, ``doIteration``, ``isDone``, ``getQuality``,
-  ``getQualityHistogram``, ``getElementQualities``, ``displayQualityGPU``.
+* ``DMOMeshTriFactory.create`` / ``DMOMeshQuadFactory.create`` /
+  ``DMOMeshTetFactory.create`` / ``DMOMeshHexFactory.create`` (reference
+  VSDMO/src/mesh/MeshFactory.h:17-41) build the device mesh: boundary faces,
+  renumbering, colouring, one-rings, surface halfedges.
+* ``DMOTetClass`` / ``DMOHexClass`` / ``DMOTriFlatClass`` / ``DMOTriClass`` /
+  ``DMOQuadClass`` (VSDMO/src/dmo/DMOTet.h:24, DMOHex.h:18, DMOTriFlat.h:25,
+  DMOTri.h:24, DMOQuad.h:18) with the virtual interface of ``DMOBaseClass``
+  (VSDMO/src/dmo/DMOBase.h:23-88): ``optimize``, ``doIteration``,
+  ``isDone``, ``getQuality``, ``getQualityHistogram``,
+  ``getElementQualities``, ``displayQualityGPU``.
+* ``laplaceSmoothing(mesh, n_iter)`` (VSDMO/src/dmo/Laplace.h:10).
 
 All arithmetic runs in the HIP kernels of libvsdmo.so; this module only moves
 host arrays across the ABI.  A missing library raises; there is no fallback.
@@ -26,9 +29,9 @@ from ._lib import VsdmoError, check, load  # noqa: F401
 
 MEAN_RATIO = 0
 
-__all__ = ["read_mesh_file", "write_mesh_file", "DMOMeshTriFactory", "DMOMeshTetFactory", "DMOMeshHexFactory", "DMOMesh",
-           "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "DMOBatch", "DMOMLP", "DMODataset", "VsdmoError",
-           "MEAN_RATIO", "load"]
+__all__ = ["read_mesh_file", "write_mesh_file", "DMOMeshTriFactory", "DMOMeshQuadFactory", "DMOMeshTetFactory",
+           "DMOMeshHexFactory", "DMOMesh", "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "DMOTriClass", "DMOQuadClass",
+           "laplaceSmoothing", "DMOBatch", "DMOMLP", "DMODataset", "VsdmoError", "MEAN_RATIO", "load"]
 
 
 def _fptr(a):
@@ -42,7 +45,7 @@ def _iptr(a):
 class DMOMesh:
     """Device mesh (owns the vsdmo_mesh handle)."""
 
-    NPE = {_lib.VSDMO_TRI: 3, _lib.VSDMO_TET: 4, _lib.VSDMO_HEX: 8}
+    NPE = {_lib.VSDMO_TRI: 3, _lib.VSDMO_QUAD: 4, _lib.VSDMO_TET: 4, _lib.VSDMO_HEX: 8}
 
     def __init__(self, elem: int, xyz, cells, coloring: int = _lib.VSDMO_COLOR_GREEDY, feature_angle: float = 20.0):
         lib = load()
@@ -204,6 +207,10 @@ class DMOMeshTriFactory(_Factory):
     ELEM = _lib.VSDMO_TRI
 
 
+class DMOMeshQuadFactory(_Factory):
+    ELEM = _lib.VSDMO_QUAD
+
+
 class DMOMeshTetFactory(_Factory):
     ELEM = _lib.VSDMO_TET
 
@@ -215,10 +222,13 @@ class DMOMeshHexFactory(_Factory):
 class _DMOBase:
     """Reference DMOBaseClass (VSDMO/src/dmo/DMOBase.h:23-88)."""
 
+    SURFACE_PATH = _lib.VSDMO_SURFACE_PATH_AUTO
+
     def __init__(self, mesh: DMOMesh, qualityCriterium: int = MEAN_RATIO, gridScale: float = 0.5,
                  n_iter: int = 30, nq: int = 8, depth: int = 3, shrink: float = 0.0,
                  extent_mode: int = _lib.VSDMO_EXTENT_AUTO, fixed_boundary: bool = False,
-                 quality_guard: bool = True, guard_tolerance: float = 1e-5, waves_per_vertex: int = 1):
+                 quality_guard: bool = True, guard_tolerance: float = 1e-5, waves_per_vertex: int = 1,
+                 surface_path: int = None):
         lib = load()
         self._lib = lib
         self.mesh = mesh
@@ -227,6 +237,7 @@ class _DMOBase:
         p.nq, p.depth, p.shrink, p.extent_mode = nq, depth, shrink, extent_mode
         p.fixed_boundary, p.quality_guard, p.guard_tolerance = int(fixed_boundary), int(quality_guard), guard_tolerance
         p.waves_per_vertex = int(waves_per_vertex)
+        p.surface_path = self.SURFACE_PATH if surface_path is None else int(surface_path)
         h = C.c_void_p()
         check(lib.vsdmo_opt_create(mesh.handle, qualityCriterium, gridScale, n_iter, C.byref(p), C.byref(h)),
               "vsdmo_opt_create")
@@ -424,7 +435,25 @@ class DMOHexClass(_DMOBase):
 
 
 class DMOTriFlatClass(_DMOBase):
-    pass
+    """Flat triangle meshes: 2D search in the plane (DMOTriFlat.h:25)."""
+    SURFACE_PATH = _lib.VSDMO_SURFACE_PATH_FLAT
+
+
+class DMOTriClass(_DMOBase):
+    """Triangle surface meshes: free vertices move on their local surfaces,
+    feature vertices along their feature curves (DMOTri.h:24)."""
+    SURFACE_PATH = _lib.VSDMO_SURFACE_PATH_SURFACE
+
+
+class DMOQuadClass(_DMOBase):
+    """Quad surface meshes (DMOQuad.h:18)."""
+    SURFACE_PATH = _lib.VSDMO_SURFACE_PATH_SURFACE
+
+
+def laplaceSmoothing(mesh: DMOMesh, n_iter: int):
+    """Laplace smoothing of the free vertices of a triangle / quad mesh,
+    colour class by colour class (reference dmo/Laplace.cu:30-41)."""
+    check(mesh._lib.vsdmo_mesh_laplace(mesh.handle, int(n_iter)), "vsdmo_mesh_laplace")
 
 
 class DMOBatch:

@@ -22,6 +22,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "HIP_ERROR", 3: "TOPOLOGY", 4: "QU
 VSDMO_TRI, VSDMO_QUAD, VSDMO_TET, VSDMO_HEX = 0, 1, 2, 3
 VSDMO_COLOR_GREEDY, VSDMO_COLOR_JPL = 0, 1
 VSDMO_EXTENT_AUTO, VSDMO_EXTENT_MAX, VSDMO_EXTENT_MIN = -1, 0, 1
+VSDMO_SURFACE_PATH_AUTO, VSDMO_SURFACE_PATH_SURFACE, VSDMO_SURFACE_PATH_FLAT = 0, 1, 2
 
 
 class MeshParams(C.Structure):
@@ -31,7 +32,7 @@ class MeshParams(C.Structure):
 class Params(C.Structure):
     _fields_ = [("nq", C.c_int32), ("depth", C.c_int32), ("shrink", C.c_float), ("extent_mode", C.c_int32),
                 ("fixed_boundary", C.c_int32), ("quality_guard", C.c_int32), ("guard_tolerance", C.c_float),
-                ("waves_per_vertex", C.c_int32)]
+                ("waves_per_vertex", C.c_int32), ("surface_path", C.c_int32)]
 
 
 class Counts(C.Structure):
@@ -69,6 +70,7 @@ EXPORTS = {
     "vsdmo_mesh_get_positions": (C.c_int, [C.c_void_p, _fp, C.c_int32]),
     "vsdmo_mesh_set_positions": (C.c_int, [C.c_void_p, _fp, C.c_int32]),
     "vsdmo_mesh_get_permutation": (C.c_int, [C.c_void_p, _ip]),
+    "vsdmo_mesh_laplace": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsdmo_mesh_get_cells": (C.c_int, [C.c_void_p, _ip]),
     "vsdmo_mesh_get_coloring": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
     "vsdmo_mesh_get_colors": (C.c_int, [C.c_void_p, _ip]),

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -94,7 +95,10 @@ struct vsdmo_opt {
     // surface-vertex state (fixed_boundary == 0)
     bool surface = false;
     int nSectors = 0;
-    DevBuf<float> pinit, fnorm, ls, ls1, fs, snapx, snapy, snapz;
+    DevBuf<float> pinit, fnorm, vnorm, ls, ls1, fs, snapx, snapy, snapz;
+    // surface meshes (tri / quad) on the surface path (DMOTriClass / DMOQuadClass)
+    bool surf_mesh = false;
+    int qelem = 0;                  // element kind of the quality kernels
     DevBuf<int> nn, frow, table;
     DevBuf<uint8_t> opt1d;
     std::vector<uint8_t> opt1d_host;
@@ -128,6 +132,7 @@ void vsdmo_params_default(vsdmo_params* p)
     p->quality_guard = 1;
     p->guard_tolerance = 1e-5f;
     p->waves_per_vertex = 1;
+    p->surface_path = VSDMO_SURFACE_PATH_AUTO;
 }
 
 const char* vsdmo_last_error(void) { return g_err.c_str(); }
@@ -140,9 +145,10 @@ vsdmo_status vsdmo_mesh_create(vsdmo_elem type, const float* xyz, int32_t nV, co
         *out = nullptr;
         if (!xyz || !cells || nV <= 0 || nCells <= 0) return fail(VSDMO_INVALID_ARG, "empty or NULL mesh arrays");
         if (boundary_or_null) return fail(VSDMO_NOT_SUPPORTED, "explicit boundary flags are not supported; pass NULL");
-        if (type != VSDMO_TET && type != VSDMO_HEX && type != VSDMO_TRI)
-            return fail(VSDMO_NOT_SUPPORTED, "element type not supported (tri, tet, hex)");
-        int npe = (type == VSDMO_TRI) ? 3 : (type == VSDMO_TET) ? 4 : 8;
+        if (type != VSDMO_TET && type != VSDMO_HEX && type != VSDMO_TRI && type != VSDMO_QUAD)
+            return fail(VSDMO_INVALID_ARG, "unknown element type");
+        int npe = (type == VSDMO_TRI) ? 3 : (type == VSDMO_TET || type == VSDMO_QUAD) ? 4 : 8;
+        const bool surfaceMesh = (type == VSDMO_TRI || type == VSDMO_QUAD);
         long long nci = (long long)nCells * npe;
         if (nci > (long long)0x7fffffff / 2) return fail(VSDMO_INVALID_ARG, "mesh too large for int32 indexing");
         for (long long i = 0; i < nci; ++i)
@@ -169,11 +175,7 @@ vsdmo_status vsdmo_mesh_create(vsdmo_elem type, const float* xyz, int32_t nV, co
                 hx[i] = xyz[3 * i]; hy[i] = xyz[3 * i + 1]; hz[i] = xyz[3 * i + 2];
                 if (hz[i] != 0.0f) flat = 0;
             }
-            m->flat = (type == VSDMO_TRI) ? flat : 0;
-            if (type == VSDMO_TRI && !flat) {
-                destroy_mesh(m);
-                return fail(VSDMO_NOT_SUPPORTED, "non-flat triangle surfaces are not supported (flat 2D meshes only)");
-            }
+            m->flat = surfaceMesh ? flat : 0;
             m->x.alloc(nV); m->y.alloc(nV); m->z.alloc(nV);
             m->x.upload(hx.data(), nV, s); m->y.upload(hy.data(), nV, s); m->z.upload(hz.data(), nV, s);
             m->cells.alloc(nci);
@@ -187,7 +189,7 @@ vsdmo_status vsdmo_mesh_create(vsdmo_elem type, const float* xyz, int32_t nV, co
                 m->newOfOld.upload(id.data(), nV, s);
                 VSDMO_HIP(hipStreamSynchronize(s));
             }
-            if (type == VSDMO_TRI) build_surface_mesh(*m, prm);
+            if (surfaceMesh) build_surface_mesh(*m, prm);
             else build_volume(*m, prm);
             m->new_of_old = m->newOfOld.to_host(s);
             if (m->maxRing > max_ring_supported(type) || m->maxHE > 1024) {
@@ -252,6 +254,23 @@ vsdmo_status vsdmo_mesh_set_positions(vsdmo_mesh* m, const float* in, int32_t or
         m->x.upload(hx.data(), m->nV, m->stream);
         m->y.upload(hy.data(), m->nV, m->stream);
         m->z.upload(hz.data(), m->nV, m->stream);
+        VSDMO_HIP(hipStreamSynchronize(m->stream));
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_mesh_laplace(vsdmo_mesh* m, int32_t n_iter)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!m) return fail(VSDMO_INVALID_ARG, "NULL mesh");
+        if (n_iter < 0) return fail(VSDMO_INVALID_ARG, "negative iteration count");
+        if (m->elem != VSDMO_TRI && m->elem != VSDMO_QUAD)
+            return fail(VSDMO_NOT_SUPPORTED, "Laplace smoothing is for triangle and quad surface meshes");
+        for (int it = 0; it < n_iter; ++it)
+            for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c)
+                laplace_launch(m->x.p, m->y.p, m->z.p, m->heRow.p, m->heTarget.p, m->colOffFree[c], m->colOffFree[c + 1],
+                               m->stream);
+        VSDMO_HIP(hipGetLastError());
         VSDMO_HIP(hipStreamSynchronize(m->stream));
         return VSDMO_OK;
     });
@@ -342,7 +361,7 @@ vsdmo_status vsdmo_mesh_get_flags(const vsdmo_mesh* m, uint8_t* b1d, uint8_t* fe
 static void opt_quality(vsdmo_opt* o)
 {
     vsdmo_mesh* m = o->m;
-    QualityArgs qa{ m->x.p, m->y.p, m->z.p, m->cells.p, m->nE, m->npe, m->elem, o->q.p };
+    QualityArgs qa{ m->x.p, m->y.p, m->z.p, m->cells.p, m->nE, m->npe, o->qelem, o->q.p };
     launch_quality(qa, o->pmin.p, o->psum.p, o->omin.p, o->osum.p, m->stream);
     VSDMO_HIP(hipGetLastError());
     float mn = 0.0f;
@@ -362,13 +381,13 @@ static SurfArgs surf_args(vsdmo_opt* o)
     a.xw = m->x.p; a.yw = m->y.p; a.zw = m->z.p;
     a.sx = o->snapx.p; a.sy = o->snapy.p; a.sz = o->snapz.p;
     a.ringRow = m->ringRow.p; a.ringVal = m->ringVal.p;
-    a.heRow = m->heRow.p; a.heTarget = m->heTarget.p; a.heFace = m->heFace.p;
+    a.heRow = m->heRow.p; a.heTarget = m->heTarget.p; a.heFace = m->heFace.p; a.heNext = m->heNext.p;
     a.faces = m->faces.p;
     a.isFeature = m->isFeature.p; a.isB1D = m->isB1D.p;
     a.nS = m->nSurf; a.nFree = m->nFree; a.nFeature = m->nFeature; a.nFaces = m->nFaces; a.fpe = m->fpe;
     a.flat = m->flat;
     a.cos160 = (float)std::cos(160.0 * 3.14159265358979323846 / 180.0);
-    a.pinit = o->pinit.p; a.fnorm = o->fnorm.p; a.ls = o->ls.p; a.ls1 = o->ls1.p; a.fs = o->fs.p;
+    a.pinit = o->pinit.p; a.fnorm = o->fnorm.p; a.vnorm = o->vnorm.p; a.ls = o->ls.p; a.ls1 = o->ls1.p; a.fs = o->fs.p;
     a.nn = o->nn.p; a.frow = o->frow.p; a.table = o->table.p; a.opt1d = o->opt1d.p;
     a.gridScale = o->grid_scale; a.shrink = o->shrink; a.aff = o->aff; a.depth = o->p.depth;
     return a;
@@ -381,7 +400,7 @@ static void surface_init(vsdmo_opt* o)
     vsdmo_mesh* m = o->m;
     hipStream_t s = m->stream;
     const int nS = m->nSurf, nF = m->nFeature;
-    o->pinit.alloc((size_t)3 * nS); o->fnorm.alloc((size_t)3 * std::max(m->nFaces, 1));
+    o->pinit.alloc((size_t)3 * nS); o->fnorm.alloc((size_t)3 * std::max(m->nFaces, 1)); o->vnorm.alloc((size_t)3 * nS);
     o->ls.alloc((size_t)16 * nS); o->ls1.alloc((size_t)10 * nS);
     o->nn.alloc(nS); o->opt1d.alloc(nS);
     o->frow.alloc((size_t)nF + 1); o->table.alloc((size_t)20 * std::max(nF, 1));
@@ -399,6 +418,8 @@ static void surface_init(vsdmo_opt* o)
     o->frow.upload(frow.data(), nF + 1, s);
     o->nSectors = frow[nF];
     o->fs.alloc((size_t)16 * std::max(o->nSectors, 1));
+    // sectors with fewer than two points keep a zero record, as in the oracle
+    VSDMO_HIP(hipMemsetAsync(o->fs.p, 0, sizeof(float) * 16 * std::max(o->nSectors, 1), s));
     a = surf_args(o);
     if (nF > 0) {
         DevBuf<int> secN((size_t)40 * std::max(o->nSectors, 1)), secCnt(std::max(o->nSectors, 1));
@@ -428,7 +449,16 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
         if (p.depth < 1 || p.depth > 16) return fail(VSDMO_INVALID_ARG, "depth must be in [1,16]");
         if (!waves_supported(p.nq, p.waves_per_vertex))
             return fail(VSDMO_INVALID_ARG, "waves_per_vertex must be 0 (auto), 1, 2 or 4 and divide nq");
-        if (!p.fixed_boundary && (m->maxHE > 40 || m->nSurf == 0))
+        if (p.surface_path < VSDMO_SURFACE_PATH_AUTO || p.surface_path > VSDMO_SURFACE_PATH_FLAT)
+            return fail(VSDMO_INVALID_ARG, "unknown surface_path");
+        const bool surfaceMesh = (m->elem == VSDMO_TRI || m->elem == VSDMO_QUAD);
+        bool surfMesh = false;   // the surface path of a tri / quad mesh
+        if (surfaceMesh) {
+            if (p.surface_path == VSDMO_SURFACE_PATH_FLAT && !(m->elem == VSDMO_TRI && m->flat))
+                return fail(VSDMO_NOT_SUPPORTED, "the flat path needs a flat triangle mesh (z == 0)");
+            surfMesh = p.surface_path == VSDMO_SURFACE_PATH_SURFACE || !(m->elem == VSDMO_TRI && m->flat);
+        }
+        if ((!p.fixed_boundary || surfMesh) && (m->maxHE > 40 || m->nSurf == 0))
             return fail(VSDMO_NOT_SUPPORTED, "surface optimisation needs surface one-rings of at most 40 vertices");
         auto* o = new vsdmo_opt();
         o->m = m;
@@ -441,12 +471,14 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
         int em = p.extent_mode;
         if (em == VSDMO_EXTENT_AUTO) em = (m->elem == VSDMO_HEX) ? VSDMO_EXTENT_MIN : VSDMO_EXTENT_MAX;
         o->extent_min = (em == VSDMO_EXTENT_MIN);
+        o->surf_mesh = surfMesh;
+        o->qelem = surfMesh ? (m->elem == VSDMO_QUAD ? VSDMO_QUAD : kElemTri3) : m->elem;
         try {
             int nb = (m->nE + 255) / 256;
             o->q.alloc(m->nE);
             o->pmin.alloc(nb); o->psum.alloc(nb);
             o->omin.alloc(1); o->osum.alloc(1);
-            if (!p.fixed_boundary) surface_init(o);
+            if (!p.fixed_boundary || surfMesh) surface_init(o);
             opt_quality(o);
         } catch (...) {
             delete o;
@@ -503,6 +535,24 @@ static void timed_learned(vsdmo_opt* o, int kind, const SearchArgs& a, size_t ld
         VSDMO_HIP(hipEventRecord(e1, s));
         o->ev.push_back(e0);
         o->ev.push_back(e1);
+    }
+    o->launches++;
+}
+
+static void timed_surface(vsdmo_opt* o, const std::function<void()>& launch)
+{
+    hipStream_t s = o->m->stream;
+    if (o->timing) {
+        hipEvent_t e0, e1;
+        VSDMO_HIP(hipEventCreate(&e0));
+        VSDMO_HIP(hipEventCreate(&e1));
+        VSDMO_HIP(hipEventRecord(e0, s));
+        launch();
+        VSDMO_HIP(hipEventRecord(e1, s));
+        o->ev.push_back(e0);
+        o->ev.push_back(e1);
+    } else {
+        launch();
     }
     o->launches++;
 }
@@ -582,6 +632,32 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                     }
                 }
                 if (o->surface) surf_nn_launch(surf_args(o), m->stream);
+            } else if (o->surf_mesh) {
+                // surface mesh on the surface path (DMOTri.cu:128-170, DMOQuad.cu:128-170):
+                // free 2D passes, feature 1D passes, NN walk, normals; no quality gate
+                SurfArgs sa = surf_args(o);
+                for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {
+                    int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
+                    VSDMO_HIP(hipMemcpyAsync(o->snapx.p, m->x.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                    VSDMO_HIP(hipMemcpyAsync(o->snapy.p, m->y.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                    VSDMO_HIP(hipMemcpyAsync(o->snapz.p, m->z.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                    timed_surface(o, [&]() { surf_fan2d_launch(sa, lo, hi, o->p.nq, m->stream); });
+                    o->s_surf += hi - lo;
+                }
+                if (!o->p.fixed_boundary) {
+                    for (size_t c = 0; c + 1 < m->colOffFeature.size(); ++c) {
+                        int lo = m->colOffFeature[c], hi = m->colOffFeature[c + 1];
+                        if (m->elem == VSDMO_QUAD) {   // quad fan diagonals read the pass snapshot
+                            VSDMO_HIP(hipMemcpyAsync(o->snapx.p, m->x.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                            VSDMO_HIP(hipMemcpyAsync(o->snapy.p, m->y.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                            VSDMO_HIP(hipMemcpyAsync(o->snapz.p, m->z.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                        }
+                        timed_surface(o, [&]() { surf_fan1d_launch(sa, lo, hi, o->p.nq, m->stream); });
+                        for (int v = lo; v < hi; ++v) o->s_feat += o->opt1d_host[v];
+                    }
+                }
+                surf_nn_launch(sa, m->stream);
+                surf_normals_launch(sa, m->stream);
             } else {
                 size_t lds = search_lds_bytes(VSDMO_TRI, m->maxHE);
                 for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {

@@ -350,6 +350,10 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
             if (!meshes[i]) return fail_status(VSDMO_INVALID_ARG, "mesh " + std::to_string(i) + " is NULL");
             for (int j = 0; j < i; ++j)
                 if (meshes[j] == meshes[i]) return fail_status(VSDMO_INVALID_ARG, "a mesh appears twice in the batch");
+            const vsdmo_mesh* m = meshes[i];
+            if (m->elem == VSDMO_QUAD || (m->elem == VSDMO_TRI && !m->flat))
+                return fail_status(VSDMO_NOT_SUPPORTED, "batches hold tet, hex and flat triangle meshes (mesh " +
+                                                            std::to_string(i) + ")");
         }
         auto* b = new vsdmo_batch();
         try {

@@ -24,6 +24,10 @@ struct SearchArgs {
     int extentMin;
 };
 
+// element kinds of the quality kernels beyond vsdmo_elem: triangles of a
+// surface mesh on the surface path (unsigned 3D mean ratio)
+constexpr int kElemTri3 = 4;
+
 struct QualityArgs {
     const float* x;
     const float* y;
@@ -49,6 +53,7 @@ struct SurfArgs {
     const int* heRow;
     const int* heTarget;
     const int* heFace;
+    const int* heNext;
     const int* faces;
     const uint8_t* isFeature;
     const uint8_t* isB1D;
@@ -57,6 +62,7 @@ struct SurfArgs {
     // owned by the optimiser
     float* pinit;     // 3*nS
     float* fnorm;     // 3*nFaces
+    float* vnorm;     // 3*nS vertex normals
     float* ls;        // 16*nS
     float* ls1;       // 10*nS
     float* fs;        // 16*nSectors
@@ -77,6 +83,13 @@ void surf_search2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int t
 void surf_search1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, int tet, size_t lds, hipStream_t s);
 void surf_flat1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s);
 void surf_points_launch(const SurfArgs& a, int vid, int nu, int nv, int mode, int sid, float* out, hipStream_t s);
+// surface meshes (tri / quad, chosen by a.fpe): free-vertex 2D and
+// feature-vertex 1D searches of one colour class, normals from the current
+// positions, Laplace smoothing of one colour class
+void surf_fan2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s);
+void surf_fan1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s);
+void surf_normals_launch(const SurfArgs& a, hipStream_t s);
+void laplace_launch(float* x, float* y, float* z, const int* heRow, const int* heTarget, int lo, int hi, hipStream_t s);
 
 // chunked per-mesh qualities of a batch
 struct BatchQArgs {

@@ -896,6 +896,11 @@ __device__ float elem_quality(const float* __restrict__ X, const float* __restri
         }
         return hex_acc_score(acc, kHexEps * s3, kHexEps * s2);
     }
+    if (elem == VSDMO_QUAD)
+        return mr_quad3(f3{ x0x, x0y, x0z }, f3{ X[c[1]], Y[c[1]], Z[c[1]] }, f3{ X[c[2]], Y[c[2]], Z[c[2]] },
+                        f3{ X[c[3]], Y[c[3]], Z[c[3]] }, nullptr);
+    if (elem == kElemTri3)
+        return mr_tri3(f3{ x0x, x0y, x0z }, f3{ X[c[1]], Y[c[1]], Z[c[1]] }, f3{ X[c[2]], Y[c[2]], Z[c[2]] }, nullptr);
     float m = 0.0f;
     for (int i = 1; i < 3; ++i) {
         m = fmaxf(m, fabsf(X[c[i]] - x0x));

@@ -1,10 +1,11 @@
 // mesh_io.cpp -- mesh files for the DMO factories (host code).
 //
 // Formats and their conventions follow the reference's readers and writers
-// (VSDMO/src/io/ParserOFF.cu:15-83, ParserTetGen.cu:13-120,
+// (VSDMO/src/io/ParserOFF.cu:15-83, 86-160, ParserTetGen.cu:13-120,
 // ParserHex.cu:115-240, FileWriter.cu:9-147):
-//   OFF      "OFF", then "nV nF nE", nV lines "x y z", nF lines "3 a b c";
-//            a mesh with any z != 0 is not flat;
+//   OFF      "OFF", then "nV nF nE", nV lines "x y z", nF lines "3 a b c"
+//            (triangles) or "4 a b c d" (quads); a mesh with any z != 0 is
+//            not flat;
 //   TetGen   <base>.node: "nV 3 nAttr nMarker", lines "i x y z [attrs]
 //            [marker]"; <base>.ele: "nT 4 nAttr", lines "i a b c d"; the
 //            numbering is 1-based when the first point index is 1; '#'
@@ -55,7 +56,7 @@ bool next_line(std::ifstream& in, std::string& line)
     return false;
 }
 
-std::string read_off(const std::string& path, MeshData& d)
+std::string read_off(const std::string& path, MeshData& d, int k)
 {
     std::ifstream in(path);
     if (!in.is_open()) return "cannot open " + path;
@@ -70,7 +71,7 @@ std::string read_off(const std::string& path, MeshData& d)
         s >> nV >> nF >> nE;
         if (!s || nV < 0 || nF < 0) return "bad counts in " + path;
     }
-    d.npe = 3;
+    d.npe = k;
     d.xyz.resize((size_t)nV * 3);
     for (long long i = 0; i < nV; ++i) {
         if (!next_line(in, line)) return "missing vertices in " + path;
@@ -80,13 +81,15 @@ std::string read_off(const std::string& path, MeshData& d)
         d.xyz[3 * i] = x; d.xyz[3 * i + 1] = y; d.xyz[3 * i + 2] = z;
         if (z != 0.0f) d.flat = false;
     }
-    d.cells.resize((size_t)nF * 3);
+    d.cells.resize((size_t)nF * k);
     for (long long i = 0; i < nF; ++i) {
         if (!next_line(in, line)) return "missing faces in " + path;
         std::istringstream s(line);
-        int k, a, b, c;
-        if (!(s >> k >> a >> b >> c) || k != 3) return "face line of " + path + " is not a triangle";
-        d.cells[3 * i] = a; d.cells[3 * i + 1] = b; d.cells[3 * i + 2] = c;
+        int kf = 0;
+        if (!(s >> kf) || kf != k)
+            return "face line of " + path + (k == 3 ? " is not a triangle" : " is not a quad");
+        for (int j = 0; j < k; ++j)
+            if (!(s >> d.cells[(size_t)k * i + j])) return "bad face line in " + path;
     }
     return "";
 }
@@ -200,7 +203,8 @@ std::string read_medit(const std::string& path, MeshData& d)
 
 std::string read_any(vsdmo_elem type, const std::string& path, MeshData& d)
 {
-    if (type == VSDMO_TRI) return read_off(path, d);
+    if (type == VSDMO_TRI) return read_off(path, d, 3);
+    if (type == VSDMO_QUAD) return read_off(path, d, 4);
     if (type == VSDMO_TET) return read_tetgen(path, d);
     if (type == VSDMO_HEX) return read_medit(path, d);
     return "unsupported element type";
@@ -210,12 +214,17 @@ std::string write_any(vsdmo_elem type, const std::string& path, const float* xyz
                       int32_t nC)
 {
     char buf[128];
-    if (type == VSDMO_TRI) {
+    if (type == VSDMO_TRI || type == VSDMO_QUAD) {
         std::FILE* f = std::fopen(path.c_str(), "w");
         if (!f) return "cannot write " + path;
         std::fprintf(f, "OFF\n%d %d 0\n", nV, nC);
         for (int i = 0; i < nV; ++i) std::fprintf(f, "%.9g %.9g %.9g\n", xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-        for (int i = 0; i < nC; ++i) std::fprintf(f, "3 %d %d %d\n", cells[3 * i], cells[3 * i + 1], cells[3 * i + 2]);
+        const int k = type == VSDMO_TRI ? 3 : 4;
+        for (int i = 0; i < nC; ++i) {
+            const int32_t* c = cells + (size_t)k * i;
+            if (k == 3) std::fprintf(f, "3 %d %d %d\n", c[0], c[1], c[2]);
+            else std::fprintf(f, "4 %d %d %d %d\n", c[0], c[1], c[2], c[3]);
+        }
         std::fclose(f);
         return "";
     }
@@ -316,7 +325,7 @@ vsdmo_status vsdmo_mesh_create_from_file(vsdmo_elem type, const char* path, cons
     int32_t nV = 0, nC = 0;
     vsdmo_status st = vsdmo_mesh_file_read(type, path, &nV, &nC, nullptr, nullptr);
     if (st != VSDMO_OK) return st;
-    const int npe = type == VSDMO_TRI ? 3 : type == VSDMO_TET ? 4 : 8;
+    const int npe = type == VSDMO_TRI ? 3 : (type == VSDMO_TET || type == VSDMO_QUAD) ? 4 : 8;
     std::vector<float> xyz((size_t)nV * 3);
     std::vector<int32_t> cells((size_t)nC * npe);
     st = vsdmo_mesh_file_read(type, path, &nV, &nC, xyz.data(), cells.data());

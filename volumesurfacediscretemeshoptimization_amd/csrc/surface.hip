@@ -7,7 +7,9 @@
 // surfaceP/SurfaceFeature.h:79-106, SurfaceFeature.cu:12-223,
 // Surface1D.h:13-114, Surface1D.cu:7-103, SurfaceEstimation.h:122-197,
 // DMOCommon.cu:11-43, DMOCommon.h:130-165, DMOHexImplCub.h:141-489,
-// DMOTetImplCub.h:141-494.  Expressions follow the reference literally
+// DMOTetImplCub.h:141-494; surface meshes DMOTriImplCub.h:25-380,
+// DMOQuadImplCub.h:23-404, DMOQualityTri.h:257-275, DMOQualityQuad.h:30-50,
+// MeshGPUCommon.cu:413-434 (vertex normals), Laplace.cu:8-41.  Expressions follow the reference literally
 // (dot = (x*x' + y*y') + z*z', normalize divides by sqrtf(dot)) with
 // contraction off, so a host restatement of the same expressions reproduces
 // every value.
@@ -138,6 +140,7 @@ __global__ void k_surf_vertex_init(SurfArgs a)
     for (int h = a.heRow[v]; h < a.heRow[v + 1]; ++h)
         if (a.heFace[h] != -1) n = wadd(n, W(a.fnorm + 3 * a.heFace[h]));
     n = wnormalize(n);
+    a.vnorm[3 * v] = n.x; a.vnorm[3 * v + 1] = n.y; a.vnorm[3 * v + 2] = n.z;
     a.pinit[3 * v] = a.x[v]; a.pinit[3 * v + 1] = a.y[v]; a.pinit[3 * v + 2] = a.z[v];
     a.nn[v] = v;
     w3 pv = WP(a.x, a.y, a.z, v);
@@ -511,27 +514,69 @@ __device__ __forceinline__ void wave_argmax_s(float& best, int& idx)
     }
 }
 
-// ------------------------------------------------------------ 2D surface search
-// Lane l holds reference thread t = l/2's candidate A (l even) or B (l odd):
-// A = (t/nq, t%nq), B = (t/nq + nq/2, t%nq); the surface kernels keep A on an
-// exact tie, so A ranks first.  Window half-width s_d * (min distance to the
-// ring points for hexes, max for tets) in the nearest neighbour's (u,v).
-__global__ __launch_bounds__(64) void k_search_surf2d(SurfArgs a, int cStart, int nq, int tet)
+// ------------------------------------------------------------- search bodies
+// Scorers: score(p) is what the search maximises; final_ref(p0, q0) is the
+// start score the last level's maximum must beat for the write.
+
+// volume one-ring of a tet / hex surface vertex
+struct VolScore {
+    RingFrame f;
+    bool tet;
+    const void* lds;
+    HexAcc hacc;
+    __device__ float score(w3 p) const { return ring_score(f, tet, lds, hacc, p); }
+    __device__ float final_ref(w3, float q0) const { return q0; }
+};
+
+// fan of a surface-mesh vertex (DMOQualityTri.h:257-275, DMOQualityQuad.h:
+// 30-50): triangles (p, r_k, r_k+1), quads (p, r_k, d_k, r_k+1); score()
+// signs each element by the nearest neighbour's vertex normal n, final_ref()
+// uses the unsigned metric
+template <bool QUAD>
+struct FanScore {
+    const w3* pts;
+    int ne;
+    w3 n;
+    __device__ float elem(w3 p, int k, bool oriented) const
+    {
+        const f3 nn = { n.x, n.y, n.z };
+        const f3* no = oriented ? &nn : nullptr;
+        const f3 q = { p.x, p.y, p.z };
+        if (QUAD) {
+            const w3 a = pts[2 * k], b = pts[2 * k + 1], c = pts[2 * k + 2];
+            return mr_quad3(q, f3{ a.x, a.y, a.z }, f3{ b.x, b.y, b.z }, f3{ c.x, c.y, c.z }, no);
+        }
+        const w3 a = pts[k], b = pts[k + 1];
+        return mr_tri3(q, f3{ a.x, a.y, a.z }, f3{ b.x, b.y, b.z }, no);
+    }
+    __device__ float score(w3 p) const
+    {
+        float q = FLT_MAX;
+        for (int k = 0; k < ne; ++k) q = fminf(q, elem(p, k, true));
+        return q;
+    }
+    __device__ float final_ref(w3 p, float) const
+    {
+        float q = FLT_MAX;
+        for (int k = 0; k < ne; ++k) q = fminf(q, elem(p, k, false));
+        return q;
+    }
+};
+
+// 2D search on the blended local surfaces.  Lane l holds reference thread
+// t = l/2's candidate A (l even) or B (l odd): A = (t/nq, t%nq), B = (t/nq +
+// nq/2, t%nq); the surface kernels keep A on an exact tie, so A ranks first.
+// Window half-width s_d * dist in the nearest neighbour's (u,v).
+template <class S>
+__device__ void surf2d_search(const SurfArgs& a, int vid, w3 vidPos, float dist, int nq, const S& sc)
 {
-    extern __shared__ float s_lds[];
     const int lane = threadIdx.x;
-    const int vid = cStart + blockIdx.x;
-    HexAcc hacc = hex_acc0();
-    const RingFrame f = stage_ring(a, vid, tet != 0, s_lds, &hacc);
-    if (f.n <= 0) return;
-    const float dist = tet ? f.maxDist : f.minDist;
     const LSurf L = reinterpret_cast<const LSurf*>(a.ls)[a.nn[vid]];
-    const w3 vidPos = { f.x0x, f.x0y, f.x0z };
     w3 cur = vidPos;
     const w3 cvec = wsub(cur, L.p0);
     float cu = wdot(L.e[0], cvec), cv = wdot(L.e[1], cvec);
-    float qcur = ring_score(f, tet != 0, s_lds, hacc, cur);
-    const float qold = qcur;
+    float qcur = sc.score(cur);
+    const float qold = sc.final_ref(cur, qcur);
     float qlast = -FLT_MAX;
     const int t = lane >> 1;
     const bool isB = (lane & 1) != 0;
@@ -545,7 +590,7 @@ __global__ __launch_bounds__(64) void k_search_surf2d(SurfArgs a, int cStart, in
         if (active) {
             u = cu + a.aff * ((float)ci * uMin + (float)(nq - 1 - ci) * uMax);
             v = cv + a.aff * ((float)cj * uMin + (float)(nq - 1 - cj) * uMax);
-            best = ring_score(f, tet != 0, s_lds, hacc, project2d(a, vid, vidPos, u, v));
+            best = sc.score(project2d(a, vid, vidPos, u, v));
         }
         int wl = lane;
         wave_argmax_s(best, wl);
@@ -561,16 +606,13 @@ __global__ __launch_bounds__(64) void k_search_surf2d(SurfArgs a, int cStart, in
     if (qold < qlast && lane == 0) { a.xw[vid] = cur.x; a.yw[vid] = cur.y; a.zw[vid] = cur.z; }
 }
 
-// ------------------------------------------------------------ 1D feature search
-__global__ __launch_bounds__(64) void k_search_feat1d(SurfArgs a, int cStart, int nq, int tet)
+// 1D search along the feature curve: lane i < nq holds candidate i; the
+// curve projection blends the fits of the feature neighbours that have
+// exactly two feature neighbours themselves
+template <class S>
+__device__ void surf1d_search(const SurfArgs& a, int vid, w3 vidPos, float dist, int nq, const S& sc)
 {
-    extern __shared__ float s_lds[];
     const int lane = threadIdx.x;
-    const int vid = cStart + blockIdx.x;
-    if (!a.opt1d[vid]) return;   // uniform over the block
-    HexAcc hacc = hex_acc0();
-    const RingFrame f = stage_ring(a, vid, tet != 0, s_lds, &hacc);
-    if (f.n <= 0) return;
     int nb[kMaxNeigh], nnb = 0;
     for (int h = a.heRow[vid]; h < a.heRow[vid + 1]; ++h) {
         int d = a.heTarget[h];
@@ -579,13 +621,12 @@ __global__ __launch_bounds__(64) void k_search_feat1d(SurfArgs a, int cStart, in
         for (int h2 = a.heRow[d]; h2 < a.heRow[d + 1]; ++h2) if (a.isFeature[a.heTarget[h2]]) ++c;
         if (c == 2 && nnb < kMaxNeigh) nb[nnb++] = d;
     }
-    const float dist = tet ? f.maxDist : f.minDist;
     const LSurf1 L = reinterpret_cast<const LSurf1*>(a.ls1)[a.nn[vid]];
-    w3 cur = { f.x0x, f.x0y, f.x0z };
+    w3 cur = vidPos;
     const w3 cvec = wsub(cur, L.p0);
     float cu = wdot(L.e[0], cvec);
-    float qcur = ring_score(f, tet != 0, s_lds, hacc, cur);
-    const float qold = qcur;
+    float qcur = sc.score(cur);
+    const float qold = sc.final_ref(cur, qcur);
     float qlast = -FLT_MAX;
     const bool active = lane < nq;
     float ds = a.gridScale;
@@ -594,7 +635,7 @@ __global__ __launch_bounds__(64) void k_search_feat1d(SurfArgs a, int cStart, in
         float best = -FLT_MAX, u = 0.0f;
         if (active) {
             u = cu + a.aff * ((float)lane * uMin + (float)(nq - 1 - lane) * uMax);
-            best = ring_score(f, tet != 0, s_lds, hacc, project1d(a, vid, nb, nnb, u));
+            best = sc.score(project1d(a, vid, nb, nnb, u));
         }
         int wl = lane;
         wave_argmax_s(best, wl);
@@ -608,6 +649,131 @@ __global__ __launch_bounds__(64) void k_search_feat1d(SurfArgs a, int cStart, in
         ds = ds * a.shrink;
     }
     if (qold < qlast && lane == 0) { a.xw[vid] = cur.x; a.yw[vid] = cur.y; a.zw[vid] = cur.z; }
+}
+
+// ------------------------------------------------- tet / hex surface searches
+// Window half-width: s_d * (min distance to the ring points for hexes, max
+// for tets).  (k_optimizeHierarchical2DTet/Hex, 1DTet/Hex)
+__global__ __launch_bounds__(64) void k_search_surf2d(SurfArgs a, int cStart, int nq, int tet)
+{
+    extern __shared__ float s_lds[];
+    const int vid = cStart + blockIdx.x;
+    VolScore sc;
+    sc.hacc = hex_acc0();
+    sc.f = stage_ring(a, vid, tet != 0, s_lds, &sc.hacc);
+    if (sc.f.n <= 0) return;
+    sc.tet = tet != 0;
+    sc.lds = s_lds;
+    surf2d_search(a, vid, w3{ sc.f.x0x, sc.f.x0y, sc.f.x0z }, tet ? sc.f.maxDist : sc.f.minDist, nq, sc);
+}
+
+__global__ __launch_bounds__(64) void k_search_feat1d(SurfArgs a, int cStart, int nq, int tet)
+{
+    extern __shared__ float s_lds[];
+    const int vid = cStart + blockIdx.x;
+    if (!a.opt1d[vid]) return;   // uniform over the block
+    VolScore sc;
+    sc.hacc = hex_acc0();
+    sc.f = stage_ring(a, vid, tet != 0, s_lds, &sc.hacc);
+    if (sc.f.n <= 0) return;
+    sc.tet = tet != 0;
+    sc.lds = s_lds;
+    surf1d_search(a, vid, w3{ sc.f.x0x, sc.f.x0y, sc.f.x0z }, tet ? sc.f.maxDist : sc.f.minDist, nq, sc);
+}
+
+// ------------------------------------------------ surface-mesh (tri / quad)
+// Stage the fan of vid in LDS: lane i < n loads the halfedge target r_i (and
+// for quads the diagonal d_i, the target of the halfedge after vid -> r_i in
+// its face); closed (r_n = r_0) unless open.  Returns false (uniformly) when
+// the vertex has no fan element.  dist: the largest distance from vid to the
+// r_i (and d_i).  The d_i come from the pass snapshot: on a surface boundary
+// the "diagonal" of the boundary halfedge is the boundary neighbour of the
+// next boundary vertex, which can be in the colour class being updated;
+// every other fan vertex shares an element with vid, so the snapshot holds
+// its current position.
+template <bool QUAD>
+__device__ bool stage_fan(const SurfArgs& a, int vid, bool open, w3* pts, FanScore<QUAD>& sc, float& dist)
+{
+    const int lane = threadIdx.x;
+    const int h0 = a.heRow[vid], n = a.heRow[vid + 1] - h0;
+    if (n <= 0 || n > kMaxNeigh) return false;
+    const w3 x0 = WP(a.x, a.y, a.z, vid);
+    const int K = QUAD ? 2 : 1;
+    float mx = -FLT_MAX;
+    if (lane < n) {
+        const int h = h0 + lane;
+        const w3 r = WP(a.x, a.y, a.z, a.heTarget[h]);
+        pts[K * lane] = r;
+        mx = wnorm(wsub(r, x0));
+        if (QUAD) {
+            const int nx = a.heNext[h];
+            const w3 dg = WP(a.sx, a.sy, a.sz, a.heTarget[nx >= 0 ? nx : h]);
+            pts[K * lane + 1] = dg;
+            mx = fmaxf(mx, wnorm(wsub(dg, x0)));
+        }
+    }
+    if (lane == 0 && !open) pts[K * n] = WP(a.x, a.y, a.z, a.heTarget[h0]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m));
+    __syncthreads();
+    sc.pts = pts;
+    sc.ne = open ? n - 1 : n;
+    sc.n = W(a.vnorm + 3 * a.nn[vid]);
+    dist = mx;
+    return sc.ne > 0;
+}
+
+// free vertices: 2D search, fan always closed (k_optimizeHierarchical2D,
+// DMOTriImplCub.h:25-200, DMOQuadImplCub.h:23-212)
+template <bool QUAD>
+__global__ __launch_bounds__(64) void k_search_fan2d(SurfArgs a, int cStart, int nq)
+{
+    __shared__ w3 s_fan[2 * kMaxNeigh + 1];
+    const int vid = cStart + blockIdx.x;
+    FanScore<QUAD> sc;
+    float dist;
+    if (!stage_fan<QUAD>(a, vid, false, s_fan, sc, dist)) return;
+    surf2d_search(a, vid, WP(a.x, a.y, a.z, vid), dist, nq, sc);
+}
+
+// feature vertices: 1D search, fan open on a surface boundary
+// (k_optimizeHierarchical1D, DMOTriImplCub.h:205-378, DMOQuadImplCub.h:216-402)
+template <bool QUAD>
+__global__ __launch_bounds__(64) void k_search_fan1d(SurfArgs a, int cStart, int nq)
+{
+    __shared__ w3 s_fan[2 * kMaxNeigh + 1];
+    const int vid = cStart + blockIdx.x;
+    if (!a.opt1d[vid]) return;   // uniform over the block
+    FanScore<QUAD> sc;
+    float dist;
+    if (!stage_fan<QUAD>(a, vid, a.isB1D[vid] != 0, s_fan, sc, dist)) return;
+    surf1d_search(a, vid, WP(a.x, a.y, a.z, vid), dist, nq, sc);
+}
+
+// vertex normals from the current face normals (k_calcVertexNormals)
+__global__ void k_surf_vertex_normals(SurfArgs a)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.nS) return;
+    w3 n = { 0.0f, 0.0f, 0.0f };
+    for (int h = a.heRow[v]; h < a.heRow[v + 1]; ++h)
+        if (a.heFace[h] != -1) n = wadd(n, W(a.fnorm + 3 * a.heFace[h]));
+    n = wnormalize(n);
+    a.vnorm[3 * v] = n.x; a.vnorm[3 * v + 1] = n.y; a.vnorm[3 * v + 2] = n.z;
+}
+
+// Laplace smoothing of one colour class of free surface vertices
+// (k_laplaceSmoothing, Laplace.cu:8-28): p <- (p + sum of the halfedge
+// targets) * (1 / (1 + n))
+__global__ void k_laplace(float* x, float* y, float* z, const int* heRow, const int* heTarget, int lo, int hi)
+{
+    int v = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= hi) return;
+    const int h0 = heRow[v], n = heRow[v + 1] - h0;
+    w3 q = WP(x, y, z, v);
+    for (int h = h0; h < h0 + n; ++h) q = wadd(q, WP(x, y, z, heTarget[h]));
+    q = wmul(1.0f / (float)(1 + n), q);
+    x[v] = q.x; y[v] = q.y; z[v] = q.z;
 }
 
 // ------------------------------------------------------ 1D flat boundary search
@@ -771,6 +937,27 @@ void surf_points_launch(const SurfArgs& a, int vid, int nu, int nv, int mode, in
 void surf_flat1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s)
 {
     if (cEnd > cStart) k_search_flat1d<<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
+}
+void surf_fan2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s)
+{
+    if (cEnd <= cStart) return;
+    if (a.fpe == 4) k_search_fan2d<true><<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
+    else k_search_fan2d<false><<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
+}
+void surf_fan1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s)
+{
+    if (cEnd <= cStart) return;
+    if (a.fpe == 4) k_search_fan1d<true><<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
+    else k_search_fan1d<false><<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
+}
+void surf_normals_launch(const SurfArgs& a, hipStream_t s)
+{
+    k_surf_face_normals<<<nb256(a.nFaces), 256, 0, s>>>(a);
+    k_surf_vertex_normals<<<nb256(a.nS), 256, 0, s>>>(a);
+}
+void laplace_launch(float* x, float* y, float* z, const int* heRow, const int* heTarget, int lo, int hi, hipStream_t s)
+{
+    if (hi > lo) k_laplace<<<nb256(hi - lo), 256, 0, s>>>(x, y, z, heRow, heTarget, lo, hi);
 }
 
 }  // namespace vsdmo

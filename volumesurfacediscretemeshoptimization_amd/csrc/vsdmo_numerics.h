@@ -78,6 +78,53 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b)
 // dot: fmaf(x, x', fmaf(y, y', z * z'))
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
 
+// ------------------------------------------------- surface triangles / quads
+// Literal evaluation order of the reference's Vec3 (dot = (x*x' + y*y') +
+// z*z', norm = sqrtf(dot), normalized divides each component), contraction
+// off, for the surface-mesh metrics (DMOQualityTri.h:240-275,
+// DMOQualityQuad.h:13-50).
+__device__ __forceinline__ float ldot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float lnorm3(f3 a) { return sqrtf((a.x * a.x + a.y * a.y) + a.z * a.z); }
+__device__ __forceinline__ f3 lunit3(f3 a)
+{
+    const float l = lnorm3(a);
+    return { a.x / l, a.y / l, a.z / l };
+}
+// 2 sqrt(3), the FP32 product 2 * sqrtf(3)
+constexpr float kTwoSqrt3 = 3.46410155296325684f;
+
+// meanRatioMetricTri of (p0, p1, p2): 2 sqrt(3) A / (|e0|^2 + |e1|^2 + |e2|^2)
+// with e0 = p1 - p0, e1 = p2 - p0, e2 = p2 - p1 and A = |e0 x e1|, times
+// sign(n . unit(e0 x e1)) when n is given
+__device__ __forceinline__ float mr_tri3(f3 p0, f3 p1, f3 p2, const f3* n)
+{
+    const f3 e0 = sub3(p1, p0), e1 = sub3(p2, p0), e2 = sub3(p2, p1);
+    const float l = (ldot3(e0, e0) + ldot3(e1, e1)) + ldot3(e2, e2);
+    const f3 N = cross3(e0, e1);
+    float area = lnorm3(N);
+    if (n) {
+        const float d = ldot3(lunit3(N), *n);
+        area = area * (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f));
+    }
+    return (kTwoSqrt3 * area) / l;
+}
+
+// meanRatioMetricQuad of (p0, p1, p2, p3): 2 (|e1 x -e0| + |e3 x -e2|) /
+// sum |e_i|^2 with e_i = p_(i+1) - p_i; with n: -1 when a corner triangle's
+// unit normal u has n . u < 0.7
+__device__ __forceinline__ float mr_quad3(f3 p0, f3 p1, f3 p2, f3 p3, const f3* n)
+{
+    const f3 e0 = sub3(p1, p0), e1 = sub3(p2, p1), e2 = sub3(p3, p2), e3 = sub3(p0, p3);
+    const float l = ((ldot3(e0, e0) + ldot3(e1, e1)) + ldot3(e2, e2)) + ldot3(e3, e3);
+    const f3 N1 = cross3(e1, neg3(e0)), N2 = cross3(e3, neg3(e2));
+    const float a1 = lnorm3(N1), a2 = lnorm3(N2);
+    if (n) {
+        const float d1 = ldot3(lunit3(N1), *n), d2 = ldot3(lunit3(N2), *n);
+        if (d1 < 0.7f || d2 < 0.7f) return -1.0f;
+    }
+    return (2.0f * (a1 + a2)) / l;
+}
+
 // ---------------------------------------------------------------- tets
 // Ring element (a, b, c) of a vertex in the local frame.  With n = (b-a)x(c-a):
 //   det(p) = 6V = D - n.p,            D = n.a

@@ -21,6 +21,15 @@ Workloads (sizes as used by the benchmark and the tests):
 * ``batch_mix(256)``   256 meshes, sizes log-uniform in [500, 50000]; even
   index: tri grid with m = round(sqrt n) vertices per side; odd: tet cube with
   m = round(cbrt n) vertices per side; jitter +-0.35h.
+* ``cube_surface(n)``  surface of the unit cube, n x n quads per face
+  (outward), or each quad (a, b, c, d) split into (a, b, c), (a, c, d); face
+  vertices jittered +-jitter*h in both tangent directions, edge vertices
+  along their edge, corners fixed.  ``sphere=True`` maps the jittered cube
+  [-1, 1]^3 surface radially onto the unit sphere.
+* ``height_field(n)``  open surface z = 0.2 sin(1.5 pi x) cos(1.25 pi y) +
+  0.1 x y over an n x n vertex grid of [0, 1]^2 (quads, or the tri_grid
+  split); interior (x, y) jittered +-jitter*h, boundary vertices along their
+  side, corners fixed.
 """
 from __future__ import annotations
 
@@ -147,3 +156,66 @@ def batch_mix(n_meshes: int = 256, seed: int = 0, lo: int = 500, hi: int = 50000
             xyz, cells = tet_cube(m - 1, seed=seed + 1000 + b, jitter=jitter)
             out.append(("tet", xyz, cells))
     return out
+
+
+def cube_surface(n: int = 16, seed: int = 0, jitter: float = 0.3, quads: bool = True, sphere: bool = False):
+    """Closed cube (or cube-sphere) surface: (xyz float32, faces int32) with
+    quads (nf, 4) or triangles (nf, 3), consistently oriented outward."""
+    m = n + 1
+    i, j, k = _grid3(m)
+    lat = np.stack([i, j, k], axis=1)
+    on = (lat == 0) | (lat == n)
+    onb = on.any(axis=1)
+    gid = np.full(m ** 3, -1, dtype=np.int64)
+    gid[onb] = np.arange(int(onb.sum()))
+    r = _jitter(seed, m ** 3)
+    h = 1.0 / n
+    pos = lat.astype(np.float64) * h + np.where(~on, r * jitter * h, 0.0)
+    pos = pos[onb]
+    if sphere:
+        pos = 2.0 * pos - 1.0
+        pos = pos / np.linalg.norm(pos, axis=1, keepdims=True)
+    U, V = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    U, V = U.ravel(), V.ravel()
+    faces = []
+    for a in range(3):
+        b, c = (a + 1) % 3, (a + 2) % 3
+        for side in (0, n):
+            def corner(du, dv):
+                L = np.zeros((n * n, 3), dtype=np.int64)
+                L[:, a] = side
+                L[:, b] = U + du
+                L[:, c] = V + dv
+                return gid[L[:, 0] + m * (L[:, 1] + m * L[:, 2])]
+            q = np.stack([corner(0, 0), corner(1, 0), corner(1, 1), corner(0, 1)], axis=1)
+            if side == 0:
+                q = q[:, ::-1]
+            faces.append(q)
+    q = np.concatenate(faces)
+    if not quads:
+        q = np.stack([q[:, [0, 1, 2]], q[:, [0, 2, 3]]], axis=1).reshape(-1, 3)
+    return pos.astype(np.float32), q.astype(np.int32)
+
+
+def height_field(n_side: int = 32, seed: int = 0, jitter: float = 0.3, quads: bool = True):
+    """Open height-field surface with a boundary: (xyz float32, faces int32)."""
+    m = n_side
+    h = 1.0 / (m - 1)
+    j, i = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    bi = (i == 0) | (i == m - 1)
+    bj = (j == 0) | (j == m - 1)
+    r = _jitter(seed, m * m)
+    x = i * h + np.where(~bi, r[:, 0] * jitter * h, 0.0)
+    y = j * h + np.where(~bj, r[:, 1] * jitter * h, 0.0)
+    z = 0.2 * np.sin(1.5 * np.pi * x) * np.cos(1.25 * np.pi * y) + 0.1 * x * y
+    xyz = np.stack([x, y, z], axis=1).astype(np.float32)
+    cj, ci = np.meshgrid(np.arange(m - 1), np.arange(m - 1), indexing="ij")
+    v00 = (cj * m + ci).ravel()
+    v10, v01 = v00 + 1, v00 + m
+    v11 = v01 + 1
+    if quads:
+        cells = np.stack([v00, v10, v11, v01], axis=1)
+    else:
+        cells = np.stack([np.stack([v00, v10, v11], axis=1), np.stack([v00, v11, v01], axis=1)], axis=1).reshape(-1, 3)
+    return xyz, cells.astype(np.int32)
