@@ -5,7 +5,9 @@ interior jitter +-0.35h, boundary fixed, 8^3 candidates x 3 levels).  A step
 is one DMO iteration over the whole mesh: every colour class searched once
 (59,319 inner vertex searches for the tet cube).  Other workloads
 (--workload): hex48 = 48^3 hex cube with the surface free (117,649
-vertices), tri64 = 64 x 64 flat triangle grid (boundary fixed).
+vertices), tri64 = 64 x 64 flat triangle grid (boundary fixed), surf_tri /
+surf_quad = cube-sphere surface meshes of 96 x 96 cells per cube face
+(55,298 vertices; triangles or quads) through the surface path.
 
 The mesh is built and resident in HBM before the timed region; W warm-up
 iterations run on a separate copy of the mesh, then K iterations of a fresh
@@ -31,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "vertex updates/s at equal final min mean-ratio quality; learned vs plain DMO"
+CPU_BUDGET_S = 10.0
 PEAK_FP32_TFLOPS = 157.3
 # Algorithmic work of one vertex update: (nq^d + 1) ring evaluations per level
 # x levels + the final check = 1,540 (3D) / 196 (2D), times the one-ring size
@@ -39,12 +42,23 @@ PEAK_FP32_TFLOPS = 157.3
 # HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE x 1.99981 copy-kernel
 # calibration + WRITE_SIZE x 1.0, mean over the dispatches of 3 iterations,
 # profiles/k_search_tet_tet40.md and profiles/k_search_hex_hex48.md).
+# Surface-mesh searches (algorithmic FLOP per vertex update, valence n = 6
+# triangles / 4 quads): per candidate the projection onto the blended local
+# surfaces (57 + 69 n) plus the signed fan metric (59 n triangles, 104 n
+# quads); 3 levels x 64 candidates (2D) or 3 x 8 (1D, projection 45 + 52 * 2
+# curve neighbours).
+SURF_FLOP = {"tri": 3 * 64 * (57 + 69 * 6 + 59 * 6), "quad": 3 * 64 * (57 + 69 * 4 + 104 * 4)}
+SURF_FLOP1D = {"tri": 3 * 8 * (149 + 59 * 6), "quad": 3 * 8 * (149 + 104 * 4)}
 WORKLOADS = {
     "tet40": {"kind": "tet", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_search_tet<8>",
               "traffic": 10_970_146},
     "hex48": {"kind": "hex", "flop": 5_014_240, "cbrt": 110_880, "kernel": "k_search_hex<8>",
               "traffic": 17_570_841},
     "tri64": {"kind": "tri", "flop": 27_048, "cbrt": 0, "kernel": "k_search_flat<8>", "traffic": None},
+    "surf_tri": {"kind": "tri", "surface": True, "flop": SURF_FLOP["tri"], "flop1d": SURF_FLOP1D["tri"], "cbrt": 0,
+                 "kernel": "k_search_fan2d<false>", "traffic": None},
+    "surf_quad": {"kind": "quad", "surface": True, "flop": SURF_FLOP["quad"], "flop1d": SURF_FLOP1D["quad"], "cbrt": 0,
+                  "kernel": "k_search_fan2d<true>", "traffic": None},
     "batch256": {"kind": "batch", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_batch_round", "traffic": None},
 }
 # algorithmic FLOP of one flat-triangle vertex update (8x8 candidates: 196 ring
@@ -165,6 +179,15 @@ def make_mesh(name: str, seed: int):
                 "run only while the minimum quality is > 0); 8^3 candidates x 3 levels per inner vertex; one "
                 "step = one iteration over all colour classes")
         return xyz, cells, desc, {"fixed_boundary": False}
+    if name in ("surf_tri", "surf_quad"):
+        quads = name == "surf_quad"
+        xyz, cells = synth.cube_surface(96, seed=seed, jitter=0.3, quads=quads, sphere=True)
+        desc = (f"plain GPU DMO on a surface mesh: cube-sphere, 96 x 96 {'quads' if quads else 'quads split in two triangles'} "
+                f"per cube face ({len(xyz):,} vertices, {len(cells):,} {'quads' if quads else 'triangles'}), face "
+                f"vertices jittered +-0.3h tangentially before the radial projection, seed {seed}; free vertices "
+                "search 8 x 8 candidates x 3 levels on the blended local quadrics, feature vertices 8 x 3 on their "
+                "curves; one step = one iteration (all colour classes, nearest-neighbour walk, normal update)")
+        return xyz, cells, desc, {"fixed_boundary": False}
     if name == "tri64":
         xyz, cells = synth.tri_grid(64, seed=seed)
         desc = ("plain GPU DMO, flat triangle grid 64 x 64 vertices (7,938 triangles), interior jitter +-0.4h, "
@@ -173,23 +196,29 @@ def make_mesh(name: str, seed: int):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(kind, xyz, cells, iters: int, threads: int, surface: bool, name: str):
+def cpu_baseline(kind, xyz, cells, iters: int, threads: int, surface: bool, name: str, surface_mesh: bool = False):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py
     m = oracle_py.OracleMesh(kind, xyz, cells)
-    if surface:
+    if surface or surface_mesh:
         m.surface_init()
     t0 = time.perf_counter()
     n = 0
-    for _ in range(iters):
-        if surface:
+    done = 0
+    # whole iterations until the CPU time budget (iters > 0: exactly iters)
+    while (done < iters) if iters > 0 else (done == 0 or time.perf_counter() - t0 < CPU_BUDGET_S):
+        done += 1
+        if surface_mesh:
+            n2, n1 = m.iterate_surface_mesh(nthreads=threads)
+            n += n2 + n1
+        elif surface:
             n2, n1 = m.iterate_surface(nthreads=threads)
             n += m.m.nInner + n2 + n1
         else:
             n += m.iterate(nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "vertex updates/s", "cores": threads, "kind": "port",
-            "sample": f"{iters} full DMO iteration(s) of the same {name} mesh ({n} vertex searches, {dt:.1f} s) "
+            "sample": f"{done} full DMO iteration(s) of the same {name} mesh ({n} vertex searches, {dt:.1f} s) "
                       f"by the C/OpenMP oracle with {threads} threads"}
 
 
@@ -336,7 +365,8 @@ def main(argv=None):
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="tet40")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=0,
+                    help="CPU baseline iterations (0: whole iterations until about 10 s of CPU time)")
     ap.add_argument("--tune-iters", type=int, default=3, help="iterations per autotuner trial (batch256)")
     ap.add_argument("--mode", choices=["plain", "learned"], default="plain")
     ap.add_argument("--waves", type=int, default=1, help="wavefronts per vertex search (1, 2, 4)")
@@ -360,13 +390,16 @@ def main(argv=None):
     if args.mode == "learned":
         return bench_learned(args)
 
-    from volumesurfacediscretemeshoptimization_amd import (DMOHexClass, DMOMeshHexFactory, DMOMeshTetFactory,
-                                                           DMOMeshTriFactory, DMOTetClass, DMOTriFlatClass)
+    from volumesurfacediscretemeshoptimization_amd import (DMOHexClass, DMOMeshHexFactory, DMOMeshQuadFactory,
+                                                           DMOMeshTetFactory, DMOMeshTriFactory, DMOQuadClass,
+                                                           DMOTetClass, DMOTriClass, DMOTriFlatClass)
 
     wl = WORKLOADS[args.workload]
     kind = wl["kind"]
-    fac = {"tet": DMOMeshTetFactory, "hex": DMOMeshHexFactory, "tri": DMOMeshTriFactory}[kind]
-    cls = {"tet": DMOTetClass, "hex": DMOHexClass, "tri": DMOTriFlatClass}[kind]
+    surface_mesh = bool(wl.get("surface"))
+    fac = {"tet": DMOMeshTetFactory, "hex": DMOMeshHexFactory, "tri": DMOMeshTriFactory, "quad": DMOMeshQuadFactory}[kind]
+    cls = {"tet": DMOTetClass, "hex": DMOHexClass, "tri": DMOTriClass if surface_mesh else DMOTriFlatClass,
+           "quad": DMOQuadClass}[kind]
     xyz, cells, desc, kw = make_mesh(args.workload, args.seed)
     kw["waves_per_vertex"] = args.waves
     # warm-up on a throwaway copy
@@ -389,10 +422,10 @@ def main(argv=None):
     st = opt.stats()
     q1 = opt.getQuality()
     qmean = opt.getMeanQuality()
-    searched = st["searches_inner"] if kind != "tri" else st["searches_surface"]
+    searched = st["searches_inner"] if kind in ("tet", "hex") else st["searches_surface"]
     updates = st["searches_inner"] + st["searches_surface"] + st["searches_feature"]
     value = updates / dt
-    flops = searched * wl["flop"]
+    flops = searched * wl["flop"] + (st["searches_feature"] * wl["flop1d"] if surface_mesh else 0)
     kernel_s = st["kernel_ms"] / 1e3
     launches = st["kernel_launches"]
     achieved = flops / kernel_s / 1e12
@@ -405,7 +438,8 @@ def main(argv=None):
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"], args.workload)
+        cpu = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"] and not surface_mesh,
+                           args.workload, surface_mesh)
     desc += f"; {args.waves} wavefront(s) per vertex"
     learned = None
     if args.compare_learned and kind in ("tet", "hex"):

@@ -619,9 +619,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                     const int tet = (m->elem == VSDMO_TET);
                     for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {
                         int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
-                        VSDMO_HIP(hipMemcpyAsync(o->snapx.p, m->x.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
-                        VSDMO_HIP(hipMemcpyAsync(o->snapy.p, m->y.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
-                        VSDMO_HIP(hipMemcpyAsync(o->snapz.p, m->z.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                        snapshot_launch(sa, m->stream);
                         surf_search2d_launch(sa, lo, hi, o->p.nq, tet, slds, m->stream);
                         o->s_surf += hi - lo;
                     }
@@ -638,9 +636,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                 SurfArgs sa = surf_args(o);
                 for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {
                     int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
-                    VSDMO_HIP(hipMemcpyAsync(o->snapx.p, m->x.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
-                    VSDMO_HIP(hipMemcpyAsync(o->snapy.p, m->y.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
-                    VSDMO_HIP(hipMemcpyAsync(o->snapz.p, m->z.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                    snapshot_launch(sa, m->stream);
                     timed_surface(o, [&]() { surf_fan2d_launch(sa, lo, hi, o->p.nq, m->stream); });
                     o->s_surf += hi - lo;
                 }
@@ -648,9 +644,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                     for (size_t c = 0; c + 1 < m->colOffFeature.size(); ++c) {
                         int lo = m->colOffFeature[c], hi = m->colOffFeature[c + 1];
                         if (m->elem == VSDMO_QUAD) {   // quad fan diagonals read the pass snapshot
-                            VSDMO_HIP(hipMemcpyAsync(o->snapx.p, m->x.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
-                            VSDMO_HIP(hipMemcpyAsync(o->snapy.p, m->y.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
-                            VSDMO_HIP(hipMemcpyAsync(o->snapz.p, m->z.p, sizeof(float) * m->nSurf, hipMemcpyDeviceToDevice, m->stream));
+                            snapshot_launch(sa, m->stream);
                         }
                         timed_surface(o, [&]() { surf_fan1d_launch(sa, lo, hi, o->p.nq, m->stream); });
                         for (int v = lo; v < hi; ++v) o->s_feat += o->opt1d_host[v];

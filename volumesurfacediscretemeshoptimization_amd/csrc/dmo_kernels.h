@@ -89,6 +89,8 @@ void surf_points_launch(const SurfArgs& a, int vid, int nu, int nv, int mode, in
 void surf_fan2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s);
 void surf_fan1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s);
 void surf_normals_launch(const SurfArgs& a, hipStream_t s);
+// snapshot of the surface positions into a.sx / a.sy / a.sz
+void snapshot_launch(const SurfArgs& a, hipStream_t s);
 void laplace_launch(float* x, float* y, float* z, const int* heRow, const int* heTarget, int lo, int hi, hipStream_t s);
 
 // chunked per-mesh qualities of a batch

@@ -304,56 +304,63 @@ __global__ void k_update_nn(SurfArgs a)
 // localSurfaceEstimationWithFeature: evaluate the nearest neighbour's quadric
 // at (u,v), then blend with its projections onto each one-ring neighbour's
 // surface (sector surfaces for feature neighbours), weights 1/(d^2 + eps).
-// Two passes over the neighbours (weights, then the weighted sum) so no
-// per-neighbour arrays are kept; the arithmetic is the single-pass one.
-__device__ w3 project2d(const SurfArgs& a, int vid, w3 vidPos, float u, float v)
+// Two passes over the neighbours (weights, then the weighted sum); the
+// arithmetic is the single-pass one.  Everything but (u,v) -- the fits, the
+// nearest neighbours' initial positions and the sector choice -- is the same
+// for all candidates of a search, so the searches stage it once in LDS
+// (Proj2 / Proj1) and blend from there.
+
+// the surface a one-ring neighbour projects onto: the fit of its nearest
+// neighbour nn, or for a feature nn the sector fit of the halfedge of nn whose
+// target (snapshot position) is closest to vidPos
+__device__ LSurf neighbour_fit(const SurfArgs& a, int nn, w3 vidPos)
 {
     const LSurf* ls = reinterpret_cast<const LSurf*>(a.ls);
-    const LSurf* fs = reinterpret_cast<const LSurf*>(a.fs);
-    const LSurf L = ls[a.nn[vid]];
+    if (!a.isFeature[nn]) return ls[nn];
+    const int off = nn - a.nFree;
+    float md = 3.40282347e+38f;
+    int arg = 0, idx = 0;
+    for (int h = a.heRow[nn]; h < a.heRow[nn + 1]; ++h, ++idx) {
+        w3 t = wsub(WP(a.sx, a.sy, a.sz, a.heTarget[h]), vidPos);
+        float dd = wdot(t, t);
+        if (dd < md) { md = dd; arg = idx; }
+    }
+    int sid = (arg < kTableSize) ? a.table[kTableSize * off + arg] : -1;
+    const int nsec = a.frow[off + 1] - a.frow[off];
+    if (sid < 0 || sid >= nsec) sid = 0;
+    return nsec > 0 ? reinterpret_cast<const LSurf*>(a.fs)[a.frow[off] + sid] : ls[nn];
+}
+
+// the blend of the 2D projection over a source of L (the NN's fit), q0 (its
+// initial position), nh, qn(i) (initial position of neighbour i's NN) and
+// LN(i) (the surface neighbour i projects onto)
+template <class Src>
+__device__ w3 blend2(const Src& s, float u, float v)
+{
+    const LSurf L = s.L();
     const float w = poly(L, u, v);
     const w3 pw = l2w(L, u, v, w);
-    const int h0 = a.heRow[vid];
-    int nh = a.heRow[vid + 1] - h0;
-    if (nh > kMaxNeigh) nh = kMaxNeigh;
+    const int nh = s.nh();
     float isum = 0.0f;
     {
-        w3 d = wsub(W(a.pinit + 3 * a.nn[vid]), pw);
+        w3 d = wsub(s.q0(), pw);
         isum += 1.0f / (wdot(d, d) + kSEps);
     }
     for (int i = 0; i < nh; ++i) {
-        int nn = a.nn[a.heTarget[h0 + i]];
-        w3 d = wsub(W(a.pinit + 3 * nn), pw);
+        w3 d = wsub(s.qn(i), pw);
         isum += 1.0f / (wdot(d, d) + kSEps);
     }
     const float ii = 1.0f / isum;
     w3 sum = { 0.0f, 0.0f, 0.0f };
     {
-        w3 d = wsub(W(a.pinit + 3 * a.nn[vid]), pw);
+        w3 d = wsub(s.q0(), pw);
         float inv = 1.0f / (wdot(d, d) + kSEps);
         sum = wadd(sum, wmul(ii, wmul(inv, pw)));
     }
     for (int i = 0; i < nh; ++i) {
-        int nn = a.nn[a.heTarget[h0 + i]];
-        w3 d = wsub(W(a.pinit + 3 * nn), pw);
+        w3 d = wsub(s.qn(i), pw);
         float inv = 1.0f / (wdot(d, d) + kSEps);
-        LSurf LN;
-        if (a.isFeature[nn]) {
-            int off = nn - a.nFree;
-            float md = 3.40282347e+38f;
-            int arg = 0, idx = 0;
-            for (int h = a.heRow[nn]; h < a.heRow[nn + 1]; ++h, ++idx) {
-                w3 t = wsub(WP(a.sx, a.sy, a.sz, a.heTarget[h]), vidPos);
-                float dd = wdot(t, t);
-                if (dd < md) { md = dd; arg = idx; }
-            }
-            int sid = (arg < kTableSize) ? a.table[kTableSize * off + arg] : -1;
-            const int nsec = a.frow[off + 1] - a.frow[off];
-            if (sid < 0 || sid >= nsec) sid = 0;
-            LN = nsec > 0 ? fs[a.frow[off] + sid] : ls[nn];
-        } else {
-            LN = ls[nn];
-        }
+        const LSurf LN = s.LN(i);
         w3 q = wsub(pw, LN.p0);
         float lu = wdot(q, LN.e[0]), lv = wdot(q, LN.e[1]);
         float lw = poly(LN, lu, lv);
@@ -362,34 +369,35 @@ __device__ w3 project2d(const SurfArgs& a, int vid, w3 vidPos, float u, float v)
     return sum;
 }
 
-__device__ w3 project1d(const SurfArgs& a, int vid, const int* nb, int n, float u)
+// 1D: the curve v = a u^2 of the NN's 1D fit, blended with the projections
+// onto the curves of the listed feature neighbours' NNs
+template <class Src>
+__device__ w3 blend1(const Src& s, float u)
 {
-    const LSurf1* ls1 = reinterpret_cast<const LSurf1*>(a.ls1);
-    const LSurf1 L = ls1[a.nn[vid]];
+    const LSurf1 L = s.L();
     const float v = L.a * u * u;
     const w3 pw = wadd(wadd(L.p0, wmul(u, L.e[0])), wmul(v, L.e[1]));
+    const int n = s.nh();
     float isum = 0.0f;
     {
-        w3 d = wsub(W(a.pinit + 3 * a.nn[vid]), pw);
+        w3 d = wsub(s.q0(), pw);
         isum += 1.0f / (wdot(d, d) + kSEps);
     }
     for (int i = 0; i < n; ++i) {
-        int nn = a.nn[nb[i]];
-        w3 d = wsub(W(a.pinit + 3 * nn), pw);
+        w3 d = wsub(s.qn(i), pw);
         isum += 1.0f / (wdot(d, d) + kSEps);
     }
     const float ii = 1.0f / isum;
     w3 sum = { 0.0f, 0.0f, 0.0f };
     {
-        w3 d = wsub(W(a.pinit + 3 * a.nn[vid]), pw);
+        w3 d = wsub(s.q0(), pw);
         float inv = 1.0f / (wdot(d, d) + kSEps);
         sum = wadd(sum, wmul(ii, wmul(inv, pw)));
     }
     for (int i = 0; i < n; ++i) {
-        int nn = a.nn[nb[i]];
-        w3 d = wsub(W(a.pinit + 3 * nn), pw);
+        w3 d = wsub(s.qn(i), pw);
         float inv = 1.0f / (wdot(d, d) + kSEps);
-        const LSurf1 LN = ls1[nn];
+        const LSurf1 LN = s.LN(i);
         w3 q = wsub(pw, LN.p0);
         float lu = wdot(q, LN.e[0]);
         float lv = LN.a * lu * lu;
@@ -397,6 +405,119 @@ __device__ w3 project1d(const SurfArgs& a, int vid, const int* nb, int n, float 
     }
     return sum;
 }
+
+// staged (LDS) sources
+struct Proj2 {
+    LSurf L;
+    w3 q0;
+    int n;
+    w3 qn[kMaxNeigh];
+    LSurf LN[kMaxNeigh];
+};
+struct Proj2Src {
+    const Proj2* c;
+    __device__ LSurf L() const { return c->L; }
+    __device__ w3 q0() const { return c->q0; }
+    __device__ int nh() const { return c->n; }
+    __device__ w3 qn(int i) const { return c->qn[i]; }
+    __device__ LSurf LN(int i) const { return c->LN[i]; }
+};
+struct Proj1 {
+    LSurf1 L;
+    w3 q0;
+    int n;
+    int nb[kMaxNeigh];
+    w3 qn[kMaxNeigh];
+    LSurf1 LN[kMaxNeigh];
+};
+struct Proj1Src {
+    const Proj1* c;
+    __device__ LSurf1 L() const { return c->L; }
+    __device__ w3 q0() const { return c->q0; }
+    __device__ int nh() const { return c->n; }
+    __device__ w3 qn(int i) const { return c->qn[i]; }
+    __device__ LSurf1 LN(int i) const { return c->LN[i]; }
+};
+
+// stage vid's 2D projection data: lane i < nh resolves one-ring neighbour i
+// (its NN, that NN's initial position and fit / sector fit); all lanes call
+// it, it ends with a barrier
+__device__ void stage_proj2(const SurfArgs& a, int vid, w3 vidPos, Proj2* c)
+{
+    const int lane = threadIdx.x;
+    const int h0 = a.heRow[vid];
+    int nh = a.heRow[vid + 1] - h0;
+    if (nh > kMaxNeigh) nh = kMaxNeigh;
+    if (lane == 0) {
+        const int n0 = a.nn[vid];
+        c->L = reinterpret_cast<const LSurf*>(a.ls)[n0];
+        c->q0 = W(a.pinit + 3 * n0);
+        c->n = nh;
+    }
+    if (lane < nh) {
+        const int nn = a.nn[a.heTarget[h0 + lane]];
+        c->qn[lane] = W(a.pinit + 3 * nn);
+        c->LN[lane] = neighbour_fit(a, nn, vidPos);
+    }
+    __syncthreads();
+}
+
+// feature neighbours of vid that have exactly two feature neighbours
+// themselves (the curve neighbours of the 1D projection), in halfedge order
+__device__ int curve_neighbours(const SurfArgs& a, int vid, int* nb)
+{
+    int nnb = 0;
+    for (int h = a.heRow[vid]; h < a.heRow[vid + 1]; ++h) {
+        int d = a.heTarget[h];
+        if (!a.isFeature[d]) continue;
+        int c = 0;
+        for (int h2 = a.heRow[d]; h2 < a.heRow[d + 1]; ++h2) if (a.isFeature[a.heTarget[h2]]) ++c;
+        if (c == 2 && nnb < kMaxNeigh) nb[nnb++] = d;
+    }
+    return nnb;
+}
+
+// stage vid's 1D projection data: lane 0 lists the curve neighbours, then
+// lane i < n resolves neighbour i; all lanes call it, it ends with a barrier
+__device__ void stage_proj1(const SurfArgs& a, int vid, Proj1* c)
+{
+    const LSurf1* ls1 = reinterpret_cast<const LSurf1*>(a.ls1);
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        c->n = curve_neighbours(a, vid, c->nb);
+        const int n0 = a.nn[vid];
+        c->L = ls1[n0];
+        c->q0 = W(a.pinit + 3 * n0);
+    }
+    __syncthreads();
+    if (lane < c->n) {
+        const int nn = a.nn[c->nb[lane]];
+        c->qn[lane] = W(a.pinit + 3 * nn);
+        c->LN[lane] = ls1[nn];
+    }
+    __syncthreads();
+}
+
+// unstaged source for single evaluations (surface sampling)
+struct Proj2Global {
+    const SurfArgs* a;
+    int vid, h0, n;
+    w3 vidPos;
+    __device__ LSurf L() const { return reinterpret_cast<const LSurf*>(a->ls)[a->nn[vid]]; }
+    __device__ w3 q0() const { return W(a->pinit + 3 * a->nn[vid]); }
+    __device__ int nh() const { return n; }
+    __device__ w3 qn(int i) const { return W(a->pinit + 3 * a->nn[a->heTarget[h0 + i]]); }
+    __device__ LSurf LN(int i) const { return neighbour_fit(*a, a->nn[a->heTarget[h0 + i]], vidPos); }
+};
+
+__device__ w3 project2d(const SurfArgs& a, int vid, w3 vidPos, float u, float v)
+{
+    const int h0 = a.heRow[vid];
+    int nh = a.heRow[vid + 1] - h0;
+    if (nh > kMaxNeigh) nh = kMaxNeigh;
+    return blend2(Proj2Global{ &a, vid, h0, nh, vidPos }, u, v);
+}
+
 
 // ------------------------------------------------------------- ring scores
 struct HexRingLds { float cd[4], cF[4]; HexP hp[5]; };
@@ -515,8 +636,9 @@ __device__ __forceinline__ void wave_argmax_s(float& best, int& idx)
 }
 
 // ------------------------------------------------------------- search bodies
-// Scorers: score(p) is what the search maximises; final_ref(p0, q0) is the
-// start score the last level's maximum must beat for the write.
+// Scorers: score(p) is what the search maximises (one candidate per lane);
+// start(p0, qs, qref) gives the start position's score and the reference the
+// last level's maximum must beat for the write (all lanes call it).
 
 // volume one-ring of a tet / hex surface vertex
 struct VolScore {
@@ -525,7 +647,8 @@ struct VolScore {
     const void* lds;
     HexAcc hacc;
     __device__ float score(w3 p) const { return ring_score(f, tet, lds, hacc, p); }
-    __device__ float final_ref(w3, float q0) const { return q0; }
+    // start score (maximised from) and the reference the last level must beat
+    __device__ void start(w3 p, float& qs, float& qref) const { qs = score(p); qref = qs; }
 };
 
 // fan of a surface-mesh vertex (DMOQualityTri.h:257-275, DMOQualityQuad.h:
@@ -549,18 +672,63 @@ struct FanScore {
         const w3 a = pts[k], b = pts[k + 1];
         return mr_tri3(q, f3{ a.x, a.y, a.z }, f3{ b.x, b.y, b.z }, no);
     }
+    __device__ void elem2(w3 p, int k, float& qs, float& qu) const
+    {
+        const f3 nn = { n.x, n.y, n.z };
+        const f3 q = { p.x, p.y, p.z };
+        if (QUAD) {
+            const w3 a = pts[2 * k], b = pts[2 * k + 1], c = pts[2 * k + 2];
+            mr_quad3_both(q, f3{ a.x, a.y, a.z }, f3{ b.x, b.y, b.z }, f3{ c.x, c.y, c.z }, nn, qs, qu);
+        } else {
+            const w3 a = pts[k], b = pts[k + 1];
+            mr_tri3_both(q, f3{ a.x, a.y, a.z }, f3{ b.x, b.y, b.z }, nn, qs, qu);
+        }
+    }
     __device__ float score(w3 p) const
     {
         float q = FLT_MAX;
         for (int k = 0; k < ne; ++k) q = fminf(q, elem(p, k, true));
         return q;
     }
-    __device__ float final_ref(w3 p, float) const
+    // lane k < ne evaluates fan element k signed and unsigned; two wave
+    // min-reductions (fminf is exact, so the order does not matter)
+    __device__ void start(w3 p, float& qs, float& qref) const
     {
-        float q = FLT_MAX;
-        for (int k = 0; k < ne; ++k) q = fminf(q, elem(p, k, false));
-        return q;
+        const int k = threadIdx.x;
+        float s = FLT_MAX, u = FLT_MAX;
+        if (k < ne) elem2(p, k, s, u);
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            s = fminf(s, __shfl_xor(s, m));
+            u = fminf(u, __shfl_xor(u, m));
+        }
+        qs = s;
+        qref = u;
     }
+};
+
+// fan of a flat triangle mesh's boundary vertex in the canonical 2D
+// evaluation (frame of the start position x0, scale sg)
+struct FlatScore {
+    const TriC* tc;
+    int ne;
+    w3 x0;
+    float sg, is2;
+    __device__ float score(w3 q) const
+    {
+        const float px = (q.x - x0.x) * sg, py = (q.y - x0.y) * sg;
+        const float Q2 = 2.0f * (py * py);
+        const float t2 = 2.0f * px;
+        TriAcc acc = tri_acc0();
+        for (int e = 0; e < ne; ++e) {
+            const TriC c = tc[e];
+            float Ty = fmaf(c.gy, py, c.C);
+            float Ry = fmaf(c.sy, py, c.K + Q2);
+            tri_acc_add(acc, fmaf(c.gx, px, Ty), fmaf(px, t2 + c.sx, Ry));
+        }
+        return tri_acc_score(acc, is2);
+    }
+    __device__ void start(w3 p, float& qs, float& qref) const { qs = score(p); qref = qs; }
 };
 
 // 2D search on the blended local surfaces.  Lane l holds reference thread
@@ -571,12 +739,15 @@ template <class S>
 __device__ void surf2d_search(const SurfArgs& a, int vid, w3 vidPos, float dist, int nq, const S& sc)
 {
     const int lane = threadIdx.x;
-    const LSurf L = reinterpret_cast<const LSurf*>(a.ls)[a.nn[vid]];
+    __shared__ Proj2 s_proj;
+    stage_proj2(a, vid, vidPos, &s_proj);
+    const Proj2Src ps{ &s_proj };
+    const LSurf L = s_proj.L;
     w3 cur = vidPos;
     const w3 cvec = wsub(cur, L.p0);
     float cu = wdot(L.e[0], cvec), cv = wdot(L.e[1], cvec);
-    float qcur = sc.score(cur);
-    const float qold = sc.final_ref(cur, qcur);
+    float qcur, qold;
+    sc.start(cur, qcur, qold);
     float qlast = -FLT_MAX;
     const int t = lane >> 1;
     const bool isB = (lane & 1) != 0;
@@ -587,17 +758,19 @@ __device__ void surf2d_search(const SurfArgs& a, int vid, w3 vidPos, float dist,
         const float uMax = ds * dist, uMin = -(ds * dist);
         float best = -FLT_MAX;
         float u = 0.0f, v = 0.0f;
+        w3 pc = { 0.0f, 0.0f, 0.0f };
         if (active) {
             u = cu + a.aff * ((float)ci * uMin + (float)(nq - 1 - ci) * uMax);
             v = cv + a.aff * ((float)cj * uMin + (float)(nq - 1 - cj) * uMax);
-            best = sc.score(project2d(a, vid, vidPos, u, v));
+            pc = blend2(ps, u, v);
+            best = sc.score(pc);
         }
         int wl = lane;
         wave_argmax_s(best, wl);
         if (qcur < best) {
-            const float bu = __shfl(u, wl), bv = __shfl(v, wl);
-            cur = project2d(a, vid, vidPos, bu, bv);
-            cu = bu; cv = bv;
+            // the winner's projected point (the projection of (bu, bv))
+            cu = __shfl(u, wl); cv = __shfl(v, wl);
+            cur = { __shfl(pc.x, wl), __shfl(pc.y, wl), __shfl(pc.z, wl) };
             qcur = best;
         }
         qlast = best;
@@ -607,42 +780,37 @@ __device__ void surf2d_search(const SurfArgs& a, int vid, w3 vidPos, float dist,
 }
 
 // 1D search along the feature curve: lane i < nq holds candidate i; the
-// curve projection blends the fits of the feature neighbours that have
-// exactly two feature neighbours themselves
+// curve projection blends the fits of the curve neighbours
 template <class S>
 __device__ void surf1d_search(const SurfArgs& a, int vid, w3 vidPos, float dist, int nq, const S& sc)
 {
     const int lane = threadIdx.x;
-    int nb[kMaxNeigh], nnb = 0;
-    for (int h = a.heRow[vid]; h < a.heRow[vid + 1]; ++h) {
-        int d = a.heTarget[h];
-        if (!a.isFeature[d]) continue;
-        int c = 0;
-        for (int h2 = a.heRow[d]; h2 < a.heRow[d + 1]; ++h2) if (a.isFeature[a.heTarget[h2]]) ++c;
-        if (c == 2 && nnb < kMaxNeigh) nb[nnb++] = d;
-    }
-    const LSurf1 L = reinterpret_cast<const LSurf1*>(a.ls1)[a.nn[vid]];
+    __shared__ Proj1 s_proj;
+    stage_proj1(a, vid, &s_proj);
+    const Proj1Src ps{ &s_proj };
+    const LSurf1 L = s_proj.L;
     w3 cur = vidPos;
     const w3 cvec = wsub(cur, L.p0);
     float cu = wdot(L.e[0], cvec);
-    float qcur = sc.score(cur);
-    const float qold = sc.final_ref(cur, qcur);
+    float qcur, qold;
+    sc.start(cur, qcur, qold);
     float qlast = -FLT_MAX;
     const bool active = lane < nq;
     float ds = a.gridScale;
     for (int d = 0; d < a.depth; ++d) {
         const float uMax = ds * dist, uMin = -(ds * dist);
         float best = -FLT_MAX, u = 0.0f;
+        w3 pc = { 0.0f, 0.0f, 0.0f };
         if (active) {
             u = cu + a.aff * ((float)lane * uMin + (float)(nq - 1 - lane) * uMax);
-            best = sc.score(project1d(a, vid, nb, nnb, u));
+            pc = blend1(ps, u);
+            best = sc.score(pc);
         }
         int wl = lane;
         wave_argmax_s(best, wl);
         if (qcur < best) {
-            const float bu = __shfl(u, wl);
-            cur = project1d(a, vid, nb, nnb, bu);
-            cu = bu;
+            cu = __shfl(u, wl);
+            cur = { __shfl(pc.x, wl), __shfl(pc.y, wl), __shfl(pc.z, wl) };
             qcur = best;
         }
         qlast = best;
@@ -762,6 +930,14 @@ __global__ void k_surf_vertex_normals(SurfArgs a)
     a.vnorm[3 * v] = n.x; a.vnorm[3 * v + 1] = n.y; a.vnorm[3 * v + 2] = n.z;
 }
 
+// positions snapshot at the start of a colour pass (one launch for x, y, z)
+__global__ void k_snapshot(const float* x, const float* y, const float* z, float* sx, float* sy, float* sz, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sx[i] = x[i]; sy[i] = y[i]; sz[i] = z[i];
+}
+
 // Laplace smoothing of one colour class of free surface vertices
 // (k_laplaceSmoothing, Laplace.cu:8-28): p <- (p + sum of the halfedge
 // targets) * (1 / (1 + n))
@@ -812,54 +988,7 @@ __global__ __launch_bounds__(64) void k_search_flat1d(SurfArgs a, int cStart, in
         s_tr[lane] = tri_consts((a.x[va] - x0.x) * sg, (a.y[va] - x0.y) * sg, (a.x[vb] - x0.x) * sg, (a.y[vb] - x0.y) * sg);
     }
     __syncthreads();
-    auto score = [&](w3 q) {
-        const float px = (q.x - x0.x) * sg, py = (q.y - x0.y) * sg;
-        const float Q2 = 2.0f * (py * py);
-        const float t2 = 2.0f * px;
-        TriAcc acc = tri_acc0();
-        for (int e = 0; e < ne; ++e) {
-            const TriC c = s_tr[e];
-            float Ty = fmaf(c.gy, py, c.C);
-            float Ry = fmaf(c.sy, py, c.K + Q2);
-            tri_acc_add(acc, fmaf(c.gx, px, Ty), fmaf(px, t2 + c.sx, Ry));
-        }
-        return tri_acc_score(acc, is2);
-    };
-    int nb[kMaxNeigh], nnb = 0;
-    for (int h = h0; h < h0 + n; ++h) {
-        int d = a.heTarget[h];
-        if (!a.isFeature[d]) continue;
-        int c = 0;
-        for (int h2 = a.heRow[d]; h2 < a.heRow[d + 1]; ++h2) if (a.isFeature[a.heTarget[h2]]) ++c;
-        if (c == 2 && nnb < kMaxNeigh) nb[nnb++] = d;
-    }
-    const LSurf1 L = reinterpret_cast<const LSurf1*>(a.ls1)[a.nn[vid]];
-    w3 cur = x0;
-    float cu = wdot(L.e[0], wsub(cur, L.p0));
-    float qcur = score(cur);
-    const float qold = qcur;
-    float qlast = -FLT_MAX;
-    const bool active = lane < nq;
-    float ds = a.gridScale;
-    for (int d = 0; d < a.depth; ++d) {
-        const float uMax = ds * mx, uMin = -(ds * mx);
-        float best = -FLT_MAX, u = 0.0f;
-        if (active) {
-            u = cu + a.aff * ((float)lane * uMin + (float)(nq - 1 - lane) * uMax);
-            best = score(project1d(a, vid, nb, nnb, u));
-        }
-        int wl = lane;
-        wave_argmax_s(best, wl);
-        if (qcur < best) {
-            const float bu = __shfl(u, wl);
-            cur = project1d(a, vid, nb, nnb, bu);
-            cu = bu;
-            qcur = best;
-        }
-        qlast = best;
-        ds = ds * a.shrink;
-    }
-    if (qold < qlast && lane == 0) { a.xw[vid] = cur.x; a.yw[vid] = cur.y; a.zw[vid] = cur.z; }
+    surf1d_search(a, vid, x0, mx, nq, FlatScore{ s_tr, ne, x0, sg, is2 });
 }
 
 // ------------------------------------------------------- surface sampling
@@ -954,6 +1083,11 @@ void surf_normals_launch(const SurfArgs& a, hipStream_t s)
 {
     k_surf_face_normals<<<nb256(a.nFaces), 256, 0, s>>>(a);
     k_surf_vertex_normals<<<nb256(a.nS), 256, 0, s>>>(a);
+}
+void snapshot_launch(const SurfArgs& a, hipStream_t s)
+{
+    if (a.nS > 0) k_snapshot<<<nb256(a.nS), 256, 0, s>>>(a.x, a.y, a.z, const_cast<float*>(a.sx), const_cast<float*>(a.sy),
+                                                         const_cast<float*>(a.sz), a.nS);
 }
 void laplace_launch(float* x, float* y, float* z, const int* heRow, const int* heTarget, int lo, int hi, hipStream_t s)
 {

@@ -109,6 +109,18 @@ __device__ __forceinline__ float mr_tri3(f3 p0, f3 p1, f3 p2, const f3* n)
     return (kTwoSqrt3 * area) / l;
 }
 
+// signed (by n) and unsigned values of one triangle from one evaluation
+__device__ __forceinline__ void mr_tri3_both(f3 p0, f3 p1, f3 p2, f3 n, float& qs, float& qu)
+{
+    const f3 e0 = sub3(p1, p0), e1 = sub3(p2, p0), e2 = sub3(p2, p1);
+    const float l = (ldot3(e0, e0) + ldot3(e1, e1)) + ldot3(e2, e2);
+    const f3 N = cross3(e0, e1);
+    const float area = lnorm3(N);
+    const float d = ldot3(lunit3(N), n);
+    qs = (kTwoSqrt3 * (area * (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)))) / l;
+    qu = (kTwoSqrt3 * area) / l;
+}
+
 // meanRatioMetricQuad of (p0, p1, p2, p3): 2 (|e1 x -e0| + |e3 x -e2|) /
 // sum |e_i|^2 with e_i = p_(i+1) - p_i; with n: -1 when a corner triangle's
 // unit normal u has n . u < 0.7
@@ -123,6 +135,17 @@ __device__ __forceinline__ float mr_quad3(f3 p0, f3 p1, f3 p2, f3 p3, const f3* 
         if (d1 < 0.7f || d2 < 0.7f) return -1.0f;
     }
     return (2.0f * (a1 + a2)) / l;
+}
+
+__device__ __forceinline__ void mr_quad3_both(f3 p0, f3 p1, f3 p2, f3 p3, f3 n, float& qs, float& qu)
+{
+    const f3 e0 = sub3(p1, p0), e1 = sub3(p2, p1), e2 = sub3(p3, p2), e3 = sub3(p0, p3);
+    const float l = ((ldot3(e0, e0) + ldot3(e1, e1)) + ldot3(e2, e2)) + ldot3(e3, e3);
+    const f3 N1 = cross3(e1, neg3(e0)), N2 = cross3(e3, neg3(e2));
+    const float a1 = lnorm3(N1), a2 = lnorm3(N2);
+    const float d1 = ldot3(lunit3(N1), n), d2 = ldot3(lunit3(N2), n);
+    qu = (2.0f * (a1 + a2)) / l;
+    qs = (d1 < 0.7f || d2 < 0.7f) ? -1.0f : qu;
 }
 
 // ---------------------------------------------------------------- tets
