@@ -181,6 +181,13 @@ vsdmo_status vsdmo_opt_element_qualities(const vsdmo_opt* opt, float* host_out);
 /* counters and timing; set_timing(1) records device time of every search launch */
 vsdmo_status vsdmo_opt_stats(const vsdmo_opt* opt, vsdmo_stats* out);
 vsdmo_status vsdmo_opt_set_timing(vsdmo_opt* opt, int32_t on);
+/* Replaces the failure dump of updateCurrentQualities (dmo/DMOTet.cu:223-233,
+   dmo/DMOHex.cu:235-245, dmo/DMOQuad.cu:218-228: writeTetgen / writeHex /
+   writeOFF of "res/fail_debug", then throw): when the quality guard fails,
+   the mesh (renumbered order, positions after the failing iteration) is
+   written to path in its format (vsdmo_mesh_write) before the call returns
+   VSDMO_QUALITY_DECREASED.  path NULL or "" turns it off (the default). */
+vsdmo_status vsdmo_opt_set_failure_dump(vsdmo_opt* opt, const char* path);
 vsdmo_status vsdmo_opt_destroy(vsdmo_opt* opt);
 
 /* thread-local message of the last failing call on this thread */

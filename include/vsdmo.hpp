@@ -170,6 +170,11 @@ public:
         vsdmo_cpp::check(vsdmo_opt_surface_points(h_, vid, nu, nv, 1, featureSid, (float*)outSurfacePoints),
                          "vsdmo_opt_surface_points");
     }
+    // mesh file written when the monotone-quality guard fails (reference res/fail_debug)
+    void setFailureDump(const std::string& path)
+    {
+        vsdmo_cpp::check(vsdmo_opt_set_failure_dump(h_, path.c_str()), "vsdmo_opt_set_failure_dump");
+    }
     vsdmo_opt* handle() const { return h_; }
 
 protected:

@@ -269,3 +269,34 @@ def test_mesh_files_through_the_factories(tmp_path):
     x2, t2 = read_mesh_file(_lib.VSDMO_TET, str(tmp_path / "out"))
     assert np.array_equal(x2, a.positions()) and np.array_equal(t2, tets)
     opt.close()
+
+
+def test_failure_dump(tmp_path):
+    """The quality guard's failure dump (reference updateCurrentQualities ->
+    writeTetgen("res/fail_debug")): the host pushes a fixed boundary vertex
+    1.5h into the cube between iterations, so the next iteration's minimum
+    falls below the last one; the call fails with QUALITY_DECREASED and the
+    mesh file holds the positions after that iteration."""
+    from volumesurfacediscretemeshoptimization_amd import VsdmoError, read_mesh_file
+    from volumesurfacediscretemeshoptimization_amd import _lib
+    n = 6
+    xyz, tets = synth.tet_cube(n, seed=12)
+    g = _build("tet", xyz, tets)
+    opt = DMOTetClass(g, n_iter=3, fixed_boundary=True)
+    opt.doIteration()
+    q1 = opt.getQuality()
+    p = g.positions(original_order=True)
+    v = 0 + (n + 1) * (3 + (n + 1) * 3)      # lattice point (0, 3, 3) on the x = 0 face
+    assert p[v, 0] == 0.0
+    p[v, 0] = 1.5 / n
+    g.set_positions(p, original_order=True)
+    path = str(tmp_path / "fail_debug")
+    opt.set_failure_dump(path)
+    with pytest.raises(VsdmoError) as e:
+        opt.doIteration()
+    assert e.value.status == _lib.VSDMO_QUALITY_DECREASED and path in str(e.value)
+    assert opt.getQuality() < q1
+    x2, t2 = read_mesh_file(_lib.VSDMO_TET, path)
+    assert np.array_equal(x2, g.positions(original_order=False))
+    assert np.array_equal(t2, g.cells())
+    opt.close()

@@ -7,7 +7,7 @@ Laplace smoothing (dmo/Laplace.cu:8-41) on the CPU.
   threshold rules, and FP64 re-evaluation;
 * search invariants: points of a plane stay on it, sphere points stay closer
   to the sphere than the mesh's facets, the unsigned minimum quality never decreases
-  (the reference asserts it, dmo/DMOTri.cu:167);
+  (the reference asserts it, dmo/DMOTri.cu:161);
 * Laplace smoothing against closed forms on single free vertices.
 """
 import numpy as np

@@ -296,6 +296,13 @@ class _DMOBase:
     def set_timing(self, on: bool):
         check(self._lib.vsdmo_opt_set_timing(self._h, int(on)), "vsdmo_opt_set_timing")
 
+    def set_failure_dump(self, path):
+        """Write the mesh to path (its own format, renumbered order) when the
+        quality guard fails (reference updateCurrentQualities's
+        res/fail_debug dump); None turns it off."""
+        check(self._lib.vsdmo_opt_set_failure_dump(self._h, path.encode() if path else None),
+              "vsdmo_opt_set_failure_dump")
+
     def stats(self) -> dict:
         s = _lib.Stats()
         check(self._lib.vsdmo_opt_stats(self._h, C.byref(s)), "vsdmo_opt_stats")

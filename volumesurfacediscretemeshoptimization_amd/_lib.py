@@ -89,6 +89,7 @@ EXPORTS = {
     "vsdmo_opt_element_qualities": (C.c_int, [C.c_void_p, _fp]),
     "vsdmo_opt_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "vsdmo_opt_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vsdmo_opt_set_failure_dump": (C.c_int, [C.c_void_p, C.c_char_p]),
     "vsdmo_opt_destroy": (C.c_int, [C.c_void_p]),
     "vsdmo_last_error": (C.c_char_p, []),
     "vsdmo_mesh_file_read": (C.c_int, [C.c_int, C.c_char_p, _ip, _ip, _fp, _ip]),

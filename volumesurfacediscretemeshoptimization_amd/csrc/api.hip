@@ -110,6 +110,7 @@ struct vsdmo_opt {
     DevBuf<int> lFallback, lList;
     int64_t s_learned = 0, fallbacks = 0;
     vsdmo_dataset* rec = nullptr;   // training pairs are recorded while set
+    std::string fail_dump;          // mesh file written when the quality guard fails
 };
 
 extern "C" {
@@ -683,8 +684,13 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                 float tol = o->p.guard_tolerance * std::fabs(prev);
                 if (o->last_min < prev - tol) {
                     collect_timing(o);
-                    return fail(VSDMO_QUALITY_DECREASED, "minimum quality decreased from " + std::to_string(prev) +
-                                                             " to " + std::to_string(o->last_min));
+                    std::string msg = "minimum quality decreased from " + std::to_string(prev) + " to " +
+                                      std::to_string(o->last_min);
+                    if (!o->fail_dump.empty()) {
+                        if (vsdmo_mesh_write(m, o->fail_dump.c_str(), 0) == VSDMO_OK) msg += "; mesh written to " + o->fail_dump;
+                        else msg += "; writing " + o->fail_dump + " failed";
+                    }
+                    return fail(VSDMO_QUALITY_DECREASED, msg);
                 }
             }
         }
@@ -835,6 +841,13 @@ vsdmo_status vsdmo_opt_set_timing(vsdmo_opt* o, int32_t on)
     o->timing = on != 0;
     o->kernel_ms = 0.0;
     o->launches = 0;
+    return VSDMO_OK;
+}
+
+vsdmo_status vsdmo_opt_set_failure_dump(vsdmo_opt* o, const char* path)
+{
+    if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
+    o->fail_dump = path ? path : "";
     return VSDMO_OK;
 }
 
