@@ -107,7 +107,7 @@ struct vsdmo_opt {
     int learn_mode = 0, learn_depth = 4;
     float learn_frac = 0.25f;
     DevBuf<float> lL, lY;
-    DevBuf<int> lFallback, lList;
+    DevBuf<int> lFallback;           // running count of fallbacks to the full search
     int64_t s_learned = 0, fallbacks = 0;
     vsdmo_dataset* rec = nullptr;   // training pairs are recorded while set
     std::string fail_dump;          // mesh file written when the quality guard fails
@@ -531,7 +531,7 @@ static void timed_learned(vsdmo_opt* o, int kind, const SearchArgs& a, size_t ld
         VSDMO_HIP(hipEventRecord(e0, s));
     }
     launch_search_learned(kind, o->p.nq, a, o->lY.p, o->lL.p, o->learn_mode, o->learn_depth, o->learn_frac,
-                          o->lFallback.p, o->lList.p, o->lFallback.p + 1, lds, s);
+                          o->lFallback.p, lds, s);
     if (o->timing) {
         VSDMO_HIP(hipEventRecord(e1, s));
         o->ev.push_back(e0);
@@ -599,9 +599,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                     if (o->learn_mode != 0 && !ds) {
                         if (o->learn_mode == 1)
                             launch_mlp_infer(o->mlp, m->x.p, m->y.p, m->z.p, m->nbr.p, a.cStart, a.cEnd, nullptr,
-                                             o->lY.p, o->lL.p, o->lFallback.p + 1, m->stream);
-                        else
-                            launch_fill_int(o->lFallback.p + 1, 1, 0, m->stream);
+                                             o->lY.p, o->lL.p, m->stream);
                         timed_learned(o, m->elem, a, lds);
                         o->s_learned += nc;
                     } else {
@@ -798,11 +796,11 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
         for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c)
             maxc = std::max(maxc, m->colOffInner[c + 1] - m->colOffInner[c]);
         if (!o->lFallback.p) {
-            o->lFallback.alloc(2);   // running total, per-class count
-            launch_fill_int(o->lFallback.p, 2, 0, m->stream);
+            o->lFallback.alloc(1);
+            launch_fill_int(o->lFallback.p, 1, 0, m->stream);
         }
         if ((int)o->lL.n < maxc) {
-            o->lL.alloc(maxc); o->lY.alloc((size_t)3 * maxc); o->lList.alloc(maxc);
+            o->lL.alloc(maxc); o->lY.alloc((size_t)3 * maxc);
         }
         if (mode == 1) {
             build_neighbour_table(*m);

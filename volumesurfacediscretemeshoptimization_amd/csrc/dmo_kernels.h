@@ -126,11 +126,12 @@ void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, c
                         int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s);
 int batch_round_blocks_per_cu(int nq, int w, size_t lds);
 void launch_fill_int(int* p, int n, int v, hipStream_t s);
-// learned-proposal search of one colour class [a.cStart, a.cEnd) (tet / hex);
-// Y, L indexed by the vertex's position in the class; *cnt must be zero on
-// entry (fallback list length)
+// learned-proposal search of one colour class [a.cStart, a.cEnd) (tet / hex):
+// the proposal window, and for a vertex whose window finds nothing better
+// than its start, the plain search in the same wave; Y, L indexed by the
+// vertex's position in the class; *fallbacks counts the fallbacks
 void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y, const float* L, int mode, int depthL,
-                           float frac, int* fallbacks, int* list, int* cnt, size_t lds, hipStream_t s);
+                           float frac, int* fallbacks, size_t lds, hipStream_t s);
 void launch_batch_quality(const BatchQArgs& a, hipStream_t s);
 size_t batch_lds_bytes(int maxRingTet, int maxRingHex, int maxHETri);
 void launch_quality(const QualityArgs& a, float* pmin, double* psum, float* omin, double* osum, hipStream_t s);

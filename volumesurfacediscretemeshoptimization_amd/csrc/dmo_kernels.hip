@@ -103,29 +103,33 @@ template <int W>
 struct RedLds { float f[W]; int i[W]; };
 
 // ------------------------------------------------------------------ tets
-// One vertex per block of W waves.  Lane l = j + nq*k of wave w owns the
-// candidates (i, j, k) with i in [w*nq/W, (w+1)*nq/W); candidate index
-// i + nq*l, so the lowest index wins ties as in the reference.
-template <int NQ, int W>
-__device__ void tet_vertex(const SearchArgs& a, int vid, bool emin, TetC* s_tc, RedLds<W>& rd)
+// The staged state of one vertex search: start position, window
+// half-extents, frame scale and ring size; the ring's constants are in LDS.
+struct TetFrame {
+    float x0x, x0y, x0z, ex, ey, ez, sg, is3;
+    int n;
+};
+
+// Stage vid's one-ring: per-axis window half-extent (max or min offset), the
+// frame scale and the constants of every ring element in LDS.  Every thread
+// of the block calls it; it ends with a barrier.
+template <int W>
+__device__ TetFrame tet_stage(const SearchArgs& a, int vid, bool emin, TetC* s_tc, RedLds<W>& rd)
 {
     constexpr int T = 64 * W;
-    constexpr int CPL = NQ / W;
-    static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
     // W = 1 bodies also run one per wave inside wider blocks (k_search_split)
     const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    TetFrame f;
     const int r0 = a.ringRow[vid];
-    const int n = a.ringRow[vid + 1] - r0;
-    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
-
-    // window half-extent per axis and the frame scale
+    f.n = a.ringRow[vid + 1] - r0;
+    f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
     float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    for (int e = tid; e < n; e += T) {
+    for (int e = tid; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 3;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             int v = h[t];
-            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
+            float dx = fabsf(a.x[v] - f.x0x), dy = fabsf(a.y[v] - f.x0y), dz = fabsf(a.z[v] - f.x0z);
             ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
             ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
             ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
@@ -135,61 +139,73 @@ __device__ void tet_vertex(const SearchArgs& a, int vid, bool emin, TetC* s_tc, 
     if (emin) { ex = block_min<W>(ex, rd.f); ey = block_min<W>(ey, rd.f); ez = block_min<W>(ez, rd.f); }
     else { ex = block_max<W>(ex, rd.f); ey = block_max<W>(ey, rd.f); ez = block_max<W>(ez, rd.f); }
     mabs = block_max<W>(mabs, rd.f);
-    const float sg = pow2_scale(mabs);
-    const float isg = 1.0f / sg;
-    const float is3 = ((1.0f * isg) * isg) * isg;
-
-    for (int e = tid; e < n; e += T) {
+    f.ex = ex; f.ey = ey; f.ez = ez;
+    f.sg = pow2_scale(mabs);
+    const float isg = 1.0f / f.sg;
+    f.is3 = ((1.0f * isg) * isg) * isg;
+    for (int e = tid; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 3;
         f3 p[3];
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             int v = h[t];
-            p[t] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+            p[t] = { (a.x[v] - f.x0x) * f.sg, (a.y[v] - f.x0y) * f.sg, (a.z[v] - f.x0z) * f.sg };
         }
         s_tc[e] = tet_consts(p[0], p[1], p[2]);
     }
     __syncthreads();
+    return f;
+}
 
-    // score of the start position (p' = 0)
-    float qcur;
-    {
-        const float px = 0.0f, py = 0.0f, pz = 0.0f;
-        const float Qyz = 3.0f * fmaf(pz, pz, py * py);
-        const float t3 = 3.0f * px;
-        TetAcc acc = tet_acc0();
-        for (int e = 0; e < n; ++e) {
-            const TetC c = s_tc[e];
-            float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
-            float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
-            tet_acc_add(acc, fmaf(c.nnx, px, Tyz), fmaf(px, t3 + c.m2sx, Ryz));
-        }
-        qcur = tet_acc_score(acc, is3);
+// ring score of one candidate p' (frame coordinates)
+__device__ __forceinline__ float tet_score(const TetFrame& f, const TetC* s_tc, float px, float py, float pz)
+{
+    const float Qyz = 3.0f * fmaf(pz, pz, py * py);
+    const float t3 = 3.0f * px;
+    TetAcc acc = tet_acc0();
+    for (int e = 0; e < f.n; ++e) {
+        const TetC c = s_tc[e];
+        float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+        float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+        tet_acc_add(acc, fmaf(c.nnx, px, Tyz), fmaf(px, t3 + c.m2sx, Ryz));
     }
+    return tet_acc_score(acc, f.is3);
+}
 
+// The nested grid search from the start position (depth levels of NQ^3
+// candidates) with W waves.  Lane l = j + nq*k of wave w owns the candidates
+// (i, j, k) with i in [w*nq/W, (w+1)*nq/W); candidate index i + nq*l, so the
+// lowest index wins ties as in the reference.  qcur: the start score.
+// Returns whether the vertex moved, and its position in (cx, cy, cz).
+template <int NQ, int W>
+__device__ bool tet_levels(const SearchArgs& a, const TetFrame& f, const TetC* s_tc, RedLds<W>& rd, float qcur,
+                           float& cx, float& cy, float& cz)
+{
+    constexpr int CPL = NQ / W;
+    static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
+    const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
     const int lane = tid & 63, i0 = (tid >> 6) * CPL;
     const int j = lane % NQ, k = lane / NQ;
     const bool active = lane < NQ * NQ;
-    float cx = x0x, cy = x0y, cz = x0z;
     float ds = a.gridScale;
     bool moved = false;
     for (int d = 0; d < a.depth; ++d) {
-        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
-        const float gyM = cy + ds * ey, gym = cy - ds * ey;
-        const float gzM = cz + ds * ez, gzm = cz - ds * ez;
-        const float py = (grid_coord(a.aff, j, NQ, gym, gyM) - x0y) * sg;
-        const float pz = (grid_coord(a.aff, k, NQ, gzm, gzM) - x0z) * sg;
+        const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
+        const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
+        const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
+        const float py = (grid_coord(a.aff, j, NQ, gym, gyM) - f.x0y) * f.sg;
+        const float pz = (grid_coord(a.aff, k, NQ, gzm, gzM) - f.x0z) * f.sg;
         float px[CPL], t3[CPL];
 #pragma unroll
         for (int i = 0; i < CPL; ++i) {
-            px[i] = (grid_coord(a.aff, i0 + i, NQ, gxm, gxM) - x0x) * sg;
+            px[i] = (grid_coord(a.aff, i0 + i, NQ, gxm, gxM) - f.x0x) * f.sg;
             t3[i] = 3.0f * px[i];
         }
         const float Qyz = 3.0f * fmaf(pz, pz, py * py);
         TetAcc acc[CPL];
 #pragma unroll
         for (int i = 0; i < CPL; ++i) acc[i] = tet_acc0();
-        for (int e = 0; e < n; ++e) {
+        for (int e = 0; e < f.n; ++e) {
             const TetC c = s_tc[e];
             const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
             const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
@@ -201,7 +217,7 @@ __device__ void tet_vertex(const SearchArgs& a, int vid, bool emin, TetC* s_tc, 
         int bi = 0;
 #pragma unroll
         for (int i = 0; i < CPL; ++i) {
-            float q = tet_acc_score(acc[i], is3);
+            float q = tet_acc_score(acc[i], f.is3);
             if (q > best) { best = q; bi = i; }
         }
         if (!active) best = -FLT_MAX;
@@ -217,30 +233,48 @@ __device__ void tet_vertex(const SearchArgs& a, int vid, bool emin, TetC* s_tc, 
         }
         ds = ds * a.shrink;
     }
-    if (moved && tid == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    return moved;
+}
+
+// One vertex per block of W waves (or per wave, W = 1).
+template <int NQ, int W>
+__device__ void tet_vertex(const SearchArgs& a, int vid, bool emin, TetC* s_tc, RedLds<W>& rd)
+{
+    const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const TetFrame f = tet_stage<W>(a, vid, emin, s_tc, rd);
+    const float qcur = tet_score(f, s_tc, 0.0f, 0.0f, 0.0f);
+    float cx = f.x0x, cy = f.x0y, cz = f.x0z;
+    if (tet_levels<NQ, W>(a, f, s_tc, rd, qcur, cx, cy, cz) && tid == 0) {
+        a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz;
+    }
 }
 
 // ------------------------------------------------------------------ hexes
 struct HexLds { float cd[4], cF[4]; HexP hp[5]; };
 
-template <int NQ, int W>
-__device__ void hex_vertex(const SearchArgs& a, int vid, bool emin, HexLds* s_hx, RedLds<W>& rd)
+// staged state of a hex vertex search (as TetFrame), plus the accumulator of
+// the 4 sub-tets per ring element that do not contain the moving vertex
+struct HexFrame {
+    float x0x, x0y, x0z, ex, ey, ez, sg, epsD, epsF;
+    int n;
+    HexAcc acc0;
+};
+
+template <int W>
+__device__ HexFrame hex_stage(const SearchArgs& a, int vid, bool emin, HexLds* s_hx, RedLds<W>& rd)
 {
     constexpr int T = 64 * W;
-    constexpr int CPL = NQ / W;
-    static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
-    // W = 1 bodies also run one per wave inside wider blocks (k_search_split)
     const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    HexFrame f;
     const int r0 = a.ringRow[vid];
-    const int n = a.ringRow[vid + 1] - r0;
-    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
-
+    f.n = a.ringRow[vid + 1] - r0;
+    f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
     float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    for (int e = tid; e < n; e += T) {
+    for (int e = tid; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 7;
         for (int t = 0; t < 7; ++t) {
             int v = h[t];
-            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
+            float dx = fabsf(a.x[v] - f.x0x), dy = fabsf(a.y[v] - f.x0y), dz = fabsf(a.z[v] - f.x0z);
             ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
             ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
             ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
@@ -250,70 +284,77 @@ __device__ void hex_vertex(const SearchArgs& a, int vid, bool emin, HexLds* s_hx
     if (emin) { ex = block_min<W>(ex, rd.f); ey = block_min<W>(ey, rd.f); ez = block_min<W>(ez, rd.f); }
     else { ex = block_max<W>(ex, rd.f); ey = block_max<W>(ey, rd.f); ez = block_max<W>(ez, rd.f); }
     mabs = block_max<W>(mabs, rd.f);
-    const float sg = pow2_scale(mabs);
-    const float s2 = sg * sg, s3 = s2 * sg;
-    const float epsD = kHexEps * s3, epsF = kHexEps * s2;
-
-    for (int e = tid; e < n; e += T) {
+    f.ex = ex; f.ey = ey; f.ez = ez;
+    f.sg = pow2_scale(mabs);
+    const float s2 = f.sg * f.sg, s3 = s2 * f.sg;
+    f.epsD = kHexEps * s3; f.epsF = kHexEps * s2;
+    for (int e = tid; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 7;
         f3 r[8];
         r[0] = { 0.0f, 0.0f, 0.0f };
         for (int t = 0; t < 7; ++t) {
             int v = h[t];
-            r[t + 1] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
+            r[t + 1] = { (a.x[v] - f.x0x) * f.sg, (a.y[v] - f.x0y) * f.sg, (a.z[v] - f.x0z) * f.sg };
         }
         HexLds L;
         hex_consts(r, L.cd, L.cF, L.hp);
         s_hx[e] = L;
     }
     __syncthreads();
-
     // p-free sub-tets of the whole ring, in ring order
-    HexAcc acc0 = hex_acc0();
-    for (int e = 0; e < n; ++e)
+    f.acc0 = hex_acc0();
+    for (int e = 0; e < f.n; ++e)
 #pragma unroll
-        for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(acc0, s_hx[e].cd[sIdx], s_hx[e].cF[sIdx]);
+        for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(f.acc0, s_hx[e].cd[sIdx], s_hx[e].cF[sIdx]);
+    return f;
+}
 
-    float qcur;
-    {
-        const float px = 0.0f, py = 0.0f, pz = 0.0f;
-        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
-        HexAcc acc = acc0;
-        for (int e = 0; e < n; ++e)
+__device__ __forceinline__ float hex_score(const HexFrame& f, const HexLds* s_hx, float px, float py, float pz)
+{
+    const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
+    HexAcc acc = f.acc0;
+    for (int e = 0; e < f.n; ++e)
 #pragma unroll
-            for (int sIdx = 0; sIdx < 5; ++sIdx) {
-                const HexP h = s_hx[e].hp[sIdx];
-                const bool a3 = (sIdx == 1 || sIdx == 4);
-                float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
-                float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
-                hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
-            }
-        qcur = hex_acc_score(acc, epsD, epsF);
-    }
+        for (int sIdx = 0; sIdx < 5; ++sIdx) {
+            const HexP h = s_hx[e].hp[sIdx];
+            const bool a3 = (sIdx == 1 || sIdx == 4);
+            float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+            float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+            hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
+        }
+    return hex_acc_score(acc, f.epsD, f.epsF);
+}
 
+// the nested grid search of a hex vertex (lane layout as tet_levels)
+template <int NQ, int W>
+__device__ bool hex_levels(const SearchArgs& a, const HexFrame& f, const HexLds* s_hx, RedLds<W>& rd, float qcur,
+                           float& cx, float& cy, float& cz)
+{
+    constexpr int CPL = NQ / W;
+    static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
+    const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
     const int lane = tid & 63, i0 = (tid >> 6) * CPL;
     const int j = lane % NQ, k = lane / NQ;
     const bool active = lane < NQ * NQ;
-    float cx = x0x, cy = x0y, cz = x0z;
     float ds = a.gridScale;
     bool moved = false;
     for (int d = 0; d < a.depth; ++d) {
-        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
-        const float gyM = cy + ds * ey, gym = cy - ds * ey;
-        const float gzM = cz + ds * ez, gzm = cz - ds * ez;
-        const float py = (grid_coord(a.aff, j, NQ, gym, gyM) - x0y) * sg;
-        const float pz = (grid_coord(a.aff, k, NQ, gzm, gzM) - x0z) * sg;
+        const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
+        const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
+        const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
+        const float py = (grid_coord(a.aff, j, NQ, gym, gyM) - f.x0y) * f.sg;
+        const float pz = (grid_coord(a.aff, k, NQ, gzm, gzM) - f.x0z) * f.sg;
         float px[CPL], t3[CPL];
 #pragma unroll
         for (int i = 0; i < CPL; ++i) {
-            px[i] = (grid_coord(a.aff, i0 + i, NQ, gxm, gxM) - x0x) * sg;
+            px[i] = (grid_coord(a.aff, i0 + i, NQ, gxm, gxM) - f.x0x) * f.sg;
             t3[i] = 3.0f * px[i];
         }
         const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
         HexAcc acc[CPL];
 #pragma unroll
-        for (int i = 0; i < CPL; ++i) acc[i] = acc0;
-        for (int e = 0; e < n; ++e) {
+        for (int i = 0; i < CPL; ++i) acc[i] = f.acc0;
+        for (int e = 0; e < f.n; ++e) {
 #pragma unroll
             for (int sIdx = 0; sIdx < 5; ++sIdx) {
                 const HexP h = s_hx[e].hp[sIdx];
@@ -329,7 +370,7 @@ __device__ void hex_vertex(const SearchArgs& a, int vid, bool emin, HexLds* s_hx
         int bi = 0;
 #pragma unroll
         for (int i = 0; i < CPL; ++i) {
-            float q = hex_acc_score(acc[i], epsD, epsF);
+            float q = hex_acc_score(acc[i], f.epsD, f.epsF);
             if (q > best) { best = q; bi = i; }
         }
         if (!active) best = -FLT_MAX;
@@ -345,7 +386,19 @@ __device__ void hex_vertex(const SearchArgs& a, int vid, bool emin, HexLds* s_hx
         }
         ds = ds * a.shrink;
     }
-    if (moved && tid == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    return moved;
+}
+
+template <int NQ, int W>
+__device__ void hex_vertex(const SearchArgs& a, int vid, bool emin, HexLds* s_hx, RedLds<W>& rd)
+{
+    const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const HexFrame f = hex_stage<W>(a, vid, emin, s_hx, rd);
+    const float qcur = hex_score(f, s_hx, 0.0f, 0.0f, 0.0f);
+    float cx = f.x0x, cy = f.x0y, cz = f.x0z;
+    if (hex_levels<NQ, W>(a, f, s_hx, rd, qcur, cx, cy, cz) && tid == 0) {
+        a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz;
+    }
 }
 
 // ------------------------------------------------------------- flat tris
@@ -470,7 +523,8 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v)
 // for `depthL` levels with the window shrunk by 2/3 per level; the centre
 // moves to a level's best candidate when it is strictly better than the best
 // quality so far, which starts at q(v).  Fallback rule: when no candidate of
-// any level beats q(v), the vertex gets the plain full search, unchanged.
+// any level beats q(v), the same wave runs the plain full search from v on
+// the ring it has staged, so the result is the plain path's, bit for bit.
 struct LearnArgs {
     const float* Y;
     const float* L;
@@ -478,63 +532,18 @@ struct LearnArgs {
     int depthL;
     float frac;
     int* fallbacks;   // running total
-    int* list;        // this class's fallback vertices (appended)
-    int* cnt;         // their number (zeroed before the class)
 };
 
-template <int NQ>
-__device__ void tet_vertex_learned(const SearchArgs& a, const LearnArgs& la, int vid, int ci, TetC* s_tc,
-                                   RedLds<1>& rd)
+// the proposal window search; score(p') scores one candidate in frame
+// coordinates.  Returns whether a candidate beat q0, and the best position.
+template <class S>
+__device__ bool learned_window(const LearnArgs& la, int ci, float x0x, float x0y, float x0z, float ex, float ey,
+                               float ez, float sg, float gridScale, const S& score, float q0, float& cx, float& cy,
+                               float& cz)
 {
-    const int lane = threadIdx.x;
-    const int r0 = a.ringRow[vid];
-    const int n = a.ringRow[vid + 1] - r0;
-    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
-    const bool emin = a.extentMin != 0;
-    float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    for (int e = lane; e < n; e += 64) {
-        const int* h = a.ringVal + (size_t)(r0 + e) * 3;
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            int v = h[t];
-            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
-            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
-            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
-            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
-            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
-        }
-    }
-    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
-    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
-    mabs = wave_max(mabs);
-    const float sg = pow2_scale(mabs);
-    const float isg = 1.0f / sg;
-    const float is3 = ((1.0f * isg) * isg) * isg;
-    for (int e = lane; e < n; e += 64) {
-        const int* h = a.ringVal + (size_t)(r0 + e) * 3;
-        f3 p[3];
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            int v = h[t];
-            p[t] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
-        }
-        s_tc[e] = tet_consts(p[0], p[1], p[2]);
-    }
-    __syncthreads();
-    auto score = [&](float px, float py, float pz) {
-        const float Qyz = 3.0f * fmaf(pz, pz, py * py);
-        const float t3 = 3.0f * px;
-        TetAcc acc = tet_acc0();
-        for (int e = 0; e < n; ++e) {
-            const TetC c = s_tc[e];
-            float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
-            float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
-            tet_acc_add(acc, fmaf(c.nnx, px, Tyz), fmaf(px, t3 + c.m2sx, Ryz));
-        }
-        return tet_acc_score(acc, is3);
-    };
-    float qbest = score(0.0f, 0.0f, 0.0f);
-    float cx = x0x, cy = x0y, cz = x0z;
+    const int lane = threadIdx.x & 63;
+    float qbest = q0;
+    cx = x0x; cy = x0y; cz = x0z;
     if (la.mode == 1) {
         const float l = la.L[ci];
         cx = x0x + l * la.Y[3 * ci];
@@ -544,7 +553,7 @@ __device__ void tet_vertex_learned(const SearchArgs& a, const LearnArgs& la, int
     constexpr int NL = 4;
     const float affL = 1.0f / (float)(NL - 1);
     const int li = lane % NL, lj = (lane / NL) % NL, lk = lane / (NL * NL);
-    float ds = la.frac * a.gridScale;
+    float ds = la.frac * gridScale;
     bool moved = false;
     for (int d = 0; d < la.depthL; ++d) {
         const float gxM = cx + ds * ex, gxm = cx - ds * ex;
@@ -563,114 +572,45 @@ __device__ void tet_vertex_learned(const SearchArgs& a, const LearnArgs& la, int
         }
         ds = ds * (2.0f / (float)(NL - 1));
     }
-    if (moved) {
-        if (lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
-        return;
+    return moved;
+}
+
+template <int NQ>
+__device__ void tet_vertex_learned(const SearchArgs& a, const LearnArgs& la, int vid, int ci, TetC* s_tc,
+                                   RedLds<1>& rd)
+{
+    const int lane = threadIdx.x & 63;
+    const TetFrame f = tet_stage<1>(a, vid, a.extentMin != 0, s_tc, rd);
+    const float q0 = tet_score(f, s_tc, 0.0f, 0.0f, 0.0f);
+    float cx, cy, cz;
+    bool moved = learned_window(la, ci, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, f.sg, a.gridScale,
+                                [&](float px, float py, float pz) { return tet_score(f, s_tc, px, py, pz); }, q0, cx,
+                                cy, cz);
+    if (!moved) {
+        if (lane == 0) atomicAdd(la.fallbacks, 1);
+        cx = f.x0x; cy = f.x0y; cz = f.x0z;
+        moved = tet_levels<NQ, 1>(a, f, s_tc, rd, q0, cx, cy, cz);
     }
-    if (lane == 0) {
-        atomicAdd(la.fallbacks, 1);
-        la.list[atomicAdd(la.cnt, 1)] = vid;
-    }
-    (void)rd;
+    if (moved && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
 }
 
 template <int NQ>
 __device__ void hex_vertex_learned(const SearchArgs& a, const LearnArgs& la, int vid, int ci, HexLds* s_hx,
                                    RedLds<1>& rd)
 {
-    const int lane = threadIdx.x;
-    const int r0 = a.ringRow[vid];
-    const int n = a.ringRow[vid + 1] - r0;
-    const float x0x = a.x[vid], x0y = a.y[vid], x0z = a.z[vid];
-    const bool emin = a.extentMin != 0;
-    float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    for (int e = lane; e < n; e += 64) {
-        const int* h = a.ringVal + (size_t)(r0 + e) * 7;
-        for (int t = 0; t < 7; ++t) {
-            int v = h[t];
-            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
-            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
-            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
-            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
-            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
-        }
+    const int lane = threadIdx.x & 63;
+    const HexFrame f = hex_stage<1>(a, vid, a.extentMin != 0, s_hx, rd);
+    const float q0 = hex_score(f, s_hx, 0.0f, 0.0f, 0.0f);
+    float cx, cy, cz;
+    bool moved = learned_window(la, ci, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, f.sg, a.gridScale,
+                                [&](float px, float py, float pz) { return hex_score(f, s_hx, px, py, pz); }, q0, cx,
+                                cy, cz);
+    if (!moved) {
+        if (lane == 0) atomicAdd(la.fallbacks, 1);
+        cx = f.x0x; cy = f.x0y; cz = f.x0z;
+        moved = hex_levels<NQ, 1>(a, f, s_hx, rd, q0, cx, cy, cz);
     }
-    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
-    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
-    mabs = wave_max(mabs);
-    const float sg = pow2_scale(mabs);
-    const float s2 = sg * sg, s3 = s2 * sg;
-    const float epsD = kHexEps * s3, epsF = kHexEps * s2;
-    for (int e = lane; e < n; e += 64) {
-        const int* h = a.ringVal + (size_t)(r0 + e) * 7;
-        f3 r[8];
-        r[0] = { 0.0f, 0.0f, 0.0f };
-        for (int t = 0; t < 7; ++t) {
-            int v = h[t];
-            r[t + 1] = { (a.x[v] - x0x) * sg, (a.y[v] - x0y) * sg, (a.z[v] - x0z) * sg };
-        }
-        HexLds Lh;
-        hex_consts(r, Lh.cd, Lh.cF, Lh.hp);
-        s_hx[e] = Lh;
-    }
-    __syncthreads();
-    HexAcc acc0 = hex_acc0();
-    for (int e = 0; e < n; ++e)
-#pragma unroll
-        for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(acc0, s_hx[e].cd[sIdx], s_hx[e].cF[sIdx]);
-    auto score = [&](float px, float py, float pz) {
-        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
-        HexAcc acc = acc0;
-        for (int e = 0; e < n; ++e)
-#pragma unroll
-            for (int sIdx = 0; sIdx < 5; ++sIdx) {
-                const HexP h = s_hx[e].hp[sIdx];
-                const bool a3 = (sIdx == 1 || sIdx == 4);
-                float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
-                float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
-                hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
-            }
-        return hex_acc_score(acc, epsD, epsF);
-    };
-    float qbest = score(0.0f, 0.0f, 0.0f);
-    float cx = x0x, cy = x0y, cz = x0z;
-    if (la.mode == 1) {
-        const float l = la.L[ci];
-        cx = x0x + l * la.Y[3 * ci];
-        cy = x0y + l * la.Y[3 * ci + 1];
-        cz = x0z + l * la.Y[3 * ci + 2];
-    }
-    constexpr int NL = 4;
-    const float affL = 1.0f / (float)(NL - 1);
-    const int li = lane % NL, lj = (lane / NL) % NL, lk = lane / (NL * NL);
-    float ds = la.frac * a.gridScale;
-    bool moved = false;
-    for (int d = 0; d < la.depthL; ++d) {
-        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
-        const float gyM = cy + ds * ey, gym = cy - ds * ey;
-        const float gzM = cz + ds * ez, gzm = cz - ds * ez;
-        const float px = grid_coord(affL, li, NL, gxm, gxM);
-        const float py = grid_coord(affL, lj, NL, gym, gyM);
-        const float pz = grid_coord(affL, lk, NL, gzm, gzM);
-        float best = score((px - x0x) * sg, (py - x0y) * sg, (pz - x0z) * sg);
-        int cand = lane;
-        wave_argmax(best, cand);
-        if (qbest < best) {
-            cx = __shfl(px, cand); cy = __shfl(py, cand); cz = __shfl(pz, cand);
-            qbest = best;
-            moved = true;
-        }
-        ds = ds * (2.0f / (float)(NL - 1));
-    }
-    if (moved) {
-        if (lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
-        return;
-    }
-    if (lane == 0) {
-        atomicAdd(la.fallbacks, 1);
-        la.list[atomicAdd(la.cnt, 1)] = vid;
-    }
-    (void)rd;
+    if (moved && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
 }
 
 template <int NQ>
@@ -688,39 +628,12 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
     hex_vertex_learned<NQ>(a, la, a.cStart + blockIdx.x, blockIdx.x, s_hxl, rd);
 }
 
-// plain search of the vertices list[0 .. *cnt) (the fallbacks of a class);
-// blocks beyond the count return at once
-template <int NQ, int W>
-__global__ __launch_bounds__(64 * W) void k_search_list_tet(SearchArgs a, const int* __restrict__ list,
-                                                           const int* __restrict__ cnt)
-{
-    extern __shared__ TetC s_tcf[];
-    __shared__ RedLds<W> rd;
-    if ((int)blockIdx.x >= *cnt) return;
-    tet_vertex<NQ, W>(a, list[blockIdx.x], a.extentMin != 0, s_tcf, rd);
-}
-template <int NQ, int W>
-__global__ __launch_bounds__(64 * W) void k_search_list_hex(SearchArgs a, const int* __restrict__ list,
-                                                           const int* __restrict__ cnt)
-{
-    extern __shared__ HexLds s_hxf[];
-    __shared__ RedLds<W> rd;
-    if ((int)blockIdx.x >= *cnt) return;
-    hex_vertex<NQ, W>(a, list[blockIdx.x], a.extentMin != 0, s_hxf, rd);
-}
-template <int NQ, int W>
-static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* cnt, int grid, size_t lds, hipStream_t s)
-{
-    if (kind == VSDMO_TET) k_search_list_tet<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
-    else k_search_list_hex<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
-}
-
 void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y, const float* L, int mode, int depthL,
-                           float frac, int* fallbacks, int* list, int* cnt, size_t lds, hipStream_t s)
+                           float frac, int* fallbacks, size_t lds, hipStream_t s)
 {
     const int nb = a.cEnd - a.cStart;
     if (nb <= 0) return;
-    LearnArgs la{ Y, L, mode, depthL, frac, fallbacks, list, cnt };
+    LearnArgs la{ Y, L, mode, depthL, frac, fallbacks };
 #define VSDMO_LEARN_CASE(Q)                                                                   \
     case Q:                                                                                  \
         if (kind == VSDMO_TET) k_search_tet_learned<Q><<<nb, 64, lds, s>>>(a, la);           \
@@ -734,14 +647,6 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y
     default: break;
     }
 #undef VSDMO_LEARN_CASE
-    // second pass: the plain full search of the fallbacks, W waves per vertex
-    switch (nq) {
-    case 2: list_nqw<2, 2>(kind, a, list, cnt, nb, lds, s); break;
-    case 4: list_nqw<4, 4>(kind, a, list, cnt, nb, lds, s); break;
-    case 6: list_nqw<6, 2>(kind, a, list, cnt, nb, lds, s); break;
-    case 8: list_nqw<8, 4>(kind, a, list, cnt, nb, lds, s); break;
-    default: break;
-    }
 }
 
 // ------------------------------------------------------- split launches

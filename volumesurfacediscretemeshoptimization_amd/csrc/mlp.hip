@@ -239,11 +239,10 @@ __global__ void k_features(const float* __restrict__ x, const float* __restrict_
     v0[3 * i] = x[v]; v0[3 * i + 1] = y[v]; v0[3 * i + 2] = z[v];
 }
 
-// Fused inference of 64 rows per block (4 waves, one 16-row tile each):
-// features (from the positions, or rows of Xin) -> LDS, layers 1 and 2 on
-// MFMA with the weights' 16-byte rows read straight from global memory,
-// bias + GELU to BF16 in LDS, layer 3 (128 -> 3) on the VALU.  Writes Y and
-// L of rows [0, n).
+// Fused inference: features (from the positions, or rows of Xin) -> LDS,
+// layers 1 and 2 on MFMA with the weights' 16-byte rows read straight from
+// global memory, bias + GELU to BF16 in LDS, layer 3 (128 -> 3) on the
+// VALU.  Writes Y and L of rows [0, n).
 constexpr int kLdx = kIn + 8;    // LDS row pitches (bf16), multiples of 8
 constexpr int kLdh = kHid + 8;
 struct InferArgs {
@@ -257,46 +256,48 @@ struct InferArgs {
     const float* w;
     float* Y;
     float* L;
-    int* cnt;              // optional counter zeroed by block 0
 };
+
+// 16 rows per block of 4 waves.  Features: 16 threads per row gather two
+// neighbours each; L is summed in neighbour order (the order of k_features).
+// Layers 1-2: wave w computes columns [32w, 32w + 32) -- two 16x16 tiles of
+// mfma_f32_16x16x32_bf16 (A: 16 LDS rows, lane l holds row l%16, k = 8(l/16)
+// + 0..7; B: 16-byte weight rows from global memory; D: column l%16, rows
+// 4(l/16) + 0..3), then bias + GELU to BF16 in LDS.  Layer 3 (128 -> 3) on
+// the VALU, one thread per (row, output).
+constexpr int kInferRows = 16;
 
 __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a)
 {
-    __shared__ __align__(16) uint16_t sX[64 * kLdx];
-    __shared__ __align__(16) uint16_t sH[2 * 64 * kLdh];   // H1 | H2; the feature gathers use it first
-    uint16_t* sH1 = sH;
-    uint16_t* sH2 = sH + 64 * kLdh;
+    constexpr int R = kInferRows;
+    __shared__ __align__(16) uint16_t sX[R * kLdx];
+    __shared__ __align__(16) uint16_t sH1[R * kLdh];
+    __shared__ __align__(16) uint16_t sH2[R * kLdh];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int row0 = blockIdx.x * 64;
-    // features
-    if (blockIdx.x == 0 && tid == 0 && a.cnt) *a.cnt = 0;   // fallback count of the search that follows
+    const int row0 = blockIdx.x * R;
     if (a.Xin) {
-        for (int i = tid; i < 64 * kIn; i += 256) {
+        for (int i = tid; i < R * kIn; i += 256) {
             int r = i / kIn, c = i - r * kIn;
             sX[r * kLdx + c] = (row0 + r < a.n) ? a.Xin[(size_t)(row0 + r) * kIn + c] : (uint16_t)0;
         }
     } else {
-        // 4 threads per row gather 8 neighbours each; L is then summed in
-        // neighbour order by one thread (the order of k_features)
-        static_assert(sizeof(float) * 64 * kNbr * 4 <= sizeof(uint16_t) * 2 * 64 * kLdh, "feature scratch");
-        auto sRel = reinterpret_cast<float (*)[kNbr][3]>(sH);
-        auto sDist = reinterpret_cast<float (*)[kNbr]>(reinterpret_cast<float*>(sH) + 64 * kNbr * 3);
-        __shared__ float sL[64];
-        __shared__ int sCnt[64];
-        const int r = tid >> 2, q = tid & 3;
+        __shared__ float sRel[R][kNbr][3];
+        __shared__ float sDist[R][kNbr];
+        __shared__ float sL[R];
+        __shared__ int sCnt[R];
+        const int r = tid >> 4, q = tid & 15;
         const int v = a.lo + row0 + r;
         const bool live = row0 + r < a.n;
-        int cntq = 0;
         if (live) {
             const int* nb = a.nbr + (size_t)v * kNbr;
             const float vx = a.x[v], vy = a.y[v], vz = a.z[v];
-            for (int k = q * 8; k < q * 8 + 8; ++k) {
+#pragma unroll
+            for (int k = 2 * q; k < 2 * q + 2; ++k) {
                 const int u = nb[k];
                 if (u < 0) { sDist[r][k] = -1.0f; continue; }
                 const float dx = a.x[u] - vx, dy = a.y[u] - vy, dz = a.z[u] - vz;
                 sRel[r][k][0] = dx; sRel[r][k][1] = dy; sRel[r][k][2] = dz;
                 sDist[r][k] = sqrtf((dx * dx + dy * dy) + dz * dz);
-                ++cntq;
             }
         }
         __syncthreads();
@@ -320,7 +321,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a)
         const float l = sL[r];
         const int cnt = sCnt[r];
         uint16_t* o = sX + r * kLdx;
-        for (int k = q * 8; k < q * 8 + 8; ++k) {
+#pragma unroll
+        for (int k = 2 * q; k < 2 * q + 2; ++k) {
             if (k < cnt) {
                 o[3 * k] = f2bf(sRel[r][k][0] / l);
                 o[3 * k + 1] = f2bf(sRel[r][k][1] / l);
@@ -329,58 +331,59 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a)
                 o[3 * k] = 0; o[3 * k + 1] = 0; o[3 * k + 2] = 0;
             }
         }
-        (void)cntq;
     }
     __syncthreads();
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
-    // layer 1: 16 rows x 128 columns per wave
+    // layer 1: 16 rows x 32 columns per wave
     {
-        f32x4 acc[8];
+        f32x4 acc[2];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
+        for (int t = 0; t < 2; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
         for (int k0 = 0; k0 < kIn; k0 += 32) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sX + (wave * 16 + r16) * kLdx + k0 + kq);
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sX + r16 * kLdx + k0 + kq);
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.wb + kOW1 + (size_t)(t * 16 + r16) * kIn + k0 + kq);
+            for (int t = 0; t < 2; ++t) {
+                const int col = (2 * wave + t) * 16 + r16;
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.wb + kOW1 + (size_t)col * kIn + k0 + kq);
                 acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int n = t * 16 + r16;
+        for (int t = 0; t < 2; ++t) {
+            const int n = (2 * wave + t) * 16 + r16;
             const float b = a.w[kOB1 + n];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sH1[(wave * 16 + (lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
+            for (int r = 0; r < 4; ++r) sH1[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
         }
     }
     __syncthreads();
     // layer 2
     {
-        f32x4 acc[8];
+        f32x4 acc[2];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
+        for (int t = 0; t < 2; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
         for (int k0 = 0; k0 < kHid; k0 += 32) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sH1 + (wave * 16 + r16) * kLdh + k0 + kq);
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sH1 + r16 * kLdh + k0 + kq);
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.wb + kOW2 + (size_t)(t * 16 + r16) * kHid + k0 + kq);
+            for (int t = 0; t < 2; ++t) {
+                const int col = (2 * wave + t) * 16 + r16;
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.wb + kOW2 + (size_t)col * kHid + k0 + kq);
                 acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int n = t * 16 + r16;
+        for (int t = 0; t < 2; ++t) {
+            const int n = (2 * wave + t) * 16 + r16;
             const float b = a.w[kOB2 + n];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sH2[(wave * 16 + (lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
+            for (int r = 0; r < 4; ++r) sH2[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
         }
     }
     __syncthreads();
     // layer 3 on the VALU: thread -> (row, output)
-    if (tid < 64 * kOut) {
+    if (tid < R * kOut) {
         const int r = tid / kOut, o = tid - r * kOut;
         if (row0 + r < a.n) {
             float acc = 0.0f;
@@ -393,12 +396,12 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a)
 }
 
 void launch_mlp_infer(vsdmo_mlp* m, const float* x, const float* y, const float* z, const int* nbr, int lo, int hi,
-                      const uint16_t* Xin, float* Y, float* L, int* cnt, hipStream_t s)
+                      const uint16_t* Xin, float* Y, float* L, hipStream_t s)
 {
     const int n = hi - lo;
     if (n <= 0) return;
-    InferArgs a{ x, y, z, nbr, lo, n, Xin, m->wb.p, m->w.p, Y, L, cnt };
-    k_mlp_infer<<<(n + 63) / 64, 256, 0, s>>>(a);
+    InferArgs a{ x, y, z, nbr, lo, n, Xin, m->wb.p, m->w.p, Y, L };
+    k_mlp_infer<<<(n + kInferRows - 1) / kInferRows, 256, 0, s>>>(a);
 }
 
 __global__ void k_targets(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, int lo,
@@ -654,7 +657,7 @@ vsdmo_status vsdmo_mlp_forward(vsdmo_mlp* m, const float* X, int32_t n, float* Y
         std::vector<uint16_t> xb((size_t)n * kIn);
         for (size_t i = 0; i < xb.size(); ++i) xb[i] = f2bf(X[i]);
         m->X.upload(xb.data(), xb.size(), m->stream);
-        launch_mlp_infer(m, nullptr, nullptr, nullptr, nullptr, 0, n, m->X.p, m->Y.p, nullptr, nullptr, m->stream);
+        launch_mlp_infer(m, nullptr, nullptr, nullptr, nullptr, 0, n, m->X.p, m->Y.p, nullptr, m->stream);
         m->Y.download(Y, (size_t)n * kOut, m->stream);
         VSDMO_HIP(hipStreamSynchronize(m->stream));
         return VSDMO_OK;

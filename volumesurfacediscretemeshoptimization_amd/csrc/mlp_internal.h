@@ -81,8 +81,7 @@ namespace vsdmo {
 void mlp_forward_dev(vsdmo_mlp* m, const uint16_t* X, int n, float* Y, hipStream_t s);
 void mlp_reserve(vsdmo_mlp* m, int n);
 // fused inference (features + 3 layers) of vertices [lo, hi) or of rows of
-// Xin (n = hi - lo); Y (n x 3) and, when computing features, L (n); *cnt
-// (optional) is zeroed for the search that follows
+// Xin (n = hi - lo); Y (n x 3) and, when computing features, L (n)
 void launch_mlp_infer(vsdmo_mlp* m, const float* x, const float* y, const float* z, const int* nbr, int lo, int hi,
-                      const uint16_t* Xin, float* Y, float* L, int* cnt, hipStream_t s);
+                      const uint16_t* Xin, float* Y, float* L, hipStream_t s);
 }  // namespace vsdmo
