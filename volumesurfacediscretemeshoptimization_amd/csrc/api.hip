@@ -91,7 +91,8 @@ struct vsdmo_opt {
     int64_t launches = 0;
     DevBuf<float> q, pmin, omin;
     DevBuf<double> psum, osum;
-    std::vector<hipEvent_t> ev;   // pairs around timed launches
+    std::vector<hipEvent_t> ev;   // pool of events; pairs [0, ev_used) bracket timed launches
+    size_t ev_used = 0;
     // surface-vertex state (fixed_boundary == 0)
     bool surface = false;
     int nSectors = 0;
@@ -108,6 +109,7 @@ struct vsdmo_opt {
     float learn_frac = 0.25f;
     DevBuf<float> lL, lY;
     DevBuf<int> lFallback;           // running count of fallbacks to the full search
+    DevBuf<int> lList, lCnt;         // fallback list of a class; per-class list lengths
     int64_t s_learned = 0, fallbacks = 0;
     vsdmo_dataset* rec = nullptr;   // training pairs are recorded while set
     std::string fail_dump;          // mesh file written when the quality guard fails
@@ -503,55 +505,23 @@ static void dispatch_search(vsdmo_opt* o, int kind, const SearchArgs& a, size_t 
         launch_search(kind, o->p.nq, w == 0 ? 1 : w, a, lds, s);
 }
 
-static void timed_launch(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds)
+// HIP events around every search launch while timing is on.  The events are
+// pooled and reused: creating two per launch would cost host time inside the
+// iteration loop that is being timed.
+static void timed(vsdmo_opt* o, const std::function<void()>& launch)
 {
     hipStream_t s = o->m->stream;
     if (o->timing) {
-        hipEvent_t e0, e1;
-        VSDMO_HIP(hipEventCreate(&e0));
-        VSDMO_HIP(hipEventCreate(&e1));
-        VSDMO_HIP(hipEventRecord(e0, s));
-        dispatch_search(o, kind, a, lds, s);
-        VSDMO_HIP(hipEventRecord(e1, s));
-        o->ev.push_back(e0);
-        o->ev.push_back(e1);
-    } else {
-        dispatch_search(o, kind, a, lds, s);
-    }
-    o->launches++;
-}
-
-static void timed_learned(vsdmo_opt* o, int kind, const SearchArgs& a, size_t lds)
-{
-    hipStream_t s = o->m->stream;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (o->timing) {
-        VSDMO_HIP(hipEventCreate(&e0));
-        VSDMO_HIP(hipEventCreate(&e1));
-        VSDMO_HIP(hipEventRecord(e0, s));
-    }
-    launch_search_learned(kind, o->p.nq, a, o->lY.p, o->lL.p, o->learn_mode, o->learn_depth, o->learn_frac,
-                          o->lFallback.p, lds, s);
-    if (o->timing) {
-        VSDMO_HIP(hipEventRecord(e1, s));
-        o->ev.push_back(e0);
-        o->ev.push_back(e1);
-    }
-    o->launches++;
-}
-
-static void timed_surface(vsdmo_opt* o, const std::function<void()>& launch)
-{
-    hipStream_t s = o->m->stream;
-    if (o->timing) {
-        hipEvent_t e0, e1;
-        VSDMO_HIP(hipEventCreate(&e0));
-        VSDMO_HIP(hipEventCreate(&e1));
-        VSDMO_HIP(hipEventRecord(e0, s));
+        const size_t k = o->ev_used;
+        while (o->ev.size() < k + 2) {
+            hipEvent_t e;
+            VSDMO_HIP(hipEventCreate(&e));
+            o->ev.push_back(e);
+        }
+        VSDMO_HIP(hipEventRecord(o->ev[k], s));
         launch();
-        VSDMO_HIP(hipEventRecord(e1, s));
-        o->ev.push_back(e0);
-        o->ev.push_back(e1);
+        VSDMO_HIP(hipEventRecord(o->ev[k + 1], s));
+        o->ev_used += 2;
     } else {
         launch();
     }
@@ -560,16 +530,14 @@ static void timed_surface(vsdmo_opt* o, const std::function<void()>& launch)
 
 static void collect_timing(vsdmo_opt* o)
 {
-    if (o->ev.empty()) return;
+    if (o->ev_used == 0) return;
     VSDMO_HIP(hipStreamSynchronize(o->m->stream));
-    for (size_t i = 0; i + 1 < o->ev.size(); i += 2) {
+    for (size_t i = 0; i + 1 < o->ev_used; i += 2) {
         float ms = 0.0f;
         VSDMO_HIP(hipEventElapsedTime(&ms, o->ev[i], o->ev[i + 1]));
         o->kernel_ms += ms;
-        (void)hipEventDestroy(o->ev[i]);
-        (void)hipEventDestroy(o->ev[i + 1]);
     }
-    o->ev.clear();
+    o->ev_used = 0;
 }
 
 vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
@@ -588,6 +556,9 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
         for (int it = 0; it < n; ++it) {
             if (m->elem == VSDMO_TET || m->elem == VSDMO_HEX) {
                 size_t lds = search_lds_bytes(m->elem, m->maxRing);
+                // the classes' fallback-list lengths start at zero (one fill per iteration)
+                if (o->learn_mode != 0 && !o->rec)
+                    launch_fill_int(o->lCnt.p, (int)m->colOffInner.size(), 0, m->stream);
                 for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c) {
                     a.cStart = m->colOffInner[c]; a.cEnd = m->colOffInner[c + 1];
                     const int nc = a.cEnd - a.cStart;
@@ -600,10 +571,14 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                         if (o->learn_mode == 1)
                             launch_mlp_infer(o->mlp, m->x.p, m->y.p, m->z.p, m->nbr.p, a.cStart, a.cEnd, nullptr,
                                              o->lY.p, o->lL.p, m->stream);
-                        timed_learned(o, m->elem, a, lds);
+                        timed(o, [&]() {
+                            launch_search_learned(m->elem, o->p.nq, a, o->lY.p, o->lL.p, o->learn_mode, o->learn_depth,
+                                                  o->learn_frac, o->lFallback.p, o->lList.p, o->lCnt.p + c, m->maxRing,
+                                                  m->stream);
+                        });
                         o->s_learned += nc;
                     } else {
-                        timed_launch(o, m->elem, a, lds);
+                        timed(o, [&]() { dispatch_search(o, m->elem, a, lds, m->stream); });
                     }
                     if (take > 0) {
                         launch_targets(m->x.p, m->y.p, m->z.p, a.cStart, a.cStart + take, ds->L.p, ds->v0.p,
@@ -636,7 +611,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                 for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {
                     int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
                     snapshot_launch(sa, m->stream);
-                    timed_surface(o, [&]() { surf_fan2d_launch(sa, lo, hi, o->p.nq, m->stream); });
+                    timed(o, [&]() { surf_fan2d_launch(sa, lo, hi, o->p.nq, m->stream); });
                     o->s_surf += hi - lo;
                 }
                 if (!o->p.fixed_boundary) {
@@ -645,7 +620,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                         if (m->elem == VSDMO_QUAD) {   // quad fan diagonals read the pass snapshot
                             snapshot_launch(sa, m->stream);
                         }
-                        timed_surface(o, [&]() { surf_fan1d_launch(sa, lo, hi, o->p.nq, m->stream); });
+                        timed(o, [&]() { surf_fan1d_launch(sa, lo, hi, o->p.nq, m->stream); });
                         for (int v = lo; v < hi; ++v) o->s_feat += o->opt1d_host[v];
                     }
                 }
@@ -655,7 +630,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                 size_t lds = search_lds_bytes(VSDMO_TRI, m->maxHE);
                 for (size_t c = 0; c + 1 < m->colOffFree.size(); ++c) {
                     a.cStart = m->colOffFree[c]; a.cEnd = m->colOffFree[c + 1];
-                    timed_launch(o, VSDMO_TRI, a, lds);
+                    timed(o, [&]() { dispatch_search(o, VSDMO_TRI, a, lds, m->stream); });
                     o->s_surf += a.cEnd - a.cStart;
                 }
                 if (o->surface && o->last_min > 0.0f) {
@@ -787,6 +762,9 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
             return fail(VSDMO_NOT_SUPPORTED, "the learned proposal is for tet and hex inner vertices");
         if (mode != 0 && (depth < 1 || depth > 8 || !(window_frac > 0.0f)))
             return fail(VSDMO_INVALID_ARG, "depth must be in [1,8] and window_frac > 0");
+        if (mode != 0 && m->maxRing > learned_ring_cap(m->elem))
+            return fail(VSDMO_NOT_SUPPORTED, "the learned search holds one-rings of at most " +
+                                                 std::to_string(learned_ring_cap(m->elem)) + " elements");
         o->learn_mode = mode;
         o->mlp = mode == 1 ? mlp : nullptr;
         o->learn_depth = depth;
@@ -800,8 +778,9 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
             launch_fill_int(o->lFallback.p, 1, 0, m->stream);
         }
         if ((int)o->lL.n < maxc) {
-            o->lL.alloc(maxc); o->lY.alloc((size_t)3 * maxc);
+            o->lL.alloc(maxc); o->lY.alloc((size_t)3 * maxc); o->lList.alloc(maxc);
         }
+        if (!o->lCnt.p) o->lCnt.alloc(std::max<size_t>(m->colOffInner.size(), 1));
         if (mode == 1) {
             build_neighbour_table(*m);
             VSDMO_HIP(hipStreamSynchronize(mlp->stream));
@@ -854,6 +833,7 @@ vsdmo_status vsdmo_opt_destroy(vsdmo_opt* o)
     return guarded([&]() -> vsdmo_status {
         if (!o) return VSDMO_OK;
         collect_timing(o);
+        for (hipEvent_t e : o->ev) (void)hipEventDestroy(e);
         o->m->n_opts--;
         delete o;
         return VSDMO_OK;

@@ -518,13 +518,13 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v)
 // ------------------------------------------------------ learned proposal
 // Vertex i of the colour class gets the proposal c = v + L_i * Y_i (network
 // output in units of its mean edge length L_i; mode 2: c = v).  A 4x4x4 grid
-// (one candidate per lane, candidate index = lane, lowest wins ties) is
-// searched around c on a window `frac` times the plain first-level window,
-// for `depthL` levels with the window shrunk by 2/3 per level; the centre
-// moves to a level's best candidate when it is strictly better than the best
-// quality so far, which starts at q(v).  Fallback rule: when no candidate of
-// any level beats q(v), the same wave runs the plain full search from v on
-// the ring it has staged, so the result is the plain path's, bit for bit.
+// (candidate index i + 4j + 16k, lowest wins ties) is searched around c on a
+// window `frac` times the plain first-level window, for `depthL` levels with
+// the window shrunk by 2/3 per level; the centre moves to a level's best
+// candidate when it is strictly better than the best quality so far, which
+// starts at q(v).  Fallback rule: when no candidate of any level beats q(v),
+// the vertex goes on the class's fallback list, whose plain full search runs
+// as a second launch; the result is the plain path's, bit for bit.
 struct LearnArgs {
     const float* Y;
     const float* L;
@@ -532,17 +532,73 @@ struct LearnArgs {
     int depthL;
     float frac;
     int* fallbacks;   // running total
+    int* list;        // this class's fallback vertices (appended)
+    int* cnt;         // their number (zero before the class)
 };
 
-// the proposal window search; score(p') scores one candidate in frame
-// coordinates.  Returns whether a candidate beat q0, and the best position.
-template <class S>
-__device__ bool learned_window(const LearnArgs& la, int ci, float x0x, float x0y, float x0z, float ex, float ey,
-                               float ez, float sg, float gridScale, const S& score, float q0, float& cx, float& cy,
-                               float& cz)
+// The proposal window runs 4 vertices per wave, 16 lanes each: lane s =
+// j + 4k of a group owns the candidates (i, j, k), i = 0..3, which share the
+// (y, z) part of every ring element, so one LDS read of an element's
+// constants serves 4 candidates (one candidate per lane would spend a third
+// of the wave's cycles waiting on LDS).
+constexpr int kLV = 4;            // vertices per wave in the proposal window
+constexpr int kLG = 64 / kLV;     // lanes per vertex
+
+template <int G>
+__device__ __forceinline__ float group_max(float v)
 {
-    const int lane = threadIdx.x & 63;
-    float qbest = q0;
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m));
+    return v;
+}
+template <int G>
+__device__ __forceinline__ float group_min(float v)
+{
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m));
+    return v;
+}
+template <int G>
+__device__ __forceinline__ void group_argmax(float& best, int& idx)
+{
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) {
+        float ob = __shfl_xor(best, m);
+        int oi = __shfl_xor(idx, m);
+        bool take = (ob > best) || (ob == best && oi < idx);
+        best = take ? ob : best;
+        idx = take ? oi : idx;
+    }
+}
+
+// window half-extents and frame scale of vid's ring from the lanes of one
+// group (G lanes, stride G over the ring elements of stride st)
+template <int G>
+__device__ void group_extents(const SearchArgs& a, int r0, int n, int st, int s, float x0x, float x0y, float x0z,
+                              float& ex, float& ey, float& ez, float& mabs)
+{
+    const bool emin = a.extentMin != 0;
+    ex = emin ? FLT_MAX : -FLT_MAX; ey = ex; ez = ex; mabs = 0.0f;
+    for (int e = s; e < n; e += G) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * st;
+        for (int t = 0; t < st; ++t) {
+            int v = h[t];
+            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
+            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+        }
+    }
+    if (emin) { ex = group_min<G>(ex); ey = group_min<G>(ey); ez = group_min<G>(ez); }
+    else { ex = group_max<G>(ex); ey = group_max<G>(ey); ez = group_max<G>(ez); }
+    mabs = group_max<G>(mabs);
+}
+
+// proposal centre of the vertex at class position ci (mode 1), or the vertex
+__device__ __forceinline__ void proposal(const LearnArgs& la, int ci, float x0x, float x0y, float x0z, float& cx,
+                                         float& cy, float& cz)
+{
     cx = x0x; cy = x0y; cz = x0z;
     if (la.mode == 1) {
         const float l = la.L[ci];
@@ -550,94 +606,249 @@ __device__ bool learned_window(const LearnArgs& la, int ci, float x0x, float x0y
         cy = x0y + l * la.Y[3 * ci + 1];
         cz = x0z + l * la.Y[3 * ci + 2];
     }
+}
+
+template <int NQ>
+__global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnArgs la, int ringCap)
+{
+    extern __shared__ TetC s_tcl[];   // kLV x ringCap
+    __shared__ RedLds<1> rd;
     constexpr int NL = 4;
-    const float affL = 1.0f / (float)(NL - 1);
-    const int li = lane % NL, lj = (lane / NL) % NL, lk = lane / (NL * NL);
-    float ds = la.frac * gridScale;
+    const int lane = threadIdx.x & 63, g = lane / kLG, s = lane % kLG;
+    const int nc = a.cEnd - a.cStart, ci0 = kLV * blockIdx.x;
+    // a group past the end of the class mirrors the class's last vertex and writes nothing
+    const int ci = min(ci0 + g, nc - 1);
+    const int vid = a.cStart + ci;
+    TetC* tc = s_tcl + (size_t)g * ringCap;
+    TetFrame f;
+    const int r0 = a.ringRow[vid];
+    f.n = a.ringRow[vid + 1] - r0;
+    f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
+    float mabs;
+    group_extents<kLG>(a, r0, f.n, 3, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
+    f.sg = pow2_scale(mabs);
+    const float isg = 1.0f / f.sg;
+    f.is3 = ((1.0f * isg) * isg) * isg;
+    for (int e = s; e < f.n; e += kLG) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 3;
+        f3 p[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            int v = h[t];
+            p[t] = { (a.x[v] - f.x0x) * f.sg, (a.y[v] - f.x0y) * f.sg, (a.z[v] - f.x0z) * f.sg };
+        }
+        tc[e] = tet_consts(p[0], p[1], p[2]);
+    }
+    __syncthreads();
+    const float q0 = tet_score(f, tc, 0.0f, 0.0f, 0.0f);
+
+    float cx, cy, cz;
+    proposal(la, ci, f.x0x, f.x0y, f.x0z, cx, cy, cz);
+    float qbest = q0;
     bool moved = false;
+    const float affL = 1.0f / (float)(NL - 1);
+    const int lj = s % NL, lk = s / NL;
+    float ds = la.frac * a.gridScale;
     for (int d = 0; d < la.depthL; ++d) {
-        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
-        const float gyM = cy + ds * ey, gym = cy - ds * ey;
-        const float gzM = cz + ds * ez, gzm = cz - ds * ez;
-        const float px = grid_coord(affL, li, NL, gxm, gxM);
-        const float py = grid_coord(affL, lj, NL, gym, gyM);
-        const float pz = grid_coord(affL, lk, NL, gzm, gzM);
-        float best = score((px - x0x) * sg, (py - x0y) * sg, (pz - x0z) * sg);
-        int cand = lane;
-        wave_argmax(best, cand);
+        const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
+        const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
+        const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
+        const float py = (grid_coord(affL, lj, NL, gym, gyM) - f.x0y) * f.sg;
+        const float pz = (grid_coord(affL, lk, NL, gzm, gzM) - f.x0z) * f.sg;
+        float px[NL], t3[NL];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            px[i] = (grid_coord(affL, i, NL, gxm, gxM) - f.x0x) * f.sg;
+            t3[i] = 3.0f * px[i];
+        }
+        const float Qyz = 3.0f * fmaf(pz, pz, py * py);
+        TetAcc acc[NL];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) acc[i] = tet_acc0();
+        for (int e = 0; e < f.n; ++e) {
+            const TetC c = tc[e];
+            const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+            const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+#pragma unroll
+            for (int i = 0; i < NL; ++i)
+                tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
+        }
+        float best = -FLT_MAX;
+        int bi = 0;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            float q = tet_acc_score(acc[i], f.is3);
+            if (q > best) { best = q; bi = i; }
+        }
+        int cand = NL * s + bi;
+        group_argmax<kLG>(best, cand);
         if (qbest < best) {
-            cx = __shfl(px, cand); cy = __shfl(py, cand); cz = __shfl(pz, cand);
+            cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
+            cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
+            cz = grid_coord(affL, cand / (NL * NL), NL, gzm, gzM);
             qbest = best;
             moved = true;
         }
         ds = ds * (2.0f / (float)(NL - 1));
     }
-    return moved;
-}
+    if (moved && s == 0 && ci0 + g < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
 
-template <int NQ>
-__device__ void tet_vertex_learned(const SearchArgs& a, const LearnArgs& la, int vid, int ci, TetC* s_tc,
-                                   RedLds<1>& rd)
-{
-    const int lane = threadIdx.x & 63;
-    const TetFrame f = tet_stage<1>(a, vid, a.extentMin != 0, s_tc, rd);
-    const float q0 = tet_score(f, s_tc, 0.0f, 0.0f, 0.0f);
-    float cx, cy, cz;
-    bool moved = learned_window(la, ci, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, f.sg, a.gridScale,
-                                [&](float px, float py, float pz) { return tet_score(f, s_tc, px, py, pz); }, q0, cx,
-                                cy, cz);
-    if (!moved) {
-        if (lane == 0) atomicAdd(la.fallbacks, 1);
-        cx = f.x0x; cy = f.x0y; cz = f.x0z;
-        moved = tet_levels<NQ, 1>(a, f, s_tc, rd, q0, cx, cy, cz);
+    if (!moved && s == 0 && ci0 + g < nc) {
+        atomicAdd(la.fallbacks, 1);
+        la.list[atomicAdd(la.cnt, 1)] = vid;
     }
-    if (moved && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    (void)rd;
 }
 
 template <int NQ>
-__device__ void hex_vertex_learned(const SearchArgs& a, const LearnArgs& la, int vid, int ci, HexLds* s_hx,
-                                   RedLds<1>& rd)
+__global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnArgs la, int ringCap)
 {
-    const int lane = threadIdx.x & 63;
-    const HexFrame f = hex_stage<1>(a, vid, a.extentMin != 0, s_hx, rd);
-    const float q0 = hex_score(f, s_hx, 0.0f, 0.0f, 0.0f);
-    float cx, cy, cz;
-    bool moved = learned_window(la, ci, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, f.sg, a.gridScale,
-                                [&](float px, float py, float pz) { return hex_score(f, s_hx, px, py, pz); }, q0, cx,
-                                cy, cz);
-    if (!moved) {
-        if (lane == 0) atomicAdd(la.fallbacks, 1);
-        cx = f.x0x; cy = f.x0y; cz = f.x0z;
-        moved = hex_levels<NQ, 1>(a, f, s_hx, rd, q0, cx, cy, cz);
+    extern __shared__ HexLds s_hxl[];   // kLV x ringCap
+    __shared__ RedLds<1> rd;
+    constexpr int NL = 4;
+    const int lane = threadIdx.x & 63, g = lane / kLG, s = lane % kLG;
+    const int nc = a.cEnd - a.cStart, ci0 = kLV * blockIdx.x;
+    const int ci = min(ci0 + g, nc - 1);
+    const int vid = a.cStart + ci;
+    HexLds* hx = s_hxl + (size_t)g * ringCap;
+    HexFrame f;
+    const int r0 = a.ringRow[vid];
+    f.n = a.ringRow[vid + 1] - r0;
+    f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
+    float mabs;
+    group_extents<kLG>(a, r0, f.n, 7, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
+    f.sg = pow2_scale(mabs);
+    const float s2 = f.sg * f.sg, s3 = s2 * f.sg;
+    f.epsD = kHexEps * s3; f.epsF = kHexEps * s2;
+    for (int e = s; e < f.n; e += kLG) {
+        const int* h = a.ringVal + (size_t)(r0 + e) * 7;
+        f3 r[8];
+        r[0] = { 0.0f, 0.0f, 0.0f };
+        for (int t = 0; t < 7; ++t) {
+            int v = h[t];
+            r[t + 1] = { (a.x[v] - f.x0x) * f.sg, (a.y[v] - f.x0y) * f.sg, (a.z[v] - f.x0z) * f.sg };
+        }
+        HexLds L;
+        hex_consts(r, L.cd, L.cF, L.hp);
+        hx[e] = L;
     }
-    if (moved && lane == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    __syncthreads();
+    f.acc0 = hex_acc0();
+    for (int e = 0; e < f.n; ++e)
+#pragma unroll
+        for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(f.acc0, hx[e].cd[sIdx], hx[e].cF[sIdx]);
+    const float q0 = hex_score(f, hx, 0.0f, 0.0f, 0.0f);
+
+    float cx, cy, cz;
+    proposal(la, ci, f.x0x, f.x0y, f.x0z, cx, cy, cz);
+    float qbest = q0;
+    bool moved = false;
+    const float affL = 1.0f / (float)(NL - 1);
+    const int lj = s % NL, lk = s / NL;
+    float ds = la.frac * a.gridScale;
+    for (int d = 0; d < la.depthL; ++d) {
+        const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
+        const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
+        const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
+        const float py = (grid_coord(affL, lj, NL, gym, gyM) - f.x0y) * f.sg;
+        const float pz = (grid_coord(affL, lk, NL, gzm, gzM) - f.x0z) * f.sg;
+        float px[NL], t3[NL];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            px[i] = (grid_coord(affL, i, NL, gxm, gxM) - f.x0x) * f.sg;
+            t3[i] = 3.0f * px[i];
+        }
+        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
+        HexAcc acc[NL];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) acc[i] = f.acc0;
+        for (int e = 0; e < f.n; ++e) {
+#pragma unroll
+            for (int sIdx = 0; sIdx < 5; ++sIdx) {
+                const HexP h = hx[e].hp[sIdx];
+                const bool a3 = (sIdx == 1 || sIdx == 4);
+                const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+                const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+#pragma unroll
+                for (int i = 0; i < NL; ++i)
+                    hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
+            }
+        }
+        float best = -FLT_MAX;
+        int bi = 0;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            float q = hex_acc_score(acc[i], f.epsD, f.epsF);
+            if (q > best) { best = q; bi = i; }
+        }
+        int cand = NL * s + bi;
+        group_argmax<kLG>(best, cand);
+        if (qbest < best) {
+            cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
+            cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
+            cz = grid_coord(affL, cand / (NL * NL), NL, gzm, gzM);
+            qbest = best;
+            moved = true;
+        }
+        ds = ds * (2.0f / (float)(NL - 1));
+    }
+    if (moved && s == 0 && ci0 + g < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+
+    if (!moved && s == 0 && ci0 + g < nc) {
+        atomicAdd(la.fallbacks, 1);
+        la.list[atomicAdd(la.cnt, 1)] = vid;
+    }
+    (void)rd;
 }
 
-template <int NQ>
-__global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnArgs la)
+// plain search of the vertices list[0 .. *cnt) (the fallbacks of a class),
+// one vertex per block of W waves; blocks beyond the count return at once
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_search_list_tet(SearchArgs a, const int* __restrict__ list,
+                                                           const int* __restrict__ cnt)
 {
-    extern __shared__ TetC s_tcl[];
-    __shared__ RedLds<1> rd;
-    tet_vertex_learned<NQ>(a, la, a.cStart + blockIdx.x, blockIdx.x, s_tcl, rd);
+    extern __shared__ TetC s_tcf[];
+    __shared__ RedLds<W> rd;
+    if ((int)blockIdx.x >= *cnt) return;
+    tet_vertex<NQ, W>(a, list[blockIdx.x], a.extentMin != 0, s_tcf, rd);
 }
-template <int NQ>
-__global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnArgs la)
+template <int NQ, int W>
+__global__ __launch_bounds__(64 * W) void k_search_list_hex(SearchArgs a, const int* __restrict__ list,
+                                                           const int* __restrict__ cnt)
 {
-    extern __shared__ HexLds s_hxl[];
-    __shared__ RedLds<1> rd;
-    hex_vertex_learned<NQ>(a, la, a.cStart + blockIdx.x, blockIdx.x, s_hxl, rd);
+    extern __shared__ HexLds s_hxf[];
+    __shared__ RedLds<W> rd;
+    if ((int)blockIdx.x >= *cnt) return;
+    hex_vertex<NQ, W>(a, list[blockIdx.x], a.extentMin != 0, s_hxf, rd);
+}
+template <int NQ, int W>
+static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* cnt, int grid, size_t lds, hipStream_t s)
+{
+    if (kind == VSDMO_TET) k_search_list_tet<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
+    else k_search_list_hex<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
+}
+
+// largest ring the window kernel's LDS (kLV rings per block, 64 KB) holds
+int learned_ring_cap(int kind)
+{
+    const size_t per = (kind == VSDMO_TET) ? sizeof(TetC) : sizeof(HexLds);
+    return (int)((64 * 1024) / (kLV * per));
 }
 
 void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y, const float* L, int mode, int depthL,
-                           float frac, int* fallbacks, size_t lds, hipStream_t s)
+                           float frac, int* fallbacks, int* list, int* cnt, int ringCap, hipStream_t s)
 {
-    const int nb = a.cEnd - a.cStart;
-    if (nb <= 0) return;
-    LearnArgs la{ Y, L, mode, depthL, frac, fallbacks };
-#define VSDMO_LEARN_CASE(Q)                                                                   \
-    case Q:                                                                                  \
-        if (kind == VSDMO_TET) k_search_tet_learned<Q><<<nb, 64, lds, s>>>(a, la);           \
-        else k_search_hex_learned<Q><<<nb, 64, lds, s>>>(a, la);                             \
+    const int nc = a.cEnd - a.cStart;
+    if (nc <= 0) return;
+    ringCap = std::max(ringCap, 1);
+    const int nb = (nc + kLV - 1) / kLV;
+    const size_t per = (kind == VSDMO_TET) ? sizeof(TetC) : sizeof(HexLds);
+    const size_t lds = (size_t)kLV * ringCap * per;
+    LearnArgs la{ Y, L, mode, depthL, frac, fallbacks, list, cnt };
+#define VSDMO_LEARN_CASE(Q)                                                                           \
+    case Q:                                                                                          \
+        if (kind == VSDMO_TET) k_search_tet_learned<Q><<<nb, 64, lds, s>>>(a, la, ringCap);          \
+        else k_search_hex_learned<Q><<<nb, 64, lds, s>>>(a, la, ringCap);                            \
         break;
     switch (nq) {
         VSDMO_LEARN_CASE(2)
@@ -647,6 +858,15 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y
     default: break;
     }
 #undef VSDMO_LEARN_CASE
+    // second launch: the plain search of the fallbacks, W waves per vertex
+    const size_t lds1 = (size_t)ringCap * per;
+    switch (nq) {
+    case 2: list_nqw<2, 2>(kind, a, list, cnt, nc, lds1, s); break;
+    case 4: list_nqw<4, 4>(kind, a, list, cnt, nc, lds1, s); break;
+    case 6: list_nqw<6, 2>(kind, a, list, cnt, nc, lds1, s); break;
+    case 8: list_nqw<8, 4>(kind, a, list, cnt, nc, lds1, s); break;
+    default: break;
+    }
 }
 
 // ------------------------------------------------------- split launches
