@@ -34,6 +34,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "vertex updates/s at equal final min mean-ratio quality; learned vs plain DMO"
 CPU_BUDGET_S = 10.0
+# learned proposal window per element type: (levels, fraction of the plain
+# first-level window), chosen by sweeps on the seed-7 40^3 tet / 48^3 hex
+# cubes (the benchmark meshes use seed 0)
+LEARN_WINDOW = {"tet": (2, 0.045), "hex": (4, 0.5)}
 PEAK_FP32_TFLOPS = 157.3
 # Algorithmic work of one vertex update: (nq^d + 1) ring evaluations per level
 # x levels + the final check = 1,540 (3D) / 196 (2D), times the one-ring size
@@ -382,8 +386,10 @@ def main(argv=None):
                     help="attach the learned-vs-plain comparison to the plain line (default: on for tet40)")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs (learned mode)")
     ap.add_argument("--epochs", type=int, default=20)
-    ap.add_argument("--learn-depth", type=int, default=4)
-    ap.add_argument("--learn-frac", type=float, default=0.25)
+    # proposal window (levels of 4^3, fraction of the plain first-level window);
+    # default per element type, chosen on the seed-7 meshes, not the benchmark's
+    ap.add_argument("--learn-depth", type=int, default=None, help="default: 2 (tet), 4 (hex)")
+    ap.add_argument("--learn-frac", type=float, default=None, help="default: 0.045 (tet), 0.5 (hex)")
     ap.add_argument("--tune-log", default=os.environ.get("VSDMO_TUNE_LOG", "autotune_batch256.jsonl"))
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -393,6 +399,11 @@ def main(argv=None):
 
     if args.compare_learned is None:
         args.compare_learned = int(args.workload == "tet40" and args.mode == "plain")
+    hexw = WORKLOADS[args.workload]["kind"] == "hex"
+    if args.learn_depth is None:
+        args.learn_depth = LEARN_WINDOW["hex" if hexw else "tet"][0]
+    if args.learn_frac is None:
+        args.learn_frac = LEARN_WINDOW["hex" if hexw else "tet"][1]
     if args.workload == "batch256":
         return bench_batch(args)
     if args.mode == "learned":

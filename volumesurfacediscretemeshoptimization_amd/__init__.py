@@ -325,9 +325,18 @@ class _DMOBase:
               "vsdmo_opt_surface_points")
         return out
 
-    def set_proposal(self, mlp=None, mode: int = 1, depth: int = 4, window_frac: float = 0.25):
+    # default proposal windows (levels, fraction of the plain first-level
+    # window), chosen on a 40^3 tet cube and a 48^3 hex cube
+    LEARN_WINDOW = {_lib.VSDMO_TET: (2, 0.045), _lib.VSDMO_HEX: (4, 0.5)}
+
+    def set_proposal(self, mlp=None, mode: int = 1, depth: int = None, window_frac: float = None):
         """Learned proposal for the inner searches: mode 1 = network, 2 = the
-        vertex itself (same windowed search and fallback, no network), 0 = off."""
+        vertex itself (same windowed search and fallback, no network), 0 = off.
+        The window is `depth` levels of 4^3 candidates on `window_frac` times
+        the plain first-level window (defaults per element type above)."""
+        d0, f0 = self.LEARN_WINDOW.get(self.mesh.elem, (2, 0.045))
+        depth = d0 if depth is None else depth
+        window_frac = f0 if window_frac is None else window_frac
         h = mlp.handle if mlp is not None else None
         check(self._lib.vsdmo_opt_set_proposal(self._h, h, int(mode), int(depth), float(window_frac)),
               "vsdmo_opt_set_proposal")

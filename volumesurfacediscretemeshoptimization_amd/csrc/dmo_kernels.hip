@@ -802,15 +802,20 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
 }
 
 // plain search of the vertices list[0 .. *cnt) (the fallbacks of a class),
-// one vertex per block of W waves; blocks beyond the count return at once
+// one vertex per block of W waves at a time, blocks striding over the list
+// (the grid is sized for a typical fallback share, the length is only known
+// on the device)
 template <int NQ, int W>
 __global__ __launch_bounds__(64 * W) void k_search_list_tet(SearchArgs a, const int* __restrict__ list,
                                                            const int* __restrict__ cnt)
 {
     extern __shared__ TetC s_tcf[];
     __shared__ RedLds<W> rd;
-    if ((int)blockIdx.x >= *cnt) return;
-    tet_vertex<NQ, W>(a, list[blockIdx.x], a.extentMin != 0, s_tcf, rd);
+    const int n = *cnt;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        tet_vertex<NQ, W>(a, list[i], a.extentMin != 0, s_tcf, rd);
+        __syncthreads();   // the next vertex restages the LDS ring
+    }
 }
 template <int NQ, int W>
 __global__ __launch_bounds__(64 * W) void k_search_list_hex(SearchArgs a, const int* __restrict__ list,
@@ -818,8 +823,11 @@ __global__ __launch_bounds__(64 * W) void k_search_list_hex(SearchArgs a, const 
 {
     extern __shared__ HexLds s_hxf[];
     __shared__ RedLds<W> rd;
-    if ((int)blockIdx.x >= *cnt) return;
-    hex_vertex<NQ, W>(a, list[blockIdx.x], a.extentMin != 0, s_hxf, rd);
+    const int n = *cnt;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        hex_vertex<NQ, W>(a, list[i], a.extentMin != 0, s_hxf, rd);
+        __syncthreads();   // the next vertex restages the LDS ring
+    }
 }
 template <int NQ, int W>
 static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* cnt, int grid, size_t lds, hipStream_t s)
@@ -858,13 +866,15 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y
     default: break;
     }
 #undef VSDMO_LEARN_CASE
-    // second launch: the plain search of the fallbacks, W waves per vertex
+    // second launch: the plain search of the fallbacks, W waves per vertex,
+    // a quarter of the class's blocks (blocks loop when more fall back)
     const size_t lds1 = (size_t)ringCap * per;
+    const int g1 = std::max(1, std::min(nc, std::max(256, nc / 4)));
     switch (nq) {
-    case 2: list_nqw<2, 2>(kind, a, list, cnt, nc, lds1, s); break;
-    case 4: list_nqw<4, 4>(kind, a, list, cnt, nc, lds1, s); break;
-    case 6: list_nqw<6, 2>(kind, a, list, cnt, nc, lds1, s); break;
-    case 8: list_nqw<8, 4>(kind, a, list, cnt, nc, lds1, s); break;
+    case 2: list_nqw<2, 2>(kind, a, list, cnt, g1, lds1, s); break;
+    case 4: list_nqw<4, 4>(kind, a, list, cnt, g1, lds1, s); break;
+    case 6: list_nqw<6, 2>(kind, a, list, cnt, g1, lds1, s); break;
+    case 8: list_nqw<8, 4>(kind, a, list, cnt, g1, lds1, s); break;
     default: break;
     }
 }
