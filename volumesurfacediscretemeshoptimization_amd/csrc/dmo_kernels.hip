@@ -874,7 +874,12 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y
     case 2: list_nqw<2, 2>(kind, a, list, cnt, g1, lds1, s); break;
     case 4: list_nqw<4, 4>(kind, a, list, cnt, g1, lds1, s); break;
     case 6: list_nqw<6, 2>(kind, a, list, cnt, g1, lds1, s); break;
-    case 8: list_nqw<8, 4>(kind, a, list, cnt, g1, lds1, s); break;
+    case 8:
+        // a short tet list is latency-bound (4 waves per vertex); hexes fall
+        // back often and are throughput-bound (one wave per vertex)
+        if (kind == VSDMO_TET) list_nqw<8, 4>(kind, a, list, cnt, g1, lds1, s);
+        else list_nqw<8, 1>(kind, a, list, cnt, std::max(1, std::min(nc, 4096)), lds1, s);
+        break;
     default: break;
     }
 }
