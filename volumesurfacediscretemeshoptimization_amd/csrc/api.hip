@@ -783,8 +783,7 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
         if (!o->lCnt.p) o->lCnt.alloc(std::max<size_t>(m->colOffInner.size(), 1));
         if (mode == 1) {
             build_neighbour_table(*m);
-            VSDMO_HIP(hipStreamSynchronize(mlp->stream));
-            mlp_reserve(mlp, maxc);
+            VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
         }
         VSDMO_HIP(hipStreamSynchronize(m->stream));
         return VSDMO_OK;

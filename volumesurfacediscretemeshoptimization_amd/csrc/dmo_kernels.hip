@@ -541,8 +541,9 @@ struct LearnArgs {
 // (y, z) part of every ring element, so one LDS read of an element's
 // constants serves 4 candidates (one candidate per lane would spend a third
 // of the wave's cycles waiting on LDS).
-constexpr int kLV = 4;            // vertices per wave in the proposal window
-constexpr int kLG = 64 / kLV;     // lanes per vertex
+// vertices per wave in the proposal window (1, 2 and 4 measured on 40^3 tet
+// and 48^3 hex cubes: 4 is fastest for hexes and within noise of 2 for tets)
+constexpr int kLV = 4;
 
 template <int G>
 __device__ __forceinline__ float group_max(float v)
@@ -608,14 +609,14 @@ __device__ __forceinline__ void proposal(const LearnArgs& la, int ci, float x0x,
     }
 }
 
-template <int NQ>
+template <int NQ, int LV>
 __global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnArgs la, int ringCap)
 {
-    extern __shared__ TetC s_tcl[];   // kLV x ringCap
+    extern __shared__ TetC s_tcl[];   // LV x ringCap
     __shared__ RedLds<1> rd;
-    constexpr int NL = 4;
-    const int lane = threadIdx.x & 63, g = lane / kLG, s = lane % kLG;
-    const int nc = a.cEnd - a.cStart, ci0 = kLV * blockIdx.x;
+    constexpr int NL = 4, G = 64 / LV, CPL = NL * NL * NL / G;   // lanes per vertex, candidates per lane
+    const int lane = threadIdx.x & 63, g = lane / G, s = lane % G;
+    const int nc = a.cEnd - a.cStart, ci0 = LV * blockIdx.x;
     // a group past the end of the class mirrors the class's last vertex and writes nothing
     const int ci = min(ci0 + g, nc - 1);
     const int vid = a.cStart + ci;
@@ -625,11 +626,11 @@ __global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnAr
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
     float mabs;
-    group_extents<kLG>(a, r0, f.n, 3, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
+    group_extents<G>(a, r0, f.n, 3, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
     f.sg = pow2_scale(mabs);
     const float isg = 1.0f / f.sg;
     f.is3 = ((1.0f * isg) * isg) * isg;
-    for (int e = s; e < f.n; e += kLG) {
+    for (int e = s; e < f.n; e += G) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 3;
         f3 p[3];
 #pragma unroll
@@ -647,7 +648,7 @@ __global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnAr
     float qbest = q0;
     bool moved = false;
     const float affL = 1.0f / (float)(NL - 1);
-    const int lj = s % NL, lk = s / NL;
+    const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
     float ds = la.frac * a.gridScale;
     for (int d = 0; d < la.depthL; ++d) {
         const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
@@ -655,33 +656,33 @@ __global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnAr
         const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
         const float py = (grid_coord(affL, lj, NL, gym, gyM) - f.x0y) * f.sg;
         const float pz = (grid_coord(affL, lk, NL, gzm, gzM) - f.x0z) * f.sg;
-        float px[NL], t3[NL];
+        float px[CPL], t3[CPL];
 #pragma unroll
-        for (int i = 0; i < NL; ++i) {
-            px[i] = (grid_coord(affL, i, NL, gxm, gxM) - f.x0x) * f.sg;
+        for (int i = 0; i < CPL; ++i) {
+            px[i] = (grid_coord(affL, i0 + i, NL, gxm, gxM) - f.x0x) * f.sg;
             t3[i] = 3.0f * px[i];
         }
         const float Qyz = 3.0f * fmaf(pz, pz, py * py);
-        TetAcc acc[NL];
+        TetAcc acc[CPL];
 #pragma unroll
-        for (int i = 0; i < NL; ++i) acc[i] = tet_acc0();
+        for (int i = 0; i < CPL; ++i) acc[i] = tet_acc0();
         for (int e = 0; e < f.n; ++e) {
             const TetC c = tc[e];
             const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
             const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
 #pragma unroll
-            for (int i = 0; i < NL; ++i)
+            for (int i = 0; i < CPL; ++i)
                 tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
         }
         float best = -FLT_MAX;
         int bi = 0;
 #pragma unroll
-        for (int i = 0; i < NL; ++i) {
+        for (int i = 0; i < CPL; ++i) {
             float q = tet_acc_score(acc[i], f.is3);
             if (q > best) { best = q; bi = i; }
         }
-        int cand = NL * s + bi;
-        group_argmax<kLG>(best, cand);
+        int cand = i0 + bi + NL * (s % (NL * NL));
+        group_argmax<G>(best, cand);
         if (qbest < best) {
             cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
             cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
@@ -700,14 +701,14 @@ __global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnAr
     (void)rd;
 }
 
-template <int NQ>
+template <int NQ, int LV>
 __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnArgs la, int ringCap)
 {
-    extern __shared__ HexLds s_hxl[];   // kLV x ringCap
+    extern __shared__ HexLds s_hxl[];   // LV x ringCap
     __shared__ RedLds<1> rd;
-    constexpr int NL = 4;
-    const int lane = threadIdx.x & 63, g = lane / kLG, s = lane % kLG;
-    const int nc = a.cEnd - a.cStart, ci0 = kLV * blockIdx.x;
+    constexpr int NL = 4, G = 64 / LV, CPL = NL * NL * NL / G;
+    const int lane = threadIdx.x & 63, g = lane / G, s = lane % G;
+    const int nc = a.cEnd - a.cStart, ci0 = LV * blockIdx.x;
     const int ci = min(ci0 + g, nc - 1);
     const int vid = a.cStart + ci;
     HexLds* hx = s_hxl + (size_t)g * ringCap;
@@ -716,11 +717,11 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
     float mabs;
-    group_extents<kLG>(a, r0, f.n, 7, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
+    group_extents<G>(a, r0, f.n, 7, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
     f.sg = pow2_scale(mabs);
     const float s2 = f.sg * f.sg, s3 = s2 * f.sg;
     f.epsD = kHexEps * s3; f.epsF = kHexEps * s2;
-    for (int e = s; e < f.n; e += kLG) {
+    for (int e = s; e < f.n; e += G) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 7;
         f3 r[8];
         r[0] = { 0.0f, 0.0f, 0.0f };
@@ -744,7 +745,7 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
     float qbest = q0;
     bool moved = false;
     const float affL = 1.0f / (float)(NL - 1);
-    const int lj = s % NL, lk = s / NL;
+    const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
     float ds = la.frac * a.gridScale;
     for (int d = 0; d < la.depthL; ++d) {
         const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
@@ -752,16 +753,16 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
         const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
         const float py = (grid_coord(affL, lj, NL, gym, gyM) - f.x0y) * f.sg;
         const float pz = (grid_coord(affL, lk, NL, gzm, gzM) - f.x0z) * f.sg;
-        float px[NL], t3[NL];
+        float px[CPL], t3[CPL];
 #pragma unroll
-        for (int i = 0; i < NL; ++i) {
-            px[i] = (grid_coord(affL, i, NL, gxm, gxM) - f.x0x) * f.sg;
+        for (int i = 0; i < CPL; ++i) {
+            px[i] = (grid_coord(affL, i0 + i, NL, gxm, gxM) - f.x0x) * f.sg;
             t3[i] = 3.0f * px[i];
         }
         const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
-        HexAcc acc[NL];
+        HexAcc acc[CPL];
 #pragma unroll
-        for (int i = 0; i < NL; ++i) acc[i] = f.acc0;
+        for (int i = 0; i < CPL; ++i) acc[i] = f.acc0;
         for (int e = 0; e < f.n; ++e) {
 #pragma unroll
             for (int sIdx = 0; sIdx < 5; ++sIdx) {
@@ -770,19 +771,19 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
                 const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
                 const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
 #pragma unroll
-                for (int i = 0; i < NL; ++i)
+                for (int i = 0; i < CPL; ++i)
                     hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
             }
         }
         float best = -FLT_MAX;
         int bi = 0;
 #pragma unroll
-        for (int i = 0; i < NL; ++i) {
+        for (int i = 0; i < CPL; ++i) {
             float q = hex_acc_score(acc[i], f.epsD, f.epsF);
             if (q > best) { best = q; bi = i; }
         }
-        int cand = NL * s + bi;
-        group_argmax<kLG>(best, cand);
+        int cand = i0 + bi + NL * (s % (NL * NL));
+        group_argmax<G>(best, cand);
         if (qbest < best) {
             cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
             cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
@@ -836,6 +837,13 @@ static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* 
     else k_search_list_hex<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
 }
 
+template <int NQ, int LV>
+static void learned_nq_lv(int kind, const SearchArgs& a, const LearnArgs& la, int ringCap, int nb, size_t lds, hipStream_t s)
+{
+    if (kind == VSDMO_TET) k_search_tet_learned<NQ, LV><<<nb, 64, lds, s>>>(a, la, ringCap);
+    else k_search_hex_learned<NQ, LV><<<nb, 64, lds, s>>>(a, la, ringCap);
+}
+
 // largest ring the window kernel's LDS (kLV rings per block, 64 KB) holds
 int learned_ring_cap(int kind)
 {
@@ -855,8 +863,7 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y
     LearnArgs la{ Y, L, mode, depthL, frac, fallbacks, list, cnt };
 #define VSDMO_LEARN_CASE(Q)                                                                           \
     case Q:                                                                                          \
-        if (kind == VSDMO_TET) k_search_tet_learned<Q><<<nb, 64, lds, s>>>(a, la, ringCap);          \
-        else k_search_hex_learned<Q><<<nb, 64, lds, s>>>(a, la, ringCap);                            \
+        learned_nq_lv<Q, kLV>(kind, a, la, ringCap, nb, lds, s);                                       \
         break;
     switch (nq) {
         VSDMO_LEARN_CASE(2)
