@@ -259,7 +259,10 @@ vsdmo_status vsdmo_dataset_destroy(vsdmo_dataset* d);
 vsdmo_status vsdmo_opt_record(vsdmo_opt* o, vsdmo_dataset* ds, int32_t n);
 /* mode 0: plain search; 1: network proposal; 2: the proposal is the vertex
  * itself (the same windowed search and fallback without the network).
- * One-rings up to 512 tets / 85 hexes (VSDMO_NOT_SUPPORTED above). */
+ * One-rings up to 512 tets / 85 hexes (VSDMO_NOT_SUPPORTED above).  With
+ * one-rings up to 128 tets / 21 hexes the window kernel runs the network
+ * itself; environment variable VSDMO_LEARN_SEPARATE_INFERENCE=1 (read here)
+ * keeps it in a launch of its own (same results, for diagnosis). */
 vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac);
 
 /* ------------------------------------------------------------------------

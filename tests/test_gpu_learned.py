@@ -194,3 +194,35 @@ def test_learned_reaches_plain_quality_and_ablation(kind):
     print(kind, "plain", q_plain, "learned", res[1], "identity", res[2])
     assert res[1][1] >= q_plain - 1e-3, res
     assert res[1][2] < res[2][2], res
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_fused_network_equals_separate_launch(kind, monkeypatch):
+    """The window kernel's own run of the network (16-vertex tiles, partial
+    last tile per class) gives the positions, fallback count and quality of
+    the separate inference launch bit for bit."""
+    mlp = DMOMLP(seed=21)
+    p = mlp.params()
+    p[-3:] = [0.05, -0.03, 0.02]         # b3: a nonzero mean displacement
+    mlp.set_params(p)
+    if kind == "tet":
+        xyz, cells = synth.tet_cube(13, seed=31)
+        fac, cls = DMOMeshTetFactory, DMOTetClass
+    else:
+        xyz, cells = synth.hex_cube(11, seed=31, jitter=0.2)
+        fac, cls = DMOMeshHexFactory, DMOHexClass
+    out = []
+    for sep in ("1", "0"):
+        monkeypatch.setenv("VSDMO_LEARN_SEPARATE_INFERENCE", sep)
+        m = fac.create(xyz, cells)
+        o = cls(m, n_iter=3, fixed_boundary=True)
+        o.set_proposal(mlp, mode=1)
+        o.iterate(3)
+        st = o.stats()
+        out.append((m.positions(original_order=False), st["fallbacks"], o.getQuality()))
+        o.close()
+        m.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and 0 < out[0][1]
+    assert out[0][2] == out[1][2]
+    mlp.close()

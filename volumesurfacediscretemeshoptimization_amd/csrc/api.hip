@@ -5,6 +5,7 @@
 #include <cstring>
 #include <functional>
 #include <new>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -107,6 +108,7 @@ struct vsdmo_opt {
     vsdmo_mlp* mlp = nullptr;
     int learn_mode = 0, learn_depth = 4;
     float learn_frac = 0.25f;
+    bool learn_separate = false;     // network in its own launch even where the window kernel could run it
     DevBuf<float> lL, lY;
     DevBuf<int> lFallback;           // running count of fallbacks to the full search
     DevBuf<int> lList, lCnt;         // fallback list of a class; per-class list lengths
@@ -568,11 +570,18 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                         launch_features(m->x.p, m->y.p, m->z.p, m->nbr.p, a.cStart, a.cStart + take,
                                         ds->X.p + ds->n * kIn, ds->L.p, ds->v0.p, m->stream);
                     if (o->learn_mode != 0 && !ds) {
-                        if (o->learn_mode == 1)
-                            launch_mlp_infer(o->mlp, m->x.p, m->y.p, m->z.p, m->nbr.p, a.cStart, a.cEnd, nullptr,
-                                             o->lY.p, o->lL.p, m->stream);
+                        LearnNet net;
+                        net.Y = o->lY.p; net.L = o->lL.p;
+                        if (o->learn_mode == 1) {
+                            if (!o->learn_separate && learned_fused_ok(m->elem, m->maxRing)) {
+                                net.wb = o->mlp->wb.p; net.w = o->mlp->w.p; net.nbr = m->nbr.p;
+                            } else {
+                                launch_mlp_infer(o->mlp, m->x.p, m->y.p, m->z.p, m->nbr.p, a.cStart, a.cEnd, nullptr,
+                                                 o->lY.p, o->lL.p, m->stream);
+                            }
+                        }
                         timed(o, [&]() {
-                            launch_search_learned(m->elem, o->p.nq, a, o->lY.p, o->lL.p, o->learn_mode, o->learn_depth,
+                            launch_search_learned(m->elem, o->p.nq, a, net, o->learn_mode, o->learn_depth,
                                                   o->learn_frac, o->lFallback.p, o->lList.p, o->lCnt.p + c, m->maxRing,
                                                   m->stream);
                         });
@@ -769,6 +778,8 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
         o->mlp = mode == 1 ? mlp : nullptr;
         o->learn_depth = depth;
         o->learn_frac = window_frac;
+        const char* sep = std::getenv("VSDMO_LEARN_SEPARATE_INFERENCE");
+        o->learn_separate = sep && sep[0] && sep[0] != '0';
         if (mode == 0) return VSDMO_OK;
         int maxc = 1;
         for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c)

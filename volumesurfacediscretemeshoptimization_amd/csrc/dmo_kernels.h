@@ -126,16 +126,29 @@ void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, c
                         int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s);
 int batch_round_blocks_per_cu(int nq, int w, size_t lds);
 void launch_fill_int(int* p, int n, int v, hipStream_t s);
+// where the learned search finds its proposals (mode 1): the network's
+// packed parameters (wb bf16, w fp32) and the neighbour table, with which the
+// window kernel runs the network itself when learned_fused_ok(); otherwise
+// Y, L of a separate inference launch, indexed by the vertex's position in
+// the class
+struct LearnNet {
+    const float* Y = nullptr;
+    const float* L = nullptr;
+    const uint16_t* wb = nullptr;
+    const float* w = nullptr;
+    const int* nbr = nullptr;
+};
 // learned-proposal search of one colour class [a.cStart, a.cEnd) (tet / hex):
 // the proposal window (4 vertices per wave) appends the vertices whose
 // window finds nothing better than their start to list, then a second
-// launch runs their plain search; Y, L indexed by the vertex's position in
-// the class; *cnt must be zero on entry; *fallbacks counts the fallbacks;
-// ringCap >= the largest one-ring
-void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y, const float* L, int mode, int depthL,
+// launch runs their plain search; *cnt must be zero on entry; *fallbacks
+// counts the fallbacks; ringCap >= the largest one-ring
+void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet& net, int mode, int depthL,
                            float frac, int* fallbacks, int* list, int* cnt, int ringCap, hipStream_t s);
 // largest one-ring (elements) the learned search supports
 int learned_ring_cap(int kind);
+// whether the window kernel can run the network itself for this ring size
+bool learned_fused_ok(int kind, int ringCap);
 void launch_batch_quality(const BatchQArgs& a, hipStream_t s);
 size_t batch_lds_bytes(int maxRingTet, int maxRingHex, int maxHETri);
 void launch_quality(const QualityArgs& a, float* pmin, double* psum, float* omin, double* osum, hipStream_t s);

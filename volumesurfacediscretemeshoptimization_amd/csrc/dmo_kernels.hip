@@ -16,6 +16,7 @@
 #include "vsdmo_internal.h"
 #include "vsdmo_numerics.h"
 #include "dmo_kernels.h"
+#include "mlp_internal.h"
 
 namespace vsdmo {
 
@@ -534,6 +535,11 @@ struct LearnArgs {
     int* fallbacks;   // running total
     int* list;        // this class's fallback vertices (appended)
     int* cnt;         // their number (zero before the class)
+    // network run inside the window kernel (mode 1, one 16-vertex tile per
+    // block): bf16 / fp32 packed parameters and the neighbour table
+    const uint16_t* wb;
+    const float* w;
+    const int* nbr;
 };
 
 // The proposal window runs 4 vertices per wave, 16 lanes each: lane s =
@@ -596,31 +602,46 @@ __device__ void group_extents(const SearchArgs& a, int r0, int n, int st, int s,
     mabs = group_max<G>(mabs);
 }
 
-// proposal centre of the vertex at class position ci (mode 1), or the vertex
-__device__ __forceinline__ void proposal(const LearnArgs& la, int ci, float x0x, float x0y, float x0z, float& cx,
-                                         float& cy, float& cz)
+// proposal displacement of the group's vertex (class position ci): c = v +
+// l * y.  FUSED: the block's 16 vertices (class positions ci0 + 0..15, group
+// gb = threadIdx.x / 16) are one tile of the network, run here in the LDS
+// lds, which is free again on return; otherwise l, y come from the separate
+// inference launch's L, Y (mode 1) or are zero (mode 2).
+template <bool FUSED>
+__device__ __forceinline__ void learned_offset(const SearchArgs& a, const LearnArgs& la, int ci0, int gb, int ci,
+                                               unsigned char* lds, float& l, float& y0, float& y1, float& y2)
 {
-    cx = x0x; cy = x0y; cz = x0z;
-    if (la.mode == 1) {
-        const float l = la.L[ci];
-        cx = x0x + l * la.Y[3 * ci];
-        cy = x0y + l * la.Y[3 * ci + 1];
-        cz = x0z + l * la.Y[3 * ci + 2];
+    l = 0.0f; y0 = 0.0f; y1 = 0.0f; y2 = 0.0f;
+    if constexpr (FUSED) {
+        MlpLds& sm = *reinterpret_cast<MlpLds*>(lds);
+        const bool live = ci0 + gb < a.cEnd - a.cStart;
+        mlp_features16(a.x, a.y, a.z, la.nbr, a.cStart + ci0 + gb, live, sm);
+        mlp_layers16(la.wb, la.w, sm);
+        l = sm.L[gb]; y0 = sm.Y[gb][0]; y1 = sm.Y[gb][1]; y2 = sm.Y[gb][2];
+        __syncthreads();   // the ring staging reuses the LDS
+    } else if (la.mode == 1) {
+        l = la.L[ci]; y0 = la.Y[3 * ci]; y1 = la.Y[3 * ci + 1]; y2 = la.Y[3 * ci + 2];
     }
+    (void)ci;
 }
 
-template <int NQ, int LV>
-__global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnArgs la, int ringCap)
+// one block: LV vertices on one wave, or (FUSED) 4 waves of LV = 4 vertices
+// that first run their 16-vertex network tile
+template <int NQ, int LV, bool FUSED>
+__global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_tet_learned(SearchArgs a, LearnArgs la, int ringCap)
 {
-    extern __shared__ TetC s_tcl[];   // LV x ringCap
-    __shared__ RedLds<1> rd;
+    extern __shared__ __align__(16) unsigned char s_tcl[];   // VPB x ringCap TetC (FUSED: first the network)
     constexpr int NL = 4, G = 64 / LV, CPL = NL * NL * NL / G;   // lanes per vertex, candidates per lane
-    const int lane = threadIdx.x & 63, g = lane / G, s = lane % G;
-    const int nc = a.cEnd - a.cStart, ci0 = LV * blockIdx.x;
+    constexpr int VPB = FUSED ? 4 * LV : LV;
+    static_assert(!FUSED || VPB == kMlpRows, "one network tile per block");
+    const int s = threadIdx.x % G, gb = threadIdx.x / G;
+    const int nc = a.cEnd - a.cStart, ci0 = VPB * blockIdx.x;
     // a group past the end of the class mirrors the class's last vertex and writes nothing
-    const int ci = min(ci0 + g, nc - 1);
+    const int ci = min(ci0 + gb, nc - 1);
     const int vid = a.cStart + ci;
-    TetC* tc = s_tcl + (size_t)g * ringCap;
+    float pl, py0, py1, py2;
+    learned_offset<FUSED>(a, la, ci0, gb, ci, s_tcl, pl, py0, py1, py2);
+    TetC* tc = reinterpret_cast<TetC*>(s_tcl) + (size_t)gb * ringCap;
     TetFrame f;
     const int r0 = a.ringRow[vid];
     f.n = a.ringRow[vid + 1] - r0;
@@ -643,8 +664,8 @@ __global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnAr
     __syncthreads();
     const float q0 = tet_score(f, tc, 0.0f, 0.0f, 0.0f);
 
-    float cx, cy, cz;
-    proposal(la, ci, f.x0x, f.x0y, f.x0z, cx, cy, cz);
+    float cx = f.x0x, cy = f.x0y, cz = f.x0z;
+    if (la.mode == 1) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
     float qbest = q0;
     bool moved = false;
     const float affL = 1.0f / (float)(NL - 1);
@@ -692,26 +713,28 @@ __global__ __launch_bounds__(64) void k_search_tet_learned(SearchArgs a, LearnAr
         }
         ds = ds * (2.0f / (float)(NL - 1));
     }
-    if (moved && s == 0 && ci0 + g < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    if (moved && s == 0 && ci0 + gb < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
 
-    if (!moved && s == 0 && ci0 + g < nc) {
+    if (!moved && s == 0 && ci0 + gb < nc) {
         atomicAdd(la.fallbacks, 1);
         la.list[atomicAdd(la.cnt, 1)] = vid;
     }
-    (void)rd;
 }
 
-template <int NQ, int LV>
-__global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnArgs la, int ringCap)
+template <int NQ, int LV, bool FUSED>
+__global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchArgs a, LearnArgs la, int ringCap)
 {
-    extern __shared__ HexLds s_hxl[];   // LV x ringCap
-    __shared__ RedLds<1> rd;
+    extern __shared__ __align__(16) unsigned char s_hxl[];   // VPB x ringCap HexLds (FUSED: first the network)
     constexpr int NL = 4, G = 64 / LV, CPL = NL * NL * NL / G;
-    const int lane = threadIdx.x & 63, g = lane / G, s = lane % G;
-    const int nc = a.cEnd - a.cStart, ci0 = LV * blockIdx.x;
-    const int ci = min(ci0 + g, nc - 1);
+    constexpr int VPB = FUSED ? 4 * LV : LV;
+    static_assert(!FUSED || VPB == kMlpRows, "one network tile per block");
+    const int s = threadIdx.x % G, gb = threadIdx.x / G;
+    const int nc = a.cEnd - a.cStart, ci0 = VPB * blockIdx.x;
+    const int ci = min(ci0 + gb, nc - 1);
     const int vid = a.cStart + ci;
-    HexLds* hx = s_hxl + (size_t)g * ringCap;
+    float pl, py0, py1, py2;
+    learned_offset<FUSED>(a, la, ci0, gb, ci, s_hxl, pl, py0, py1, py2);
+    HexLds* hx = reinterpret_cast<HexLds*>(s_hxl) + (size_t)gb * ringCap;
     HexFrame f;
     const int r0 = a.ringRow[vid];
     f.n = a.ringRow[vid + 1] - r0;
@@ -740,8 +763,8 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
         for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(f.acc0, hx[e].cd[sIdx], hx[e].cF[sIdx]);
     const float q0 = hex_score(f, hx, 0.0f, 0.0f, 0.0f);
 
-    float cx, cy, cz;
-    proposal(la, ci, f.x0x, f.x0y, f.x0z, cx, cy, cz);
+    float cx = f.x0x, cy = f.x0y, cz = f.x0z;
+    if (la.mode == 1) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
     float qbest = q0;
     bool moved = false;
     const float affL = 1.0f / (float)(NL - 1);
@@ -793,13 +816,12 @@ __global__ __launch_bounds__(64) void k_search_hex_learned(SearchArgs a, LearnAr
         }
         ds = ds * (2.0f / (float)(NL - 1));
     }
-    if (moved && s == 0 && ci0 + g < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    if (moved && s == 0 && ci0 + gb < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
 
-    if (!moved && s == 0 && ci0 + g < nc) {
+    if (!moved && s == 0 && ci0 + gb < nc) {
         atomicAdd(la.fallbacks, 1);
         la.list[atomicAdd(la.cnt, 1)] = vid;
     }
-    (void)rd;
 }
 
 // plain search of the vertices list[0 .. *cnt) (the fallbacks of a class),
@@ -837,33 +859,41 @@ static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* 
     else k_search_list_hex<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
 }
 
-template <int NQ, int LV>
+template <int NQ, int LV, bool FUSED>
 static void learned_nq_lv(int kind, const SearchArgs& a, const LearnArgs& la, int ringCap, int nb, size_t lds, hipStream_t s)
 {
-    if (kind == VSDMO_TET) k_search_tet_learned<NQ, LV><<<nb, 64, lds, s>>>(a, la, ringCap);
-    else k_search_hex_learned<NQ, LV><<<nb, 64, lds, s>>>(a, la, ringCap);
+    constexpr int T = FUSED ? 256 : 64;
+    if (kind == VSDMO_TET) k_search_tet_learned<NQ, LV, FUSED><<<nb, T, lds, s>>>(a, la, ringCap);
+    else k_search_hex_learned<NQ, LV, FUSED><<<nb, T, lds, s>>>(a, la, ringCap);
 }
+
+static size_t learned_per(int kind) { return (kind == VSDMO_TET) ? sizeof(TetC) : sizeof(HexLds); }
 
 // largest ring the window kernel's LDS (kLV rings per block, 64 KB) holds
-int learned_ring_cap(int kind)
+int learned_ring_cap(int kind) { return (int)((64 * 1024) / (kLV * learned_per(kind))); }
+
+// the fused kernel holds 16 rings per block in the same 64 KB
+bool learned_fused_ok(int kind, int ringCap)
 {
-    const size_t per = (kind == VSDMO_TET) ? sizeof(TetC) : sizeof(HexLds);
-    return (int)((64 * 1024) / (kLV * per));
+    return (size_t)kMlpRows * std::max(ringCap, 1) * learned_per(kind) <= 64 * 1024;
 }
 
-void launch_search_learned(int kind, int nq, const SearchArgs& a, const float* Y, const float* L, int mode, int depthL,
+void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet& net, int mode, int depthL,
                            float frac, int* fallbacks, int* list, int* cnt, int ringCap, hipStream_t s)
 {
     const int nc = a.cEnd - a.cStart;
     if (nc <= 0) return;
     ringCap = std::max(ringCap, 1);
-    const int nb = (nc + kLV - 1) / kLV;
-    const size_t per = (kind == VSDMO_TET) ? sizeof(TetC) : sizeof(HexLds);
-    const size_t lds = (size_t)kLV * ringCap * per;
-    LearnArgs la{ Y, L, mode, depthL, frac, fallbacks, list, cnt };
+    const size_t per = learned_per(kind);
+    const bool fused = mode == 1 && net.wb && learned_fused_ok(kind, ringCap);
+    const int vpb = fused ? kMlpRows : kLV;
+    const int nb = (nc + vpb - 1) / vpb;
+    const size_t lds = fused ? std::max(sizeof(MlpLds), (size_t)kMlpRows * ringCap * per) : (size_t)kLV * ringCap * per;
+    LearnArgs la{ net.Y, net.L, mode, depthL, frac, fallbacks, list, cnt, net.wb, net.w, net.nbr };
 #define VSDMO_LEARN_CASE(Q)                                                                           \
     case Q:                                                                                          \
-        learned_nq_lv<Q, kLV>(kind, a, la, ringCap, nb, lds, s);                                       \
+        if (fused) learned_nq_lv<Q, kLV, true>(kind, a, la, ringCap, nb, lds, s);                     \
+        else learned_nq_lv<Q, kLV, false>(kind, a, la, ringCap, nb, lds, s);                          \
         break;
     switch (nq) {
         VSDMO_LEARN_CASE(2)

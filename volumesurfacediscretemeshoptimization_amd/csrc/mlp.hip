@@ -25,10 +25,6 @@
 
 namespace vsdmo {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_d(float z)
 {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
@@ -239,12 +235,6 @@ __global__ void k_features(const float* __restrict__ x, const float* __restrict_
     v0[3 * i] = x[v]; v0[3 * i + 1] = y[v]; v0[3 * i + 2] = z[v];
 }
 
-// Fused inference: features (from the positions, or rows of Xin) -> LDS,
-// layers 1 and 2 on MFMA with the weights' 16-byte rows read straight from
-// global memory, bias + GELU to BF16 in LDS, layer 3 (128 -> 3) on the
-// VALU.  Writes Y and L of rows [0, n).
-constexpr int kLdx = kIn + 8;    // LDS row pitches (bf16), multiples of 8
-constexpr int kLdh = kHid + 8;
 struct InferArgs {
     const float* x;
     const float* y;
@@ -258,140 +248,30 @@ struct InferArgs {
     float* L;
 };
 
-// 16 rows per block of 4 waves.  Features: 16 threads per row gather two
-// neighbours each; L is summed in neighbour order (the order of k_features).
-// Layers 1-2: wave w computes columns [32w, 32w + 32) -- two 16x16 tiles of
-// mfma_f32_16x16x32_bf16 (A: 16 LDS rows, lane l holds row l%16, k = 8(l/16)
-// + 0..7; B: 16-byte weight rows from global memory; D: column l%16, rows
-// 4(l/16) + 0..3), then bias + GELU to BF16 in LDS.  Layer 3 (128 -> 3) on
-// the VALU, one thread per (row, output).
-constexpr int kInferRows = 16;
-
+// Fused inference, 16 rows per block of 4 waves (mlp_features16 /
+// mlp_layers16): features (from the positions, or rows of Xin) -> LDS,
+// layers 1-2 on MFMA, layer 3 on the VALU.  Writes Y and L of rows [0, n).
 __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a)
 {
-    constexpr int R = kInferRows;
-    __shared__ __align__(16) uint16_t sX[R * kLdx];
-    __shared__ __align__(16) uint16_t sH1[R * kLdh];
-    __shared__ __align__(16) uint16_t sH2[R * kLdh];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int row0 = blockIdx.x * R;
+    __shared__ MlpLds sm;
+    const int tid = threadIdx.x;
+    const int row0 = blockIdx.x * kMlpRows;
     if (a.Xin) {
-        for (int i = tid; i < R * kIn; i += 256) {
+        for (int i = tid; i < kMlpRows * kIn; i += 256) {
             int r = i / kIn, c = i - r * kIn;
-            sX[r * kLdx + c] = (row0 + r < a.n) ? a.Xin[(size_t)(row0 + r) * kIn + c] : (uint16_t)0;
+            sm.X[r * kLdx + c] = (row0 + r < a.n) ? a.Xin[(size_t)(row0 + r) * kIn + c] : (uint16_t)0;
         }
     } else {
-        __shared__ float sRel[R][kNbr][3];
-        __shared__ float sDist[R][kNbr];
-        __shared__ float sL[R];
-        __shared__ int sCnt[R];
-        const int r = tid >> 4, q = tid & 15;
-        const int v = a.lo + row0 + r;
+        const int r = tid >> 4;
         const bool live = row0 + r < a.n;
-        if (live) {
-            const int* nb = a.nbr + (size_t)v * kNbr;
-            const float vx = a.x[v], vy = a.y[v], vz = a.z[v];
-#pragma unroll
-            for (int k = 2 * q; k < 2 * q + 2; ++k) {
-                const int u = nb[k];
-                if (u < 0) { sDist[r][k] = -1.0f; continue; }
-                const float dx = a.x[u] - vx, dy = a.y[u] - vy, dz = a.z[u] - vz;
-                sRel[r][k][0] = dx; sRel[r][k][1] = dy; sRel[r][k][2] = dz;
-                sDist[r][k] = sqrtf((dx * dx + dy * dy) + dz * dz);
-            }
-        }
-        __syncthreads();
-        if (q == 0) {
-            float sum = 0.0f, l = 1.0f;
-            int cnt = 0;
-            if (live) {
-                for (int k = 0; k < kNbr; ++k) {
-                    if (sDist[r][k] < 0.0f) break;
-                    sum += sDist[r][k];
-                    ++cnt;
-                }
-                l = cnt > 0 ? sum / (float)cnt : 1.0f;
-                if (!(l > 0.0f)) l = 1.0f;
-                a.L[row0 + r] = l;
-            }
-            sL[r] = l;
-            sCnt[r] = live ? cnt : 0;
-        }
-        __syncthreads();
-        const float l = sL[r];
-        const int cnt = sCnt[r];
-        uint16_t* o = sX + r * kLdx;
-#pragma unroll
-        for (int k = 2 * q; k < 2 * q + 2; ++k) {
-            if (k < cnt) {
-                o[3 * k] = f2bf(sRel[r][k][0] / l);
-                o[3 * k + 1] = f2bf(sRel[r][k][1] / l);
-                o[3 * k + 2] = f2bf(sRel[r][k][2] / l);
-            } else {
-                o[3 * k] = 0; o[3 * k + 1] = 0; o[3 * k + 2] = 0;
-            }
-        }
+        mlp_features16(a.x, a.y, a.z, a.nbr, a.lo + row0 + r, live, sm);
+        if (live && (tid & 15) == 0) a.L[row0 + r] = sm.L[r];
     }
     __syncthreads();
-    const int r16 = lane & 15, kq = (lane >> 4) * 8;
-    // layer 1: 16 rows x 32 columns per wave
-    {
-        f32x4 acc[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-        for (int k0 = 0; k0 < kIn; k0 += 32) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sX + r16 * kLdx + k0 + kq);
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int col = (2 * wave + t) * 16 + r16;
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.wb + kOW1 + (size_t)col * kIn + k0 + kq);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int n = (2 * wave + t) * 16 + r16;
-            const float b = a.w[kOB1 + n];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) sH1[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
-        }
-    }
-    __syncthreads();
-    // layer 2
-    {
-        f32x4 acc[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-        for (int k0 = 0; k0 < kHid; k0 += 32) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sH1 + r16 * kLdh + k0 + kq);
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int col = (2 * wave + t) * 16 + r16;
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(a.wb + kOW2 + (size_t)col * kHid + k0 + kq);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int n = (2 * wave + t) * 16 + r16;
-            const float b = a.w[kOB2 + n];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) sH2[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
-        }
-    }
-    __syncthreads();
-    // layer 3 on the VALU: thread -> (row, output)
-    if (tid < R * kOut) {
+    mlp_layers16(a.wb, a.w, sm);
+    if (tid < kMlpRows * kOut) {
         const int r = tid / kOut, o = tid - r * kOut;
-        if (row0 + r < a.n) {
-            float acc = 0.0f;
-            const uint16_t* h = sH2 + r * kLdh;
-            const uint16_t* w3 = a.wb + kOW3 + o * kHid;
-            for (int k = 0; k < kHid; ++k) acc = fmaf(bf2f(h[k]), bf2f(w3[k]), acc);
-            a.Y[(size_t)(row0 + r) * kOut + o] = acc + a.w[kOB3 + o];
-        }
+        if (row0 + r < a.n) a.Y[(size_t)(row0 + r) * kOut + o] = sm.Y[r][o];
     }
 }
 
@@ -401,7 +281,7 @@ void launch_mlp_infer(vsdmo_mlp* m, const float* x, const float* y, const float*
     const int n = hi - lo;
     if (n <= 0) return;
     InferArgs a{ x, y, z, nbr, lo, n, Xin, m->wb.p, m->w.p, Y, L };
-    k_mlp_infer<<<(n + kInferRows - 1) / kInferRows, 256, 0, s>>>(a);
+    k_mlp_infer<<<(n + kMlpRows - 1) / kMlpRows, 256, 0, s>>>(a);
 }
 
 __global__ void k_targets(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, int lo,

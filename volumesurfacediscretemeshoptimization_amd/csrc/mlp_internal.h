@@ -40,6 +40,149 @@ __host__ __device__ inline float bf2f(uint16_t h)
     return f;
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+
+// ------------------------------------------------------------- inference
+// 16 rows (vertices) per block of 4 waves (256 threads), all in LDS.
+constexpr int kMlpRows = 16;
+constexpr int kLdx = kIn + 8;    // LDS row pitches (bf16), multiples of 8
+constexpr int kLdh = kHid + 8;
+struct MlpLds {
+    __align__(16) uint16_t X[kMlpRows * kLdx];
+    __align__(16) uint16_t H1[kMlpRows * kLdh];
+    __align__(16) uint16_t H2[kMlpRows * kLdh];
+    float rel[kMlpRows][kNbr][3];
+    float dist[kMlpRows][kNbr];
+    float L[kMlpRows];
+    int cnt[kMlpRows];
+    float Y[kMlpRows][kOut];
+};
+
+// features of row r = threadIdx.x >> 4 (vertex v; live = it exists): 16
+// threads per row gather two table neighbours each; L = mean distance summed
+// in neighbour order (the order of k_features); X = bf16((n_k - v) / L),
+// zero padded.  All 256 threads call it; ends with a barrier.
+__device__ __forceinline__ void mlp_features16(const float* __restrict__ x, const float* __restrict__ y,
+                                               const float* __restrict__ z, const int* __restrict__ nbr, int v,
+                                               bool live, MlpLds& sm)
+{
+    const int r = threadIdx.x >> 4, q = threadIdx.x & 15;
+    if (live) {
+        const int* nb = nbr + (size_t)v * kNbr;
+        const float vx = x[v], vy = y[v], vz = z[v];
+#pragma unroll
+        for (int k = 2 * q; k < 2 * q + 2; ++k) {
+            const int u = nb[k];
+            if (u < 0) { sm.dist[r][k] = -1.0f; continue; }
+            const float dx = x[u] - vx, dy = y[u] - vy, dz = z[u] - vz;
+            sm.rel[r][k][0] = dx; sm.rel[r][k][1] = dy; sm.rel[r][k][2] = dz;
+            sm.dist[r][k] = sqrtf((dx * dx + dy * dy) + dz * dz);
+        }
+    }
+    __syncthreads();
+    if (q == 0) {
+        float sum = 0.0f, l = 1.0f;
+        int cnt = 0;
+        if (live) {
+            for (int k = 0; k < kNbr; ++k) {
+                if (sm.dist[r][k] < 0.0f) break;
+                sum += sm.dist[r][k];
+                ++cnt;
+            }
+            l = cnt > 0 ? sum / (float)cnt : 1.0f;
+            if (!(l > 0.0f)) l = 1.0f;
+        }
+        sm.L[r] = l;
+        sm.cnt[r] = live ? cnt : 0;
+    }
+    __syncthreads();
+    const float l = sm.L[r];
+    const int cnt = sm.cnt[r];
+    uint16_t* o = sm.X + r * kLdx;
+#pragma unroll
+    for (int k = 2 * q; k < 2 * q + 2; ++k) {
+        if (k < cnt) {
+            o[3 * k] = f2bf(sm.rel[r][k][0] / l);
+            o[3 * k + 1] = f2bf(sm.rel[r][k][1] / l);
+            o[3 * k + 2] = f2bf(sm.rel[r][k][2] / l);
+        } else {
+            o[3 * k] = 0; o[3 * k + 1] = 0; o[3 * k + 2] = 0;
+        }
+    }
+    __syncthreads();
+}
+
+// the three layers of the 16 rows of sm.X into sm.Y.  Layers 1-2: wave w
+// computes columns [32w, 32w + 32) -- two 16x16 tiles of
+// mfma_f32_16x16x32_bf16 (A: 16 LDS rows, lane l holds row l%16, k = 8(l/16)
+// + 0..7; B: 16-byte weight rows from global memory; D: column l%16, rows
+// 4(l/16) + 0..3), then bias + GELU to BF16 in LDS.  Layer 3 (128 -> 3) on
+// the VALU, one thread per (row, output).  All 256 threads call it; ends
+// with a barrier.
+__device__ __forceinline__ void mlp_layers16(const uint16_t* __restrict__ wb, const float* __restrict__ w, MlpLds& sm)
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    {
+        f32x4 acc[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int k0 = 0; k0 < kIn; k0 += 32) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.X + r16 * kLdx + k0 + kq);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int col = (2 * wave + t) * 16 + r16;
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wb + kOW1 + (size_t)col * kIn + k0 + kq);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int n = (2 * wave + t) * 16 + r16;
+            const float b = w[kOB1 + n];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sm.H1[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
+        }
+    }
+    __syncthreads();
+    {
+        f32x4 acc[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int k0 = 0; k0 < kHid; k0 += 32) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.H1 + r16 * kLdh + k0 + kq);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int col = (2 * wave + t) * 16 + r16;
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wb + kOW2 + (size_t)col * kHid + k0 + kq);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int n = (2 * wave + t) * 16 + r16;
+            const float b = w[kOB2 + n];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sm.H2[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
+        }
+    }
+    __syncthreads();
+    if (tid < kMlpRows * kOut) {
+        const int r = tid / kOut, o = tid - r * kOut;
+        float acc = 0.0f;
+        const uint16_t* h = sm.H2 + r * kLdh;
+        const uint16_t* w3 = wb + kOW3 + o * kHid;
+        for (int k = 0; k < kHid; ++k) acc = fmaf(bf2f(h[k]), bf2f(w3[k]), acc);
+        sm.Y[r][o] = acc + w[kOB3 + o];
+    }
+    __syncthreads();
+}
+
 // canonical neighbour table of the inner vertices: the distinct vertices of
 // the vertex's volume one-ring in order of first appearance (ring elements in
 // their stored order), at most kNbr, -1 padded; rows of surface vertices are
