@@ -108,6 +108,25 @@ def test_dmo_positions_parity(oracle, kind, n, iters):
     opt.close()
 
 
+@pytest.mark.parametrize("w", [1, 4])
+def test_hex_pinned_vertices_parity(oracle, w):
+    """Jitter 0.45h inverts hexes: a vertex whose ring is held at quality 0
+    by sub-tets it is not part of skips its search on the GPU (no position
+    can beat 0), and the positions stay bit-exact with the oracle's full
+    search, with one and with four waves per vertex."""
+    xyz, hexes = synth.hex_cube(9, seed=61, jitter=0.45)
+    g = _build("hex", xyz, hexes)
+    opt = DMOHexClass(g, n_iter=3, fixed_boundary=True, waves_per_vertex=w)
+    o = oracle.OracleMesh("hex", xyz, hexes)
+    assert (o.element_qualities() == 0.0).sum() >= 20
+    for it in range(3):
+        opt.doIteration()
+        o.iterate()
+        gp = g.positions(original_order=False)
+        assert np.array_equal(gp, o.positions()), (it, int((gp != o.positions()).any(axis=1).sum()))
+    opt.close()
+
+
 def test_c1_full_size(oracle):
     """40^3 x 6 tet cube at full size: two iterations bit-exact against the
     oracle, then ten in total with a non-decreasing minimum quality."""

@@ -112,7 +112,7 @@ struct vsdmo_opt {
     DevBuf<float> lL, lY;
     DevBuf<int> lFallback;           // running count of fallbacks to the full search
     DevBuf<int> lList, lCnt;         // fallback list of a class; per-class list lengths
-    int64_t s_learned = 0, fallbacks = 0;
+    int64_t s_learned = 0;
     vsdmo_dataset* rec = nullptr;   // training pairs are recorded while set
     std::string fail_dump;          // mesh file written when the quality guard fails
 };
@@ -656,12 +656,6 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
             float prev = o->last_min;
             opt_quality(o);
             o->curr_it++;
-            if (o->lFallback.p) {
-                int f = 0;
-                o->lFallback.download(&f, 1, m->stream);
-                VSDMO_HIP(hipStreamSynchronize(m->stream));
-                o->fallbacks = f;
-            }
             if (o->p.quality_guard) {
                 float tol = o->p.guard_tolerance * std::fabs(prev);
                 if (o->last_min < prev - tol) {
@@ -727,6 +721,11 @@ vsdmo_status vsdmo_opt_element_qualities(const vsdmo_opt* o, float* host_out)
 vsdmo_status vsdmo_opt_stats(const vsdmo_opt* o, vsdmo_stats* out)
 {
     if (!o || !out) return fail(VSDMO_INVALID_ARG, "NULL argument");
+    int fb = 0;
+    if (o->lFallback.p) {   // the device counter is read here, not per iteration
+        vsdmo_status st = copy_dev(o->lFallback, 1, &fb, o->m->stream);
+        if (st != VSDMO_OK) return st;
+    }
     out->searches_inner = o->s_inner;
     out->searches_surface = o->s_surf;
     out->searches_feature = o->s_feat;
@@ -735,7 +734,7 @@ vsdmo_status vsdmo_opt_stats(const vsdmo_opt* o, vsdmo_stats* out)
     out->kernel_ms = o->kernel_ms;
     out->kernel_launches = o->launches;
     out->searches_learned = o->s_learned;
-    out->fallbacks = o->fallbacks;
+    out->fallbacks = fb;
     return VSDMO_OK;
 }
 

@@ -310,6 +310,17 @@ __device__ HexFrame hex_stage(const SearchArgs& a, int vid, bool emin, HexLds* s
     return f;
 }
 
+// Whether the vertex cannot move: when the sub-tets it is not part of
+// already score 0 (a determinant below epsD or a Frobenius term below epsF),
+// every candidate's accumulator starts from acc0 and takes fminf of those
+// minima, so every candidate -- the start included -- scores exactly 0 and
+// none is strictly better than the start.  The search is skipped; the result
+// is the full search's.
+__device__ __forceinline__ bool hex_pinned(const HexFrame& f)
+{
+    return f.acc0.dmin < f.epsD || f.acc0.fmin < f.epsF;
+}
+
 __device__ __forceinline__ float hex_score(const HexFrame& f, const HexLds* s_hx, float px, float py, float pz)
 {
     const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
@@ -395,6 +406,7 @@ __device__ void hex_vertex(const SearchArgs& a, int vid, bool emin, HexLds* s_hx
 {
     const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
     const HexFrame f = hex_stage<W>(a, vid, emin, s_hx, rd);
+    if (hex_pinned(f)) return;   // uniform over the block: f is computed by every thread from the LDS
     const float qcur = hex_score(f, s_hx, 0.0f, 0.0f, 0.0f);
     float cx = f.x0x, cy = f.x0y, cz = f.x0z;
     if (hex_levels<NQ, W>(a, f, s_hx, rd, qcur, cx, cy, cz) && tid == 0) {
@@ -523,16 +535,21 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v)
 // window `frac` times the plain first-level window, for `depthL` levels with
 // the window shrunk by 2/3 per level; the centre moves to a level's best
 // candidate when it is strictly better than the best quality so far, which
-// starts at q(v).  Fallback rule: when no candidate of any level beats q(v),
-// the vertex goes on the class's fallback list, whose plain full search runs
-// as a second launch; the result is the plain path's, bit for bit.
+// starts at q(v).  Fallback rule: when the window's best candidate is lower
+// than q(v), the vertex goes on the class's fallback list, whose plain full
+// search runs as a second launch; the result is the plain path's, bit for
+// bit.  A tie at a positive quality (a level's best equals q(v) > 0) leaves
+// the vertex where it is: it is the mark of a ring whose minimum is held by
+// sub-tets the vertex is not part of (hexes), which no position raises.  At
+// quality 0 a tie can also be the vertex's own inverted sub-tet, which the
+// full window may still repair: such vertices fall back, unless hex_pinned.
 struct LearnArgs {
     const float* Y;
     const float* L;
     int mode;
     int depthL;
     float frac;
-    int* fallbacks;   // running total
+    int* fallbacks;   // running total (the list launch adds the class's count)
     int* list;        // this class's fallback vertices (appended)
     int* cnt;         // their number (zero before the class)
     // network run inside the window kernel (mode 1, one 16-vertex tile per
@@ -625,6 +642,20 @@ __device__ __forceinline__ void learned_offset(const SearchArgs& a, const LearnA
     (void)ci;
 }
 
+// append vid to the class's fallback list where need (one lane per group):
+// one atomic per wave for the wave's fallbacks (the list order does not
+// matter: the class's vertices are independent).  Every lane calls it.
+__device__ __forceinline__ void fallback_append(const LearnArgs& la, bool need, int vid)
+{
+    const unsigned long long bal = __ballot(need);
+    if (bal == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(la.cnt, __popcll(bal));
+    base = __shfl(base, 0);
+    if (need) la.list[base + __popcll(bal & ((1ull << lane) - 1ull))] = vid;
+}
+
 // one block: LV vertices on one wave, or (FUSED) 4 waves of LV = 4 vertices
 // that first run their 16-vertex network tile
 template <int NQ, int LV, bool FUSED>
@@ -667,7 +698,7 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_tet_learned(SearchA
     float cx = f.x0x, cy = f.x0y, cz = f.x0z;
     if (la.mode == 1) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
     float qbest = q0;
-    bool moved = false;
+    bool moved = false, tie = false;
     const float affL = 1.0f / (float)(NL - 1);
     const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
     float ds = la.frac * a.gridScale;
@@ -704,6 +735,7 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_tet_learned(SearchA
         }
         int cand = i0 + bi + NL * (s % (NL * NL));
         group_argmax<G>(best, cand);
+        tie = tie || (best == q0 && q0 > 0.0f);
         if (qbest < best) {
             cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
             cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
@@ -714,11 +746,7 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_tet_learned(SearchA
         ds = ds * (2.0f / (float)(NL - 1));
     }
     if (moved && s == 0 && ci0 + gb < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
-
-    if (!moved && s == 0 && ci0 + gb < nc) {
-        atomicAdd(la.fallbacks, 1);
-        la.list[atomicAdd(la.cnt, 1)] = vid;
-    }
+    fallback_append(la, !moved && !tie && s == 0 && ci0 + gb < nc, vid);
 }
 
 template <int NQ, int LV, bool FUSED>
@@ -766,7 +794,7 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchA
     float cx = f.x0x, cy = f.x0y, cz = f.x0z;
     if (la.mode == 1) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
     float qbest = q0;
-    bool moved = false;
+    bool moved = false, tie = false;
     const float affL = 1.0f / (float)(NL - 1);
     const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
     float ds = la.frac * a.gridScale;
@@ -807,6 +835,7 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchA
         }
         int cand = i0 + bi + NL * (s % (NL * NL));
         group_argmax<G>(best, cand);
+        tie = tie || (best == q0 && q0 > 0.0f);
         if (qbest < best) {
             cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
             cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
@@ -817,11 +846,7 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchA
         ds = ds * (2.0f / (float)(NL - 1));
     }
     if (moved && s == 0 && ci0 + gb < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
-
-    if (!moved && s == 0 && ci0 + gb < nc) {
-        atomicAdd(la.fallbacks, 1);
-        la.list[atomicAdd(la.cnt, 1)] = vid;
-    }
+    fallback_append(la, !moved && !tie && !hex_pinned(f) && s == 0 && ci0 + gb < nc, vid);
 }
 
 // plain search of the vertices list[0 .. *cnt) (the fallbacks of a class),
@@ -830,11 +855,12 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchA
 // on the device)
 template <int NQ, int W>
 __global__ __launch_bounds__(64 * W) void k_search_list_tet(SearchArgs a, const int* __restrict__ list,
-                                                           const int* __restrict__ cnt)
+                                                           const int* __restrict__ cnt, int* fallbacks)
 {
     extern __shared__ TetC s_tcf[];
     __shared__ RedLds<W> rd;
     const int n = *cnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(fallbacks, n);
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         tet_vertex<NQ, W>(a, list[i], a.extentMin != 0, s_tcf, rd);
         __syncthreads();   // the next vertex restages the LDS ring
@@ -842,21 +868,23 @@ __global__ __launch_bounds__(64 * W) void k_search_list_tet(SearchArgs a, const 
 }
 template <int NQ, int W>
 __global__ __launch_bounds__(64 * W) void k_search_list_hex(SearchArgs a, const int* __restrict__ list,
-                                                           const int* __restrict__ cnt)
+                                                           const int* __restrict__ cnt, int* fallbacks)
 {
     extern __shared__ HexLds s_hxf[];
     __shared__ RedLds<W> rd;
     const int n = *cnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(fallbacks, n);
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         hex_vertex<NQ, W>(a, list[i], a.extentMin != 0, s_hxf, rd);
         __syncthreads();   // the next vertex restages the LDS ring
     }
 }
 template <int NQ, int W>
-static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* cnt, int grid, size_t lds, hipStream_t s)
+static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* cnt, int* fallbacks, int grid,
+                     size_t lds, hipStream_t s)
 {
-    if (kind == VSDMO_TET) k_search_list_tet<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
-    else k_search_list_hex<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt);
+    if (kind == VSDMO_TET) k_search_list_tet<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt, fallbacks);
+    else k_search_list_hex<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt, fallbacks);
 }
 
 template <int NQ, int LV, bool FUSED>
@@ -908,14 +936,14 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
     const size_t lds1 = (size_t)ringCap * per;
     const int g1 = std::max(1, std::min(nc, std::max(256, nc / 4)));
     switch (nq) {
-    case 2: list_nqw<2, 2>(kind, a, list, cnt, g1, lds1, s); break;
-    case 4: list_nqw<4, 4>(kind, a, list, cnt, g1, lds1, s); break;
-    case 6: list_nqw<6, 2>(kind, a, list, cnt, g1, lds1, s); break;
+    case 2: list_nqw<2, 2>(kind, a, list, cnt, fallbacks, g1, lds1, s); break;
+    case 4: list_nqw<4, 4>(kind, a, list, cnt, fallbacks, g1, lds1, s); break;
+    case 6: list_nqw<6, 2>(kind, a, list, cnt, fallbacks, g1, lds1, s); break;
     case 8:
         // a short tet list is latency-bound (4 waves per vertex); hexes fall
         // back often and are throughput-bound (one wave per vertex)
-        if (kind == VSDMO_TET) list_nqw<8, 4>(kind, a, list, cnt, g1, lds1, s);
-        else list_nqw<8, 1>(kind, a, list, cnt, std::max(1, std::min(nc, 4096)), lds1, s);
+        if (kind == VSDMO_TET) list_nqw<8, 4>(kind, a, list, cnt, fallbacks, g1, lds1, s);
+        else list_nqw<8, 1>(kind, a, list, cnt, fallbacks, std::max(1, std::min(nc, 4096)), lds1, s);
         break;
     default: break;
     }
