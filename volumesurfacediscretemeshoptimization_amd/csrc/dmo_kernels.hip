@@ -101,7 +101,32 @@ __device__ __forceinline__ void block_argmax(float& best, int& idx, float* sf, i
 
 // LDS scratch of the block reductions
 template <int W>
-struct RedLds { float f[W]; int i[W]; };
+struct RedLds { float f[W]; int i[W]; float e[4][W]; };
+
+// The staging reductions in one exchange: per-axis window half-extents (min
+// or max of the offsets) and the largest offset.  One barrier: rd.e is read
+// here and written again only by the next vertex's staging, after the
+// barrier that ends this one.
+template <int W>
+__device__ __forceinline__ void block_extents(bool emin, float& ex, float& ey, float& ez, float& mabs, RedLds<W>& rd)
+{
+    if (emin) { ex = wave_min(ex); ey = wave_min(ey); ez = wave_min(ez); }
+    else { ex = wave_max(ex); ey = wave_max(ey); ez = wave_max(ez); }
+    mabs = wave_max(mabs);
+    if constexpr (W > 1) {
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { rd.e[0][w] = ex; rd.e[1][w] = ey; rd.e[2][w] = ez; rd.e[3][w] = mabs; }
+        __syncthreads();
+        ex = rd.e[0][0]; ey = rd.e[1][0]; ez = rd.e[2][0]; mabs = rd.e[3][0];
+#pragma unroll
+        for (int k = 1; k < W; ++k) {
+            ex = emin ? fminf(ex, rd.e[0][k]) : fmaxf(ex, rd.e[0][k]);
+            ey = emin ? fminf(ey, rd.e[1][k]) : fmaxf(ey, rd.e[1][k]);
+            ez = emin ? fminf(ez, rd.e[2][k]) : fmaxf(ez, rd.e[2][k]);
+            mabs = fmaxf(mabs, rd.e[3][k]);
+        }
+    }
+}
 
 // ------------------------------------------------------------------ tets
 // The staged state of one vertex search: start position, window
@@ -137,9 +162,7 @@ __device__ TetFrame tet_stage(const SearchArgs& a, int vid, bool emin, TetC* s_t
             mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
         }
     }
-    if (emin) { ex = block_min<W>(ex, rd.f); ey = block_min<W>(ey, rd.f); ez = block_min<W>(ez, rd.f); }
-    else { ex = block_max<W>(ex, rd.f); ey = block_max<W>(ey, rd.f); ez = block_max<W>(ez, rd.f); }
-    mabs = block_max<W>(mabs, rd.f);
+    block_extents<W>(emin, ex, ey, ez, mabs, rd);
     f.ex = ex; f.ey = ey; f.ez = ez;
     f.sg = pow2_scale(mabs);
     const float isg = 1.0f / f.sg;
@@ -282,9 +305,7 @@ __device__ HexFrame hex_stage(const SearchArgs& a, int vid, bool emin, HexLds* s
             mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
         }
     }
-    if (emin) { ex = block_min<W>(ex, rd.f); ey = block_min<W>(ey, rd.f); ez = block_min<W>(ez, rd.f); }
-    else { ex = block_max<W>(ex, rd.f); ey = block_max<W>(ey, rd.f); ez = block_max<W>(ez, rd.f); }
-    mabs = block_max<W>(mabs, rd.f);
+    block_extents<W>(emin, ex, ey, ez, mabs, rd);
     f.ex = ex; f.ey = ey; f.ez = ez;
     f.sg = pow2_scale(mabs);
     const float s2 = f.sg * f.sg, s3 = s2 * f.sg;
@@ -437,9 +458,8 @@ __device__ void flat_vertex(const SearchArgs& a, int vid, TriC* s_tr, RedLds<W>&
         ey = fmaxf(ey, dy);
         mabs = fmaxf(mabs, fmaxf(dx, dy));
     }
-    ex = block_max<W>(ex, rd.f);
-    ey = block_max<W>(ey, rd.f);
-    mabs = block_max<W>(mabs, rd.f);
+    float ez = 0.0f;
+    block_extents<W>(false, ex, ey, ez, mabs, rd);
     const float sg = pow2_scale(mabs);
     const float isg = 1.0f / sg;
     const float is2 = (1.0f * isg) * isg;
@@ -940,10 +960,10 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
     case 4: list_nqw<4, 4>(kind, a, list, cnt, fallbacks, g1, lds1, s); break;
     case 6: list_nqw<6, 2>(kind, a, list, cnt, fallbacks, g1, lds1, s); break;
     case 8:
-        // a short tet list is latency-bound (4 waves per vertex); hexes fall
-        // back often and are throughput-bound (one wave per vertex)
-        if (kind == VSDMO_TET) list_nqw<8, 4>(kind, a, list, cnt, fallbacks, g1, lds1, s);
-        else list_nqw<8, 1>(kind, a, list, cnt, fallbacks, std::max(1, std::min(nc, 4096)), lds1, s);
+        // the fallback lists are short (a few per cent of the class): latency
+        // bound, 4 waves per vertex (1, 2, 4 and 8 measured on the 40^3 tet
+        // and 48^3 hex cubes)
+        list_nqw<8, 4>(kind, a, list, cnt, fallbacks, g1, lds1, s);
         break;
     default: break;
     }

@@ -23,6 +23,7 @@ constexpr int kOB2 = kOW2 + kHid * kHid;
 constexpr int kOW3 = kOB2 + kHid;
 constexpr int kOB3 = kOW3 + kOut * kHid;
 constexpr int kNParams = kOB3 + kOut;
+static_assert(kOW1 % 8 == 0 && kOW2 % 8 == 0 && kOW3 % 8 == 0, "weight rows load as 16-byte bf16x8");
 
 // round-to-nearest-even FP32 -> BF16 bits
 __host__ __device__ inline uint16_t f2bf(float f)
@@ -57,7 +58,6 @@ struct MlpLds {
     float rel[kMlpRows][kNbr][3];
     float dist[kMlpRows][kNbr];
     float L[kMlpRows];
-    int cnt[kMlpRows];
     float Y[kMlpRows][kOut];
 };
 
@@ -82,25 +82,26 @@ __device__ __forceinline__ void mlp_features16(const float* __restrict__ x, cons
             sm.dist[r][k] = sqrtf((dx * dx + dy * dy) + dz * dz);
         }
     }
+    // the table's valid entries are a prefix: cnt = their number, counted by
+    // a ballot over the row's 16 threads (lanes 16 (r % 4) + q of the wave)
+    const int* nbv = nbr + (size_t)v * kNbr + 2 * q;
+    const bool v0 = live && nbv[0] >= 0, v1 = live && nbv[1] >= 0;
+    const unsigned long long rowMask = 0xffffull << (16 * (r & 3));
+    const int cnt = __popcll(__ballot(v0) & rowMask) + __popcll(__ballot(v1) & rowMask);
     __syncthreads();
     if (q == 0) {
         float sum = 0.0f, l = 1.0f;
-        int cnt = 0;
         if (live) {
-            for (int k = 0; k < kNbr; ++k) {
-                if (sm.dist[r][k] < 0.0f) break;
-                sum += sm.dist[r][k];
-                ++cnt;
-            }
+#pragma unroll
+            for (int k = 0; k < kNbr; ++k)   // neighbour order, as k_features
+                if (k < cnt) sum += sm.dist[r][k];
             l = cnt > 0 ? sum / (float)cnt : 1.0f;
             if (!(l > 0.0f)) l = 1.0f;
         }
         sm.L[r] = l;
-        sm.cnt[r] = live ? cnt : 0;
     }
     __syncthreads();
     const float l = sm.L[r];
-    const int cnt = sm.cnt[r];
     uint16_t* o = sm.X + r * kLdx;
 #pragma unroll
     for (int k = 2 * q; k < 2 * q + 2; ++k) {
@@ -119,9 +120,9 @@ __device__ __forceinline__ void mlp_features16(const float* __restrict__ x, cons
 // computes columns [32w, 32w + 32) -- two 16x16 tiles of
 // mfma_f32_16x16x32_bf16 (A: 16 LDS rows, lane l holds row l%16, k = 8(l/16)
 // + 0..7; B: 16-byte weight rows from global memory; D: column l%16, rows
-// 4(l/16) + 0..3), then bias + GELU to BF16 in LDS.  Layer 3 (128 -> 3) on
-// the VALU, one thread per (row, output).  All 256 threads call it; ends
-// with a barrier.
+// 4(l/16) + 0..3), then bias + GELU to BF16 in LDS.  Layer 3 (128 -> 3):
+// one tile on wave 0 with W3 zero padded to 16 columns.  All 256 threads
+// call it; ends with a barrier.
 __device__ __forceinline__ void mlp_layers16(const uint16_t* __restrict__ wb, const float* __restrict__ w, MlpLds& sm)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -172,13 +173,22 @@ __device__ __forceinline__ void mlp_layers16(const uint16_t* __restrict__ wb, co
         }
     }
     __syncthreads();
-    if (tid < kMlpRows * kOut) {
-        const int r = tid / kOut, o = tid - r * kOut;
-        float acc = 0.0f;
-        const uint16_t* h = sm.H2 + r * kLdh;
-        const uint16_t* w3 = wb + kOW3 + o * kHid;
-        for (int k = 0; k < kHid; ++k) acc = fmaf(bf2f(h[k]), bf2f(w3[k]), acc);
-        sm.Y[r][o] = acc + w[kOB3 + o];
+    if (wave == 0) {
+        // layer 3 on one 16x16 tile: columns 0..2 are W3's rows, the rest zero
+        typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+        f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int k0 = 0; k0 < kHid; k0 += 32) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.H2 + r16 * kLdh + k0 + kq);
+            bf16x8 bv = __builtin_bit_cast(bf16x8, (u32x4){ 0u, 0u, 0u, 0u });
+            if (r16 < kOut) bv = *reinterpret_cast<const bf16x8*>(wb + kOW3 + (size_t)r16 * kHid + k0 + kq);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+        }
+        if (r16 < kOut) {
+            const float b = w[kOB3 + r16];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sm.Y[(lane >> 4) * 4 + r][r16] = acc[r] + b;
+        }
     }
     __syncthreads();
 }
