@@ -251,8 +251,9 @@ def bench_learned(args):
         "metric": METRIC, "value": r["value"], "unit": "vertex updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["learned_ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 search, bf16 MLP", "data": "synthetic",
-        "config": dict(workload=desc + "; learned proposal (4x4x4 window 1/4 of the plain one, "
-                       f"{args.learn_depth} levels, fallback to the full search) vs plain", name=args.workload,
+        "config": dict(workload=desc + f"; learned proposal ({args.learn_depth} levels of 4x4x4 on "
+                       f"{args.learn_frac} of the plain first-level window, {args.learn_fallback} fallback to the "
+                       "full search) vs plain", name=args.workload,
                        mode="learned", **{k: v for k, v in r.items() if k not in ("value", "roofline")}),
         "roofline": r["roofline"],
         "cpu_baseline": None,
@@ -291,11 +292,12 @@ def _learned(args):
     train_s = time.perf_counter() - t0
     n_pairs = len(ds)
     ds.close()
-    # warm-up of both paths on a throwaway copy
+    # warm-up of both paths (and both fallback policies) on a throwaway copy
     wm = fac.create(xyz, cells)
-    wo = cls(wm, n_iter=2, **kw)
-    wo.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac)
-    wo.iterate(1)
+    wo = cls(wm, n_iter=4, **kw)
+    for pol in ("immediate", "deferred"):
+        wo.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac, fallback=pol)
+        wo.iterate(1)
     wo.set_proposal(None, mode=0)
     wo.iterate(1)
     wo.close()
@@ -311,24 +313,32 @@ def _learned(args):
     q_plain, qm_plain = op.getQuality(), op.getMeanQuality()
     plain_updates = stp["searches_inner"] + stp["searches_surface"] + stp["searches_feature"]
     # learned (wall clock without per-launch timing events, like the plain run)
-    ml = fac.create(xyz, cells)
-    ol = cls(ml, n_iter=3 * args.steps, **kw)
-    ol.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac)
-    its = 0
-    t0 = time.perf_counter()
-    while its < 3 * args.steps:
-        ol.doIteration()
-        its += 1
-        # equal quality: minimum and mean within 1e-3 of the plain run's
-        if ol.getQuality() >= q_plain - 1e-3 and ol.getMeanQuality() >= qm_plain - 1e-3:
-            break
-    dt_learned = time.perf_counter() - t0
-    stl = ol.stats()
-    q_l, qm_l = ol.getQuality(), ol.getMeanQuality()
+    def to_quality(policy):
+        ml = fac.create(xyz, cells)
+        ol = cls(ml, n_iter=3 * args.steps, **kw)
+        ol.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac, fallback=policy)
+        its = 0
+        t0 = time.perf_counter()
+        while its < 3 * args.steps:
+            ol.doIteration()
+            its += 1
+            # equal quality: minimum and mean within 1e-3 of the plain run's
+            if ol.getQuality() >= q_plain - 1e-3 and ol.getMeanQuality() >= qm_plain - 1e-3:
+                break
+        dt = time.perf_counter() - t0
+        out = (its, dt, ol.stats(), ol.getQuality(), ol.getMeanQuality())
+        ol.close()
+        ml.close()
+        return out
+
+    its, dt_learned, stl, q_l, qm_l = to_quality(args.learn_fallback)
+    other = "immediate" if args.learn_fallback == "deferred" else "deferred"
+    its_o, dt_o, st_o, q_o, qm_o = to_quality(other)
+    reached_o = q_o >= q_plain - 1e-3 and qm_o >= qm_plain - 1e-3
     # the same run again with HIP events around every launch, for the kernel time
     mk = fac.create(xyz, cells)
     ok = cls(mk, n_iter=its, **kw)
-    ok.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac)
+    ok.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac, fallback=args.learn_fallback)
     ok.set_timing(True)
     ok.iterate(its)
     stk = ok.stats()
@@ -342,8 +352,8 @@ def _learned(args):
     evals = stl["searches_learned"] * (64 * args.learn_depth + 1) + stl["fallbacks"] * 1540
     flops = evals * ring * per_elem
     achieved = flops / (stk["kernel_ms"] / 1e3) / 1e12
-    for o in (op, ol):
-        o.close()
+    op.close()
+    mp.close()
     mlp.close()
     return {
         "value": value,
@@ -357,13 +367,21 @@ def _learned(args):
         "reached_plain_quality": bool(reached), "min_quality_start": q0,
         "min_quality_plain": q_plain, "min_quality_learned": q_l,
         "mean_quality_plain": qm_plain, "mean_quality_learned": qm_l,
+        "fallback_policy": args.learn_fallback,
         "fallback_rate": stl["fallbacks"] / max(1, stl["searches_learned"]),
+        f"{other}_fallback": {"learned_iterations": its_o, "learned_ms_per_step": 1e3 * dt_o / its_o,
+                              "reached_plain_quality": bool(reached_o),
+                              "speedup_vs_plain": (plain_updates / dt_o) / (plain_updates / dt_plain) if reached_o else 0.0,
+                              "fallback_rate": st_o["fallbacks"] / max(1, st_o["searches_learned"])},
         "training_pairs": n_pairs, "training_meshes": s, "record_s": rec_s, "train_s": train_s,
         "epochs": args.epochs, "loss_first_epoch": float(losses[0]) if len(losses) else None,
         "loss_last_epoch": float(losses[-1]) if len(losses) else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": f"k_search_{kind}_learned<8> (proposal window + in-wave fallback)",
+                     "kernel": (f"k_search_{kind}_learned_def<8> (network, proposal windows and the deferred "
+                                "full searches in one launch per class)" if args.learn_fallback == "deferred" else
+                                f"k_search_{kind}_learned<8> + k_search_list_{kind}<8, 4> (windows, then the "
+                                "class's full searches)"),
                      "launches": stk["kernel_launches"],
                      "mean_launch_us": 1e3 * stk["kernel_ms"] / max(1, stk["kernel_launches"])},
     }
@@ -390,6 +408,8 @@ def main(argv=None):
     # default per element type, chosen on the seed-7 meshes, not the benchmark's
     ap.add_argument("--learn-depth", type=int, default=None, help="default: 2 (tet), 4 (hex)")
     ap.add_argument("--learn-frac", type=float, default=None, help="default: 0.045 (tet), 0.5 (hex)")
+    ap.add_argument("--learn-fallback", choices=["deferred", "immediate"], default="deferred",
+                    help="fallback policy of the measured learned run (the other one is reported beside it)")
     ap.add_argument("--tune-log", default=os.environ.get("VSDMO_TUNE_LOG", "autotune_batch256.jsonl"))
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -264,6 +264,13 @@ vsdmo_status vsdmo_opt_record(vsdmo_opt* o, vsdmo_dataset* ds, int32_t n);
  * itself; environment variable VSDMO_LEARN_SEPARATE_INFERENCE=1 (read here)
  * keeps it in a launch of its own (same results, for diagnosis). */
 vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac);
+/* when a vertex's proposal window finds nothing better than its start:
+ * IMMEDIATE -- its full search runs in the same class pass (a second launch);
+ * DEFERRED -- it keeps its position and its full search runs in the class's
+ * next pass, beside that pass's windows (one launch per class; nq 4 or 8,
+ * else VSDMO_NOT_SUPPORTED).  Setting it clears pending deferred searches. */
+enum { VSDMO_FALLBACK_IMMEDIATE = 0, VSDMO_FALLBACK_DEFERRED = 1 };
+vsdmo_status vsdmo_opt_set_fallback(vsdmo_opt* o, int32_t policy);
 
 /* ------------------------------------------------------------------------
  * Batched DMO over many independent meshes.  No counterpart in the

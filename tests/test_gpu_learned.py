@@ -14,7 +14,7 @@ import pytest
 
 import mlp_ref
 from volumesurfacediscretemeshoptimization_amd import (DMODataset, DMOHexClass, DMOMeshHexFactory, DMOMeshTetFactory,
-                                                       DMOMLP, DMOTetClass)
+                                                       DMOMLP, DMOTetClass, VsdmoError)
 from volumesurfacediscretemeshoptimization_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -227,3 +227,67 @@ def test_fused_network_equals_separate_launch(kind, monkeypatch):
     assert out[0][1] == out[1][1] and 0 < out[0][1]
     assert out[0][2] == out[1][2]
     mlp.close()
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_deferred_fallback_bad_proposal(kind):
+    """Deferred fallback with a proposal far outside every one-ring: one pass
+    defers every vertex (none moves), the class's next pass runs their full
+    searches and skips their windows, so 2k iterations equal k plain ones
+    bit for bit (hexes: pinned vertices are neither deferred nor moved by
+    the full search)."""
+    if kind == "tet":
+        xyz, cells = synth.tet_cube(10, seed=12, jitter=0.2)
+        fac, cls = DMOMeshTetFactory, DMOTetClass
+    else:
+        xyz, cells = synth.hex_cube(9, seed=12, jitter=0.3)
+        fac, cls = DMOMeshHexFactory, DMOHexClass
+    mlp = DMOMLP(seed=1)
+    p = np.zeros(mlp.n_params, dtype=np.float32)
+    p[-3:] = 50.0
+    mlp.set_params(p)
+    m1, m2 = fac.create(xyz, cells), fac.create(xyz, cells)
+    o1 = cls(m1, n_iter=2, fixed_boundary=True)
+    o2 = cls(m2, n_iter=4, fixed_boundary=True)
+    o2.set_proposal(mlp, mode=1, depth=2, window_frac=0.25, fallback="deferred")
+    p0 = m2.positions(original_order=False)
+    for k in range(2):
+        o2.doIteration()
+        if k == 0:
+            assert np.array_equal(m2.positions(original_order=False), p0)
+        o2.doIteration()
+        o1.doIteration()
+        assert np.array_equal(m1.positions(original_order=False), m2.positions(original_order=False)), k
+        assert o1.getQuality() == o2.getQuality()
+    st = o2.stats()
+    if kind == "tet":
+        assert st["fallbacks"] == 2 * m1.counts()["n_inner"]
+    assert st["searches_learned"] == 4 * m1.counts()["n_inner"]
+    for o, m in ((o1, m1), (o2, m2)):
+        o.close()
+        m.close()
+    mlp.close()
+
+
+def test_deferred_fallback_monotone_and_policy_checks():
+    """Identity proposal with deferred fallback: the minimum quality never
+    decreases; the policy needs nq 4 or 8."""
+    xyz, tets = synth.tet_cube(12, seed=13)
+    m = DMOMeshTetFactory.create(xyz, tets)
+    o = DMOTetClass(m, n_iter=6, fixed_boundary=True)
+    o.set_proposal(None, mode=2, depth=2, window_frac=0.25, fallback="deferred")
+    q = [o.getQuality()]
+    for _ in range(6):
+        o.doIteration()
+        q.append(o.getQuality())
+    assert all(b >= a for a, b in zip(q, q[1:])), q
+    st = o.stats()
+    assert 0 < st["fallbacks"] < st["searches_learned"]
+    with pytest.raises(ValueError):
+        o.set_fallback("later")
+    o.close()
+    o6 = DMOTetClass(m, n_iter=1, fixed_boundary=True, nq=6)
+    with pytest.raises(VsdmoError):
+        o6.set_fallback("deferred")
+    o6.close()
+    m.close()

@@ -329,17 +329,33 @@ class _DMOBase:
     # window), chosen on a 40^3 tet cube and a 48^3 hex cube
     LEARN_WINDOW = {_lib.VSDMO_TET: (2, 0.045), _lib.VSDMO_HEX: (4, 0.5)}
 
-    def set_proposal(self, mlp=None, mode: int = 1, depth: int = None, window_frac: float = None):
+    def set_proposal(self, mlp=None, mode: int = 1, depth: int = None, window_frac: float = None,
+                     fallback: str = None):
         """Learned proposal for the inner searches: mode 1 = network, 2 = the
         vertex itself (same windowed search and fallback, no network), 0 = off.
         The window is `depth` levels of 4^3 candidates on `window_frac` times
-        the plain first-level window (defaults per element type above)."""
+        the plain first-level window (defaults per element type above).
+        fallback: "immediate" (the full search of a vertex whose window finds
+        nothing better runs in the same class pass) or "deferred" (it runs in
+        the class's next pass, beside that pass's windows); None keeps the
+        current policy (initially immediate)."""
         d0, f0 = self.LEARN_WINDOW.get(self.mesh.elem, (2, 0.045))
         depth = d0 if depth is None else depth
         window_frac = f0 if window_frac is None else window_frac
+        if fallback is not None:
+            self.set_fallback(fallback)
         h = mlp.handle if mlp is not None else None
         check(self._lib.vsdmo_opt_set_proposal(self._h, h, int(mode), int(depth), float(window_frac)),
               "vsdmo_opt_set_proposal")
+
+    FALLBACK = {"immediate": 0, "deferred": 1}
+
+    def set_fallback(self, policy: str):
+        """Fallback policy of the learned proposal ("immediate" / "deferred");
+        clears pending deferred searches."""
+        if policy not in self.FALLBACK:
+            raise ValueError(f"fallback must be one of {sorted(self.FALLBACK)}")
+        check(self._lib.vsdmo_opt_set_fallback(self._h, self.FALLBACK[policy]), "vsdmo_opt_set_fallback")
 
     def record(self, dataset, n: int):
         """n plain iterations, recording one training pair per inner search."""

@@ -112,6 +112,7 @@ EXPORTS = {
     "vsdmo_opt_record": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "vsdmo_opt_surface_points": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp]),
     "vsdmo_opt_set_proposal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float]),
+    "vsdmo_opt_set_fallback": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsdmo_batch_params_default": (None, [C.POINTER(BatchParams)]),
     "vsdmo_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(BatchParams),
                                      C.POINTER(C.c_void_p)]),

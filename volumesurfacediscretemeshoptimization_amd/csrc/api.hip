@@ -1,6 +1,7 @@
 // api.hip -- C ABI of libvsdmo (declarations and reference citations in
 // include/vsdmo.h).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -109,6 +110,11 @@ struct vsdmo_opt {
     int learn_mode = 0, learn_depth = 4;
     float learn_frac = 0.25f;
     bool learn_separate = false;     // network in its own launch even where the window kernel could run it
+    int learn_fallback = VSDMO_FALLBACK_IMMEDIATE;
+    // deferred fallback: per-class lists of two iterations (class c at its
+    // offset in the inner range, buffer it & 1 written in iteration it), their
+    // lengths, and the iteration that deferred each vertex
+    DevBuf<int> dList, dCnt, dEpoch;
     DevBuf<float> lL, lY;
     DevBuf<int> lFallback;           // running count of fallbacks to the full search
     DevBuf<int> lList, lCnt;         // fallback list of a class; per-class list lengths
@@ -378,6 +384,22 @@ static void opt_quality(vsdmo_opt* o)
     o->last_sum = sm;
 }
 
+// empty deferred lists and no deferral epochs (allocated on first use)
+static void reset_deferred(vsdmo_opt* o)
+{
+    vsdmo_mesh* m = o->m;
+    if (o->learn_fallback != VSDMO_FALLBACK_DEFERRED || m->colOffInner.size() < 2) return;
+    const int nCls = (int)m->colOffInner.size() - 1;
+    const int nIn = m->colOffInner.back() - m->colOffInner.front();
+    if (!o->dList.p) {
+        o->dList.alloc((size_t)2 * std::max(nIn, 1));
+        o->dCnt.alloc((size_t)2 * nCls);
+        o->dEpoch.alloc(std::max(m->nV, 1));
+    }
+    launch_fill_int(o->dCnt.p, 2 * nCls, 0, m->stream);
+    launch_fill_int(o->dEpoch.p, std::max(m->nV, 1), INT_MIN, m->stream);
+}
+
 static SurfArgs surf_args(vsdmo_opt* o)
 {
     vsdmo_mesh* m = o->m;
@@ -559,7 +581,12 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
             if (m->elem == VSDMO_TET || m->elem == VSDMO_HEX) {
                 size_t lds = search_lds_bytes(m->elem, m->maxRing);
                 // the classes' fallback-list lengths start at zero (one fill per iteration)
-                if (o->learn_mode != 0 && !o->rec)
+                const int nCls = (int)m->colOffInner.size() - 1;
+                const bool deferred = o->learn_mode != 0 && !o->rec && o->learn_fallback == VSDMO_FALLBACK_DEFERRED;
+                const int par = (int)(o->curr_it & 1);
+                const int nIn = m->colOffInner.empty() ? 0 : m->colOffInner.back() - m->colOffInner.front();
+                if (deferred) launch_fill_int(o->dCnt.p + (size_t)par * std::max(nCls, 1), std::max(nCls, 0), 0, m->stream);
+                else if (o->learn_mode != 0 && !o->rec)
                     launch_fill_int(o->lCnt.p, (int)m->colOffInner.size(), 0, m->stream);
                 for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c) {
                     a.cStart = m->colOffInner[c]; a.cEnd = m->colOffInner[c + 1];
@@ -580,11 +607,23 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                                                  o->lY.p, o->lL.p, m->stream);
                             }
                         }
-                        timed(o, [&]() {
-                            launch_search_learned(m->elem, o->p.nq, a, net, o->learn_mode, o->learn_depth,
-                                                  o->learn_frac, o->lFallback.p, o->lList.p, o->lCnt.p + c, m->maxRing,
-                                                  m->stream);
-                        });
+                        if (deferred) {
+                            const size_t off = (size_t)(a.cStart - m->colOffInner.front());
+                            const size_t cs = (size_t)std::max(nCls, 1);
+                            LearnDefer def{ o->dList.p + (size_t)(1 - par) * nIn + off, o->dCnt.p + (1 - par) * cs + c,
+                                            o->dEpoch.p, (int)o->curr_it };
+                            timed(o, [&]() {
+                                launch_search_learned(m->elem, o->p.nq, a, net, o->learn_mode, o->learn_depth,
+                                                      o->learn_frac, o->lFallback.p, o->dList.p + (size_t)par * nIn + off,
+                                                      o->dCnt.p + par * cs + c, m->maxRing, &def, m->stream);
+                            });
+                        } else {
+                            timed(o, [&]() {
+                                launch_search_learned(m->elem, o->p.nq, a, net, o->learn_mode, o->learn_depth,
+                                                      o->learn_frac, o->lFallback.p, o->lList.p, o->lCnt.p + c,
+                                                      m->maxRing, nullptr, m->stream);
+                            });
+                        }
                         o->s_learned += nc;
                     } else {
                         timed(o, [&]() { dispatch_search(o, m->elem, a, lds, m->stream); });
@@ -718,6 +757,21 @@ vsdmo_status vsdmo_opt_element_qualities(const vsdmo_opt* o, float* host_out)
     return copy_dev(o->q, (size_t)o->m->nE, host_out, o->m->stream);
 }
 
+vsdmo_status vsdmo_opt_set_fallback(vsdmo_opt* o, int32_t policy)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
+        if (policy != VSDMO_FALLBACK_IMMEDIATE && policy != VSDMO_FALLBACK_DEFERRED)
+            return fail(VSDMO_INVALID_ARG, "policy must be VSDMO_FALLBACK_IMMEDIATE or VSDMO_FALLBACK_DEFERRED");
+        if (policy == VSDMO_FALLBACK_DEFERRED && !learned_deferred_ok(o->p.nq))
+            return fail(VSDMO_NOT_SUPPORTED, "the deferred fallback needs nq 4 or 8");
+        o->learn_fallback = policy;
+        reset_deferred(o);
+        VSDMO_HIP(hipStreamSynchronize(o->m->stream));
+        return VSDMO_OK;
+    });
+}
+
 vsdmo_status vsdmo_opt_stats(const vsdmo_opt* o, vsdmo_stats* out)
 {
     if (!o || !out) return fail(VSDMO_INVALID_ARG, "NULL argument");
@@ -791,6 +845,7 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
             o->lL.alloc(maxc); o->lY.alloc((size_t)3 * maxc); o->lList.alloc(maxc);
         }
         if (!o->lCnt.p) o->lCnt.alloc(std::max<size_t>(m->colOffInner.size(), 1));
+        reset_deferred(o);
         if (mode == 1) {
             build_neighbour_table(*m);
             VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final

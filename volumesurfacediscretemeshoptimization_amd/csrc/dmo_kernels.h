@@ -138,15 +138,28 @@ struct LearnNet {
     const float* w = nullptr;
     const int* nbr = nullptr;
 };
+// deferred fallback of one class pass: the previous iteration's list of the
+// class (listIn[0 .. *cntIn)), the per-vertex deferral epochs, the iteration
+struct LearnDefer {
+    const int* listIn;
+    const int* cntIn;
+    int* epoch;
+    int it;
+};
 // learned-proposal search of one colour class [a.cStart, a.cEnd) (tet / hex):
 // the proposal window (4 vertices per wave) appends the vertices whose
-// window finds nothing better than their start to list, then a second
-// launch runs their plain search; *cnt must be zero on entry; *fallbacks
-// counts the fallbacks; ringCap >= the largest one-ring
+// window finds nothing better than their start to list (*cnt zero on entry);
+// def null: a second launch runs their full search now; def set: they keep
+// their position and take the full search in the class's next pass, which
+// runs def->listIn beside the windows.  *fallbacks counts the full searches
+// run; ringCap >= the largest one-ring
 void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet& net, int mode, int depthL,
-                           float frac, int* fallbacks, int* list, int* cnt, int ringCap, hipStream_t s);
+                           float frac, int* fallbacks, int* list, int* cnt, int ringCap, const LearnDefer* def,
+                           hipStream_t s);
 // largest one-ring (elements) the learned search supports
 int learned_ring_cap(int kind);
+// whether the deferred fallback policy runs with this candidate grid
+bool learned_deferred_ok(int nq);
 // whether the window kernel can run the network itself for this ring size
 bool learned_fused_ok(int kind, int ringCap);
 void launch_batch_quality(const BatchQArgs& a, hipStream_t s);

@@ -577,6 +577,14 @@ struct LearnArgs {
     const uint16_t* wb;
     const float* w;
     const int* nbr;
+    // deferred fallback (epoch non-null): the vertices appended to list in
+    // iteration `it` take the full search in the class's next pass, from
+    // listIn / cntIn (the previous iteration's list); epoch[vid] = the
+    // iteration that deferred vid, whose window the next pass skips
+    const int* listIn;
+    const int* cntIn;
+    int* epoch;
+    int it;
 };
 
 // The proposal window runs 4 vertices per wave, 16 lanes each: lane s =
@@ -676,20 +684,23 @@ __device__ __forceinline__ void fallback_append(const LearnArgs& la, bool need, 
     if (need) la.list[base + __popcll(bal & ((1ull << lane) - 1ull))] = vid;
 }
 
-// one block: LV vertices on one wave, or (FUSED) 4 waves of LV = 4 vertices
-// that first run their 16-vertex network tile
-template <int NQ, int LV, bool FUSED>
-__global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_tet_learned(SearchArgs a, LearnArgs la, int ringCap)
+// The window search of the WPB * LV class vertices of block blk: LV
+// vertices per wave on WPB waves; with FUSED (WPB = 4, LV = 4) the block
+// first runs their 16-vertex network tile.  lds: WPB * LV rings (after the
+// network's LDS).
+template <int NQ, int LV, int WPB, bool FUSED>
+__device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap, int blk, unsigned char* s_tcl)
 {
-    extern __shared__ __align__(16) unsigned char s_tcl[];   // VPB x ringCap TetC (FUSED: first the network)
     constexpr int NL = 4, G = 64 / LV, CPL = NL * NL * NL / G;   // lanes per vertex, candidates per lane
-    constexpr int VPB = FUSED ? 4 * LV : LV;
+    constexpr int VPB = WPB * LV;
     static_assert(!FUSED || VPB == kMlpRows, "one network tile per block");
     const int s = threadIdx.x % G, gb = threadIdx.x / G;
-    const int nc = a.cEnd - a.cStart, ci0 = VPB * blockIdx.x;
+    const int nc = a.cEnd - a.cStart, ci0 = VPB * blk;
     // a group past the end of the class mirrors the class's last vertex and writes nothing
     const int ci = min(ci0 + gb, nc - 1);
     const int vid = a.cStart + ci;
+    // a vertex deferred by the previous iteration takes the full search in this launch instead
+    const bool own = ci0 + gb < nc && !(la.epoch && la.epoch[vid] == la.it - 1);
     float pl, py0, py1, py2;
     learned_offset<FUSED>(a, la, ci0, gb, ci, s_tcl, pl, py0, py1, py2);
     TetC* tc = reinterpret_cast<TetC*>(s_tcl) + (size_t)gb * ringCap;
@@ -765,21 +776,23 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_tet_learned(SearchA
         }
         ds = ds * (2.0f / (float)(NL - 1));
     }
-    if (moved && s == 0 && ci0 + gb < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
-    fallback_append(la, !moved && !tie && s == 0 && ci0 + gb < nc, vid);
+    if (moved && own && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    const bool need = !moved && !tie && own && s == 0;
+    fallback_append(la, need, vid);
+    if (need && la.epoch) la.epoch[vid] = la.it;
 }
 
-template <int NQ, int LV, bool FUSED>
-__global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchArgs a, LearnArgs la, int ringCap)
+template <int NQ, int LV, int WPB, bool FUSED>
+__device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap, int blk, unsigned char* s_hxl)
 {
-    extern __shared__ __align__(16) unsigned char s_hxl[];   // VPB x ringCap HexLds (FUSED: first the network)
     constexpr int NL = 4, G = 64 / LV, CPL = NL * NL * NL / G;
-    constexpr int VPB = FUSED ? 4 * LV : LV;
+    constexpr int VPB = WPB * LV;
     static_assert(!FUSED || VPB == kMlpRows, "one network tile per block");
     const int s = threadIdx.x % G, gb = threadIdx.x / G;
-    const int nc = a.cEnd - a.cStart, ci0 = VPB * blockIdx.x;
+    const int nc = a.cEnd - a.cStart, ci0 = VPB * blk;
     const int ci = min(ci0 + gb, nc - 1);
     const int vid = a.cStart + ci;
+    const bool own = ci0 + gb < nc && !(la.epoch && la.epoch[vid] == la.it - 1);
     float pl, py0, py1, py2;
     learned_offset<FUSED>(a, la, ci0, gb, ci, s_hxl, pl, py0, py1, py2);
     HexLds* hx = reinterpret_cast<HexLds*>(s_hxl) + (size_t)gb * ringCap;
@@ -865,8 +878,64 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchA
         }
         ds = ds * (2.0f / (float)(NL - 1));
     }
-    if (moved && s == 0 && ci0 + gb < nc) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
-    fallback_append(la, !moved && !tie && !hex_pinned(f) && s == 0 && ci0 + gb < nc, vid);
+    if (moved && own && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
+    const bool need = !moved && !tie && !hex_pinned(f) && own && s == 0;
+    fallback_append(la, need, vid);
+    if (need && la.epoch) la.epoch[vid] = la.it;
+}
+
+// immediate fallback: one block = LV vertices on one wave, or (FUSED) 4 waves
+// of LV = 4 vertices that first run their network tile; the fallbacks go to a
+// second launch (k_search_list_*)
+template <int NQ, int LV, bool FUSED>
+__global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_tet_learned(SearchArgs a, LearnArgs la, int ringCap)
+{
+    extern __shared__ __align__(16) unsigned char s_tcl[];
+    tet_window<NQ, LV, FUSED ? 4 : 1, FUSED>(a, la, ringCap, blockIdx.x, s_tcl);
+}
+template <int NQ, int LV, bool FUSED>
+__global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchArgs a, LearnArgs la, int ringCap)
+{
+    extern __shared__ __align__(16) unsigned char s_hxl[];
+    hex_window<NQ, LV, FUSED ? 4 : 1, FUSED>(a, la, ringCap, blockIdx.x, s_hxl);
+}
+
+// deferred fallback: blocks [0, nbList) run the full search (4 waves per
+// vertex) of the vertices the previous iteration deferred, blocks from
+// nbList on the windows of 16 class vertices each (the vertices deferred
+// last iteration excepted); both are vertices of one colour class, so the
+// two kinds run side by side.
+template <int NQ, bool FUSED>
+__global__ __launch_bounds__(256) void k_search_tet_learned_def(SearchArgs a, LearnArgs la, int ringCap, int nbList)
+{
+    extern __shared__ __align__(16) unsigned char s_tcd[];
+    __shared__ RedLds<4> rd;
+    if ((int)blockIdx.x < nbList) {
+        const int n = *la.cntIn;
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
+        for (int i = blockIdx.x; i < n; i += nbList) {
+            tet_vertex<NQ, 4>(a, la.listIn[i], a.extentMin != 0, reinterpret_cast<TetC*>(s_tcd), rd);
+            __syncthreads();   // the next vertex restages the LDS ring
+        }
+        return;
+    }
+    tet_window<NQ, 4, 4, FUSED>(a, la, ringCap, (int)blockIdx.x - nbList, s_tcd);
+}
+template <int NQ, bool FUSED>
+__global__ __launch_bounds__(256) void k_search_hex_learned_def(SearchArgs a, LearnArgs la, int ringCap, int nbList)
+{
+    extern __shared__ __align__(16) unsigned char s_hxd[];
+    __shared__ RedLds<4> rd;
+    if ((int)blockIdx.x < nbList) {
+        const int n = *la.cntIn;
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
+        for (int i = blockIdx.x; i < n; i += nbList) {
+            hex_vertex<NQ, 4>(a, la.listIn[i], a.extentMin != 0, reinterpret_cast<HexLds*>(s_hxd), rd);
+            __syncthreads();
+        }
+        return;
+    }
+    hex_window<NQ, 4, 4, FUSED>(a, la, ringCap, (int)blockIdx.x - nbList, s_hxd);
 }
 
 // plain search of the vertices list[0 .. *cnt) (the fallbacks of a class),
@@ -920,24 +989,55 @@ static size_t learned_per(int kind) { return (kind == VSDMO_TET) ? sizeof(TetC) 
 // largest ring the window kernel's LDS (kLV rings per block, 64 KB) holds
 int learned_ring_cap(int kind) { return (int)((64 * 1024) / (kLV * learned_per(kind))); }
 
+// the deferred policy's full searches run 4 waves per vertex
+bool learned_deferred_ok(int nq) { return nq == 4 || nq == 8; }
+
 // the fused kernel holds 16 rings per block in the same 64 KB
 bool learned_fused_ok(int kind, int ringCap)
 {
     return (size_t)kMlpRows * std::max(ringCap, 1) * learned_per(kind) <= 64 * 1024;
 }
 
+template <int NQ, bool FUSED>
+static void learned_def_nq(int kind, const SearchArgs& a, const LearnArgs& la, int ringCap, int nbList, int nb,
+                           size_t lds, hipStream_t s)
+{
+    if (kind == VSDMO_TET) k_search_tet_learned_def<NQ, FUSED><<<nbList + nb, 256, lds, s>>>(a, la, ringCap, nbList);
+    else k_search_hex_learned_def<NQ, FUSED><<<nbList + nb, 256, lds, s>>>(a, la, ringCap, nbList);
+}
+
 void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet& net, int mode, int depthL,
-                           float frac, int* fallbacks, int* list, int* cnt, int ringCap, hipStream_t s)
+                           float frac, int* fallbacks, int* list, int* cnt, int ringCap, const LearnDefer* def,
+                           hipStream_t s)
 {
     const int nc = a.cEnd - a.cStart;
     if (nc <= 0) return;
     ringCap = std::max(ringCap, 1);
     const size_t per = learned_per(kind);
     const bool fused = mode == 1 && net.wb && learned_fused_ok(kind, ringCap);
+    LearnArgs la{ net.Y, net.L, mode, depthL, frac, fallbacks, list, cnt, net.wb, net.w, net.nbr,
+                  nullptr, nullptr, nullptr, 0 };
+    if (def) {
+        // one launch: the previous iteration's fallbacks (list blocks sized for
+        // an eighth of the class; they loop when more were deferred) beside
+        // this iteration's windows, 16 vertices per block
+        la.listIn = def->listIn; la.cntIn = def->cntIn; la.epoch = def->epoch; la.it = def->it;
+        const int nbList = std::max(1, std::min(nc, std::max(64, nc / 8)));
+        const int nb = (nc + kMlpRows - 1) / kMlpRows;
+        size_t lds = (size_t)kMlpRows * ringCap * per;
+        if (fused) lds = std::max(lds, sizeof(MlpLds));
+        switch (nq) {   // the full searches use 4 waves: nq 4 and 8 (learned_deferred_ok)
+        case 4: fused ? learned_def_nq<4, true>(kind, a, la, ringCap, nbList, nb, lds, s)
+                      : learned_def_nq<4, false>(kind, a, la, ringCap, nbList, nb, lds, s); break;
+        case 8: fused ? learned_def_nq<8, true>(kind, a, la, ringCap, nbList, nb, lds, s)
+                      : learned_def_nq<8, false>(kind, a, la, ringCap, nbList, nb, lds, s); break;
+        default: break;
+        }
+        return;
+    }
     const int vpb = fused ? kMlpRows : kLV;
     const int nb = (nc + vpb - 1) / vpb;
     const size_t lds = fused ? std::max(sizeof(MlpLds), (size_t)kMlpRows * ringCap * per) : (size_t)kLV * ringCap * per;
-    LearnArgs la{ net.Y, net.L, mode, depthL, frac, fallbacks, list, cnt, net.wb, net.w, net.nbr };
 #define VSDMO_LEARN_CASE(Q)                                                                           \
     case Q:                                                                                          \
         if (fused) learned_nq_lv<Q, kLV, true>(kind, a, la, ringCap, nb, lds, s);                     \
