@@ -326,8 +326,8 @@ class _DMOBase:
         return out
 
     # default proposal windows (levels, fraction of the plain first-level
-    # window), chosen on a 40^3 tet cube and a 48^3 hex cube
-    LEARN_WINDOW = {_lib.VSDMO_TET: (2, 0.045), _lib.VSDMO_HEX: (4, 0.5)}
+    # window), chosen with the deferred fallback on 40^3 tet and 48^3 hex cubes
+    LEARN_WINDOW = {_lib.VSDMO_TET: (1, 0.025), _lib.VSDMO_HEX: (1, 2.0)}
 
     def set_proposal(self, mlp=None, mode: int = 1, depth: int = None, window_frac: float = None,
                      fallback: str = None):
@@ -339,7 +339,7 @@ class _DMOBase:
         nothing better runs in the same class pass) or "deferred" (it runs in
         the class's next pass, beside that pass's windows); None keeps the
         current policy (initially immediate)."""
-        d0, f0 = self.LEARN_WINDOW.get(self.mesh.elem, (2, 0.045))
+        d0, f0 = self.LEARN_WINDOW.get(self.mesh.elem, (1, 0.025))
         depth = d0 if depth is None else depth
         window_frac = f0 if window_frac is None else window_frac
         if fallback is not None:
