@@ -76,6 +76,19 @@ static void destroy_mesh(vsdmo_mesh* m)
 
 using namespace vsdmo;
 
+// 16 bytes of pinned host memory for the per-iteration quality readback
+struct PinnedQ {
+    double* p = nullptr;
+    PinnedQ() = default;
+    PinnedQ(const PinnedQ&) = delete;
+    PinnedQ& operator=(const PinnedQ&) = delete;
+    ~PinnedQ() { if (p) (void)hipHostFree(p); }
+    void alloc()
+    {
+        if (!p) VSDMO_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), 2 * sizeof(double), hipHostMallocDefault));
+    }
+};
+
 struct vsdmo_opt {
     vsdmo_mesh* m = nullptr;
     int criterion = 0;
@@ -91,8 +104,10 @@ struct vsdmo_opt {
     bool timing = false;
     double kernel_ms = 0.0;
     int64_t launches = 0;
-    DevBuf<float> q, pmin, omin;
-    DevBuf<double> psum, osum;
+    DevBuf<float> q, pmin;
+    DevBuf<double> psum;
+    DevBuf<double> qout;   // [0]: minimum (float in the first 4 bytes), [1]: FP64 sum
+    PinnedQ hq;            // their host copy
     std::vector<hipEvent_t> ev;   // pool of events; pairs [0, ev_used) bracket timed launches
     size_t ev_used = 0;
     // surface-vertex state (fixed_boundary == 0)
@@ -373,15 +388,15 @@ static void opt_quality(vsdmo_opt* o)
 {
     vsdmo_mesh* m = o->m;
     QualityArgs qa{ m->x.p, m->y.p, m->z.p, m->cells.p, m->nE, m->npe, o->qelem, o->q.p };
-    launch_quality(qa, o->pmin.p, o->psum.p, o->omin.p, o->osum.p, m->stream);
+    launch_quality(qa, o->pmin.p, o->psum.p, reinterpret_cast<float*>(o->qout.p), o->qout.p + 1, m->stream);
     VSDMO_HIP(hipGetLastError());
-    float mn = 0.0f;
-    double sm = 0.0;
-    VSDMO_HIP(hipMemcpyAsync(&mn, o->omin.p, sizeof(float), hipMemcpyDeviceToHost, m->stream));
-    VSDMO_HIP(hipMemcpyAsync(&sm, o->osum.p, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    // one 16-byte copy into pinned memory (no staging, one synchronisation)
+    VSDMO_HIP(hipMemcpyAsync(o->hq.p, o->qout.p, 2 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
     VSDMO_HIP(hipStreamSynchronize(m->stream));
+    float mn = 0.0f;
+    std::memcpy(&mn, o->hq.p, sizeof(float));
     o->last_min = mn;
-    o->last_sum = sm;
+    o->last_sum = o->hq.p[1];
 }
 
 // empty deferred lists and no deferral epochs (allocated on first use)
@@ -504,7 +519,8 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
             int nb = (m->nE + 255) / 256;
             o->q.alloc(m->nE);
             o->pmin.alloc(nb); o->psum.alloc(nb);
-            o->omin.alloc(1); o->osum.alloc(1);
+            o->qout.alloc(2);
+            o->hq.alloc();
             if (!p.fixed_boundary || surfMesh) surface_init(o);
             opt_quality(o);
         } catch (...) {

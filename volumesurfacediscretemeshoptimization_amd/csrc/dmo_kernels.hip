@@ -1243,11 +1243,32 @@ __device__ float elem_quality(const float* __restrict__ X, const float* __restri
     return tri_acc_score(acc, (1.0f * isg) * isg);
 }
 
-__global__ void k_elem_quality(QualityArgs a)
+// the qualities of the block's 256 elements, written to q, and their min and
+// FP64 sum in a fixed tree order (per-block partials for k_reduce_final)
+__global__ __launch_bounds__(256) void k_elem_quality(QualityArgs a, float* __restrict__ pmin,
+                                                      double* __restrict__ psum)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.nE) return;
-    a.q[e] = elem_quality(a.x, a.y, a.z, a.cells + (size_t)e * a.npe, a.elem);
+    __shared__ float smin[256];
+    __shared__ double ssum[256];
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    float v = FLT_MAX;
+    double sv = 0.0;
+    if (e < a.nE) {
+        v = elem_quality(a.x, a.y, a.z, a.cells + (size_t)e * a.npe, a.elem);
+        a.q[e] = v;
+        sv = (double)v;
+    }
+    smin[threadIdx.x] = v;
+    ssum[threadIdx.x] = sv;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + k]);
+            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { pmin[blockIdx.x] = smin[0]; psum[blockIdx.x] = ssum[0]; }
 }
 
 // batch qualities: one block of 256 per chunk of at most 256 elements of one
@@ -1315,26 +1336,6 @@ __global__ void k_batch_mesh_reduce(BatchQArgs a)
 __global__ void k_batch_history_next(int* hctr)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *hctr = *hctr + 1;
-}
-
-// per-block min and FP64 sum in a fixed tree order
-__global__ void k_reduce_q(const float* __restrict__ q, int n, float* __restrict__ pmin, double* __restrict__ psum)
-{
-    __shared__ float smin[256];
-    __shared__ double ssum[256];
-    int i = blockIdx.x * 256 + threadIdx.x;
-    float v = (i < n) ? q[i] : FLT_MAX;
-    smin[threadIdx.x] = v;
-    ssum[threadIdx.x] = (i < n) ? (double)v : 0.0;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + s]);
-            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { pmin[blockIdx.x] = smin[0]; psum[blockIdx.x] = ssum[0]; }
 }
 
 // one block of 256 threads: strided partial sums then a fixed tree (the order
@@ -1537,8 +1538,7 @@ void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hip
 void launch_quality(const QualityArgs& a, float* pmin, double* psum, float* omin, double* osum, hipStream_t s)
 {
     int nb = (a.nE + 255) / 256;
-    k_elem_quality<<<nb, 256, 0, s>>>(a);
-    k_reduce_q<<<nb, 256, 0, s>>>(a.q, a.nE, pmin, psum);
+    k_elem_quality<<<nb, 256, 0, s>>>(a, pmin, psum);
     k_reduce_final<<<1, 256, 0, s>>>(pmin, psum, nb, omin, osum);
 }
 
