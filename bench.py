@@ -336,10 +336,16 @@ def _learned(args):
 
     its, dt_learned, stl, q_l, qm_l = to_quality(args.learn_fallback)
     other = "immediate" if args.learn_fallback == "deferred" else "deferred"
-    its_o, dt_o, st_o, q_o, qm_o = to_quality(other)
-    reached_o = q_o >= q_plain - 1e-3 and qm_o >= qm_plain - 1e-3
-    its_s, dt_s, st_s, q_s, qm_s = to_quality(args.learn_fallback, *SPEC_WINDOW)
-    reached_s = q_s >= q_plain - 1e-3 and qm_s >= qm_plain - 1e-3
+    extras = {}
+    if args.learned_extras:
+        for key, (pol, depth, frac) in ((f"{other}_fallback", (other, args.learn_depth, args.learn_frac)),
+                                        ("specified_window", (args.learn_fallback,) + SPEC_WINDOW)):
+            its_x, dt_x, st_x, q_x, qm_x = to_quality(pol, depth, frac)
+            ok_x = q_x >= q_plain - 1e-3 and qm_x >= qm_plain - 1e-3
+            extras[key] = {"fallback": pol, "levels": depth, "window_frac": frac, "learned_iterations": its_x,
+                           "learned_ms_per_step": 1e3 * dt_x / its_x, "reached_plain_quality": bool(ok_x),
+                           "speedup_vs_plain": (plain_updates / dt_x) / (plain_updates / dt_plain) if ok_x else 0.0,
+                           "fallback_rate": st_x["fallbacks"] / max(1, st_x["searches_learned"])}
     # the same run again with HIP events around every launch, for the kernel time
     mk = fac.create(xyz, cells)
     ok = cls(mk, n_iter=its, **kw)
@@ -374,15 +380,7 @@ def _learned(args):
         "mean_quality_plain": qm_plain, "mean_quality_learned": qm_l,
         "fallback_policy": args.learn_fallback,
         "fallback_rate": stl["fallbacks"] / max(1, stl["searches_learned"]),
-        f"{other}_fallback": {"learned_iterations": its_o, "learned_ms_per_step": 1e3 * dt_o / its_o,
-                              "reached_plain_quality": bool(reached_o),
-                              "speedup_vs_plain": (plain_updates / dt_o) / (plain_updates / dt_plain) if reached_o else 0.0,
-                              "fallback_rate": st_o["fallbacks"] / max(1, st_o["searches_learned"])},
-        "specified_window": {"levels": SPEC_WINDOW[0], "window_frac": SPEC_WINDOW[1],
-                             "learned_iterations": its_s, "learned_ms_per_step": 1e3 * dt_s / its_s,
-                             "reached_plain_quality": bool(reached_s),
-                             "speedup_vs_plain": (plain_updates / dt_s) / (plain_updates / dt_plain) if reached_s else 0.0,
-                             "fallback_rate": st_s["fallbacks"] / max(1, st_s["searches_learned"])},
+        **extras,
         "training_pairs": n_pairs, "training_meshes": s, "record_s": rec_s, "train_s": train_s,
         "epochs": args.epochs, "loss_first_epoch": float(losses[0]) if len(losses) else None,
         "loss_last_epoch": float(losses[-1]) if len(losses) else None,
@@ -420,6 +418,8 @@ def main(argv=None):
     ap.add_argument("--learn-frac", type=float, default=None, help="default: 0.025 (tet), 2.0 (hex)")
     ap.add_argument("--learn-fallback", choices=["deferred", "immediate"], default="deferred",
                     help="fallback policy of the measured learned run (the other one is reported beside it)")
+    ap.add_argument("--learned-extras", type=int, default=1,
+                    help="1: also run the other fallback policy and the specified 1/4 window (0: skip them)")
     ap.add_argument("--tune-log", default=os.environ.get("VSDMO_TUNE_LOG", "autotune_batch256.jsonl"))
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
