@@ -197,8 +197,9 @@ def test_learned_reaches_plain_quality_and_ablation(kind):
     assert res[1][0] < res[2][0] or (res[1][0] == res[2][0] and res[1][2] < res[2][2]), res
 
 
+@pytest.mark.parametrize("fallback", ["immediate", "deferred"])
 @pytest.mark.parametrize("kind", ["tet", "hex"])
-def test_fused_network_equals_separate_launch(kind, monkeypatch):
+def test_fused_network_equals_separate_launch(kind, fallback, monkeypatch):
     """The window kernel's own run of the network (16-vertex tiles, partial
     last tile per class) gives the positions, fallback count and quality of
     the separate inference launch bit for bit."""
@@ -217,7 +218,7 @@ def test_fused_network_equals_separate_launch(kind, monkeypatch):
         monkeypatch.setenv("VSDMO_LEARN_SEPARATE_INFERENCE", sep)
         m = fac.create(xyz, cells)
         o = cls(m, n_iter=3, fixed_boundary=True)
-        o.set_proposal(mlp, mode=1)
+        o.set_proposal(mlp, mode=1, fallback=fallback)
         o.iterate(3)
         st = o.stats()
         out.append((m.positions(original_order=False), st["fallbacks"], o.getQuality()))
@@ -328,3 +329,24 @@ def test_default_window_deferred_reaches_plain_quality(kind):
     assert all(b >= a for a, b in zip(q, q[1:])), q
     for x in (o, m, op, mp, mlp):
         x.close()
+
+
+def test_deferred_with_surface_passes():
+    """A valid hex mesh with the surface free: the learned inner passes
+    (deferred fallback) are followed by the surface and feature passes every
+    iteration; the minimum quality never decreases and the surface searches
+    run."""
+    xyz, hexes = synth.hex_cube(10, seed=21, jitter=0.15)
+    m = DMOMeshHexFactory.create(xyz, hexes)
+    o = DMOHexClass(m, n_iter=4, fixed_boundary=False)
+    assert o.getQuality() > 0
+    o.set_proposal(None, mode=2, fallback="deferred")
+    q = [o.getQuality()]
+    for _ in range(4):
+        o.doIteration()
+        q.append(o.getQuality())
+    assert all(b >= a for a, b in zip(q, q[1:])), q
+    st = o.stats()
+    assert st["searches_surface"] > 0 and st["searches_learned"] == 4 * m.counts()["n_inner"]
+    o.close()
+    m.close()
