@@ -556,9 +556,11 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v)
 // the window shrunk by 2/3 per level; the centre moves to a level's best
 // candidate when it is strictly better than the best quality so far, which
 // starts at q(v).  Fallback rule: when the window's best candidate is lower
-// than q(v), the vertex goes on the class's fallback list, whose plain full
-// search runs as a second launch; the result is the plain path's, bit for
-// bit.  A tie at a positive quality (a level's best equals q(v) > 0) leaves
+// than q(v), the vertex goes on the class's fallback list.  Its plain full
+// search runs in a second launch before the next class (immediate policy:
+// the update is the plain path's, bit for bit) or in the class's next pass,
+// beside that pass's windows (deferred policy: the vertex keeps its position
+// until then).  A tie at a positive quality (a level's best equals q(v) > 0) leaves
 // the vertex where it is: it is the mark of a ring whose minimum is held by
 // sub-tets the vertex is not part of (hexes), which no position raises.  At
 // quality 0 a tie can also be the vertex's own inverted sub-tet, which the
