@@ -350,3 +350,36 @@ def test_deferred_with_surface_passes():
     assert st["searches_surface"] > 0 and st["searches_learned"] == 4 * m.counts()["n_inner"]
     o.close()
     m.close()
+
+
+def test_deferred_lists_dropped_by_recording():
+    """Deferred learned iterations, a recorded (plain) iteration, then deferred
+    iterations again: the recording drops the pending deferrals, so no vertex
+    is searched twice at once -- the run is reproducible bit for bit and its
+    minimum quality never decreases."""
+    xyz, tets = synth.tet_cube(12, seed=17)
+    mlp = DMOMLP(seed=3)
+    p = mlp.params()
+    p[-3:] = [0.04, -0.02, 0.03]
+    mlp.set_params(p)
+    runs = []
+    for _ in range(2):
+        m = DMOMeshTetFactory.create(xyz, tets)
+        o = DMOTetClass(m, n_iter=10, fixed_boundary=True)
+        o.set_proposal(mlp, mode=1, fallback="deferred")
+        q = [o.getQuality()]
+        for k in range(5):
+            if k == 2:
+                ds = DMODataset(100000)
+                o.record(ds, 1)
+                ds.close()
+            else:
+                o.doIteration()
+            q.append(o.getQuality())
+        assert all(b >= a for a, b in zip(q, q[1:])), q
+        assert o.stats()["fallbacks"] > 0
+        runs.append(m.positions(original_order=False))
+        o.close()
+        m.close()
+    assert np.array_equal(runs[0], runs[1])
+    mlp.close()

@@ -130,6 +130,7 @@ struct vsdmo_opt {
     // offset in the inner range, buffer it & 1 written in iteration it), their
     // lengths, and the iteration that deferred each vertex
     DevBuf<int> dList, dCnt, dEpoch;
+    bool defer_pending = false;      // the lists hold the last iteration's deferrals
     DevBuf<float> lL, lY;
     DevBuf<int> lFallback;           // running count of fallbacks to the full search
     DevBuf<int> lList, lCnt;         // fallback list of a class; per-class list lengths
@@ -602,6 +603,14 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                 const int par = (int)(o->curr_it & 1);
                 const int nIn = m->colOffInner.empty() ? 0 : m->colOffInner.back() - m->colOffInner.front();
                 if (deferred) launch_fill_int(o->dCnt.p + (size_t)par * std::max(nCls, 1), std::max(nCls, 0), 0, m->stream);
+                // an iteration that does not keep the deferred lists (recording,
+                // learning off) drops them: a later deferred pass must not take a
+                // stale list, whose vertices its windows would not skip
+                if (deferred) o->defer_pending = true;
+                else if (o->defer_pending) {
+                    reset_deferred(o);
+                    o->defer_pending = false;
+                }
                 else if (o->learn_mode != 0 && !o->rec)
                     launch_fill_int(o->lCnt.p, (int)m->colOffInner.size(), 0, m->stream);
                 for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c) {
