@@ -48,7 +48,9 @@ PEAK_FP32_TFLOPS = 157.3
 # (tet 73, hex 407, tri 23), cube roots counted separately.
 # HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE x 1.99981 copy-kernel
 # calibration + WRITE_SIZE x 1.0, mean over the dispatches of 3 iterations,
-# profiles/k_search_tet_tet40.md and profiles/k_search_hex_hex48.md).
+# profiles/k_search_tet_tet40.md, profiles/k_search_hex_hex48.md,
+# profiles/k_search_fan2d_surf.md; the learned pass over the headline run,
+# profiles/learned_tet40.md and profiles/learned_hex48.md).
 # Surface-mesh searches (algorithmic FLOP per vertex update, valence n = 6
 # triangles / 4 quads): per candidate the projection onto the blended local
 # surfaces (57 + 69 n) plus the signed fan metric (59 n triangles, 104 n
@@ -58,14 +60,14 @@ SURF_FLOP = {"tri": 3 * 64 * (57 + 69 * 6 + 59 * 6), "quad": 3 * 64 * (57 + 69 *
 SURF_FLOP1D = {"tri": 3 * 8 * (149 + 59 * 6), "quad": 3 * 8 * (149 + 104 * 4)}
 WORKLOADS = {
     "tet40": {"kind": "tet", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_search_tet<8>",
-              "traffic": 10_970_146},
+              "traffic": 10_970_146, "learned_traffic": 8_042_968},
     "hex48": {"kind": "hex", "flop": 5_014_240, "cbrt": 110_880, "kernel": "k_search_hex<8>",
-              "traffic": 17_570_841},
+              "traffic": 17_570_841, "learned_traffic": 12_274_568},
     "tri64": {"kind": "tri", "flop": 27_048, "cbrt": 0, "kernel": "k_search_flat<8>", "traffic": None},
     "surf_tri": {"kind": "tri", "surface": True, "flop": SURF_FLOP["tri"], "flop1d": SURF_FLOP1D["tri"], "cbrt": 0,
-                 "kernel": "k_search_fan2d<false>", "traffic": None},
+                 "kernel": "k_search_fan2d<false>", "traffic": 18_791_143},
     "surf_quad": {"kind": "quad", "surface": True, "flop": SURF_FLOP["quad"], "flop1d": SURF_FLOP1D["quad"], "cbrt": 0,
-                  "kernel": "k_search_fan2d<true>", "traffic": None},
+                  "kernel": "k_search_fan2d<true>", "traffic": 20_562_741},
     "batch256": {"kind": "batch", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_batch_round", "traffic": None},
 }
 # algorithmic FLOP of one flat-triangle vertex update (8x8 candidates: 196 ring
@@ -385,7 +387,9 @@ def _learned(args):
         "epochs": args.epochs, "loss_first_epoch": float(losses[0]) if len(losses) else None,
         "loss_last_epoch": float(losses[-1]) if len(losses) else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": (WORKLOADS[args.workload].get("learned_traffic")
+                                 if args.learn_fallback == "deferred" else None),
                      "kernel": (f"k_search_{kind}_learned_def<8> (network, proposal windows and the deferred "
                                 "full searches in one launch per class)" if args.learn_fallback == "deferred" else
                                 f"k_search_{kind}_learned<8> + k_search_list_{kind}<8, 4> (windows, then the "
