@@ -383,3 +383,34 @@ def test_deferred_lists_dropped_by_recording():
         m.close()
     assert np.array_equal(runs[0], runs[1])
     mlp.close()
+
+
+def test_fallback_policy_switch_mid_run():
+    """Deferred -> immediate -> deferred on one optimiser: every switch starts
+    from clean lists (the immediate policy's list lengths are zeroed, pending
+    deferrals are dropped); the minimum quality never decreases and the run
+    is reproducible bit for bit."""
+    xyz, tets = synth.tet_cube(12, seed=19)
+    mlp = DMOMLP(seed=4)
+    p = mlp.params()
+    p[-3:] = [0.03, 0.02, -0.04]
+    mlp.set_params(p)
+    runs = []
+    for _ in range(2):
+        m = DMOMeshTetFactory.create(xyz, tets)
+        o = DMOTetClass(m, n_iter=12, fixed_boundary=True)
+        o.set_proposal(mlp, mode=1, fallback="deferred")
+        q = [o.getQuality()]
+        for k, pol in enumerate(["deferred", "deferred", "immediate", "immediate", "deferred", "deferred"]):
+            if k in (2, 4):
+                o.set_fallback(pol)
+            o.doIteration()
+            q.append(o.getQuality())
+        assert all(b >= a for a, b in zip(q, q[1:])), q
+        st = o.stats()
+        assert 0 < st["fallbacks"] < st["searches_learned"]
+        runs.append(m.positions(original_order=False))
+        o.close()
+        m.close()
+    assert np.array_equal(runs[0], runs[1])
+    mlp.close()

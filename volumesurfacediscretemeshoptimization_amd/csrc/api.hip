@@ -603,16 +603,17 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                 const int par = (int)(o->curr_it & 1);
                 const int nIn = m->colOffInner.empty() ? 0 : m->colOffInner.back() - m->colOffInner.front();
                 if (deferred) launch_fill_int(o->dCnt.p + (size_t)par * std::max(nCls, 1), std::max(nCls, 0), 0, m->stream);
+                else if (o->learn_mode != 0 && !o->rec)
+                    launch_fill_int(o->lCnt.p, (int)m->colOffInner.size(), 0, m->stream);
                 // an iteration that does not keep the deferred lists (recording,
-                // learning off) drops them: a later deferred pass must not take a
-                // stale list, whose vertices its windows would not skip
+                // learning off, the immediate policy) drops them: a later deferred
+                // pass must not take a stale list, whose vertices its windows
+                // would not skip
                 if (deferred) o->defer_pending = true;
                 else if (o->defer_pending) {
                     reset_deferred(o);
                     o->defer_pending = false;
                 }
-                else if (o->learn_mode != 0 && !o->rec)
-                    launch_fill_int(o->lCnt.p, (int)m->colOffInner.size(), 0, m->stream);
                 for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c) {
                     a.cStart = m->colOffInner[c]; a.cEnd = m->colOffInner[c + 1];
                     const int nc = a.cEnd - a.cStart;

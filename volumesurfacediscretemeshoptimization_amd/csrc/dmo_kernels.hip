@@ -674,8 +674,10 @@ __device__ __forceinline__ void learned_offset(const SearchArgs& a, const LearnA
 
 // append vid to the class's fallback list where need (one lane per group):
 // one atomic per wave for the wave's fallbacks (the list order does not
-// matter: the class's vertices are independent).  Every lane calls it.
-__device__ __forceinline__ void fallback_append(const LearnArgs& la, bool need, int vid)
+// matter: the class's vertices are independent).  Every lane calls it.  A
+// class appends at most its cap = class size vertices (its count starts at
+// zero); the bound only guards the list.
+__device__ __forceinline__ void fallback_append(const LearnArgs& la, bool need, int vid, int cap)
 {
     const unsigned long long bal = __ballot(need);
     if (bal == 0ull) return;
@@ -683,7 +685,8 @@ __device__ __forceinline__ void fallback_append(const LearnArgs& la, bool need, 
     int base = 0;
     if (lane == 0) base = atomicAdd(la.cnt, __popcll(bal));
     base = __shfl(base, 0);
-    if (need) la.list[base + __popcll(bal & ((1ull << lane) - 1ull))] = vid;
+    const int idx = base + __popcll(bal & ((1ull << lane) - 1ull));
+    if (need && idx >= 0 && idx < cap) la.list[idx] = vid;
 }
 
 // The window search of the WPB * LV class vertices of block blk: LV
@@ -780,7 +783,7 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     }
     if (moved && own && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
     const bool need = !moved && !tie && own && s == 0;
-    fallback_append(la, need, vid);
+    fallback_append(la, need, vid, nc);
     if (need && la.epoch) la.epoch[vid] = la.it;
 }
 
@@ -882,7 +885,7 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     }
     if (moved && own && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
     const bool need = !moved && !tie && !hex_pinned(f) && own && s == 0;
-    fallback_append(la, need, vid);
+    fallback_append(la, need, vid, nc);
     if (need && la.epoch) la.epoch[vid] = la.it;
 }
 
@@ -913,7 +916,7 @@ __global__ __launch_bounds__(256) void k_search_tet_learned_def(SearchArgs a, Le
     extern __shared__ __align__(16) unsigned char s_tcd[];
     __shared__ RedLds<4> rd;
     if ((int)blockIdx.x < nbList) {
-        const int n = *la.cntIn;
+        const int n = min(*la.cntIn, a.cEnd - a.cStart);
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
         for (int i = blockIdx.x; i < n; i += nbList) {
             tet_vertex<NQ, 4>(a, la.listIn[i], a.extentMin != 0, reinterpret_cast<TetC*>(s_tcd), rd);
@@ -929,7 +932,7 @@ __global__ __launch_bounds__(256) void k_search_hex_learned_def(SearchArgs a, Le
     extern __shared__ __align__(16) unsigned char s_hxd[];
     __shared__ RedLds<4> rd;
     if ((int)blockIdx.x < nbList) {
-        const int n = *la.cntIn;
+        const int n = min(*la.cntIn, a.cEnd - a.cStart);
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
         for (int i = blockIdx.x; i < n; i += nbList) {
             hex_vertex<NQ, 4>(a, la.listIn[i], a.extentMin != 0, reinterpret_cast<HexLds*>(s_hxd), rd);
@@ -950,7 +953,7 @@ __global__ __launch_bounds__(64 * W) void k_search_list_tet(SearchArgs a, const 
 {
     extern __shared__ TetC s_tcf[];
     __shared__ RedLds<W> rd;
-    const int n = *cnt;
+    const int n = min(*cnt, a.cEnd - a.cStart);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(fallbacks, n);
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         tet_vertex<NQ, W>(a, list[i], a.extentMin != 0, s_tcf, rd);
@@ -963,7 +966,7 @@ __global__ __launch_bounds__(64 * W) void k_search_list_hex(SearchArgs a, const 
 {
     extern __shared__ HexLds s_hxf[];
     __shared__ RedLds<W> rd;
-    const int n = *cnt;
+    const int n = min(*cnt, a.cEnd - a.cStart);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(fallbacks, n);
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         hex_vertex<NQ, W>(a, list[i], a.extentMin != 0, s_hxf, rd);
