@@ -267,8 +267,9 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
 /* when a vertex's proposal window finds nothing better than its start:
  * IMMEDIATE -- its full search runs in the same class pass (a second launch);
  * DEFERRED -- it keeps its position and its full search runs in the class's
- * next pass, beside that pass's windows (one launch per class; nq 4 or 8,
- * else VSDMO_NOT_SUPPORTED).  Setting it clears pending deferred searches. */
+ * next pass, beside that pass's windows (one launch per class; nq 4 or 8 and
+ * one-rings of at most 128 tets / 21 hexes, else VSDMO_NOT_SUPPORTED).  Setting
+ * it clears pending deferred searches. */
 enum { VSDMO_FALLBACK_IMMEDIATE = 0, VSDMO_FALLBACK_DEFERRED = 1 };
 vsdmo_status vsdmo_opt_set_fallback(vsdmo_opt* o, int32_t policy);
 

@@ -789,8 +789,9 @@ vsdmo_status vsdmo_opt_set_fallback(vsdmo_opt* o, int32_t policy)
         if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
         if (policy != VSDMO_FALLBACK_IMMEDIATE && policy != VSDMO_FALLBACK_DEFERRED)
             return fail(VSDMO_INVALID_ARG, "policy must be VSDMO_FALLBACK_IMMEDIATE or VSDMO_FALLBACK_DEFERRED");
-        if (policy == VSDMO_FALLBACK_DEFERRED && !learned_deferred_ok(o->p.nq))
-            return fail(VSDMO_NOT_SUPPORTED, "the deferred fallback needs nq 4 or 8");
+        if (policy == VSDMO_FALLBACK_DEFERRED && !learned_deferred_ok(o->m->elem, o->p.nq, o->m->maxRing))
+            return fail(VSDMO_NOT_SUPPORTED, "the deferred fallback needs nq 4 or 8 and one-rings of at most 128 tets "
+                                             "or 21 hexes");
         o->learn_fallback = policy;
         reset_deferred(o);
         VSDMO_HIP(hipStreamSynchronize(o->m->stream));

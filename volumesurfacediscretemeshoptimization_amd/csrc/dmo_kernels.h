@@ -158,8 +158,8 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
                            hipStream_t s);
 // largest one-ring (elements) the learned search supports
 int learned_ring_cap(int kind);
-// whether the deferred fallback policy runs with this candidate grid
-bool learned_deferred_ok(int nq);
+// whether the deferred fallback policy runs with this candidate grid and ring size
+bool learned_deferred_ok(int kind, int nq, int ringCap);
 // whether the window kernel can run the network itself for this ring size
 bool learned_fused_ok(int kind, int ringCap);
 void launch_batch_quality(const BatchQArgs& a, hipStream_t s);

@@ -994,14 +994,19 @@ static size_t learned_per(int kind) { return (kind == VSDMO_TET) ? sizeof(TetC) 
 // largest ring the window kernel's LDS (kLV rings per block, 64 KB) holds
 int learned_ring_cap(int kind) { return (int)((64 * 1024) / (kLV * learned_per(kind))); }
 
-// the deferred policy's full searches run 4 waves per vertex
-bool learned_deferred_ok(int nq) { return nq == 4 || nq == 8; }
-
 // the fused kernel holds 16 rings per block in the same 64 KB
 bool learned_fused_ok(int kind, int ringCap)
 {
     return (size_t)kMlpRows * std::max(ringCap, 1) * learned_per(kind) <= 64 * 1024;
 }
+
+// the deferred policy's full searches run 4 waves per vertex, and its blocks
+// hold 16 rings in LDS (learned_fused_ok's budget) with or without the network
+bool learned_deferred_ok(int kind, int nq, int ringCap)
+{
+    return (nq == 4 || nq == 8) && learned_fused_ok(kind, ringCap);
+}
+
 
 template <int NQ, bool FUSED>
 static void learned_def_nq(int kind, const SearchArgs& a, const LearnArgs& la, int ringCap, int nbList, int nb,
