@@ -249,8 +249,8 @@ struct InferArgs {
 };
 
 // Fused inference, 16 rows per block of 4 waves (mlp_features16 /
-// mlp_layers16): features (from the positions, or rows of Xin) -> LDS,
-// layers 1-2 on MFMA, layer 3 on the VALU.  Writes Y and L of rows [0, n).
+// mlp_layers16): features (from the positions, or rows of Xin) -> LDS, the
+// three layers on MFMA tiles.  Writes Y and L of rows [0, n).
 __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a)
 {
     __shared__ MlpLds sm;
