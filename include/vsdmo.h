@@ -230,7 +230,10 @@ vsdmo_status vsdmo_opt_surface_points(vsdmo_opt* o, int32_t vid, int32_t nu, int
  * the full DMO search gives it.  The optimiser then searches a 4x4x4 grid on
  * a window `window_frac` times the plain first-level window around the
  * proposal for `depth` levels and falls back to the plain full search when
- * no candidate beats the vertex's current quality.
+ * the window's best candidate is lower than the vertex's current quality
+ * (a tie at a positive quality -- a ring minimum held by sub-tets the vertex
+ * is not part of -- keeps the vertex, as does a hex vertex whose
+ * vertex-free sub-tets already score 0).
  * ------------------------------------------------------------------------ */
 typedef struct vsdmo_mlp vsdmo_mlp;
 typedef struct vsdmo_dataset vsdmo_dataset;
