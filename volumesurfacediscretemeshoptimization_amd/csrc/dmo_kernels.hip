@@ -706,19 +706,70 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const int vid = a.cStart + ci;
     // a vertex deferred by the previous iteration takes the full search in this launch instead
     const bool own = ci0 + gb < nc && !(la.epoch && la.epoch[vid] == la.it - 1);
-    float pl, py0, py1, py2;
-    learned_offset<FUSED>(a, la, ci0, gb, ci, s_tcl, pl, py0, py1, py2);
-    TetC* tc = reinterpret_cast<TetC*>(s_tcl) + (size_t)gb * ringCap;
     TetFrame f;
     const int r0 = a.ringRow[vid];
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
-    float mabs;
-    group_extents<G>(a, r0, f.n, 3, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
+    // the lane's first PRE ring elements are gathered into registers before
+    // the network runs (their latency overlaps it; no other vertex of this
+    // launch moves them: ring vertices belong to other colour classes)
+    constexpr int PRE = 2;
+    float rx[PRE][3], ry[PRE][3], rz[PRE][3];
+#pragma unroll
+    for (int k = 0; k < PRE; ++k) {
+        const int e = s + k * G;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            rx[k][t] = f.x0x; ry[k][t] = f.x0y; rz[k][t] = f.x0z;
+            if (e < f.n) {
+                const int v = a.ringVal[(size_t)(r0 + e) * 3 + t];
+                rx[k][t] = a.x[v]; ry[k][t] = a.y[v]; rz[k][t] = a.z[v];
+            }
+        }
+    }
+    float pl, py0, py1, py2;
+    learned_offset<FUSED>(a, la, ci0, gb, ci, s_tcl, pl, py0, py1, py2);
+    TetC* tc = reinterpret_cast<TetC*>(s_tcl) + (size_t)gb * ringCap;
+    // window half-extents and frame scale (max / min: the order of the
+    // elements does not matter), then the ring constants into LDS
+    const bool emin = a.extentMin != 0;
+    float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
+    auto extent = [&](float px, float py, float pz) {
+        const float dx = fabsf(px - f.x0x), dy = fabsf(py - f.x0y), dz = fabsf(pz - f.x0z);
+        ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+        ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+        ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+        mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+    };
+#pragma unroll
+    for (int k = 0; k < PRE; ++k)
+        if (s + k * G < f.n)
+#pragma unroll
+            for (int t = 0; t < 3; ++t) extent(rx[k][t], ry[k][t], rz[k][t]);
+    for (int e = s + PRE * G; e < f.n; e += G)
+        for (int t = 0; t < 3; ++t) {
+            const int v = a.ringVal[(size_t)(r0 + e) * 3 + t];
+            extent(a.x[v], a.y[v], a.z[v]);
+        }
+    if (emin) { ex = group_min<G>(ex); ey = group_min<G>(ey); ez = group_min<G>(ez); }
+    else { ex = group_max<G>(ex); ey = group_max<G>(ey); ez = group_max<G>(ez); }
+    mabs = group_max<G>(mabs);
+    f.ex = ex; f.ey = ey; f.ez = ez;
     f.sg = pow2_scale(mabs);
     const float isg = 1.0f / f.sg;
     f.is3 = ((1.0f * isg) * isg) * isg;
-    for (int e = s; e < f.n; e += G) {
+#pragma unroll
+    for (int k = 0; k < PRE; ++k) {
+        const int e = s + k * G;
+        if (e < f.n) {
+            f3 p[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+                p[t] = { (rx[k][t] - f.x0x) * f.sg, (ry[k][t] - f.x0y) * f.sg, (rz[k][t] - f.x0z) * f.sg };
+            tc[e] = tet_consts(p[0], p[1], p[2]);
+        }
+    }
+    for (int e = s + PRE * G; e < f.n; e += G) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 3;
         f3 p[3];
 #pragma unroll
