@@ -977,8 +977,11 @@ __global__ __launch_bounds__(256) void k_search_tet_learned_def(SearchArgs a, Le
     }
     tet_window<NQ, 4, 4, FUSED>(a, la, ringCap, (int)blockIdx.x - nbList, s_tcd);
 }
+// 4 waves per SIMD (<= 128 registers): a 48^3 class (~13k vertices, 3.3k
+// waves) then fits the chip's wave slots in one round
 template <int NQ, bool FUSED>
-__global__ __launch_bounds__(256) void k_search_hex_learned_def(SearchArgs a, LearnArgs la, int ringCap, int nbList)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_search_hex_learned_def(SearchArgs a, LearnArgs la, int ringCap, int nbList)
 {
     extern __shared__ __align__(16) unsigned char s_hxd[];
     __shared__ RedLds<4> rd;
