@@ -8,6 +8,8 @@
 //       (reference VSDMO/src/dmo/DMOBase.h:23-88, DMOTet.h:24), and the
 //       DMOHex / DMOTriFlat / DMOTri / DMOQuad classes
 //   laplaceSmoothing(mesh, n_iter) (reference VSDMO/src/dmo/Laplace.h:10)
+// plus the learned proposal (Dataset, MLP, DMOBaseClass::record /
+// setProposal / setFallback), which has no counterpart in the reference.
 // The file-name overloads read OFF / TetGen / MEDIT files (vsdmo_mesh_file_read).
 // Errors become std::runtime_error carrying vsdmo_last_error().
 #pragma once
@@ -108,6 +110,45 @@ struct DMOMeshHexFactory {
     static std::shared_ptr<DMOMeshHex> create(const std::string& meditFile) { return create_from_file(VSDMO_HEX, 8, meditFile); }
 };
 
+// The learned proposal (no counterpart in the reference): training pairs
+// recorded from plain DMO, and the 96-128-128-3 network trained on them.
+class Dataset {
+public:
+    explicit Dataset(int64_t capacity) { check(vsdmo_dataset_create(capacity, &h_), "vsdmo_dataset_create"); }
+    ~Dataset() { if (h_) vsdmo_dataset_destroy(h_); }
+    Dataset(const Dataset&) = delete;
+    Dataset& operator=(const Dataset&) = delete;
+    int64_t size() const
+    {
+        int64_t n = 0;
+        check(vsdmo_dataset_size(h_, &n), "vsdmo_dataset_size");
+        return n;
+    }
+    vsdmo_dataset* handle() const { return h_; }
+
+private:
+    vsdmo_dataset* h_ = nullptr;
+};
+
+class MLP {
+public:
+    explicit MLP(uint64_t seed = 0) { check(vsdmo_mlp_create(seed, &h_), "vsdmo_mlp_create"); }
+    ~MLP() { if (h_) vsdmo_mlp_destroy(h_); }
+    MLP(const MLP&) = delete;
+    MLP& operator=(const MLP&) = delete;
+    // Adam training; returns the mean batch loss of every epoch
+    std::vector<double> train(Dataset& ds, int epochs = 20, int batch = 4096, float lr = 1e-3f, uint64_t seed = 0)
+    {
+        std::vector<double> loss((size_t)epochs);
+        check(vsdmo_mlp_train(h_, ds.handle(), epochs, batch, lr, seed, loss.data()), "vsdmo_mlp_train");
+        return loss;
+    }
+    vsdmo_mlp* handle() const { return h_; }
+
+private:
+    vsdmo_mlp* h_ = nullptr;
+};
+
 }  // namespace vsdmo_cpp
 
 namespace DMO {
@@ -175,6 +216,18 @@ public:
     {
         vsdmo_cpp::check(vsdmo_opt_set_failure_dump(h_, path.c_str()), "vsdmo_opt_set_failure_dump");
     }
+    // learned proposal (no counterpart in the reference): n plain iterations
+    // recording training pairs; the proposal mode (1 network, 2 the vertex
+    // itself, 0 off) and its window (measured defaults: tets 1 level on 0.025,
+    // hexes 1 level on 2.0); the fallback policy (VSDMO_FALLBACK_IMMEDIATE /
+    // VSDMO_FALLBACK_DEFERRED)
+    void record(vsdmo_cpp::Dataset& ds, int n) { vsdmo_cpp::check(vsdmo_opt_record(h_, ds.handle(), n), "vsdmo_opt_record"); }
+    void setProposal(vsdmo_cpp::MLP* mlp, int mode, int depth, float windowFrac)
+    {
+        vsdmo_cpp::check(vsdmo_opt_set_proposal(h_, mlp ? mlp->handle() : nullptr, mode, depth, windowFrac),
+                         "vsdmo_opt_set_proposal");
+    }
+    void setFallback(int policy) { vsdmo_cpp::check(vsdmo_opt_set_fallback(h_, policy), "vsdmo_opt_set_fallback"); }
     vsdmo_opt* handle() const { return h_; }
 
 protected:

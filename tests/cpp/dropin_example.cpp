@@ -43,6 +43,26 @@ int main()
         dmo.optimize();
         std::printf("min quality %g done %d\n", dmo.getQuality(), (int)dmo.isDone());
         dmo.displayQualityGPU(10);
+        // the learned proposal (no counterpart in the reference): training
+        // pairs from plain iterations, a network, the proposal search
+        vsdmo_params prm;
+        vsdmo_params_default(&prm);
+        prm.fixed_boundary = 1;
+        vsdmo_cpp::Dataset ds(100000);
+        auto recMesh = vsdmo_cpp::DMOMeshTetFactory::create(xyz, tets);
+        DMO::DMOTetClass rec(*recMesh, DMO::QualityCriterium::MEAN_RATIO, 0.5f, 3, &prm);
+        rec.record(ds, 3);
+        vsdmo_cpp::MLP mlp(1);
+        std::vector<double> loss = mlp.train(ds, 3, 64, 1e-3f, 2);
+        auto learnMesh = vsdmo_cpp::DMOMeshTetFactory::create(xyz, tets);
+        DMO::DMOTetClass learned(*learnMesh, DMO::QualityCriterium::MEAN_RATIO, 0.5f, 3, &prm);
+        learned.setProposal(&mlp, 1, 1, 0.025f);
+        learned.setFallback(VSDMO_FALLBACK_DEFERRED);
+        const float q0 = learned.getQuality();
+        learned.optimize();
+        std::printf("learned: %lld pairs, loss %g -> %g, min quality %g -> %g\n", (long long)ds.size(), loss.front(),
+                    loss.back(), q0, learned.getQuality());
+        if (!(learned.getQuality() >= q0)) return 2;
     } catch (const std::exception& e) {
         std::printf("error: %s\n", e.what());
         return 1;
