@@ -63,6 +63,10 @@ WORKLOADS = {
               "traffic": 10_970_146, "learned_traffic": 8_042_968},
     "hex48": {"kind": "hex", "flop": 5_014_240, "cbrt": 110_880, "kernel": "k_search_hex<8>",
               "traffic": 17_570_841, "learned_traffic": 12_274_568},
+    # the 48^3 hex cube at jitter 0.15h: a valid mesh, so the surface and
+    # feature passes run every iteration (the roofline is the inner kernel's)
+    "hex48_surface": {"kind": "hex", "flop": 5_014_240, "cbrt": 110_880, "kernel": "k_search_hex<8>",
+                      "traffic": None},
     "tri64": {"kind": "tri", "flop": 27_048, "cbrt": 0, "kernel": "k_search_flat<8>", "traffic": None},
     "surf_tri": {"kind": "tri", "surface": True, "flop": SURF_FLOP["tri"], "flop1d": SURF_FLOP1D["tri"], "cbrt": 0,
                  "kernel": "k_search_fan2d<false>", "traffic": 18_791_143},
@@ -187,6 +191,13 @@ def make_mesh(name: str, seed: int):
                 f"surface vertices +-0.3h tangentially, seed {seed}, surface free (quadric fits; surface passes "
                 "run only while the minimum quality is > 0); 8^3 candidates x 3 levels per inner vertex; one "
                 "step = one iteration over all colour classes")
+        return xyz, cells, desc, {"fixed_boundary": False}
+    if name == "hex48_surface":
+        xyz, cells = synth.hex_cube(48, seed=seed, jitter=0.15)
+        desc = ("plain GPU DMO, hex cube 48^3 cells (117,649 vertices, 110,592 hexes), interior jitter +-0.15h, "
+                f"surface vertices +-0.15h tangentially, seed {seed}, surface free: a valid mesh, so the surface "
+                "(quadric fits, 8 x 8 candidates x 3 levels, projection) and feature (8 x 3) passes run every "
+                "iteration after the inner passes; one step = one iteration")
         return xyz, cells, desc, {"fixed_boundary": False}
     if name in ("surf_tri", "surf_quad"):
         quads = name == "surf_quad"
