@@ -291,6 +291,16 @@ def test_deferred_fallback_monotone_and_policy_checks():
     with pytest.raises(VsdmoError):
         o6.set_fallback("deferred")
     o6.close()
+    # nq 4: the deferred full searches run the 4^3 grid (one candidate per lane)
+    o4 = DMOTetClass(m, n_iter=4, fixed_boundary=True, nq=4)
+    o4.set_proposal(None, mode=2, depth=1, window_frac=0.25, fallback="deferred")
+    q = [o4.getQuality()]
+    for _ in range(4):
+        o4.doIteration()
+        q.append(o4.getQuality())
+    assert all(b >= a for a, b in zip(q, q[1:])), q
+    assert o4.stats()["fallbacks"] > 0
+    o4.close()
     m.close()
 
 
