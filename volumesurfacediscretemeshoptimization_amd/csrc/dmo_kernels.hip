@@ -1083,8 +1083,9 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
                   nullptr, nullptr, nullptr, 0 };
     if (def) {
         // one launch: the previous iteration's fallbacks (list blocks sized for
-        // an eighth of the class; they loop when more were deferred) beside
-        // this iteration's windows, 16 vertices per block
+        // an eighth of the class, they loop when more were deferred: a quarter
+        // and a twelfth measured slower on the 40^3 tet cube) beside this
+        // iteration's windows, 16 vertices per block
         la.listIn = def->listIn; la.cntIn = def->cntIn; la.epoch = def->epoch; la.it = def->it;
         const int nbList = std::max(1, std::min(nc, std::max(64, nc / 8)));
         const int nb = (nc + kMlpRows - 1) / kMlpRows;
