@@ -418,14 +418,21 @@ std::vector<int> colour_and_sort(vsdmo_mesh& m, Renumber& rn, const int* incRow,
     Incidence inc{ incRow, incVal, rn.oldOfCur.p, m.cells.p, m.npe };
     DevBuf<int> col(n), left(1);
     VSDMO_HIP(hipMemsetAsync(col.p, 0xff, sizeof(int) * n, s));
+    // rounds go out in batches of kRoundsPerCheck and the host reads the
+    // count of the batch's last round: a round after the colouring is
+    // complete finds every vertex coloured and changes nothing, so the colours
+    // are those of one check per round, with 1/kRoundsPerCheck of the syncs
+    constexpr int kRoundsPerCheck = 16;
     int h_left = n;
-    for (int round = 0; h_left > 0; ++round) {
+    for (int round = 0; h_left > 0;) {
         if (round > n + 1) throw StatusError{ VSDMO_TOPOLOGY, "colouring did not converge" };
-        VSDMO_HIP(hipMemsetAsync(left.p, 0, sizeof(int), s));
-        if (mode == VSDMO_COLOR_JPL)
-            k_color_jpl_round<<<nblk(n), TB, 0, s>>>(inc, lo, hi, round, col.p, left.p);
-        else
-            k_color_greedy_round<<<nblk(n), TB, 0, s>>>(inc, lo, hi, col.p, left.p);
+        for (int k = 0; k < kRoundsPerCheck; ++k, ++round) {
+            VSDMO_HIP(hipMemsetAsync(left.p, 0, sizeof(int), s));
+            if (mode == VSDMO_COLOR_JPL)
+                k_color_jpl_round<<<nblk(n), TB, 0, s>>>(inc, lo, hi, round, col.p, left.p);
+            else
+                k_color_greedy_round<<<nblk(n), TB, 0, s>>>(inc, lo, hi, col.p, left.p);
+        }
         VSDMO_HIP(hipGetLastError());
         VSDMO_HIP(hipMemcpyAsync(&h_left, left.p, sizeof(int), hipMemcpyDeviceToHost, s));
         VSDMO_HIP(hipStreamSynchronize(s));
