@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "dmo_kernels.h"
+#include "mlp_internal.h"
 #include "vsdmo_internal.h"
 
 namespace vsdmo {
@@ -63,7 +64,8 @@ struct vsdmo_batch {
     DevBuf<int> rowTet, ringTet, rowHex, ringHex, heRow, heTarget;
     int maxRingTet = 0, maxRingHex = 0, maxHE = 0;
     // work items per round, in mesh order, and the order used (shuffled)
-    std::vector<int> itemsMeshOrder, roundOff, roundMid;   // round r: heavy [off, mid), light [mid, off+1)
+    // round r: tets [off, midT), hexes [midT, mid), flat triangles [mid, off+1)
+    std::vector<int> itemsMeshOrder, roundOff, roundMidT, roundMid;
     DevBuf<int> items, ctr;
     std::vector<int> itemsPerMesh;               // searches per iteration per mesh
     // qualities
@@ -84,6 +86,13 @@ struct vsdmo_batch {
     long long searches = 0;
     int iterations = 0;
     double gpu_ms = 0.0;
+    // learned proposal of the tet items (immediate fallback): the network,
+    // the global neighbour table, each round's fallback list and its length
+    vsdmo_mlp* mlp = nullptr;
+    int learn_mode = 0, learn_depth = 1;
+    float learn_frac = 0.025f;
+    DevBuf<int> nbr, lList, lCnt, lFallback;
+    long long learned = 0;
 };
 
 namespace {
@@ -143,7 +152,8 @@ void upload_items(vsdmo_batch* b)
             }
         };
         for (size_t r = 0; r + 1 < b->roundOff.size(); ++r) {
-            shuffle(b->roundOff[r], b->roundMid[r]);
+            shuffle(b->roundOff[r], b->roundMidT[r]);
+            shuffle(b->roundMidT[r], b->roundMid[r]);
             shuffle(b->roundMid[r], b->roundOff[r + 1]);
         }
     }
@@ -168,10 +178,26 @@ void record_iteration(vsdmo_batch* b)
     hipStream_t s = b->stream;
     const int nR = (int)b->grid.size();
     launch_fill_int(b->ctr.p, nR, 0, s);
+    const bool learn = b->learn_mode != 0;
+    if (learn) launch_fill_int(b->lCnt.p, nR, 0, s);
     SearchArgs a = search_args(b);
-    for (int r = 0; r < nR; ++r)
-        launch_batch_round(b->p.nq, b->p.waves_per_vertex, a, b->rowHex.p, b->ringHex.p, b->items.p, b->roundOff[r],
+    for (int r = 0; r < nR; ++r) {
+        int lo = b->roundOff[r];
+        if (learn && b->roundMidT[r] > lo) {
+            // the round's tet items: proposal windows, then the full search of
+            // their fallbacks (immediate policy); hexes and triangles follow
+            SearchArgs al = a;
+            al.cStart = lo; al.cEnd = b->roundMidT[r];
+            LearnNet net;
+            net.items = b->items.p;
+            if (b->learn_mode == 1) { net.wb = b->mlp->wb.p; net.w = b->mlp->w.p; net.nbr = b->nbr.p; }
+            launch_search_learned(VSDMO_TET, b->p.nq, al, net, b->learn_mode, b->learn_depth, b->learn_frac,
+                                  b->lFallback.p, b->lList.p, b->lCnt.p + r, b->maxRingTet, nullptr, s);
+            lo = b->roundMidT[r];
+        }
+        launch_batch_round(b->p.nq, b->p.waves_per_vertex, a, b->rowHex.p, b->ringHex.p, b->items.p, lo,
                            b->roundMid[r], b->roundOff[r + 1], b->ctr.p + r, b->grid[r], b->lds, s);
+    }
     launch_batch_quality(quality_args(b), s);
 }
 
@@ -236,6 +262,11 @@ vsdmo_status run(vsdmo_batch* b, int n, std::vector<float>* rows_out)
     b->hrow += n;
     b->iterations += n;
     b->searches += (long long)n * (long long)b->itemsMeshOrder.size();
+    if (b->learn_mode != 0) {
+        long long tets = 0;
+        for (size_t r = 0; r + 1 < b->roundOff.size(); ++r) tets += b->roundMidT[r] - b->roundOff[r];
+        b->learned += (long long)n * tets;
+    }
     vsdmo_status st = VSDMO_OK;
     std::string msg;
     for (int i = 0; i < n; ++i)
@@ -440,15 +471,19 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
                 nRounds = std::max(nRounds, (int)co.size() - 1);
             }
             b->roundOff.assign(nRounds + 1, 0);
+            b->roundMidT.assign(nRounds, 0);
             b->roundMid.assign(nRounds, 0);
             b->itemsPerMesh.assign(nm, 0);
             for (int r = 0; r < nRounds; ++r) {
-                for (int pass = 0; pass < 2; ++pass) {
-                    if (pass == 1) b->roundMid[r] = (int)b->itemsMeshOrder.size();
+                // tets, then hexes, then flat triangles (mesh order inside each)
+                for (int pass = 0; pass < 3; ++pass) {
+                    if (pass == 1) b->roundMidT[r] = (int)b->itemsMeshOrder.size();
+                    if (pass == 2) b->roundMid[r] = (int)b->itemsMeshOrder.size();
                     for (int i = 0; i < nm; ++i) {
                         vsdmo_mesh* m = b->meshes[i];
                         const bool tri = (m->elem == VSDMO_TRI);
-                        if ((pass == 0) == tri) continue;
+                        const int want = (m->elem == VSDMO_TET) ? 0 : (m->elem == VSDMO_HEX) ? 1 : 2;
+                        if (want != pass) continue;
                         const auto& co = tri ? m->colOffFree : m->colOffInner;
                         if (r + 1 >= (int)co.size()) continue;
                         const int type = (m->elem == VSDMO_TET) ? 0 : (m->elem == VSDMO_HEX) ? 1 : 2;
@@ -598,6 +633,79 @@ vsdmo_status vsdmo_batch_set_params(vsdmo_batch* b, const vsdmo_batch_params* p)
     });
 }
 
+// Learned proposal for the tet items (immediate fallback: a round's windows,
+// then the full search of the vertices whose window does not improve them),
+// the single-mesh rule of vsdmo_opt_set_proposal over a round's tet items.
+// The network runs inside the window kernel, so mode 1 needs every tet ring
+// to fit its LDS tile.  The neighbour table is global: each tet mesh's table
+// with the mesh's vertex offset added.
+vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac)
+{
+    return guarded_b([&]() -> vsdmo_status {
+        if (!b) return fail_status(VSDMO_INVALID_ARG, "NULL batch");
+        if (mode < 0 || mode > 2) return fail_status(VSDMO_INVALID_ARG, "mode must be 0 (off), 1 (network) or 2 (identity)");
+        if (mode == 1 && !mlp) return fail_status(VSDMO_INVALID_ARG, "mode 1 needs a network");
+        if (mode != 0 && (depth < 1 || depth > 8 || !(window_frac > 0.0f)))
+            return fail_status(VSDMO_INVALID_ARG, "depth must be in [1,8] and window_frac > 0");
+        if (mode == 1 && !learned_fused_ok(VSDMO_TET, b->maxRingTet))
+            return fail_status(VSDMO_NOT_SUPPORTED, "the batched learned search runs the network in the window "
+                                                    "kernel, whose LDS tile of 16 rings does not hold tet one-rings of " +
+                                                        std::to_string(b->maxRingTet) + " elements");
+        if (mode == 2 && b->maxRingTet > learned_ring_cap(VSDMO_TET))
+            return fail_status(VSDMO_NOT_SUPPORTED, "the learned search holds tet one-rings of at most " +
+                                                        std::to_string(learned_ring_cap(VSDMO_TET)) + " elements");
+        hipStream_t s = b->stream;
+        VSDMO_HIP(hipStreamSynchronize(s));
+        drop_graph(b);   // the recorded launches change
+        b->learn_mode = mode;
+        b->mlp = mode == 1 ? mlp : nullptr;
+        b->learn_depth = depth;
+        b->learn_frac = window_frac;
+        if (mode == 0) return VSDMO_OK;
+        const int nR = (int)b->grid.size();
+        int maxTet = 1;
+        for (int r = 0; r < nR; ++r) maxTet = std::max(maxTet, b->roundMidT[r] - b->roundOff[r]);
+        if ((int)b->lList.n < maxTet) b->lList.alloc(maxTet);
+        if (!b->lCnt.p) b->lCnt.alloc(std::max(nR, 1));
+        if (!b->lFallback.p) {
+            b->lFallback.alloc(1);
+            launch_fill_int(b->lFallback.p, 1, 0, s);
+        }
+        if (mode == 1 && !b->nbr.p) {
+            std::vector<int> t((size_t)b->totalV * kNbr, -1);
+            for (size_t i = 0; i < b->meshes.size(); ++i) {
+                vsdmo_mesh* m = b->meshes[i];
+                if (m->elem != VSDMO_TET) continue;
+                build_neighbour_table(*m);
+                std::vector<int> loc = m->nbr.to_host(m->stream);
+                const int off = b->vOff[i];
+                int* dst = t.data() + (size_t)off * kNbr;
+                for (size_t k = 0; k < loc.size(); ++k) dst[k] = loc[k] >= 0 ? loc[k] + off : -1;
+            }
+            b->nbr.alloc(t.size());
+            b->nbr.upload(t.data(), t.size(), s);
+        }
+        if (mode == 1) VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
+        VSDMO_HIP(hipStreamSynchronize(s));
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_batch_learned_stats(vsdmo_batch* b, int64_t* visits, int64_t* fallbacks)
+{
+    return guarded_b([&]() -> vsdmo_status {
+        if (!b || !visits || !fallbacks) return fail_status(VSDMO_INVALID_ARG, "NULL argument");
+        *visits = b->learned;
+        int f = 0;
+        if (b->lFallback.p) {
+            VSDMO_HIP(hipMemcpyAsync(&f, b->lFallback.p, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+            VSDMO_HIP(hipStreamSynchronize(b->stream));
+        }
+        *fallbacks = f;
+        return VSDMO_OK;
+    });
+}
+
 // Autotuner over (nq, depth, waves per vertex).  Every trial starts from the
 // positions at the call and runs `iters` iterations.  waves_per_vertex does
 // not change results, so it is tuned on time alone at the current (nq,
@@ -610,6 +718,8 @@ vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, cons
     return guarded_b([&]() -> vsdmo_status {
         if (!b) return fail_status(VSDMO_INVALID_ARG, "NULL batch");
         if (iters <= 0 || !(tol >= 0.0f)) return fail_status(VSDMO_INVALID_ARG, "iters must be > 0 and tol >= 0");
+        if (b->learn_mode != 0)
+            return fail_status(VSDMO_NOT_SUPPORTED, "the autotuner runs plain searches: set the proposal to mode 0 first");
         const size_t nm = b->meshes.size();
         const size_t V = (size_t)b->totalV;
         // snapshot of the start positions, guard state and counters

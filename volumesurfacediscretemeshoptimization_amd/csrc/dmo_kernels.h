@@ -137,6 +137,9 @@ struct LearnNet {
     const uint16_t* wb = nullptr;
     const float* w = nullptr;
     const int* nbr = nullptr;
+    // vertex ids of the class positions through a batch round's work items
+    // (id in the low 30 bits); null: positions a.cStart + i of one mesh
+    const int* items = nullptr;
 };
 // deferred fallback of one class pass: the previous iteration's list of the
 // class (listIn[0 .. *cntIn)), the per-vertex deferral epochs, the iteration

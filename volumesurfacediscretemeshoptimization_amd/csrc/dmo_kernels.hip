@@ -587,7 +587,16 @@ struct LearnArgs {
     const int* cntIn;
     int* epoch;
     int it;
+    // class positions -> vertex ids through a work-item list (a batch round:
+    // id in the low 30 bits); null: the class is the range a.cStart + position
+    const int* items;
 };
+
+// vertex id of class position ci
+__device__ __forceinline__ int learned_vertex(const SearchArgs& a, const LearnArgs& la, int ci)
+{
+    return la.items ? (la.items[a.cStart + ci] & 0x3fffffff) : a.cStart + ci;
+}
 
 // The proposal window runs 4 vertices per wave, 16 lanes each: lane s =
 // j + 4k of a group owns the candidates (i, j, k), i = 0..3, which share the
@@ -656,13 +665,13 @@ __device__ void group_extents(const SearchArgs& a, int r0, int n, int st, int s,
 // inference launch's L, Y (mode 1) or are zero (mode 2).
 template <bool FUSED>
 __device__ __forceinline__ void learned_offset(const SearchArgs& a, const LearnArgs& la, int ci0, int gb, int ci,
-                                               unsigned char* lds, float& l, float& y0, float& y1, float& y2)
+                                               int vid, unsigned char* lds, float& l, float& y0, float& y1, float& y2)
 {
     l = 0.0f; y0 = 0.0f; y1 = 0.0f; y2 = 0.0f;
     if constexpr (FUSED) {
         MlpLds& sm = *reinterpret_cast<MlpLds*>(lds);
-        const bool live = ci0 + gb < a.cEnd - a.cStart;
-        mlp_features16(a.x, a.y, a.z, la.nbr, a.cStart + ci0 + gb, live, sm);
+        const bool live = ci0 + gb < a.cEnd - a.cStart;   // vid: row gb's vertex (clamped for a dead row, unused)
+        mlp_features16(a.x, a.y, a.z, la.nbr, vid, live, sm);
         mlp_layers16(la.wb, la.w, sm);
         l = sm.L[gb]; y0 = sm.Y[gb][0]; y1 = sm.Y[gb][1]; y2 = sm.Y[gb][2];
         __syncthreads();   // the ring staging reuses the LDS
@@ -703,7 +712,7 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const int nc = a.cEnd - a.cStart, ci0 = VPB * blk;
     // a group past the end of the class mirrors the class's last vertex and writes nothing
     const int ci = min(ci0 + gb, nc - 1);
-    const int vid = a.cStart + ci;
+    const int vid = learned_vertex(a, la, ci);
     // a vertex deferred by the previous iteration takes the full search in this launch instead
     const bool own = ci0 + gb < nc && !(la.epoch && la.epoch[vid] == la.it - 1);
     TetFrame f;
@@ -728,7 +737,7 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
         }
     }
     float pl, py0, py1, py2;
-    learned_offset<FUSED>(a, la, ci0, gb, ci, s_tcl, pl, py0, py1, py2);
+    learned_offset<FUSED>(a, la, ci0, gb, ci, vid, s_tcl, pl, py0, py1, py2);
     TetC* tc = reinterpret_cast<TetC*>(s_tcl) + (size_t)gb * ringCap;
     // window half-extents and frame scale (max / min: the order of the
     // elements does not matter), then the ring constants into LDS
@@ -847,10 +856,10 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const int s = threadIdx.x % G, gb = threadIdx.x / G;
     const int nc = a.cEnd - a.cStart, ci0 = VPB * blk;
     const int ci = min(ci0 + gb, nc - 1);
-    const int vid = a.cStart + ci;
+    const int vid = learned_vertex(a, la, ci);
     const bool own = ci0 + gb < nc && !(la.epoch && la.epoch[vid] == la.it - 1);
     float pl, py0, py1, py2;
-    learned_offset<FUSED>(a, la, ci0, gb, ci, s_hxl, pl, py0, py1, py2);
+    learned_offset<FUSED>(a, la, ci0, gb, ci, vid, s_hxl, pl, py0, py1, py2);
     HexLds* hx = reinterpret_cast<HexLds*>(s_hxl) + (size_t)gb * ringCap;
     HexFrame f;
     const int r0 = a.ringRow[vid];
@@ -1080,7 +1089,7 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
     const size_t per = learned_per(kind);
     const bool fused = mode == 1 && net.wb && learned_fused_ok(kind, ringCap);
     LearnArgs la{ net.Y, net.L, mode, depthL, frac, fallbacks, list, cnt, net.wb, net.w, net.nbr,
-                  nullptr, nullptr, nullptr, 0 };
+                  nullptr, nullptr, nullptr, 0, net.items };
     if (def) {
         // one launch: the previous iteration's fallbacks (list blocks sized for
         // an eighth of the class, they loop when more were deferred: a quarter
