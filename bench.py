@@ -79,6 +79,63 @@ WORKLOADS = {
 FLOP_PER_UPDATE_TRI = 27_048
 
 
+def _batch_learned(args, b, dt_plain, q_plain, qm_plain):
+    """The batch's tet meshes with the learned proposal (immediate fallback,
+    the tet window unless --learn-depth / --learn-frac), K iterations from
+    the start positions, against the
+    plain batch's K.  The triangle meshes keep the plain search, so the batch
+    needs K iterations for them in either run; the learned run must bring
+    every mesh to the plain run's final minimum and mean quality (-1e-3).
+    Network: trained on pairs recorded from plain DMO on the tet meshes of
+    another mix (seed + 5000)."""
+    from volumesurfacediscretemeshoptimization_amd import DMODataset, DMOMeshTetFactory, DMOMLP, DMOTetClass, synth
+    t0 = time.perf_counter()
+    ds = DMODataset(args.pairs)
+    used = 0
+    for k, xyz, cells in synth.batch_mix(256, seed=args.seed + 5000):
+        if k != "tet" or len(ds) >= args.pairs:
+            continue
+        m = DMOMeshTetFactory.create(xyz, cells)
+        o = DMOTetClass(m, n_iter=args.steps, fixed_boundary=True)
+        o.record(ds, args.steps)
+        o.close()
+        m.close()
+        used += 1
+    rec_s = time.perf_counter() - t0
+    mlp = DMOMLP(seed=args.seed + 1)
+    t0 = time.perf_counter()
+    losses = mlp.train(ds, epochs=args.epochs, batch=4096, lr=1e-3, seed=args.seed + 2)
+    train_s = time.perf_counter() - t0
+    n_pairs = len(ds)
+    ds.close()
+    depth, frac = args.learn_depth, args.learn_frac
+    b.restore()
+    b.set_proposal(mlp, mode=1, depth=depth, window_frac=frac)
+    b.iterate(max(args.warmup, 1))           # warm-up, then back to the start positions
+    b.restore()
+    ls0 = b.learned_stats()
+    st0 = b.stats()
+    t0 = time.perf_counter()
+    b.iterate(args.steps)
+    dt = time.perf_counter() - t0
+    st = b.stats()
+    ls = b.learned_stats()
+    q_l, qm_l = b.min_quality(), b.mean_quality()
+    below = int(np.sum((q_l < q_plain - 1e-3) | (qm_l < qm_plain - 1e-3)))
+    b.set_proposal(None, mode=0)
+    mlp.close()
+    visits = ls["learned_visits"] - ls0["learned_visits"]
+    return {"policy": "immediate", "levels": depth, "window_frac": frac, "iterations": args.steps,
+            "learned_ms_per_step": 1e3 * dt / args.steps, "plain_ms_per_step": 1e3 * dt_plain / args.steps,
+            "speedup_vs_plain": dt_plain / dt if below == 0 else 0.0,
+            "meshes_below_plain_quality": below,
+            "min_quality_end_worst": float(q_l.min()), "device_ms_per_step": (st["gpu_ms"] - st0["gpu_ms"]) / args.steps,
+            "learned_visits_per_step": visits // args.steps,
+            "fallback_rate": (ls["fallbacks"] - ls0["fallbacks"]) / max(1, visits),
+            "training_pairs": n_pairs, "training_meshes": used, "record_s": rec_s, "train_s": train_s,
+            "loss_last_epoch": float(losses[-1]) if len(losses) else None}
+
+
 def bench_batch(args):
     """256 meshes (half flat triangle grids, half tet cubes, vertex counts
     log-uniform in [500, 50000]) swept colour round by colour round, one
@@ -104,6 +161,7 @@ def bench_batch(args):
     dt = time.perf_counter() - t0
     st = b.stats()
     q1 = b.min_quality()
+    qm1 = b.mean_quality()
     searches = st["searches"] - st0["searches"]
     gpu_s = (st["gpu_ms"] - st0["gpu_ms"]) / 1e3
     # algorithmic work per iteration: tet searches x tet FLOP + tri searches x tri FLOP
@@ -131,6 +189,12 @@ def bench_batch(args):
     dt_plain = time.perf_counter() - t1
     plain_updates = sum(o.stats()["searches_inner"] + o.stats()["searches_surface"] for o in opts)
     q_plain = np.array([o.getQuality() for o in opts], dtype=np.float32)
+    learned = None
+    if args.batch_learned:
+        try:
+            learned = _batch_learned(args, b, dt, q1, qm1)
+        except Exception as e:        # the plain line must not depend on it
+            learned = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -164,6 +228,7 @@ def bench_batch(args):
                    "unbatched_value": plain_updates / dt_plain, "unbatched_ms_per_step": 1e3 * dt_plain / args.steps,
                    "unbatched_min_quality_end_worst": float(q_plain.min()),
                    "batched_over_unbatched": (searches / dt) / (plain_updates / dt_plain),
+                   "learned_tets_vs_plain": learned,
                    "tune_log": args.tune_log},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_batch_round (all rounds + "
@@ -435,6 +500,8 @@ def main(argv=None):
                     help="fallback policy of the measured learned run (the other one is reported beside it)")
     ap.add_argument("--learned-extras", type=int, default=1,
                     help="1: also run the other fallback policy and the specified 1/4 window (0: skip them)")
+    ap.add_argument("--batch-learned", type=int, default=1,
+                    help="batch256: also run the batch with the learned proposal on its tet meshes (0: skip)")
     ap.add_argument("--tune-log", default=os.environ.get("VSDMO_TUNE_LOG", "autotune_batch256.jsonl"))
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
