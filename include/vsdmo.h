@@ -240,6 +240,8 @@ typedef struct vsdmo_dataset vsdmo_dataset;
 
 int32_t vsdmo_mlp_num_params(void);   /* 29,315 FP32 parameters: W1[128x96] b1 W2[128x128] b2 W3[3x128] b3 */
 vsdmo_status vsdmo_mlp_create(uint64_t seed, vsdmo_mlp** out);
+/* refuses (VSDMO_INVALID_ARG) while an optimiser or batch has the network as
+ * its proposal: set the proposal to mode 0 or destroy those first */
 vsdmo_status vsdmo_mlp_destroy(vsdmo_mlp* m);
 vsdmo_status vsdmo_mlp_get_params(vsdmo_mlp* m, float* params);
 vsdmo_status vsdmo_mlp_set_params(vsdmo_mlp* m, const float* params);   /* also resets Adam */
@@ -343,9 +345,9 @@ vsdmo_status vsdmo_batch_get_stats(vsdmo_batch* b, vsdmo_batch_stats* out);
  * vsdmo_opt of the mesh with fixed_boundary = 1 and the same proposal.
  * mode 1 runs the network inside the window kernel and needs every tet ring
  * to fit its LDS tile (VSDMO_NOT_SUPPORTED otherwise); mode 0 switches it
- * off.  The network is read at every iteration: it must outlive the batch
- * or the proposal must be switched off first.  The autotuner refuses while
- * a proposal is set. */
+ * off.  The network is read at every iteration: vsdmo_mlp_destroy refuses
+ * while the batch holds it.  The autotuner refuses while a proposal is
+ * set. */
 vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac);
 /* learned visits so far and how many of them took the full search */
 vsdmo_status vsdmo_batch_learned_stats(vsdmo_batch* b, int64_t* visits, int64_t* fallbacks);

@@ -125,3 +125,27 @@ def test_batch_proposal_off_and_refusals():
     plain.close()
     b.close()
     mlp.close()
+
+
+def test_network_outlives_its_users():
+    """vsdmo_mlp_destroy refuses while an optimiser or batch holds the network
+    as its proposal, and succeeds once the proposal is off or the holder is
+    destroyed."""
+    mlp = _network()
+    xyz, cells = synth.tet_cube(5, seed=47)
+    m = DMOMeshTetFactory.create(xyz, cells)
+    o = DMOTetClass(m, n_iter=1, fixed_boundary=True)
+    o.set_proposal(mlp, mode=1)
+    b = DMOBatch([DMOMeshTetFactory.create(xyz, cells)])
+    b.set_proposal(mlp, mode=1)
+    with pytest.raises(VsdmoError):
+        mlp.close()
+    o.set_proposal(None, mode=0)
+    with pytest.raises(VsdmoError):
+        mlp.close()               # still held by the batch
+    b.close()
+    mlp.close()
+    # a proposal switched off leaves a working plain optimiser
+    o.iterate(1)
+    o.close()
+    m.close()

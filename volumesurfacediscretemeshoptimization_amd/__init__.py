@@ -347,6 +347,7 @@ class _DMOBase:
         h = mlp.handle if mlp is not None else None
         check(self._lib.vsdmo_opt_set_proposal(self._h, h, int(mode), int(depth), float(window_frac)),
               "vsdmo_opt_set_proposal")
+        self._mlp = mlp if mode == 1 else None   # the network is destroyed after the optimiser
 
     FALLBACK = {"immediate": 0, "deferred": 1}
 
@@ -585,7 +586,7 @@ class DMOBatch:
         h = mlp.handle if mlp is not None else None
         check(self._lib.vsdmo_batch_set_proposal(self._h, h, int(mode), int(depth), float(window_frac)),
               "vsdmo_batch_set_proposal")
-        self._mlp = mlp if mode == 1 else None   # keeps the network alive
+        self._mlp = mlp if mode == 1 else None   # the network is destroyed after the batch
 
     def learned_stats(self) -> dict:
         v, f = C.c_int64(), C.c_int64()

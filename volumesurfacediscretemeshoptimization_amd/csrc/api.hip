@@ -854,30 +854,34 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
         if (mode != 0 && m->maxRing > learned_ring_cap(m->elem))
             return fail(VSDMO_NOT_SUPPORTED, "the learned search holds one-rings of at most " +
                                                  std::to_string(learned_ring_cap(m->elem)) + " elements");
+        if (mode != 0) {
+            // buffers first: a failure here leaves the previous setting in place
+            int maxc = 1;
+            for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c)
+                maxc = std::max(maxc, m->colOffInner[c + 1] - m->colOffInner[c]);
+            if (!o->lFallback.p) {
+                o->lFallback.alloc(1);
+                launch_fill_int(o->lFallback.p, 1, 0, m->stream);
+            }
+            if ((int)o->lL.n < maxc) {
+                o->lL.alloc(maxc); o->lY.alloc((size_t)3 * maxc); o->lList.alloc(maxc);
+            }
+            if (!o->lCnt.p) o->lCnt.alloc(std::max<size_t>(m->colOffInner.size(), 1));
+            reset_deferred(o);
+            if (mode == 1) {
+                build_neighbour_table(*m);
+                VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
+            }
+            VSDMO_HIP(hipStreamSynchronize(m->stream));
+        }
         o->learn_mode = mode;
+        if (o->mlp) o->mlp->n_users--;
         o->mlp = mode == 1 ? mlp : nullptr;
+        if (o->mlp) o->mlp->n_users++;
         o->learn_depth = depth;
         o->learn_frac = window_frac;
         const char* sep = std::getenv("VSDMO_LEARN_SEPARATE_INFERENCE");
         o->learn_separate = sep && sep[0] && sep[0] != '0';
-        if (mode == 0) return VSDMO_OK;
-        int maxc = 1;
-        for (size_t c = 0; c + 1 < m->colOffInner.size(); ++c)
-            maxc = std::max(maxc, m->colOffInner[c + 1] - m->colOffInner[c]);
-        if (!o->lFallback.p) {
-            o->lFallback.alloc(1);
-            launch_fill_int(o->lFallback.p, 1, 0, m->stream);
-        }
-        if ((int)o->lL.n < maxc) {
-            o->lL.alloc(maxc); o->lY.alloc((size_t)3 * maxc); o->lList.alloc(maxc);
-        }
-        if (!o->lCnt.p) o->lCnt.alloc(std::max<size_t>(m->colOffInner.size(), 1));
-        reset_deferred(o);
-        if (mode == 1) {
-            build_neighbour_table(*m);
-            VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
-        }
-        VSDMO_HIP(hipStreamSynchronize(m->stream));
         return VSDMO_OK;
     });
 }
@@ -926,6 +930,7 @@ vsdmo_status vsdmo_opt_destroy(vsdmo_opt* o)
         collect_timing(o);
         for (hipEvent_t e : o->ev) (void)hipEventDestroy(e);
         o->m->n_opts--;
+        if (o->mlp) o->mlp->n_users--;
         delete o;
         return VSDMO_OK;
     });

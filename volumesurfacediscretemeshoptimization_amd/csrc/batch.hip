@@ -657,36 +657,40 @@ vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, vsdmo_mlp* mlp, int32_t mo
         hipStream_t s = b->stream;
         VSDMO_HIP(hipStreamSynchronize(s));
         drop_graph(b);   // the recorded launches change
+        if (mode != 0) {
+            // buffers first: a failure here leaves the previous setting in place
+            const int nR = (int)b->grid.size();
+            int maxTet = 1;
+            for (int r = 0; r < nR; ++r) maxTet = std::max(maxTet, b->roundMidT[r] - b->roundOff[r]);
+            if ((int)b->lList.n < maxTet) b->lList.alloc(maxTet);
+            if (!b->lCnt.p) b->lCnt.alloc(std::max(nR, 1));
+            if (!b->lFallback.p) {
+                b->lFallback.alloc(1);
+                launch_fill_int(b->lFallback.p, 1, 0, s);
+            }
+            if (mode == 1 && !b->nbr.p) {
+                std::vector<int> t((size_t)b->totalV * kNbr, -1);
+                for (size_t i = 0; i < b->meshes.size(); ++i) {
+                    vsdmo_mesh* m = b->meshes[i];
+                    if (m->elem != VSDMO_TET) continue;
+                    build_neighbour_table(*m);
+                    std::vector<int> loc = m->nbr.to_host(m->stream);
+                    const int off = b->vOff[i];
+                    int* dst = t.data() + (size_t)off * kNbr;
+                    for (size_t k = 0; k < loc.size(); ++k) dst[k] = loc[k] >= 0 ? loc[k] + off : -1;
+                }
+                b->nbr.alloc(t.size());
+                b->nbr.upload(t.data(), t.size(), s);
+            }
+            if (mode == 1) VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
+            VSDMO_HIP(hipStreamSynchronize(s));
+        }
         b->learn_mode = mode;
+        if (b->mlp) b->mlp->n_users--;
         b->mlp = mode == 1 ? mlp : nullptr;
+        if (b->mlp) b->mlp->n_users++;
         b->learn_depth = depth;
         b->learn_frac = window_frac;
-        if (mode == 0) return VSDMO_OK;
-        const int nR = (int)b->grid.size();
-        int maxTet = 1;
-        for (int r = 0; r < nR; ++r) maxTet = std::max(maxTet, b->roundMidT[r] - b->roundOff[r]);
-        if ((int)b->lList.n < maxTet) b->lList.alloc(maxTet);
-        if (!b->lCnt.p) b->lCnt.alloc(std::max(nR, 1));
-        if (!b->lFallback.p) {
-            b->lFallback.alloc(1);
-            launch_fill_int(b->lFallback.p, 1, 0, s);
-        }
-        if (mode == 1 && !b->nbr.p) {
-            std::vector<int> t((size_t)b->totalV * kNbr, -1);
-            for (size_t i = 0; i < b->meshes.size(); ++i) {
-                vsdmo_mesh* m = b->meshes[i];
-                if (m->elem != VSDMO_TET) continue;
-                build_neighbour_table(*m);
-                std::vector<int> loc = m->nbr.to_host(m->stream);
-                const int off = b->vOff[i];
-                int* dst = t.data() + (size_t)off * kNbr;
-                for (size_t k = 0; k < loc.size(); ++k) dst[k] = loc[k] >= 0 ? loc[k] + off : -1;
-            }
-            b->nbr.alloc(t.size());
-            b->nbr.upload(t.data(), t.size(), s);
-        }
-        if (mode == 1) VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
-        VSDMO_HIP(hipStreamSynchronize(s));
         return VSDMO_OK;
     });
 }
@@ -854,6 +858,7 @@ vsdmo_status vsdmo_batch_destroy(vsdmo_batch* b)
         (void)hipStreamDestroy(b->stream);
     }
     for (vsdmo_mesh* m : b->meshes) m->n_opts--;
+    if (b->mlp) b->mlp->n_users--;
     delete b;
     return VSDMO_OK;
 }

@@ -495,6 +495,8 @@ vsdmo_status vsdmo_mlp_create(uint64_t seed, vsdmo_mlp** out)
 vsdmo_status vsdmo_mlp_destroy(vsdmo_mlp* m)
 {
     if (!m) return VSDMO_OK;
+    if (m->n_users > 0)
+        return fail_m(VSDMO_INVALID_ARG, "network still set as the proposal of an optimiser or batch");
     if (m->stream) {
         (void)hipStreamSynchronize(m->stream);
         (void)hipStreamDestroy(m->stream);

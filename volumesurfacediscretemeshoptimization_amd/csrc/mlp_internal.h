@@ -223,6 +223,7 @@ struct vsdmo_mlp {
     vsdmo::DevBuf<uint16_t> wb;          // bf16 copies of the packed parameters
     int step = 0;
     int cap = 0;                         // batch capacity of the work buffers
+    int n_users = 0;                     // optimisers and batches with this network as their proposal
     vsdmo::DevBuf<uint16_t> X, H1, H2, dY16, dZ1, dZ2;
     vsdmo::DevBuf<float> Z1, Z2, Y, T, part;
     vsdmo::DevBuf<double> lossd;
