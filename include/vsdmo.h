@@ -338,17 +338,19 @@ vsdmo_status vsdmo_batch_set_params(vsdmo_batch* b, const vsdmo_batch_params* p)
 vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, const char* log_path,
                                   vsdmo_batch_params* chosen);
 vsdmo_status vsdmo_batch_get_stats(vsdmo_batch* b, vsdmo_batch_stats* out);
-/* Learned proposal for the batch's tet vertices (no counterpart in the
- * reference; the rule of vsdmo_opt_set_proposal with the immediate fallback
- * policy, applied per colour round to the round's tet items; hex and
- * triangle items keep the plain search).  Each tet mesh's positions equal a
- * vsdmo_opt of the mesh with fixed_boundary = 1 and the same proposal.
- * mode 1 runs the network inside the window kernel and needs every tet ring
- * to fit its LDS tile (VSDMO_NOT_SUPPORTED otherwise); mode 0 switches it
- * off.  The network is read at every iteration: vsdmo_mlp_destroy refuses
- * while the batch holds it.  The autotuner refuses while a proposal is
- * set. */
-vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac);
+/* Learned proposal for the batch's tet or hex vertices, elem = VSDMO_TET or
+ * VSDMO_HEX, each type with its own network and window (no counterpart in
+ * the reference; the rule of vsdmo_opt_set_proposal with the immediate
+ * fallback policy, applied per colour round to the round's items of that
+ * type; the other items keep the plain search).  Each such mesh's positions
+ * equal a vsdmo_opt of the mesh with fixed_boundary = 1 and the same
+ * proposal.  mode 1 runs the network inside the window kernel and needs
+ * every ring of the type to fit its LDS tile (VSDMO_NOT_SUPPORTED
+ * otherwise); mode 0 switches the type's proposal off.  The network is read
+ * at every iteration: vsdmo_mlp_destroy refuses while the batch holds it.
+ * The autotuner refuses while a proposal is set. */
+vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* mlp, int32_t mode, int32_t depth,
+                                      float window_frac);
 /* learned visits so far and how many of them took the full search */
 vsdmo_status vsdmo_batch_learned_stats(vsdmo_batch* b, int64_t* visits, int64_t* fallbacks);
 vsdmo_status vsdmo_batch_destroy(vsdmo_batch* b);

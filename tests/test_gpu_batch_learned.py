@@ -1,11 +1,12 @@
 """GPU tests of the learned proposal inside the batched sweep.
 
-Bars: every tet mesh of a batch ends bit-exact where one DMOTetClass per
-mesh (boundary fixed) with the same network, window and the immediate
-fallback policy ends, with the same fallback count; triangle and hex meshes
-of the same batch end where their plain optimisers end; graph replays and
-shuffled work orders equal eager launches in mesh order; switching the
-proposal off gives plain searches again.
+Bars: every tet (hex) mesh of a batch with a tet (hex) proposal ends
+bit-exact where one DMOTetClass (DMOHexClass) per mesh (boundary fixed)
+with the same network, window and the immediate fallback policy ends, with
+the same fallback count; the other meshes of the same batch end where their
+plain optimisers end; graph replays and shuffled work orders equal eager
+launches in mesh order; switching the proposal off gives plain searches
+again.
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ import pytest
 from volumesurfacediscretemeshoptimization_amd import (DMOBatch, DMOHexClass, DMOMeshHexFactory, DMOMeshTetFactory,
                                                        DMOMeshTriFactory, DMOMLP, DMOTetClass, DMOTriFlatClass,
                                                        VsdmoError)
-from volumesurfacediscretemeshoptimization_amd import synth
+from volumesurfacediscretemeshoptimization_amd import _lib, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -35,57 +36,66 @@ def _meshes(inputs):
     return [FAC[k].create(xyz, cells) for k, xyz, cells in inputs]
 
 
-def _network():
-    mlp = DMOMLP(seed=21)
+ELEM = {"tet": _lib.VSDMO_TET, "hex": _lib.VSDMO_HEX}
+
+
+def _network(seed=21, b3=(0.05, -0.03, 0.02)):
+    mlp = DMOMLP(seed=seed)
     p = mlp.params()
-    p[-3:] = [0.05, -0.03, 0.02]         # b3: a nonzero mean displacement
+    p[-3:] = b3                          # a nonzero mean displacement
     mlp.set_params(p)
     return mlp
 
 
-@pytest.mark.parametrize("mode,depth,frac", [(1, 1, 0.25), (1, 2, 0.025), (2, 2, 0.25)])
-def test_batch_learned_matches_single_meshes(mode, depth, frac):
-    mlp = _network() if mode == 1 else None
+@pytest.mark.parametrize("types,mode,depth,frac", [(("tet",), 1, 1, 0.25), (("tet",), 1, 2, 0.025),
+                                                   (("tet",), 2, 2, 0.25), (("hex",), 1, 1, 2.0),
+                                                   (("tet", "hex"), 1, 1, 0.25), (("tet", "hex"), 2, 1, 0.5)])
+def test_batch_learned_matches_single_meshes(types, mode, depth, frac):
+    nets = {k: (_network(seed=21 + i, b3=(0.05, -0.03, 0.02 + 0.01 * i)) if mode == 1 else None)
+            for i, k in enumerate(types)}
     inputs = _inputs()
     iters = 3
-    ref, falls, tet_inner = [], 0, 0
+    ref, falls, visits = [], 0, 0
     for k, xyz, cells in inputs:
         m = FAC[k].create(xyz, cells)
         o = CLS[k](m, n_iter=iters, fixed_boundary=True)
-        if k == "tet":
-            o.set_proposal(mlp, mode=mode, depth=depth, window_frac=frac, fallback="immediate")
-            tet_inner += int(np.diff(m.color_offsets("inner")).sum())
+        if k in types:
+            o.set_proposal(nets[k], mode=mode, depth=depth, window_frac=frac, fallback="immediate")
+            visits += iters * int(np.diff(m.color_offsets("inner")).sum())
         o.iterate(iters)
-        if k == "tet":
+        if k in types:
             falls += o.stats()["fallbacks"]
         ref.append((m.positions(original_order=False), o.getQuality()))
         o.close()
         m.close()
     meshes = _meshes(inputs)
     b = DMOBatch(meshes)
-    b.set_proposal(mlp, mode=mode, depth=depth, window_frac=frac)
+    for k in types:
+        b.set_proposal(nets[k], mode=mode, depth=depth, window_frac=frac, elem=ELEM[k])
     b.iterate(iters)
     q = b.min_quality()
     for i, (p, qm) in enumerate(ref):
         assert np.array_equal(b.positions(i), p), (inputs[i][0], i)
         assert q[i] == qm, (i, q[i], qm)
     ls = b.learned_stats()
-    assert ls["learned_visits"] == iters * tet_inner
-    assert ls["fallbacks"] == falls and 0 < falls < iters * tet_inner, (ls, falls)
+    assert ls["learned_visits"] == visits
+    assert ls["fallbacks"] == falls and falls < visits, (ls, falls)
     b.close()
-    if mlp is not None:
-        mlp.close()
+    for n in nets.values():
+        if n is not None:
+            n.close()
 
 
 def test_batch_learned_graph_and_order_independence():
     """Graph replays (interleaved with host reads) and a shuffled work order
     give the eager, mesh-order positions after every iteration."""
-    mlp = _network()
+    mlp, mlp_h = _network(), _network(seed=23)
     inputs = _inputs()
     eager = DMOBatch(_meshes(inputs), use_graph=False)
     graph = DMOBatch(_meshes(inputs), use_graph=True, shuffle_seed=777, waves_per_vertex=2)
     for b in (eager, graph):
         b.set_proposal(mlp, mode=1)
+        b.set_proposal(mlp_h, mode=1, elem=_lib.VSDMO_HEX)
     for it in range(3):
         eager.iterate(1)
         graph.iterate(1)
@@ -100,6 +110,7 @@ def test_batch_learned_graph_and_order_independence():
     eager.close()
     graph.close()
     mlp.close()
+    mlp_h.close()
 
 
 def test_batch_proposal_off_and_refusals():
@@ -117,6 +128,12 @@ def test_batch_proposal_off_and_refusals():
         b.set_proposal(None, mode=1)
     with pytest.raises(VsdmoError):
         b.set_proposal(mlp, mode=1, depth=0)
+    with pytest.raises(VsdmoError):
+        b.set_proposal(mlp, mode=1, elem=_lib.VSDMO_TRI)
+    b.set_proposal(mlp, mode=2, elem=_lib.VSDMO_HEX)
+    with pytest.raises(VsdmoError):
+        b.autotune(iters=1)
+    b.set_proposal(None, mode=0, elem=_lib.VSDMO_HEX)
     b.set_proposal(None, mode=0)
     b.iterate(2)
     for i in range(len(inputs)):

@@ -573,20 +573,25 @@ class DMOBatch:
         check(self._lib.vsdmo_batch_get_stats(self._h, C.byref(s)), "vsdmo_batch_get_stats")
         return {name: getattr(s, name) for name, _ in _lib.BatchStats._fields_}
 
-    def set_proposal(self, mlp=None, mode: int = 1, depth: int = None, window_frac: float = None):
-        """Learned proposal for the tet meshes' vertices (immediate fallback:
-        a vertex whose window finds nothing better takes the full search in
-        the same round); hex and triangle meshes keep the plain search.  Each
-        tet mesh gets the positions of a DMOTetClass with fixed_boundary=True
-        and the same proposal.  Defaults: the tet window of DMOTetClass.  The
-        network must stay alive while the proposal is set."""
-        d0, f0 = _DMOBase.LEARN_WINDOW[_lib.VSDMO_TET]
+    def set_proposal(self, mlp=None, mode: int = 1, depth: int = None, window_frac: float = None,
+                     elem: int = _lib.VSDMO_TET):
+        """Learned proposal for the vertices of the batch's meshes of type
+        elem (VSDMO_TET or VSDMO_HEX; each type has its own network and
+        window), immediate fallback: a vertex whose window finds nothing
+        better takes the full search in the same round.  The other meshes
+        keep the plain search.  Each such mesh gets the positions of a
+        DMOTetClass / DMOHexClass with fixed_boundary=True and the same
+        proposal.  Defaults: the type's window in DMOTetClass /
+        DMOHexClass."""
+        d0, f0 = _DMOBase.LEARN_WINDOW.get(elem, (1, 0.025))
         depth = d0 if depth is None else depth
         window_frac = f0 if window_frac is None else window_frac
         h = mlp.handle if mlp is not None else None
-        check(self._lib.vsdmo_batch_set_proposal(self._h, h, int(mode), int(depth), float(window_frac)),
+        check(self._lib.vsdmo_batch_set_proposal(self._h, int(elem), h, int(mode), int(depth), float(window_frac)),
               "vsdmo_batch_set_proposal")
-        self._mlp = mlp if mode == 1 else None   # the network is destroyed after the batch
+        if not hasattr(self, "_mlps"):
+            self._mlps = {}
+        self._mlps[int(elem)] = mlp if mode == 1 else None   # networks are destroyed after the batch
 
     def learned_stats(self) -> dict:
         v, f = C.c_int64(), C.c_int64()

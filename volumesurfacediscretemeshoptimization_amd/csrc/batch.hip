@@ -86,11 +86,15 @@ struct vsdmo_batch {
     long long searches = 0;
     int iterations = 0;
     double gpu_ms = 0.0;
-    // learned proposal of the tet items (immediate fallback): the network,
-    // the global neighbour table, each round's fallback list and its length
-    vsdmo_mlp* mlp = nullptr;
-    int learn_mode = 0, learn_depth = 1;
-    float learn_frac = 0.025f;
+    // learned proposal of the tet items ([0]) and the hex items ([1]),
+    // immediate fallback: network and window per type, the global neighbour
+    // table, the fallback list (reused by each learned launch in stream
+    // order), the list lengths (tets: [0, rounds), hexes: [rounds, 2 rounds))
+    struct Learn {
+        vsdmo_mlp* mlp = nullptr;
+        int mode = 0, depth = 1;
+        float frac = 0.025f;
+    } learn[2];
     DevBuf<int> nbr, lList, lCnt, lFallback;
     long long learned = 0;
 };
@@ -172,31 +176,51 @@ void size_grids(vsdmo_batch* b)
     }
 }
 
+bool learning(const vsdmo_batch* b) { return b->learn[0].mode != 0 || b->learn[1].mode != 0; }
+
+// one type's items [lo, hi) of a round through the learned window kernel,
+// then the full search of their fallbacks (immediate policy)
+void launch_learned_items(vsdmo_batch* b, int t, const SearchArgs& a, int lo, int hi, int* cnt, hipStream_t s)
+{
+    const vsdmo_batch::Learn& L = b->learn[t];
+    if (L.mode == 0 || hi <= lo) return;
+    SearchArgs al = a;
+    al.cStart = lo; al.cEnd = hi;
+    if (t == 1) {   // hexes: their CSR, the minimum-extent window (as the batch's plain hex search)
+        al.ringRow = b->rowHex.p; al.ringVal = b->ringHex.p; al.extentMin = 1;
+    }
+    LearnNet net;
+    net.items = b->items.p;
+    if (L.mode == 1) { net.wb = L.mlp->wb.p; net.w = L.mlp->w.p; net.nbr = b->nbr.p; }
+    launch_search_learned(t == 0 ? VSDMO_TET : VSDMO_HEX, b->p.nq, al, net, L.mode, L.depth, L.frac, b->lFallback.p,
+                          b->lList.p, cnt, t == 0 ? b->maxRingTet : b->maxRingHex, nullptr, s);
+}
+
 // the launches of one iteration, in stream order
 void record_iteration(vsdmo_batch* b)
 {
     hipStream_t s = b->stream;
     const int nR = (int)b->grid.size();
     launch_fill_int(b->ctr.p, nR, 0, s);
-    const bool learn = b->learn_mode != 0;
-    if (learn) launch_fill_int(b->lCnt.p, nR, 0, s);
+    if (learning(b)) launch_fill_int(b->lCnt.p, 2 * nR, 0, s);
     SearchArgs a = search_args(b);
     for (int r = 0; r < nR; ++r) {
-        int lo = b->roundOff[r];
-        if (learn && b->roundMidT[r] > lo) {
-            // the round's tet items: proposal windows, then the full search of
-            // their fallbacks (immediate policy); hexes and triangles follow
-            SearchArgs al = a;
-            al.cStart = lo; al.cEnd = b->roundMidT[r];
-            LearnNet net;
-            net.items = b->items.p;
-            if (b->learn_mode == 1) { net.wb = b->mlp->wb.p; net.w = b->mlp->w.p; net.nbr = b->nbr.p; }
-            launch_search_learned(VSDMO_TET, b->p.nq, al, net, b->learn_mode, b->learn_depth, b->learn_frac,
-                                  b->lFallback.p, b->lList.p, b->lCnt.p + r, b->maxRingTet, nullptr, s);
-            lo = b->roundMidT[r];
+        const int off = b->roundOff[r], midT = b->roundMidT[r], mid = b->roundMid[r], hi = b->roundOff[r + 1];
+        launch_learned_items(b, 0, a, off, midT, b->lCnt.p + r, s);
+        launch_learned_items(b, 1, a, midT, mid, b->lCnt.p + nR + r, s);
+        // plain searches of the other items: tets [off, midT) and hexes
+        // [midT, mid) from the round's queue, flat triangles [mid, hi) strided
+        const bool lt = b->learn[0].mode != 0, lh = b->learn[1].mode != 0;
+        auto plain = [&](int lo, int qend, int end) {
+            launch_batch_round(b->p.nq, b->p.waves_per_vertex, a, b->rowHex.p, b->ringHex.p, b->items.p, lo, qend, end,
+                               b->ctr.p + r, b->grid[r], b->lds, s);
+        };
+        if (lh && !lt) {   // hexes learned, tets plain: the tets, then the triangles
+            plain(off, midT, midT);
+            plain(mid, mid, hi);
+        } else {
+            plain(!lt ? off : (lh ? mid : midT), mid, hi);
         }
-        launch_batch_round(b->p.nq, b->p.waves_per_vertex, a, b->rowHex.p, b->ringHex.p, b->items.p, lo,
-                           b->roundMid[r], b->roundOff[r + 1], b->ctr.p + r, b->grid[r], b->lds, s);
     }
     launch_batch_quality(quality_args(b), s);
 }
@@ -262,10 +286,9 @@ vsdmo_status run(vsdmo_batch* b, int n, std::vector<float>* rows_out)
     b->hrow += n;
     b->iterations += n;
     b->searches += (long long)n * (long long)b->itemsMeshOrder.size();
-    if (b->learn_mode != 0) {
-        long long tets = 0;
-        for (size_t r = 0; r + 1 < b->roundOff.size(); ++r) tets += b->roundMidT[r] - b->roundOff[r];
-        b->learned += (long long)n * tets;
+    for (size_t r = 0; r + 1 < b->roundOff.size(); ++r) {
+        if (b->learn[0].mode != 0) b->learned += (long long)n * (b->roundMidT[r] - b->roundOff[r]);
+        if (b->learn[1].mode != 0) b->learned += (long long)n * (b->roundMid[r] - b->roundMidT[r]);
     }
     vsdmo_status st = VSDMO_OK;
     std::string msg;
@@ -633,64 +656,72 @@ vsdmo_status vsdmo_batch_set_params(vsdmo_batch* b, const vsdmo_batch_params* p)
     });
 }
 
-// Learned proposal for the tet items (immediate fallback: a round's windows,
-// then the full search of the vertices whose window does not improve them),
-// the single-mesh rule of vsdmo_opt_set_proposal over a round's tet items.
-// The network runs inside the window kernel, so mode 1 needs every tet ring
-// to fit its LDS tile.  The neighbour table is global: each tet mesh's table
-// with the mesh's vertex offset added.
-vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac)
+// Learned proposal for the tet or hex items (immediate fallback: a round's
+// windows, then the full search of the vertices whose window does not
+// improve them), the single-mesh rule of vsdmo_opt_set_proposal over a
+// round's items of that type; each type has its own network and window.
+// The network runs inside the window kernel, so mode 1 needs every ring of
+// the type to fit its LDS tile.  The neighbour table is global: each tet and
+// hex mesh's table with the mesh's vertex offset added, built once.
+vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* mlp, int32_t mode, int32_t depth,
+                                      float window_frac)
 {
     return guarded_b([&]() -> vsdmo_status {
         if (!b) return fail_status(VSDMO_INVALID_ARG, "NULL batch");
+        if (elem != VSDMO_TET && elem != VSDMO_HEX)
+            return fail_status(VSDMO_NOT_SUPPORTED, "the learned proposal is for the tet and hex meshes of a batch");
         if (mode < 0 || mode > 2) return fail_status(VSDMO_INVALID_ARG, "mode must be 0 (off), 1 (network) or 2 (identity)");
         if (mode == 1 && !mlp) return fail_status(VSDMO_INVALID_ARG, "mode 1 needs a network");
         if (mode != 0 && (depth < 1 || depth > 8 || !(window_frac > 0.0f)))
             return fail_status(VSDMO_INVALID_ARG, "depth must be in [1,8] and window_frac > 0");
-        if (mode == 1 && !learned_fused_ok(VSDMO_TET, b->maxRingTet))
+        const int t = elem == VSDMO_TET ? 0 : 1;
+        const int ring = t == 0 ? b->maxRingTet : b->maxRingHex;
+        if (mode == 1 && !learned_fused_ok(elem, ring))
             return fail_status(VSDMO_NOT_SUPPORTED, "the batched learned search runs the network in the window "
-                                                    "kernel, whose LDS tile of 16 rings does not hold tet one-rings of " +
-                                                        std::to_string(b->maxRingTet) + " elements");
-        if (mode == 2 && b->maxRingTet > learned_ring_cap(VSDMO_TET))
-            return fail_status(VSDMO_NOT_SUPPORTED, "the learned search holds tet one-rings of at most " +
-                                                        std::to_string(learned_ring_cap(VSDMO_TET)) + " elements");
+                                                    "kernel, whose LDS tile of 16 rings does not hold one-rings of " +
+                                                        std::to_string(ring) + " elements");
+        if (mode == 2 && ring > learned_ring_cap(elem))
+            return fail_status(VSDMO_NOT_SUPPORTED, "the learned search holds one-rings of at most " +
+                                                        std::to_string(learned_ring_cap(elem)) + " elements");
         hipStream_t s = b->stream;
         VSDMO_HIP(hipStreamSynchronize(s));
         drop_graph(b);   // the recorded launches change
         if (mode != 0) {
             // buffers first: a failure here leaves the previous setting in place
             const int nR = (int)b->grid.size();
-            int maxTet = 1;
-            for (int r = 0; r < nR; ++r) maxTet = std::max(maxTet, b->roundMidT[r] - b->roundOff[r]);
-            if ((int)b->lList.n < maxTet) b->lList.alloc(maxTet);
-            if (!b->lCnt.p) b->lCnt.alloc(std::max(nR, 1));
+            int maxItems = 1;
+            for (int r = 0; r < nR; ++r)
+                maxItems = std::max({ maxItems, b->roundMidT[r] - b->roundOff[r], b->roundMid[r] - b->roundMidT[r] });
+            if ((int)b->lList.n < maxItems) b->lList.alloc(maxItems);
+            if (!b->lCnt.p) b->lCnt.alloc(std::max(2 * nR, 1));
             if (!b->lFallback.p) {
                 b->lFallback.alloc(1);
                 launch_fill_int(b->lFallback.p, 1, 0, s);
             }
             if (mode == 1 && !b->nbr.p) {
-                std::vector<int> t((size_t)b->totalV * kNbr, -1);
+                std::vector<int> tab((size_t)b->totalV * kNbr, -1);
                 for (size_t i = 0; i < b->meshes.size(); ++i) {
                     vsdmo_mesh* m = b->meshes[i];
-                    if (m->elem != VSDMO_TET) continue;
+                    if (m->elem != VSDMO_TET && m->elem != VSDMO_HEX) continue;
                     build_neighbour_table(*m);
                     std::vector<int> loc = m->nbr.to_host(m->stream);
                     const int off = b->vOff[i];
-                    int* dst = t.data() + (size_t)off * kNbr;
+                    int* dst = tab.data() + (size_t)off * kNbr;
                     for (size_t k = 0; k < loc.size(); ++k) dst[k] = loc[k] >= 0 ? loc[k] + off : -1;
                 }
-                b->nbr.alloc(t.size());
-                b->nbr.upload(t.data(), t.size(), s);
+                b->nbr.alloc(tab.size());
+                b->nbr.upload(tab.data(), tab.size(), s);
             }
             if (mode == 1) VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
             VSDMO_HIP(hipStreamSynchronize(s));
         }
-        b->learn_mode = mode;
-        if (b->mlp) b->mlp->n_users--;
-        b->mlp = mode == 1 ? mlp : nullptr;
-        if (b->mlp) b->mlp->n_users++;
-        b->learn_depth = depth;
-        b->learn_frac = window_frac;
+        vsdmo_batch::Learn& L = b->learn[t];
+        L.mode = mode;
+        if (L.mlp) L.mlp->n_users--;
+        L.mlp = mode == 1 ? mlp : nullptr;
+        if (L.mlp) L.mlp->n_users++;
+        L.depth = depth;
+        L.frac = window_frac;
         return VSDMO_OK;
     });
 }
@@ -722,8 +753,8 @@ vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, cons
     return guarded_b([&]() -> vsdmo_status {
         if (!b) return fail_status(VSDMO_INVALID_ARG, "NULL batch");
         if (iters <= 0 || !(tol >= 0.0f)) return fail_status(VSDMO_INVALID_ARG, "iters must be > 0 and tol >= 0");
-        if (b->learn_mode != 0)
-            return fail_status(VSDMO_NOT_SUPPORTED, "the autotuner runs plain searches: set the proposal to mode 0 first");
+        if (learning(b))
+            return fail_status(VSDMO_NOT_SUPPORTED, "the autotuner runs plain searches: set the proposals to mode 0 first");
         const size_t nm = b->meshes.size();
         const size_t V = (size_t)b->totalV;
         // snapshot of the start positions, guard state and counters
@@ -858,7 +889,8 @@ vsdmo_status vsdmo_batch_destroy(vsdmo_batch* b)
         (void)hipStreamDestroy(b->stream);
     }
     for (vsdmo_mesh* m : b->meshes) m->n_opts--;
-    if (b->mlp) b->mlp->n_users--;
+    for (auto& L : b->learn)
+        if (L.mlp) L.mlp->n_users--;
     delete b;
     return VSDMO_OK;
 }
