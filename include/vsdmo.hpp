@@ -149,6 +149,48 @@ private:
     vsdmo_mlp* h_ = nullptr;
 };
 
+// Batched DMO over many meshes (no counterpart in the reference): each mesh
+// ends where one optimiser per mesh with fixed_boundary = 1 ends.  The
+// meshes, and the networks set as proposals, must outlive the batch.
+class Batch {
+public:
+    explicit Batch(const std::vector<DMOMesh*>& meshes, const vsdmo_batch_params* prm = nullptr)
+    {
+        std::vector<vsdmo_mesh*> h;
+        for (DMOMesh* m : meshes) h.push_back(m->handle());
+        check(vsdmo_batch_create(h.data(), (int32_t)h.size(), prm, &h_), "vsdmo_batch_create");
+        n_ = h.size();
+    }
+    ~Batch() { if (h_) vsdmo_batch_destroy(h_); }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+    void iterate(int n) { check(vsdmo_batch_iterate(h_, n), "vsdmo_batch_iterate"); }
+    std::vector<float> minQuality() const
+    {
+        std::vector<float> q(n_);
+        check(vsdmo_batch_min_quality(h_, q.data()), "vsdmo_batch_min_quality");
+        return q;
+    }
+    void writeback() { check(vsdmo_batch_writeback(h_), "vsdmo_batch_writeback"); }
+    vsdmo_batch_params autotune(int iters = 3, float tol = 1e-3f, const char* log = "")
+    {
+        vsdmo_batch_params p;
+        check(vsdmo_batch_autotune(h_, iters, tol, log, &p), "vsdmo_batch_autotune");
+        return p;
+    }
+    // learned proposal of the batch's VSDMO_TET or VSDMO_HEX meshes
+    void setProposal(int32_t elem, MLP* mlp, int mode, int depth, float windowFrac)
+    {
+        check(vsdmo_batch_set_proposal(h_, elem, mlp ? mlp->handle() : nullptr, mode, depth, windowFrac),
+              "vsdmo_batch_set_proposal");
+    }
+    vsdmo_batch* handle() const { return h_; }
+
+private:
+    vsdmo_batch* h_ = nullptr;
+    size_t n_ = 0;
+};
+
 }  // namespace vsdmo_cpp
 
 namespace DMO {

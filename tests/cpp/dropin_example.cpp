@@ -63,6 +63,16 @@ int main()
         std::printf("learned: %lld pairs, loss %g -> %g, min quality %g -> %g\n", (long long)ds.size(), loss.front(),
                     loss.back(), q0, learned.getQuality());
         if (!(learned.getQuality() >= q0)) return 2;
+        // the batch (no counterpart in the reference) over two copies of the
+        // cube, the network as the proposal of its tet meshes
+        auto b0 = vsdmo_cpp::DMOMeshTetFactory::create(xyz, tets);
+        auto b1 = vsdmo_cpp::DMOMeshTetFactory::create(xyz, tets);
+        vsdmo_cpp::Batch batch({ b0.get(), b1.get() });
+        batch.setProposal(VSDMO_TET, &mlp, 1, 1, 0.025f);
+        batch.iterate(3);
+        const std::vector<float> bq = batch.minQuality();
+        std::printf("batch: min quality %g %g\n", bq[0], bq[1]);
+        if (!(bq[0] == bq[1] && bq[0] >= q0)) return 3;
     } catch (const std::exception& e) {
         std::printf("error: %s\n", e.what());
         return 1;

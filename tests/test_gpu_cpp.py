@@ -2,7 +2,8 @@
 shaped like the reference CLI's dmoVolumeTet (reference src/main.cu:51-73)
 is compiled with g++ against libvsdmo.so and run; it optimises a Kuhn cube
 through DMO::DMOTetClass, then records training pairs, trains the network
-and runs the learned proposal with the deferred fallback."""
+and runs the learned proposal with the deferred fallback, then a batch of
+two cubes through vsdmo_cpp::Batch with the network as its tet proposal."""
 import os
 import shutil
 import subprocess
@@ -28,3 +29,4 @@ def test_cpp_dropin_example_runs(tmp_path):
     assert r.returncode == 0, (r.stdout, r.stderr)
     assert "min quality" in r.stdout and "q_min" in r.stdout
     assert "learned:" in r.stdout, r.stdout
+    assert "batch:" in r.stdout, r.stdout
