@@ -23,11 +23,13 @@ CLS = {"tri": DMOTriFlatClass, "tet": DMOTetClass, "hex": DMOHexClass}
 
 
 def _inputs():
-    """Tet cubes of uneven sizes between flat tri grids and one hex cube."""
+    """Tet cubes of uneven sizes between flat tri grids and one hex cube, and
+    a one-cell tet cube whose 8 vertices are all on the boundary (no items)."""
     return [("tet", *synth.tet_cube(9, seed=41)),
             ("tri", *synth.tri_grid(11, seed=42, jitter=0.35)),
             ("tet", *synth.tet_cube(5, seed=43)),
             ("hex", *synth.hex_cube(6, seed=44, jitter=0.2)),
+            ("tet", *synth.tet_cube(1, seed=48)),
             ("tet", *synth.tet_cube(12, seed=45)),
             ("tri", *synth.tri_grid(6, seed=46, jitter=0.35))]
 
