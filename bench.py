@@ -356,6 +356,56 @@ def bench_learned(args):
     return 0
 
 
+def _learned_batched(args, mlp, fac, xyz, cells, kind, single_plain_s):
+    """Learned + batched on the bench mesh: a one-mesh DMOBatch (one
+    iteration replayed as a hipGraph, no host check between iterations),
+    the learned proposal with the immediate fallback against the same
+    batch's plain search.  The learned iteration count n is the first at
+    which the min and mean quality reach the plain run's final ones (-1e-3),
+    found in an untimed run (the runs are deterministic, bit for bit); the
+    timed runs are one replay call each, K plain iterations and n learned
+    ones, and the timed learned run's final quality is checked again.  The
+    ratio to the single-mesh optimiser's plain K iterations (single_plain_s)
+    is reported beside it."""
+    from volumesurfacediscretemeshoptimization_amd import DMOBatch, _lib
+    elem = _lib.VSDMO_TET if kind == "tet" else _lib.VSDMO_HEX
+    m = fac.create(xyz, cells)
+    b = DMOBatch([m])
+    b.iterate(1)                             # warm-up (captures the graph)
+    b.restore()
+    t0 = time.perf_counter()
+    b.iterate(args.steps)
+    dt_plain = time.perf_counter() - t0
+    qp, qmp = float(b.min_quality()[0]), float(b.mean_quality()[0])
+    b.restore()
+    b.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac, elem=elem)
+    n = 0
+    while n < 3 * args.steps:
+        b.iterate(1)
+        n += 1
+        if b.min_quality()[0] >= qp - 1e-3 and b.mean_quality()[0] >= qmp - 1e-3:
+            break
+    b.restore()
+    ls0 = b.learned_stats()
+    t0 = time.perf_counter()
+    b.iterate(n)
+    dt = time.perf_counter() - t0
+    ql, qml = float(b.min_quality()[0]), float(b.mean_quality()[0])
+    ls = b.learned_stats()
+    reached = ql >= qp - 1e-3 and qml >= qmp - 1e-3
+    b.set_proposal(None, mode=0, elem=elem)
+    b.close()
+    m.close()
+    visits = ls["learned_visits"] - ls0["learned_visits"]
+    return {"policy": "immediate", "plain_iterations": args.steps, "learned_iterations": n,
+            "plain_ms_per_step": 1e3 * dt_plain / args.steps, "learned_ms_per_step": 1e3 * dt / n,
+            "speedup_vs_plain": dt_plain / dt if reached else 0.0,
+            "speedup_vs_single_mesh_plain": single_plain_s / dt if reached else 0.0,
+            "reached_plain_quality": bool(reached),
+            "min_quality_plain": qp, "min_quality_learned": ql, "mean_quality_plain": qmp,
+            "mean_quality_learned": qml, "fallback_rate": (ls["fallbacks"] - ls0["fallbacks"]) / max(1, visits)}
+
+
 def _learned(args):
     from volumesurfacediscretemeshoptimization_amd import (DMODataset, DMOHexClass, DMOMeshHexFactory,
                                                            DMOMeshTetFactory, DMOMLP, DMOTetClass)
@@ -442,6 +492,12 @@ def _learned(args):
     stk = ok.stats()
     ok.close()
     mk.close()
+    batched = None
+    if args.learned_batched:
+        try:
+            batched = _learned_batched(args, mlp, fac, xyz, cells, kind, dt_plain)
+        except Exception as e:    # the learned line must not depend on it
+            batched = {"error": f"{type(e).__name__}: {e}"}
     learned_updates = stl["searches_inner"] + stl["searches_surface"] + stl["searches_feature"]
     reached = q_l >= q_plain - 1e-3 and qm_l >= qm_plain - 1e-3
     value = plain_updates / dt_learned if reached else 0.0
@@ -468,6 +524,7 @@ def _learned(args):
         "fallback_policy": args.learn_fallback,
         "fallback_rate": stl["fallbacks"] / max(1, stl["searches_learned"]),
         **extras,
+        "learned_batched": batched,
         "training_pairs": n_pairs, "training_meshes": s, "record_s": rec_s, "train_s": train_s,
         "epochs": args.epochs, "loss_first_epoch": float(losses[0]) if len(losses) else None,
         "loss_last_epoch": float(losses[-1]) if len(losses) else None,
@@ -509,6 +566,8 @@ def main(argv=None):
                     help="fallback policy of the measured learned run (the other one is reported beside it)")
     ap.add_argument("--learned-extras", type=int, default=1,
                     help="1: also run the other fallback policy and the specified 1/4 window (0: skip them)")
+    ap.add_argument("--learned-batched", type=int, default=1,
+                    help="learned mode / comparison: also run the mesh as a one-mesh batch, learned vs plain (0: skip)")
     ap.add_argument("--batch-learned", type=int, default=1,
                     help="batch256: also run the batch with the learned proposal on its tet meshes (0: skip)")
     # the batch's tet meshes span 500-50,000 vertices: twice the pairs (from
