@@ -351,6 +351,15 @@ vsdmo_status vsdmo_batch_get_stats(vsdmo_batch* b, vsdmo_batch_stats* out);
  * The autotuner refuses while a proposal is set. */
 vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* mlp, int32_t mode, int32_t depth,
                                       float window_frac);
+/* Fallback policy of the batch's learned proposals, as vsdmo_opt_set_fallback
+ * (VSDMO_FALLBACK_IMMEDIATE, the default, or VSDMO_FALLBACK_DEFERRED: a
+ * vertex whose window finds nothing better takes the full search in its
+ * round of the next iteration; the iteration number and the double-buffered
+ * lists live on the device, so the captured iteration replays unchanged).
+ * Clears pending deferrals, as do vsdmo_batch_restore and
+ * vsdmo_batch_set_proposal.  Each mesh then ends where a vsdmo_opt with the
+ * same policy ends.  Deferred needs nq 4 or 8. */
+vsdmo_status vsdmo_batch_set_fallback(vsdmo_batch* b, int32_t policy);
 /* learned visits so far and how many of them took the full search */
 vsdmo_status vsdmo_batch_learned_stats(vsdmo_batch* b, int64_t* visits, int64_t* fallbacks);
 vsdmo_status vsdmo_batch_destroy(vsdmo_batch* b);

@@ -168,3 +168,76 @@ def test_network_outlives_its_users():
     o.iterate(1)
     o.close()
     m.close()
+
+
+@pytest.mark.parametrize("types", [("tet",), ("hex",), ("tet", "hex")])
+def test_batch_deferred_matches_single_meshes(types):
+    """Deferred fallback in the batch (iteration number and double-buffered
+    lists on the device, one captured iteration replayed): every learned mesh
+    ends bit-exact where a single-mesh optimiser with the deferred policy
+    ends, with the same fallback count, after every iteration."""
+    nets = {k: _network(seed=31 + i, b3=(0.04, 0.03, -0.02 - 0.01 * i)) for i, k in enumerate(types)}
+    inputs = _inputs()
+    iters = 4
+    ref, falls = [], 0
+    for k, xyz, cells in inputs:
+        m = FAC[k].create(xyz, cells)
+        o = CLS[k](m, n_iter=iters, fixed_boundary=True)
+        if k in types:
+            o.set_proposal(nets[k], mode=1, depth=1, window_frac=0.25, fallback="deferred")
+        rows = []
+        for _ in range(iters):
+            o.doIteration()
+            rows.append(m.positions(original_order=False))
+        if k in types:
+            falls += o.stats()["fallbacks"]
+        ref.append(rows)
+        o.close()
+        m.close()
+    b = DMOBatch(_meshes(inputs))
+    b.set_fallback("deferred")
+    for k in types:
+        b.set_proposal(nets[k], mode=1, depth=1, window_frac=0.25, elem=ELEM[k])
+    for it in range(iters):
+        b.iterate(1)
+        for i, rows in enumerate(ref):
+            assert np.array_equal(b.positions(i), rows[it]), (inputs[i][0], i, it)
+    ls = b.learned_stats()
+    assert ls["fallbacks"] == falls and falls > 0, (ls, falls)
+    # restore clears the pending deferrals: the same iterations again
+    b.restore()
+    b.iterate(iters)
+    for i, rows in enumerate(ref):
+        assert np.array_equal(b.positions(i), rows[-1]), (inputs[i][0], i)
+    b.close()
+    for n in nets.values():
+        n.close()
+
+
+def test_batch_deferred_graph_equals_eager_and_refusals():
+    mlp, mlp_h = _network(), _network(seed=23)
+    inputs = _inputs()
+    eager = DMOBatch(_meshes(inputs), use_graph=False)
+    graph = DMOBatch(_meshes(inputs), use_graph=True, shuffle_seed=99)
+    for b in (eager, graph):
+        b.set_fallback("deferred")
+        b.set_proposal(mlp, mode=1)
+        b.set_proposal(mlp_h, mode=1, elem=_lib.VSDMO_HEX)
+    for it in range(3):
+        eager.iterate(1)
+        graph.iterate(2 if it == 1 else 1)
+        if it == 1:
+            eager.iterate(1)
+        for i in range(len(inputs)):
+            assert np.array_equal(eager.positions(i), graph.positions(i)), (it, i)
+    assert eager.learned_stats() == graph.learned_stats()
+    with pytest.raises(VsdmoError):
+        graph.set_params(nq=6)           # the deferred full searches need nq 4 or 8
+    with pytest.raises(ValueError):
+        graph.set_fallback("later")
+    graph.set_fallback("immediate")
+    graph.set_params(nq=6)
+    for b in (eager, graph):
+        b.close()
+    mlp.close()
+    mlp_h.close()

@@ -593,6 +593,13 @@ class DMOBatch:
             self._mlps = {}
         self._mlps[int(elem)] = mlp if mode == 1 else None   # networks are destroyed after the batch
 
+    def set_fallback(self, policy: str):
+        """Fallback policy of the learned proposals ("immediate", the
+        default, or "deferred"); clears pending deferred searches."""
+        if policy not in _DMOBase.FALLBACK:
+            raise ValueError(f"fallback must be one of {sorted(_DMOBase.FALLBACK)}")
+        check(self._lib.vsdmo_batch_set_fallback(self._h, _DMOBase.FALLBACK[policy]), "vsdmo_batch_set_fallback")
+
     def learned_stats(self) -> dict:
         v, f = C.c_int64(), C.c_int64()
         check(self._lib.vsdmo_batch_learned_stats(self._h, C.byref(v), C.byref(f)), "vsdmo_batch_learned_stats")

@@ -127,6 +127,7 @@ EXPORTS = {
     "vsdmo_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_char_p, C.POINTER(BatchParams)]),
     "vsdmo_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
     "vsdmo_batch_set_proposal": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_float]),
+    "vsdmo_batch_set_fallback": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsdmo_batch_learned_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "vsdmo_batch_destroy": (C.c_int, [C.c_void_p]),
 }

@@ -16,6 +16,7 @@
 // per mesh.
 #include <float.h>
 
+#include <climits>
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -97,6 +98,14 @@ struct vsdmo_batch {
     } learn[2];
     DevBuf<int> nbr, lList, lCnt, lFallback;
     long long learned = 0;
+    // deferred fallback: the lists of (type t, round r) at dOff[t * rounds + r]
+    // (parity 1 at + dTotal), their counts at t * rounds + r (parity 1 at
+    // + 2 rounds), the per-vertex deferral epochs and the iteration counter,
+    // all on the device so that one captured iteration serves every replay
+    int fallback = VSDMO_FALLBACK_IMMEDIATE;
+    std::vector<int> dOff;
+    int dTotal = 0;
+    DevBuf<int> dList, dCnt, dEpoch, itDev;
 };
 
 namespace {
@@ -178,9 +187,51 @@ void size_grids(vsdmo_batch* b)
 
 bool learning(const vsdmo_batch* b) { return b->learn[0].mode != 0 || b->learn[1].mode != 0; }
 
-// one type's items [lo, hi) of a round through the learned window kernel,
-// then the full search of their fallbacks (immediate policy)
-void launch_learned_items(vsdmo_batch* b, int t, const SearchArgs& a, int lo, int hi, int* cnt, hipStream_t s)
+bool deferred(const vsdmo_batch* b) { return b->fallback == VSDMO_FALLBACK_DEFERRED; }
+
+void alloc_deferred(vsdmo_batch* b)
+{
+    if (b->dList.p) return;
+    const int nR = (int)b->grid.size();
+    b->dOff.assign((size_t)2 * nR, 0);
+    int total = 0;
+    for (int t = 0; t < 2; ++t)
+        for (int r = 0; r < nR; ++r) {
+            b->dOff[(size_t)t * nR + r] = total;
+            total += t == 0 ? b->roundMidT[r] - b->roundOff[r] : b->roundMid[r] - b->roundMidT[r];
+        }
+    b->dTotal = std::max(total, 1);
+    b->dList.alloc((size_t)2 * b->dTotal);
+    b->dCnt.alloc((size_t)std::max(4 * nR, 1));
+    b->dEpoch.alloc((size_t)std::max(b->totalV, 1));
+    b->itDev.alloc(1);
+    launch_fill_int(b->itDev.p, 1, 0, b->stream);
+}
+
+// no pending deferrals: both parities' counts zero, no vertex deferred
+void reset_deferred(vsdmo_batch* b)
+{
+    if (!b->dList.p) return;
+    launch_fill_int(b->dCnt.p, std::max(4 * (int)b->grid.size(), 1), 0, b->stream);
+    launch_fill_int(b->dEpoch.p, std::max(b->totalV, 1), INT_MIN, b->stream);
+    VSDMO_HIP(hipStreamSynchronize(b->stream));
+}
+
+// the deferred policy's full searches need nq 4 / 8 and every learned type's
+// rings within the window kernel's LDS tile
+std::string deferred_check(const vsdmo_batch* b, int nq)
+{
+    for (int t = 0; t < 2; ++t)
+        if (b->learn[t].mode != 0 &&
+            !learned_deferred_ok(t == 0 ? VSDMO_TET : VSDMO_HEX, nq, t == 0 ? b->maxRingTet : b->maxRingHex))
+            return "the deferred fallback needs nq 4 or 8 and one-rings of at most 128 tets or 21 hexes";
+    return "";
+}
+
+// one type's items [lo, hi) of round r through the learned window kernel;
+// the full search of their fallbacks follows (immediate policy) or runs in
+// the round's launch of the next iteration (deferred)
+void launch_learned_items(vsdmo_batch* b, int t, const SearchArgs& a, int lo, int hi, int r, hipStream_t s)
 {
     const vsdmo_batch::Learn& L = b->learn[t];
     if (L.mode == 0 || hi <= lo) return;
@@ -192,8 +243,20 @@ void launch_learned_items(vsdmo_batch* b, int t, const SearchArgs& a, int lo, in
     LearnNet net;
     net.items = b->items.p;
     if (L.mode == 1) { net.wb = L.mlp->wb.p; net.w = L.mlp->w.p; net.nbr = b->nbr.p; }
-    launch_search_learned(t == 0 ? VSDMO_TET : VSDMO_HEX, b->p.nq, al, net, L.mode, L.depth, L.frac, b->lFallback.p,
-                          b->lList.p, cnt, t == 0 ? b->maxRingTet : b->maxRingHex, nullptr, s);
+    const int nR = (int)b->grid.size();
+    const int ring = t == 0 ? b->maxRingTet : b->maxRingHex;
+    const int kind = t == 0 ? VSDMO_TET : VSDMO_HEX;
+    if (deferred(b)) {
+        LearnDefer d{ nullptr, nullptr, b->dEpoch.p, 0 };
+        d.itDev = b->itDev.p;
+        d.listStride = b->dTotal;
+        d.cntStride = 2 * nR;
+        launch_search_learned(kind, b->p.nq, al, net, L.mode, L.depth, L.frac, b->lFallback.p,
+                              b->dList.p + b->dOff[(size_t)t * nR + r], b->dCnt.p + (size_t)t * nR + r, ring, &d, s);
+    } else {
+        launch_search_learned(kind, b->p.nq, al, net, L.mode, L.depth, L.frac, b->lFallback.p, b->lList.p,
+                              b->lCnt.p + (size_t)t * nR + r, ring, nullptr, s);
+    }
 }
 
 // the launches of one iteration, in stream order
@@ -202,12 +265,14 @@ void record_iteration(vsdmo_batch* b)
     hipStream_t s = b->stream;
     const int nR = (int)b->grid.size();
     launch_fill_int(b->ctr.p, nR, 0, s);
-    if (learning(b)) launch_fill_int(b->lCnt.p, 2 * nR, 0, s);
+    const bool learn = learning(b), def = learn && deferred(b);
+    if (def) launch_parity_zero(b->dCnt.p, 2 * nR, b->itDev.p, s);   // this iteration's lists
+    else if (learn) launch_fill_int(b->lCnt.p, 2 * nR, 0, s);
     SearchArgs a = search_args(b);
     for (int r = 0; r < nR; ++r) {
         const int off = b->roundOff[r], midT = b->roundMidT[r], mid = b->roundMid[r], hi = b->roundOff[r + 1];
-        launch_learned_items(b, 0, a, off, midT, b->lCnt.p + r, s);
-        launch_learned_items(b, 1, a, midT, mid, b->lCnt.p + nR + r, s);
+        launch_learned_items(b, 0, a, off, midT, r, s);
+        launch_learned_items(b, 1, a, midT, mid, r, s);
         // plain searches of the other items: tets [off, midT) and hexes
         // [midT, mid) from the round's queue, flat triangles [mid, hi) strided
         const bool lt = b->learn[0].mode != 0, lh = b->learn[1].mode != 0;
@@ -222,6 +287,7 @@ void record_iteration(vsdmo_batch* b)
             plain(!lt ? off : (lh ? mid : midT), mid, hi);
         }
     }
+    if (def) launch_add_int(b->itDev.p, 1, s);
     launch_batch_quality(quality_args(b), s);
 }
 
@@ -633,6 +699,7 @@ vsdmo_status vsdmo_batch_restore(vsdmo_batch* b)
     return guarded_b([&]() -> vsdmo_status {
         if (!b) return fail_status(VSDMO_INVALID_ARG, "NULL batch");
         restore_positions(b);
+        reset_deferred(b);   // deferrals of the abandoned trajectory
         b->lastMin = current_min(b);
         return VSDMO_OK;
     });
@@ -651,6 +718,8 @@ vsdmo_status vsdmo_batch_set_params(vsdmo_batch* b, const vsdmo_batch_params* p)
         if (!b || !p) return fail_status(VSDMO_INVALID_ARG, "NULL argument");
         std::string perr = check_params(*p);
         if (!perr.empty()) return fail_status(VSDMO_INVALID_ARG, perr);
+        if (deferred(b) && !deferred_check(b, p->nq).empty())
+            return fail_status(VSDMO_NOT_SUPPORTED, deferred_check(b, p->nq));
         apply_params(b, *p);
         return VSDMO_OK;
     });
@@ -683,6 +752,9 @@ vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* m
         if (mode == 2 && ring > learned_ring_cap(elem))
             return fail_status(VSDMO_NOT_SUPPORTED, "the learned search holds one-rings of at most " +
                                                         std::to_string(learned_ring_cap(elem)) + " elements");
+        if (mode != 0 && deferred(b) && !learned_deferred_ok(elem, b->p.nq, ring))
+            return fail_status(VSDMO_NOT_SUPPORTED, "the deferred fallback needs nq 4 or 8 and one-rings of at most "
+                                                    "128 tets or 21 hexes");
         hipStream_t s = b->stream;
         VSDMO_HIP(hipStreamSynchronize(s));
         drop_graph(b);   // the recorded launches change
@@ -722,6 +794,26 @@ vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* m
         if (L.mlp) L.mlp->n_users++;
         L.depth = depth;
         L.frac = window_frac;
+        reset_deferred(b);   // pending deferrals belong to the previous setting
+        return VSDMO_OK;
+    });
+}
+
+vsdmo_status vsdmo_batch_set_fallback(vsdmo_batch* b, int32_t policy)
+{
+    return guarded_b([&]() -> vsdmo_status {
+        if (!b) return fail_status(VSDMO_INVALID_ARG, "NULL batch");
+        if (policy != VSDMO_FALLBACK_IMMEDIATE && policy != VSDMO_FALLBACK_DEFERRED)
+            return fail_status(VSDMO_INVALID_ARG, "policy must be VSDMO_FALLBACK_IMMEDIATE or VSDMO_FALLBACK_DEFERRED");
+        if (policy == VSDMO_FALLBACK_DEFERRED) {
+            std::string err = deferred_check(b, b->p.nq);
+            if (!err.empty()) return fail_status(VSDMO_NOT_SUPPORTED, err);
+            alloc_deferred(b);
+        }
+        VSDMO_HIP(hipStreamSynchronize(b->stream));
+        drop_graph(b);   // the recorded launches change
+        b->fallback = policy;
+        reset_deferred(b);
         return VSDMO_OK;
     });
 }

@@ -126,6 +126,10 @@ void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, c
                         int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s);
 int batch_round_blocks_per_cu(int nq, int w, size_t lds);
 void launch_fill_int(int* p, int n, int v, hipStream_t s);
+// zero the n counters cnt + (*itDev & 1) * n (device-side iteration parity)
+void launch_parity_zero(int* cnt, int n, const int* itDev, hipStream_t s);
+// *p += v on the device (one thread)
+void launch_add_int(int* p, int v, hipStream_t s);
 // where the learned search finds its proposals (mode 1): the network's
 // packed parameters (wb bf16, w fp32) and the neighbour table, with which the
 // window kernel runs the network itself when learned_fused_ok(); otherwise
@@ -148,6 +152,11 @@ struct LearnDefer {
     const int* cntIn;
     int* epoch;
     int it;
+    // iteration in device memory (graph replay): listIn / cntIn / it are
+    // ignored, list / cnt are parity-0 buffers with the parity-1 ones at
+    // + listStride / cntStride (see launch_parity_zero, launch_add_int)
+    const int* itDev = nullptr;
+    int listStride = 0, cntStride = 0;
 };
 // learned-proposal search of one colour class [a.cStart, a.cEnd) (tet / hex):
 // the proposal window (4 vertices per wave) appends the vertices whose

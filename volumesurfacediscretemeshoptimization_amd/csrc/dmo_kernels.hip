@@ -590,7 +590,28 @@ struct LearnArgs {
     // class positions -> vertex ids through a work-item list (a batch round:
     // id in the low 30 bits); null: the class is the range a.cStart + position
     const int* items;
+    // deferred fallback with the iteration in device memory (a replayed
+    // graph): it = *itDev; list / cnt are the parity-0 buffers of the class,
+    // parity p at + p * listStride / cntStride; this iteration appends to
+    // parity it & 1 and reads the previous iteration's from the other
+    const int* itDev;
+    int listStride, cntStride;
 };
+
+// the concrete iteration and list buffers of a deferred pass
+__device__ __forceinline__ LearnArgs learn_resolve(const LearnArgs& la)
+{
+    if (!la.itDev) return la;
+    LearnArgs r = la;
+    const int it = *la.itDev;
+    const size_t p = (size_t)(it & 1), q = p ^ 1;
+    r.it = it;
+    r.list = la.list + p * la.listStride;
+    r.cnt = la.cnt + p * la.cntStride;
+    r.listIn = la.list + q * la.listStride;
+    r.cntIn = la.cnt + q * la.cntStride;
+    return r;
+}
 
 // vertex id of class position ci
 __device__ __forceinline__ int learned_vertex(const SearchArgs& a, const LearnArgs& la, int ci)
@@ -971,10 +992,11 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchA
 // last iteration excepted); both are vertices of one colour class, so the
 // two kinds run side by side.
 template <int NQ, bool FUSED>
-__global__ __launch_bounds__(256) void k_search_tet_learned_def(SearchArgs a, LearnArgs la, int ringCap, int nbList)
+__global__ __launch_bounds__(256) void k_search_tet_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbList)
 {
     extern __shared__ __align__(16) unsigned char s_tcd[];
     __shared__ RedLds<4> rd;
+    const LearnArgs la = learn_resolve(la0);
     if ((int)blockIdx.x < nbList) {
         const int n = min(*la.cntIn, a.cEnd - a.cStart);
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
@@ -990,10 +1012,11 @@ __global__ __launch_bounds__(256) void k_search_tet_learned_def(SearchArgs a, Le
 // waves) then fits the chip's wave slots in one round
 template <int NQ, bool FUSED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-void k_search_hex_learned_def(SearchArgs a, LearnArgs la, int ringCap, int nbList)
+void k_search_hex_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbList)
 {
     extern __shared__ __align__(16) unsigned char s_hxd[];
     __shared__ RedLds<4> rd;
+    const LearnArgs la = learn_resolve(la0);
     if ((int)blockIdx.x < nbList) {
         const int n = min(*la.cntIn, a.cEnd - a.cStart);
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
@@ -1096,6 +1119,7 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
         // and a twelfth measured slower on the 40^3 tet cube) beside this
         // iteration's windows, 16 vertices per block
         la.listIn = def->listIn; la.cntIn = def->cntIn; la.epoch = def->epoch; la.it = def->it;
+        la.itDev = def->itDev; la.listStride = def->listStride; la.cntStride = def->cntStride;
         const int nbList = std::max(1, std::min(nc, std::max(64, nc / 8)));
         const int nb = (nc + kMlpRows - 1) / kMlpRows;
         size_t lds = (size_t)kMlpRows * ringCap * per;
@@ -1575,6 +1599,25 @@ void launch_fill_int(int* p, int n, int v, hipStream_t s)
 {
     if (n > 0) k_fill_int<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
 }
+
+// zero the n counters of parity (*itDev & 1): cnt + parity * n
+__global__ void k_parity_zero(int* __restrict__ cnt, int n, const int* __restrict__ itDev)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[(size_t)(*itDev & 1) * n + i] = 0;
+}
+
+__global__ void k_add_int(int* __restrict__ p, int v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p += v;
+}
+
+void launch_parity_zero(int* cnt, int n, const int* itDev, hipStream_t s)
+{
+    if (n > 0) k_parity_zero<<<(n + 255) / 256, 256, 0, s>>>(cnt, n, itDev);
+}
+
+void launch_add_int(int* p, int v, hipStream_t s) { k_add_int<<<1, 64, 0, s>>>(p, v); }
 
 void launch_batch_quality(const BatchQArgs& a, hipStream_t s)
 {
