@@ -80,7 +80,7 @@ FLOP_PER_UPDATE_TRI = 27_048
 
 
 def _batch_learned(args, b, dt_plain, q_plain, qm_plain):
-    """The batch's tet meshes with the learned proposal (immediate fallback,
+    """The batch's tet meshes with the learned proposal (--batch-fallback policy,
     the tet window unless --learn-depth / --learn-frac) against the plain
     batch's K iterations, time to equal quality: K learned iterations from
     the start positions, then single iterations while any mesh is below the
@@ -110,6 +110,7 @@ def _batch_learned(args, b, dt_plain, q_plain, qm_plain):
     ds.close()
     depth, frac = args.learn_depth, args.learn_frac
     b.restore()
+    b.set_fallback(args.batch_fallback)
     b.set_proposal(mlp, mode=1, depth=depth, window_frac=frac)
     b.iterate(max(args.warmup, 1))           # warm-up, then back to the start positions
     b.restore()
@@ -133,7 +134,7 @@ def _batch_learned(args, b, dt_plain, q_plain, qm_plain):
     b.set_proposal(None, mode=0)
     mlp.close()
     visits = ls["learned_visits"] - ls0["learned_visits"]
-    return {"policy": "immediate", "levels": depth, "window_frac": frac, "plain_iterations": args.steps,
+    return {"policy": args.batch_fallback, "levels": depth, "window_frac": frac, "plain_iterations": args.steps,
             "learned_iterations": its, "meshes_below_plain_quality_at_plain_iterations": below_k,
             "learned_ms_per_step": 1e3 * dt / its, "plain_ms_per_step": 1e3 * dt_plain / args.steps,
             "speedup_vs_plain": dt_plain / dt if below == 0 else 0.0,
@@ -359,7 +360,7 @@ def bench_learned(args):
 def _learned_batched(args, mlp, fac, xyz, cells, kind, single_plain_s):
     """Learned + batched on the bench mesh: a one-mesh DMOBatch (one
     iteration replayed as a hipGraph, no host check between iterations),
-    the learned proposal with the immediate fallback against the same
+    the learned proposal with the line's fallback policy against the same
     batch's plain search.  The learned iteration count n is the first at
     which the min and mean quality reach the plain run's final ones (-1e-3),
     found in an untimed run (the runs are deterministic, bit for bit); the
@@ -378,6 +379,7 @@ def _learned_batched(args, mlp, fac, xyz, cells, kind, single_plain_s):
     dt_plain = time.perf_counter() - t0
     qp, qmp = float(b.min_quality()[0]), float(b.mean_quality()[0])
     b.restore()
+    b.set_fallback(args.learn_fallback)
     b.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac, elem=elem)
     n = 0
     while n < 3 * args.steps:
@@ -397,7 +399,7 @@ def _learned_batched(args, mlp, fac, xyz, cells, kind, single_plain_s):
     b.close()
     m.close()
     visits = ls["learned_visits"] - ls0["learned_visits"]
-    return {"policy": "immediate", "plain_iterations": args.steps, "learned_iterations": n,
+    return {"policy": args.learn_fallback, "plain_iterations": args.steps, "learned_iterations": n,
             "plain_ms_per_step": 1e3 * dt_plain / args.steps, "learned_ms_per_step": 1e3 * dt / n,
             "speedup_vs_plain": dt_plain / dt if reached else 0.0,
             "speedup_vs_single_mesh_plain": single_plain_s / dt if reached else 0.0,
@@ -575,6 +577,8 @@ def main(argv=None):
     # rate from 13 % to 9 % (on the tet40 cube the same change did not help)
     ap.add_argument("--batch-pairs", type=int, default=4_000_000, help="training pairs of the batch256 network")
     ap.add_argument("--batch-epochs", type=int, default=30)
+    ap.add_argument("--batch-fallback", choices=["deferred", "immediate"], default="deferred",
+                    help="fallback policy of the batch256 learned run")
     ap.add_argument("--tune-log", default=os.environ.get("VSDMO_TUNE_LOG", "autotune_batch256.jsonl"))
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
