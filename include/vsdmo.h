@@ -340,9 +340,9 @@ vsdmo_status vsdmo_batch_autotune(vsdmo_batch* b, int32_t iters, float tol, cons
 vsdmo_status vsdmo_batch_get_stats(vsdmo_batch* b, vsdmo_batch_stats* out);
 /* Learned proposal for the batch's tet or hex vertices, elem = VSDMO_TET or
  * VSDMO_HEX, each type with its own network and window (no counterpart in
- * the reference; the rule of vsdmo_opt_set_proposal with the immediate
- * fallback policy, applied per colour round to the round's items of that
- * type; the other items keep the plain search).  Each such mesh's positions
+ * the reference; the rule of vsdmo_opt_set_proposal with the policy of
+ * vsdmo_batch_set_fallback, applied per colour round to the round's items
+ * of that type; the other items keep the plain search).  Each such mesh's positions
  * equal a vsdmo_opt of the mesh with fixed_boundary = 1 and the same
  * proposal.  mode 1 runs the network inside the window kernel and needs
  * every ring of the type to fit its LDS tile (VSDMO_NOT_SUPPORTED

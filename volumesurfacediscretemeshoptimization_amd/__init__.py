@@ -577,11 +577,12 @@ class DMOBatch:
                      elem: int = _lib.VSDMO_TET):
         """Learned proposal for the vertices of the batch's meshes of type
         elem (VSDMO_TET or VSDMO_HEX; each type has its own network and
-        window), immediate fallback: a vertex whose window finds nothing
-        better takes the full search in the same round.  The other meshes
-        keep the plain search.  Each such mesh gets the positions of a
-        DMOTetClass / DMOHexClass with fixed_boundary=True and the same
-        proposal.  Defaults: the type's window in DMOTetClass /
+        window).  A vertex whose window finds nothing better takes the full
+        search in the same round, or in its round of the next iteration
+        after set_fallback("deferred").  The other meshes keep the plain
+        search.  Each such mesh gets the positions of a DMOTetClass /
+        DMOHexClass with fixed_boundary=True and the same proposal and
+        policy.  Defaults: the type's window in DMOTetClass /
         DMOHexClass."""
         d0, f0 = _DMOBase.LEARN_WINDOW.get(elem, (1, 0.025))
         depth = d0 if depth is None else depth

@@ -87,9 +87,9 @@ struct vsdmo_batch {
     long long searches = 0;
     int iterations = 0;
     double gpu_ms = 0.0;
-    // learned proposal of the tet items ([0]) and the hex items ([1]),
-    // immediate fallback: network and window per type, the global neighbour
-    // table, the fallback list (reused by each learned launch in stream
+    // learned proposal of the tet items ([0]) and the hex items ([1]):
+    // network and window per type, the global neighbour table; immediate
+    // policy: the fallback list (reused by each learned launch in stream
     // order), the list lengths (tets: [0, rounds), hexes: [rounds, 2 rounds))
     struct Learn {
         vsdmo_mlp* mlp = nullptr;
@@ -725,10 +725,11 @@ vsdmo_status vsdmo_batch_set_params(vsdmo_batch* b, const vsdmo_batch_params* p)
     });
 }
 
-// Learned proposal for the tet or hex items (immediate fallback: a round's
-// windows, then the full search of the vertices whose window does not
-// improve them), the single-mesh rule of vsdmo_opt_set_proposal over a
-// round's items of that type; each type has its own network and window.
+// Learned proposal for the tet or hex items (a round's windows; the full
+// search of the vertices whose window does not improve them follows in the
+// round, or in its next iteration under the deferred policy), the
+// single-mesh rule of vsdmo_opt_set_proposal over a round's items of that
+// type; each type has its own network and window.
 // The network runs inside the window kernel, so mode 1 needs every ring of
 // the type to fit its LDS tile.  The neighbour table is global: each tet and
 // hex mesh's table with the mesh's vertex offset added, built once.
