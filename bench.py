@@ -371,7 +371,7 @@ def _learned_batched(args, mlp, fac, xyz, cells, kind, single_plain_s):
     from volumesurfacediscretemeshoptimization_amd import DMOBatch, _lib
     elem = _lib.VSDMO_TET if kind == "tet" else _lib.VSDMO_HEX
     m = fac.create(xyz, cells)
-    b = DMOBatch([m])
+    b = DMOBatch([m], use_graph=bool(args.batched_graph))
     b.iterate(1)                             # warm-up (captures the graph)
     b.restore()
     t0 = time.perf_counter()
@@ -570,6 +570,8 @@ def main(argv=None):
                     help="1: also run the other fallback policy and the specified 1/4 window (0: skip them)")
     ap.add_argument("--learned-batched", type=int, default=1,
                     help="learned mode / comparison: also run the mesh as a one-mesh batch, learned vs plain (0: skip)")
+    ap.add_argument("--batched-graph", type=int, default=1,
+                    help="learned-batched comparison: 1 replays a captured iteration, 0 launches it eagerly")
     ap.add_argument("--batch-learned", type=int, default=1,
                     help="batch256: also run the batch with the learned proposal on its tet meshes (0: skip)")
     # the batch's tet meshes span 500-50,000 vertices: twice the pairs (from
