@@ -159,7 +159,7 @@ def bench_batch(args):
     t0 = time.perf_counter()
     meshes = [fac[k].create(xyz, cells) for k, xyz, cells in mix]
     build_s = time.perf_counter() - t0
-    b = DMOBatch(meshes)
+    b = DMOBatch(meshes, use_graph=bool(args.batch_graph))
     q0 = b.min_quality()
     chosen = b.autotune(iters=args.tune_iters, tol=1e-3, log_path=args.tune_log)
     # warm-up: one iteration, then back to the start positions
@@ -572,6 +572,8 @@ def main(argv=None):
                     help="learned mode / comparison: also run the mesh as a one-mesh batch, learned vs plain (0: skip)")
     ap.add_argument("--batched-graph", type=int, default=1,
                     help="learned-batched comparison: 1 replays a captured iteration, 0 launches it eagerly")
+    ap.add_argument("--batch-graph", type=int, default=1,
+                    help="batch256: 1 replays a captured iteration, 0 launches it eagerly (same results)")
     ap.add_argument("--batch-learned", type=int, default=1,
                     help="batch256: also run the batch with the learned proposal on its tet meshes (0: skip)")
     # the batch's tet meshes span 500-50,000 vertices: twice the pairs (from

@@ -276,6 +276,20 @@ void record_iteration(vsdmo_batch* b)
         // plain searches of the other items: tets [off, midT) and hexes
         // [midT, mid) from the round's queue, flat triangles [mid, hi) strided
         const bool lt = b->learn[0].mode != 0, lh = b->learn[1].mode != 0;
+        if (!b->p.persistent) {
+            // one block per item: a launch per element type, each with its
+            // own body's registers and LDS
+            const int w = b->p.waves_per_vertex;
+            if (!lt)
+                launch_batch_typed(b->p.nq, w, a, b->rowHex.p, b->ringHex.p, b->items.p, off, midT, 0,
+                                   batch_lds_bytes(b->maxRingTet, 0, 0), s);
+            if (!lh)
+                launch_batch_typed(b->p.nq, w, a, b->rowHex.p, b->ringHex.p, b->items.p, midT, mid, 1,
+                                   batch_lds_bytes(0, b->maxRingHex, 0), s);
+            launch_batch_typed(b->p.nq, w, a, b->rowHex.p, b->ringHex.p, b->items.p, mid, hi, 2,
+                               batch_lds_bytes(0, 0, b->maxHE), s);
+            continue;
+        }
         auto plain = [&](int lo, int qend, int end) {
             launch_batch_round(b->p.nq, b->p.waves_per_vertex, a, b->rowHex.p, b->ringHex.p, b->items.p, lo, qend, end,
                                b->ctr.p + r, b->grid[r], b->lds, s);

@@ -119,13 +119,17 @@ void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hip
 // `tail` vertices, which get several waves each (waves_per_vertex = 0)
 bool split_supported(int kind, int nq, int ringCap);
 void launch_search_split(int kind, int nq, const SearchArgs& a, int tail, int ringCap, hipStream_t s);
-// one colour round of a batch: persistent (grid > 0 blocks, tet/hex items
-// [lo, mid) from the atomic counter *ctr, flat items [mid, hi) strided) or,
-// with grid <= 0, one block per item
+// one colour round of a batch, persistent: grid blocks take the tet / hex
+// items [lo, mid) from the atomic counter *ctr and stride over the flat
+// items [mid, hi) (the one-block-per-item schedule is launch_batch_typed)
 void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
                         int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s);
 int batch_round_blocks_per_cu(int nq, int w, size_t lds);
 void launch_fill_int(int* p, int n, int v, hipStream_t s);
+// one block per item of items[lo, hi), all of element type `type` (0 tet,
+// 1 hex, 2 flat tri): that type's search body alone
+void launch_batch_typed(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
+                        int lo, int hi, int type, size_t lds, hipStream_t s);
 // zero the n counters cnt + (*itDev & 1) * n (device-side iteration parity)
 void launch_parity_zero(int* cnt, int n, const int* itDev, hipStream_t s);
 // *p += v on the device (one thread)

@@ -1485,15 +1485,46 @@ static void launch_nq(int kind, int w, const SearchArgs& a, int nBlocks, size_t 
     else launch_nqw<NQ, 1>(kind, a, nBlocks, lds, s);
 }
 // Non-persistent variant of a round: one block per item, scheduled by the
-// hardware dispatcher (no queue, no atomics).
-template <int NQ, int W>
-__global__ __launch_bounds__(64 * W) void k_batch_direct(SearchArgs a, const int* __restrict__ rowHex,
-                                                        const int* __restrict__ valHex, const int* __restrict__ items,
-                                                        int lo)
+// hardware dispatcher (no queue, no atomics), one launch per element type
+// (0 tet, 1 hex, 2 flat tri) with the search body of that type alone, so
+// that each keeps the registers and LDS of its own body.
+template <int NQ, int W, int TYPE>
+__global__ __launch_bounds__(64 * W) void k_batch_typed(SearchArgs a, const int* __restrict__ rowHex,
+                                                       const int* __restrict__ valHex, const int* __restrict__ items,
+                                                       int lo)
 {
     extern __shared__ float4 s_dyn[];
     __shared__ RedLds<W> rd;
-    batch_item<NQ, W>(a, rowHex, valHex, items[lo + blockIdx.x], s_dyn, rd);
+    const int vid = items[lo + blockIdx.x] & 0x3fffffff;
+    if constexpr (TYPE == 0) {
+        tet_vertex<NQ, W>(a, vid, false, reinterpret_cast<TetC*>(s_dyn), rd);
+    } else if constexpr (TYPE == 1) {
+        SearchArgs ah = a;
+        ah.ringRow = rowHex;
+        ah.ringVal = valHex;
+        hex_vertex<NQ, W>(ah, vid, true, reinterpret_cast<HexLds*>(s_dyn), rd);
+    } else {
+        flat_vertex<NQ, W>(a, vid, reinterpret_cast<TriC*>(s_dyn), rd);
+    }
+}
+
+template <int NQ, int W>
+static void batch_typed_nqw(const SearchArgs& a, const int* rowHex, const int* valHex, const int* items, int lo, int hi,
+                            int type, size_t lds, hipStream_t s)
+{
+    const unsigned n = (unsigned)(hi - lo);
+    if (type == 0) k_batch_typed<NQ, W, 0><<<n, 64 * W, lds, s>>>(a, rowHex, valHex, items, lo);
+    else if (type == 1) k_batch_typed<NQ, W, 1><<<n, 64 * W, lds, s>>>(a, rowHex, valHex, items, lo);
+    else k_batch_typed<NQ, W, 2><<<n, 64 * W, lds, s>>>(a, rowHex, valHex, items, lo);
+}
+template <int NQ>
+static void batch_typed_nq(int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items, int lo,
+                           int hi, int type, size_t lds, hipStream_t s)
+{
+    if (w == 2 && NQ % 2 == 0) batch_typed_nqw<NQ, (NQ % 2 == 0 ? 2 : 1)>(a, rowHex, valHex, items, lo, hi, type, lds, s);
+    else if (w == 4 && NQ % 4 == 0)
+        batch_typed_nqw<NQ, (NQ % 4 == 0 ? 4 : 1)>(a, rowHex, valHex, items, lo, hi, type, lds, s);
+    else batch_typed_nqw<NQ, 1>(a, rowHex, valHex, items, lo, hi, type, lds, s);
 }
 
 struct BatchLaunch {
@@ -1510,10 +1541,7 @@ struct BatchLaunch {
 template <int NQ, int W>
 static void batch_nqw(const BatchLaunch& L)
 {
-    if (L.grid > 0)
-        k_batch_round<NQ, W><<<L.grid, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo, L.mid, L.hi, L.ctr);
-    else
-        k_batch_direct<NQ, W><<<L.hi - L.lo, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo);
+    k_batch_round<NQ, W><<<L.grid, 64 * W, L.lds, L.s>>>(L.a, L.rowHex, L.valHex, L.items, L.lo, L.mid, L.hi, L.ctr);
 }
 template <int NQ, int W>
 static int batch_occ(size_t lds)
@@ -1584,13 +1612,26 @@ int batch_round_blocks_per_cu(int nq, int w, size_t lds)
 void launch_batch_round(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
                         int lo, int mid, int hi, int* ctr, int grid, size_t lds, hipStream_t s)
 {
-    if (hi <= lo) return;   // grid <= 0: one block per item
+    if (hi <= lo || grid <= 0) return;
     BatchLaunch L{ a, rowHex, valHex, items, lo, mid, hi, ctr, grid, lds, s };
     switch (nq) {
     case 2: batch_nq<2>(w, L); break;
     case 4: batch_nq<4>(w, L); break;
     case 6: batch_nq<6>(w, L); break;
     case 8: batch_nq<8>(w, L); break;
+    default: break;
+    }
+}
+
+void launch_batch_typed(int nq, int w, const SearchArgs& a, const int* rowHex, const int* valHex, const int* items,
+                        int lo, int hi, int type, size_t lds, hipStream_t s)
+{
+    if (hi <= lo) return;
+    switch (nq) {
+    case 2: batch_typed_nq<2>(w, a, rowHex, valHex, items, lo, hi, type, lds, s); break;
+    case 4: batch_typed_nq<4>(w, a, rowHex, valHex, items, lo, hi, type, lds, s); break;
+    case 6: batch_typed_nq<6>(w, a, rowHex, valHex, items, lo, hi, type, lds, s); break;
+    case 8: batch_typed_nq<8>(w, a, rowHex, valHex, items, lo, hi, type, lds, s); break;
     default: break;
     }
 }
