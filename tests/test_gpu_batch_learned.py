@@ -170,12 +170,15 @@ def test_network_outlives_its_users():
     m.close()
 
 
+@pytest.mark.parametrize("persistent", [True, False])
 @pytest.mark.parametrize("types", [("tet",), ("hex",), ("tet", "hex")])
-def test_batch_deferred_matches_single_meshes(types):
+def test_batch_deferred_matches_single_meshes(types, persistent):
     """Deferred fallback in the batch (iteration number and double-buffered
     lists on the device, one captured iteration replayed): every learned mesh
     ends bit-exact where a single-mesh optimiser with the deferred policy
-    ends, with the same fallback count, after every iteration."""
+    ends, with the same fallback count, after every iteration; the plain
+    items run on the persistent queue or one block per item (one launch per
+    element type)."""
     nets = {k: _network(seed=31 + i, b3=(0.04, 0.03, -0.02 - 0.01 * i)) for i, k in enumerate(types)}
     inputs = _inputs()
     iters = 4
@@ -194,7 +197,7 @@ def test_batch_deferred_matches_single_meshes(types):
         ref.append(rows)
         o.close()
         m.close()
-    b = DMOBatch(_meshes(inputs))
+    b = DMOBatch(_meshes(inputs), persistent=persistent)
     b.set_fallback("deferred")
     for k in types:
         b.set_proposal(nets[k], mode=1, depth=1, window_frac=0.25, elem=ELEM[k])
