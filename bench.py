@@ -371,7 +371,9 @@ def _learned_batched(args, mlp, fac, xyz, cells, kind, single_plain_s):
     from volumesurfacediscretemeshoptimization_amd import DMOBatch, _lib
     elem = _lib.VSDMO_TET if kind == "tet" else _lib.VSDMO_HEX
     m = fac.create(xyz, cells)
-    b = DMOBatch([m], use_graph=bool(args.batched_graph))
+    # one block per item (one launch per element type): the schedule the
+    # autotuner keeps on the 256-mesh batch
+    b = DMOBatch([m], use_graph=bool(args.batched_graph), persistent=False)
     b.iterate(1)                             # warm-up (captures the graph)
     b.restore()
     t0 = time.perf_counter()
