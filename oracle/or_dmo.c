@@ -48,23 +48,29 @@ static inline float gridc(float aff, int i, int nq, float gmin, float gmax)
     return aff * ((float)i * gmin + (float)(nq - 1 - i) * gmax);
 }
 
-static int vertex_volume(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz)
+/* Search of one vertex on an explicit ring: ringPts holds n elements of st
+   points (3 per tet halfface, 7 per halfhex), world coordinates; x0 is the
+   vertex's start position.  Writes the final position to out (x0 when the
+   vertex does not move) and returns 1 iff it moved. */
+static int search_volume_ring(int tet, int n, const float* ringPts, const float* x0p, const or_dmo_params* p,
+                              float* out)
 {
-    int tet = (m->elem == OR_TET);
-    int st = m->ringStride;
-    int r0 = m->ringRow[vid], n = m->ringRow[vid + 1] - r0;
+    const int st = tet ? 3 : 7;
+    out[0] = x0p[0]; out[1] = x0p[1]; out[2] = x0p[2];
     if (n <= 0 || n > (tet ? MAXR : 64)) return 0;
-    v3 x0 = ld3(xyz + 3 * (size_t)vid);
-    float ext[3] = { p->extent_mode == OR_EXTENT_MIN ? FLT_MAX : -FLT_MAX,
-                     p->extent_mode == OR_EXTENT_MIN ? FLT_MAX : -FLT_MAX,
-                     p->extent_mode == OR_EXTENT_MIN ? FLT_MAX : -FLT_MAX };
+    v3 x0 = ld3(x0p);
+    int emin = (p->extent_mode == OR_EXTENT_MIN);
+#if OR_MUTATION == 1
+    emin = !emin;                      /* planted: min and max extent swapped */
+#endif
+    float ext[3] = { emin ? FLT_MAX : -FLT_MAX, emin ? FLT_MAX : -FLT_MAX, emin ? FLT_MAX : -FLT_MAX };
     float mabs = 0.0f;
     for (int e = 0; e < n; ++e)
         for (int k = 0; k < st; ++k) {
-            const float* q = xyz + 3 * (size_t)m->ringVal[(size_t)(r0 + e) * st + k];
+            const float* q = ringPts + 3 * (size_t)(e * st + k);
             float d[3] = { fabsf(q[0] - x0.x), fabsf(q[1] - x0.y), fabsf(q[2] - x0.z) };
             for (int a = 0; a < 3; ++a) {
-                ext[a] = (p->extent_mode == OR_EXTENT_MIN) ? fminf(ext[a], d[a]) : fmaxf(ext[a], d[a]);
+                ext[a] = emin ? fminf(ext[a], d[a]) : fmaxf(ext[a], d[a]);
                 mabs = fmaxf(mabs, d[a]);
             }
         }
@@ -78,16 +84,15 @@ static int vertex_volume(const or_mesh* m, const or_dmo_params* p, int vid, floa
     hexacc hacc = hex_acc_init();
     if (tet) {
         for (int e = 0; e < n; ++e) {
-            const int* h = m->ringVal + (size_t)(r0 + e) * 3;
-            tcs[e] = tet_consts(loc3(xyz + 3 * (size_t)h[0], x0, sigma), loc3(xyz + 3 * (size_t)h[1], x0, sigma),
-                                loc3(xyz + 3 * (size_t)h[2], x0, sigma));
+            const float* h = ringPts + 9 * (size_t)e;
+            tcs[e] = tet_consts(loc3(h, x0, sigma), loc3(h + 3, x0, sigma), loc3(h + 6, x0, sigma));
         }
     } else {
         for (int e = 0; e < n; ++e) {
-            const int* h = m->ringVal + (size_t)(r0 + e) * 7;
+            const float* h = ringPts + 21 * (size_t)e;
             v3 r[8];
             r[0] = x0;
-            for (int k = 1; k < 8; ++k) r[k] = loc3(xyz + 3 * (size_t)h[k - 1], x0, sigma);
+            for (int k = 1; k < 8; ++k) r[k] = loc3(h + 3 * (k - 1), x0, sigma);
             hex_consts(r, cd + 4 * e, cF + 4 * e, hp + 5 * e);
         }
         hacc = hex_const_acc(n, cd, cF);
@@ -116,7 +121,13 @@ static int vertex_volume(const or_mesh* m, const or_dmo_params* p, int vid, floa
                     float pxg = gridc(aff, i, nq, gmin[0], gmax[0]);
                     float px = (pxg - x0.x) * sigma;
                     float q = SCORE(px, py, pz);
-                    if (q > best) { best = q; bp[0] = pxg; bp[1] = pyg; bp[2] = pzg; }
+#if OR_MUTATION == 6
+                    if (q >= best) {           /* planted: the highest index wins a tie */
+#else
+                    if (q > best) {
+#endif
+                        best = q; bp[0] = pxg; bp[1] = pyg; bp[2] = pzg;
+                    }
                 }
             }
         }
@@ -124,29 +135,50 @@ static int vertex_volume(const or_mesh* m, const or_dmo_params* p, int vid, floa
         ds = ds * shr;
     }
 #undef SCORE
-    if (moved) { xyz[3 * (size_t)vid] = cur[0]; xyz[3 * (size_t)vid + 1] = cur[1]; xyz[3 * (size_t)vid + 2] = cur[2]; }
+    if (moved) { out[0] = cur[0]; out[1] = cur[1]; out[2] = cur[2]; }
     return moved;
 }
 
-static int vertex_flat(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz)
+static int vertex_volume(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz)
 {
-    int h0 = m->heRow[vid], n = m->heRow[vid + 1] - h0;
-    if (n <= 0 || n >= MAXR) return 0;
+    int tet = (m->elem == OR_TET);
+    int st = m->ringStride;
+    int r0 = m->ringRow[vid], n = m->ringRow[vid + 1] - r0;
+    if (n <= 0 || n > (tet ? MAXR : 64)) return 0;
+    float pts[MAXR * 9];
+    for (int e = 0; e < n; ++e)
+        for (int k = 0; k < st; ++k) {
+            const float* q = xyz + 3 * (size_t)m->ringVal[(size_t)(r0 + e) * st + k];
+            float* o = pts + 3 * (size_t)(e * st + k);
+            o[0] = q[0]; o[1] = q[1]; o[2] = q[2];
+        }
     float x0[3] = { xyz[3 * (size_t)vid], xyz[3 * (size_t)vid + 1], xyz[3 * (size_t)vid + 2] };
+    float out[3];
+    int moved = search_volume_ring(tet, n, pts, x0, p, out);
+    if (moved) { xyz[3 * (size_t)vid] = out[0]; xyz[3 * (size_t)vid + 1] = out[1]; xyz[3 * (size_t)vid + 2] = out[2]; }
+    return moved;
+}
+
+/* Flat 2D search on an explicit closed fan of n ring points (x, y, z with
+   z ignored): elements (ring[k], ring[k+1]), ring[n] = ring[0].  Writes the
+   final position to out (x0 when it does not move); returns 1 iff written. */
+static int search_flat_fan(int n, const float* fan, const float* x0, const or_dmo_params* p, float* out)
+{
+    out[0] = x0[0]; out[1] = x0[1]; out[2] = x0[2];
+    if (n <= 0 || n >= MAXR) return 0;
     float ext[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
     float mabs = 0.0f;
     for (int i = 0; i < n; ++i) {
-        const float* q = xyz + 3 * (size_t)m->heTarget[h0 + i];
+        const float* q = fan + 3 * (size_t)i;
         for (int a = 0; a < 3; ++a) ext[a] = fmaxf(ext[a], fabsf(q[a] - x0[a]));
         mabs = fmaxf(mabs, fmaxf(fabsf(q[0] - x0[0]), fabsf(q[1] - x0[1])));
     }
     float sigma = or_pow2_scale(mabs);
     float is2 = invpow(sigma, 2);
-    /* closed fan: elements (ring[k], ring[k+1]), ring[n] = ring[0] */
     tric cs[MAXR];
     for (int e = 0; e < n; ++e) {
-        const float* a = xyz + 3 * (size_t)m->heTarget[h0 + e];
-        const float* b = xyz + 3 * (size_t)m->heTarget[h0 + (e + 1) % n];
+        const float* a = fan + 3 * (size_t)e;
+        const float* b = fan + 3 * (size_t)((e + 1) % n);
         cs[e] = tri_consts((a[0] - x0[0]) * sigma, (a[1] - x0[1]) * sigma, (b[0] - x0[0]) * sigma, (b[1] - x0[1]) * sigma);
     }
     const int nq = p->nq, half = nq / 2, T = nq * nq / 2;
@@ -170,14 +202,53 @@ static int vertex_flat(const or_mesh* m, const or_dmo_params* p, int vid, float*
             float qm = (q2 > q1) ? q2 : q1;
             if (qm > best) {
                 best = qm;
-                if (q1 > q2) { bp[0] = ax; bp[1] = ay; } else { bp[0] = bx; bp[1] = by; }
+#if OR_MUTATION == 2
+                if (q1 >= q2) {                /* planted: A wins an exact tie */
+#else
+                if (q1 > q2) {
+#endif
+                    bp[0] = ax; bp[1] = ay;
+                } else {
+                    bp[0] = bx; bp[1] = by;
+                }
             }
         }
         if (qcur < best) { cur[0] = bp[0]; cur[1] = bp[1]; qcur = best; }
         qmaxLast = best;
         ds = ds * shr;
     }
-    if (qold < qmaxLast) { xyz[3 * (size_t)vid] = cur[0]; xyz[3 * (size_t)vid + 1] = cur[1]; return 1; }
+#if OR_MUTATION == 5
+    if (qold < qcur) {                         /* planted: written whenever a level improved */
+#else
+    if (qold < qmaxLast) {
+#endif
+        out[0] = cur[0]; out[1] = cur[1];
+        return 1;
+    }
+    return 0;
+}
+
+static int vertex_flat(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz)
+{
+    int h0 = m->heRow[vid], n = m->heRow[vid + 1] - h0;
+    if (n <= 0 || n >= MAXR) return 0;
+    float fan[3 * MAXR];
+    for (int i = 0; i < n; ++i) {
+        const float* q = xyz + 3 * (size_t)m->heTarget[h0 + i];
+        fan[3 * i] = q[0]; fan[3 * i + 1] = q[1]; fan[3 * i + 2] = q[2];
+    }
+    float x0[3] = { xyz[3 * (size_t)vid], xyz[3 * (size_t)vid + 1], xyz[3 * (size_t)vid + 2] };
+    float out[3];
+    int w = search_flat_fan(n, fan, x0, p, out);
+    if (w) { xyz[3 * (size_t)vid] = out[0]; xyz[3 * (size_t)vid + 1] = out[1]; }
+    return w;
+}
+
+int or_search_ring(int elem, int n, const float* ringPts, const float* x0, const or_dmo_params* p, float* out)
+{
+    if (elem == OR_TET || elem == OR_HEX) return search_volume_ring(elem == OR_TET, n, ringPts, x0, p, out);
+    if (elem == OR_TRI) return search_flat_fan(n, ringPts, x0, p, out);
+    out[0] = x0[0]; out[1] = x0[1]; out[2] = x0[2];
     return 0;
 }
 

@@ -11,6 +11,13 @@
 #include <stdint.h>
 #include <string.h>
 
+/* OR_MUTATION k > 0 plants deliberate mistake k in the oracle; the mutation
+   tests compile such copies and check that the pinning tests reject them.
+   The oracle proper is built with 0. */
+#ifndef OR_MUTATION
+#define OR_MUTATION 0
+#endif
+
 static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
@@ -201,8 +208,13 @@ static inline float hex_score(int n, hexacc a, const hexp* hp, float px, float p
     float Q1 = fmaf(pz, pz, py * py);
     float Q3 = 3.0f * Q1;
     float t3 = 3.0f * px;
+#if OR_MUTATION == 3
+    const int nsub = 4;                /* planted: the principal-axes sub-tet dropped */
+#else
+    const int nsub = 5;
+#endif
     for (int e = 0; e < n; ++e)
-        for (int s = 0; s < 5; ++s) {
+        for (int s = 0; s < nsub; ++s) {
             const hexp* h = hp + 5 * e + s;
             int a3 = HEX_A3(s);
             float Tyz = fmaf(h->gy, py, fmaf(h->gz, pz, h->D));

@@ -112,7 +112,7 @@ static void fit_quadric(lsurf* s, const w3* pts, int np)
 }
 
 struct or_surf {
-    int nS, nFree;
+    int nS, nFree, nF;
     float* pinit;         /* 3*nS: surface positions at optimiser creation */
     float* vnorm;         /* 3*nS */
     float* fnorm;         /* 3*nFaces */
@@ -153,6 +153,7 @@ or_surf* or_surface_init(const or_mesh* m)
     int nS = m->nSurf;
     s->nS = nS; s->nFree = m->nFree;
     int nF = m->nFeature;
+    s->nF = nF;
     s->pinit = (float*)malloc(sizeof(float) * 3 * (size_t)(nS > 0 ? nS : 1));
     memcpy(s->pinit, m->xyz, sizeof(float) * 3 * (size_t)nS);
     s->vnorm = (float*)malloc(sizeof(float) * 3 * (size_t)(nS > 0 ? nS : 1));
@@ -306,6 +307,9 @@ static w3 project2d(const or_mesh* m, const or_surf* s, const float* snap, int v
     {
         w3 d = wsub(W(s->pinit + 3 * s->nn[vid]), pw);
         inv[0] = 1.0f / (wdot(d, d) + SEPS);
+#if OR_MUTATION == 4
+        inv[0] = inv[0] * inv[0];      /* planted: blend weight squared */
+#endif
         isum += inv[0];
     }
     int h0 = m->heRow[vid], nh = m->heRow[vid + 1] - h0;
@@ -315,6 +319,9 @@ static w3 project2d(const or_mesh* m, const or_surf* s, const float* snap, int v
         int nn = s->nn[vn];
         w3 d = wsub(W(s->pinit + 3 * nn), pw);
         inv[i + 1] = 1.0f / (wdot(d, d) + SEPS);
+#if OR_MUTATION == 4
+        inv[i + 1] = inv[i + 1] * inv[i + 1];
+#endif
         isum += inv[i + 1];
         const lsurf* LN;
         if (m->isFeature[nn]) {
@@ -330,6 +337,9 @@ static w3 project2d(const or_mesh* m, const or_surf* s, const float* snap, int v
             int sid = (arg < TABLE_SIZE) ? s->table[TABLE_SIZE * off + arg] : -1;
             const int nsec = s->frow[off + 1] - s->frow[off];
             if (sid < 0 || sid >= nsec) sid = 0;
+#if OR_MUTATION == 11
+            sid = 0;                   /* planted: the first sector always */
+#endif
             LN = nsec > 0 ? &s->fs[s->frow[off] + sid] : &s->ls[nn];
         } else {
             LN = &s->ls[nn];
@@ -436,14 +446,20 @@ static float vring_score(const vring* r, w3 p)
 /* 2D surface search of a free surface vertex (k_optimizeHierarchical2DHex /
    2DTet): (u,v) window around the current position in the nearest
    neighbour's frame, half-width s_d * (min distance to the ring points for
-   hexes, max for tets); thread t holds (t/nq, t%nq) and (t/nq + nq/2, t%nq);
-   B on exact ties; lowest thread wins; write iff q(old) < last level max */
-static int search2d(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz)
+   hexes, max for tets); thread t holds A = (t/nq, t%nq) and B = (t/nq + nq/2,
+   t%nq) and keeps A unless q_B > q_A (A on an exact tie); lowest thread wins;
+   write iff q(old) < last level max.  uvOut (optional): the final (u, v). */
+static int search2d(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz,
+                    float* uvOut)
 {
     vring* r = (vring*)malloc(sizeof(vring));
     vring_init(m, vid, xyz, r);
     if (r->n <= 0) { free(r); return 0; }
+#if OR_MUTATION == 7
+    const float dist = (m->elem == OR_HEX) ? r->maxDist : r->minDist;   /* planted: window rules swapped */
+#else
     const float dist = (m->elem == OR_HEX) ? r->minDist : r->maxDist;
+#endif
     const lsurf* L = &s->ls[s->nn[vid]];
     float vidPos[3] = { xyz[3 * (size_t)vid], xyz[3 * (size_t)vid + 1], xyz[3 * (size_t)vid + 2] };
     w3 cur = W(vidPos);
@@ -468,7 +484,11 @@ static int search2d(const or_mesh* m, const or_surf* s, const float* snap, const
             float q1 = vring_score(r, project2d(m, s, snap, vid, vidPos, u1, v1));
             float q2 = vring_score(r, project2d(m, s, snap, vid, vidPos, u2, v2));
             float qm = q1, uu = u1, vv = v1;
+#if OR_MUTATION == 8
+            if (q2 >= q1) { qm = q2; uu = u2; vv = v2; }   /* planted: B wins an exact tie */
+#else
             if (q2 > q1) { qm = q2; uu = u2; vv = v2; }
+#endif
             if (qm > best) { best = qm; bu = uu; bv = vv; }
         }
         if (qcur < best) {
@@ -480,12 +500,14 @@ static int search2d(const or_mesh* m, const or_surf* s, const float* snap, const
         ds = ds * shr;
     }
     free(r);
+    if (uvOut) { uvOut[0] = cu; uvOut[1] = cvv; }
     if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
     return 0;
 }
 
-/* 1D feature-curve search (k_optimizeHierarchical1DHex / 1DTet) */
-static int search1d(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, float* xyz)
+/* 1D feature-curve search (k_optimizeHierarchical1DHex / 1DTet).  uvOut
+   (optional): the final u. */
+static int search1d(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, float* xyz, float* uvOut)
 {
     if (!s->opt1d[vid]) return 0;
     vring* r = (vring*)malloc(sizeof(vring));
@@ -500,7 +522,11 @@ static int search1d(const or_mesh* m, const or_surf* s, const or_dmo_params* p, 
         for (int h2 = m->heRow[d]; h2 < m->heRow[d + 1]; ++h2) if (m->isFeature[m->heTarget[h2]]) ++c;
         if (c == 2 && nnb < MAX_NEIGH) nb[nnb++] = d;
     }
+#if OR_MUTATION == 10
+    const float dist = (m->elem == OR_HEX) ? r->maxDist : r->minDist;   /* planted: window rules swapped */
+#else
     const float dist = (m->elem == OR_HEX) ? r->minDist : r->maxDist;
+#endif
     const lsurf1* L = &s->ls1[s->nn[vid]];
     w3 cur = W(xyz + 3 * (size_t)vid);
     w3 cv = wsub(cur, L->p0);
@@ -529,6 +555,7 @@ static int search1d(const or_mesh* m, const or_surf* s, const or_dmo_params* p, 
         ds = ds * shr;
     }
     free(r);
+    if (uvOut) uvOut[0] = cu;
     if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
     return 0;
 }
@@ -616,7 +643,9 @@ static void update_nn(const or_mesh* m, or_surf* s)
             int c = vh;
             for (int h = m->heRow[c]; h < m->heRow[c + 1]; ++h) {
                 int dst = m->heTarget[h];
+#if OR_MUTATION != 9                   /* 9 plants: the feature-class filter dropped */
                 if (m->isFeature[v] != m->isFeature[dst]) continue;
+#endif
                 w3 dd = wsub(p, W(s->pinit + 3 * dst));
                 float sd = wdot(dd, dd);
                 if (sd < sq) { sq = sd; vh = dst; }
@@ -656,7 +685,7 @@ long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* 
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif
-            for (int v = lo; v < hi; ++v) search2d(m, s, snap, p, v, m->xyz);
+            for (int v = lo; v < hi; ++v) search2d(m, s, snap, p, v, m->xyz, NULL);
             *nSurf2d += hi - lo;
         }
         free(snap);
@@ -665,7 +694,7 @@ long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* 
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif
-            for (int v = lo; v < hi; ++v) search1d(m, s, p, v, m->xyz);
+            for (int v = lo; v < hi; ++v) search1d(m, s, p, v, m->xyz, NULL);
             for (int v = lo; v < hi; ++v) *nFeat1d += s->opt1d[v];
         }
     }
@@ -1057,3 +1086,33 @@ void or_surface_get(const or_surf* s, int what, void* out)
 }
 
 int or_surface_nsectors(const or_surf* s, int nFeature) { return s->frow[nFeature]; }
+
+void or_surface_get_sectors(const or_surf* s, int* table, int* frow, float* surf16)
+{
+    if (table) memcpy(table, s->table, sizeof(int) * (size_t)s->nF * TABLE_SIZE);
+    if (frow) memcpy(frow, s->frow, sizeof(int) * ((size_t)s->nF + 1));
+    if (surf16)
+        for (int q = 0; q < s->frow[s->nF]; ++q) {
+            float* o = surf16 + 16 * (size_t)q;
+            const lsurf* f = &s->fs[q];
+            o[0] = f->p0.x; o[1] = f->p0.y; o[2] = f->p0.z;
+            for (int k = 0; k < 3; ++k) { o[3 + 3 * k] = f->e[k].x; o[4 + 3 * k] = f->e[k].y; o[5 + 3 * k] = f->e[k].z; }
+            o[12] = f->a; o[13] = f->b; o[14] = f->c; o[15] = (float)f->np;
+        }
+}
+
+int or_surface_search_vertex(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, int dim,
+                             float* xyz, float* uvOut)
+{
+    if (dim == 1) return search1d(m, s, p, vid, xyz, uvOut);
+    /* a single vertex: the snapshot of the pass is the current positions */
+    return search2d(m, s, xyz, p, vid, xyz, uvOut);
+}
+
+void or_surface_project(const or_mesh* m, const or_surf* s, int vid, float u, float v, float* out3)
+{
+    w3 q = project2d(m, s, m->xyz, vid, P(m, vid), u, v);
+    out3[0] = q.x; out3[1] = q.y; out3[2] = q.z;
+}
+
+void or_surface_update_nn(const or_mesh* m, or_surf* s) { update_nn(m, s); }

@@ -136,6 +136,11 @@ typedef struct or_dmo_params {
 long long or_dmo_iteration(or_mesh* m, const or_dmo_params* p);
 /* search one vertex; exported for brute-force tests.  Returns 1 if moved. */
 int or_dmo_vertex(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz_inout);
+/* the same search on an explicit ring: elem OR_TET / OR_HEX with nElem
+   halffaces (3 points) / halfhexes (7 points), or OR_TRI with a closed fan of
+   nElem points; x0 = start position; out = final position (x0 when the vertex
+   is not written).  Returns 1 iff the position is written. */
+int or_search_ring(int elem, int nElem, const float* ringPts, const float* x0, const or_dmo_params* p, float* out3);
 
 /* ---- surface vertices ---- */
 typedef struct or_surf or_surf;
@@ -163,6 +168,20 @@ void or_laplace(or_mesh* m, int n_iter);
    floats: p0, e0, e1, a), 4 vertex normals (3 floats) */
 void or_surface_get(const or_surf* s, int what, void* out);
 int or_surface_nsectors(const or_surf* s, int nFeature);
+/* feature sectors: the halfedge-to-sector table (TABLE_SIZE = 20 ints per
+   feature vertex), the sector row offsets (nFeature + 1) and the sector fits
+   (16 floats each, as the local surfaces) */
+void or_surface_get_sectors(const or_surf* s, int* table, int* frow, float* surf16);
+/* one surface vertex's search on the current positions (dim 2: the free
+   vertex 2D search, 1: the feature 1D search); xyz updated in place, uvOut:
+   the final local coordinates (u, v) / u.  Returns 1 iff written. */
+int or_surface_search_vertex(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, int dim,
+                             float* xyz, float* uvOut);
+/* the blended projection of (u, v) in vid's nearest neighbour's frame, at
+   the current positions */
+void or_surface_project(const or_mesh* m, const or_surf* s, int vid, float u, float v, float* out3);
+/* the nearest-neighbour walk of all surface vertices from their current table */
+void or_surface_update_nn(const or_mesh* m, or_surf* s);
 
 /* element quality statistics in FP32 canonical (min) and FP64 (min, mean) */
 float or_min_quality(const or_mesh* m);   /* flat tri: 2D metric; surface tri / quad: unsigned 3D */

@@ -5,6 +5,7 @@ never the thing measured or shipped.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 
@@ -53,64 +54,89 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
-        L = C.CDLL(LIB_PATH)
-        L.or_cbrt.restype = C.c_float; L.or_cbrt.argtypes = [C.c_float]
-        L.or_pow2_scale.restype = C.c_float; L.or_pow2_scale.argtypes = [C.c_float]
-        for n in ("or_q_tet", "or_q_hex", "or_q_tri2d"):
-            getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp]
-        for n in ("or_q_tet64", "or_q_hex64", "or_q_tri2d64"):
-            getattr(L, n).restype = C.c_double; getattr(L, n).argtypes = [_fp]
-        for n in ("or_ring_score_tet", "or_ring_score_hex", "or_ring_score_tri2d"):
-            getattr(L, n).restype = C.c_float
-            getattr(L, n).argtypes = [C.c_int, _fp, _fp, C.c_float, _fp]
-        L.or_surface_points.restype = None
-        L.or_surface_points.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                        C.c_float, _fp]
-        L.or_fit_local_surface.restype = None
-        L.or_fit_local_surface.argtypes = [_fp, C.c_int, _fp, _fp, _fp]
-        L.or_lu_solve3.restype = None
-        L.or_lu_solve3.argtypes = [_fp, _fp, _fp]
-        L.or_mesh_build.restype = C.c_int
-        L.or_mesh_build.argtypes = [C.c_int, _fp, C.c_int, _ip, C.c_int, C.POINTER(BuildParams), C.POINTER(C.POINTER(OrMesh))]
-        L.or_mesh_free.argtypes = [C.POINTER(OrMesh)]
-        L.or_boundary_faces.restype = C.c_int
-        L.or_boundary_faces.argtypes = [C.c_int, C.c_int, _ip, _ip]
-        L.or_color.restype = C.c_int
-        L.or_color.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip]
-        L.or_adjacency_range.restype = C.c_int
-        L.or_adjacency_range.argtypes = [C.c_int, C.c_int, _ip, C.c_int, C.c_int, C.POINTER(_ip), C.POINTER(_ip)]
-        L.or_dmo_iteration.restype = C.c_longlong
-        L.or_dmo_iteration.argtypes = [C.POINTER(OrMesh), C.POINTER(DmoParams)]
-        L.or_dmo_vertex.restype = C.c_int
-        L.or_dmo_vertex.argtypes = [C.POINTER(OrMesh), C.POINTER(DmoParams), C.c_int, _fp]
-        L.or_min_quality.restype = C.c_float
-        L.or_min_quality.argtypes = [C.POINTER(OrMesh)]
-        L.or_quality64.argtypes = [C.POINTER(OrMesh), C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        L.or_element_qualities.argtypes = [C.POINTER(OrMesh), _fp]
-        L.or_surface_init.restype = C.c_void_p
-        L.or_surface_init.argtypes = [C.POINTER(OrMesh)]
-        L.or_surface_free.argtypes = [C.c_void_p]
-        L.or_dmo_iteration_surface.restype = C.c_longlong
-        L.or_dmo_iteration_surface.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams), C.c_float,
-                                               C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
-        L.or_dmo_iteration_surface_mesh.restype = C.c_longlong
-        L.or_dmo_iteration_surface_mesh.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams),
-                                                    C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
-        L.or_surface_mesh_qualities.argtypes = [C.POINTER(OrMesh), _fp]
-        L.or_surface_mesh_min_quality.restype = C.c_float
-        L.or_surface_mesh_min_quality.argtypes = [C.POINTER(OrMesh)]
-        L.or_laplace.argtypes = [C.POINTER(OrMesh), C.c_int]
-        for n in ("or_q_tri3", "or_q_quad"):
-            getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp]
-        for n in ("or_q_tri3n", "or_q_quadn"):
-            getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp, _fp]
-        for n in ("or_q_tri364", "or_q_quad64"):
-            getattr(L, n).restype = C.c_double; getattr(L, n).argtypes = [_fp]
-        L.or_surface_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
-        L.or_surface_nsectors.restype = C.c_int
-        L.or_surface_nsectors.argtypes = [C.c_void_p, C.c_int]
-        _lib = L
+        _lib = _configure(C.CDLL(LIB_PATH))
     return _lib
+
+
+@contextlib.contextmanager
+def use_library(path: str):
+    """Runs the block against another build of the oracle (the mutation tests'
+    copies with a planted mistake); the default library is restored after."""
+    global _lib
+    saved = lib()
+    _lib = _configure(C.CDLL(path))
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def _configure(L):
+    """Sets the argument and result types of every exported oracle function."""
+    L.or_cbrt.restype = C.c_float; L.or_cbrt.argtypes = [C.c_float]
+    L.or_pow2_scale.restype = C.c_float; L.or_pow2_scale.argtypes = [C.c_float]
+    for n in ("or_q_tet", "or_q_hex", "or_q_tri2d"):
+        getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp]
+    for n in ("or_q_tet64", "or_q_hex64", "or_q_tri2d64"):
+        getattr(L, n).restype = C.c_double; getattr(L, n).argtypes = [_fp]
+    for n in ("or_ring_score_tet", "or_ring_score_hex", "or_ring_score_tri2d"):
+        getattr(L, n).restype = C.c_float
+        getattr(L, n).argtypes = [C.c_int, _fp, _fp, C.c_float, _fp]
+    L.or_surface_points.restype = None
+    L.or_surface_points.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_float, _fp]
+    L.or_fit_local_surface.restype = None
+    L.or_fit_local_surface.argtypes = [_fp, C.c_int, _fp, _fp, _fp]
+    L.or_lu_solve3.restype = None
+    L.or_lu_solve3.argtypes = [_fp, _fp, _fp]
+    L.or_mesh_build.restype = C.c_int
+    L.or_mesh_build.argtypes = [C.c_int, _fp, C.c_int, _ip, C.c_int, C.POINTER(BuildParams), C.POINTER(C.POINTER(OrMesh))]
+    L.or_mesh_free.argtypes = [C.POINTER(OrMesh)]
+    L.or_boundary_faces.restype = C.c_int
+    L.or_boundary_faces.argtypes = [C.c_int, C.c_int, _ip, _ip]
+    L.or_color.restype = C.c_int
+    L.or_color.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip]
+    L.or_adjacency_range.restype = C.c_int
+    L.or_adjacency_range.argtypes = [C.c_int, C.c_int, _ip, C.c_int, C.c_int, C.POINTER(_ip), C.POINTER(_ip)]
+    L.or_dmo_iteration.restype = C.c_longlong
+    L.or_dmo_iteration.argtypes = [C.POINTER(OrMesh), C.POINTER(DmoParams)]
+    L.or_dmo_vertex.restype = C.c_int
+    L.or_dmo_vertex.argtypes = [C.POINTER(OrMesh), C.POINTER(DmoParams), C.c_int, _fp]
+    L.or_min_quality.restype = C.c_float
+    L.or_min_quality.argtypes = [C.POINTER(OrMesh)]
+    L.or_quality64.argtypes = [C.POINTER(OrMesh), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_element_qualities.argtypes = [C.POINTER(OrMesh), _fp]
+    L.or_surface_init.restype = C.c_void_p
+    L.or_surface_init.argtypes = [C.POINTER(OrMesh)]
+    L.or_surface_free.argtypes = [C.c_void_p]
+    L.or_dmo_iteration_surface.restype = C.c_longlong
+    L.or_dmo_iteration_surface.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams), C.c_float,
+                                           C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.or_dmo_iteration_surface_mesh.restype = C.c_longlong
+    L.or_dmo_iteration_surface_mesh.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams),
+                                                C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.or_surface_mesh_qualities.argtypes = [C.POINTER(OrMesh), _fp]
+    L.or_surface_mesh_min_quality.restype = C.c_float
+    L.or_surface_mesh_min_quality.argtypes = [C.POINTER(OrMesh)]
+    L.or_laplace.argtypes = [C.POINTER(OrMesh), C.c_int]
+    for n in ("or_q_tri3", "or_q_quad"):
+        getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp]
+    for n in ("or_q_tri3n", "or_q_quadn"):
+        getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp, _fp]
+    for n in ("or_q_tri364", "or_q_quad64"):
+        getattr(L, n).restype = C.c_double; getattr(L, n).argtypes = [_fp]
+    L.or_surface_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.or_surface_nsectors.restype = C.c_int
+    L.or_surface_nsectors.argtypes = [C.c_void_p, C.c_int]
+    L.or_search_ring.restype = C.c_int
+    L.or_search_ring.argtypes = [C.c_int, C.c_int, _fp, _fp, C.POINTER(DmoParams), _fp]
+    L.or_surface_get_sectors.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_surface_search_vertex.restype = C.c_int
+    L.or_surface_search_vertex.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams), C.c_int, C.c_int,
+                                           _fp, _fp]
+    L.or_surface_project.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.c_int, C.c_float, C.c_float, _fp]
+    L.or_surface_update_nn.argtypes = [C.POINTER(OrMesh), C.c_void_p]
+    return L
 
 
 def _f(a):
@@ -184,7 +210,8 @@ class OracleMesh:
         c, cp = _i(cells)
         prm = BuildParams(coloring, feature_angle)
         out = C.POINTER(OrMesh)()
-        rc = lib().or_mesh_build(_ELEM[kind], xp, len(x), cp, len(c), C.byref(prm), C.byref(out))
+        self.L = lib()
+        rc = self.L.or_mesh_build(_ELEM[kind], xp, len(x), cp, len(c), C.byref(prm), C.byref(out))
         if not out:
             raise ValueError(f"oracle mesh build failed: {rc}")
         self.ptr = out
@@ -192,10 +219,10 @@ class OracleMesh:
 
     def __del__(self):
         if getattr(self, "surf", None):
-            lib().or_surface_free(self.surf)
+            self.L.or_surface_free(self.surf)
             self.surf = None
         if getattr(self, "ptr", None):
-            lib().or_mesh_free(self.ptr)
+            self.L.or_mesh_free(self.ptr)
             self.ptr = None
 
     @property
@@ -249,21 +276,21 @@ class OracleMesh:
         if extent is None:
             extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
         p = DmoParams(nq, depth, grid_scale, shrink, extent, fixed_boundary, nthreads)
-        return lib().or_dmo_iteration(self.ptr, C.byref(p))
+        return self.L.or_dmo_iteration(self.ptr, C.byref(p))
 
     def search_vertex(self, vid, xyz, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None):
         if extent is None:
             extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
         p = DmoParams(nq, depth, grid_scale, shrink, extent, 1, 1)
         a = np.ascontiguousarray(xyz, dtype=np.float32).copy()
-        moved = lib().or_dmo_vertex(self.ptr, C.byref(p), vid, a.ctypes.data_as(_fp))
+        moved = self.L.or_dmo_vertex(self.ptr, C.byref(p), vid, a.ctypes.data_as(_fp))
         return moved, a
 
     def surface_init(self):
         """Local surfaces etc. from the current positions (optimiser init)."""
         if getattr(self, "surf", None):
-            lib().or_surface_free(self.surf)
-        self.surf = lib().or_surface_init(self.ptr)
+            self.L.or_surface_free(self.surf)
+        self.surf = self.L.or_surface_init(self.ptr)
         self.last_min = self.min_quality()
 
     def iterate_surface(self, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None, nthreads=0):
@@ -273,7 +300,7 @@ class OracleMesh:
             extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
         p = DmoParams(nq, depth, grid_scale, shrink, extent, 0, nthreads)
         a, b = C.c_longlong(), C.c_longlong()
-        lib().or_dmo_iteration_surface(self.ptr, self.surf, C.byref(p), C.c_float(self.last_min), C.byref(a), C.byref(b))
+        self.L.or_dmo_iteration_surface(self.ptr, self.surf, C.byref(p), C.c_float(self.last_min), C.byref(a), C.byref(b))
         self.last_min = self.min_quality()
         return a.value, b.value
 
@@ -283,23 +310,23 @@ class OracleMesh:
         doIteration); returns (2D, 1D) counts."""
         p = DmoParams(nq, depth, grid_scale, shrink, OR_EXTENT_MAX, fixed_boundary, nthreads)
         a, b = C.c_longlong(), C.c_longlong()
-        lib().or_dmo_iteration_surface_mesh(self.ptr, self.surf, C.byref(p), C.byref(a), C.byref(b))
+        self.L.or_dmo_iteration_surface_mesh(self.ptr, self.surf, C.byref(p), C.byref(a), C.byref(b))
         return a.value, b.value
 
     def surface_mesh_qualities(self) -> np.ndarray:
         out = np.empty(self.m.nE, dtype=np.float32)
-        lib().or_surface_mesh_qualities(self.ptr, out.ctypes.data_as(_fp))
+        self.L.or_surface_mesh_qualities(self.ptr, out.ctypes.data_as(_fp))
         return out
 
     def surface_mesh_min_quality(self) -> float:
-        return lib().or_surface_mesh_min_quality(self.ptr)
+        return self.L.or_surface_mesh_min_quality(self.ptr)
 
     def laplace(self, n_iter: int):
-        lib().or_laplace(self.ptr, int(n_iter))
+        self.L.or_laplace(self.ptr, int(n_iter))
 
     def surface_points(self, vid: int, nu: int, nv: int, mode: int, sid: int = 0, grid_scale: float = 0.5):
         out = np.zeros((nu * nv, 3), dtype=np.float32)
-        lib().or_surface_points(self.ptr, self.surf, vid, nu, nv, mode, sid, C.c_float(grid_scale),
+        self.L.or_surface_points(self.ptr, self.surf, vid, nu, nv, mode, sid, C.c_float(grid_scale),
                                 out.ctypes.data_as(_fp))
         return out
 
@@ -315,21 +342,67 @@ class OracleMesh:
             out = np.empty((n, 3), np.float32)
         else:
             out = np.empty((n, 10), np.float32)
-        lib().or_surface_get(self.surf, what, out.ctypes.data_as(C.c_void_p))
+        self.L.or_surface_get(self.surf, what, out.ctypes.data_as(C.c_void_p))
         return out
 
+    def sectors(self):
+        """(table int[nFeature, 20], row offsets int[nFeature + 1], fits
+        float[nSectors, 16])"""
+        nF = self.m.nFeature
+        table = np.empty((max(nF, 1), 20), np.int32)
+        frow = np.empty(nF + 1, np.int32)
+        self.L.or_surface_get_sectors(self.surf, table.ctypes.data_as(C.c_void_p), frow.ctypes.data_as(C.c_void_p), None)
+        fits = np.empty((max(int(frow[-1]), 1), 16), np.float32)
+        self.L.or_surface_get_sectors(self.surf, None, None, fits.ctypes.data_as(C.c_void_p))
+        return table[:nF], frow, fits[: int(frow[-1])]
+
+    def surface_search_vertex(self, vid: int, dim: int, nq=8, depth=3, grid_scale=0.5, shrink=0.0):
+        """One surface vertex's 2D (dim 2) or 1D (dim 1) search on the current
+        positions; returns (written, final position, final local (u, v))."""
+        extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, 0, 1)
+        a = self.positions().copy()
+        uv = np.zeros(2, np.float32)
+        w = self.L.or_surface_search_vertex(self.ptr, self.surf, C.byref(p), vid, dim, a.ctypes.data_as(_fp),
+                                            uv.ctypes.data_as(_fp))
+        return w, a[vid], uv
+
+    def surface_project(self, vid: int, u: float, v: float) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        self.L.or_surface_project(self.ptr, self.surf, vid, C.c_float(u), C.c_float(v), out.ctypes.data_as(_fp))
+        return out
+
+    def surface_update_nn(self):
+        self.L.or_surface_update_nn(self.ptr, self.surf)
+
     def min_quality(self) -> float:
-        return lib().or_min_quality(self.ptr)
+        return self.L.or_min_quality(self.ptr)
 
     def quality64(self):
         a, b = C.c_double(), C.c_double()
-        lib().or_quality64(self.ptr, C.byref(a), C.byref(b))
+        self.L.or_quality64(self.ptr, C.byref(a), C.byref(b))
         return a.value, b.value
 
     def element_qualities(self) -> np.ndarray:
         out = np.empty(self.m.nE, dtype=np.float32)
-        lib().or_element_qualities(self.ptr, out.ctypes.data_as(_fp))
+        self.L.or_element_qualities(self.ptr, out.ctypes.data_as(_fp))
         return out
+
+
+def search_ring(kind: str, ring, x0, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None):
+    """The oracle's vertex search on an explicit ring (tet: n x 3 points,
+    hex: n x 7 points, tri: a closed fan of n points); returns (written,
+    final position)."""
+    if extent is None:
+        extent = OR_EXTENT_MIN if kind == "hex" else OR_EXTENT_MAX
+    r = np.ascontiguousarray(ring, dtype=np.float32).reshape(-1, 3)
+    per = {"tet": 3, "hex": 7, "tri": 1}[kind]
+    x, xp = _f(np.asarray(x0, dtype=np.float32).reshape(3))
+    out = np.zeros(3, np.float32)
+    p = DmoParams(nq, depth, grid_scale, shrink, extent, 1, 1)
+    w = lib().or_search_ring(_ELEM[kind], r.shape[0] // per, r.ctypes.data_as(_fp), xp, C.byref(p),
+                             out.ctypes.data_as(_fp))
+    return w, out
 
 
 def fit_local_surface(pts, normal, refp) -> np.ndarray:

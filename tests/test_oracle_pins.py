@@ -1,0 +1,85 @@
+"""Pins the oracle's searches against the independent FP64 restatement of the
+reference (tests/ref64.py), CPU only.  Each test names the reference code it
+pins; tests/test_oracle_mutations.py shows that each check rejects a planted
+one-line mistake."""
+import oracle_pins as P
+
+
+def test_tet_search_matches_restatement(oracle):
+    """k_optimizeHierarchicalInnerTet (dmo/DMOTetImplCub.h:20-137) on the 125
+    interior vertices of a jittered 6^3 tet cube"""
+    r = P.check_volume_search(oracle, "tet", n=6, seed=5, jitter=0.35)
+    assert r["vertices"] == 125 and r["agree"] >= 120
+
+
+def test_tet_exact_tie_lowest_index(oracle):
+    """cub::ArgMax keeps the lower thread and each thread the first of equal
+    candidates: an exact tie goes to the lowest candidate index"""
+    P.check_tet_tie(oracle)
+
+
+def test_hex_metric_principal_axes(oracle):
+    """shapeMetricHex (dmo/DMOQualityHex.h:54-71) on hexes whose minimum is the
+    principal-axes sub-tet, and on random distorted hexes"""
+    P.check_hex_metric(oracle)
+
+
+def test_hex_search_matches_restatement(oracle):
+    """k_optimizeHierarchicalInnerHex (dmo/DMOHexImplCub.h:20-137): the window
+    is the per-axis minimum extent"""
+    r = P.check_volume_search(oracle, "hex", n=5, seed=7, jitter=0.2, min_clean=0.75)
+    assert r["vertices"] == 64
+
+
+def test_flat_search_matches_restatement(oracle):
+    """k_optimizeHierarchical2DFlat (dmo/DMOTriFlatImplCub.h:23-152)"""
+    P.check_flat_search(oracle)
+
+
+def test_flat_exact_tie_takes_b(oracle):
+    """an exact A/B tie goes to B in the flat search"""
+    P.check_flat_tie(oracle)
+
+
+def test_flat_last_level_write_rule(oracle):
+    """the flat search writes only when the last level's maximum beats the
+    start, dropping an earlier level's gain"""
+    P.check_flat_write_rule(oracle)
+
+
+def test_surface2d_search_tet(oracle):
+    """k_optimizeHierarchical2DTet (dmo/DMOTetImplCub.h:141-322): Euclidean
+    max window, A unless q_B > q_A, the last-level write check"""
+    P.check_surface_search(oracle, "tet", 2)
+
+
+def test_surface2d_search_hex(oracle):
+    """k_optimizeHierarchical2DHex (dmo/DMOHexImplCub.h:141-318): Euclidean
+    min window"""
+    P.check_surface_search(oracle, "hex", 2)
+
+
+def test_surface2d_exact_tie_takes_a(oracle):
+    P.check_surface_tie(oracle)
+
+
+def test_feature1d_search_tet(oracle):
+    """k_optimizeHierarchical1DTet (dmo/DMOTetImplCub.h:326-494)"""
+    P.check_surface_search(oracle, "tet", 1)
+
+
+def test_feature1d_search_hex(oracle):
+    """k_optimizeHierarchical1DHex (dmo/DMOHexImplCub.h:320-489)"""
+    P.check_surface_search(oracle, "hex", 1)
+
+
+def test_projection_blend_and_sectors(oracle):
+    """localSurfaceEstimationWithFeature (dmo/SurfaceEstimation.h:122-197),
+    including the feature neighbours' sector lookup"""
+    assert P.check_projection(oracle) <= 1e-6
+
+
+def test_nearest_neighbour_walk(oracle):
+    """k_updateNearestNeighbor (dmo/DMOCommon.cu:11-43) against the exhaustive
+    nearest initial position of the vertex's class"""
+    assert P.check_nn_walk(oracle) >= 0.95
