@@ -48,6 +48,67 @@ static inline float gridc(float aff, int i, int nq, float gmin, float gmax)
     return aff * ((float)i * gmin + (float)(nq - 1 - i) * gmax);
 }
 
+/* One-ring of a volume vertex in its local frame (p' = (p - x0) * sigma):
+   the window half-extents, the scale and the element constants of the
+   canonical evaluation (or_internal.h). */
+typedef struct {
+    int tet, n;
+    v3 x0;
+    float ext[3], sigma, is3, epsD, epsF;
+    tetc tcs[MAXR];
+    float cd[4 * 64], cF[4 * 64];
+    hexp hp[5 * 64];
+    hexacc hacc;
+} ringctx;
+
+/* ringPts: n elements of 3 (tet halfface) or 7 (halfhex) points, world
+   coordinates; emin: per-axis minimum extent instead of the maximum */
+static void ring_setup(ringctx* r, int tet, int n, const float* ringPts, const float* x0p, int emin)
+{
+    const int st = tet ? 3 : 7;
+    r->tet = tet; r->n = n;
+    r->x0 = ld3(x0p);
+    for (int a = 0; a < 3; ++a) r->ext[a] = emin ? FLT_MAX : -FLT_MAX;
+    float mabs = 0.0f;
+    for (int e = 0; e < n; ++e)
+        for (int k = 0; k < st; ++k) {
+            const float* q = ringPts + 3 * (size_t)(e * st + k);
+            float d[3] = { fabsf(q[0] - r->x0.x), fabsf(q[1] - r->x0.y), fabsf(q[2] - r->x0.z) };
+            for (int a = 0; a < 3; ++a) {
+                r->ext[a] = emin ? fminf(r->ext[a], d[a]) : fmaxf(r->ext[a], d[a]);
+                mabs = fmaxf(mabs, d[a]);
+            }
+        }
+    r->sigma = or_pow2_scale(mabs);
+    float s2 = r->sigma * r->sigma, s3 = s2 * r->sigma;
+    r->is3 = invpow(r->sigma, 3);
+    r->epsD = HEX_EPS * s3; r->epsF = HEX_EPS * s2;
+    r->hacc = hex_acc_init();
+    if (tet) {
+        for (int e = 0; e < n; ++e) {
+            const float* h = ringPts + 9 * (size_t)e;
+            r->tcs[e] = tet_consts(loc3(h, r->x0, r->sigma), loc3(h + 3, r->x0, r->sigma), loc3(h + 6, r->x0, r->sigma));
+        }
+    } else {
+        for (int e = 0; e < n; ++e) {
+            const float* h = ringPts + 21 * (size_t)e;
+            v3 q[8];
+            q[0] = r->x0;
+            for (int k = 1; k < 8; ++k) q[k] = loc3(h + 3 * (k - 1), r->x0, r->sigma);
+            hex_consts(q, r->cd + 4 * e, r->cF + 4 * e, r->hp + 5 * e);
+        }
+        r->hacc = hex_const_acc(n, r->cd, r->cF);
+    }
+}
+
+/* ring score at the world point (x, y, z) */
+static inline float ring_score_at(const ringctx* r, float x, float y, float z)
+{
+    float px = (x - r->x0.x) * r->sigma, py = (y - r->x0.y) * r->sigma, pz = (z - r->x0.z) * r->sigma;
+    return r->tet ? tet_score(r->n, r->tcs, px, py, pz, r->is3)
+                  : hex_score(r->n, r->hacc, r->hp, px, py, pz, r->epsD, r->epsF);
+}
+
 /* Search of one vertex on an explicit ring: ringPts holds n elements of st
    points (3 per tet halfface, 7 per halfhex), world coordinates; x0 is the
    vertex's start position.  Writes the final position to out (x0 when the
@@ -55,72 +116,32 @@ static inline float gridc(float aff, int i, int nq, float gmin, float gmax)
 static int search_volume_ring(int tet, int n, const float* ringPts, const float* x0p, const or_dmo_params* p,
                               float* out)
 {
-    const int st = tet ? 3 : 7;
     out[0] = x0p[0]; out[1] = x0p[1]; out[2] = x0p[2];
     if (n <= 0 || n > (tet ? MAXR : 64)) return 0;
-    v3 x0 = ld3(x0p);
     int emin = (p->extent_mode == OR_EXTENT_MIN);
 #if OR_MUTATION == 1
     emin = !emin;                      /* planted: min and max extent swapped */
 #endif
-    float ext[3] = { emin ? FLT_MAX : -FLT_MAX, emin ? FLT_MAX : -FLT_MAX, emin ? FLT_MAX : -FLT_MAX };
-    float mabs = 0.0f;
-    for (int e = 0; e < n; ++e)
-        for (int k = 0; k < st; ++k) {
-            const float* q = ringPts + 3 * (size_t)(e * st + k);
-            float d[3] = { fabsf(q[0] - x0.x), fabsf(q[1] - x0.y), fabsf(q[2] - x0.z) };
-            for (int a = 0; a < 3; ++a) {
-                ext[a] = emin ? fminf(ext[a], d[a]) : fmaxf(ext[a], d[a]);
-                mabs = fmaxf(mabs, d[a]);
-            }
-        }
-    float sigma = or_pow2_scale(mabs);
-    float s2 = sigma * sigma, s3 = s2 * sigma;
-    float is3 = invpow(sigma, 3);
-
-    tetc tcs[MAXR];
-    float cd[4 * 64], cF[4 * 64];
-    hexp hp[5 * 64];
-    hexacc hacc = hex_acc_init();
-    if (tet) {
-        for (int e = 0; e < n; ++e) {
-            const float* h = ringPts + 9 * (size_t)e;
-            tcs[e] = tet_consts(loc3(h, x0, sigma), loc3(h + 3, x0, sigma), loc3(h + 6, x0, sigma));
-        }
-    } else {
-        for (int e = 0; e < n; ++e) {
-            const float* h = ringPts + 21 * (size_t)e;
-            v3 r[8];
-            r[0] = x0;
-            for (int k = 1; k < 8; ++k) r[k] = loc3(h + 3 * (k - 1), x0, sigma);
-            hex_consts(r, cd + 4 * e, cF + 4 * e, hp + 5 * e);
-        }
-        hacc = hex_const_acc(n, cd, cF);
-    }
-    const float epsD = HEX_EPS * s3, epsF = HEX_EPS * s2;
-#define SCORE(px, py, pz) (tet ? tet_score(n, tcs, (px), (py), (pz), is3) : hex_score(n, hacc, hp, (px), (py), (pz), epsD, epsF))
-
+    ringctx* r = (ringctx*)malloc(sizeof(ringctx));
+    ring_setup(r, tet, n, ringPts, x0p, emin);
     const int nq = p->nq;
     const float aff = 1.0f / (float)(nq - 1);
     const float shr = shrink_of(p);
-    float cur[3] = { x0.x, x0.y, x0.z };
-    float qcur = SCORE(0.0f, 0.0f, 0.0f);
+    float cur[3] = { x0p[0], x0p[1], x0p[2] };
+    float qcur = ring_score_at(r, cur[0], cur[1], cur[2]);
     float ds = p->grid_scale;
     int moved = 0;
     for (int d = 0; d < p->depth; ++d) {
         float gmax[3], gmin[3];
-        for (int a = 0; a < 3; ++a) { gmax[a] = cur[a] + ds * ext[a]; gmin[a] = cur[a] - ds * ext[a]; }
+        for (int a = 0; a < 3; ++a) { gmax[a] = cur[a] + ds * r->ext[a]; gmin[a] = cur[a] - ds * r->ext[a]; }
         float best = -FLT_MAX, bp[3] = { 0, 0, 0 };
         for (int k = 0; k < nq; ++k) {
             float pzg = gridc(aff, k, nq, gmin[2], gmax[2]);
-            float pz = (pzg - x0.z) * sigma;
             for (int j = 0; j < nq; ++j) {
                 float pyg = gridc(aff, j, nq, gmin[1], gmax[1]);
-                float py = (pyg - x0.y) * sigma;
                 for (int i = 0; i < nq; ++i) {
                     float pxg = gridc(aff, i, nq, gmin[0], gmax[0]);
-                    float px = (pxg - x0.x) * sigma;
-                    float q = SCORE(px, py, pz);
+                    float q = ring_score_at(r, pxg, pyg, pzg);
 #if OR_MUTATION == 6
                     if (q >= best) {           /* planted: the highest index wins a tie */
 #else
@@ -134,9 +155,63 @@ static int search_volume_ring(int tet, int n, const float* ringPts, const float*
         if (qcur < best) { cur[0] = bp[0]; cur[1] = bp[1]; cur[2] = bp[2]; qcur = best; moved = 1; }
         ds = ds * shr;
     }
-#undef SCORE
+    free(r);
     if (moved) { out[0] = cur[0]; out[1] = cur[1]; out[2] = cur[2]; }
     return moved;
+}
+
+/* Learned proposal window of one inner vertex (this build's own search, no
+   reference counterpart): a 4x4x4 grid (candidate i + 4j + 16k, the lowest
+   index wins ties) on frac times the plain first-level window around the
+   proposal c, for depthL levels with the window shrunk by 2/3 per level;
+   the centre moves to a level's best candidate when it beats the best
+   quality so far, which starts at q(x0).  Returns 1 when the vertex moved
+   (out = its new position), 0 when it stays because a level's best tied a
+   positive q(x0) or a hex vertex is pinned (its sub-tets without it already
+   score 0), -1 when it needs the plain full search (fallback). */
+static int learned_window_ring(int tet, int n, const float* ringPts, const float* x0p, const float* c,
+                               const or_dmo_params* p, int depthL, float frac, float* out)
+{
+    out[0] = x0p[0]; out[1] = x0p[1]; out[2] = x0p[2];
+    if (n <= 0 || n > (tet ? MAXR : 64)) return -1;
+    ringctx* r = (ringctx*)malloc(sizeof(ringctx));
+    ring_setup(r, tet, n, ringPts, x0p, p->extent_mode == OR_EXTENT_MIN);
+    const int NL = 4;
+    const float aff = 1.0f / (float)(NL - 1);
+    const float q0 = ring_score_at(r, x0p[0], x0p[1], x0p[2]);
+    float cur[3] = { c[0], c[1], c[2] };
+    float qbest = q0;
+    int moved = 0, tie = 0;
+    float ds = frac * p->grid_scale;
+    for (int d = 0; d < depthL; ++d) {
+        float gmax[3], gmin[3];
+        for (int a = 0; a < 3; ++a) { gmax[a] = cur[a] + ds * r->ext[a]; gmin[a] = cur[a] - ds * r->ext[a]; }
+        float best = -FLT_MAX, bp[3] = { 0, 0, 0 };
+        for (int k = 0; k < NL; ++k) {
+            float pzg = gridc(aff, k, NL, gmin[2], gmax[2]);
+            for (int j = 0; j < NL; ++j) {
+                float pyg = gridc(aff, j, NL, gmin[1], gmax[1]);
+                for (int i = 0; i < NL; ++i) {
+                    float pxg = gridc(aff, i, NL, gmin[0], gmax[0]);
+                    float q = ring_score_at(r, pxg, pyg, pzg);
+                    if (q > best) { best = q; bp[0] = pxg; bp[1] = pyg; bp[2] = pzg; }
+                }
+            }
+        }
+#if OR_MUTATION != 12                  /* 12 plants: the positive-tie rule dropped */
+        if (best == q0 && q0 > 0.0f) tie = 1;
+#endif
+        if (qbest < best) { cur[0] = bp[0]; cur[1] = bp[1]; cur[2] = bp[2]; qbest = best; moved = 1; }
+#if OR_MUTATION == 13
+        ds = ds * 0.5f;                /* planted: the window halved per level */
+#else
+        ds = ds * (2.0f / (float)(NL - 1));
+#endif
+    }
+    const int pinned = !tet && (r->hacc.dmin < r->epsD || r->hacc.fmin < r->epsF);
+    free(r);
+    if (moved) { out[0] = cur[0]; out[1] = cur[1]; out[2] = cur[2]; return 1; }
+    return (tie || pinned) ? 0 : -1;
 }
 
 static int vertex_volume(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz)
@@ -244,6 +319,24 @@ static int vertex_flat(const or_mesh* m, const or_dmo_params* p, int vid, float*
     return w;
 }
 
+int or_learned_window(const or_mesh* m, const or_dmo_params* p, int vid, const float* xyz, const float* c,
+                      int depthL, float frac, float* out)
+{
+    int tet = (m->elem == OR_TET);
+    int st = m->ringStride;
+    int r0 = m->ringRow[vid], n = m->ringRow[vid + 1] - r0;
+    const float* x0 = xyz + 3 * (size_t)vid;
+    if (n <= 0 || n > (tet ? MAXR : 64)) { out[0] = x0[0]; out[1] = x0[1]; out[2] = x0[2]; return -1; }
+    float pts[MAXR * 9];
+    for (int e = 0; e < n; ++e)
+        for (int k = 0; k < st; ++k) {
+            const float* q = xyz + 3 * (size_t)m->ringVal[(size_t)(r0 + e) * st + k];
+            float* o = pts + 3 * (size_t)(e * st + k);
+            o[0] = q[0]; o[1] = q[1]; o[2] = q[2];
+        }
+    return learned_window_ring(tet, n, pts, x0, c, p, depthL, frac, out);
+}
+
 int or_search_ring(int elem, int n, const float* ringPts, const float* x0, const or_dmo_params* p, float* out)
 {
     if (elem == OR_TET || elem == OR_HEX) return search_volume_ring(elem == OR_TET, n, ringPts, x0, p, out);
@@ -268,6 +361,7 @@ long long or_dmo_iteration(or_mesh* m, const or_dmo_params* p)
     if (m->elem == OR_TET || m->elem == OR_HEX) {
         for (int c = 0; c < m->nColorsInner; ++c) {
             int lo = m->colOffInner[c], hi = m->colOffInner[c + 1];
+            if (p->max_per_class > 0 && hi - lo > p->max_per_class) hi = lo + p->max_per_class;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif
@@ -277,6 +371,7 @@ long long or_dmo_iteration(or_mesh* m, const or_dmo_params* p)
     } else if (m->elem == OR_TRI && m->flat) {
         for (int c = 0; c < m->nColorsFree; ++c) {
             int lo = m->colOffFree[c], hi = m->colOffFree[c + 1];
+            if (p->max_per_class > 0 && hi - lo > p->max_per_class) hi = lo + p->max_per_class;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif

@@ -113,6 +113,7 @@ static void fit_quadric(lsurf* s, const w3* pts, int np)
 
 struct or_surf {
     int nS, nFree, nF;
+    int err;              /* 1: a projection found no sector, 2: unsectored neighbours (as the kernels' bits) */
     float* pinit;         /* 3*nS: surface positions at optimiser creation */
     float* vnorm;         /* 3*nS */
     float* fnorm;         /* 3*nFaces */
@@ -224,12 +225,13 @@ or_surf* or_surface_init(const or_mesh* m)
         int sid = 0, nneigh = 0, lastWasFeature = 1, last = h0, vindex = 1;
         s->table[TABLE_SIZE * i] = 0;
         const int nsec = s->frow[i + 1] - s->frow[i];
-        /* non-feature neighbours after the last feature neighbour of an open
-           boundary vertex belong to no sector: they are dropped (sid == nsec) */
+        /* neighbours after the last feature neighbour of an open fan belong
+           to no sector (sid == nsec): recorded in err, left out of the lists */
         for (int h = h0 + 1; h < h1; ++h) {
             int d = m->heTarget[h];
             int* L = secN + (size_t)MAX_NEIGH * (s->frow[i] + (sid < nsec ? sid : 0));
             const int own = sid < nsec;
+            if (!own) s->err |= 2;
             if (lastWasFeature) { nneigh = 0; if (own) { L[nneigh++] = v; L[nneigh++] = m->heTarget[last]; } }
             if (m->isFeature[d]) {
                 if (vindex < TABLE_SIZE) s->table[TABLE_SIZE * i + vindex] = sid;
@@ -336,7 +338,14 @@ static w3 project2d(const or_mesh* m, const or_surf* s, const float* snap, int v
             }
             int sid = (arg < TABLE_SIZE) ? s->table[TABLE_SIZE * off + arg] : -1;
             const int nsec = s->frow[off + 1] - s->frow[off];
-            if (sid < 0 || sid >= nsec) sid = 0;
+            if (sid < 0 || sid >= nsec) {
+                or_surf* sw = (or_surf*)s;
+#ifdef _OPENMP
+#pragma omp atomic update
+#endif
+                sw->err |= 1;
+                sid = 0;
+            }
 #if OR_MUTATION == 11
             sid = 0;                   /* planted: the first sector always */
 #endif
@@ -1086,6 +1095,8 @@ void or_surface_get(const or_surf* s, int what, void* out)
 }
 
 int or_surface_nsectors(const or_surf* s, int nFeature) { return s->frow[nFeature]; }
+
+int or_surface_error(const or_surf* s) { return s->err; }
 
 void or_surface_get_sectors(const or_surf* s, int* table, int* frow, float* surf16)
 {

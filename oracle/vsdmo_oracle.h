@@ -128,6 +128,8 @@ typedef struct or_dmo_params {
     int extent_mode;     /* OR_EXTENT_MAX (tet/tri) or OR_EXTENT_MIN (hex) */
     int fixed_boundary;  /* 1: surface/feature passes skipped */
     int nthreads;        /* OpenMP threads, 0 = default */
+    int max_per_class;   /* > 0: search only the first max_per_class vertices of
+                            each colour class (a timing sample); 0 = all */
 } or_dmo_params;
 
 /* one DMO iteration over the inner colour classes (volume meshes) or the free
@@ -141,6 +143,14 @@ int or_dmo_vertex(const or_mesh* m, const or_dmo_params* p, int vid, float* xyz_
    nElem points; x0 = start position; out = final position (x0 when the vertex
    is not written).  Returns 1 iff the position is written. */
 int or_search_ring(int elem, int nElem, const float* ringPts, const float* x0, const or_dmo_params* p, float* out3);
+/* the learned proposal's window search of inner vertex vid (tet / hex) around
+   the proposal c (world coordinates) on the positions xyz: depthL levels of
+   a 4x4x4 grid on frac x the plain first-level window, shrunk by 2/3 per
+   level.  Returns 1 when the vertex moves (out = its new position), 0 when
+   it stays (a level's best ties a positive q(x0), or a hex vertex is
+   pinned), -1 when the vertex takes the plain full search (fallback). */
+int or_learned_window(const or_mesh* m, const or_dmo_params* p, int vid, const float* xyz, const float* c,
+                      int depthL, float frac, float* out3);
 
 /* ---- surface vertices ---- */
 typedef struct or_surf or_surf;
@@ -168,6 +178,10 @@ void or_laplace(or_mesh* m, int n_iter);
    floats: p0, e0, e1, a), 4 vertex normals (3 floats) */
 void or_surface_get(const or_surf* s, int what, void* out);
 int or_surface_nsectors(const or_surf* s, int nFeature);
+/* the surface errors met so far (the optimiser's VSDMO_TOPOLOGY cases): bit 0
+   a projection found no sector for a feature neighbour, bit 1 a feature
+   vertex has neighbours that belong to no sector */
+int or_surface_error(const or_surf* s);
 /* feature sectors: the halfedge-to-sector table (TABLE_SIZE = 20 ints per
    feature vertex), the sector row offsets (nFeature + 1) and the sector fits
    (16 floats each, as the local surfaces) */

@@ -329,3 +329,30 @@ def check_nn_walk(o, kind="hex", step=0.6):
         exact += nn[v] == best[v] or d2[v, best[v]] >= dn * (1 - 1e-6)
     assert exact >= 0.95 * m.nSurf, (exact, m.nSurf)
     return exact / m.nSurf
+
+
+def check_learned_window(o, kind="tet", depth_l=1, frac=0.25, spread=0.3, seed=9, min_clean=0.8):
+    """The oracle's learned window (or_learned_window) against
+    ref64.learned_window on every interior vertex of a jittered cube, with
+    proposals scattered around the vertex by up to `spread` h."""
+    n, s, jit = (6, 5, 0.35) if kind == "tet" else (5, 7, 0.2)
+    M = _volume_mesh(o, kind, n, s, jit)
+    row, val = M.ring()
+    pos = M.positions()
+    m = M.m
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    clean = 0
+    status = {-1: 0, 0: 0, 1: 0}
+    for v in range(m.nSurf, m.nV):
+        ring = pos[val[row[v]:row[v + 1]].ravel()]
+        c = (pos[v] + rng.uniform(-spread * h, spread * h, size=3)).astype(np.float32)
+        ref = ref64.learned_window(kind, pos[v], ring, c, depth_l, frac)
+        st, out = M.learned_window(v, pos, c, depth_l, frac)
+        if ref.margin < MARGIN:
+            continue
+        clean += 1
+        status[st] += 1
+        assert st == ref.extra["status"] and np.array_equal(out, ref.pos), (v, st, ref.extra["status"], out, ref.pos)
+    assert clean >= min_clean * (m.nV - m.nSurf), (clean, m.nV - m.nSurf)
+    return status

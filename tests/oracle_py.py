@@ -43,7 +43,8 @@ class BuildParams(C.Structure):
 
 class DmoParams(C.Structure):
     _fields_ = [("nq", C.c_int), ("depth", C.c_int), ("grid_scale", C.c_float), ("shrink", C.c_float),
-                ("extent_mode", C.c_int), ("fixed_boundary", C.c_int), ("nthreads", C.c_int)]
+                ("extent_mode", C.c_int), ("fixed_boundary", C.c_int), ("nthreads", C.c_int),
+                ("max_per_class", C.c_int)]
 
 
 _lib = None
@@ -136,6 +137,10 @@ def _configure(L):
                                            _fp, _fp]
     L.or_surface_project.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.c_int, C.c_float, C.c_float, _fp]
     L.or_surface_update_nn.argtypes = [C.POINTER(OrMesh), C.c_void_p]
+    L.or_surface_error.restype = C.c_int
+    L.or_surface_error.argtypes = [C.c_void_p]
+    L.or_learned_window.restype = C.c_int
+    L.or_learned_window.argtypes = [C.POINTER(OrMesh), C.POINTER(DmoParams), C.c_int, _fp, _fp, C.c_int, C.c_float, _fp]
     return L
 
 
@@ -272,10 +277,13 @@ class OracleMesh:
         m = self.m
         return _arr(m.isBoundary1D, m.nSurf, np.uint8), _arr(m.isFeature, m.nSurf, np.uint8)
 
-    def iterate(self, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None, fixed_boundary=1, nthreads=0) -> int:
+    def iterate(self, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None, fixed_boundary=1, nthreads=0,
+                max_per_class=0) -> int:
+        """one iteration; max_per_class > 0 searches only the first vertices
+        of each colour class (a timing sample).  Returns the searches run."""
         if extent is None:
             extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
-        p = DmoParams(nq, depth, grid_scale, shrink, extent, fixed_boundary, nthreads)
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, fixed_boundary, nthreads, max_per_class)
         return self.L.or_dmo_iteration(self.ptr, C.byref(p))
 
     def search_vertex(self, vid, xyz, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None):
@@ -285,6 +293,19 @@ class OracleMesh:
         a = np.ascontiguousarray(xyz, dtype=np.float32).copy()
         moved = self.L.or_dmo_vertex(self.ptr, C.byref(p), vid, a.ctypes.data_as(_fp))
         return moved, a
+
+    def learned_window(self, vid, xyz, c, depth_l, frac, nq=8, depth=3, grid_scale=0.5):
+        """the learned proposal's window search of inner vertex vid around c
+        on the positions xyz: (status, position); status 1 moved, 0 stays,
+        -1 falls back to the full search"""
+        extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
+        p = DmoParams(nq, depth, grid_scale, 0.0, extent, 1, 1)
+        a = np.ascontiguousarray(xyz, dtype=np.float32)
+        cc = np.ascontiguousarray(c, dtype=np.float32).reshape(3)
+        out = np.zeros(3, np.float32)
+        st = self.L.or_learned_window(self.ptr, C.byref(p), vid, a.ctypes.data_as(_fp), cc.ctypes.data_as(_fp),
+                                      depth_l, C.c_float(frac), out.ctypes.data_as(_fp))
+        return st, out
 
     def surface_init(self):
         """Local surfaces etc. from the current positions (optimiser init)."""
@@ -366,6 +387,11 @@ class OracleMesh:
         w = self.L.or_surface_search_vertex(self.ptr, self.surf, C.byref(p), vid, dim, a.ctypes.data_as(_fp),
                                             uv.ctypes.data_as(_fp))
         return w, a[vid], uv
+
+    def surface_error(self) -> int:
+        """bit 0: a projection found no feature sector; bit 1: a feature vertex
+        has neighbours that belong to no sector"""
+        return self.L.or_surface_error(self.surf)
 
     def surface_project(self, vid: int, u: float, v: float) -> np.ndarray:
         out = np.zeros(3, np.float32)

@@ -461,3 +461,57 @@ def nearest_same_class(pos, pinit, is_feature):
     d2 = np.where(same, d2, np.inf)
     idx = np.argmin(d2, axis=1)
     return idx, d2[np.arange(len(pos)), idx], d2
+
+
+# --------------------------------------------------- learned proposal window
+def learned_window(kind: str, x0, ring, c, depth_l, frac, grid_scale=0.5) -> Result:
+    """The learned proposal's window search, restated from its specification
+    (this build's own search; no reference counterpart): 4 x 4 x 4
+    candidates i + 4j + 16k (the lowest index wins ties) on `frac` times the
+    plain first-level window (per-axis max extent for tets, min for hexes)
+    around the proposal c, `depth_l` levels shrinking by 2/3; the centre
+    moves to a level's best candidate when it beats the best quality so far,
+    which starts at q(x0).  extra["status"]: 1 moved, 0 stays (a level's best
+    ties a positive q(x0), or a hex vertex whose sub-tets without it already
+    fail the shape metric's thresholds), -1 falls back to the full search."""
+    x0 = np.asarray(x0, F).reshape(3)
+    pts = np.asarray(ring, F).reshape(-1, 3)
+    d = np.abs(pts - x0)
+    ext = d.min(axis=0) if kind == "hex" else d.max(axis=0)
+    nl = 4
+    aff = F(1.0) / F(nl - 1)
+    idx = np.arange(nl ** 3)
+    ii, jj, kk = idx % nl, (idx // nl) % nl, idx // (nl * nl)
+    q0 = float(ring_quality(kind, pts, x0))
+    cur = np.asarray(c, F).reshape(3).copy()
+    qbest = q0
+    moved = tie = False
+    margin = np.inf
+    ds = F(frac) * F(grid_scale)
+    for _ in range(depth_l):
+        gmax = cur + ds * ext
+        gmin = cur - ds * ext
+        cand = np.stack([_grid(aff, ii, nl, gmin[0], gmax[0]), _grid(aff, jj, nl, gmin[1], gmax[1]),
+                         _grid(aff, kk, nl, gmin[2], gmax[2])], axis=1)
+        q = ring_quality(kind, pts, cand)
+        b = 0
+        for k in range(1, len(q)):
+            if gt(q[k], q[b]):
+                b = k
+        margin = min(margin, _top_gap(q), rel_gap(q[b], qbest), rel_gap(q[b], q0))
+        if eq(q[b], q0) and q0 > 0.0:
+            tie = True
+        if gt(q[b], qbest):
+            cur = cand[b].copy()
+            qbest = q[b]
+            moved = True
+        ds = ds * (F(2.0) / F(nl - 1))
+    pinned = False
+    if kind == "hex":
+        els = pts.reshape(-1, 7, 3).astype(np.float64)
+        P = np.concatenate([np.broadcast_to(x0.astype(np.float64), (len(els), 1, 3)), els], axis=1)
+        det, jn = hex_subtets(P)
+        free = [1, 2, 3, 6]                         # the sub-tets without vertex 0
+        pinned = bool(np.any((det[:, free] < float(EPS)) | (jn[:, free] < float(EPS))))
+    status = 1 if moved else (0 if (tie or pinned) else -1)
+    return Result(cur if moved else x0, moved, margin, {"status": status})

@@ -69,6 +69,7 @@ def test_surface_dmo_oracle(oracle, kind):
     assert np.all(onb >= 2)
     nnt = M.surface_get(0)
     assert np.all((nnt >= 0) & (nnt < m.nSurf))
+    assert M.surface_error() == 0                  # every feature lookup found its sector
 
 
 def test_flat_boundary_1d_oracle(oracle):

@@ -27,6 +27,8 @@ MUTATIONS = {
     9: ("nearest-neighbour walk: feature-class filter dropped", P.check_nn_walk),
     10: ("feature 1D window: tet and hex distance rules swapped", lambda o: P.check_surface_search(o, "tet", 1)),
     11: ("projection: a feature neighbour always uses its first sector", P.check_projection),
+    12: ("learned window: a tie at a positive q(v) falls back", lambda o: P.check_learned_window(o, "hex", 1, 2.0)),
+    13: ("learned window: halved per level instead of 2/3", lambda o: P.check_learned_window(o, "tet", 2, 0.5)),
 }
 
 

@@ -83,3 +83,15 @@ def test_nearest_neighbour_walk(oracle):
     """k_updateNearestNeighbor (dmo/DMOCommon.cu:11-43) against the exhaustive
     nearest initial position of the vertex's class"""
     assert P.check_nn_walk(oracle) >= 0.95
+
+
+def test_learned_window_matches_restatement(oracle):
+    """the learned proposal's window search (this build's own; its
+    specification restated in tests/ref64.py) at the shipped windows, tet
+    (1 level, 0.025) and hex (1, 2.0), the specified 1/4 window, and two
+    levels"""
+    for kind, depth, frac in (("tet", 1, 0.025), ("tet", 1, 0.25), ("tet", 2, 0.5), ("hex", 1, 2.0), ("hex", 2, 0.5)):
+        st = P.check_learned_window(oracle, kind, depth, frac)
+        assert st[1] > 0 and st[-1] > 0, (kind, depth, frac, st)
+        if kind == "hex":
+            assert st[0] > 0, st          # vertices kept in place by the positive-tie / pinned rules
