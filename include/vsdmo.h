@@ -144,6 +144,13 @@ vsdmo_status vsdmo_mesh_get_cells(const vsdmo_mesh* mesh, int32_t* out);
 vsdmo_status vsdmo_mesh_get_coloring(const vsdmo_mesh* mesh, int32_t which, int32_t* offsets);
 /* per-vertex colour within its class family, nV */
 vsdmo_status vsdmo_mesh_get_colors(const vsdmo_mesh* mesh, int32_t* colors);
+/* Checks a partition of the vertex range [offsets[0], offsets[n_classes]) of
+   the current numbering into the classes [offsets[c], offsets[c+1]): no
+   element may hold two vertices of one class (the rule of k_checkColoring,
+   mesh/MeshGPUCommon.cu:1470-1680, called after each colour sort, e.g.
+   mesh/MeshTetGPU.cu:385).  VSDMO_TOPOLOGY names the first offending class.
+   vsdmo_mesh_create runs the same check on every colouring it builds. */
+vsdmo_status vsdmo_mesh_check_coloring(const vsdmo_mesh* mesh, const int32_t* offsets, int32_t n_classes);
 /* boundary faces of a volume mesh (tri/quad, renumbered ids), fpe*n_faces */
 vsdmo_status vsdmo_mesh_get_faces(const vsdmo_mesh* mesh, int32_t* out);
 /* volume one-ring CSR (halffaces / halfhexes, mesh/MeshTetGPU.cu:113-142,

@@ -122,6 +122,13 @@ class DMOMesh:
         check(self._lib.vsdmo_mesh_get_coloring(self._h, idx, _iptr(out)), "get_coloring")
         return out
 
+    def check_coloring(self, offsets):
+        """Checks a class partition [offsets[c], offsets[c+1]) of the current
+        numbering: VsdmoError (VSDMO_TOPOLOGY) when an element holds two
+        vertices of one class."""
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        check(self._lib.vsdmo_mesh_check_coloring(self._h, _iptr(off), len(off) - 1), "check_coloring")
+
     def colors(self) -> np.ndarray:
         out = np.empty(self.nVertices, dtype=np.int32)
         check(self._lib.vsdmo_mesh_get_colors(self._h, _iptr(out)), "get_colors")

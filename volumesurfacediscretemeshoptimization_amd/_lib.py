@@ -74,6 +74,7 @@ EXPORTS = {
     "vsdmo_mesh_get_cells": (C.c_int, [C.c_void_p, _ip]),
     "vsdmo_mesh_get_coloring": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
     "vsdmo_mesh_get_colors": (C.c_int, [C.c_void_p, _ip]),
+    "vsdmo_mesh_check_coloring": (C.c_int, [C.c_void_p, _ip, C.c_int32]),
     "vsdmo_mesh_get_faces": (C.c_int, [C.c_void_p, _ip]),
     "vsdmo_mesh_get_onering": (C.c_int, [C.c_void_p, _ip, _ip]),
     "vsdmo_mesh_get_halfedges": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, _ip]),

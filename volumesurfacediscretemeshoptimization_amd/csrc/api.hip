@@ -17,6 +17,7 @@
 namespace vsdmo {
 void build_volume(vsdmo_mesh& m, const vsdmo_mesh_params& prm);
 void build_surface_mesh(vsdmo_mesh& m, const vsdmo_mesh_params& prm);
+void check_classes(const vsdmo_mesh& m, const std::vector<int>& off);
 
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
@@ -106,7 +107,7 @@ struct vsdmo_opt {
     int64_t launches = 0;
     DevBuf<float> q, pmin;
     DevBuf<double> psum;
-    DevBuf<double> qout;   // [0]: minimum (float in the first 4 bytes), [1]: FP64 sum
+    DevBuf<double> qout;   // [0]: minimum (float in bytes 0-3, surface error bits in 4-7), [1]: FP64 sum
     PinnedQ hq;            // their host copy
     std::vector<hipEvent_t> ev;   // pool of events; pairs [0, ev_used) bracket timed launches
     size_t ev_used = 0;
@@ -325,6 +326,19 @@ vsdmo_status vsdmo_mesh_get_coloring(const vsdmo_mesh* m, int32_t which, int32_t
     return VSDMO_OK;
 }
 
+vsdmo_status vsdmo_mesh_check_coloring(const vsdmo_mesh* m, const int32_t* offsets, int32_t n_classes)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!m || !offsets || n_classes < 1) return fail(VSDMO_INVALID_ARG, "bad argument");
+        std::vector<int> off(offsets, offsets + n_classes + 1);
+        if (off[0] < 0 || off[n_classes] > m->nV) return fail(VSDMO_INVALID_ARG, "offsets outside [0, nV]");
+        for (int c = 0; c < n_classes; ++c)
+            if (off[c + 1] < off[c]) return fail(VSDMO_INVALID_ARG, "offsets must not decrease");
+        check_classes(*m, off);
+        return VSDMO_OK;
+    });
+}
+
 vsdmo_status vsdmo_mesh_get_colors(const vsdmo_mesh* m, int32_t* colors)
 {
     if (!m) return fail(VSDMO_INVALID_ARG, "NULL mesh");
@@ -395,9 +409,17 @@ static void opt_quality(vsdmo_opt* o)
     VSDMO_HIP(hipMemcpyAsync(o->hq.p, o->qout.p, 2 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
     VSDMO_HIP(hipStreamSynchronize(m->stream));
     float mn = 0.0f;
+    int err = 0;
     std::memcpy(&mn, o->hq.p, sizeof(float));
+    std::memcpy(&err, reinterpret_cast<const char*>(o->hq.p) + sizeof(float), sizeof(int));
     o->last_min = mn;
     o->last_sum = o->hq.p[1];
+    if (err & kSurfErrUnsectored)
+        throw StatusError{ VSDMO_TOPOLOGY, "a feature vertex has surface neighbours after its last feature "
+                                           "neighbour (open fan): they belong to no feature sector" };
+    if (err & kSurfErrNoSector)
+        throw StatusError{ VSDMO_TOPOLOGY, "projection onto a feature neighbour's sectors found no sector for "
+                                           "its halfedge nearest to the vertex" };
 }
 
 // empty deferred lists and no deferral epochs (allocated on first use)
@@ -433,6 +455,7 @@ static SurfArgs surf_args(vsdmo_opt* o)
     a.pinit = o->pinit.p; a.fnorm = o->fnorm.p; a.vnorm = o->vnorm.p; a.ls = o->ls.p; a.ls1 = o->ls1.p; a.fs = o->fs.p;
     a.nn = o->nn.p; a.frow = o->frow.p; a.table = o->table.p; a.opt1d = o->opt1d.p;
     a.gridScale = o->grid_scale; a.shrink = o->shrink; a.aff = o->aff; a.depth = o->p.depth;
+    a.err = reinterpret_cast<int*>(o->qout.p) + 1;
     return a;
 }
 
@@ -521,6 +544,7 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
             o->q.alloc(m->nE);
             o->pmin.alloc(nb); o->psum.alloc(nb);
             o->qout.alloc(2);
+            VSDMO_HIP(hipMemsetAsync(o->qout.p, 0, 2 * sizeof(double), m->stream));
             o->hq.alloc();
             if (!p.fixed_boundary || surfMesh) surface_init(o);
             opt_quality(o);

@@ -72,7 +72,16 @@ struct SurfArgs {
     uint8_t* opt1d;   // nS
     float gridScale, shrink, aff;
     int depth;
+    // error bits (kSurfErr*), or-ed by the kernels; the host reads them with
+    // the quality after every iteration
+    int* err;
 };
+// a feature neighbour's projection found no sector for the halfedge nearest
+// to the vertex (table entry missing or past the vertex's sectors)
+constexpr int kSurfErrNoSector = 1;
+// a feature vertex has neighbours after its last feature neighbour on an
+// open fan: they belong to no sector
+constexpr int kSurfErrUnsectored = 2;
 
 size_t surf_lds_bytes(int elem, int maxRing);
 void surf_init_launch(const SurfArgs& a, hipStream_t s);

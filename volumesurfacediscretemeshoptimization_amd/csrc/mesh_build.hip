@@ -407,8 +407,57 @@ __global__ void k_fill_range(int* __restrict__ a, int lo, int n, const int* __re
     if (i < n) a[lo + i] = col[i];
 }
 
+// Class partition check (the rule of k_checkColoring, MeshGPUCommon.cu:
+// 1470-1680): the vertices [off[0], off[nc]) are split into the classes
+// [off[c], off[c+1]); no element may hold two vertices of one class, or
+// their searches in one launch would read each other's positions.  One
+// thread per element; a corner's class by binary search of the offsets.
+__global__ void k_check_classes(const int* __restrict__ cells, int nE, int npe, const int* __restrict__ off, int nc,
+                                int* __restrict__ bad)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    const int lo = off[0], hi = off[nc];
+    int cls[8];
+    for (int i = 0; i < npe; ++i) {
+        const int v = cells[(size_t)e * npe + i];
+        int c = -1;
+        if (v >= lo && v < hi) {
+            int a = 0, b = nc - 1;          // largest c with off[c] <= v
+            while (a < b) {
+                const int mid = (a + b + 1) >> 1;
+                if (off[mid] <= v) a = mid; else b = mid - 1;
+            }
+            c = a;
+        }
+        cls[i] = c;
+        for (int j = 0; j < i; ++j)
+            if (c >= 0 && cls[j] == c) atomicMin(bad, c);
+    }
+}
+
+void check_classes(const vsdmo_mesh& m, const std::vector<int>& off)
+{
+    const int nc = (int)off.size() - 1;
+    if (nc <= 0 || m.nE == 0) return;
+    hipStream_t s = m.stream;
+    DevBuf<int> doff(off.size()), bad(1);
+    doff.upload(off.data(), off.size(), s);
+    const int none = INT_MAX;
+    bad.upload(&none, 1, s);
+    k_check_classes<<<nblk(m.nE), TB, 0, s>>>(m.cells.p, m.nE, m.npe, doff.p, nc, bad.p);
+    VSDMO_HIP(hipGetLastError());
+    int h = none;
+    bad.download(&h, 1, s);
+    VSDMO_HIP(hipStreamSynchronize(s));
+    if (h != none)
+        throw StatusError{ VSDMO_TOPOLOGY, "colour class " + std::to_string(h) + " (vertices " +
+                                               std::to_string(off[h]) + ".." + std::to_string(off[h + 1] - 1) +
+                                               ") holds two vertices of one element" };
+}
+
 // colour [lo,hi) of the current numbering and sort it stably by colour.
-// Returns absolute colour offsets.
+// Returns absolute colour offsets, checked to be independent sets.
 std::vector<int> colour_and_sort(vsdmo_mesh& m, Renumber& rn, const int* incRow, const int* incVal, int lo, int hi,
                                  int mode, std::vector<DevBuf<uint8_t>*> u8)
 {
@@ -460,6 +509,7 @@ std::vector<int> colour_and_sort(vsdmo_mesh& m, Renumber& rn, const int* incRow,
     std::vector<DevBuf<int>*> i32 = { &m.colorOf };
     rn.apply(newid.p, u8, i32);
     for (int& o : off) o += lo;
+    check_classes(m, off);
     return off;
 }
 

@@ -226,12 +226,13 @@ __global__ void k_sector_lists(SurfArgs a, int* secN, int* secCnt)
     for (int t = 0; t < kTableSize; ++t) a.table[kTableSize * i + t] = -1;
     a.table[kTableSize * i] = 0;
     const int nsec = a.frow[i + 1] - a.frow[i];
-    // non-feature neighbours after the last feature neighbour of an open
-    // boundary vertex belong to no sector: they are dropped (sid == nsec)
+    // neighbours after the last feature neighbour of an open fan belong to no
+    // sector (sid == nsec): reported, and left out of every list
     for (int h = h0 + 1; h < h1; ++h) {
         int d = a.heTarget[h];
         int* L = secN + (size_t)kMaxNeigh * (a.frow[i] + (sid < nsec ? sid : 0));
         const bool own = sid < nsec;
+        if (!own) atomicOr(a.err, kSurfErrUnsectored);
         if (lastWasFeature) { nneigh = 0; if (own) { L[nneigh++] = v; L[nneigh++] = a.heTarget[last]; } }
         if (a.isFeature[d]) {
             if (vindex < kTableSize) a.table[kTableSize * i + vindex] = sid;
@@ -327,7 +328,12 @@ __device__ LSurf neighbour_fit(const SurfArgs& a, int nn, w3 vidPos)
     }
     int sid = (arg < kTableSize) ? a.table[kTableSize * off + arg] : -1;
     const int nsec = a.frow[off + 1] - a.frow[off];
-    if (sid < 0 || sid >= nsec) sid = 0;
+    if (sid < 0 || sid >= nsec) {
+        // the reference asserts sid != -1 (SurfaceEstimation.h:172); here the
+        // optimiser reports VSDMO_TOPOLOGY after the pass
+        atomicOr(a.err, kSurfErrNoSector);
+        sid = 0;
+    }
     return nsec > 0 ? reinterpret_cast<const LSurf*>(a.fs)[a.frow[off] + sid] : ls[nn];
 }
 
