@@ -73,6 +73,9 @@ WORKLOADS = {
     "surf_quad": {"kind": "quad", "surface": True, "flop": SURF_FLOP["quad"], "flop1d": SURF_FLOP1D["quad"], "cbrt": 0,
                   "kernel": "k_search_fan2d<true>", "traffic": 20_562_741},
     "batch256": {"kind": "batch", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_batch_round", "traffic": None},
+    # the default line: learned (+ batched) vs plain on tet40 and hex48
+    "north_star": {"kind": "tet", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_search_tet_learned_def<8>",
+                   "traffic": None},
 }
 # algorithmic FLOP of one flat-triangle vertex update (8x8 candidates: 196 ring
 # evaluations x 6 triangles x 23 FLOP)
@@ -207,7 +210,7 @@ def bench_batch(args):
             learned = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cpu()["threads_all"]
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_py
         n = 0
@@ -350,7 +353,7 @@ def bench_learned(args):
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cpu()["threads_all"]
         out["cpu_baseline"] = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"],
                                            args.workload)
     print(json.dumps(out), flush=True)
@@ -545,12 +548,356 @@ def _learned(args):
     }
 
 
+# ---------------------------------------------------------------------------
+# north-star line: learned (+ batched) vs plain DMO at equal final quality on
+# the 40^3 tet cube and the 48^3 hex cube
+# ---------------------------------------------------------------------------
+# FP32 operations of the canonical evaluation per (candidate, ring element),
+# counted from the kernels' inner loops (add / sub / mul / compare / min = 1,
+# fma = 2; selects and moves not counted).  Tet: det = fma (2), L = add + fma
+# (3), the running selection d^2, L^3, two cross products, compare, min (7),
+# plus the lane's (y, z) terms of the element (two fma for det, add + two fma
+# for L: 9) shared by its candidates per lane (8 in the full search, 4 in the
+# proposal window).  Hex, per p-dependent sub-tet (5 per hex): det (2), F
+# (3), selection with both minima (8), plus 9 shared terms.
+def canon_flop(kind: str, cand_per_lane: int) -> float:
+    if kind == "tet":
+        return 2 + 3 + 7 + 9.0 / cand_per_lane
+    return 5 * (2 + 3 + 8 + 9.0 / cand_per_lane)
+
+
+RING = {"tet": 24, "hex": 8}            # elements per interior vertex of the structured cubes
+FULL_EVALS = 3 * 512 + 1                 # candidates of a full search + the start position
+MLP_FLOP = 2 * (96 * 128 + 128 * 128 + 128 * 3)   # per network row (BF16 MFMA)
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    share = min(avail, int(env)) if env and env.isdigit() and int(env) > 0 else avail
+    return {"model": model, "nproc": os.cpu_count(), "affinity": avail, "threads_all": share}
+
+
+def cpu_baseline_tet(xyz, cells, name: str, reps: int = 3):
+    """The oracle (C/OpenMP restatement of the reference search) on the host
+    cores: whole iterations with every core of the job's share, and with one
+    core a sample of each iteration (the first 1/8 of every colour class; the
+    structured cube's vertices all cost the same), median of `reps` runs each."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+    cpu = host_cpu()
+    res = {}
+    for label, threads in (("all", cpu["threads_all"]), ("one", 1)):
+        om = oracle_py.OracleMesh("tet", xyz, cells)
+        off = om.offsets("inner")
+        cap = 0 if label == "all" else int(np.max(np.diff(off))) // 8 + 1
+        rates, secs = [], 0.0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            n = om.iterate(nthreads=threads, max_per_class=cap)
+            dt = time.perf_counter() - t0
+            secs += dt
+            rates.append(n / dt)
+        res[label] = {"threads": threads, "rate": float(np.median(rates)), "rates": rates, "seconds": secs,
+                      "per_class_cap": cap}
+    return {"value": res["all"]["rate"], "unit": "vertex updates/s", "cores": res["all"]["threads"], "kind": "port",
+            "value_one_core": res["one"]["rate"],
+            "sample": (f"{name}: {reps} whole DMO iterations with {res['all']['threads']} threads (median "
+                       f"{res['all']['rate']:.0f}/s, {res['all']['seconds']:.1f} s), and {reps} iterations on 1 "
+                       f"thread over the first {res['one']['per_class_cap']} vertices of each colour class "
+                       f"(median {res['one']['rate']:.0f}/s, {res['one']['seconds']:.1f} s), C/OpenMP oracle"),
+            "cpu_model": cpu["model"], "nproc": cpu["nproc"], "affinity_cpus": cpu["affinity"],
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def _train_net(name, args, fac, cls, kw, pairs, epochs):
+    """pairs recorded from plain DMO (K iterations each) on meshes of the same
+    family with seeds 1000, 1001, ..., then Adam training on the GPU"""
+    from volumesurfacediscretemeshoptimization_amd import DMODataset, DMOMLP
+    t0 = time.perf_counter()
+    ds = DMODataset(pairs)
+    s = 0
+    while len(ds) < pairs:
+        txyz, tcells, _, _ = make_mesh(name, 1000 + s)
+        m = fac.create(txyz, tcells)
+        o = cls(m, n_iter=args.steps, **kw)
+        o.record(ds, args.steps)
+        o.close()
+        m.close()
+        s += 1
+    rec_s = time.perf_counter() - t0
+    mlp = DMOMLP(seed=args.seed + 1)
+    t0 = time.perf_counter()
+    losses = mlp.train(ds, epochs=epochs, batch=4096, lr=1e-3, seed=args.seed + 2)
+    info = {"training_pairs": len(ds), "training_meshes": s, "record_s": rec_s, "train_s": time.perf_counter() - t0,
+            "epochs": epochs, "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])}
+    ds.close()
+    return mlp, info
+
+
+def _to_quality(fac, cls, xyz, cells, kw, mlp, mode, depth, frac, policy, qp, qmp, max_its, timing=False):
+    """iterations of one optimiser with the proposal until its minimum and
+    mean quality reach qp, qmp (-1e-3) or max_its; wall time of the loop"""
+    m = fac.create(xyz, cells)
+    o = cls(m, n_iter=max_its, **kw)
+    o.set_proposal(mlp if mode == 1 else None, mode=mode, depth=depth, window_frac=frac, fallback=policy)
+    if timing:
+        o.set_timing(True)
+    its, hist = 0, []
+    t0 = time.perf_counter()
+    while its < max_its:
+        o.doIteration()
+        its += 1
+        if timing:
+            st = o.stats()
+            hist.append({"min": o.getQuality(), "mean": o.getMeanQuality(), "fallbacks": st["fallbacks"],
+                         "visits": st["searches_learned"], "kernel_ms": st["kernel_ms"]})
+        if o.getQuality() >= qp - 1e-3 and o.getMeanQuality() >= qmp - 1e-3:
+            break
+    dt = time.perf_counter() - t0
+    st = o.stats()
+    out = {"iterations": its, "seconds": dt, "min_quality": o.getQuality(), "mean_quality": o.getMeanQuality(),
+           "reached": o.getQuality() >= qp - 1e-3 and o.getMeanQuality() >= qmp - 1e-3,
+           "visits": st["searches_learned"], "fallbacks": st["fallbacks"], "kernel_ms": st["kernel_ms"],
+           "launches": st["kernel_launches"], "history": hist}
+    o.close()
+    m.close()
+    return out
+
+
+def north_star_mesh(name, args):
+    """plain vs learned vs learned + batched on one mesh at equal final quality"""
+    from volumesurfacediscretemeshoptimization_amd import (DMOHexClass, DMOMeshHexFactory, DMOMeshTetFactory,
+                                                           DMOTetClass)
+    kind = WORKLOADS[name]["kind"]
+    fac, cls = (DMOMeshTetFactory, DMOTetClass) if kind == "tet" else (DMOMeshHexFactory, DMOHexClass)
+    xyz, cells, desc, kw = make_mesh(name, args.seed)
+    kw = {"fixed_boundary": kw["fixed_boundary"]}
+    depth, frac = LEARN_WINDOW[kind]
+    policy = args.learn_fallback
+    K = args.steps
+    mlp, train = _train_net(name, args, fac, cls, kw, args.pairs, args.epochs)
+    # warm-up: W iterations of each path on throwaway copies
+    for mode in (0, 1):
+        wm = fac.create(xyz, cells)
+        wo = cls(wm, n_iter=max(args.warmup, 1), **kw)
+        if mode:
+            wo.set_proposal(mlp, mode=1, depth=depth, window_frac=frac, fallback=policy)
+        wo.iterate(max(args.warmup, 1))
+        wo.close()
+        wm.close()
+    # plain, timed (no per-launch events), then its kernel time with events
+    mp = fac.create(xyz, cells)
+    build_t0 = time.perf_counter()
+    op = cls(mp, n_iter=K, **kw)
+    q0 = op.getQuality()
+    t0 = time.perf_counter()
+    op.iterate(K)
+    dt_plain = time.perf_counter() - t0
+    stp = op.stats()
+    qp, qmp = op.getQuality(), op.getMeanQuality()
+    plain_updates = stp["searches_inner"] + stp["searches_surface"] + stp["searches_feature"]
+    op.close()
+    mp.close()
+    del build_t0
+    mk = fac.create(xyz, cells)
+    ok = cls(mk, n_iter=K, **kw)
+    ok.set_timing(True)
+    ok.iterate(K)
+    stk = ok.stats()
+    ok.close()
+    mk.close()
+    plain_kernel_us = 1e3 * stk["kernel_ms"] / max(1, stk["kernel_launches"])
+    # learned, timed; the same run with events for the kernel time and the
+    # per-iteration fallbacks
+    lrn = _to_quality(fac, cls, xyz, cells, kw, mlp, 1, depth, frac, policy, qp, qmp, 3 * K)
+    lrk = _to_quality(fac, cls, xyz, cells, kw, mlp, 1, depth, frac, policy, qp, qmp, lrn["iterations"], timing=True)
+    # the same windowed search around the vertex itself (no network)
+    abl = _to_quality(fac, cls, xyz, cells, kw, None, 2, depth, frac, policy, qp, qmp, 3 * K)
+    # the specified configuration: one level on 1/4 of the plain window
+    spec = _to_quality(fac, cls, xyz, cells, kw, mlp, 1, SPEC_WINDOW[0], SPEC_WINDOW[1], policy, qp, qmp, 3 * K)
+    # learned + batched: one-mesh batch, one iteration replayed as a graph
+    saved = (args.learn_depth, args.learn_frac)
+    args.learn_depth, args.learn_frac = depth, frac
+    try:
+        lb = _learned_batched(args, mlp, fac, xyz, cells, kind, dt_plain)
+    except Exception as e:          # reported, the other figures stand
+        lb = {"error": f"{type(e).__name__}: {e}"}
+    args.learn_depth, args.learn_frac = saved
+    mlp.close()
+
+    def rate(r):
+        return plain_updates / r["seconds"] if r["reached"] else 0.0
+
+    # the learned class kernel: canonical FP32 work of its windows and of the
+    # full searches it runs for fallbacks, per launch
+    ring = RING[kind]
+    win_evals = lrk["visits"] * (64 * depth + 1)
+    flop = (win_evals * ring * canon_flop(kind, 4) + lrk["fallbacks"] * FULL_EVALS * ring * canon_flop(kind, 8))
+    launches = max(1, lrk["launches"])
+    learned_kernel_us = 1e3 * lrk["kernel_ms"] / launches
+    wl = WORKLOADS[name]
+    plain_flop_launch = plain_updates / max(1, stk["kernel_launches"]) * FULL_EVALS * ring * canon_flop(kind, 8)
+    res = {
+        "workload": desc,
+        "plain": {"iterations": K, "ms_per_step": 1e3 * dt_plain / K, "value": plain_updates / dt_plain,
+                  "min_quality_start": q0, "min_quality": qp, "mean_quality": qmp,
+                  "kernel": wl["kernel"], "kernel_mean_launch_us": plain_kernel_us,
+                  "kernel_tflops_canonical": plain_flop_launch / (plain_kernel_us * 1e-6) / 1e12},
+        "learned": {"window": [depth, frac], "fallback_policy": policy, "iterations": lrn["iterations"],
+                    "ms_per_learned_iteration": 1e3 * lrn["seconds"] / lrn["iterations"],
+                    "value": rate(lrn), "speedup_vs_plain": dt_plain / lrn["seconds"] if lrn["reached"] else 0.0,
+                    "reached_plain_quality": lrn["reached"], "min_quality": lrn["min_quality"],
+                    "mean_quality": lrn["mean_quality"],
+                    "fallback_rate": lrn["fallbacks"] / max(1, lrn["visits"]),
+                    "kernel": f"k_search_{kind}_learned_def<8,true>",
+                    "kernel_mean_launch_us": learned_kernel_us, "kernel_launches": lrk["launches"],
+                    "kernel_tflops_canonical": flop / (lrk["kernel_ms"] * 1e-3) / 1e12,
+                    "mfma_tflops": lrk["visits"] * MLP_FLOP / (lrk["kernel_ms"] * 1e-3) / 1e12,
+                    "per_iteration": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in h.items()}
+                                      for h in lrk["history"]]},
+        "learned_batched": lb,
+        "ablation_same_window_no_network": {
+            "iterations": abl["iterations"], "reached_plain_quality": abl["reached"],
+            "speedup_vs_plain": dt_plain / abl["seconds"] if abl["reached"] else 0.0,
+            "min_quality": abl["min_quality"], "mean_quality": abl["mean_quality"],
+            "fallback_rate": abl["fallbacks"] / max(1, abl["visits"]),
+            "network_gain": (abl["seconds"] / lrn["seconds"]) if (abl["reached"] and lrn["reached"]) else None},
+        "specified_window": {"window": list(SPEC_WINDOW), "iterations": spec["iterations"],
+                             "reached_plain_quality": spec["reached"],
+                             "speedup_vs_plain": dt_plain / spec["seconds"] if spec["reached"] else 0.0,
+                             "fallback_rate": spec["fallbacks"] / max(1, spec["visits"])},
+        "network": train,
+        "_plain_updates": plain_updates, "_dt_plain": dt_plain, "_learned_flop_launch": flop / launches,
+        "_learned_kernel_us": learned_kernel_us, "_xyz": xyz, "_cells": cells,
+    }
+    if kind == "hex":
+        res["note"] = ("the jitter leaves inverted hexes that no search repairs: the minimum quality is 0 in every "
+                       "run, so equal quality is decided by the mean")
+    return res
+
+
+CALIB_LIB = os.path.join(ROOT, "bench_calibrate.so")
+
+
+def calibrate(copy_mib: int = 256, copy_reps: int = 5, fma_iters: int = 20000, fma_reps: int = 5):
+    """(1) a device-to-device copy of copy_mib MiB (copy_reps kernels), whose
+    HBM traffic is exactly known, to scale the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings of the DMO kernels; (2) bench_calibrate.hip's
+    k_fma_probe, a known count of non-packed FP32 FMA instructions on every
+    lane, timed with HIP events: the VALU issue rate the search kernels are
+    compared with.  Prints one JSON line."""
+    import ctypes
+    import torch
+    n = copy_mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda:0", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    for _ in range(copy_reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    if not os.path.exists(CALIB_LIB):
+        raise SystemExit(f"{CALIB_LIB} missing: run __graft_entry__.build()")
+    lib = ctypes.CDLL(CALIB_LIB)
+    lib.calib_fma_run.restype = ctypes.c_int
+    lib.calib_fma_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    lib.calib_fma_per_lane.restype = ctypes.c_longlong
+    lib.calib_fma_per_lane.argtypes = [ctypes.c_int]
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    out = {"copy": {"mib": copy_mib, "kernels": copy_reps}, "cus": cus, "fma": []}
+    for waves_per_simd in (2, 4, 8):
+        blocks = cus * waves_per_simd           # 256-thread blocks: 4 waves, one per SIMD
+        ms = (ctypes.c_float * fma_reps)()
+        rc = lib.calib_fma_run(blocks, fma_iters, fma_reps, ms)
+        if rc != 0:
+            raise SystemExit(f"calib_fma_run failed: HIP error {rc}")
+        t = float(np.median(np.array(list(ms), dtype=np.float64))) * 1e-3
+        fma = lib.calib_fma_per_lane(fma_iters) * blocks * 256
+        waves_instr = fma / 64
+        out["fma"].append({"waves_per_simd": waves_per_simd, "blocks": blocks, "median_ms": 1e3 * t,
+                           "tflops": 2 * fma / t / 1e12, "valu_instr_per_s": waves_instr / t,
+                           "valu_instr_per_cu_per_ns": waves_instr / t / cus / 1e9})
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_north_star(args):
+    t_start = time.perf_counter()
+    tet = north_star_mesh("tet40", args)
+    hexr = north_star_mesh("hex48", args) if args.with_hex else None
+    lb = tet["learned_batched"]
+    if "error" in lb or not lb.get("reached_plain_quality"):
+        value = 0.0
+        lb_s = None
+    else:
+        lb_s = lb["learned_ms_per_step"] * lb["learned_iterations"] / 1e3
+        value = tet["_plain_updates"] / lb_s
+    K = args.steps
+    roof_achieved = tet["_learned_flop_launch"] / (tet["_learned_kernel_us"] * 1e-6) / 1e12
+    roof = {"bound": "valu", "achieved": roof_achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": roof_achieved / PEAK_FP32_TFLOPS, "traffic": WORKLOADS["tet40"].get("learned_traffic"),
+            "kernel": "k_search_tet_learned_def<8,true> (network, proposal windows and the deferred full "
+                      "searches of one colour class)",
+            "mean_launch_us": tet["_learned_kernel_us"], "flop_per_launch": tet["_learned_flop_launch"],
+            "flop_basis": "FP32 operations of the canonical evaluation per (candidate, ring element): "
+                          f"{canon_flop('tet', 4):.2f} in the windows, {canon_flop('tet', 8):.3f} in the full "
+                          "searches; the network's BF16 MFMA work is reported as mfma_tflops"}
+    cpu = None if args.no_cpu else cpu_baseline_tet(tet["_xyz"], tet["_cells"], "tet40")
+    for r in (tet, hexr):
+        if r:
+            for k in [k for k in r if k.startswith("_")]:
+                r.pop(k)
+    out = {
+        "metric": METRIC, "value": value, "unit": "vertex updates/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": (1e3 * lb_s / K) if lb_s else None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (search), bf16 (network)", "data": "synthetic",
+        "config": {
+            "workload": ("learned + batched DMO against plain GPU DMO at equal final min and mean mean-ratio "
+                         "quality (-1e-3), 40^3 x 6 tet cube (68,921 vertices, 384,000 tets, jitter 0.35h, seed "
+                         f"{args.seed}, boundary fixed): value = the plain run's {K} iterations of vertex updates "
+                         "(59,319 inner searches each) / the wall time of the learned + batched run to the plain "
+                         "run's final quality (a one-mesh batch, one iteration replayed as a hipGraph); "
+                         "ms_per_step = that time / K; the 48^3 hex cube beside it"),
+            "name": "north_star",
+            "speedup_vs_plain": {"tet40_learned_batched": lb.get("speedup_vs_single_mesh_plain"),
+                                 "tet40_learned": tet["learned"]["speedup_vs_plain"],
+                                 "hex48_learned_batched": (hexr["learned_batched"].get("speedup_vs_single_mesh_plain")
+                                                           if hexr else None),
+                                 "hex48_learned": hexr["learned"]["speedup_vs_plain"] if hexr else None},
+            "tet40": tet, "hex48": hexr,
+            "bench_wall_s": time.perf_counter() - t_start,
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="tet40")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="north_star",
+                    help="north_star (default): learned + batched vs plain at equal quality on tet40 and hex48; "
+                         "the others: one plain (or --mode learned) line per workload")
+    ap.add_argument("--with-hex", type=int, default=1, help="north_star: also the 48^3 hex cube (0: skip)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="measurement calibration instead of a bench line: a 256 MiB device copy (HBM counter "
+                         "scale) and a kernel with a known count of FP32 FMA instructions (VALU issue rate)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=0,
@@ -591,9 +938,13 @@ def main(argv=None):
         print("bench.py measures one GPU only: this DMO build runs on a single MI355X and has no multi-GPU "
               "path; run it with --gpus 1.", file=sys.stderr)
         return 2
+    if args.calibrate:
+        return calibrate()
+    if args.workload == "north_star":
+        return bench_north_star(args)
 
     if args.compare_learned is None:
-        args.compare_learned = int(args.workload == "tet40" and args.mode == "plain")
+        args.compare_learned = 0          # the north_star line carries the comparison
     hexw = WORKLOADS[args.workload]["kind"] == "hex"
     if args.learn_depth is None:
         args.learn_depth = LEARN_WINDOW["hex" if hexw else "tet"][0]
@@ -651,7 +1002,7 @@ def main(argv=None):
             "flop_per_update": wl["flop"], "cbrt_per_update": wl["cbrt"]}
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cpu()["threads_all"]
         cpu = cpu_baseline(kind, xyz, cells, args.cpu_iters, threads, not kw["fixed_boundary"] and not surface_mesh,
                            args.workload, surface_mesh)
     desc += f"; {args.waves} wavefront(s) per vertex"

@@ -195,7 +195,10 @@ private:
 
 namespace DMO {
 
-enum class QualityCriterium { MEAN_RATIO = 0 };
+// the reference's criteria (dmo/DMOQualityUtil.h:10-18), same names and
+// order; this library optimises MEAN_RATIO only, and an optimiser created
+// with any other criterion throws (VSDMO_NOT_SUPPORTED)
+enum class QualityCriterium { MEAN_RATIO = 0, AREA, RIGHT_ANGLE, JACOBIAN, MIN_ANGLE, RADIUS_RATIO, MAX_ANGLE };
 
 class DMOBaseClass {
 public:
@@ -291,8 +294,12 @@ public:
 };
 class DMOTriFlatClass : public DMOBaseClass {
 public:
-    DMOTriFlatClass(vsdmo_cpp::DMOMeshTri& m, QualityCriterium q = QualityCriterium::MEAN_RATIO,
-                    float gridScale = 0.5f, int n_iter = 30, const vsdmo_params* prm = nullptr)
+    // the reference's default criterion for flat triangle meshes is MIN_ANGLE
+    // (dmo/DMOTriFlat.h:25), which this library does not optimise: the
+    // criterion has no default here, so reference code that relied on it
+    // fails to compile instead of silently optimising another objective
+    DMOTriFlatClass(vsdmo_cpp::DMOMeshTri& m, QualityCriterium q, float gridScale = 0.5f, int n_iter = 30,
+                    const vsdmo_params* prm = nullptr)
         : DMOBaseClass(m, q, gridScale, n_iter, prm, VSDMO_SURFACE_PATH_FLAT) {}
 };
 // triangle / quad surface meshes: vertices move on their local surfaces

@@ -214,44 +214,52 @@ __global__ void k_sector_count(SurfArgs a, int* cnt)
     cnt[i] = c;
 }
 
-// sector neighbour lists and the halfedge -> sector table (k_setNeighborsFeature)
+// Feature sectors of feature vertex v (the local surfaces of its smooth
+// pieces between feature edges; reference SurfaceFeature.cu:134-223).  The
+// CCW fan t_0 .. t_{n-1} of v is cut at every target t_k, k >= 1, that is a
+// feature vertex.  Sector s holds v, its opening target (t_0 for sector 0,
+// the previous cut target after that), the targets after it and its closing
+// cut target.  On a closed fan (v not on a surface boundary) one more sector
+// wraps from the last cut target round to t_0.  The table maps fan position
+// k (< kTableSize) to the sector holding t_k after its opening target;
+// position 0 maps to sector 0.  Targets after the last cut of an open fan
+// belong to no sector: reported as an error.  Lists hold at most kMaxNeigh
+// vertices.
 __global__ void k_sector_lists(SurfArgs a, int* secN, int* secCnt)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nFeature) return;
-    int v = a.nFree + i;
-    int h0 = a.heRow[v], h1 = a.heRow[v + 1];
-    int sid = 0, nneigh = 0, last = h0, vindex = 1;
-    bool lastWasFeature = true;
-    for (int t = 0; t < kTableSize; ++t) a.table[kTableSize * i + t] = -1;
-    a.table[kTableSize * i] = 0;
-    const int nsec = a.frow[i + 1] - a.frow[i];
-    // neighbours after the last feature neighbour of an open fan belong to no
-    // sector (sid == nsec): reported, and left out of every list
-    for (int h = h0 + 1; h < h1; ++h) {
-        int d = a.heTarget[h];
-        int* L = secN + (size_t)kMaxNeigh * (a.frow[i] + (sid < nsec ? sid : 0));
-        const bool own = sid < nsec;
-        if (!own) atomicOr(a.err, kSurfErrUnsectored);
-        if (lastWasFeature) { nneigh = 0; if (own) { L[nneigh++] = v; L[nneigh++] = a.heTarget[last]; } }
-        if (a.isFeature[d]) {
-            if (vindex < kTableSize) a.table[kTableSize * i + vindex] = sid;
-            if (own && nneigh < kMaxNeigh) L[nneigh++] = d;
-            if (own) secCnt[a.frow[i] + sid] = nneigh;
-            ++sid; nneigh = 0; lastWasFeature = true;
-        } else {
-            if (vindex < kTableSize) a.table[kTableSize * i + vindex] = sid;
-            if (own && nneigh < kMaxNeigh) L[nneigh++] = d;
-            lastWasFeature = false;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nFeature) return;
+    const int v = a.nFree + f;
+    const int* fan = a.heTarget + a.heRow[v];
+    const int n = a.heRow[v + 1] - a.heRow[v];
+    const int first = a.frow[f], count = a.frow[f + 1] - first;
+    int* tab = a.table + (size_t)kTableSize * f;
+    for (int k = 0; k < kTableSize; ++k) tab[k] = -1;
+    tab[0] = 0;
+    // emit sector sec: v, fan[open], fan[open + 1 .. close) and closing target c
+    auto emit = [&](int sec, int open, int close, int c) {
+        int* list = secN + (size_t)kMaxNeigh * (first + sec);
+        int len = 0;
+        list[len++] = v;
+        list[len++] = fan[open];
+        for (int k = open + 1; k < close && len < kMaxNeigh; ++k) list[len++] = fan[k];
+        if (len < kMaxNeigh) list[len++] = c;
+        secCnt[first + sec] = len;
+    };
+    int sec = 0, open = 0;
+    for (int k = 1; k < n; ++k) {
+        if (sec >= count) {                       // past the last cut of an open fan
+            atomicOr(a.err, kSurfErrUnsectored);
+            break;
         }
-        ++last; ++vindex;
+        if (k < kTableSize) tab[k] = sec;
+        if (a.isFeature[fan[k]]) {
+            emit(sec, open, k, fan[k]);
+            ++sec;
+            open = k;
+        }
     }
-    if (!a.isB1D[v] && sid < nsec) {
-        int* L = secN + (size_t)kMaxNeigh * (a.frow[i] + sid);
-        if (lastWasFeature) { nneigh = 0; L[nneigh++] = v; L[nneigh++] = a.heTarget[last]; }
-        if (nneigh < kMaxNeigh) L[nneigh++] = a.heTarget[h0];
-        secCnt[a.frow[i] + sid] = nneigh;
-    }
+    if (!a.isB1D[v] && sec < count) emit(sec, open, n, fan[0]);
 }
 
 __global__ void k_sector_fit(SurfArgs a, const int* secN, const int* secCnt, int nSec)
