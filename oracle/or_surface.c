@@ -575,11 +575,15 @@ static int search1d(const or_mesh* m, const or_surf* s, const or_dmo_params* p, 
    surface boundary; window half-width s_d * (max Euclidean distance to the
    fan); scored by the 2D mean ratio of the fan triangles (p, r_k, r_k+1) in
    the canonical evaluation with the frame of the start position. */
-static int search1d_flat(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, float* xyz)
+static int search1d_flat(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, float* xyz, float* uvOut)
 {
     if (!s->opt1d[vid]) return 0;
     const int h0 = m->heRow[vid], n = m->heRow[vid + 1] - h0;
+#if OR_MUTATION == 15
+    const int nOne = n + 1;            /* planted: the boundary vertex's open fan closed */
+#else
     const int nOne = n + (m->isBoundary1D[vid] ? 0 : 1);
+#endif
     if (n <= 0 || nOne < 2 || n > MAX_NEIGH) return 0;
     const w3 x0 = W(xyz + 3 * (size_t)vid);
     float mabs = 0.0f, mx = -FLT_MAX;
@@ -633,6 +637,7 @@ static int search1d_flat(const or_mesh* m, const or_surf* s, const or_dmo_params
         qlast = best;
         ds = ds * shr;
     }
+    if (uvOut) uvOut[0] = cu;
     if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
     return 0;
 }
@@ -680,7 +685,7 @@ long long or_dmo_iteration_surface(or_mesh* m, or_surf* s, const or_dmo_params* 
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif
-            for (int v = lo; v < hi; ++v) search1d_flat(m, s, p, v, m->xyz);
+            for (int v = lo; v < hi; ++v) search1d_flat(m, s, p, v, m->xyz, NULL);
             for (int v = lo; v < hi; ++v) *nFeat1d += s->opt1d[v];
         }
     } else if (!p->fixed_boundary && lastMin > 0.0f && s) {
@@ -832,19 +837,26 @@ static int fan_init(const or_mesh* m, const or_surf* s, const float* snap, int v
     f->quad = (m->fpe == 4);
     const int K = f->quad ? 2 : 1;
     const w3 x0 = W(P(m, vid));
+#if OR_MUTATION == 14
+    float mx = FLT_MAX;                /* planted: the window from the nearest fan vertex */
+#define FAN_EXT fminf
+#else
     float mx = -FLT_MAX;
+#define FAN_EXT fmaxf
+#endif
     for (int i = 0; i < n; ++i) {
         const int h = h0 + i;
         w3 r = W(P(m, m->heTarget[h]));
         f->pts[K * i] = r;
-        mx = fmaxf(mx, wnorm(wsub(r, x0)));
+        mx = FAN_EXT(mx, wnorm(wsub(r, x0)));
         if (f->quad) {
             const int nx = m->heNext[h];
             w3 d = W(snap + 3 * (size_t)m->heTarget[nx >= 0 ? nx : h]);
             f->pts[K * i + 1] = d;
-            mx = fmaxf(mx, wnorm(wsub(d, x0)));
+            mx = FAN_EXT(mx, wnorm(wsub(d, x0)));
         }
     }
+#undef FAN_EXT
     f->ne = open ? n - 1 : n;
     if (!open) f->pts[K * n] = f->pts[0];
     f->maxDist = mx;
@@ -868,7 +880,8 @@ static float fan_score(const fan* f, w3 p, int oriented)
    DMOTriImplCub.h:25-200, DMOQuadImplCub.h:23-212): window half-width
    s_d * maxDist in the nearest neighbour's (u, v); candidates, ties and the
    update rule as search2d; write iff q_unsigned(old) < last level max */
-static int search2d_fan(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz)
+static int search2d_fan(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz,
+                        float* uvOut)
 {
     fan f;
     if (!fan_init(m, s, snap, vid, 0, &f)) return 0;
@@ -906,6 +919,7 @@ static int search2d_fan(const or_mesh* m, const or_surf* s, const float* snap, c
         qlast = best;
         ds = ds * shr;
     }
+    if (uvOut) { uvOut[0] = cu; uvOut[1] = cvv; }
     if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
     return 0;
 }
@@ -913,7 +927,8 @@ static int search2d_fan(const or_mesh* m, const or_surf* s, const float* snap, c
 /* 1D search of a feature vertex of a surface mesh (k_optimizeHierarchical1D,
    DMOTriImplCub.h:205-378, DMOQuadImplCub.h:216-402): the fan is open on a
    surface boundary */
-static int search1d_fan(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz)
+static int search1d_fan(const or_mesh* m, const or_surf* s, const float* snap, const or_dmo_params* p, int vid, float* xyz,
+                        float* uvOut)
 {
     if (!s->opt1d[vid]) return 0;
     fan f;
@@ -952,6 +967,7 @@ static int search1d_fan(const or_mesh* m, const or_surf* s, const float* snap, c
         qlast = best;
         ds = ds * shr;
     }
+    if (uvOut) uvOut[0] = cu;
     if (qold < qlast) { xyz[3 * (size_t)vid] = cur.x; xyz[3 * (size_t)vid + 1] = cur.y; xyz[3 * (size_t)vid + 2] = cur.z; return 1; }
     return 0;
 }
@@ -969,7 +985,7 @@ long long or_dmo_iteration_surface_mesh(or_mesh* m, or_surf* s, const or_dmo_par
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif
-        for (int v = lo; v < hi; ++v) search2d_fan(m, s, snap, p, v, m->xyz);
+        for (int v = lo; v < hi; ++v) search2d_fan(m, s, snap, p, v, m->xyz, NULL);
         *nSurf2d += hi - lo;
     }
     if (!p->fixed_boundary) {
@@ -979,7 +995,7 @@ long long or_dmo_iteration_surface_mesh(or_mesh* m, or_surf* s, const or_dmo_par
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif
-            for (int v = lo; v < hi; ++v) search1d_fan(m, s, snap, p, v, m->xyz);
+            for (int v = lo; v < hi; ++v) search1d_fan(m, s, snap, p, v, m->xyz, NULL);
             for (int v = lo; v < hi; ++v) *nFeat1d += s->opt1d[v];
         }
     }
@@ -1115,8 +1131,11 @@ void or_surface_get_sectors(const or_surf* s, int* table, int* frow, float* surf
 int or_surface_search_vertex(const or_mesh* m, const or_surf* s, const or_dmo_params* p, int vid, int dim,
                              float* xyz, float* uvOut)
 {
-    if (dim == 1) return search1d(m, s, p, vid, xyz, uvOut);
     /* a single vertex: the snapshot of the pass is the current positions */
+    const int surfaceMesh = m->elem == OR_QUAD || (m->elem == OR_TRI && !m->flat);
+    if (surfaceMesh) return dim == 1 ? search1d_fan(m, s, xyz, p, vid, xyz, uvOut) : search2d_fan(m, s, xyz, p, vid, xyz, uvOut);
+    if (m->elem == OR_TRI) return dim == 1 ? search1d_flat(m, s, p, vid, xyz, uvOut) : 0;
+    if (dim == 1) return search1d(m, s, p, vid, xyz, uvOut);
     return search2d(m, s, xyz, p, vid, xyz, uvOut);
 }
 

@@ -195,13 +195,13 @@ def curved_cube(kind, n=6, seed=21, jitter=0.25, dome=0.2):
 
 def surface_state(M, pinit):
     row, he = M.halfedges()
-    rrow, rval = M.ring()
-    _, isf = M.flags()
+    rrow, rval = M.ring() if M.m.ringStride > 0 and M.m.ringRow else (np.zeros(1, np.int32), np.zeros((0, 1), np.int32))
+    b1d, isf = M.flags()
     table, frow, fits = M.sectors()
-    per = M.m.ringStride
+    per = max(M.m.ringStride, 1)
     return ref64.SurfaceState(M.kind, M.positions(), row, he[:, 0], isf, M.m.nFree, M.surface_get(1),
                               M.surface_get(3), table, frow, fits, pinit, M.surface_get(0), M.surface_get(2),
-                              rrow, rval.reshape(-1, per))
+                              rrow, rval.reshape(-1, per), he_next=he[:, 3], is_b1d=b1d, vnorm=M.surface_get(4))
 
 
 def surface_mesh(o, kind, iterations=1, seed=21, jitter=0.25):
@@ -356,3 +356,74 @@ def check_learned_window(o, kind="tet", depth_l=1, frac=0.25, spread=0.3, seed=9
         assert st == ref.extra["status"] and np.array_equal(out, ref.pos), (v, st, ref.extra["status"], out, ref.pos)
     assert clean >= min_clean * (m.nV - m.nSurf), (clean, m.nV - m.nSurf)
     return status
+
+
+def fan_mesh(o, kind, shape, iterations=1):
+    """oracle surface mesh after its surface init and `iterations` surface
+    iterations: a cube-sphere (curved, closed) or a height field (open, its
+    boundary the feature curves), triangles or quads"""
+    quads = kind == "quad"
+    if shape == "sphere":
+        xyz, faces = synth.cube_surface(6, seed=3, jitter=0.3, quads=quads, sphere=True)
+    else:
+        xyz, faces = synth.height_field(10, seed=3, jitter=0.3, quads=quads)
+    M = o.OracleMesh("quad" if quads else "tri", xyz, faces)
+    M.surface_init()
+    pinit = M.positions()[: M.m.nSurf].copy()
+    for _ in range(iterations):
+        M.iterate_surface_mesh()
+    return M, pinit
+
+
+def check_fan_search(o, kind="tri", dim=2, shape="sphere", limit=48, min_clean=0.7):
+    """Surface-mesh vertices one at a time: the oracle's fan searches against
+    ref64.search_fan2d / search_fan1d (same rules as check_surface_search)."""
+    M, pinit = fan_mesh(o, kind, shape)
+    S = surface_state(M, pinit)
+    m = M.m
+    verts = list(range(0, m.nFree)) if dim == 2 else [v for v in range(m.nFree, m.nSurf) if S.opt1d[v]]
+    verts = verts[:: max(1, len(verts) // limit)][:limit]
+    assert verts
+    clean = written = 0
+    for v in verts:
+        ref = ref64.search_fan2d(S, v) if dim == 2 else ref64.search_fan1d(S, v)
+        w, pos, uv = M.surface_search_vertex(v, dim)
+        if ref.margin < MARGIN:
+            continue
+        clean += 1
+        written += ref.written
+        assert bool(w) == ref.written, (v, w, ref.written, ref.margin)
+        if dim == 2:
+            assert uv[0] == ref.extra["uv"][0] and uv[1] == ref.extra["uv"][1], (v, uv, ref.extra["uv"])
+        else:
+            assert uv[0] == ref.extra["uv"], (v, uv, ref.extra["uv"])
+        assert np.abs(pos.astype(np.float64) - ref.pos).max() <= 1e-6, (v, pos, ref.pos)
+    assert clean >= min_clean * len(verts), (clean, len(verts))
+    assert written > 0
+    return {"vertices": len(verts), "clean": clean, "written": written}
+
+
+def check_flat1d_search(o, n=20, seed=17, jitter=0.45, min_clean=0.7):
+    """Boundary vertices of a flat triangle grid with the boundary free, one at
+    a time: the oracle's boundary 1D pass against ref64.search_flat1d."""
+    xyz, tris = synth.tri_grid(n, seed=seed, jitter=jitter)
+    M = o.OracleMesh("tri", xyz, tris)
+    M.surface_init()
+    pinit = M.positions()[: M.m.nSurf].copy()
+    M.iterate_surface()                      # the NN table moves
+    S = surface_state(M, pinit)
+    verts = [v for v in range(M.m.nFree, M.m.nSurf) if S.opt1d[v]]
+    assert verts
+    clean = written = 0
+    for v in verts:
+        ref = ref64.search_flat1d(S, v)
+        w, pos, uv = M.surface_search_vertex(v, 1)
+        if ref.margin < MARGIN:
+            continue
+        clean += 1
+        written += ref.written
+        assert bool(w) == ref.written and uv[0] == ref.extra["uv"], (v, w, ref.written, uv, ref.extra["uv"])
+        assert np.abs(pos.astype(np.float64) - ref.pos).max() <= 1e-6, (v, pos, ref.pos)
+    assert clean >= min_clean * len(verts), (clean, len(verts))
+    assert written > 0
+    return {"vertices": len(verts), "clean": clean, "written": written}

@@ -256,8 +256,10 @@ class SurfaceState:
     initial positions and the nearest-neighbour table."""
 
     def __init__(self, kind, pos, he_row, he_target, is_feature, n_free, ls, ls1, table, frow, fits, pinit, nn,
-                 opt1d, ring_row, ring_val):
+                 opt1d, ring_row, ring_val, he_next=None, is_b1d=None, vnorm=None):
         self.kind = kind
+        self.he_next, self.is_b1d = he_next, is_b1d
+        self.vnorm = None if vnorm is None else np.asarray(vnorm, np.float64)
         self.pos = np.asarray(pos, F)
         self.he_row, self.he_target = he_row, he_target
         self.is_feature, self.n_free = is_feature, n_free
@@ -515,3 +517,186 @@ def learned_window(kind: str, x0, ring, c, depth_l, frac, grid_scale=0.5) -> Res
         pinned = bool(np.any((det[:, free] < float(EPS)) | (jn[:, free] < float(EPS))))
     status = 1 if moved else (0 if (tie or pinned) else -1)
     return Result(cur if moved else x0, moved, margin, {"status": status})
+
+
+# ------------------------------------------------------ surface-mesh fans
+def _fan_points(S: SurfaceState, vid: int, quad: bool):
+    """the fan of vid as the kernels load it (dmo/DMOTriImplCub.h:77-90,
+    dmo/DMOQuadImplCub.h:82-103): the halfedge targets r_i and, for quads,
+    after each the target d_i of the halfedge that follows vid -> r_i in its
+    face (a boundary halfedge has no face: its own target stands in, where
+    the reference reads out of range); FP32 positions"""
+    h0, h1 = S.he_row[vid], S.he_row[vid + 1]
+    pts = []
+    for h in range(h0, h1):
+        pts.append(S.pos[S.he_target[h]])
+        if quad:
+            nx = S.he_next[h]
+            pts.append(S.pos[S.he_target[nx if nx >= 0 else h]])
+    return np.asarray(pts, F)
+
+
+def _q_fan(quad: bool, pts, ne: int, p, n=None):
+    """qualityShapeMRMTri / qualityShapeMRMQuad over ne fan elements with p as
+    vertex 0 (dmo/DMOQualityTri.h:313-343, dmo/DMOQualityQuad.h:55-90), in
+    FP64; with a normal n the signed variants meanRatioMetricTri(v, n)
+    (DMOQualityTri.h:280-296: 2 sqrt(3) |N| sign(N/|N| . n) / sum l^2) and
+    meanRatioMetricQuad(v, n) (DMOQualityQuad.h:31-50: -1 when a corner
+    triangle's unit normal has . n < 0.7)"""
+    P = np.asarray(pts, np.float64)
+    p = np.asarray(p, np.float64)
+    q = np.inf
+    for k in range(ne):
+        if quad:
+            v = [p, P[2 * k], P[2 * k + 1], P[2 * k + 2]]
+            e = [v[1] - v[0], v[2] - v[1], v[3] - v[2], v[0] - v[3]]
+            l = sum(x @ x for x in e)
+            n1, n2 = np.cross(e[1], -e[0]), np.cross(e[3], -e[2])
+            a1, a2 = np.linalg.norm(n1), np.linalg.norm(n2)
+            if n is not None and ((n1 / a1) @ n < 0.7 or (n2 / a2) @ n < 0.7):
+                val = -1.0
+            else:
+                val = 2.0 * (a1 + a2) / l
+        else:
+            v = [p, P[k], P[k + 1]]
+            e0, e1, e2 = v[1] - v[0], v[2] - v[0], v[2] - v[1]
+            l = e0 @ e0 + e1 @ e1 + e2 @ e2
+            N = np.cross(e0, e1)
+            a = np.linalg.norm(N)
+            sgn = 1.0 if n is None else float(np.sign((N / a) @ n)) if a > 0 else 0.0
+            val = 2.0 * np.sqrt(3.0) * a * sgn / l
+        q = min(q, val)
+    return q
+
+
+def _fan_setup(S: SurfaceState, vid: int, open_fan: bool):
+    quad = S.kind == "quad"
+    pts = _fan_points(S, vid, quad)
+    nh = S.he_row[vid + 1] - S.he_row[vid]
+    ne = nh - 1 if open_fan else nh
+    if not open_fan:
+        pts = np.concatenate([pts, pts[:1]])                   # r_n = r_0
+    dist = _norm32(pts - S.pos[vid]).max()
+    n = S.vnorm[S.nn[vid]].astype(np.float64)
+    return quad, pts, ne, dist, n
+
+
+def search_fan2d(S: SurfaceState, vid: int, nq=8, depth=3, grid_scale=0.5) -> Result:
+    """k_optimizeHierarchical2D of triangle / quad surface meshes
+    (dmo/DMOTriImplCub.h:25-200, dmo/DMOQuadImplCub.h:23-212): as
+    search_surface2d, with the closed fan's metric signed by the NN's vertex
+    normal as the score, the largest distance to the fan points as the
+    window, and the unsigned metric of the start position in the write check."""
+    quad, pts, ne, dist, nrm = _fan_setup(S, vid, False)
+    x0 = S.pos[vid]
+    L32 = S.ls[S.nn[vid]].astype(F)
+    c = x0 - L32[0:3]
+    cu, cv = _dot32(L32[3:6], c), _dot32(L32[6:9], c)
+    aff = F(1.0) / F(nq - 1)
+    T = nq * nq // 2
+    t = np.arange(T)
+    i1, j1 = t // nq, t % nq
+    i2 = i1 + 4
+    cur = x0.astype(np.float64)
+    ds = F(grid_scale)
+    margin = np.inf
+    key, qm = 0, None
+    for _ in range(depth):
+        umax, umin = ds * dist, -ds * dist
+        u1 = cu + _grid(aff, i1, nq, umin, umax)
+        v1 = cv + _grid(aff, j1, nq, umin, umax)
+        u2 = cu + _grid(aff, i2, nq, umin, umax)
+        qa = np.array([_q_fan(quad, pts, ne, project2d(S, vid, u1[k], v1[k]), nrm) for k in range(T)])
+        qb = np.array([_q_fan(quad, pts, ne, project2d(S, vid, u2[k], v1[k]), nrm) for k in range(T)])
+        qlast = _q_fan(quad, pts, ne, cur, nrm)
+        takeb = np.array([gt(b, a) for a, b in zip(qa, qb)])
+        qm = np.where(takeb, qb, qa)
+        key = _argmax_threads(qm)
+        margin = min(margin, _top_gap(np.concatenate([qa, qb])), rel_gap(qm[key], qlast))
+        if gt(qm[key], qlast):
+            cu, cv = (u2[key], v1[key]) if takeb[key] else (u1[key], v1[key])
+            cur = project2d(S, vid, cu, cv)
+        ds = ds * (F(2.0) / F(nq - 1))
+    qold = _q_fan(quad, pts, ne, x0)
+    margin = min(margin, rel_gap(qm[key], qold))
+    written = gt(qm[key], qold)
+    return Result(cur if written else x0.astype(np.float64), written, margin, {"uv": (F(cu), F(cv))})
+
+
+def search_fan1d(S: SurfaceState, vid: int, nq=8, depth=3, grid_scale=0.5) -> Result:
+    """k_optimizeHierarchical1D of triangle / quad surface meshes
+    (dmo/DMOTriImplCub.h:205-378, dmo/DMOQuadImplCub.h:216-402): the feature
+    curve search of search_feature1d scored on the fan (open on a surface
+    boundary) signed by the NN's vertex normal; write check unsigned."""
+    x0 = S.pos[vid]
+    if not S.opt1d[vid]:
+        return Result(x0.astype(np.float64), False, np.inf, {"uv": None})
+    quad, pts, ne, dist, nrm = _fan_setup(S, vid, bool(S.is_b1d[vid]))
+    nb = feature_neighbours(S, vid)
+    L32 = S.ls1[S.nn[vid]].astype(F)
+    cu = _dot32(L32[3:6], x0 - L32[0:3])
+    aff = F(1.0) / F(nq - 1)
+    i1 = np.arange(nq)
+    cur = x0.astype(np.float64)
+    ds = F(grid_scale)
+    margin = np.inf
+    key, q = 0, None
+    for _ in range(depth):
+        umax, umin = ds * dist, -ds * dist
+        u = cu + _grid(aff, i1, nq, umin, umax)
+        q = np.array([_q_fan(quad, pts, ne, project1d(S, vid, nb, uu), nrm) for uu in u])
+        qlast = _q_fan(quad, pts, ne, cur, nrm)
+        key = _argmax_threads(q)
+        margin = min(margin, _top_gap(q), rel_gap(q[key], qlast))
+        if gt(q[key], qlast):
+            cu = u[key]
+            cur = project1d(S, vid, nb, cu)
+        ds = ds * (F(2.0) / F(nq - 1))
+    qold = _q_fan(quad, pts, ne, x0)
+    margin = min(margin, rel_gap(q[key], qold))
+    written = gt(q[key], qold)
+    return Result(cur if written else x0.astype(np.float64), written, margin, {"uv": F(cu)})
+
+
+def search_flat1d(S: SurfaceState, vid: int, nq=8, depth=3, grid_scale=0.5) -> Result:
+    """k_optimizeHierarchical1DFlat (dmo/DMOTriFlatImplCub.h:158-333): a
+    boundary vertex of a flat triangle mesh follows its boundary curve; the
+    score is the 2D mean ratio over its fan (closed unless the vertex lies on
+    a surface boundary), the window the largest distance to the fan targets;
+    written iff q(old) < the last level's maximum."""
+    x0 = S.pos[vid]
+    if not S.opt1d[vid]:
+        return Result(x0.astype(np.float64), False, np.inf, {"uv": None})
+    r = S.pos[S.fan(vid)].astype(F)
+    dist = _norm32(r - x0).max()
+    ring = r if S.is_b1d[vid] else np.concatenate([r, r[:1]])
+    els = np.stack([ring[:-1], ring[1:]], axis=1).astype(np.float64)     # (r_k, r_k+1)
+
+    def score(p):
+        P = np.concatenate([np.broadcast_to(np.asarray(p, np.float64), (len(els), 1, 3)), els], axis=1)
+        return float(q_tri2d(P).min())
+
+    nb = feature_neighbours(S, vid)
+    L32 = S.ls1[S.nn[vid]].astype(F)
+    cu = _dot32(L32[3:6], x0 - L32[0:3])
+    aff = F(1.0) / F(nq - 1)
+    i1 = np.arange(nq)
+    cur = x0.astype(np.float64)
+    ds = F(grid_scale)
+    margin = np.inf
+    key, q = 0, None
+    for _ in range(depth):
+        umax, umin = ds * dist, -ds * dist
+        u = cu + _grid(aff, i1, nq, umin, umax)
+        q = np.array([score(project1d(S, vid, nb, uu)) for uu in u])
+        qlast = score(cur)
+        key = _argmax_threads(q)
+        margin = min(margin, _top_gap(q), rel_gap(q[key], qlast))
+        if gt(q[key], qlast):
+            cu = u[key]
+            cur = project1d(S, vid, nb, cu)
+        ds = ds * (F(2.0) / F(nq - 1))
+    qold = score(x0)
+    margin = min(margin, rel_gap(q[key], qold))
+    written = gt(q[key], qold)
+    return Result(cur if written else x0.astype(np.float64), written, margin, {"uv": F(cu)})

@@ -29,6 +29,8 @@ MUTATIONS = {
     11: ("projection: a feature neighbour always uses its first sector", P.check_projection),
     12: ("learned window: a tie at a positive q(v) falls back", lambda o: P.check_learned_window(o, "hex", 1, 2.0)),
     13: ("learned window: halved per level instead of 2/3", lambda o: P.check_learned_window(o, "tet", 2, 0.5)),
+    14: ("surface-mesh fan window from the nearest fan vertex", lambda o: P.check_fan_search(o, "tri", 2, "sphere")),
+    15: ("flat boundary 1D pass: the open fan closed", P.check_flat1d_search),
 }
 
 

@@ -2,6 +2,8 @@
 reference (tests/ref64.py), CPU only.  Each test names the reference code it
 pins; tests/test_oracle_mutations.py shows that each check rejects a planted
 one-line mistake."""
+import pytest
+
 import oracle_pins as P
 
 
@@ -95,3 +97,17 @@ def test_learned_window_matches_restatement(oracle):
         assert st[1] > 0 and st[-1] > 0, (kind, depth, frac, st)
         if kind == "hex":
             assert st[0] > 0, st          # vertices kept in place by the positive-tie / pinned rules
+
+
+@pytest.mark.parametrize("kind", ["tri", "quad"])
+@pytest.mark.parametrize("dim,shape", [(2, "sphere"), (2, "height"), (1, "height")])
+def test_fan_search_matches_restatement(oracle, kind, dim, shape):
+    """k_optimizeHierarchical2D / 1D of triangle and quad surface meshes
+    (dmo/DMOTriImplCub.h:25-378, dmo/DMOQuadImplCub.h:23-402) on a
+    cube-sphere and an open height field"""
+    P.check_fan_search(oracle, kind, dim, shape)
+
+
+def test_flat_boundary_1d_matches_restatement(oracle):
+    """k_optimizeHierarchical1DFlat (dmo/DMOTriFlatImplCub.h:158-333)"""
+    P.check_flat1d_search(oracle)
