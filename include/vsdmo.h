@@ -183,6 +183,15 @@ vsdmo_status vsdmo_opt_min_quality(const vsdmo_opt* opt, float* out);
 vsdmo_status vsdmo_opt_mean_quality(const vsdmo_opt* opt, double* out);
 /* Replaces getQualityHistogram (dmo/DMOTet.cu:202-212): n_cols bins on [0,1) */
 vsdmo_status vsdmo_opt_histogram(const vsdmo_opt* opt, int32_t n_cols, int32_t* out);
+/* No counterpart in the reference.  vsdmo_opt_histogram without any device
+   allocation, for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops): the bins live in a file-scope device array
+   of VSDMO_HISTOGRAM_NOALLOC_MAX_COLS ints that each GPU holds until the
+   process ends.  n_cols above the maximum is refused (VSDMO_INVALID_ARG).
+   Calls on one GPU share the array, so calls that overlap on one GPU race.
+   The result equals vsdmo_opt_histogram's. */
+#define VSDMO_HISTOGRAM_NOALLOC_MAX_COLS 4096
+vsdmo_status vsdmo_opt_histogram_noalloc(const vsdmo_opt* opt, int32_t n_cols, int32_t* out);
 /* Replaces getElementQualities (dmo/DMOTet.cu:235-240), into host memory */
 vsdmo_status vsdmo_opt_element_qualities(const vsdmo_opt* opt, float* host_out);
 /* counters and timing; set_timing(1) records device time of every search launch */

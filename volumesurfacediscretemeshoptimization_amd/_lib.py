@@ -87,6 +87,7 @@ EXPORTS = {
     "vsdmo_opt_min_quality": (C.c_int, [C.c_void_p, _fp]),
     "vsdmo_opt_mean_quality": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "vsdmo_opt_histogram": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
+    "vsdmo_opt_histogram_noalloc": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
     "vsdmo_opt_element_qualities": (C.c_int, [C.c_void_p, _fp]),
     "vsdmo_opt_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "vsdmo_opt_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),

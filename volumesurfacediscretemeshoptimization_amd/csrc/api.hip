@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <new>
 #include <cstdlib>
 #include <string>
@@ -797,6 +798,60 @@ vsdmo_status vsdmo_opt_histogram(const vsdmo_opt* o, int32_t n_cols, int32_t* ou
         VSDMO_HIP(hipGetLastError());
         bins.download(out, n_cols, o->m->stream);
         VSDMO_HIP(hipStreamSynchronize(o->m->stream));
+        return VSDMO_OK;
+    });
+}
+
+// ------------------------------------------------ allocation-free histogram
+// The bins of vsdmo_opt_histogram_noalloc live in this file-scope device
+// array: each GPU the library runs on holds one copy of
+// VSDMO_HISTOGRAM_NOALLOC_MAX_COLS ints until the process ends.  Host code
+// reaches it only through the per-device address from hipGetSymbolAddress;
+// the array's name is not a device address on the host.
+static __device__ __align__(256) int scratch[VSDMO_HISTOGRAM_NOALLOC_MAX_COLS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch[kMaxDevices] = {};
+
+// the scratch array's address on device dev, looked up once per device with
+// that device current
+int* scratch_on(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) throw StatusError{ VSDMO_NOT_SUPPORTED, "device index out of range" };
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        int cur = 0;
+        VSDMO_HIP(hipGetDevice(&cur));
+        if (cur != dev) VSDMO_HIP(hipSetDevice(dev));
+        int* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&ptr), HIP_SYMBOL(scratch));
+        if (cur != dev) VSDMO_HIP(hipSetDevice(cur));
+        if (e != hipSuccess) throw HipError{ e, "hipGetSymbolAddress(scratch)" };
+        g_scratch[dev] = ptr;
+    }
+    return g_scratch[dev];
+}
+}  // namespace
+
+vsdmo_status vsdmo_opt_histogram_noalloc(const vsdmo_opt* o, int32_t n_cols, int32_t* out)
+{
+    return guarded([&]() -> vsdmo_status {
+        if (!o || !out || n_cols <= 0) return fail(VSDMO_INVALID_ARG, "bad argument");
+        if (n_cols > VSDMO_HISTOGRAM_NOALLOC_MAX_COLS)
+            return fail(VSDMO_INVALID_ARG, "n_cols above VSDMO_HISTOGRAM_NOALLOC_MAX_COLS (" +
+                                               std::to_string(VSDMO_HISTOGRAM_NOALLOC_MAX_COLS) + ")");
+        hipStream_t s = o->m->stream;
+        int dev = 0;
+        VSDMO_HIP(hipStreamGetDevice(s, &dev));
+        int* bins = scratch_on(dev);
+        // the array keeps the last call's counts: cleared by a kernel first
+        launch_fill_int(bins, n_cols, 0, s);
+        launch_histogram(o->q.p, o->m->nE, n_cols, bins, s);
+        VSDMO_HIP(hipGetLastError());
+        VSDMO_HIP(hipMemcpyAsync(out, bins, sizeof(int) * (size_t)n_cols, hipMemcpyDeviceToHost, s));
+        VSDMO_HIP(hipStreamSynchronize(s));
         return VSDMO_OK;
     });
 }
