@@ -38,6 +38,11 @@ CPU_BUDGET_S = 10.0
 # first-level window), chosen by sweeps with the deferred fallback on the
 # seed-7 and seed-8 40^3 tet / 48^3 hex cubes (the benchmark meshes use seed 0)
 LEARN_WINDOW = {"tet": (1, 0.025), "hex": (1, 2.0)}
+# proposal mode per element type (3: the network's window, then the vertex's
+# own window before the full search; 1: the network's window alone): on the
+# 40^3 tet cube mode 3 reaches the plain quality 1.13x faster than mode 1, on
+# the 48^3 hex cube mode 1 is 1.17x faster than mode 3 (measured at K = 20)
+LEARN_MODE = {"tet": 3, "hex": 1}
 # the learned configuration as specified: one 4x4x4 grid on 1/4 of the plain
 # first-level window, reported beside the tuned window
 SPEC_WINDOW = (1, 0.25)
@@ -114,7 +119,7 @@ def _batch_learned(args, b, dt_plain, q_plain, qm_plain):
     depth, frac = args.learn_depth, args.learn_frac
     b.restore()
     b.set_fallback(args.batch_fallback)
-    b.set_proposal(mlp, mode=1, depth=depth, window_frac=frac)
+    b.set_proposal(mlp, mode=args.learn_mode, depth=depth, window_frac=frac)
     b.iterate(max(args.warmup, 1))           # warm-up, then back to the start positions
     b.restore()
     ls0 = b.learned_stats()
@@ -164,7 +169,13 @@ def bench_batch(args):
     build_s = time.perf_counter() - t0
     b = DMOBatch(meshes, use_graph=bool(args.batch_graph))
     q0 = b.min_quality()
-    chosen = b.autotune(iters=args.tune_iters, tol=1e-3, log_path=args.tune_log)
+    if args.tune_iters > 0:
+        chosen = b.autotune(iters=args.tune_iters, tol=1e-3, log_path=args.tune_log)
+    else:
+        # the configuration the autotuner keeps on this mix (nq 8, depth 3, one
+        # wave per vertex, one block per item), without its trial graphs
+        b.set_params(nq=8, depth=3, waves_per_vertex=1, persistent=0)
+        chosen = b.params()
     # warm-up: one iteration, then back to the start positions
     b.iterate(max(args.warmup, 1))
     b.restore()
@@ -385,7 +396,7 @@ def _learned_batched(args, mlp, fac, xyz, cells, kind, single_plain_s):
     qp, qmp = float(b.min_quality()[0]), float(b.mean_quality()[0])
     b.restore()
     b.set_fallback(args.learn_fallback)
-    b.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac, elem=elem)
+    b.set_proposal(mlp, mode=args.learn_mode, depth=args.learn_depth, window_frac=args.learn_frac, elem=elem)
     n = 0
     while n < 3 * args.steps:
         b.iterate(1)
@@ -404,7 +415,8 @@ def _learned_batched(args, mlp, fac, xyz, cells, kind, single_plain_s):
     b.close()
     m.close()
     visits = ls["learned_visits"] - ls0["learned_visits"]
-    return {"policy": args.learn_fallback, "plain_iterations": args.steps, "learned_iterations": n,
+    return {"policy": args.learn_fallback, "mode": args.learn_mode, "plain_iterations": args.steps,
+            "learned_iterations": n,
             "plain_ms_per_step": 1e3 * dt_plain / args.steps, "learned_ms_per_step": 1e3 * dt / n,
             "speedup_vs_plain": dt_plain / dt if reached else 0.0,
             "speedup_vs_single_mesh_plain": single_plain_s / dt if reached else 0.0,
@@ -443,7 +455,7 @@ def _learned(args):
     wm = fac.create(xyz, cells)
     wo = cls(wm, n_iter=4, **kw)
     for pol in ("immediate", "deferred"):
-        wo.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac, fallback=pol)
+        wo.set_proposal(mlp, mode=args.learn_mode, depth=args.learn_depth, window_frac=args.learn_frac, fallback=pol)
         wo.iterate(1)
     wo.set_proposal(None, mode=0)
     wo.iterate(1)
@@ -463,7 +475,7 @@ def _learned(args):
     def to_quality(policy, depth=args.learn_depth, frac=args.learn_frac):
         ml = fac.create(xyz, cells)
         ol = cls(ml, n_iter=3 * args.steps, **kw)
-        ol.set_proposal(mlp, mode=1, depth=depth, window_frac=frac, fallback=policy)
+        ol.set_proposal(mlp, mode=args.learn_mode, depth=depth, window_frac=frac, fallback=policy)
         its = 0
         t0 = time.perf_counter()
         while its < 3 * args.steps:
@@ -493,7 +505,7 @@ def _learned(args):
     # the same run again with HIP events around every launch, for the kernel time
     mk = fac.create(xyz, cells)
     ok = cls(mk, n_iter=its, **kw)
-    ok.set_proposal(mlp, mode=1, depth=args.learn_depth, window_frac=args.learn_frac, fallback=args.learn_fallback)
+    ok.set_proposal(mlp, mode=args.learn_mode, depth=args.learn_depth, window_frac=args.learn_frac, fallback=args.learn_fallback)
     ok.set_timing(True)
     ok.iterate(its)
     stk = ok.stats()
@@ -652,7 +664,7 @@ def _to_quality(fac, cls, xyz, cells, kw, mlp, mode, depth, frac, policy, qp, qm
     mean quality reach qp, qmp (-1e-3) or max_its; wall time of the loop"""
     m = fac.create(xyz, cells)
     o = cls(m, n_iter=max_its, **kw)
-    o.set_proposal(mlp if mode == 1 else None, mode=mode, depth=depth, window_frac=frac, fallback=policy)
+    o.set_proposal(mlp if mode in (1, 3) else None, mode=mode, depth=depth, window_frac=frac, fallback=policy)
     if timing:
         o.set_timing(True)
     its, hist = 0, []
@@ -686,15 +698,16 @@ def north_star_mesh(name, args):
     xyz, cells, desc, kw = make_mesh(name, args.seed)
     kw = {"fixed_boundary": kw["fixed_boundary"]}
     depth, frac = LEARN_WINDOW[kind]
+    mode = args.learn_mode or LEARN_MODE[kind]
     policy = args.learn_fallback
     K = args.steps
     mlp, train = _train_net(name, args, fac, cls, kw, args.pairs, args.epochs)
     # warm-up: W iterations of each path on throwaway copies
-    for mode in (0, 1):
+    for learned in (False, True):
         wm = fac.create(xyz, cells)
         wo = cls(wm, n_iter=max(args.warmup, 1), **kw)
-        if mode:
-            wo.set_proposal(mlp, mode=1, depth=depth, window_frac=frac, fallback=policy)
+        if learned:
+            wo.set_proposal(mlp, mode=mode, depth=depth, window_frac=frac, fallback=policy)
         wo.iterate(max(args.warmup, 1))
         wo.close()
         wm.close()
@@ -721,21 +734,24 @@ def north_star_mesh(name, args):
     mk.close()
     plain_kernel_us = 1e3 * stk["kernel_ms"] / max(1, stk["kernel_launches"])
     # learned, timed; the same run with events for the kernel time and the
-    # per-iteration fallbacks
-    lrn = _to_quality(fac, cls, xyz, cells, kw, mlp, 1, depth, frac, policy, qp, qmp, 3 * K)
-    lrk = _to_quality(fac, cls, xyz, cells, kw, mlp, 1, depth, frac, policy, qp, qmp, lrn["iterations"], timing=True)
+    # per-iteration fallbacks; the headline mode and the other network mode
+    other = 1 if mode == 3 else 3
+    lrn = _to_quality(fac, cls, xyz, cells, kw, mlp, mode, depth, frac, policy, qp, qmp, 3 * K)
+    lrk = _to_quality(fac, cls, xyz, cells, kw, mlp, mode, depth, frac, policy, qp, qmp, lrn["iterations"],
+                      timing=True)
+    lro = _to_quality(fac, cls, xyz, cells, kw, mlp, other, depth, frac, policy, qp, qmp, 3 * K)
     # the same windowed search around the vertex itself (no network)
     abl = _to_quality(fac, cls, xyz, cells, kw, None, 2, depth, frac, policy, qp, qmp, 3 * K)
     # the specified configuration: one level on 1/4 of the plain window
-    spec = _to_quality(fac, cls, xyz, cells, kw, mlp, 1, SPEC_WINDOW[0], SPEC_WINDOW[1], policy, qp, qmp, 3 * K)
+    spec = _to_quality(fac, cls, xyz, cells, kw, mlp, mode, SPEC_WINDOW[0], SPEC_WINDOW[1], policy, qp, qmp, 3 * K)
     # learned + batched: one-mesh batch, one iteration replayed as a graph
-    saved = (args.learn_depth, args.learn_frac)
-    args.learn_depth, args.learn_frac = depth, frac
+    saved = (args.learn_depth, args.learn_frac, args.learn_mode)
+    args.learn_depth, args.learn_frac, args.learn_mode = depth, frac, mode
     try:
         lb = _learned_batched(args, mlp, fac, xyz, cells, kind, dt_plain)
     except Exception as e:          # reported, the other figures stand
         lb = {"error": f"{type(e).__name__}: {e}"}
-    args.learn_depth, args.learn_frac = saved
+    args.learn_depth, args.learn_frac, args.learn_mode = saved
     mlp.close()
 
     def rate(r):
@@ -756,7 +772,8 @@ def north_star_mesh(name, args):
                   "min_quality_start": q0, "min_quality": qp, "mean_quality": qmp,
                   "kernel": wl["kernel"], "kernel_mean_launch_us": plain_kernel_us,
                   "kernel_tflops_canonical": plain_flop_launch / (plain_kernel_us * 1e-6) / 1e12},
-        "learned": {"window": [depth, frac], "fallback_policy": policy, "iterations": lrn["iterations"],
+        "learned": {"mode": mode, "window": [depth, frac], "fallback_policy": policy,
+                    "iterations": lrn["iterations"],
                     "ms_per_learned_iteration": 1e3 * lrn["seconds"] / lrn["iterations"],
                     "value": rate(lrn), "speedup_vs_plain": dt_plain / lrn["seconds"] if lrn["reached"] else 0.0,
                     "reached_plain_quality": lrn["reached"], "min_quality": lrn["min_quality"],
@@ -769,6 +786,10 @@ def north_star_mesh(name, args):
                     "per_iteration": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in h.items()}
                                       for h in lrk["history"]]},
         "learned_batched": lb,
+        f"learned_mode{other}": {"iterations": lro["iterations"], "reached_plain_quality": lro["reached"],
+                                 "speedup_vs_plain": dt_plain / lro["seconds"] if lro["reached"] else 0.0,
+                                 "ms_per_learned_iteration": 1e3 * lro["seconds"] / lro["iterations"],
+                                 "fallback_rate": lro["fallbacks"] / max(1, lro["visits"])},
         "ablation_same_window_no_network": {
             "iterations": abl["iterations"], "reached_plain_quality": abl["reached"],
             "speedup_vs_plain": dt_plain / abl["seconds"] if abl["reached"] else 0.0,
@@ -902,7 +923,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=0,
                     help="CPU baseline iterations (0: whole iterations until about 10 s of CPU time)")
-    ap.add_argument("--tune-iters", type=int, default=3, help="iterations per autotuner trial (batch256)")
+    ap.add_argument("--tune-iters", type=int, default=3,
+                    help="iterations per autotuner trial (batch256); 0 skips the tuner and sets the configuration "
+                         "it keeps on the 256-mesh mix")
     ap.add_argument("--mode", choices=["plain", "learned"], default="plain")
     ap.add_argument("--waves", type=int, default=1, help="wavefronts per vertex search (1, 2, 4)")
     ap.add_argument("--compare-learned", type=int, default=None,
@@ -913,6 +936,10 @@ def main(argv=None):
     # default per element type, chosen on the seed-7 meshes, not the benchmark's
     ap.add_argument("--learn-depth", type=int, default=None, help="default: 1 (tet), 1 (hex)")
     ap.add_argument("--learn-frac", type=float, default=None, help="default: 0.025 (tet), 2.0 (hex)")
+    ap.add_argument("--learn-mode", type=int, choices=[1, 3], default=None,
+                    help="proposal of the learned runs: 1 the network's window, 3 the network's window and, where it "
+                         "finds nothing better, the same window around the vertex before the full search "
+                         "(default: 3 tet, 1 hex)")
     ap.add_argument("--learn-fallback", choices=["deferred", "immediate"], default="deferred",
                     help="fallback policy of the measured learned run (the other one is reported beside it)")
     ap.add_argument("--learned-extras", type=int, default=1,
@@ -950,6 +977,8 @@ def main(argv=None):
         args.learn_depth = LEARN_WINDOW["hex" if hexw else "tet"][0]
     if args.learn_frac is None:
         args.learn_frac = LEARN_WINDOW["hex" if hexw else "tet"][1]
+    if args.learn_mode is None:
+        args.learn_mode = LEARN_MODE["hex" if hexw else "tet"]
     if args.workload == "batch256":
         return bench_batch(args)
     if args.mode == "learned":

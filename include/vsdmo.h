@@ -279,7 +279,9 @@ vsdmo_status vsdmo_dataset_destroy(vsdmo_dataset* d);
  * pair per inner vertex search until the dataset is full */
 vsdmo_status vsdmo_opt_record(vsdmo_opt* o, vsdmo_dataset* ds, int32_t n);
 /* mode 0: plain search; 1: network proposal; 2: the proposal is the vertex
- * itself (the same windowed search and fallback without the network).
+ * itself (the same windowed search and fallback without the network); 3:
+ * the network's proposal and, where its window finds nothing better, the
+ * same window around the vertex itself before the full search.
  * One-rings up to 512 tets / 85 hexes (VSDMO_NOT_SUPPORTED above).  With
  * one-rings up to 128 tets / 21 hexes the window kernel runs the network
  * itself; environment variable VSDMO_LEARN_SEPARATE_INFERENCE=1 (read here)
@@ -360,7 +362,7 @@ vsdmo_status vsdmo_batch_get_stats(vsdmo_batch* b, vsdmo_batch_stats* out);
  * vsdmo_batch_set_fallback, applied per colour round to the round's items
  * of that type; the other items keep the plain search).  Each such mesh's positions
  * equal a vsdmo_opt of the mesh with fixed_boundary = 1 and the same
- * proposal.  mode 1 runs the network inside the window kernel and needs
+ * proposal.  Modes 1 and 3 run the network inside the window kernel and need
  * every ring of the type to fit its LDS tile (VSDMO_NOT_SUPPORTED
  * otherwise); mode 0 switches the type's proposal off.  The network is read
  * at every iteration: vsdmo_mlp_destroy refuses while the batch holds it.

@@ -153,18 +153,38 @@ def test_identity_proposal_monotone():
     assert 0 < st["fallbacks"] < st["searches_learned"]
 
 
-@pytest.mark.parametrize("kind", ["tet", "hex"])
-def test_learned_reaches_plain_quality_and_ablation(kind):
+_ABLATION = [("tet", 4, 0.25, "immediate", 1), ("hex", 4, 0.25, "immediate", 1),
+             # the shipped windows and policy (bench default), with the network
+             # alone (mode 1) and with the vertex's own window as a second chance (mode 3)
+             ("tet", 1, 0.025, "deferred", 1), ("hex", 1, 2.0, "deferred", 1),
+             ("tet", 1, 0.025, "deferred", 3), ("hex", 1, 2.0, "deferred", 3)]
+
+
+@pytest.fixture(scope="module")
+def ablation_nets():
+    out = {}
+    for kind in ("tet", "hex"):
+        n = 14 if kind == "tet" else 12
+        ds = _record(kind, [(n, 200 + s) for s in range(8)], 8, 1 << 21)
+        mlp = DMOMLP(seed=5)
+        loss = mlp.train(ds, epochs=20, batch=1024, lr=1e-3, seed=6)
+        assert loss[-1] < loss[0]
+        ds.close()
+        out[kind] = mlp
+    yield out
+    for m in out.values():
+        m.close()
+
+
+@pytest.mark.parametrize("kind,depth,frac,policy,mode", _ABLATION)
+def test_learned_reaches_plain_quality_and_ablation(ablation_nets, kind, depth, frac, policy, mode):
     """Trained on seeded jittered meshes, the proposal reaches the plain
     run's final minimum quality within 1e-3, and gains over the same windowed
-    search around the vertex itself (mode 2): fewer iterations to that
-    quality, or as many with fewer fallbacks to the full search."""
+    search around the vertex itself (mode 2) at the same window and policy:
+    fewer iterations to that quality, or as many with fewer fallbacks to the
+    full search."""
     n = 14 if kind == "tet" else 12
-    train = [(n, 200 + s) for s in range(8)]
-    ds = _record(kind, train, 8, 1 << 21)
-    mlp = DMOMLP(seed=5)
-    loss = mlp.train(ds, epochs=20, batch=1024, lr=1e-3, seed=6)
-    assert loss[-1] < loss[0]
+    mlp = ablation_nets[kind]
     if kind == "tet":
         xyz, cells = synth.tet_cube(n, seed=999)
         fac, cls = DMOMeshTetFactory, DMOTetClass
@@ -177,10 +197,10 @@ def test_learned_reaches_plain_quality_and_ablation(kind):
     op.iterate(iters)
     q_plain = op.getQuality()
     res = {}
-    for mode in (1, 2):
+    for md in (mode, 2):
         m = fac.create(xyz, cells)
         o = cls(m, n_iter=3 * iters, fixed_boundary=True)
-        o.set_proposal(mlp if mode == 1 else None, mode=mode, depth=4, window_frac=0.25)
+        o.set_proposal(mlp if md != 2 else None, mode=md, depth=depth, window_frac=frac, fallback=policy)
         q = [o.getQuality()]
         it = 0
         while it < 2 * iters and q[-1] < q_plain - 1e-3:
@@ -188,13 +208,13 @@ def test_learned_reaches_plain_quality_and_ablation(kind):
             it += 1
             q.append(o.getQuality())
         st = o.stats()
-        res[mode] = (it, q[-1], st["fallbacks"] / max(1, st["searches_learned"]))
+        res[md] = (it, q[-1], st["fallbacks"] / max(1, st["searches_learned"]))
         assert all(b >= a for a, b in zip(q, q[1:]))
         o.close()
         m.close()
-    print(kind, "plain", q_plain, "learned", res[1], "identity", res[2])
-    assert res[1][1] >= q_plain - 1e-3, res
-    assert res[1][0] < res[2][0] or (res[1][0] == res[2][0] and res[1][2] < res[2][2]), res
+    print(kind, depth, frac, policy, "plain", q_plain, f"mode {mode}", res[mode], "identity", res[2])
+    assert res[mode][1] >= q_plain - 1e-3, res
+    assert res[mode][0] < res[2][0] or (res[mode][0] == res[2][0] and res[mode][2] < res[2][2]), res
 
 
 @pytest.mark.parametrize("fallback", ["immediate", "deferred"])

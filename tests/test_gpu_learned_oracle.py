@@ -4,7 +4,8 @@ The oracle runs the learned iteration class by class on the CPU: each
 class's one-ring features (tests/mlp_ref.py, bit-exact with the GPU's), the
 network's outputs for them (vsdmo_mlp_forward, the network the search
 kernels run inside them, bit for bit), the proposal c = v + L y, the window
-search (or_learned_window, pinned against tests/ref64.py), and the plain full
+search (or_learned_window, pinned against tests/ref64.py; mode 3 searches the
+same window around v where the network's finds nothing better), and the plain full
 search (or_dmo_vertex) for the vertices that fall back -- in the same class
 pass (immediate policy) or in the class's next pass (deferred).  The GPU
 optimiser and a one-mesh batch must end every iteration on the same
@@ -67,7 +68,7 @@ def oracle_learned(oracle, kind, xyz, cells, mlp, mode, depth_l, frac, policy, i
             lo, hi = int(off[c]), int(off[c + 1])
             pos = om.positions()
             new = pos.copy()
-            if mode == 1:
+            if mode in (1, 3):
                 feats, ls = zip(*(mlp_ref.features(pos, nbr[v], v) for v in range(lo, hi)))
                 Y = mlp.forward(np.stack(feats)).astype(np.float32)
             for i, v in enumerate(range(lo, hi)):
@@ -77,8 +78,10 @@ def oracle_learned(oracle, kind, xyz, cells, mlp, mode, depth_l, frac, policy, i
                     fallbacks += 1
                     continue
                 x0 = pos[v]
-                c3 = (x0 + np.float32(ls[i]) * Y[i]).astype(np.float32) if mode == 1 else x0
+                c3 = (x0 + np.float32(ls[i]) * Y[i]).astype(np.float32) if mode in (1, 3) else x0
                 st, p = om.learned_window(v, pos, c3, depth_l, frac)
+                if mode == 3 and st == -1:                 # the same window around the vertex itself
+                    st, p = om.learned_window(v, pos, x0, depth_l, frac)
                 if st == 1:
                     new[v] = p
                 elif st == -1:
@@ -102,6 +105,9 @@ CASES = [
     ("hex", 1, 1, 2.0, "deferred"),        # the shipped hex window
     ("hex", 1, 1, 2.0, "immediate"),
     ("hex", 2, 1, 0.25, "deferred"),
+    ("tet", 3, 1, 0.025, "deferred"),      # network, then the vertex's own window
+    ("tet", 3, 2, 0.05, "immediate"),
+    ("hex", 3, 1, 2.0, "deferred"),
 ]
 
 
@@ -109,7 +115,7 @@ CASES = [
 def test_learned_iterations_match_oracle(oracle, nets, kind, mode, depth, frac, policy):
     n = 10 if kind == "tet" else 8
     xyz, cells = _mesh(kind, n, 17)
-    mlp = nets[kind] if mode == 1 else None
+    mlp = nets[kind] if mode in (1, 3) else None
     ref, ref_fb = oracle_learned(oracle, kind, xyz, cells, mlp, mode, depth, frac, policy, ITERS)
     fac, cls = (DMOMeshTetFactory, DMOTetClass) if kind == "tet" else (DMOMeshHexFactory, DMOHexClass)
     m = fac.create(xyz, cells)

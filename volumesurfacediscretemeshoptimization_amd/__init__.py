@@ -339,7 +339,9 @@ class _DMOBase:
     def set_proposal(self, mlp=None, mode: int = 1, depth: int = None, window_frac: float = None,
                      fallback: str = None):
         """Learned proposal for the inner searches: mode 1 = network, 2 = the
-        vertex itself (same windowed search and fallback, no network), 0 = off.
+        vertex itself (same windowed search and fallback, no network), 3 = the
+        network, then (where its window finds nothing better) the vertex
+        itself before the full search, 0 = off.
         The window is `depth` levels of 4^3 candidates on `window_frac` times
         the plain first-level window (defaults per element type above).
         fallback: "immediate" (the full search of a vertex whose window finds
@@ -354,7 +356,7 @@ class _DMOBase:
         h = mlp.handle if mlp is not None else None
         check(self._lib.vsdmo_opt_set_proposal(self._h, h, int(mode), int(depth), float(window_frac)),
               "vsdmo_opt_set_proposal")
-        self._mlp = mlp if mode == 1 else None   # the network is destroyed after the optimiser
+        self._mlp = mlp if mode in (1, 3) else None   # the network is destroyed after the optimiser
 
     FALLBACK = {"immediate": 0, "deferred": 1}
 
@@ -599,7 +601,7 @@ class DMOBatch:
               "vsdmo_batch_set_proposal")
         if not hasattr(self, "_mlps"):
             self._mlps = {}
-        self._mlps[int(elem)] = mlp if mode == 1 else None   # networks are destroyed after the batch
+        self._mlps[int(elem)] = mlp if mode in (1, 3) else None   # networks are destroyed after the batch
 
     def set_fallback(self, policy: str):
         """Fallback policy of the learned proposals ("immediate", the

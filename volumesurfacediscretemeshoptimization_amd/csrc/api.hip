@@ -650,7 +650,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                     if (o->learn_mode != 0 && !ds) {
                         LearnNet net;
                         net.Y = o->lY.p; net.L = o->lL.p;
-                        if (o->learn_mode == 1) {
+                        if (learn_uses_net(o->learn_mode)) {
                             if (!o->learn_separate && learned_fused_ok(m->elem, m->maxRing)) {
                                 net.wb = o->mlp->wb.p; net.w = o->mlp->w.p; net.nbr = m->nbr.p;
                             } else {
@@ -924,8 +924,9 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
     return guarded([&]() -> vsdmo_status {
         if (!o) return fail(VSDMO_INVALID_ARG, "NULL optimiser");
         vsdmo_mesh* m = o->m;
-        if (mode < 0 || mode > 2) return fail(VSDMO_INVALID_ARG, "mode must be 0 (off), 1 (network) or 2 (identity)");
-        if (mode == 1 && !mlp) return fail(VSDMO_INVALID_ARG, "mode 1 needs a network");
+        if (mode < 0 || mode > 3)
+            return fail(VSDMO_INVALID_ARG, "mode must be 0 (off), 1 (network), 2 (identity) or 3 (network, then identity)");
+        if (learn_uses_net(mode) && !mlp) return fail(VSDMO_INVALID_ARG, "modes 1 and 3 need a network");
         if (mode != 0 && m->elem != VSDMO_TET && m->elem != VSDMO_HEX)
             return fail(VSDMO_NOT_SUPPORTED, "the learned proposal is for tet and hex inner vertices");
         if (mode != 0 && (depth < 1 || depth > 8 || !(window_frac > 0.0f)))
@@ -947,7 +948,7 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
             }
             if (!o->lCnt.p) o->lCnt.alloc(std::max<size_t>(m->colOffInner.size(), 1));
             reset_deferred(o);
-            if (mode == 1) {
+            if (learn_uses_net(mode)) {
                 build_neighbour_table(*m);
                 VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
             }
@@ -955,7 +956,7 @@ vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, 
         }
         o->learn_mode = mode;
         if (o->mlp) o->mlp->n_users--;
-        o->mlp = mode == 1 ? mlp : nullptr;
+        o->mlp = learn_uses_net(mode) ? mlp : nullptr;
         if (o->mlp) o->mlp->n_users++;
         o->learn_depth = depth;
         o->learn_frac = window_frac;

@@ -242,7 +242,7 @@ void launch_learned_items(vsdmo_batch* b, int t, const SearchArgs& a, int lo, in
     }
     LearnNet net;
     net.items = b->items.p;
-    if (L.mode == 1) { net.wb = L.mlp->wb.p; net.w = L.mlp->w.p; net.nbr = b->nbr.p; }
+    if (learn_uses_net(L.mode)) { net.wb = L.mlp->wb.p; net.w = L.mlp->w.p; net.nbr = b->nbr.p; }
     const int nR = (int)b->grid.size();
     const int ring = t == 0 ? b->maxRingTet : b->maxRingHex;
     const int kind = t == 0 ? VSDMO_TET : VSDMO_HEX;
@@ -754,13 +754,15 @@ vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* m
         if (!b) return fail_status(VSDMO_INVALID_ARG, "NULL batch");
         if (elem != VSDMO_TET && elem != VSDMO_HEX)
             return fail_status(VSDMO_NOT_SUPPORTED, "the learned proposal is for the tet and hex meshes of a batch");
-        if (mode < 0 || mode > 2) return fail_status(VSDMO_INVALID_ARG, "mode must be 0 (off), 1 (network) or 2 (identity)");
-        if (mode == 1 && !mlp) return fail_status(VSDMO_INVALID_ARG, "mode 1 needs a network");
+        if (mode < 0 || mode > 3)
+            return fail_status(VSDMO_INVALID_ARG,
+                               "mode must be 0 (off), 1 (network), 2 (identity) or 3 (network, then identity)");
+        if (learn_uses_net(mode) && !mlp) return fail_status(VSDMO_INVALID_ARG, "modes 1 and 3 need a network");
         if (mode != 0 && (depth < 1 || depth > 8 || !(window_frac > 0.0f)))
             return fail_status(VSDMO_INVALID_ARG, "depth must be in [1,8] and window_frac > 0");
         const int t = elem == VSDMO_TET ? 0 : 1;
         const int ring = t == 0 ? b->maxRingTet : b->maxRingHex;
-        if (mode == 1 && !learned_fused_ok(elem, ring))
+        if (learn_uses_net(mode) && !learned_fused_ok(elem, ring))
             return fail_status(VSDMO_NOT_SUPPORTED, "the batched learned search runs the network in the window "
                                                     "kernel, whose LDS tile of 16 rings does not hold one-rings of " +
                                                         std::to_string(ring) + " elements");
@@ -785,7 +787,7 @@ vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* m
                 b->lFallback.alloc(1);
                 launch_fill_int(b->lFallback.p, 1, 0, s);
             }
-            if (mode == 1 && !b->nbr.p) {
+            if (learn_uses_net(mode) && !b->nbr.p) {
                 std::vector<int> tab((size_t)b->totalV * kNbr, -1);
                 for (size_t i = 0; i < b->meshes.size(); ++i) {
                     vsdmo_mesh* m = b->meshes[i];
@@ -799,13 +801,13 @@ vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* m
                 b->nbr.alloc(tab.size());
                 b->nbr.upload(tab.data(), tab.size(), s);
             }
-            if (mode == 1) VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
+            if (learn_uses_net(mode)) VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
             VSDMO_HIP(hipStreamSynchronize(s));
         }
         vsdmo_batch::Learn& L = b->learn[t];
         L.mode = mode;
         if (L.mlp) L.mlp->n_users--;
-        L.mlp = mode == 1 ? mlp : nullptr;
+        L.mlp = learn_uses_net(mode) ? mlp : nullptr;
         if (L.mlp) L.mlp->n_users++;
         L.depth = depth;
         L.frac = window_frac;

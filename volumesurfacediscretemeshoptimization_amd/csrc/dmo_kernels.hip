@@ -696,7 +696,7 @@ __device__ __forceinline__ void learned_offset(const SearchArgs& a, const LearnA
         mlp_layers16(la.wb, la.w, sm);
         l = sm.L[gb]; y0 = sm.Y[gb][0]; y1 = sm.Y[gb][1]; y2 = sm.Y[gb][2];
         __syncthreads();   // the ring staging reuses the LDS
-    } else if (la.mode == 1) {
+    } else if (learn_uses_net(la.mode)) {
         l = la.L[ci]; y0 = la.Y[3 * ci]; y1 = la.Y[3 * ci + 1]; y2 = la.Y[3 * ci + 2];
     }
     (void)ci;
@@ -813,54 +813,65 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const float q0 = tet_score(f, tc, 0.0f, 0.0f, 0.0f);
 
     float cx = f.x0x, cy = f.x0y, cz = f.x0z;
-    if (la.mode == 1) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
+    if (learn_uses_net(la.mode)) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
     float qbest = q0;
     bool moved = false, tie = false;
     const float affL = 1.0f / (float)(NL - 1);
     const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
-    float ds = la.frac * a.gridScale;
-    for (int d = 0; d < la.depthL; ++d) {
-        const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
-        const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
-        const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
-        const float py = (grid_coord(affL, lj, NL, gym, gyM) - f.x0y) * f.sg;
-        const float pz = (grid_coord(affL, lk, NL, gzm, gzM) - f.x0z) * f.sg;
-        float px[CPL], t3[CPL];
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-            px[i] = (grid_coord(affL, i0 + i, NL, gxm, gxM) - f.x0x) * f.sg;
-            t3[i] = 3.0f * px[i];
+    // depthL levels of the 4x4x4 window around (cx, cy, cz); the branch
+    // conditions are uniform over the vertex's lane group
+    auto levels = [&]() {
+        float ds = la.frac * a.gridScale;
+        for (int d = 0; d < la.depthL; ++d) {
+            const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
+            const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
+            const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
+            const float py = (grid_coord(affL, lj, NL, gym, gyM) - f.x0y) * f.sg;
+            const float pz = (grid_coord(affL, lk, NL, gzm, gzM) - f.x0z) * f.sg;
+            float px[CPL], t3[CPL];
+    #pragma unroll
+            for (int i = 0; i < CPL; ++i) {
+                px[i] = (grid_coord(affL, i0 + i, NL, gxm, gxM) - f.x0x) * f.sg;
+                t3[i] = 3.0f * px[i];
+            }
+            const float Qyz = 3.0f * fmaf(pz, pz, py * py);
+            TetAcc acc[CPL];
+    #pragma unroll
+            for (int i = 0; i < CPL; ++i) acc[i] = tet_acc0();
+            for (int e = 0; e < f.n; ++e) {
+                const TetC c = tc[e];
+                const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+                const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+    #pragma unroll
+                for (int i = 0; i < CPL; ++i)
+                    tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
+            }
+            float best = -FLT_MAX;
+            int bi = 0;
+    #pragma unroll
+            for (int i = 0; i < CPL; ++i) {
+                float q = tet_acc_score(acc[i], f.is3);
+                if (q > best) { best = q; bi = i; }
+            }
+            int cand = i0 + bi + NL * (s % (NL * NL));
+            group_argmax<G>(best, cand);
+            tie = tie || (best == q0 && q0 > 0.0f);
+            if (qbest < best) {
+                cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
+                cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
+                cz = grid_coord(affL, cand / (NL * NL), NL, gzm, gzM);
+                qbest = best;
+                moved = true;
+            }
+            ds = ds * (2.0f / (float)(NL - 1));
         }
-        const float Qyz = 3.0f * fmaf(pz, pz, py * py);
-        TetAcc acc[CPL];
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) acc[i] = tet_acc0();
-        for (int e = 0; e < f.n; ++e) {
-            const TetC c = tc[e];
-            const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
-            const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
-#pragma unroll
-            for (int i = 0; i < CPL; ++i)
-                tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
-        }
-        float best = -FLT_MAX;
-        int bi = 0;
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-            float q = tet_acc_score(acc[i], f.is3);
-            if (q > best) { best = q; bi = i; }
-        }
-        int cand = i0 + bi + NL * (s % (NL * NL));
-        group_argmax<G>(best, cand);
-        tie = tie || (best == q0 && q0 > 0.0f);
-        if (qbest < best) {
-            cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
-            cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
-            cz = grid_coord(affL, cand / (NL * NL), NL, gzm, gzM);
-            qbest = best;
-            moved = true;
-        }
-        ds = ds * (2.0f / (float)(NL - 1));
+    };
+    levels();
+    // mode 3: where the network's window finds nothing better, the same
+    // window around the vertex itself before the full search
+    if (la.mode == 3 && !moved && !tie) {
+        cx = f.x0x; cy = f.x0y; cz = f.x0z;
+        levels();
     }
     if (moved && own && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
     const bool need = !moved && !tie && own && s == 0;
@@ -911,58 +922,65 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const float q0 = hex_score(f, hx, 0.0f, 0.0f, 0.0f);
 
     float cx = f.x0x, cy = f.x0y, cz = f.x0z;
-    if (la.mode == 1) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
+    if (learn_uses_net(la.mode)) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
     float qbest = q0;
     bool moved = false, tie = false;
     const float affL = 1.0f / (float)(NL - 1);
     const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
-    float ds = la.frac * a.gridScale;
-    for (int d = 0; d < la.depthL; ++d) {
-        const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
-        const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
-        const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
-        const float py = (grid_coord(affL, lj, NL, gym, gyM) - f.x0y) * f.sg;
-        const float pz = (grid_coord(affL, lk, NL, gzm, gzM) - f.x0z) * f.sg;
-        float px[CPL], t3[CPL];
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-            px[i] = (grid_coord(affL, i0 + i, NL, gxm, gxM) - f.x0x) * f.sg;
-            t3[i] = 3.0f * px[i];
-        }
-        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
-        HexAcc acc[CPL];
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) acc[i] = f.acc0;
-        for (int e = 0; e < f.n; ++e) {
-#pragma unroll
-            for (int sIdx = 0; sIdx < 5; ++sIdx) {
-                const HexP h = hx[e].hp[sIdx];
-                const bool a3 = (sIdx == 1 || sIdx == 4);
-                const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
-                const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
-#pragma unroll
-                for (int i = 0; i < CPL; ++i)
-                    hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
+    auto levels = [&]() {
+        float ds = la.frac * a.gridScale;
+        for (int d = 0; d < la.depthL; ++d) {
+            const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
+            const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
+            const float gzM = cz + ds * f.ez, gzm = cz - ds * f.ez;
+            const float py = (grid_coord(affL, lj, NL, gym, gyM) - f.x0y) * f.sg;
+            const float pz = (grid_coord(affL, lk, NL, gzm, gzM) - f.x0z) * f.sg;
+            float px[CPL], t3[CPL];
+    #pragma unroll
+            for (int i = 0; i < CPL; ++i) {
+                px[i] = (grid_coord(affL, i0 + i, NL, gxm, gxM) - f.x0x) * f.sg;
+                t3[i] = 3.0f * px[i];
             }
+            const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
+            HexAcc acc[CPL];
+    #pragma unroll
+            for (int i = 0; i < CPL; ++i) acc[i] = f.acc0;
+            for (int e = 0; e < f.n; ++e) {
+    #pragma unroll
+                for (int sIdx = 0; sIdx < 5; ++sIdx) {
+                    const HexP h = hx[e].hp[sIdx];
+                    const bool a3 = (sIdx == 1 || sIdx == 4);
+                    const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+                    const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+    #pragma unroll
+                    for (int i = 0; i < CPL; ++i)
+                        hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
+                }
+            }
+            float best = -FLT_MAX;
+            int bi = 0;
+    #pragma unroll
+            for (int i = 0; i < CPL; ++i) {
+                float q = hex_acc_score(acc[i], f.epsD, f.epsF);
+                if (q > best) { best = q; bi = i; }
+            }
+            int cand = i0 + bi + NL * (s % (NL * NL));
+            group_argmax<G>(best, cand);
+            tie = tie || (best == q0 && q0 > 0.0f);
+            if (qbest < best) {
+                cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
+                cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
+                cz = grid_coord(affL, cand / (NL * NL), NL, gzm, gzM);
+                qbest = best;
+                moved = true;
+            }
+            ds = ds * (2.0f / (float)(NL - 1));
         }
-        float best = -FLT_MAX;
-        int bi = 0;
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-            float q = hex_acc_score(acc[i], f.epsD, f.epsF);
-            if (q > best) { best = q; bi = i; }
-        }
-        int cand = i0 + bi + NL * (s % (NL * NL));
-        group_argmax<G>(best, cand);
-        tie = tie || (best == q0 && q0 > 0.0f);
-        if (qbest < best) {
-            cx = grid_coord(affL, cand % NL, NL, gxm, gxM);
-            cy = grid_coord(affL, (cand / NL) % NL, NL, gym, gyM);
-            cz = grid_coord(affL, cand / (NL * NL), NL, gzm, gzM);
-            qbest = best;
-            moved = true;
-        }
-        ds = ds * (2.0f / (float)(NL - 1));
+    };
+    levels();
+    if (la.mode == 3 && !moved && !tie && !hex_pinned(f)) {
+        cx = f.x0x; cy = f.x0y; cz = f.x0z;
+        levels();
     }
     if (moved && own && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
     const bool need = !moved && !tie && !hex_pinned(f) && own && s == 0;
@@ -1110,7 +1128,7 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
     if (nc <= 0) return;
     ringCap = std::max(ringCap, 1);
     const size_t per = learned_per(kind);
-    const bool fused = mode == 1 && net.wb && learned_fused_ok(kind, ringCap);
+    const bool fused = learn_uses_net(mode) && net.wb && learned_fused_ok(kind, ringCap);
     LearnArgs la{ net.Y, net.L, mode, depthL, frac, fallbacks, list, cnt, net.wb, net.w, net.nbr,
                   nullptr, nullptr, nullptr, 0, net.items };
     if (def) {
