@@ -910,8 +910,8 @@ def bench_north_star(args):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="north_star",
                     help="north_star (default): learned + batched vs plain at equal quality on tet40 and hex48; "
                          "the others: one plain (or --mode learned) line per workload")

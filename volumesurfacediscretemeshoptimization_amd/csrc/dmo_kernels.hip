@@ -679,27 +679,30 @@ __device__ void group_extents(const SearchArgs& a, int r0, int n, int st, int s,
     mabs = group_max<G>(mabs);
 }
 
-// proposal displacement of the group's vertex (class position ci): c = v +
-// l * y.  FUSED: the block's 16 vertices (class positions ci0 + 0..15, group
-// gb = threadIdx.x / 16) are one tile of the network, run here in the LDS
-// lds, which is free again on return; otherwise l, y come from the separate
-// inference launch's L, Y (mode 1) or are zero (mode 2).
+// proposal displacement of the group's vertex (class position ci, at (vx,
+// vy, vz)): c = v + l * y.  FUSED: the block's 16 vertices (class positions
+// ci0 + 0..15, group gb = threadIdx.x / 16) are one tile of the network, run
+// here in the LDS lds (free again on return) from the inputs the caller
+// gathered (ng: the row's table neighbours, nw: the wave's weights);
+// otherwise l, y come from the separate inference launch's L, Y (mode 1) or
+// are zero (mode 2).
 template <bool FUSED>
 __device__ __forceinline__ void learned_offset(const SearchArgs& a, const LearnArgs& la, int ci0, int gb, int ci,
-                                               int vid, unsigned char* lds, float& l, float& y0, float& y1, float& y2)
+                                               float vx, float vy, float vz, const MlpGather& ng, const MlpW& nw,
+                                               unsigned char* lds, float& l, float& y0, float& y1, float& y2)
 {
     l = 0.0f; y0 = 0.0f; y1 = 0.0f; y2 = 0.0f;
     if constexpr (FUSED) {
         MlpLds& sm = *reinterpret_cast<MlpLds*>(lds);
-        const bool live = ci0 + gb < a.cEnd - a.cStart;   // vid: row gb's vertex (clamped for a dead row, unused)
-        mlp_features16(a.x, a.y, a.z, la.nbr, vid, live, sm);
-        mlp_layers16(la.wb, la.w, sm);
+        const bool live = ci0 + gb < a.cEnd - a.cStart;
+        mlp_features16_from(ng, vx, vy, vz, live, sm);
+        mlp_layers16_w(nw, la.wb, la.w, sm);
         l = sm.L[gb]; y0 = sm.Y[gb][0]; y1 = sm.Y[gb][1]; y2 = sm.Y[gb][2];
         __syncthreads();   // the ring staging reuses the LDS
     } else if (learn_uses_net(la.mode)) {
         l = la.L[ci]; y0 = la.Y[3 * ci]; y1 = la.Y[3 * ci + 1]; y2 = la.Y[3 * ci + 2];
     }
-    (void)ci;
+    (void)ci; (void)vx; (void)vy; (void)vz; (void)ng; (void)nw;
 }
 
 // append vid to the class's fallback list where need (one lane per group):
@@ -740,9 +743,12 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const int r0 = a.ringRow[vid];
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
-    // the lane's first PRE ring elements are gathered into registers before
-    // the network runs (their latency overlaps it; no other vertex of this
-    // launch moves them: ring vertices belong to other colour classes)
+    // the network's neighbours, the lane's first PRE ring elements and the
+    // network's weights are loaded into registers before the network runs
+    // (their latencies overlap; no other vertex of this launch moves these
+    // positions: ring and table neighbours belong to other colour classes)
+    MlpGather ng{};
+    if constexpr (FUSED) ng = mlp_gather16(a.x, a.y, a.z, la.nbr, vid, ci0 + gb < nc);
     constexpr int PRE = 2;
     float rx[PRE][3], ry[PRE][3], rz[PRE][3];
 #pragma unroll
@@ -757,8 +763,10 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
             }
         }
     }
+    MlpW nw{};
+    if constexpr (FUSED) nw = mlp_load_w16(la.wb, la.w);
     float pl, py0, py1, py2;
-    learned_offset<FUSED>(a, la, ci0, gb, ci, vid, s_tcl, pl, py0, py1, py2);
+    learned_offset<FUSED>(a, la, ci0, gb, ci, f.x0x, f.x0y, f.x0z, ng, nw, s_tcl, pl, py0, py1, py2);
     TetC* tc = reinterpret_cast<TetC*>(s_tcl) + (size_t)gb * ringCap;
     // window half-extents and frame scale (max / min: the order of the
     // elements does not matter), then the ring constants into LDS
@@ -810,11 +818,14 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
         tc[e] = tet_consts(p[0], p[1], p[2]);
     }
     __syncthreads();
-    const float q0 = tet_score(f, tc, 0.0f, 0.0f, 0.0f);
 
     float cx = f.x0x, cy = f.x0y, cz = f.x0z;
     if (learn_uses_net(la.mode)) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
-    float qbest = q0;
+    // q0, the start position's score, is accumulated beside the candidates
+    // of the first level (at p = 0: det = D, L = K exactly, the terms
+    // tet_score adds, in the same ring order)
+    float q0 = 0.0f, qbest = 0.0f;
+    bool haveQ0 = false;
     bool moved = false, tie = false;
     const float affL = 1.0f / (float)(NL - 1);
     const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
@@ -835,7 +846,7 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
                 t3[i] = 3.0f * px[i];
             }
             const float Qyz = 3.0f * fmaf(pz, pz, py * py);
-            TetAcc acc[CPL];
+            TetAcc acc[CPL], acc0 = tet_acc0();
     #pragma unroll
             for (int i = 0; i < CPL; ++i) acc[i] = tet_acc0();
             for (int e = 0; e < f.n; ++e) {
@@ -845,7 +856,9 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     #pragma unroll
                 for (int i = 0; i < CPL; ++i)
                     tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
+                tet_acc_add(acc0, c.D, c.K);
             }
+            if (!haveQ0) { q0 = tet_acc_score(acc0, f.is3); qbest = q0; haveQ0 = true; }
             float best = -FLT_MAX;
             int bi = 0;
     #pragma unroll
@@ -890,13 +903,20 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const int ci = min(ci0 + gb, nc - 1);
     const int vid = learned_vertex(a, la, ci);
     const bool own = ci0 + gb < nc && !(la.epoch && la.epoch[vid] == la.it - 1);
-    float pl, py0, py1, py2;
-    learned_offset<FUSED>(a, la, ci0, gb, ci, vid, s_hxl, pl, py0, py1, py2);
-    HexLds* hx = reinterpret_cast<HexLds*>(s_hxl) + (size_t)gb * ringCap;
     HexFrame f;
     const int r0 = a.ringRow[vid];
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
+    // the network's inputs (table neighbours, weights) are loaded before it runs
+    MlpGather ng{};
+    MlpW nw{};
+    if constexpr (FUSED) {
+        ng = mlp_gather16(a.x, a.y, a.z, la.nbr, vid, ci0 + gb < nc);
+        nw = mlp_load_w16(la.wb, la.w);
+    }
+    float pl, py0, py1, py2;
+    learned_offset<FUSED>(a, la, ci0, gb, ci, f.x0x, f.x0y, f.x0z, ng, nw, s_hxl, pl, py0, py1, py2);
+    HexLds* hx = reinterpret_cast<HexLds*>(s_hxl) + (size_t)gb * ringCap;
     float mabs;
     group_extents<G>(a, r0, f.n, 7, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
     f.sg = pow2_scale(mabs);
@@ -919,11 +939,14 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     for (int e = 0; e < f.n; ++e)
 #pragma unroll
         for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(f.acc0, hx[e].cd[sIdx], hx[e].cF[sIdx]);
-    const float q0 = hex_score(f, hx, 0.0f, 0.0f, 0.0f);
 
     float cx = f.x0x, cy = f.x0y, cz = f.x0z;
     if (learn_uses_net(la.mode)) { cx = f.x0x + pl * py0; cy = f.x0y + pl * py1; cz = f.x0z + pl * py2; }
-    float qbest = q0;
+    // q0, the start position's score, is accumulated beside the candidates
+    // of the first level (at p = 0: det = D, |J|^2 = K exactly, the terms
+    // hex_score adds, in the same order)
+    float q0 = 0.0f, qbest = 0.0f;
+    bool haveQ0 = false;
     bool moved = false, tie = false;
     const float affL = 1.0f / (float)(NL - 1);
     const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
@@ -942,7 +965,7 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
                 t3[i] = 3.0f * px[i];
             }
             const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
-            HexAcc acc[CPL];
+            HexAcc acc[CPL], accO = f.acc0;
     #pragma unroll
             for (int i = 0; i < CPL; ++i) acc[i] = f.acc0;
             for (int e = 0; e < f.n; ++e) {
@@ -955,8 +978,10 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     #pragma unroll
                     for (int i = 0; i < CPL; ++i)
                         hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
+                    hex_acc_add(accO, h.D, h.K);
                 }
             }
+            if (!haveQ0) { q0 = hex_acc_score(accO, f.epsD, f.epsF); qbest = q0; haveQ0 = true; }
             float best = -FLT_MAX;
             int bi = 0;
     #pragma unroll

@@ -55,76 +55,133 @@ struct MlpLds {
     __align__(16) uint16_t X[kMlpRows * kLdx];
     __align__(16) uint16_t H1[kMlpRows * kLdh];
     __align__(16) uint16_t H2[kMlpRows * kLdh];
-    float rel[kMlpRows][kNbr][3];
     float dist[kMlpRows][kNbr];
     float L[kMlpRows];
     float Y[kMlpRows][kOut];
 };
 
-// features of row r = threadIdx.x >> 4 (vertex v; live = it exists): 16
-// threads per row gather two table neighbours each; L = mean distance summed
-// in neighbour order (the order of k_features); X = bf16((n_k - v) / L),
-// zero padded.  All 256 threads call it; ends with a barrier.
-__device__ __forceinline__ void mlp_features16(const float* __restrict__ x, const float* __restrict__ y,
-                                               const float* __restrict__ z, const int* __restrict__ nbr, int v,
-                                               bool live, MlpLds& sm)
+// The two table neighbours 2q, 2q+1 (q = threadIdx.x & 15) of row vertex v
+// and their positions, gathered into registers; a caller issues this first
+// so that its latency overlaps the caller's other loads.
+struct MlpGather {
+    int u[2];
+    float x[2], y[2], z[2];
+};
+__device__ __forceinline__ MlpGather mlp_gather16(const float* __restrict__ x, const float* __restrict__ y,
+                                                  const float* __restrict__ z, const int* __restrict__ nbr, int v,
+                                                  bool live)
+{
+    const int q = threadIdx.x & 15;
+    MlpGather g;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        g.u[k] = live ? nbr[(size_t)v * kNbr + 2 * q + k] : -1;
+        g.x[k] = 0.0f; g.y[k] = 0.0f; g.z[k] = 0.0f;
+        if (g.u[k] >= 0) { g.x[k] = x[g.u[k]]; g.y[k] = y[g.u[k]]; g.z[k] = z[g.u[k]]; }
+    }
+    return g;
+}
+
+// features of row r = threadIdx.x >> 4 (vertex at (vx, vy, vz); live = it
+// exists) from the row's gathered neighbours: L = mean distance summed in
+// neighbour order (the order of k_features); X = bf16((n_k - v) / L), zero
+// padded.  All 256 threads call it; ends with a barrier.
+__device__ __forceinline__ void mlp_features16_from(const MlpGather& g, float vx, float vy, float vz, bool live,
+                                                    MlpLds& sm)
 {
     const int r = threadIdx.x >> 4, q = threadIdx.x & 15;
-    if (live) {
-        const int* nb = nbr + (size_t)v * kNbr;
-        const float vx = x[v], vy = y[v], vz = z[v];
+    float rel[2][3];
 #pragma unroll
-        for (int k = 2 * q; k < 2 * q + 2; ++k) {
-            const int u = nb[k];
-            if (u < 0) { sm.dist[r][k] = -1.0f; continue; }
-            const float dx = x[u] - vx, dy = y[u] - vy, dz = z[u] - vz;
-            sm.rel[r][k][0] = dx; sm.rel[r][k][1] = dy; sm.rel[r][k][2] = dz;
-            sm.dist[r][k] = sqrtf((dx * dx + dy * dy) + dz * dz);
-        }
+    for (int k = 0; k < 2; ++k) {
+        rel[k][0] = g.x[k] - vx; rel[k][1] = g.y[k] - vy; rel[k][2] = g.z[k] - vz;
+        sm.dist[r][2 * q + k] = sqrtf((rel[k][0] * rel[k][0] + rel[k][1] * rel[k][1]) + rel[k][2] * rel[k][2]);
     }
     // the table's valid entries are a prefix: cnt = their number, counted by
     // a ballot over the row's 16 threads (lanes 16 (r % 4) + q of the wave)
-    const int* nbv = nbr + (size_t)v * kNbr + 2 * q;
-    const bool v0 = live && nbv[0] >= 0, v1 = live && nbv[1] >= 0;
+    const bool v0 = live && g.u[0] >= 0, v1 = live && g.u[1] >= 0;
     const unsigned long long rowMask = 0xffffull << (16 * (r & 3));
     const int cnt = __popcll(__ballot(v0) & rowMask) + __popcll(__ballot(v1) & rowMask);
     __syncthreads();
-    if (q == 0) {
-        float sum = 0.0f, l = 1.0f;
-        if (live) {
+    // every thread of the row sums the row's distances in neighbour order
+    // (all 32 read first, then added: no read waits inside the sum)
+    float l = 1.0f;
+    if (live) {
+        float d[kNbr];
 #pragma unroll
-            for (int k = 0; k < kNbr; ++k)   // neighbour order, as k_features
-                if (k < cnt) sum += sm.dist[r][k];
-            l = cnt > 0 ? sum / (float)cnt : 1.0f;
-            if (!(l > 0.0f)) l = 1.0f;
-        }
-        sm.L[r] = l;
+        for (int k = 0; k < kNbr; ++k) d[k] = sm.dist[r][k];
+        float sum = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kNbr; ++k) sum = (k < cnt) ? sum + d[k] : sum;
+        l = cnt > 0 ? sum / (float)cnt : 1.0f;
+        if (!(l > 0.0f)) l = 1.0f;
     }
-    __syncthreads();
-    const float l = sm.L[r];
+    if (q == 0) sm.L[r] = l;
     uint16_t* o = sm.X + r * kLdx;
 #pragma unroll
-    for (int k = 2 * q; k < 2 * q + 2; ++k) {
-        if (k < cnt) {
-            o[3 * k] = f2bf(sm.rel[r][k][0] / l);
-            o[3 * k + 1] = f2bf(sm.rel[r][k][1] / l);
-            o[3 * k + 2] = f2bf(sm.rel[r][k][2] / l);
+    for (int k = 0; k < 2; ++k) {
+        const int kk = 2 * q + k;
+        if (kk < cnt) {
+            o[3 * kk] = f2bf(rel[k][0] / l);
+            o[3 * kk + 1] = f2bf(rel[k][1] / l);
+            o[3 * kk + 2] = f2bf(rel[k][2] / l);
         } else {
-            o[3 * k] = 0; o[3 * k + 1] = 0; o[3 * k + 2] = 0;
+            o[3 * kk] = 0; o[3 * kk + 1] = 0; o[3 * kk + 2] = 0;
         }
     }
     __syncthreads();
 }
 
-// the three layers of the 16 rows of sm.X into sm.Y.  Layers 1-2: wave w
-// computes columns [32w, 32w + 32) -- two 16x16 tiles of
-// mfma_f32_16x16x32_bf16 (A: 16 LDS rows, lane l holds row l%16, k = 8(l/16)
-// + 0..7; B: 16-byte weight rows from global memory; D: column l%16, rows
-// 4(l/16) + 0..3), then bias + GELU to BF16 in LDS.  Layer 3 (128 -> 3):
-// one tile on wave 0 with W3 zero padded to 16 columns.  All 256 threads
-// call it; ends with a barrier.
-__device__ __forceinline__ void mlp_layers16(const uint16_t* __restrict__ wb, const float* __restrict__ w, MlpLds& sm)
+// gather + features of row r = threadIdx.x >> 4 (vertex v)
+__device__ __forceinline__ void mlp_features16(const float* __restrict__ x, const float* __restrict__ y,
+                                               const float* __restrict__ z, const int* __restrict__ nbr, int v,
+                                               bool live, MlpLds& sm)
 {
+    const MlpGather g = mlp_gather16(x, y, z, nbr, v, live);
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    if (live) { vx = x[v]; vy = y[v]; vz = z[v]; }
+    mlp_features16_from(g, vx, vy, vz, live, sm);
+}
+
+// The weight operands of layers 1-2 that wave w = threadIdx.x / 64 reads
+// (its column tiles 2w, 2w+1; lane l: row (2w+t)*16 + l%16, k = 8(l/16) + 32i
+// + 0..7) and their biases, loaded into registers; a caller issues this
+// first so that the loads overlap its other work.
+struct MlpW {
+    bf16x8 b1[kIn / 32][2];
+    bf16x8 b2[kHid / 32][2];
+    float bias1[2], bias2[2];
+};
+__device__ __forceinline__ MlpW mlp_load_w16(const uint16_t* __restrict__ wb, const float* __restrict__ w)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    MlpW m;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = (2 * wave + t) * 16 + r16;
+#pragma unroll
+        for (int i = 0; i < kIn / 32; ++i)
+            m.b1[i][t] = *reinterpret_cast<const bf16x8*>(wb + kOW1 + (size_t)col * kIn + 32 * i + kq);
+#pragma unroll
+        for (int i = 0; i < kHid / 32; ++i)
+            m.b2[i][t] = *reinterpret_cast<const bf16x8*>(wb + kOW2 + (size_t)col * kHid + 32 * i + kq);
+        m.bias1[t] = w[kOB1 + col];
+        m.bias2[t] = w[kOB2 + col];
+    }
+    return m;
+}
+
+// the three layers of the 16 rows of sm.X into sm.Y, layers 1-2 with the
+// operands of mlp_load_w16.  Layers 1-2: wave w computes columns [32w, 32w +
+// 32) -- two 16x16 tiles of mfma_f32_16x16x32_bf16 (A: 16 LDS rows, lane l
+// holds row l%16, k = 8(l/16) + 0..7; B: the wave's weight rows; D: column
+// l%16, rows 4(l/16) + 0..3), then bias + GELU to BF16 in LDS.  Layer 3 (128
+// -> 3): one tile on wave 0 with W3 zero padded to 16 columns, its operands
+// loaded while layer 2 runs.  All 256 threads call it; ends with a barrier.
+__device__ __forceinline__ void mlp_layers16_w(const MlpW& m, const uint16_t* __restrict__ wb,
+                                               const float* __restrict__ w, MlpLds& sm)
+{
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
     {
@@ -132,65 +189,66 @@ __device__ __forceinline__ void mlp_layers16(const uint16_t* __restrict__ wb, co
 #pragma unroll
         for (int t = 0; t < 2; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-        for (int k0 = 0; k0 < kIn; k0 += 32) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.X + r16 * kLdx + k0 + kq);
+        for (int i = 0; i < kIn / 32; ++i) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.X + r16 * kLdx + 32 * i + kq);
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int col = (2 * wave + t) * 16 + r16;
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wb + kOW1 + (size_t)col * kIn + k0 + kq);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
-            }
+            for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, m.b1[i][t], acc[t], 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const int n = (2 * wave + t) * 16 + r16;
-            const float b = w[kOB1 + n];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sm.H1[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
+            for (int r = 0; r < 4; ++r) sm.H1[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + m.bias1[t]));
         }
     }
     __syncthreads();
+    // layer 3's operands (wave 0, columns 0..2), loaded while layer 2 runs
+    bf16x8 b3[kHid / 32];
+    float bias3 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kHid / 32; ++i) b3[i] = __builtin_bit_cast(bf16x8, (u32x4){ 0u, 0u, 0u, 0u });
+    if (wave == 0 && r16 < kOut) {
+#pragma unroll
+        for (int i = 0; i < kHid / 32; ++i)
+            b3[i] = *reinterpret_cast<const bf16x8*>(wb + kOW3 + (size_t)r16 * kHid + 32 * i + kq);
+        bias3 = w[kOB3 + r16];
+    }
     {
         f32x4 acc[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t) acc[t] = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-        for (int k0 = 0; k0 < kHid; k0 += 32) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.H1 + r16 * kLdh + k0 + kq);
+        for (int i = 0; i < kHid / 32; ++i) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.H1 + r16 * kLdh + 32 * i + kq);
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int col = (2 * wave + t) * 16 + r16;
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wb + kOW2 + (size_t)col * kHid + k0 + kq);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
-            }
+            for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, m.b2[i][t], acc[t], 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const int n = (2 * wave + t) * 16 + r16;
-            const float b = w[kOB2 + n];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sm.H2[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + b));
+            for (int r = 0; r < 4; ++r) sm.H2[((lane >> 4) * 4 + r) * kLdh + n] = f2bf(gelu(acc[t][r] + m.bias2[t]));
         }
     }
     __syncthreads();
     if (wave == 0) {
-        // layer 3 on one 16x16 tile: columns 0..2 are W3's rows, the rest zero
-        typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
         f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-        for (int k0 = 0; k0 < kHid; k0 += 32) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.H2 + r16 * kLdh + k0 + kq);
-            bf16x8 bv = __builtin_bit_cast(bf16x8, (u32x4){ 0u, 0u, 0u, 0u });
-            if (r16 < kOut) bv = *reinterpret_cast<const bf16x8*>(wb + kOW3 + (size_t)r16 * kHid + k0 + kq);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+        for (int i = 0; i < kHid / 32; ++i) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.H2 + r16 * kLdh + 32 * i + kq);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b3[i], acc, 0, 0, 0);
         }
         if (r16 < kOut) {
-            const float b = w[kOB3 + r16];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sm.Y[(lane >> 4) * 4 + r][r16] = acc[r] + b;
+            for (int r = 0; r < 4; ++r) sm.Y[(lane >> 4) * 4 + r][r16] = acc[r] + bias3;
         }
     }
     __syncthreads();
+}
+
+__device__ __forceinline__ void mlp_layers16(const uint16_t* __restrict__ wb, const float* __restrict__ w, MlpLds& sm)
+{
+    mlp_layers16_w(mlp_load_w16(wb, w), wb, w, sm);
 }
 
 // canonical neighbour table of the inner vertices: the distinct vertices of
