@@ -813,13 +813,17 @@ def north_star_mesh(name, args):
 CALIB_LIB = os.path.join(ROOT, "bench_calibrate.so")
 
 
-def calibrate(copy_mib: int = 256, copy_reps: int = 5, fma_iters: int = 20000, fma_reps: int = 5):
+PROBES = ("v_fma_f32", "v_mul_f32", "v_add_f32", "v_cmp_lt_f32+v_cndmask_b32", "tet selection step (mixed)")
+
+
+def calibrate(copy_mib: int = 256, copy_reps: int = 5, iters: int = 20000, reps: int = 5):
     """(1) a device-to-device copy of copy_mib MiB (copy_reps kernels), whose
     HBM traffic is exactly known, to scale the rocprofv3 FETCH_SIZE /
     WRITE_SIZE readings of the DMO kernels; (2) bench_calibrate.hip's
-    k_fma_probe, a known count of non-packed FP32 FMA instructions on every
-    lane, timed with HIP events: the VALU issue rate the search kernels are
-    compared with.  Prints one JSON line."""
+    k_valu_probe kernels, each a known count of one non-packed VALU
+    instruction class on every lane (FP32 FMA at 2, 4 and 8 waves per SIMD,
+    the others at 8), timed with HIP events: the VALU issue rates the search
+    kernels are compared with.  Prints one JSON line."""
     import ctypes
     import torch
     n = copy_mib * (1 << 20) // 4
@@ -832,24 +836,29 @@ def calibrate(copy_mib: int = 256, copy_reps: int = 5, fma_iters: int = 20000, f
     if not os.path.exists(CALIB_LIB):
         raise SystemExit(f"{CALIB_LIB} missing: run __graft_entry__.build()")
     lib = ctypes.CDLL(CALIB_LIB)
-    lib.calib_fma_run.restype = ctypes.c_int
-    lib.calib_fma_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    lib.calib_probe_run.restype = ctypes.c_int
+    lib.calib_probe_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_float)]
     lib.calib_fma_per_lane.restype = ctypes.c_longlong
     lib.calib_fma_per_lane.argtypes = [ctypes.c_int]
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    out = {"copy": {"mib": copy_mib, "kernels": copy_reps}, "cus": cus, "fma": []}
-    for waves_per_simd in (2, 4, 8):
+    out = {"copy": {"mib": copy_mib, "kernels": copy_reps}, "cus": cus, "probes": []}
+    runs = [(0, w) for w in (2, 4, 8)] + [(op, 8) for op in (1, 2, 3, 4)]
+    for op, waves_per_simd in runs:
         blocks = cus * waves_per_simd           # 256-thread blocks: 4 waves, one per SIMD
-        ms = (ctypes.c_float * fma_reps)()
-        rc = lib.calib_fma_run(blocks, fma_iters, fma_reps, ms)
+        ms = (ctypes.c_float * reps)()
+        rc = lib.calib_probe_run(op, blocks, iters, reps, ms)
         if rc != 0:
-            raise SystemExit(f"calib_fma_run failed: HIP error {rc}")
+            raise SystemExit(f"calib_probe_run({op}) failed: HIP error {rc}")
         t = float(np.median(np.array(list(ms), dtype=np.float64))) * 1e-3
-        fma = lib.calib_fma_per_lane(fma_iters) * blocks * 256
-        waves_instr = fma / 64
-        out["fma"].append({"waves_per_simd": waves_per_simd, "blocks": blocks, "median_ms": 1e3 * t,
-                           "tflops": 2 * fma / t / 1e12, "valu_instr_per_s": waves_instr / t,
-                           "valu_instr_per_cu_per_ns": waves_instr / t / cus / 1e9})
+        rec = {"probe": PROBES[op], "waves_per_simd": waves_per_simd, "blocks": blocks, "median_ms": 1e3 * t}
+        if op < 4:   # op 4's instruction count comes from the SQ_INSTS_VALU counter
+            per_lane = lib.calib_fma_per_lane(iters) * (2 if op == 3 else 1)   # instructions
+            wave_instr = per_lane * blocks * 256 / 64
+            rec.update({"valu_instr_per_s": wave_instr / t, "valu_instr_per_cu_per_ns": wave_instr / t / cus / 1e9})
+        if op == 0:
+            rec["tflops"] = 2 * per_lane * blocks * 256 / t / 1e12
+        out["probes"].append(rec)
     print(json.dumps(out), flush=True)
     return 0
 

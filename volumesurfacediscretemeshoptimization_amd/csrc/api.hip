@@ -9,6 +9,7 @@
 #include <new>
 #include <cstdlib>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "dmo_kernels.h"
@@ -110,8 +111,9 @@ struct vsdmo_opt {
     DevBuf<double> psum;
     DevBuf<double> qout;   // [0]: minimum (float in bytes 0-3, surface error bits in 4-7), [1]: FP64 sum
     PinnedQ hq;            // their host copy
-    std::vector<hipEvent_t> ev;   // pool of events; pairs [0, ev_used) bracket timed launches
+    std::vector<hipEvent_t> ev;   // pool of events, [0, ev_used) in use by the timed launches
     size_t ev_used = 0;
+    std::vector<std::pair<size_t, size_t>> ev_pairs;   // (begin, end) event of every timed launch
     // surface-vertex state (fixed_boundary == 0)
     bool surface = false;
     int nSectors = 0;
@@ -571,12 +573,12 @@ static void dispatch_search(vsdmo_opt* o, int kind, const SearchArgs& a, size_t 
         launch_search(kind, o->p.nq, w == 0 ? 1 : w, a, lds, s);
 }
 
-// HIP events around every search launch while timing is on.  The events are
-// pooled and reused: creating two per launch would cost host time inside the
-// iteration loop that is being timed.
+// HIP events on every search launch while timing is on: the kernels' own
+// begin and end (KernelEvents; every timed launch goes through
+// launch_kernel).  The events are pooled and reused: creating them per
+// launch would cost host time inside the iteration loop that is being timed.
 static void timed(vsdmo_opt* o, const std::function<void()>& launch)
 {
-    hipStream_t s = o->m->stream;
     if (o->timing) {
         const size_t k = o->ev_used;
         while (o->ev.size() < k + 2) {
@@ -584,9 +586,21 @@ static void timed(vsdmo_opt* o, const std::function<void()>& launch)
             VSDMO_HIP(hipEventCreate(&e));
             o->ev.push_back(e);
         }
-        VSDMO_HIP(hipEventRecord(o->ev[k], s));
-        launch();
-        VSDMO_HIP(hipEventRecord(o->ev[k + 1], s));
+        KernelEvents ke;
+        ke.start = o->ev[k];
+        ke.stop = o->ev[k + 1];
+        KernelEvents* outer = kernel_events();
+        kernel_events() = &ke;
+        try {
+            launch();
+        } catch (...) {
+            kernel_events() = outer;
+            throw;
+        }
+        kernel_events() = outer;
+        VSDMO_HIP(hipGetLastError());
+        if (!ke.started) throw StatusError{ VSDMO_NOT_SUPPORTED, "timing: a search launch recorded no kernel events" };
+        o->ev_pairs.push_back({ k, k + 1 });
         o->ev_used += 2;
     } else {
         launch();
@@ -598,11 +612,12 @@ static void collect_timing(vsdmo_opt* o)
 {
     if (o->ev_used == 0) return;
     VSDMO_HIP(hipStreamSynchronize(o->m->stream));
-    for (size_t i = 0; i + 1 < o->ev_used; i += 2) {
+    for (const auto& pr : o->ev_pairs) {
         float ms = 0.0f;
-        VSDMO_HIP(hipEventElapsedTime(&ms, o->ev[i], o->ev[i + 1]));
+        VSDMO_HIP(hipEventElapsedTime(&ms, o->ev[pr.first], o->ev[pr.second]));
         o->kernel_ms += ms;
     }
+    o->ev_pairs.clear();
     o->ev_used = 0;
 }
 

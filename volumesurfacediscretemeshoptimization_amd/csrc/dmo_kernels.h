@@ -1,11 +1,35 @@
 // dmo_kernels.h -- launch interface of the DMO kernels (internal).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
 
 namespace vsdmo {
+
+// Kernel-exact timing.  While a KernelEvents is installed in the calling
+// thread's slot (the optimiser's timed launches), launch_kernel records
+// `start` with the first kernel's dispatch and `stop` with every kernel's
+// completion (hipExtLaunchKernel's events: the kernels' own begin and end,
+// without the stream's dispatch latency around them); otherwise it is a
+// plain launch.
+struct KernelEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool started = false;
+};
+KernelEvents*& kernel_events();
+template <typename... Params, typename... Args>
+inline void launch_kernel(void (*k)(Params...), dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args)
+{
+    if (KernelEvents* e = kernel_events()) {
+        hipExtLaunchKernelGGL(k, grid, block, (uint32_t)lds, s, e->started ? nullptr : e->start, e->stop, 0u,
+                              args...);
+        e->started = true;
+    } else {
+        k<<<grid, block, lds, s>>>(args...);
+    }
+}
 
 struct SearchArgs {
     const float* __restrict__ x;

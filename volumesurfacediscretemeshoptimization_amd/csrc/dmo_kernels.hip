@@ -542,6 +542,12 @@ __global__ __launch_bounds__(64 * W) void k_search_flat(SearchArgs a)
     flat_vertex<NQ, W>(a, a.cStart + blockIdx.x, s_tr, rd);
 }
 
+KernelEvents*& kernel_events()
+{
+    static thread_local KernelEvents* slot = nullptr;
+    return slot;
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int n, int v)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1035,7 +1041,8 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchA
 // last iteration excepted); both are vertices of one colour class, so the
 // two kinds run side by side.
 template <int NQ, bool FUSED>
-__global__ __launch_bounds__(256) void k_search_tet_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbList)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_search_tet_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbList)
 {
     extern __shared__ __align__(16) unsigned char s_tcd[];
     __shared__ RedLds<4> rd;
@@ -1106,16 +1113,16 @@ template <int NQ, int W>
 static void list_nqw(int kind, const SearchArgs& a, const int* list, const int* cnt, int* fallbacks, int grid,
                      size_t lds, hipStream_t s)
 {
-    if (kind == VSDMO_TET) k_search_list_tet<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt, fallbacks);
-    else k_search_list_hex<NQ, W><<<grid, 64 * W, lds, s>>>(a, list, cnt, fallbacks);
+    if (kind == VSDMO_TET) launch_kernel(k_search_list_tet<NQ, W>, grid, 64 * W, lds, s, a, list, cnt, fallbacks);
+    else launch_kernel(k_search_list_hex<NQ, W>, grid, 64 * W, lds, s, a, list, cnt, fallbacks);
 }
 
 template <int NQ, int LV, bool FUSED>
 static void learned_nq_lv(int kind, const SearchArgs& a, const LearnArgs& la, int ringCap, int nb, size_t lds, hipStream_t s)
 {
     constexpr int T = FUSED ? 256 : 64;
-    if (kind == VSDMO_TET) k_search_tet_learned<NQ, LV, FUSED><<<nb, T, lds, s>>>(a, la, ringCap);
-    else k_search_hex_learned<NQ, LV, FUSED><<<nb, T, lds, s>>>(a, la, ringCap);
+    if (kind == VSDMO_TET) launch_kernel(k_search_tet_learned<NQ, LV, FUSED>, nb, T, lds, s, a, la, ringCap);
+    else launch_kernel(k_search_hex_learned<NQ, LV, FUSED>, nb, T, lds, s, a, la, ringCap);
 }
 
 static size_t learned_per(int kind) { return (kind == VSDMO_TET) ? sizeof(TetC) : sizeof(HexLds); }
@@ -1141,8 +1148,8 @@ template <int NQ, bool FUSED>
 static void learned_def_nq(int kind, const SearchArgs& a, const LearnArgs& la, int ringCap, int nbList, int nb,
                            size_t lds, hipStream_t s)
 {
-    if (kind == VSDMO_TET) k_search_tet_learned_def<NQ, FUSED><<<nbList + nb, 256, lds, s>>>(a, la, ringCap, nbList);
-    else k_search_hex_learned_def<NQ, FUSED><<<nbList + nb, 256, lds, s>>>(a, la, ringCap, nbList);
+    if (kind == VSDMO_TET) launch_kernel(k_search_tet_learned_def<NQ, FUSED>, nbList + nb, 256, lds, s, a, la, ringCap, nbList);
+    else launch_kernel(k_search_hex_learned_def<NQ, FUSED>, nbList + nb, 256, lds, s, a, la, ringCap, nbList);
 }
 
 void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet& net, int mode, int depthL,
@@ -1516,9 +1523,9 @@ __global__ void k_histogram(const float* __restrict__ q, int n, int ncols, int* 
 template <int NQ, int W>
 static void launch_nqw(int kind, const SearchArgs& a, int nBlocks, size_t lds, hipStream_t s)
 {
-    if (kind == VSDMO_TET) k_search_tet<NQ, W><<<nBlocks, 64 * W, lds, s>>>(a);
-    else if (kind == VSDMO_HEX) k_search_hex<NQ, W><<<nBlocks, 64 * W, lds, s>>>(a);
-    else k_search_flat<NQ, W><<<nBlocks, 64 * W, lds, s>>>(a);
+    if (kind == VSDMO_TET) launch_kernel(k_search_tet<NQ, W>, nBlocks, 64 * W, lds, s, a);
+    else if (kind == VSDMO_HEX) launch_kernel(k_search_hex<NQ, W>, nBlocks, 64 * W, lds, s, a);
+    else launch_kernel(k_search_flat<NQ, W>, nBlocks, 64 * W, lds, s, a);
 }
 template <int NQ>
 static void launch_nq(int kind, int w, const SearchArgs& a, int nBlocks, size_t lds, hipStream_t s)
@@ -1619,9 +1626,9 @@ static void split_nq(int kind, const SearchArgs& a, int nHead, int nTail, int ri
     const int grid = (nHead + 3) / 4 + nTail;
     if (grid <= 0) return;
     if (kind == VSDMO_TET)
-        k_search_tet_split<NQ, WT><<<grid, 256, sizeof(TetC) * 4 * (size_t)ringCap, s>>>(a, nHead, ringCap);
+        launch_kernel(k_search_tet_split<NQ, WT>, grid, 256, sizeof(TetC) * 4 * (size_t)ringCap, s, a, nHead, ringCap);
     else
-        k_search_hex_split<NQ, WT><<<grid, 256, sizeof(HexLds) * 4 * (size_t)ringCap, s>>>(a, nHead, ringCap);
+        launch_kernel(k_search_hex_split<NQ, WT>, grid, 256, sizeof(HexLds) * 4 * (size_t)ringCap, s, a, nHead, ringCap);
 }
 
 bool split_supported(int kind, int nq, int ringCap)

@@ -142,13 +142,13 @@ __device__ __forceinline__ void mlp_features16(const float* __restrict__ x, cons
     mlp_features16_from(g, vx, vy, vz, live, sm);
 }
 
-// The weight operands of layers 1-2 that wave w = threadIdx.x / 64 reads
-// (its column tiles 2w, 2w+1; lane l: row (2w+t)*16 + l%16, k = 8(l/16) + 32i
-// + 0..7) and their biases, loaded into registers; a caller issues this
-// first so that the loads overlap its other work.
+// The layer-1 weight operands that wave w = threadIdx.x / 64 reads (its
+// column tiles 2w, 2w+1; lane l: row (2w+t)*16 + l%16, k = 8(l/16) + 32i +
+// 0..7) and the wave's layer 1-2 biases, loaded into registers; a caller
+// issues this first so that the loads overlap its other work (layer 2's
+// operands are loaded while layer 1 runs, layer 3's while layer 2 runs).
 struct MlpW {
     bf16x8 b1[kIn / 32][2];
-    bf16x8 b2[kHid / 32][2];
     float bias1[2], bias2[2];
 };
 __device__ __forceinline__ MlpW mlp_load_w16(const uint16_t* __restrict__ wb, const float* __restrict__ w)
@@ -162,28 +162,35 @@ __device__ __forceinline__ MlpW mlp_load_w16(const uint16_t* __restrict__ wb, co
 #pragma unroll
         for (int i = 0; i < kIn / 32; ++i)
             m.b1[i][t] = *reinterpret_cast<const bf16x8*>(wb + kOW1 + (size_t)col * kIn + 32 * i + kq);
-#pragma unroll
-        for (int i = 0; i < kHid / 32; ++i)
-            m.b2[i][t] = *reinterpret_cast<const bf16x8*>(wb + kOW2 + (size_t)col * kHid + 32 * i + kq);
         m.bias1[t] = w[kOB1 + col];
         m.bias2[t] = w[kOB2 + col];
     }
     return m;
 }
 
-// the three layers of the 16 rows of sm.X into sm.Y, layers 1-2 with the
+// the three layers of the 16 rows of sm.X into sm.Y, layer 1 with the
 // operands of mlp_load_w16.  Layers 1-2: wave w computes columns [32w, 32w +
 // 32) -- two 16x16 tiles of mfma_f32_16x16x32_bf16 (A: 16 LDS rows, lane l
 // holds row l%16, k = 8(l/16) + 0..7; B: the wave's weight rows; D: column
 // l%16, rows 4(l/16) + 0..3), then bias + GELU to BF16 in LDS.  Layer 3 (128
-// -> 3): one tile on wave 0 with W3 zero padded to 16 columns, its operands
-// loaded while layer 2 runs.  All 256 threads call it; ends with a barrier.
+// -> 3): one tile on wave 0 with W3 zero padded to 16 columns.  Each layer's
+// weight operands are loaded while the layer before it runs.  All 256
+// threads call it; ends with a barrier.
 __device__ __forceinline__ void mlp_layers16_w(const MlpW& m, const uint16_t* __restrict__ wb,
                                                const float* __restrict__ w, MlpLds& sm)
 {
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    // layer 2's operands, loaded while layer 1 runs
+    bf16x8 b2[kHid / 32][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = (2 * wave + t) * 16 + r16;
+#pragma unroll
+        for (int i = 0; i < kHid / 32; ++i)
+            b2[i][t] = *reinterpret_cast<const bf16x8*>(wb + kOW2 + (size_t)col * kHid + 32 * i + kq);
+    }
     {
         f32x4 acc[2];
 #pragma unroll
@@ -221,7 +228,7 @@ __device__ __forceinline__ void mlp_layers16_w(const MlpW& m, const uint16_t* __
         for (int i = 0; i < kHid / 32; ++i) {
             const bf16x8 av = *reinterpret_cast<const bf16x8*>(sm.H1 + r16 * kLdh + 32 * i + kq);
 #pragma unroll
-            for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, m.b2[i][t], acc[t], 0, 0, 0);
+            for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b2[i][t], acc[t], 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < 2; ++t) {

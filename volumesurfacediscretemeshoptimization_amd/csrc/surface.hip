@@ -1084,14 +1084,14 @@ void surf_flat1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStre
 void surf_fan2d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s)
 {
     if (cEnd <= cStart) return;
-    if (a.fpe == 4) k_search_fan2d<true><<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
-    else k_search_fan2d<false><<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
+    if (a.fpe == 4) launch_kernel(k_search_fan2d<true>, cEnd - cStart, 64, 0, s, a, cStart, nq);
+    else launch_kernel(k_search_fan2d<false>, cEnd - cStart, 64, 0, s, a, cStart, nq);
 }
 void surf_fan1d_launch(const SurfArgs& a, int cStart, int cEnd, int nq, hipStream_t s)
 {
     if (cEnd <= cStart) return;
-    if (a.fpe == 4) k_search_fan1d<true><<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
-    else k_search_fan1d<false><<<cEnd - cStart, 64, 0, s>>>(a, cStart, nq);
+    if (a.fpe == 4) launch_kernel(k_search_fan1d<true>, cEnd - cStart, 64, 0, s, a, cStart, nq);
+    else launch_kernel(k_search_fan1d<false>, cEnd - cStart, 64, 0, s, a, cStart, nq);
 }
 void surf_normals_launch(const SurfArgs& a, hipStream_t s)
 {
