@@ -668,6 +668,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
                         if (learn_uses_net(o->learn_mode)) {
                             if (!o->learn_separate && learned_fused_ok(m->elem, m->maxRing)) {
                                 net.wb = o->mlp->wb.p; net.w = o->mlp->w.p; net.nbr = m->nbr.p;
+                                net.loc = m->ringLoc.p;
                             } else {
                                 launch_mlp_infer(o->mlp, m->x.p, m->y.p, m->z.p, m->nbr.p, a.cStart, a.cEnd, nullptr,
                                                  o->lY.p, o->lL.p, m->stream);

@@ -97,6 +97,10 @@ struct vsdmo_batch {
         float frac = 0.025f;
     } learn[2];
     DevBuf<int> nbr, lList, lCnt, lFallback;
+    // the meshes' ring-corner table positions in the batch's tet / hex ring
+    // index space (built with nbr); first ring element of every mesh in them
+    DevBuf<uint8_t> locTet, locHex;
+    std::vector<long long> ringBaseTet, ringBaseHex;
     long long learned = 0;
     // deferred fallback: the lists of (type t, round r) at dOff[t * rounds + r]
     // (parity 1 at + dTotal), their counts at t * rounds + r (parity 1 at
@@ -242,7 +246,10 @@ void launch_learned_items(vsdmo_batch* b, int t, const SearchArgs& a, int lo, in
     }
     LearnNet net;
     net.items = b->items.p;
-    if (learn_uses_net(L.mode)) { net.wb = L.mlp->wb.p; net.w = L.mlp->w.p; net.nbr = b->nbr.p; }
+    if (learn_uses_net(L.mode)) {
+        net.wb = L.mlp->wb.p; net.w = L.mlp->w.p; net.nbr = b->nbr.p;
+        net.loc = t == 0 ? b->locTet.p : b->locHex.p;
+    }
     const int nR = (int)b->grid.size();
     const int ring = t == 0 ? b->maxRingTet : b->maxRingHex;
     const int kind = t == 0 ? VSDMO_TET : VSDMO_HEX;
@@ -522,6 +529,8 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
             if (b->maxRingTet > 1024 || b->maxRingHex > 256 || b->maxHE > 1024)
                 throw StatusError{ VSDMO_LDS_OVERFLOW, "one-ring too large for the LDS staging" };
             b->totalV = b->vOff[nm];
+            b->ringBaseTet = ringBase;
+            b->ringBaseHex = hexBase;
             const size_t V = (size_t)b->totalV;
             b->x.alloc(V); b->y.alloc(V); b->z.alloc(V);
             b->x0.alloc(V); b->y0.alloc(V); b->z0.alloc(V);
@@ -800,6 +809,20 @@ vsdmo_status vsdmo_batch_set_proposal(vsdmo_batch* b, int32_t elem, vsdmo_mlp* m
                 }
                 b->nbr.alloc(tab.size());
                 b->nbr.upload(tab.data(), tab.size(), s);
+                const size_t nm = b->meshes.size();
+                b->locTet.alloc((size_t)std::max<long long>(1, kRingLocStrideTet * b->ringBaseTet[nm]));
+                b->locHex.alloc((size_t)std::max<long long>(1, kRingLocStrideHex * b->ringBaseHex[nm]));
+                for (size_t i = 0; i < nm; ++i) {
+                    vsdmo_mesh* m = b->meshes[i];
+                    const bool tet = m->elem == VSDMO_TET;
+                    if (!tet && m->elem != VSDMO_HEX) continue;
+                    const std::vector<long long>& base = tet ? b->ringBaseTet : b->ringBaseHex;
+                    const size_t stride = tet ? kRingLocStrideTet : kRingLocStrideHex;
+                    const size_t bytes = stride * (size_t)(base[i + 1] - base[i]);   // the mesh's ring elements
+                    if (bytes > 0)
+                        VSDMO_HIP(hipMemcpyAsync((tet ? b->locTet.p : b->locHex.p) + stride * base[i], m->ringLoc.p,
+                                                 bytes, hipMemcpyDeviceToDevice, s));
+                }
             }
             if (learn_uses_net(mode)) VSDMO_HIP(hipStreamSynchronize(mlp->stream));   // the weights are final
             VSDMO_HIP(hipStreamSynchronize(s));

@@ -186,6 +186,10 @@ struct LearnNet {
     // vertex ids of the class positions through a batch round's work items
     // (id in the low 30 bits); null: positions a.cStart + i of one mesh
     const int* items = nullptr;
+    // positions of the one-ring corners in the neighbour table (ringLoc of
+    // the mesh, in the index space of the SearchArgs ring), with which the
+    // window kernel takes corner positions from the network's features
+    const uint8_t* loc = nullptr;
 };
 // deferred fallback of one class pass: the previous iteration's list of the
 // class (listIn[0 .. *cntIn)), the per-vertex deferral epochs, the iteration

@@ -602,6 +602,8 @@ struct LearnArgs {
     // parity it & 1 and reads the previous iteration's from the other
     const int* itDev;
     int listStride, cntStride;
+    // FUSED: the ring corners' positions in the neighbour table (LearnNet.loc)
+    const uint8_t* loc;
 };
 
 // the concrete iteration and list buffers of a deferred pass
@@ -661,30 +663,6 @@ __device__ __forceinline__ void group_argmax(float& best, int& idx)
     }
 }
 
-// window half-extents and frame scale of vid's ring from the lanes of one
-// group (G lanes, stride G over the ring elements of stride st)
-template <int G>
-__device__ void group_extents(const SearchArgs& a, int r0, int n, int st, int s, float x0x, float x0y, float x0z,
-                              float& ex, float& ey, float& ez, float& mabs)
-{
-    const bool emin = a.extentMin != 0;
-    ex = emin ? FLT_MAX : -FLT_MAX; ey = ex; ez = ex; mabs = 0.0f;
-    for (int e = s; e < n; e += G) {
-        const int* h = a.ringVal + (size_t)(r0 + e) * st;
-        for (int t = 0; t < st; ++t) {
-            int v = h[t];
-            float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y), dz = fabsf(a.z[v] - x0z);
-            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
-            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
-            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
-            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
-        }
-    }
-    if (emin) { ex = group_min<G>(ex); ey = group_min<G>(ey); ez = group_min<G>(ez); }
-    else { ex = group_max<G>(ex); ey = group_max<G>(ey); ez = group_max<G>(ez); }
-    mabs = group_max<G>(mabs);
-}
-
 // proposal displacement of the group's vertex (class position ci, at (vx,
 // vy, vz)): c = v + l * y.  FUSED: the block's 16 vertices (class positions
 // ci0 + 0..15, group gb = threadIdx.x / 16) are one tile of the network, run
@@ -695,21 +673,32 @@ __device__ void group_extents(const SearchArgs& a, int r0, int n, int st, int s,
 template <bool FUSED>
 __device__ __forceinline__ void learned_offset(const SearchArgs& a, const LearnArgs& la, int ci0, int gb, int ci,
                                                float vx, float vy, float vz, const MlpGather& ng, const MlpW& nw,
-                                               unsigned char* lds, float& l, float& y0, float& y1, float& y2)
+                                               unsigned char* lds, float* relOut, float& l, float& y0, float& y1,
+                                               float& y2)
 {
     l = 0.0f; y0 = 0.0f; y1 = 0.0f; y2 = 0.0f;
     if constexpr (FUSED) {
         MlpLds& sm = *reinterpret_cast<MlpLds*>(lds);
         const bool live = ci0 + gb < a.cEnd - a.cStart;
-        mlp_features16_from(ng, vx, vy, vz, live, sm);
+        mlp_features16_from(ng, vx, vy, vz, live, sm, relOut);
         mlp_layers16_w(nw, la.wb, la.w, sm);
         l = sm.L[gb]; y0 = sm.Y[gb][0]; y1 = sm.Y[gb][1]; y2 = sm.Y[gb][2];
         __syncthreads();   // the ring staging reuses the LDS
     } else if (learn_uses_net(la.mode)) {
         l = la.L[ci]; y0 = la.Y[3 * ci]; y1 = la.Y[3 * ci + 1]; y2 = la.Y[3 * ci + 2];
     }
-    (void)ci; (void)vx; (void)vy; (void)vz; (void)ng; (void)nw;
+    (void)ci; (void)vx; (void)vy; (void)vz; (void)ng; (void)nw; (void)relOut;
 }
+
+// LDS of a FUSED window block: the network's LDS, reused for the 16 rings,
+// then (at learned_rel_offset) the 16 rows of table-neighbour positions
+// relative to their vertex, which the ring staging reads
+__host__ __device__ constexpr size_t learned_rel_offset(size_t per, int ringCap)
+{
+    return ((sizeof(MlpLds) > (size_t)kMlpRows * ringCap * per ? sizeof(MlpLds) : (size_t)kMlpRows * ringCap * per)
+            + 15) / 16 * 16;
+}
+constexpr size_t kLearnedRelBytes = sizeof(float) * kMlpRows * kNbr * 3;
 
 // append vid to the class's fallback list where need (one lane per group):
 // one atomic per wave for the wave's fallbacks (the list order does not
@@ -752,49 +741,88 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     // the network's neighbours, the lane's first PRE ring elements and the
     // network's weights are loaded into registers before the network runs
     // (their latencies overlap; no other vertex of this launch moves these
-    // positions: ring and table neighbours belong to other colour classes)
+    // positions: ring and table neighbours belong to other colour classes).
+    // FUSED: a ring element is its corners' positions in the neighbour table
+    // (one 4-byte word), whose positions relative to v the features leave in
+    // LDS; otherwise its corner positions.
     MlpGather ng{};
     if constexpr (FUSED) ng = mlp_gather16(a.x, a.y, a.z, la.nbr, vid, ci0 + gb < nc);
     constexpr int PRE = 2;
+    uint32_t lw[PRE];
     float rx[PRE][3], ry[PRE][3], rz[PRE][3];
 #pragma unroll
     for (int k = 0; k < PRE; ++k) {
         const int e = s + k * G;
+        lw[k] = 0xffffffffu;
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            rx[k][t] = f.x0x; ry[k][t] = f.x0y; rz[k][t] = f.x0z;
-            if (e < f.n) {
-                const int v = a.ringVal[(size_t)(r0 + e) * 3 + t];
-                rx[k][t] = a.x[v]; ry[k][t] = a.y[v]; rz[k][t] = a.z[v];
+        for (int t = 0; t < 3; ++t) { rx[k][t] = f.x0x; ry[k][t] = f.x0y; rz[k][t] = f.x0z; }
+        if (e < f.n) {
+            if constexpr (FUSED) {
+                lw[k] = *reinterpret_cast<const uint32_t*>(la.loc + (size_t)(r0 + e) * kRingLocStrideTet);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const int v = a.ringVal[(size_t)(r0 + e) * 3 + t];
+                    rx[k][t] = a.x[v]; ry[k][t] = a.y[v]; rz[k][t] = a.z[v];
+                }
             }
         }
     }
     MlpW nw{};
     if constexpr (FUSED) nw = mlp_load_w16(la.wb, la.w);
+    float* rel = FUSED ? reinterpret_cast<float*>(s_tcl + learned_rel_offset(sizeof(TetC), ringCap)) : nullptr;
     float pl, py0, py1, py2;
-    learned_offset<FUSED>(a, la, ci0, gb, ci, f.x0x, f.x0y, f.x0z, ng, nw, s_tcl, pl, py0, py1, py2);
+    learned_offset<FUSED>(a, la, ci0, gb, ci, f.x0x, f.x0y, f.x0z, ng, nw, s_tcl, rel, pl, py0, py1, py2);
+    const float* relRow = FUSED ? rel + (size_t)gb * kNbr * 3 : nullptr;
     TetC* tc = reinterpret_cast<TetC*>(s_tcl) + (size_t)gb * ringCap;
+    // corner t of ring element e relative to v: from the table (w: the
+    // element's packed table positions) or, for a corner the table does not
+    // hold (or without a table), from the positions
+    auto corner = [&](int e, int t, uint32_t w, float& dx, float& dy, float& dz) {
+        const uint32_t li = (w >> (8 * t)) & 0xffu;
+        if (FUSED && li != kRingLocNone) {
+            dx = relRow[3 * li]; dy = relRow[3 * li + 1]; dz = relRow[3 * li + 2];
+        } else {
+            const int v = a.ringVal[(size_t)(r0 + e) * 3 + t];
+            dx = a.x[v] - f.x0x; dy = a.y[v] - f.x0y; dz = a.z[v] - f.x0z;
+        }
+    };
+    auto word = [&](int e) -> uint32_t {
+        if constexpr (FUSED) return *reinterpret_cast<const uint32_t*>(la.loc + (size_t)(r0 + e) * kRingLocStrideTet);
+        else return 0xffffffffu;
+    };
     // window half-extents and frame scale (max / min: the order of the
     // elements does not matter), then the ring constants into LDS
     const bool emin = a.extentMin != 0;
     float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    auto extent = [&](float px, float py, float pz) {
-        const float dx = fabsf(px - f.x0x), dy = fabsf(py - f.x0y), dz = fabsf(pz - f.x0z);
+    auto extent = [&](float dxs, float dys, float dzs) {   // signed offsets from v
+        const float dx = fabsf(dxs), dy = fabsf(dys), dz = fabsf(dzs);
         ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
         ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
         ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
         mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
     };
 #pragma unroll
-    for (int k = 0; k < PRE; ++k)
-        if (s + k * G < f.n)
+    for (int k = 0; k < PRE; ++k) {
+        const int e = s + k * G;
+        if (e < f.n) {
 #pragma unroll
-            for (int t = 0; t < 3; ++t) extent(rx[k][t], ry[k][t], rz[k][t]);
-    for (int e = s + PRE * G; e < f.n; e += G)
-        for (int t = 0; t < 3; ++t) {
-            const int v = a.ringVal[(size_t)(r0 + e) * 3 + t];
-            extent(a.x[v], a.y[v], a.z[v]);
+            for (int t = 0; t < 3; ++t) {
+                float dx, dy, dz;
+                if constexpr (FUSED) corner(e, t, lw[k], dx, dy, dz);
+                else { dx = rx[k][t] - f.x0x; dy = ry[k][t] - f.x0y; dz = rz[k][t] - f.x0z; }
+                extent(dx, dy, dz);
+            }
         }
+    }
+    for (int e = s + PRE * G; e < f.n; e += G) {
+        const uint32_t w = word(e);
+        for (int t = 0; t < 3; ++t) {
+            float dx, dy, dz;
+            corner(e, t, w, dx, dy, dz);
+            extent(dx, dy, dz);
+        }
+    }
     if (emin) { ex = group_min<G>(ex); ey = group_min<G>(ey); ez = group_min<G>(ez); }
     else { ex = group_max<G>(ex); ey = group_max<G>(ey); ez = group_max<G>(ez); }
     mabs = group_max<G>(mabs);
@@ -808,18 +836,23 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
         if (e < f.n) {
             f3 p[3];
 #pragma unroll
-            for (int t = 0; t < 3; ++t)
-                p[t] = { (rx[k][t] - f.x0x) * f.sg, (ry[k][t] - f.x0y) * f.sg, (rz[k][t] - f.x0z) * f.sg };
+            for (int t = 0; t < 3; ++t) {
+                float dx, dy, dz;
+                if constexpr (FUSED) corner(e, t, lw[k], dx, dy, dz);
+                else { dx = rx[k][t] - f.x0x; dy = ry[k][t] - f.x0y; dz = rz[k][t] - f.x0z; }
+                p[t] = { dx * f.sg, dy * f.sg, dz * f.sg };
+            }
             tc[e] = tet_consts(p[0], p[1], p[2]);
         }
     }
     for (int e = s + PRE * G; e < f.n; e += G) {
-        const int* h = a.ringVal + (size_t)(r0 + e) * 3;
+        const uint32_t w = word(e);
         f3 p[3];
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-            int v = h[t];
-            p[t] = { (a.x[v] - f.x0x) * f.sg, (a.y[v] - f.x0y) * f.sg, (a.z[v] - f.x0z) * f.sg };
+            float dx, dy, dz;
+            corner(e, t, w, dx, dy, dz);
+            p[t] = { dx * f.sg, dy * f.sg, dz * f.sg };
         }
         tc[e] = tet_consts(p[0], p[1], p[2]);
     }
@@ -913,28 +946,69 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const int r0 = a.ringRow[vid];
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
-    // the network's inputs (table neighbours, weights) are loaded before it runs
+    // the network's inputs (table neighbours, weights) and the packed table
+    // positions of the lane's first ring element are loaded before it runs
     MlpGather ng{};
     MlpW nw{};
+    uint2 lw0 = make_uint2(0xffffffffu, 0xffffffffu);
     if constexpr (FUSED) {
         ng = mlp_gather16(a.x, a.y, a.z, la.nbr, vid, ci0 + gb < nc);
+        if (s < f.n) lw0 = *reinterpret_cast<const uint2*>(la.loc + (size_t)(r0 + s) * kRingLocStrideHex);
         nw = mlp_load_w16(la.wb, la.w);
     }
+    float* rel = FUSED ? reinterpret_cast<float*>(s_hxl + learned_rel_offset(sizeof(HexLds), ringCap)) : nullptr;
     float pl, py0, py1, py2;
-    learned_offset<FUSED>(a, la, ci0, gb, ci, f.x0x, f.x0y, f.x0z, ng, nw, s_hxl, pl, py0, py1, py2);
+    learned_offset<FUSED>(a, la, ci0, gb, ci, f.x0x, f.x0y, f.x0z, ng, nw, s_hxl, rel, pl, py0, py1, py2);
+    const float* relRow = FUSED ? rel + (size_t)gb * kNbr * 3 : nullptr;
     HexLds* hx = reinterpret_cast<HexLds*>(s_hxl) + (size_t)gb * ringCap;
-    float mabs;
-    group_extents<G>(a, r0, f.n, 7, s, f.x0x, f.x0y, f.x0z, f.ex, f.ey, f.ez, mabs);
-    f.sg = pow2_scale(mabs);
+    // corner t of ring element e relative to v: from the table (w: the
+    // element's packed table positions) or, for a corner the table does not
+    // hold (or without a table), from the positions
+    auto word = [&](int e) -> uint2 {
+        if constexpr (FUSED)
+            return e == s ? lw0 : *reinterpret_cast<const uint2*>(la.loc + (size_t)(r0 + e) * kRingLocStrideHex);
+        else return make_uint2(0xffffffffu, 0xffffffffu);
+    };
+    auto corner = [&](int e, int t, uint2 w, float& dx, float& dy, float& dz) {
+        const uint32_t li = ((t < 4 ? w.x : w.y) >> (8 * (t & 3))) & 0xffu;
+        if (FUSED && li != kRingLocNone) {
+            dx = relRow[3 * li]; dy = relRow[3 * li + 1]; dz = relRow[3 * li + 2];
+        } else {
+            const int v = a.ringVal[(size_t)(r0 + e) * 7 + t];
+            dx = a.x[v] - f.x0x; dy = a.y[v] - f.x0y; dz = a.z[v] - f.x0z;
+        }
+    };
+    // window half-extents (group_extents's rule) and frame scale
+    {
+        const bool emin = a.extentMin != 0;
+        float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
+        for (int e = s; e < f.n; e += G) {
+            const uint2 w = word(e);
+            for (int t = 0; t < 7; ++t) {
+                float dx, dy, dz;
+                corner(e, t, w, dx, dy, dz);
+                dx = fabsf(dx); dy = fabsf(dy); dz = fabsf(dz);
+                ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+                ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+                ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+                mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+            }
+        }
+        if (emin) { ex = group_min<G>(ex); ey = group_min<G>(ey); ez = group_min<G>(ez); }
+        else { ex = group_max<G>(ex); ey = group_max<G>(ey); ez = group_max<G>(ez); }
+        f.ex = ex; f.ey = ey; f.ez = ez;
+        f.sg = pow2_scale(group_max<G>(mabs));
+    }
     const float s2 = f.sg * f.sg, s3 = s2 * f.sg;
     f.epsD = kHexEps * s3; f.epsF = kHexEps * s2;
     for (int e = s; e < f.n; e += G) {
-        const int* h = a.ringVal + (size_t)(r0 + e) * 7;
+        const uint2 w = word(e);
         f3 r[8];
         r[0] = { 0.0f, 0.0f, 0.0f };
         for (int t = 0; t < 7; ++t) {
-            int v = h[t];
-            r[t + 1] = { (a.x[v] - f.x0x) * f.sg, (a.y[v] - f.x0y) * f.sg, (a.z[v] - f.x0z) * f.sg };
+            float dx, dy, dz;
+            corner(e, t, w, dx, dy, dz);
+            r[t + 1] = { dx * f.sg, dy * f.sg, dz * f.sg };
         }
         HexLds L;
         hex_consts(r, L.cd, L.cF, L.hp);
@@ -1130,17 +1204,31 @@ static size_t learned_per(int kind) { return (kind == VSDMO_TET) ? sizeof(TetC) 
 // largest ring the window kernel's LDS (kLV rings per block, 64 KB) holds
 int learned_ring_cap(int kind) { return (int)((64 * 1024) / (kLV * learned_per(kind))); }
 
-// the fused kernel holds 16 rings per block in the same 64 KB
-bool learned_fused_ok(int kind, int ringCap)
+// 16 rings per block in 64 KB of LDS
+static bool rings16_fit(int kind, int ringCap)
 {
     return (size_t)kMlpRows * std::max(ringCap, 1) * learned_per(kind) <= 64 * 1024;
 }
 
+// LDS of a FUSED window block: the network's (reused for the 16 rings) and
+// the table-neighbour positions
+static size_t fused_lds(int kind, int ringCap)
+{
+    return learned_rel_offset(learned_per(kind), std::max(ringCap, 1)) + kLearnedRelBytes;
+}
+
+// the fused kernel holds 16 rings and the table-neighbour positions per
+// block in the same 64 KB
+bool learned_fused_ok(int kind, int ringCap)
+{
+    return fused_lds(kind, ringCap) <= 64 * 1024;
+}
+
 // the deferred policy's full searches run 4 waves per vertex, and its blocks
-// hold 16 rings in LDS (learned_fused_ok's budget) with or without the network
+// hold 16 rings in LDS with or without the network
 bool learned_deferred_ok(int kind, int nq, int ringCap)
 {
-    return (nq == 4 || nq == 8) && learned_fused_ok(kind, ringCap);
+    return (nq == 4 || nq == 8) && rings16_fit(kind, ringCap);
 }
 
 
@@ -1160,9 +1248,10 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
     if (nc <= 0) return;
     ringCap = std::max(ringCap, 1);
     const size_t per = learned_per(kind);
-    const bool fused = learn_uses_net(mode) && net.wb && learned_fused_ok(kind, ringCap);
+    const bool fused = learn_uses_net(mode) && net.wb && net.loc && learned_fused_ok(kind, ringCap);
     LearnArgs la{ net.Y, net.L, mode, depthL, frac, fallbacks, list, cnt, net.wb, net.w, net.nbr,
                   nullptr, nullptr, nullptr, 0, net.items };
+    la.loc = net.loc;
     if (def) {
         // one launch: the previous iteration's fallbacks (list blocks sized for
         // an eighth of the class, they loop when more were deferred: a quarter
@@ -1172,8 +1261,7 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
         la.itDev = def->itDev; la.listStride = def->listStride; la.cntStride = def->cntStride;
         const int nbList = std::max(1, std::min(nc, std::max(64, nc / 8)));
         const int nb = (nc + kMlpRows - 1) / kMlpRows;
-        size_t lds = (size_t)kMlpRows * ringCap * per;
-        if (fused) lds = std::max(lds, sizeof(MlpLds));
+        const size_t lds = fused ? fused_lds(kind, ringCap) : (size_t)kMlpRows * ringCap * per;
         switch (nq) {   // the full searches use 4 waves: nq 4 and 8 (learned_deferred_ok)
         case 4: fused ? learned_def_nq<4, true>(kind, a, la, ringCap, nbList, nb, lds, s)
                       : learned_def_nq<4, false>(kind, a, la, ringCap, nbList, nb, lds, s); break;
@@ -1185,7 +1273,7 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
     }
     const int vpb = fused ? kMlpRows : kLV;
     const int nb = (nc + vpb - 1) / vpb;
-    const size_t lds = fused ? std::max(sizeof(MlpLds), (size_t)kMlpRows * ringCap * per) : (size_t)kLV * ringCap * per;
+    const size_t lds = fused ? fused_lds(kind, ringCap) : (size_t)kLV * ringCap * per;
 #define VSDMO_LEARN_CASE(Q)                                                                           \
     case Q:                                                                                          \
         if (fused) learned_nq_lv<Q, kLV, true>(kind, a, la, ringCap, nb, lds, s);                     \

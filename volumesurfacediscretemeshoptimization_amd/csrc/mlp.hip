@@ -312,22 +312,28 @@ void build_neighbour_table(vsdmo_mesh& m)
 {
     if (m.nbr.p) return;
     const int st = (m.elem == VSDMO_TET) ? 3 : 7;
+    const int ls = (m.elem == VSDMO_TET) ? kRingLocStrideTet : kRingLocStrideHex;
     std::vector<int> row = m.ringRow.to_host(m.stream);
     std::vector<int> val = m.ringVal.to_host(m.stream);
     std::vector<int> t((size_t)m.nV * kNbr, -1);
+    const size_t nE = (size_t)row[m.nV];
+    std::vector<uint8_t> loc(std::max<size_t>(nE * ls, 1), kRingLocNone);
     for (int v = m.nSurf; v < m.nV; ++v) {
         int* o = t.data() + (size_t)v * kNbr;
         int cnt = 0;
-        for (int e = row[v]; e < row[v + 1] && cnt < kNbr; ++e)
-            for (int k = 0; k < st && cnt < kNbr; ++k) {
-                int u = val[(size_t)e * st + k];
-                bool seen = false;
-                for (int q = 0; q < cnt; ++q) seen = seen || (o[q] == u);
-                if (!seen) o[cnt++] = u;
+        for (int e = row[v]; e < row[v + 1]; ++e)
+            for (int k = 0; k < st; ++k) {
+                const int u = val[(size_t)e * st + k];
+                int q = 0;
+                while (q < cnt && o[q] != u) ++q;
+                if (q == cnt && cnt < kNbr) o[cnt++] = u;   // first appearance
+                if (q < cnt) loc[(size_t)e * ls + k] = (uint8_t)q;
             }
     }
     m.nbr.alloc(t.size());
     m.nbr.upload(t.data(), t.size(), m.stream);
+    m.ringLoc.alloc(loc.size());
+    m.ringLoc.upload(loc.data(), loc.size(), m.stream);
     VSDMO_HIP(hipStreamSynchronize(m.stream));
 }
 

@@ -12,6 +12,11 @@
 namespace vsdmo {
 
 constexpr int kNbr = 32;            // neighbours per one-ring feature vector
+// ring-local corner positions (vsdmo_mesh::ringLoc): bytes per ring element
+// (3 tet / 7 hex corners, padded for aligned 4- / 8-byte loads) and the mark
+// of a corner outside the table
+constexpr int kRingLocStrideTet = 4, kRingLocStrideHex = 8;
+constexpr uint8_t kRingLocNone = 0xff;
 constexpr int kIn = 3 * kNbr;       // 96 input features
 constexpr int kHid = 128;
 constexpr int kOut = 3;
@@ -85,9 +90,11 @@ __device__ __forceinline__ MlpGather mlp_gather16(const float* __restrict__ x, c
 // features of row r = threadIdx.x >> 4 (vertex at (vx, vy, vz); live = it
 // exists) from the row's gathered neighbours: L = mean distance summed in
 // neighbour order (the order of k_features); X = bf16((n_k - v) / L), zero
-// padded.  All 256 threads call it; ends with a barrier.
+// padded.  relOut (optional, LDS): n_k - v of every table entry, kNbr x 3
+// floats per row (entries past the row's valid prefix hold no position).
+// All 256 threads call it; ends with a barrier.
 __device__ __forceinline__ void mlp_features16_from(const MlpGather& g, float vx, float vy, float vz, bool live,
-                                                    MlpLds& sm)
+                                                    MlpLds& sm, float* relOut = nullptr)
 {
     const int r = threadIdx.x >> 4, q = threadIdx.x & 15;
     float rel[2][3];
@@ -95,6 +102,10 @@ __device__ __forceinline__ void mlp_features16_from(const MlpGather& g, float vx
     for (int k = 0; k < 2; ++k) {
         rel[k][0] = g.x[k] - vx; rel[k][1] = g.y[k] - vy; rel[k][2] = g.z[k] - vz;
         sm.dist[r][2 * q + k] = sqrtf((rel[k][0] * rel[k][0] + rel[k][1] * rel[k][1]) + rel[k][2] * rel[k][2]);
+        if (relOut) {
+            float* o = relOut + ((size_t)r * kNbr + 2 * q + k) * 3;
+            o[0] = rel[k][0]; o[1] = rel[k][1]; o[2] = rel[k][2];
+        }
     }
     // the table's valid entries are a prefix: cnt = their number, counted by
     // a ballot over the row's 16 threads (lanes 16 (r % 4) + q of the wave)
@@ -261,7 +272,9 @@ __device__ __forceinline__ void mlp_layers16(const uint16_t* __restrict__ wb, co
 // canonical neighbour table of the inner vertices: the distinct vertices of
 // the vertex's volume one-ring in order of first appearance (ring elements in
 // their stored order), at most kNbr, -1 padded; rows of surface vertices are
-// all -1
+// all -1.  Also m.ringLoc: for every corner of a one-ring element, its
+// position in the vertex's table row (kRingLocNone when the row is full
+// without it, and for surface vertices)
 void build_neighbour_table(vsdmo_mesh& m);
 
 // features of vertices [lo, hi): X (bf16, 96 per vertex) = (n_i - v) / L for

@@ -99,6 +99,10 @@ struct vsdmo_mesh {
     vsdmo::DevBuf<int> heRow, heTarget, heOpp, heFace, heNext;
     vsdmo::DevBuf<uint8_t> isB1D, isFeature;
     vsdmo::DevBuf<int> nbr;          // canonical neighbour table (learned proposal), built on demand
+    // position of every one-ring corner in its vertex's neighbour table
+    // (kRingLocNone: not in it), kRingLocStride(elem) bytes per ring element;
+    // built with nbr
+    vsdmo::DevBuf<uint8_t> ringLoc;
     hipStream_t stream = nullptr;
     int n_opts = 0;   // live optimisers referencing this mesh
 };
