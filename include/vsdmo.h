@@ -283,7 +283,7 @@ vsdmo_status vsdmo_opt_record(vsdmo_opt* o, vsdmo_dataset* ds, int32_t n);
  * the network's proposal and, where its window finds nothing better, the
  * same window around the vertex itself before the full search.
  * One-rings up to 512 tets / 85 hexes (VSDMO_NOT_SUPPORTED above).  With
- * one-rings up to 128 tets / 21 hexes the window kernel runs the network
+ * one-rings up to 116 tets / 19 hexes the window kernel runs the network
  * itself; environment variable VSDMO_LEARN_SEPARATE_INFERENCE=1 (read here)
  * keeps it in a launch of its own (same results, for diagnosis). */
 vsdmo_status vsdmo_opt_set_proposal(vsdmo_opt* o, vsdmo_mlp* mlp, int32_t mode, int32_t depth, float window_frac);

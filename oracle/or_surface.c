@@ -234,7 +234,11 @@ or_surf* or_surface_init(const or_mesh* m)
             if (!own) s->err |= 2;
             if (lastWasFeature) { nneigh = 0; if (own) { L[nneigh++] = v; L[nneigh++] = m->heTarget[last]; } }
             if (m->isFeature[d]) {
+#if OR_MUTATION == 16                   /* 16 plants: a cut mapped to the sector it opens */
+                if (vindex < TABLE_SIZE) s->table[TABLE_SIZE * i + vindex] = sid + 1;
+#else
                 if (vindex < TABLE_SIZE) s->table[TABLE_SIZE * i + vindex] = sid;
+#endif
                 if (own && nneigh < MAX_NEIGH) L[nneigh++] = d;
                 if (own) secCnt[s->frow[i] + sid] = nneigh;
                 ++sid; nneigh = 0; lastWasFeature = 1;

@@ -427,3 +427,55 @@ def check_flat1d_search(o, n=20, seed=17, jitter=0.45, min_clean=0.7):
     assert clean >= min_clean * len(verts), (clean, len(verts))
     assert written > 0
     return {"vertices": len(verts), "clean": clean, "written": written}
+
+
+def sectors_from_spec(row, target, is_feature, is_b1d, n_free, n_surf, table_size=20, max_neigh=40):
+    """Feature sectors written from their specification (the reference's
+    SurfaceFeature.cu:134-223 as the build documents it): v's CCW fan
+    t_0 .. t_{n-1} is cut at every feature target t_k, k >= 1; sector s holds
+    v, its opening target (t_0, then the previous cut), the targets after it
+    and its closing cut; a closed fan adds the sector that wraps from the last
+    cut to t_0.  Table: fan position k -> the sector holding t_k after its
+    opening target (position 0 -> 0).  Returns per feature vertex (sector
+    point counts, table row)."""
+    out = []
+    for v in range(n_free, n_surf):
+        t = [int(x) for x in target[row[v]:row[v + 1]]]
+        n = len(t)
+        cuts = [k for k in range(1, n) if is_feature[t[k]]]
+        counts, opening = [], 0
+        for c in cuts:
+            counts.append(min(c - opening + 2, max_neigh))
+            opening = c
+        if not is_b1d[v]:
+            counts.append(min(n - opening + 2, max_neigh))   # v, t_open .. t_{n-1}, t_0
+        tab = [-1] * table_size
+        tab[0] = 0
+        sid = 0
+        for k in range(1, min(n, table_size)):
+            tab[k] = sid
+            if k in cuts:
+                sid += 1
+        out.append((counts, tab))
+    return out
+
+
+def check_sectors(o, kind="tet"):
+    """The oracle's feature sectors of the curved cube (counts per feature
+    vertex, table, the point count of every sector's fit) against
+    sectors_from_spec."""
+    M, _ = surface_mesh(o, kind, iterations=0)
+    row, he = M.halfedges()
+    b1d, isf = M.flags()
+    table, frow, fits = M.sectors()
+    spec = sectors_from_spec(row, he[:, 0], isf, b1d, M.m.nFree, M.m.nSurf)
+    assert len(spec) == M.m.nSurf - M.m.nFree > 0
+    nsec = 0
+    for i, (counts, tab) in enumerate(spec):
+        assert frow[i + 1] - frow[i] == len(counts), (i, frow[i + 1] - frow[i], counts)
+        assert list(table[i]) == tab, (i, list(table[i]), tab)
+        for q, npts in enumerate(counts):
+            fitted = npts + 1 if npts == 5 else npts      # a 5-point sector fits with one midpoint added
+            assert int(fits[frow[i] + q][15]) == fitted, (i, q, fits[frow[i] + q][15], fitted)
+        nsec += len(counts)
+    assert nsec > len(spec)          # vertices with more than one sector were checked

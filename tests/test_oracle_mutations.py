@@ -31,6 +31,7 @@ MUTATIONS = {
     13: ("learned window: halved per level instead of 2/3", lambda o: P.check_learned_window(o, "tet", 2, 0.5)),
     14: ("surface-mesh fan window from the nearest fan vertex", lambda o: P.check_fan_search(o, "tri", 2, "sphere")),
     15: ("flat boundary 1D pass: the open fan closed", P.check_flat1d_search),
+    16: ("feature sectors: a cut mapped to the sector it opens", P.check_sectors),
 }
 
 

@@ -111,3 +111,8 @@ def test_fan_search_matches_restatement(oracle, kind, dim, shape):
 def test_flat_boundary_1d_matches_restatement(oracle):
     """k_optimizeHierarchical1DFlat (dmo/DMOTriFlatImplCub.h:158-333)"""
     P.check_flat1d_search(oracle)
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_feature_sectors_follow_the_specification(oracle, kind):
+    P.check_sectors(oracle, kind)
