@@ -15,9 +15,10 @@ mesh are timed (wall clock around the blocking iterate call, which returns
 after the last kernel and the per-iteration quality check).
 
 Output: one JSON line with the metric, a roofline object for the dominant
-kernel (the inner search; algorithmic FLOP per vertex update x updates per
-launch / mean launch duration from HIP events on the launch stream) and a
-CPU baseline (the oracle, a C/OpenMP restatement, on the host cores).
+kernel (the inner search; canonical FP32 operations per vertex update x
+updates per launch / mean launch duration from HIP events on the kernel
+dispatches) and a CPU baseline (the oracle, a C/OpenMP restatement, on the
+host cores).
 """
 from __future__ import annotations
 
@@ -38,24 +39,39 @@ CPU_BUDGET_S = 10.0
 # first-level window), chosen by sweeps with the deferred fallback on the
 # seed-7 and seed-8 40^3 tet / 48^3 hex cubes (the benchmark meshes use seed 0)
 LEARN_WINDOW = {"tet": (1, 0.025), "hex": (1, 2.0)}
-# proposal mode per element type (3: the network's window, then the vertex's
-# own window before the full search; 1: the network's window alone): on the
-# 40^3 tet cube mode 3 reaches the plain quality 1.13x faster than mode 1, on
-# the 48^3 hex cube mode 1 is 1.17x faster than mode 3 (measured at K = 20)
+# proposal mode per element type (3: the network's window, then a quarter-size
+# window around the vertex before the full search; 1: the network's window
+# alone), chosen on the seed-7 / seed-8 cubes at K = 20: on the 40^3 tet cube
+# mode 3 reaches the plain quality 1.3x faster than mode 1, on the 48^3 hex
+# cube mode 1 is 1.2x faster than mode 3
 LEARN_MODE = {"tet": 3, "hex": 1}
 # the learned configuration as specified: one 4x4x4 grid on 1/4 of the plain
 # first-level window, reported beside the tuned window
 SPEC_WINDOW = (1, 0.25)
 PEAK_FP32_TFLOPS = 157.3
-# Algorithmic work of one vertex update: (nq^d + 1) ring evaluations per level
-# x levels + the final check = 1,540 (3D) / 196 (2D), times the one-ring size
-# (24 tets / 8 hexes / 6 triangles) times the per-element FLOP of the metric
-# (tet 73, hex 407, tri 23), cube roots counted separately.
-# HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE x 1.99981 copy-kernel
-# calibration + WRITE_SIZE x 1.0, mean over the dispatches of 3 iterations,
-# profiles/k_search_tet_tet40.md, profiles/k_search_hex_hex48.md,
-# profiles/k_search_fan2d_surf.md; the learned pass over the headline run,
-# profiles/learned_tet40.md and profiles/learned_hex48.md).
+# Work of one tet / hex vertex update in canonical FP32 operations: the
+# candidates of a full search (3 levels x 512) and the start position, times
+# the one-ring size (24 tets / 8 hexes on the structured cubes), times the
+# operations the canonical evaluation executes per (candidate, ring element)
+# with CPL candidates per lane (canon_flop); the cube root per candidate is
+# counted separately.  Flat triangles: the reference formula's 23 FLOP per
+# triangle evaluation, 196 ring evaluations x 6 triangles per update.
+# HBM traffic per launch: rocprofv3 --pmc FETCH_SIZE x 1.99981 (the
+# copy-kernel scale of bench.py --calibrate) + WRITE_SIZE x 1.0, mean over
+# the dispatches of the default bench line's runs
+# (profiles/north_star_kernels.md; the fan kernels in
+# profiles/k_search_fan2d_surf.md).
+def canon_flop(kind: str, cand_per_lane: int) -> float:
+    if kind == "tet":
+        return 2 + 3 + 7 + 9.0 / cand_per_lane
+    return 5 * (2 + 3 + 8 + 9.0 / cand_per_lane)
+
+
+RING = {"tet": 24, "hex": 8}            # elements per interior vertex of the structured cubes
+FULL_EVALS = 3 * 512 + 1                 # candidates of a full search + the start position
+MLP_FLOP = 2 * (96 * 128 + 128 * 128 + 128 * 3)   # per network row (BF16 MFMA)
+FLOP_TET = FULL_EVALS * RING["tet"] * canon_flop("tet", 8)
+FLOP_HEX = FULL_EVALS * RING["hex"] * canon_flop("hex", 8)
 # Surface-mesh searches (algorithmic FLOP per vertex update, valence n = 6
 # triangles / 4 quads): per candidate the projection onto the blended local
 # surfaces (57 + 69 n) plus the signed fan metric (59 n triangles, 104 n
@@ -64,22 +80,22 @@ PEAK_FP32_TFLOPS = 157.3
 SURF_FLOP = {"tri": 3 * 64 * (57 + 69 * 6 + 59 * 6), "quad": 3 * 64 * (57 + 69 * 4 + 104 * 4)}
 SURF_FLOP1D = {"tri": 3 * 8 * (149 + 59 * 6), "quad": 3 * 8 * (149 + 104 * 4)}
 WORKLOADS = {
-    "tet40": {"kind": "tet", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_search_tet<8>",
-              "traffic": 10_970_146, "learned_traffic": 8_042_968},
-    "hex48": {"kind": "hex", "flop": 5_014_240, "cbrt": 110_880, "kernel": "k_search_hex<8>",
-              "traffic": 17_570_841, "learned_traffic": 12_274_568},
+    "tet40": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet<8>",
+              "traffic": 9_158_412, "learned_traffic": 7_582_135},
+    "hex48": {"kind": "hex", "flop": FLOP_HEX, "cbrt": FULL_EVALS, "kernel": "k_search_hex<8>",
+              "traffic": 16_070_745, "learned_traffic": 10_497_171},
     # the 48^3 hex cube at jitter 0.15h: a valid mesh, so the surface and
     # feature passes run every iteration (the roofline is the inner kernel's)
-    "hex48_surface": {"kind": "hex", "flop": 5_014_240, "cbrt": 110_880, "kernel": "k_search_hex<8>",
+    "hex48_surface": {"kind": "hex", "flop": FLOP_HEX, "cbrt": FULL_EVALS, "kernel": "k_search_hex<8>",
                       "traffic": None},
     "tri64": {"kind": "tri", "flop": 27_048, "cbrt": 0, "kernel": "k_search_flat<8>", "traffic": None},
     "surf_tri": {"kind": "tri", "surface": True, "flop": SURF_FLOP["tri"], "flop1d": SURF_FLOP1D["tri"], "cbrt": 0,
                  "kernel": "k_search_fan2d<false>", "traffic": 18_791_143},
     "surf_quad": {"kind": "quad", "surface": True, "flop": SURF_FLOP["quad"], "flop1d": SURF_FLOP1D["quad"], "cbrt": 0,
                   "kernel": "k_search_fan2d<true>", "traffic": 20_562_741},
-    "batch256": {"kind": "batch", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_batch_round", "traffic": None},
+    "batch256": {"kind": "batch", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_batch_round", "traffic": None},
     # the default line: learned (+ batched) vs plain on tet40 and hex48
-    "north_star": {"kind": "tet", "flop": 2_698_080, "cbrt": 36_960, "kernel": "k_search_tet_learned_def<8>",
+    "north_star": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet_learned_def<8>",
                    "traffic": None},
 }
 # algorithmic FLOP of one flat-triangle vertex update (8x8 candidates: 196 ring
@@ -188,7 +204,7 @@ def bench_batch(args):
     qm1 = b.mean_quality()
     searches = st["searches"] - st0["searches"]
     gpu_s = (st["gpu_ms"] - st0["gpu_ms"]) / 1e3
-    # algorithmic work per iteration: tet searches x tet FLOP + tri searches x tri FLOP
+    # work per iteration: tet searches x canonical tet operations + tri searches x tri FLOP
     per_mesh = []
     for i, (k, _, _) in enumerate(mix):
         c = meshes[i].counts()
@@ -255,8 +271,10 @@ def bench_batch(args):
                    "learned_tets_vs_plain": learned,
                    "tune_log": args.tune_log},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_batch_round (all rounds + "
-                     "per-mesh quality, device time of the replayed iterations)"},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_batch_typed (all rounds + "
+                     "per-mesh quality, device time of the replayed iterations)",
+                     "flop_basis": "canonical FP32 operations per tet update (13.125 per candidate and ring "
+                                   "element), 23 FLOP per flat-triangle evaluation"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -520,10 +538,11 @@ def _learned(args):
     learned_updates = stl["searches_inner"] + stl["searches_surface"] + stl["searches_feature"]
     reached = q_l >= q_plain - 1e-3 and qm_l >= qm_plain - 1e-3
     value = plain_updates / dt_learned if reached else 0.0
-    ring = 24 if kind == "tet" else 8
-    per_elem = 73 if kind == "tet" else 407
-    evals = stl["searches_learned"] * (64 * args.learn_depth + 1) + stl["fallbacks"] * 1540
-    flops = evals * ring * per_elem
+    # canonical FP32 operations of the windows (4 candidates per lane) and of
+    # the full searches of the fallbacks (8 per lane)
+    ring = RING[kind]
+    flops = (stl["searches_learned"] * (64 * args.learn_depth + 1) * ring * canon_flop(kind, 4)
+             + stl["fallbacks"] * FULL_EVALS * ring * canon_flop(kind, 8))
     achieved = flops / (stk["kernel_ms"] / 1e3) / 1e12
     op.close()
     mp.close()
@@ -572,17 +591,6 @@ def _learned(args):
 # for L: 9) shared by its candidates per lane (8 in the full search, 4 in the
 # proposal window).  Hex, per p-dependent sub-tet (5 per hex): det (2), F
 # (3), selection with both minima (8), plus 9 shared terms.
-def canon_flop(kind: str, cand_per_lane: int) -> float:
-    if kind == "tet":
-        return 2 + 3 + 7 + 9.0 / cand_per_lane
-    return 5 * (2 + 3 + 8 + 9.0 / cand_per_lane)
-
-
-RING = {"tet": 24, "hex": 8}            # elements per interior vertex of the structured cubes
-FULL_EVALS = 3 * 512 + 1                 # candidates of a full search + the start position
-MLP_FLOP = 2 * (96 * 128 + 128 * 128 + 128 * 3)   # per network row (BF16 MFMA)
-
-
 def host_cpu():
     model = "unknown"
     try:
@@ -947,7 +955,7 @@ def main(argv=None):
     ap.add_argument("--learn-frac", type=float, default=None, help="default: 0.025 (tet), 2.0 (hex)")
     ap.add_argument("--learn-mode", type=int, choices=[1, 3], default=None,
                     help="proposal of the learned runs: 1 the network's window, 3 the network's window and, where it "
-                         "finds nothing better, the same window around the vertex before the full search "
+                         "finds nothing better, a quarter-size window around the vertex before the full search "
                          "(default: 3 tet, 1 hex)")
     ap.add_argument("--learn-fallback", choices=["deferred", "immediate"], default="deferred",
                     help="fallback policy of the measured learned run (the other one is reported beside it)")
@@ -1037,7 +1045,9 @@ def main(argv=None):
             "traffic": wl["traffic"],
             "kernel": wl["kernel"], "launches": launches,
             "mean_launch_us": 1e3 * st["kernel_ms"] / max(launches, 1),
-            "flop_per_update": wl["flop"], "cbrt_per_update": wl["cbrt"]}
+            "flop_per_update": wl["flop"], "cbrt_per_update": wl["cbrt"],
+            "flop_basis": ("canonical FP32 operations of a full search (candidates x ring elements x operations "
+                           "per pair)" if kind in ("tet", "hex") else "FLOP of the reference formulas")}
     cpu = None
     if not args.no_cpu:
         threads = host_cpu()["threads_all"]

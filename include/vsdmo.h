@@ -280,8 +280,9 @@ vsdmo_status vsdmo_dataset_destroy(vsdmo_dataset* d);
 vsdmo_status vsdmo_opt_record(vsdmo_opt* o, vsdmo_dataset* ds, int32_t n);
 /* mode 0: plain search; 1: network proposal; 2: the proposal is the vertex
  * itself (the same windowed search and fallback without the network); 3:
- * the network's proposal and, where its window finds nothing better, the
- * same window around the vertex itself before the full search.
+ * the network's proposal and, where its window finds nothing better, a
+ * window a quarter of its size around the vertex itself before the full
+ * search.
  * One-rings up to 512 tets / 85 hexes (VSDMO_NOT_SUPPORTED above).  With
  * one-rings up to 116 tets / 19 hexes the window kernel runs the network
  * itself; environment variable VSDMO_LEARN_SEPARATE_INFERENCE=1 (read here)

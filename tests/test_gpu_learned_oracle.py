@@ -4,8 +4,9 @@ The oracle runs the learned iteration class by class on the CPU: each
 class's one-ring features (tests/mlp_ref.py, bit-exact with the GPU's), the
 network's outputs for them (vsdmo_mlp_forward, the network the search
 kernels run inside them, bit for bit), the proposal c = v + L y, the window
-search (or_learned_window, pinned against tests/ref64.py; mode 3 searches the
-same window around v where the network's finds nothing better), and the plain full
+search (or_learned_window, pinned against tests/ref64.py; mode 3 searches a
+window a quarter that size around v where the network's finds nothing better),
+and the plain full
 search (or_dmo_vertex) for the vertices that fall back -- in the same class
 pass (immediate policy) or in the class's next pass (deferred).  The GPU
 optimiser and a one-mesh batch must end every iteration on the same
@@ -80,8 +81,8 @@ def oracle_learned(oracle, kind, xyz, cells, mlp, mode, depth_l, frac, policy, i
                 x0 = pos[v]
                 c3 = (x0 + np.float32(ls[i]) * Y[i]).astype(np.float32) if mode in (1, 3) else x0
                 st, p = om.learned_window(v, pos, c3, depth_l, frac)
-                if mode == 3 and st == -1:                 # the same window around the vertex itself
-                    st, p = om.learned_window(v, pos, x0, depth_l, frac)
+                if mode == 3 and st == -1:                 # a quarter-size window around the vertex itself
+                    st, p = om.learned_window(v, pos, x0, depth_l, float(np.float32(frac) * np.float32(0.25)))
                 if st == 1:
                     new[v] = p
                 elif st == -1:

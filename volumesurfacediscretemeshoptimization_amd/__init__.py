@@ -340,8 +340,9 @@ class _DMOBase:
                      fallback: str = None):
         """Learned proposal for the inner searches: mode 1 = network, 2 = the
         vertex itself (same windowed search and fallback, no network), 3 = the
-        network, then (where its window finds nothing better) the vertex
-        itself before the full search, 0 = off.
+        network, then (where its window finds nothing better) a window a
+        quarter of its size around the vertex itself before the full search,
+        0 = off.
         The window is `depth` levels of 4^3 candidates on `window_frac` times
         the plain first-level window (defaults per element type above).
         fallback: "immediate" (the full search of a vertex whose window finds

@@ -169,8 +169,8 @@ void launch_parity_zero(int* cnt, int n, const int* itDev, hipStream_t s);
 void launch_add_int(int* p, int v, hipStream_t s);
 // proposal modes of the learned search: 1 the network's proposal, 2 the
 // vertex itself (the same window without the network), 3 the network's
-// proposal and, where its window finds nothing better, the vertex itself
-// before the full search
+// proposal and, where its window finds nothing better, a window a quarter of
+// its size around the vertex itself before the full search
 __host__ __device__ constexpr bool learn_uses_net(int mode) { return mode == 1 || mode == 3; }
 // where the learned search finds its proposals (modes 1, 3): the network's
 // packed parameters (wb bf16, w fp32) and the neighbour table, with which the

@@ -635,6 +635,12 @@ __device__ __forceinline__ int learned_vertex(const SearchArgs& a, const LearnAr
 // vertices per wave in the proposal window (1, 2 and 4 measured on 40^3 tet
 // and 48^3 hex cubes: 4 is fastest for hexes and within noise of 2 for tets)
 constexpr int kLV = 4;
+// mode 3's second window, around the vertex itself, is this fraction of the
+// first (the vertices whose proposal window fails are close to a converged
+// position: on the seed-7 and seed-8 40^3 tet cubes 1/4 cut the fallbacks
+// ~9x against a second window of the first's size, 1/6.7 and 1/5 measured
+// the same, 1/2 and 1/1 slower; a power of two keeps the scaling exact)
+constexpr float kMode3Window = 0.25f;
 
 template <int G>
 __device__ __forceinline__ float group_max(float v)
@@ -870,8 +876,8 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
     // depthL levels of the 4x4x4 window around (cx, cy, cz); the branch
     // conditions are uniform over the vertex's lane group
-    auto levels = [&]() {
-        float ds = la.frac * a.gridScale;
+    auto levels = [&](float scale) {
+        float ds = (la.frac * scale) * a.gridScale;
         for (int d = 0; d < la.depthL; ++d) {
             const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
             const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
@@ -918,12 +924,12 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
             ds = ds * (2.0f / (float)(NL - 1));
         }
     };
-    levels();
-    // mode 3: where the network's window finds nothing better, the same
-    // window around the vertex itself before the full search
+    levels(1.0f);
+    // mode 3: where the network's window finds nothing better, a window a
+    // quarter of its size around the vertex itself before the full search
     if (la.mode == 3 && !moved && !tie) {
         cx = f.x0x; cy = f.x0y; cz = f.x0z;
-        levels();
+        levels(kMode3Window);
     }
     if (moved && own && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
     const bool need = !moved && !tie && own && s == 0;
@@ -1030,8 +1036,8 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     bool moved = false, tie = false;
     const float affL = 1.0f / (float)(NL - 1);
     const int lj = s % NL, lk = (s / NL) % NL, i0 = (s / (NL * NL)) * CPL;
-    auto levels = [&]() {
-        float ds = la.frac * a.gridScale;
+    auto levels = [&](float scale) {
+        float ds = (la.frac * scale) * a.gridScale;
         for (int d = 0; d < la.depthL; ++d) {
             const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
             const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
@@ -1082,10 +1088,10 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
             ds = ds * (2.0f / (float)(NL - 1));
         }
     };
-    levels();
+    levels(1.0f);
     if (la.mode == 3 && !moved && !tie && !hex_pinned(f)) {
         cx = f.x0x; cy = f.x0y; cz = f.x0z;
-        levels();
+        levels(kMode3Window);
     }
     if (moved && own && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz; }
     const bool need = !moved && !tie && !hex_pinned(f) && own && s == 0;
