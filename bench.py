@@ -81,9 +81,9 @@ SURF_FLOP = {"tri": 3 * 64 * (57 + 69 * 6 + 59 * 6), "quad": 3 * 64 * (57 + 69 *
 SURF_FLOP1D = {"tri": 3 * 8 * (149 + 59 * 6), "quad": 3 * 8 * (149 + 104 * 4)}
 WORKLOADS = {
     "tet40": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet<8>",
-              "traffic": 9_158_412, "learned_traffic": 7_582_135},
+              "traffic": 9_158_470, "learned_traffic": 6_796_992},
     "hex48": {"kind": "hex", "flop": FLOP_HEX, "cbrt": FULL_EVALS, "kernel": "k_search_hex<8>",
-              "traffic": 16_070_745, "learned_traffic": 10_497_171},
+              "traffic": 16_070_752, "learned_traffic": 10_493_544},
     # the 48^3 hex cube at jitter 0.15h: a valid mesh, so the surface and
     # feature passes run every iteration (the roofline is the inner kernel's)
     "hex48_surface": {"kind": "hex", "flop": FLOP_HEX, "cbrt": FULL_EVALS, "kernel": "k_search_hex<8>",
