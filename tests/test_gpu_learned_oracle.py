@@ -150,3 +150,92 @@ def test_learned_iterations_match_oracle(oracle, nets, kind, mode, depth, frac, 
     finally:
         b.close()
         mb.close()
+
+
+def _icosphere():
+    """42 unit vectors and 80 triangles: the icosahedron subdivided once"""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    pts = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    mid = {}
+
+    def m(a, b):
+        k = (min(a, b), max(a, b))
+        if k not in mid:
+            p = pts[a] + pts[b]
+            pts.append(p / np.linalg.norm(p))
+            mid[k] = len(pts) - 1
+        return mid[k]
+    tri = []
+    for a, b, c in f:
+        ab, bc, ca = m(a, b), m(b, c), m(c, a)
+        tri += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+    return np.array(pts), np.array(tri)
+
+
+def star_mesh(seed=3):
+    """A vertex with 42 distinct one-ring neighbours (more than the network's
+    32-entry neighbour table holds): the centre of two icosphere shells (r =
+    0.5, 1; the inner shell's vertices are interior too), tets from the centre
+    to the inner shell and prisms between the shells split into 3 tets by
+    vertex index; positions jittered, tets oriented positive."""
+    s, tri = _icosphere()
+    n = len(s)
+    rng = np.random.default_rng(seed)
+    xyz = np.concatenate([np.zeros((1, 3)), 0.5 * s, s])
+    xyz[: 1 + n] += rng.uniform(-0.04, 0.04, size=(1 + n, 3))
+    cells = []
+    for a, b, c in tri:
+        cells.append((0, 1 + a, 1 + b, 1 + c))
+        i, j, k = sorted((a, b, c))
+        lo = lambda q: 1 + q
+        hi = lambda q: 1 + n + q
+        cells += [(lo(i), lo(j), lo(k), hi(k)), (lo(i), lo(j), hi(j), hi(k)), (lo(i), hi(i), hi(j), hi(k))]
+    cells = np.array(cells, np.int32)
+    p = xyz[cells]
+    vol = np.einsum("ij,ij->i", np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), p[:, 3] - p[:, 0])
+    cells[vol < 0, 1], cells[vol < 0, 2] = cells[vol < 0, 2].copy(), cells[vol < 0, 1].copy()
+    return xyz.astype(np.float32), cells
+
+
+@pytest.mark.parametrize("mode", [1, 3])
+def test_learned_truncated_neighbour_table_matches_oracle(oracle, nets, mode):
+    """The window kernel stages a ring corner from the network's neighbour
+    table when the table holds it, and gathers it otherwise: the star mesh's
+    centre has 42 distinct neighbours, 10 of which are outside its table."""
+    xyz, cells = star_mesh()
+    centre_nbrs = set(cells[(cells == 0).any(axis=1)].ravel().tolist()) - {0}
+    assert len(centre_nbrs) > 32
+    ref, ref_fb = oracle_learned(oracle, "tet", xyz, cells, nets["tet"], mode, 1, 0.25, "deferred", ITERS)
+    m = DMOMeshTetFactory.create(xyz, cells)
+    o = DMOTetClass(m, n_iter=ITERS, fixed_boundary=True)
+    try:
+        p0 = m.positions(original_order=False)
+        o.set_proposal(nets["tet"], mode=mode, depth=1, window_frac=0.25, fallback="deferred")
+        for it in range(ITERS):
+            o.iterate(1)
+            gp = m.positions(original_order=False)
+            bad = np.flatnonzero((gp != ref[it]).any(axis=1))
+            assert bad.size == 0, f"iteration {it}: {bad.size} vertices differ, first {bad[:5]}"
+        assert o.stats()["fallbacks"] == ref_fb
+        assert (m.positions(original_order=False) != p0).any(axis=1).sum() > 0   # the search moved vertices
+    finally:
+        o.close()
+        m.close()
+    # the same mesh as a one-mesh batch (its ring-corner table positions copied into the batch)
+    mb = DMOMeshTetFactory.create(xyz, cells)
+    b = DMOBatch([mb], use_graph=True)
+    try:
+        b.set_fallback("deferred")
+        b.set_proposal(nets["tet"], mode=mode, depth=1, window_frac=0.25, elem=_lib.VSDMO_TET)
+        for it in range(ITERS):
+            b.iterate(1)
+            assert np.array_equal(b.positions(0), ref[it]), f"batch, iteration {it}"
+        assert b.learned_stats()["fallbacks"] == ref_fb
+    finally:
+        b.close()
+        mb.close()
