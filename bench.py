@@ -78,6 +78,15 @@ FLOP_HEX = FULL_EVALS * RING["hex"] * canon_flop("hex", 8)
 # quads); 3 levels x 64 candidates (2D) or 3 x 8 (1D, projection 45 + 52 * 2
 # curve neighbours).
 SURF_FLOP = {"tri": 3 * 64 * (57 + 69 * 6 + 59 * 6), "quad": 3 * 64 * (57 + 69 * 4 + 104 * 4)}
+# executed VALU issue rate of the search kernels (SQ_INSTS_VALU per launch /
+# the kernel trace's mean duration / 256 CUs) over the highest rate measured
+# for the search's own instruction mix (bench.py --calibrate's selection-step
+# probe, 3.81 wave-instructions per CU per ns), from the counter passes of
+# profiles/north_star_kernels.md
+VALU_ISSUE_FRAC = {"k_search_tet": 0.81, "k_search_hex": 0.93, "k_search_tet_learned_def": 0.39,
+                   "k_search_hex_learned_def": 0.47}
+VALU_ISSUE_BASIS = ("SQ_INSTS_VALU / kernel-trace duration / CUs over the calibrated mixed-instruction issue "
+                    "ceiling (profiles/north_star_kernels.md, profiles/valu_issue_calibration.md)")
 SURF_FLOP1D = {"tri": 3 * 8 * (149 + 59 * 6), "quad": 3 * 8 * (149 + 104 * 4)}
 WORKLOADS = {
     "tet40": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet<8>",
@@ -891,7 +900,10 @@ def bench_north_star(args):
             "mean_launch_us": tet["_learned_kernel_us"], "flop_per_launch": tet["_learned_flop_launch"],
             "flop_basis": "FP32 operations of the canonical evaluation per (candidate, ring element): "
                           f"{canon_flop('tet', 4):.2f} in the windows, {canon_flop('tet', 8):.3f} in the full "
-                          "searches; the network's BF16 MFMA work is reported as mfma_tflops"}
+                          "searches; the network's BF16 MFMA work is reported as mfma_tflops",
+            # the executed view from the kernel's counter pass (not measured by this run)
+            "valu_issue_frac_profiled": VALU_ISSUE_FRAC["k_search_tet_learned_def"],
+            "valu_issue_basis": VALU_ISSUE_BASIS}
     cpu = None if args.no_cpu else cpu_baseline_tet(tet["_xyz"], tet["_cells"], "tet40")
     for r in (tet, hexr):
         if r:
@@ -1048,6 +1060,9 @@ def main(argv=None):
             "flop_per_update": wl["flop"], "cbrt_per_update": wl["cbrt"],
             "flop_basis": ("canonical FP32 operations of a full search (candidates x ring elements x operations "
                            "per pair)" if kind in ("tet", "hex") else "FLOP of the reference formulas")}
+    if kind in ("tet", "hex") and not surface_mesh:
+        roof["valu_issue_frac_profiled"] = VALU_ISSUE_FRAC[f"k_search_{kind}"]
+        roof["valu_issue_basis"] = VALU_ISSUE_BASIS
     cpu = None
     if not args.no_cpu:
         threads = host_cpu()["threads_all"]

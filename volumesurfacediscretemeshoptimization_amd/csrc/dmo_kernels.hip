@@ -1115,11 +1115,14 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchA
     hex_window<NQ, LV, FUSED ? 4 : 1, FUSED>(a, la, ringCap, blockIdx.x, s_hxl);
 }
 
-// deferred fallback: blocks [0, nbList) run the full search (4 waves per
-// vertex) of the vertices the previous iteration deferred, blocks from
-// nbList on the windows of 16 class vertices each (the vertices deferred
-// last iteration excepted); both are vertices of one colour class, so the
-// two kinds run side by side.
+// deferred fallback: the first gridDim.x - nbList blocks take the windows of
+// 16 class vertices each (the vertices deferred last iteration excepted),
+// the last nbList blocks the full searches (4 waves per vertex) of the
+// vertices the previous iteration deferred; both are vertices of one colour
+// class, so the two kinds run side by side.  The windows come first so that
+// the dispatcher starts them before the list blocks, most of which find no
+// vertex (list blocks first: 0.27 against 0.226 ms per learned 40^3 tet
+// iteration).
 template <int NQ, bool FUSED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_search_tet_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbList)
@@ -1127,16 +1130,18 @@ void k_search_tet_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbLi
     extern __shared__ __align__(16) unsigned char s_tcd[];
     __shared__ RedLds<4> rd;
     const LearnArgs la = learn_resolve(la0);
-    if ((int)blockIdx.x < nbList) {
+    const int nbWin = (int)gridDim.x - nbList;
+    if ((int)blockIdx.x >= nbWin) {
+        const int lb = (int)blockIdx.x - nbWin;
         const int n = min(*la.cntIn, a.cEnd - a.cStart);
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
-        for (int i = blockIdx.x; i < n; i += nbList) {
+        if (lb == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
+        for (int i = lb; i < n; i += nbList) {
             tet_vertex<NQ, 4>(a, la.listIn[i], a.extentMin != 0, reinterpret_cast<TetC*>(s_tcd), rd);
             __syncthreads();   // the next vertex restages the LDS ring
         }
         return;
     }
-    tet_window<NQ, 4, 4, FUSED>(a, la, ringCap, (int)blockIdx.x - nbList, s_tcd);
+    tet_window<NQ, 4, 4, FUSED>(a, la, ringCap, (int)blockIdx.x, s_tcd);
 }
 // 4 waves per SIMD (<= 128 registers): a 48^3 class (~13k vertices, 3.3k
 // waves) then fits the chip's wave slots in one round
@@ -1147,16 +1152,18 @@ void k_search_hex_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbLi
     extern __shared__ __align__(16) unsigned char s_hxd[];
     __shared__ RedLds<4> rd;
     const LearnArgs la = learn_resolve(la0);
-    if ((int)blockIdx.x < nbList) {
+    const int nbWin = (int)gridDim.x - nbList;
+    if ((int)blockIdx.x >= nbWin) {
+        const int lb = (int)blockIdx.x - nbWin;
         const int n = min(*la.cntIn, a.cEnd - a.cStart);
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
-        for (int i = blockIdx.x; i < n; i += nbList) {
+        if (lb == 0 && threadIdx.x == 0) atomicAdd(la.fallbacks, n);
+        for (int i = lb; i < n; i += nbList) {
             hex_vertex<NQ, 4>(a, la.listIn[i], a.extentMin != 0, reinterpret_cast<HexLds*>(s_hxd), rd);
             __syncthreads();
         }
         return;
     }
-    hex_window<NQ, 4, 4, FUSED>(a, la, ringCap, (int)blockIdx.x - nbList, s_hxd);
+    hex_window<NQ, 4, 4, FUSED>(a, la, ringCap, (int)blockIdx.x, s_hxd);
 }
 
 // plain search of the vertices list[0 .. *cnt) (the fallbacks of a class),
@@ -1259,10 +1266,10 @@ void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet
                   nullptr, nullptr, nullptr, 0, net.items };
     la.loc = net.loc;
     if (def) {
-        // one launch: the previous iteration's fallbacks (list blocks sized for
-        // an eighth of the class, they loop when more were deferred: a quarter
-        // and a twelfth measured slower on the 40^3 tet cube) beside this
-        // iteration's windows, 16 vertices per block
+        // one launch: this iteration's windows, 16 vertices per block, beside
+        // the previous iteration's fallbacks (list blocks sized for an eighth
+        // of the class, they loop when more were deferred: a sixteenth
+        // measured the same, a thirty-second slower on the 40^3 tet cube)
         la.listIn = def->listIn; la.cntIn = def->cntIn; la.epoch = def->epoch; la.it = def->it;
         la.itDev = def->itDev; la.listStride = def->listStride; la.cntStride = def->cntStride;
         const int nbList = std::max(1, std::min(nc, std::max(64, nc / 8)));
