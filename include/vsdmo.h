@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define VSDMO_ABI_VERSION 1
+#define VSDMO_ABI_VERSION 2
 
 typedef enum vsdmo_status {
     VSDMO_OK = 0,
@@ -51,6 +51,17 @@ typedef enum vsdmo_surface_path {
     VSDMO_SURFACE_PATH_SURFACE = 1,  /* DMOTriClass / DMOQuadClass: vertices move on their local surfaces */
     VSDMO_SURFACE_PATH_FLAT = 2      /* DMOTriFlatClass: 2D search in the plane (flat triangle meshes only) */
 } vsdmo_surface_path;
+
+/* the hex element metric.  The reference's shapeMetricHex takes the minimum
+   over 9 sub-tets: the 8 corner tets and the principal-axes tet
+   (dmo/DMOQualityHex.h:54-71, dmo/DMOQualityUtil.h:307-349).  CORNER8 takes
+   the minimum over the 8 corner tets alone (the "min over 8 corner
+   Jacobians" objective), with the same scaling and the same zero for a
+   corner determinant or Frobenius term below the threshold. */
+typedef enum vsdmo_hex_metric {
+    VSDMO_HEX_SHAPE9 = 0,   /* reference default */
+    VSDMO_HEX_CORNER8 = 1
+} vsdmo_hex_metric;
 
 typedef enum vsdmo_extent {
     VSDMO_EXTENT_AUTO = -1,  /* reference choice: max for tet/tri, min for hex */
@@ -85,6 +96,13 @@ typedef struct vsdmo_params {
                                   on it */
     int32_t surface_path;      /* vsdmo_surface_path, triangle and quad meshes only; on the
                                   surface path fixed_boundary = 1 keeps the feature vertices */
+    int32_t hex_metric;        /* vsdmo_hex_metric: the objective of the hex searches and the
+                                  optimiser's hex qualities (min, mean, histogram, guard) */
+    int32_t surface_ring;      /* support of the surface vertices' quadric fits: 1 = {v} and its
+                                  one-ring (reference dmo/surfaceP/Surface.cu:11-28), 2 = {v},
+                                  its one-ring and its 2-ring: the one-ring's CCW targets not yet
+                                  taken, neighbour by neighbour in CCW order (at most 64 points,
+                                  else VSDMO_NOT_SUPPORTED); feature sectors keep one-ring fits */
 } vsdmo_params;
 
 typedef struct vsdmo_counts {
@@ -106,7 +124,8 @@ typedef struct vsdmo_stats {
 /* defaults: greedy colouring, 20 degrees / nq 8, depth 3, reference shrink,
    auto extent, surface and boundary vertices optimised as in the reference
    (fixed_boundary = 0; pass 1 to keep them fixed), guard on, one wave per
-   vertex, automatic surface path */
+   vertex, automatic surface path, the reference hex metric (9 sub-tets) and
+   one-ring quadric fits */
 void vsdmo_mesh_params_default(vsdmo_mesh_params* p);
 void vsdmo_params_default(vsdmo_params* p);
 
@@ -322,6 +341,7 @@ typedef struct vsdmo_batch_params {
     float guard_tolerance;
     int32_t persistent;        /* 1: persistent blocks take a round's items from a queue; 0: one block per
                                   item; does not change results */
+    int32_t hex_metric;        /* vsdmo_hex_metric of the hex meshes' searches and qualities */
 } vsdmo_batch_params;
 
 typedef struct vsdmo_batch_stats {

@@ -69,6 +69,11 @@ float or_ring_score_tet(int n, const float* ring, const float* x0p, float sigma,
 
 float or_ring_score_hex(int n, const float* ring, const float* x0p, float sigma, const float* p)
 {
+    return or_ring_score_hex_m(n, ring, x0p, sigma, p, OR_HEX_SHAPE9);
+}
+
+float or_ring_score_hex_m(int n, const float* ring, const float* x0p, float sigma, const float* p, int metric)
+{
     v3 x0 = ld3(x0p);
     float cd[4 * 64], cF[4 * 64];
     hexp hp[5 * 64];
@@ -82,7 +87,7 @@ float or_ring_score_hex(int n, const float* ring, const float* x0p, float sigma,
     float s2 = sigma * sigma, s3 = s2 * sigma;
     hexacc a = hex_const_acc(n, cd, cF);
     v3 pl = loc3(p, x0, sigma);
-    return hex_score(n, a, hp, pl.x, pl.y, pl.z, HEX_EPS * s3, HEX_EPS * s2);
+    return hex_score(n, a, hp, hex_nsub(metric), pl.x, pl.y, pl.z, HEX_EPS * s3, HEX_EPS * s2);
 }
 
 float or_ring_score_tri2d(int n, const float* ring, const float* x0p, float sigma, const float* p)
@@ -111,7 +116,11 @@ static float ext_scale(const float* pts, int n, const float* x0, int dims)
 }
 
 float or_q_tet(const float* p) { return or_ring_score_tet(1, p + 3, p, ext_scale(p + 3, 3, p, 3), p); }
-float or_q_hex(const float* p) { return or_ring_score_hex(1, p + 3, p, ext_scale(p + 3, 7, p, 3), p); }
+float or_q_hex(const float* p) { return or_q_hex_m(p, OR_HEX_SHAPE9); }
+float or_q_hex_m(const float* p, int metric)
+{
+    return or_ring_score_hex_m(1, p + 3, p, ext_scale(p + 3, 7, p, 3), p, metric);
+}
 float or_q_tri2d(const float* p)
 {
     float ring[6] = { p[3], p[4], p[5], p[6], p[7], p[8] };
@@ -141,8 +150,10 @@ double or_q_tet64(const float* p)
     return 12.0 * cbrt(9.0) * mu / l;
 }
 
-/* reference DMOQualityHex.h:54-71 with DMOQualityUtil.h:307-349 in double */
-double or_q_hex64(const float* p)
+/* reference DMOQualityHex.h:54-71 with DMOQualityUtil.h:307-349 in double
+   (CORNER8: the first 8 sub-tets, the corner tets, alone) */
+double or_q_hex64(const float* p) { return or_q_hex64_m(p, OR_HEX_SHAPE9); }
+double or_q_hex64_m(const float* p, int metric)
 {
     d3 q[8];
     for (int i = 0; i < 4; ++i) { q[i] = dld(p + 3 * (i + 4)); q[i + 4] = dld(p + 3 * i); }
@@ -159,7 +170,8 @@ double or_q_hex64(const float* p)
         { dneg(L[3]), dneg(L[2]), L[7] }, { L[11], L[8], dneg(L[4]) }, { dneg(L[8]), L[9], dneg(L[5]) },
         { dneg(L[9]), L[10], dneg(L[6]) }, { dneg(L[10]), dneg(L[11]), dneg(L[7]) }, { X[0], X[1], X[2] } };
     double ret = 1e30;
-    for (int i = 0; i < 9; ++i) {
+    const int nt = metric == OR_HEX_CORNER8 ? 8 : 9;
+    for (int i = 0; i < nt; ++i) {
         double sj = ddot(T[i][0], dcross(T[i][1], T[i][2]));
         double Aj = ddot(T[i][0], T[i][0]) + ddot(T[i][1], T[i][1]) + ddot(T[i][2], T[i][2]);
         if (sj < 1.19209e-7 || Aj < 1.19209e-7) return 0.0;

@@ -53,6 +53,7 @@ static inline float gridc(float aff, int i, int nq, float gmin, float gmax)
    canonical evaluation (or_internal.h). */
 typedef struct {
     int tet, n;
+    int nsub;        /* hex: p-dependent sub-tets scored per element (5 or 4) */
     v3 x0;
     float ext[3], sigma, is3, epsD, epsF;
     tetc tcs[MAXR];
@@ -63,10 +64,14 @@ typedef struct {
 
 /* ringPts: n elements of 3 (tet halfface) or 7 (halfhex) points, world
    coordinates; emin: per-axis minimum extent instead of the maximum */
-static void ring_setup(ringctx* r, int tet, int n, const float* ringPts, const float* x0p, int emin)
+static void ring_setup(ringctx* r, int tet, int n, const float* ringPts, const float* x0p, int emin, int hexMetric)
 {
     const int st = tet ? 3 : 7;
     r->tet = tet; r->n = n;
+    r->nsub = hex_nsub(hexMetric);
+#if OR_MUTATION == 18
+    r->nsub = 5;                       /* planted: the inner search scores the nine sub-tets whatever the metric */
+#endif
     r->x0 = ld3(x0p);
     for (int a = 0; a < 3; ++a) r->ext[a] = emin ? FLT_MAX : -FLT_MAX;
     float mabs = 0.0f;
@@ -106,7 +111,7 @@ static inline float ring_score_at(const ringctx* r, float x, float y, float z)
 {
     float px = (x - r->x0.x) * r->sigma, py = (y - r->x0.y) * r->sigma, pz = (z - r->x0.z) * r->sigma;
     return r->tet ? tet_score(r->n, r->tcs, px, py, pz, r->is3)
-                  : hex_score(r->n, r->hacc, r->hp, px, py, pz, r->epsD, r->epsF);
+                  : hex_score(r->n, r->hacc, r->hp, r->nsub, px, py, pz, r->epsD, r->epsF);
 }
 
 /* Search of one vertex on an explicit ring: ringPts holds n elements of st
@@ -123,7 +128,7 @@ static int search_volume_ring(int tet, int n, const float* ringPts, const float*
     emin = !emin;                      /* planted: min and max extent swapped */
 #endif
     ringctx* r = (ringctx*)malloc(sizeof(ringctx));
-    ring_setup(r, tet, n, ringPts, x0p, emin);
+    ring_setup(r, tet, n, ringPts, x0p, emin, p->hex_metric);
     const int nq = p->nq;
     const float aff = 1.0f / (float)(nq - 1);
     const float shr = shrink_of(p);
@@ -175,7 +180,7 @@ static int learned_window_ring(int tet, int n, const float* ringPts, const float
     out[0] = x0p[0]; out[1] = x0p[1]; out[2] = x0p[2];
     if (n <= 0 || n > (tet ? MAXR : 64)) return -1;
     ringctx* r = (ringctx*)malloc(sizeof(ringctx));
-    ring_setup(r, tet, n, ringPts, x0p, p->extent_mode == OR_EXTENT_MIN);
+    ring_setup(r, tet, n, ringPts, x0p, p->extent_mode == OR_EXTENT_MIN, p->hex_metric);
     const int NL = 4;
     const float aff = 1.0f / (float)(NL - 1);
     const float q0 = ring_score_at(r, x0p[0], x0p[1], x0p[2]);
@@ -394,57 +399,61 @@ static void elem_pts(const or_mesh* m, int e, float* pts)
 /* the element metric of each mesh kind: tet / hex shape metrics, the 2D
    mean ratio of flat triangles, the unsigned 3D mean ratio of surface
    triangles and quads */
-static float elem_q(const or_mesh* m, const float* pts)
+static float elem_q(const or_mesh* m, const float* pts, int hexMetric)
 {
     switch (m->elem) {
     case OR_TET: return or_q_tet(pts);
-    case OR_HEX: return or_q_hex(pts);
+    case OR_HEX: return or_q_hex_m(pts, hexMetric);
     case OR_QUAD: return or_q_quad(pts);
     default: return m->flat ? or_q_tri2d(pts) : or_q_tri3(pts);
     }
 }
 
-static double elem_q64(const or_mesh* m, const float* pts)
+static double elem_q64(const or_mesh* m, const float* pts, int hexMetric)
 {
     switch (m->elem) {
     case OR_TET: return or_q_tet64(pts);
-    case OR_HEX: return or_q_hex64(pts);
+    case OR_HEX: return or_q_hex64_m(pts, hexMetric);
     case OR_QUAD: return or_q_quad64(pts);
     default: return m->flat ? or_q_tri2d64(pts) : or_q_tri364(pts);
     }
 }
 
-void or_element_qualities(const or_mesh* m, float* out)
+void or_element_qualities_m(const or_mesh* m, int hexMetric, float* out)
 {
     for (int e = 0; e < m->nE; ++e) {
         float pts[24];
         elem_pts(m, e, pts);
-        out[e] = elem_q(m, pts);
+        out[e] = elem_q(m, pts, hexMetric);
     }
 }
 
-float or_min_quality(const or_mesh* m)
+float or_min_quality_m(const or_mesh* m, int hexMetric)
 {
     float q = FLT_MAX;
     for (int e = 0; e < m->nE; ++e) {
         float pts[24];
         elem_pts(m, e, pts);
-        float v = elem_q(m, pts);
+        float v = elem_q(m, pts, hexMetric);
         q = fminf(q, v);
     }
     return q;
 }
 
-void or_quality64(const or_mesh* m, double* qmin, double* qmean)
+void or_quality64_m(const or_mesh* m, int hexMetric, double* qmin, double* qmean)
 {
     double mn = 1e300, sum = 0.0;
     for (int e = 0; e < m->nE; ++e) {
         float pts[24];
         elem_pts(m, e, pts);
-        double v = elem_q64(m, pts);
+        double v = elem_q64(m, pts, hexMetric);
         if (v < mn) mn = v;
         sum += v;
     }
     *qmin = mn;
     *qmean = m->nE > 0 ? sum / m->nE : 0.0;
 }
+
+void or_element_qualities(const or_mesh* m, float* out) { or_element_qualities_m(m, OR_HEX_SHAPE9, out); }
+float or_min_quality(const or_mesh* m) { return or_min_quality_m(m, OR_HEX_SHAPE9); }
+void or_quality64(const or_mesh* m, double* qmin, double* qmean) { or_quality64_m(m, OR_HEX_SHAPE9, qmin, qmean); }

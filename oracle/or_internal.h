@@ -182,6 +182,9 @@ static inline void hex_consts(const v3 r[8], float* cd, float* cF, hexp* hp)
 
 typedef struct { float bd2, bF3, dmin, fmin; } hexacc;
 
+/* p-dependent sub-tets per element of a hex metric (OR_HEX_SHAPE9 / _CORNER8) */
+static inline int hex_nsub(int metric) { return metric == OR_HEX_CORNER8 ? 4 : 5; }
+
 static inline void hex_sel(hexacc* a, float d, float F)
 {
     float d2 = d * d;
@@ -202,16 +205,16 @@ static inline hexacc hex_const_acc(int n, const float* cd, const float* cF)
     return a;
 }
 
-static inline float hex_score(int n, hexacc a, const hexp* hp, float px, float py, float pz,
+/* nsub: the p-dependent sub-tets scored per element, 5 (the reference's 9
+   sub-tets) or 4 (the 8 corner tets: the principal-axes tet hp[4] left out) */
+static inline float hex_score(int n, hexacc a, const hexp* hp, int nsub, float px, float py, float pz,
                               float epsD, float epsF)
 {
     float Q1 = fmaf(pz, pz, py * py);
     float Q3 = 3.0f * Q1;
     float t3 = 3.0f * px;
 #if OR_MUTATION == 3
-    const int nsub = 4;                /* planted: the principal-axes sub-tet dropped */
-#else
-    const int nsub = 5;
+    nsub = 4;                          /* planted: the principal-axes sub-tet dropped */
 #endif
     for (int e = 0; e < n; ++e)
         for (int s = 0; s < nsub; ++s) {

@@ -113,7 +113,8 @@ static void fit_quadric(lsurf* s, const w3* pts, int np)
 
 struct or_surf {
     int nS, nFree, nF;
-    int err;              /* 1: a projection found no sector, 2: unsectored neighbours (as the kernels' bits) */
+    int err;              /* 1: a projection found no sector, 2: unsectored neighbours, 4: a ring-2 fit
+                             would hold more than MAX_FIT points (as the kernels' bits) */
     float* pinit;         /* 3*nS: surface positions at optimiser creation */
     float* vnorm;         /* 3*nS */
     float* fnorm;         /* 3*nFaces */
@@ -148,7 +149,46 @@ static void normals(const or_mesh* m, float* fn, float* vn)
     }
 }
 
-or_surf* or_surface_init(const or_mesh* m)
+#define MAX_FIT 64
+
+/* the points of v's quadric fit: v, its one-ring in CCW order (at most
+   MAX_NEIGH points in all, Surface.cu:15-28) and, for ring 2, each one-ring
+   neighbour's CCW targets not taken yet, neighbour by neighbour; returns
+   the count, or -1 when a ring-2 set exceeds MAX_FIT points */
+static int fit_points(const or_mesh* m, int v, int ring, w3* pts)
+{
+    int ids[MAX_FIT];
+    int np = 0;
+    ids[np] = v; pts[np++] = W(P(m, v));
+    for (int h = m->heRow[v]; h < m->heRow[v + 1] && np < MAX_NEIGH; ++h) {
+        ids[np] = m->heTarget[h];
+        pts[np++] = W(P(m, m->heTarget[h]));
+    }
+    if (ring == 2) {
+        const int n1 = np;
+        for (int k = 1; k < n1; ++k) {
+            const int d = ids[k];
+            for (int h = m->heRow[d]; h < m->heRow[d + 1]; ++h) {
+                const int t = m->heTarget[h];
+                int seen = 0;
+#if OR_MUTATION == 17
+                seen = (t == v);   /* planted: only v is skipped, so shared neighbours enter twice */
+#else
+                for (int q = 0; q < np && !seen; ++q) seen = ids[q] == t;
+#endif
+                if (seen) continue;
+                if (np == MAX_FIT) return -1;
+                ids[np] = t;
+                pts[np++] = W(P(m, t));
+            }
+        }
+    }
+    return np;
+}
+
+or_surf* or_surface_init(const or_mesh* m) { return or_surface_init_ring(m, 1); }
+
+or_surf* or_surface_init_ring(const or_mesh* m, int ring)
 {
     or_surf* s = (or_surf*)calloc(1, sizeof(or_surf));
     int nS = m->nSurf;
@@ -192,12 +232,12 @@ or_surf* or_surface_init(const or_mesh* m)
         }
         q->a = ATb / ATA;
     }
-    /* quadric fits of all surface vertices (Surface.cu:11-105) */
+    /* quadric fits of all surface vertices (Surface.cu:11-105), over the
+       one-ring or the 2-ring */
     for (int v = 0; v < nS; ++v) {
-        w3 pts[MAX_NEIGH + 1];
-        int np = 0;
-        pts[np++] = W(P(m, v));
-        for (int h = m->heRow[v]; h < m->heRow[v + 1] && np < MAX_NEIGH; ++h) pts[np++] = W(P(m, m->heTarget[h]));
+        w3 pts[MAX_FIT + 1];
+        int np = fit_points(m, v, ring, pts);
+        if (np < 0) { s->err |= 4; np = MAX_FIT; }
         if (np == 5) pts[np++] = wmul(0.5f, wadd(pts[2], pts[3]));
         lsurf* q = &s->ls[v];
         q->p0 = pts[0];
@@ -405,6 +445,7 @@ static w3 project1d(const or_surf* s, int vid, const int* nb, int n, float u)
    (or_internal.h) with the frame of the vertex's start position */
 typedef struct {
     int tet, n;
+    int nsub;     /* hex: p-dependent sub-tets scored per element (5 or 4) */
     v3 x0;
     float sigma, is3, epsD, epsF;
     tetc tc[512];
@@ -414,8 +455,9 @@ typedef struct {
     float minDist, maxDist;
 } vring;
 
-static void vring_init(const or_mesh* m, int vid, const float* xyz, vring* r)
+static void vring_init(const or_mesh* m, int vid, const float* xyz, int hexMetric, vring* r)
 {
+    r->nsub = hex_nsub(hexMetric);
     r->tet = (m->elem == OR_TET);
     int st = m->ringStride, r0 = m->ringRow[vid];
     r->n = m->ringRow[vid + 1] - r0;
@@ -453,7 +495,8 @@ static void vring_init(const or_mesh* m, int vid, const float* xyz, vring* r)
 static float vring_score(const vring* r, w3 p)
 {
     float px = (p.x - r->x0.x) * r->sigma, py = (p.y - r->x0.y) * r->sigma, pz = (p.z - r->x0.z) * r->sigma;
-    return r->tet ? tet_score(r->n, r->tc, px, py, pz, r->is3) : hex_score(r->n, r->hacc, r->hp, px, py, pz, r->epsD, r->epsF);
+    return r->tet ? tet_score(r->n, r->tc, px, py, pz, r->is3)
+                  : hex_score(r->n, r->hacc, r->hp, r->nsub, px, py, pz, r->epsD, r->epsF);
 }
 
 /* 2D surface search of a free surface vertex (k_optimizeHierarchical2DHex /
@@ -466,7 +509,7 @@ static int search2d(const or_mesh* m, const or_surf* s, const float* snap, const
                     float* uvOut)
 {
     vring* r = (vring*)malloc(sizeof(vring));
-    vring_init(m, vid, xyz, r);
+    vring_init(m, vid, xyz, p->hex_metric, r);
     if (r->n <= 0) { free(r); return 0; }
 #if OR_MUTATION == 7
     const float dist = (m->elem == OR_HEX) ? r->maxDist : r->minDist;   /* planted: window rules swapped */
@@ -524,7 +567,7 @@ static int search1d(const or_mesh* m, const or_surf* s, const or_dmo_params* p, 
 {
     if (!s->opt1d[vid]) return 0;
     vring* r = (vring*)malloc(sizeof(vring));
-    vring_init(m, vid, xyz, r);
+    vring_init(m, vid, xyz, p->hex_metric, r);
     if (r->n <= 0) { free(r); return 0; }
     /* feature neighbours that themselves have exactly two feature neighbours */
     int nb[MAX_NEIGH], nnb = 0;

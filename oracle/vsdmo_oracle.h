@@ -38,6 +38,8 @@ extern "C" {
 enum { OR_TRI = 0, OR_QUAD = 1, OR_TET = 2, OR_HEX = 3 };
 enum { OR_COLOR_GREEDY = 0, OR_COLOR_JPL = 1 };
 enum { OR_EXTENT_MAX = 0, OR_EXTENT_MIN = 1 };
+/* hex metric: the reference's 9 sub-tets, or the 8 corner tets alone */
+enum { OR_HEX_SHAPE9 = 0, OR_HEX_CORNER8 = 1 };
 
 /* ---- canonical scalar primitives ---- */
 float or_cbrt(float x);                 /* own cube root (Newton on x^-1/3) */
@@ -46,6 +48,7 @@ float or_pow2_scale(float m);           /* 2^-e with m = f*2^e, f in [1,2)   */
 /* ---- element qualities (canonical FP32, vertex-0 frame) ---- */
 float or_q_tet(const float* p4x3);      /* mean ratio, signed volume if inverted */
 float or_q_hex(const float* p8x3);      /* 9-sub-tet shape metric, internal order */
+float or_q_hex_m(const float* p8x3, int metric);   /* OR_HEX_SHAPE9 / OR_HEX_CORNER8 */
 float or_q_tri2d(const float* p3x3);    /* 2D mean ratio (x,y), signed if inverted */
 /* surface triangles / quads (literal FP32 with contraction off): unsigned
    3D mean ratio, and the variant signed by a unit normal n (quads: -1 when
@@ -57,6 +60,7 @@ float or_q_quadn(const float* p4x3, const float* n);
 /* ---- FP64 literal re-evaluation (reference formulas in double) ---- */
 double or_q_tet64(const float* p4x3);
 double or_q_hex64(const float* p8x3);
+double or_q_hex64_m(const float* p8x3, int metric);
 double or_q_tri2d64(const float* p3x3);
 double or_q_tri364(const float* p3x3);
 double or_q_quad64(const float* p4x3);
@@ -66,6 +70,7 @@ double or_q_quad64(const float* p4x3);
    point xyz; x0: frame origin (vertex position at search start); p: candidate */
 float or_ring_score_tet(int nElem, const float* ring, const float* x0, float sigma, const float* p);
 float or_ring_score_hex(int nElem, const float* ring, const float* x0, float sigma, const float* p);
+float or_ring_score_hex_m(int nElem, const float* ring, const float* x0, float sigma, const float* p, int metric);
 float or_ring_score_tri2d(int nElem, const float* ring, const float* x0, float sigma, const float* p);
 
 /* ---- mesh build ---- */
@@ -130,6 +135,7 @@ typedef struct or_dmo_params {
     int nthreads;        /* OpenMP threads, 0 = default */
     int max_per_class;   /* > 0: search only the first max_per_class vertices of
                             each colour class (a timing sample); 0 = all */
+    int hex_metric;      /* OR_HEX_SHAPE9 (reference) or OR_HEX_CORNER8 */
 } or_dmo_params;
 
 /* one DMO iteration over the inner colour classes (volume meshes) or the free
@@ -157,6 +163,9 @@ typedef struct or_surf or_surf;
 /* local surfaces, feature sectors, 1D curves, NN table, 1D eligibility;
    built from the mesh's current positions */
 or_surf* or_surface_init(const or_mesh* m);
+/* the same with the quadric fits over {v} + one-ring (ring 1, the
+   reference) or {v} + one-ring + 2-ring (ring 2) */
+or_surf* or_surface_init_ring(const or_mesh* m, int ring);
 void or_surface_free(or_surf* s);
 /* inner pass, then (if !fixed_boundary and lastMin > 0) free-surface 2D and
    feature 1D passes, then the nearest-neighbour update; returns searches */
@@ -201,6 +210,10 @@ void or_surface_update_nn(const or_mesh* m, or_surf* s);
 float or_min_quality(const or_mesh* m);   /* flat tri: 2D metric; surface tri / quad: unsigned 3D */
 void or_quality64(const or_mesh* m, double* qmin, double* qmean);
 void or_element_qualities(const or_mesh* m, float* out);
+/* the same with the hex metric named (OR_HEX_SHAPE9 / OR_HEX_CORNER8) */
+float or_min_quality_m(const or_mesh* m, int hex_metric);
+void or_quality64_m(const or_mesh* m, int hex_metric, double* qmin, double* qmean);
+void or_element_qualities_m(const or_mesh* m, int hex_metric, float* out);
 
 /* sample points of vid's local surfaces on an nu x nv grid (mode 0: blended
    projection, mode 1: the NN's fit / sector fit sid); out: 3*nu*nv floats */

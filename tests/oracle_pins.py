@@ -31,10 +31,12 @@ def _volume_mesh(o, kind, n, seed, jitter):
     return o.OracleMesh(kind, xyz, cells)
 
 
-def check_volume_search(o, kind="tet", n=6, seed=5, jitter=0.35, limit=None, min_clean=0.85):
+def check_volume_search(o, kind="tet", n=6, seed=5, jitter=0.35, limit=None, min_clean=0.85, corner8=False):
     """Every interior vertex of a jittered cube searched on its own by the
-    oracle (or_dmo_vertex) and by ref64.search_volume."""
+    oracle (or_dmo_vertex) and by ref64.search_volume (corner8: both on the
+    hex metric over the eight corner sub-tets)."""
     M = _volume_mesh(o, kind, n, seed, jitter)
+    M.hex_metric = o.OR_HEX_CORNER8 if corner8 else o.OR_HEX_SHAPE9
     row, val = M.ring()
     pos = M.positions()
     m = M.m
@@ -42,7 +44,7 @@ def check_volume_search(o, kind="tet", n=6, seed=5, jitter=0.35, limit=None, min
     clean = agree = 0
     for v in verts:
         ring = pos[val[row[v]:row[v + 1]].ravel()]
-        ref = ref64.search_volume(kind, pos[v], ring)
+        ref = ref64.search_volume(kind, pos[v], ring, corner8=corner8)
         moved, out = M.search_vertex(v, pos)
         same = bool(moved) == ref.written and np.array_equal(out[v], ref.pos)
         agree += same
@@ -116,6 +118,54 @@ def check_hex_metric(o, n_random=400, seed=3):
             continue
         got = o.q_elem("hex", P[k])
         assert abs(got - ref[k]) <= 2e-6 + 2e-6 * abs(ref[k]), (k, got, ref[k])
+
+
+def check_hex_corner8(o, n_random=400, seed=4):
+    """The eight-corner hex metric (OR_HEX_CORNER8) against ref64.q_hex with
+    corner8: on the twisted hexes, whose strict minimum is the principal-axes
+    sub-tet, it must score above the nine-sub-tet metric (the ninth tet left
+    out); on random distorted hexes it must match the restatement."""
+    for P in hexes_with_axes_minimum():
+        got = o.q_elem("hex", P, hex_metric=o.OR_HEX_CORNER8)
+        ref = float(ref64.q_hex(P.astype(np.float64), corner8=True))
+        nine = float(ref64.q_hex(P.astype(np.float64)))
+        assert abs(got - ref) <= 2e-6 + 2e-6 * abs(ref), (got, ref)
+        assert got > 1.02 * nine, (got, nine)
+        assert o.q_elem("hex", P, fp64=True, hex_metric=o.OR_HEX_CORNER8) == ref or \
+            abs(o.q_elem("hex", P, fp64=True, hex_metric=o.OR_HEX_CORNER8) - ref) <= 1e-12 * abs(ref)
+    rng = np.random.default_rng(seed)
+    base = np.array([[0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1], [0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]],
+                    np.float32)
+    P = (base + rng.uniform(-0.35, 0.35, size=(n_random, 8, 3))).astype(np.float32)
+    ref = ref64.q_hex(P.astype(np.float64), corner8=True)
+    det, _ = ref64.hex_subtets(P.astype(np.float64))
+    near = np.any(np.abs(det[:, :8] - float(ref64.EPS)) < 1e-5, axis=1)
+    for k in range(n_random):
+        if near[k]:
+            continue
+        got = o.q_elem("hex", P[k], hex_metric=o.OR_HEX_CORNER8)
+        assert abs(got - ref[k]) <= 2e-6 + 2e-6 * abs(ref[k]), (k, got, ref[k])
+
+
+def check_hex_corner8_rings(o, count=24):
+    """The inner hex search on the eight-corner metric over one-element rings
+    (vertex 0 of a twisted hex moving, the other seven fixed): the twisted
+    hexes' minimum is the principal-axes sub-tet, so the two metrics pull
+    vertex 0 to different places.  Clear decisions must match ref64 bit for
+    bit, and some searches must end elsewhere than on the nine-tet metric."""
+    clean = differ = 0
+    for P in hexes_with_axes_minimum(count=count, seed=12):
+        x0, ring = P[0], P[1:8]
+        ref = ref64.search_volume("hex", x0, ring, corner8=True)
+        w, out = o.search_ring("hex", ring, x0, hex_metric=o.OR_HEX_CORNER8)
+        w9, out9 = o.search_ring("hex", ring, x0)
+        differ += not np.array_equal(out, out9)
+        if ref.margin < MARGIN:
+            continue
+        clean += 1
+        assert bool(w) == ref.written and np.array_equal(out, ref.pos), (w, ref.written, out, ref.pos, ref.margin)
+    assert clean >= count // 2 and differ > 0, (clean, differ)
+    return clean, differ
 
 
 def check_flat_search(o, n=16, seed=3, jitter=0.4, min_clean=0.85):
@@ -204,25 +254,28 @@ def surface_state(M, pinit):
                               rrow, rval.reshape(-1, per), he_next=he[:, 3], is_b1d=b1d, vnorm=M.surface_get(4))
 
 
-def surface_mesh(o, kind, iterations=1, seed=21, jitter=0.25):
+def surface_mesh(o, kind, iterations=1, seed=21, jitter=0.25, corner8=False, ring=1):
     """oracle mesh of curved_cube after its surface init and `iterations`
-    surface iterations (so the nearest-neighbour table has moved)"""
+    surface iterations (so the nearest-neighbour table has moved); corner8:
+    hex scores over the eight corner sub-tets; ring 2: 2-ring quadric fits"""
     xyz, cells = curved_cube(kind, seed=seed, jitter=jitter)
     M = o.OracleMesh(kind, xyz, cells)
-    M.surface_init()
+    M.hex_metric = o.OR_HEX_CORNER8 if corner8 else o.OR_HEX_SHAPE9
+    M.surface_init(ring)
     pinit = M.positions()[: M.m.nSurf].copy()
     for _ in range(iterations):
         M.iterate_surface()
     return M, pinit
 
 
-def check_surface_search(o, kind="tet", dim=2, limit=48, min_clean=0.7):
+def check_surface_search(o, kind="tet", dim=2, limit=48, min_clean=0.7, corner8=False, ring=1):
     """Free surface vertices (2D) / eligible feature vertices (1D) of the
     curved cube, one at a time: the oracle's search against
     ref64.search_surface2d / search_feature1d.  Clear decisions: the same
     written flag and final local coordinates, positions within 1e-6."""
-    M, pinit = surface_mesh(o, kind)
+    M, pinit = surface_mesh(o, kind, corner8=corner8, ring=ring)
     S = surface_state(M, pinit)
+    S.corner8 = corner8
     m = M.m
     if dim == 2:
         verts = list(range(0, m.nFree))
@@ -331,12 +384,13 @@ def check_nn_walk(o, kind="hex", step=0.6):
     return exact / m.nSurf
 
 
-def check_learned_window(o, kind="tet", depth_l=1, frac=0.25, spread=0.3, seed=9, min_clean=0.8):
+def check_learned_window(o, kind="tet", depth_l=1, frac=0.25, spread=0.3, seed=9, min_clean=0.8, corner8=False):
     """The oracle's learned window (or_learned_window) against
     ref64.learned_window on every interior vertex of a jittered cube, with
     proposals scattered around the vertex by up to `spread` h."""
     n, s, jit = (6, 5, 0.35) if kind == "tet" else (5, 7, 0.2)
     M = _volume_mesh(o, kind, n, s, jit)
+    M.hex_metric = o.OR_HEX_CORNER8 if corner8 else o.OR_HEX_SHAPE9
     row, val = M.ring()
     pos = M.positions()
     m = M.m
@@ -347,7 +401,7 @@ def check_learned_window(o, kind="tet", depth_l=1, frac=0.25, spread=0.3, seed=9
     for v in range(m.nSurf, m.nV):
         ring = pos[val[row[v]:row[v + 1]].ravel()]
         c = (pos[v] + rng.uniform(-spread * h, spread * h, size=3)).astype(np.float32)
-        ref = ref64.learned_window(kind, pos[v], ring, c, depth_l, frac)
+        ref = ref64.learned_window(kind, pos[v], ring, c, depth_l, frac, corner8=corner8)
         st, out = M.learned_window(v, pos, c, depth_l, frac)
         if ref.margin < MARGIN:
             continue
@@ -479,3 +533,60 @@ def check_sectors(o, kind="tet"):
             assert int(fits[frow[i] + q][15]) == fitted, (i, q, fits[frow[i] + q][15], fitted)
         nsec += len(counts)
     assert nsec > len(spec)          # vertices with more than one sector were checked
+
+
+def two_ring_sets(faces, n_surf):
+    """{v} + one-ring + 2-ring of every surface vertex from the boundary
+    faces' edges alone (breadth-first to depth 2 over the face edges: the
+    vertices a halfedge fan reaches), as sets"""
+    nb = [set() for _ in range(n_surf)]
+    for f in faces:
+        k = len(f)
+        for i in range(k):
+            a, b = int(f[i]), int(f[(i + 1) % k])
+            nb[a].add(b)
+            nb[b].add(a)
+    out = []
+    for v in range(n_surf):
+        s1 = nb[v]
+        s2 = set().union(*[nb[d] for d in s1]) if s1 else set()
+        out.append({v} | s1 | s2)
+    return out
+
+
+def fit64(points, p0, e):
+    """least-squares height field w = a u^2 + b uv + c v^2 in the frame e
+    (rows e0, e1, e2) through p0, in FP64 (Surface.cu:30-105 solves the
+    same normal equations)"""
+    d = np.asarray(points, np.float64) - np.asarray(p0, np.float64)
+    u, v, w = d @ e[0], d @ e[1], d @ e[2]
+    A = np.stack([u * u, u * v, v * v], axis=1)
+    return np.linalg.solve(A.T @ A, A.T @ w)
+
+
+def check_ring2_fits(o, kind="hex", seed=21):
+    """Quadric fits over the 2-ring: for every surface vertex of the curved
+    cube, the oracle's point count equals the size of the depth-2 set of the
+    face graph, and its (a, b, c) equal an FP64 least-squares fit over that
+    set in the oracle's frame (1e-3 relative to the largest coefficient of
+    the cube's fits).  The same mesh's one-ring fits must differ."""
+    xyz, cells = curved_cube(kind, seed=seed)
+    M = o.OracleMesh(kind, xyz, cells)
+    M.surface_init(2)
+    assert M.surface_error() == 0
+    ls2 = M.surface_get(1).astype(np.float64)
+    M.surface_init(1)
+    ls1 = M.surface_get(1).astype(np.float64)
+    pos = M.positions().astype(np.float64)
+    sets = two_ring_sets(M.faces(), M.m.nSurf)
+    scale = np.abs(ls2[:, 12:15]).max()
+    worst = 0.0
+    for v in range(M.m.nSurf):
+        pts = sorted(sets[v])
+        assert int(ls2[v, 15]) == len(pts), (v, ls2[v, 15], len(pts))
+        e = ls2[v, 3:12].reshape(3, 3)
+        ref = fit64(pos[pts], ls2[v, 0:3], e)
+        worst = max(worst, float(np.abs(ls2[v, 12:15] - ref).max()) / scale)
+    assert worst <= 1e-3, worst
+    assert np.abs(ls1[:, 12:15] - ls2[:, 12:15]).max() > 1e-2 * scale
+    return worst

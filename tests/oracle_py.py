@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 OR_TRI, OR_QUAD, OR_TET, OR_HEX = 0, 1, 2, 3
 OR_COLOR_GREEDY, OR_COLOR_JPL = 0, 1
 OR_EXTENT_MAX, OR_EXTENT_MIN = 0, 1
+OR_HEX_SHAPE9, OR_HEX_CORNER8 = 0, 1
 
 _ip = C.POINTER(C.c_int)
 _fp = C.POINTER(C.c_float)
@@ -44,7 +45,7 @@ class BuildParams(C.Structure):
 class DmoParams(C.Structure):
     _fields_ = [("nq", C.c_int), ("depth", C.c_int), ("grid_scale", C.c_float), ("shrink", C.c_float),
                 ("extent_mode", C.c_int), ("fixed_boundary", C.c_int), ("nthreads", C.c_int),
-                ("max_per_class", C.c_int)]
+                ("max_per_class", C.c_int), ("hex_metric", C.c_int)]
 
 
 _lib = None
@@ -80,6 +81,10 @@ def _configure(L):
         getattr(L, n).restype = C.c_float; getattr(L, n).argtypes = [_fp]
     for n in ("or_q_tet64", "or_q_hex64", "or_q_tri2d64"):
         getattr(L, n).restype = C.c_double; getattr(L, n).argtypes = [_fp]
+    L.or_q_hex_m.restype = C.c_float; L.or_q_hex_m.argtypes = [_fp, C.c_int]
+    L.or_q_hex64_m.restype = C.c_double; L.or_q_hex64_m.argtypes = [_fp, C.c_int]
+    L.or_ring_score_hex_m.restype = C.c_float
+    L.or_ring_score_hex_m.argtypes = [C.c_int, _fp, _fp, C.c_float, _fp, C.c_int]
     for n in ("or_ring_score_tet", "or_ring_score_hex", "or_ring_score_tri2d"):
         getattr(L, n).restype = C.c_float
         getattr(L, n).argtypes = [C.c_int, _fp, _fp, C.c_float, _fp]
@@ -107,8 +112,14 @@ def _configure(L):
     L.or_min_quality.argtypes = [C.POINTER(OrMesh)]
     L.or_quality64.argtypes = [C.POINTER(OrMesh), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.or_element_qualities.argtypes = [C.POINTER(OrMesh), _fp]
+    L.or_min_quality_m.restype = C.c_float
+    L.or_min_quality_m.argtypes = [C.POINTER(OrMesh), C.c_int]
+    L.or_quality64_m.argtypes = [C.POINTER(OrMesh), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_element_qualities_m.argtypes = [C.POINTER(OrMesh), C.c_int, _fp]
     L.or_surface_init.restype = C.c_void_p
     L.or_surface_init.argtypes = [C.POINTER(OrMesh)]
+    L.or_surface_init_ring.restype = C.c_void_p
+    L.or_surface_init_ring.argtypes = [C.POINTER(OrMesh), C.c_int]
     L.or_surface_free.argtypes = [C.c_void_p]
     L.or_dmo_iteration_surface.restype = C.c_longlong
     L.or_dmo_iteration_surface.argtypes = [C.POINTER(OrMesh), C.c_void_p, C.POINTER(DmoParams), C.c_float,
@@ -158,10 +169,13 @@ def cbrt(x: float) -> float:
     return lib().or_cbrt(float(x))
 
 
-def q_elem(kind: str, pts, fp64: bool = False, normal=None) -> float:
+def q_elem(kind: str, pts, fp64: bool = False, normal=None, hex_metric: int = 0) -> float:
     """kind: tet, hex, tri (2D), tri3 (surface triangle), quad (surface quad);
-    normal: signed surface variants (tri3 / quad)"""
+    normal: signed surface variants (tri3 / quad); hex_metric: OR_HEX_SHAPE9
+    or OR_HEX_CORNER8 for hexes"""
     a, p = _f(pts)
+    if kind == "hex" and hex_metric:
+        return (lib().or_q_hex64_m if fp64 else lib().or_q_hex_m)(p, int(hex_metric))
     name = {"tet": "or_q_tet", "hex": "or_q_hex", "tri": "or_q_tri2d", "tri3": "or_q_tri3", "quad": "or_q_quad"}[kind]
     if normal is not None:
         n, npp = _f(normal)
@@ -169,12 +183,14 @@ def q_elem(kind: str, pts, fp64: bool = False, normal=None) -> float:
     return getattr(lib(), name + ("64" if fp64 else ""))(p)
 
 
-def ring_score(kind: str, ring, x0, sigma, p) -> float:
+def ring_score(kind: str, ring, x0, sigma, p, hex_metric: int = 0) -> float:
     r, rp = _f(ring)
     x, xp = _f(x0)
     q, qp = _f(p)
     per = {"tet": 9, "hex": 21, "tri": 6}[kind]
     n = r.size // per
+    if kind == "hex" and hex_metric:
+        return lib().or_ring_score_hex_m(n, rp, xp, C.c_float(sigma), qp, int(hex_metric))
     return getattr(lib(), {"tet": "or_ring_score_tet", "hex": "or_ring_score_hex", "tri": "or_ring_score_tri2d"}[kind])(
         n, rp, xp, C.c_float(sigma), qp)
 
@@ -221,6 +237,8 @@ class OracleMesh:
             raise ValueError(f"oracle mesh build failed: {rc}")
         self.ptr = out
         self.rc = rc
+        # the hex metric of every search and quality call (OR_HEX_SHAPE9 or OR_HEX_CORNER8)
+        self.hex_metric = OR_HEX_SHAPE9
 
     def __del__(self):
         if getattr(self, "surf", None):
@@ -283,13 +301,13 @@ class OracleMesh:
         of each colour class (a timing sample).  Returns the searches run."""
         if extent is None:
             extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
-        p = DmoParams(nq, depth, grid_scale, shrink, extent, fixed_boundary, nthreads, max_per_class)
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, fixed_boundary, nthreads, max_per_class, self.hex_metric)
         return self.L.or_dmo_iteration(self.ptr, C.byref(p))
 
     def search_vertex(self, vid, xyz, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None):
         if extent is None:
             extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
-        p = DmoParams(nq, depth, grid_scale, shrink, extent, 1, 1)
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, 1, 1, 0, self.hex_metric)
         a = np.ascontiguousarray(xyz, dtype=np.float32).copy()
         moved = self.L.or_dmo_vertex(self.ptr, C.byref(p), vid, a.ctypes.data_as(_fp))
         return moved, a
@@ -299,7 +317,7 @@ class OracleMesh:
         on the positions xyz: (status, position); status 1 moved, 0 stays,
         -1 falls back to the full search"""
         extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
-        p = DmoParams(nq, depth, grid_scale, 0.0, extent, 1, 1)
+        p = DmoParams(nq, depth, grid_scale, 0.0, extent, 1, 1, 0, self.hex_metric)
         a = np.ascontiguousarray(xyz, dtype=np.float32)
         cc = np.ascontiguousarray(c, dtype=np.float32).reshape(3)
         out = np.zeros(3, np.float32)
@@ -307,11 +325,12 @@ class OracleMesh:
                                       depth_l, C.c_float(frac), out.ctypes.data_as(_fp))
         return st, out
 
-    def surface_init(self):
-        """Local surfaces etc. from the current positions (optimiser init)."""
+    def surface_init(self, ring: int = 1):
+        """Local surfaces etc. from the current positions (optimiser init);
+        ring 2: quadric fits over the 2-ring"""
         if getattr(self, "surf", None):
             self.L.or_surface_free(self.surf)
-        self.surf = self.L.or_surface_init(self.ptr)
+        self.surf = self.L.or_surface_init_ring(self.ptr, int(ring))
         self.last_min = self.min_quality()
 
     def iterate_surface(self, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None, nthreads=0):
@@ -319,7 +338,7 @@ class OracleMesh:
         quality, NN update (reference doIteration); returns (2D, 1D) counts."""
         if extent is None:
             extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
-        p = DmoParams(nq, depth, grid_scale, shrink, extent, 0, nthreads)
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, 0, nthreads, 0, self.hex_metric)
         a, b = C.c_longlong(), C.c_longlong()
         self.L.or_dmo_iteration_surface(self.ptr, self.surf, C.byref(p), C.c_float(self.last_min), C.byref(a), C.byref(b))
         self.last_min = self.min_quality()
@@ -381,7 +400,7 @@ class OracleMesh:
         """One surface vertex's 2D (dim 2) or 1D (dim 1) search on the current
         positions; returns (written, final position, final local (u, v))."""
         extent = OR_EXTENT_MIN if self.kind == "hex" else OR_EXTENT_MAX
-        p = DmoParams(nq, depth, grid_scale, shrink, extent, 0, 1)
+        p = DmoParams(nq, depth, grid_scale, shrink, extent, 0, 1, 0, self.hex_metric)
         a = self.positions().copy()
         uv = np.zeros(2, np.float32)
         w = self.L.or_surface_search_vertex(self.ptr, self.surf, C.byref(p), vid, dim, a.ctypes.data_as(_fp),
@@ -390,7 +409,8 @@ class OracleMesh:
 
     def surface_error(self) -> int:
         """bit 0: a projection found no feature sector; bit 1: a feature vertex
-        has neighbours that belong to no sector"""
+        has neighbours that belong to no sector; bit 2: a ring-2 fit would
+        hold more than 64 points"""
         return self.L.or_surface_error(self.surf)
 
     def surface_project(self, vid: int, u: float, v: float) -> np.ndarray:
@@ -402,20 +422,20 @@ class OracleMesh:
         self.L.or_surface_update_nn(self.ptr, self.surf)
 
     def min_quality(self) -> float:
-        return self.L.or_min_quality(self.ptr)
+        return self.L.or_min_quality_m(self.ptr, self.hex_metric)
 
     def quality64(self):
         a, b = C.c_double(), C.c_double()
-        self.L.or_quality64(self.ptr, C.byref(a), C.byref(b))
+        self.L.or_quality64_m(self.ptr, self.hex_metric, C.byref(a), C.byref(b))
         return a.value, b.value
 
     def element_qualities(self) -> np.ndarray:
         out = np.empty(self.m.nE, dtype=np.float32)
-        self.L.or_element_qualities(self.ptr, out.ctypes.data_as(_fp))
+        self.L.or_element_qualities_m(self.ptr, self.hex_metric, out.ctypes.data_as(_fp))
         return out
 
 
-def search_ring(kind: str, ring, x0, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None):
+def search_ring(kind: str, ring, x0, nq=8, depth=3, grid_scale=0.5, shrink=0.0, extent=None, hex_metric=0):
     """The oracle's vertex search on an explicit ring (tet: n x 3 points,
     hex: n x 7 points, tri: a closed fan of n points); returns (written,
     final position)."""
@@ -425,7 +445,7 @@ def search_ring(kind: str, ring, x0, nq=8, depth=3, grid_scale=0.5, shrink=0.0, 
     per = {"tet": 3, "hex": 7, "tri": 1}[kind]
     x, xp = _f(np.asarray(x0, dtype=np.float32).reshape(3))
     out = np.zeros(3, np.float32)
-    p = DmoParams(nq, depth, grid_scale, shrink, extent, 1, 1)
+    p = DmoParams(nq, depth, grid_scale, shrink, extent, 1, 1, 0, hex_metric)
     w = lib().or_search_ring(_ELEM[kind], r.shape[0] // per, r.ctypes.data_as(_fp), xp, C.byref(p),
                              out.ctypes.data_as(_fp))
     return w, out

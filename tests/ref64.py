@@ -78,10 +78,14 @@ def hex_subtets(P):
     return det, jn
 
 
-def q_hex(P):
+def q_hex(P, corner8=False):
     """shapeMetricHex (dmo/DMOQualityHex.h:54-71): 0 when any sub-tet has
-    det < EPSILON or |J|^2 < EPSILON, else 3 min cbrt(det^2) / |J|^2."""
+    det < EPSILON or |J|^2 < EPSILON, else 3 min cbrt(det^2) / |J|^2.
+    corner8: the same over the eight corner sub-tets (the first eight of
+    hex_subtets) without the principal-axes tet."""
     det, jn = hex_subtets(P)
+    if corner8:
+        det, jn = det[..., :8], jn[..., :8]
     bad = np.any((det < float(EPS)) | (jn < float(EPS)), axis=-1)
     with np.errstate(divide="ignore", invalid="ignore"):
         v = np.min(np.cbrt(det * det) / jn, axis=-1)
@@ -100,7 +104,7 @@ def q_tri2d(P):
     return np.where(area < 0.0, area, q)
 
 
-def ring_quality(kind: str, ring, p):
+def ring_quality(kind: str, ring, p, corner8=False):
     """Ring quality at candidate points p[..., 3]: the minimum over the ring
     elements with p as vertex 0 (qualityShapeMRMTet, DMOQualityTet.h:146-162;
     qualityShapeMRMHex, DMOQualityHex.h:75-95; the closed 2D fan of
@@ -119,7 +123,7 @@ def ring_quality(kind: str, ring, p):
     els = ring.reshape(-1, per, 3)
     P = np.concatenate([np.broadcast_to(pp, (pp.shape[0], len(els), 1, 3)),
                         np.broadcast_to(els, (pp.shape[0],) + els.shape)], axis=2)
-    q = q_tet(P) if kind == "tet" else q_hex(P)
+    q = q_tet(P) if kind == "tet" else q_hex(P, corner8)
     return q.min(axis=1).reshape(lead)
 
 
@@ -155,7 +159,7 @@ class Result:
 
 
 # ------------------------------------------------------------ inner searches
-def search_volume(kind: str, x0, ring, nq=8, depth=3, grid_scale=0.5, block=128) -> Result:
+def search_volume(kind: str, x0, ring, nq=8, depth=3, grid_scale=0.5, block=128, corner8=False) -> Result:
     """k_optimizeHierarchicalInnerTet / Hex (dmo/DMOTetImplCub.h:20-137,
     dmo/DMOHexImplCub.h:20-137), launched with 128 threads
     (dmo/DMOTet.cu:134, dmo/DMOHex.cu:146): thread t scans candidates
@@ -180,8 +184,8 @@ def search_volume(kind: str, x0, ring, nq=8, depth=3, grid_scale=0.5, block=128)
         gmin = cur - ds * ext
         cand = np.stack([_grid(aff, ii, nq, gmin[0], gmax[0]), _grid(aff, jj, nq, gmin[1], gmax[1]),
                          _grid(aff, kk, nq, gmin[2], gmax[2])], axis=1)
-        q = ring_quality(kind, pts, cand)
-        qlast = float(ring_quality(kind, pts, cur))
+        q = ring_quality(kind, pts, cand, corner8)
+        qlast = float(ring_quality(kind, pts, cur, corner8))
         pmax = [None] * block
         for t in range(block):
             for c in range(t * ppt, (t + 1) * ppt):
@@ -195,7 +199,7 @@ def search_volume(kind: str, x0, ring, nq=8, depth=3, grid_scale=0.5, block=128)
             cur = cand[pmax[key]].copy()
         levels.append((int(pmax[key]) if pmax[key] is not None else -1, moved))
         ds = ds * (F(2.0) / F(nq - 1))
-    qold = float(ring_quality(kind, pts, x0))
+    qold = float(ring_quality(kind, pts, x0, corner8))
     margin = min(margin, rel_gap(qmax[key], qold))
     written = gt(qmax[key], qold)
     return Result(cur if written else x0, written, margin, {"levels": levels})
@@ -269,6 +273,7 @@ class SurfaceState:
         self.pinit = np.asarray(pinit, np.float64)
         self.nn, self.opt1d = nn, opt1d
         self.ring_row, self.ring_val = ring_row, ring_val
+        self.corner8 = False      # hex ring scores over the eight corner sub-tets
 
     def fan(self, v):
         return self.he_target[self.he_row[v]:self.he_row[v + 1]]
@@ -392,8 +397,8 @@ def search_surface2d(S: SurfaceState, vid: int, nq=8, depth=3, grid_scale=0.5) -
         u2 = cu + _grid(aff, i2, nq, umin, umax)
         pa = np.array([project2d(S, vid, u1[k], v1[k]) for k in range(T)])
         pb = np.array([project2d(S, vid, u2[k], v1[k]) for k in range(T)])
-        qa, qb = ring_quality(kind, ring, pa), ring_quality(kind, ring, pb)
-        qlast = float(ring_quality(kind, ring, cur))
+        qa, qb = ring_quality(kind, ring, pa, S.corner8), ring_quality(kind, ring, pb, S.corner8)
+        qlast = float(ring_quality(kind, ring, cur, S.corner8))
         takeb = np.array([gt(b, a) for a, b in zip(qa, qb)])
         qm = np.where(takeb, qb, qa)
         key = _argmax_threads(qm)
@@ -403,7 +408,7 @@ def search_surface2d(S: SurfaceState, vid: int, nq=8, depth=3, grid_scale=0.5) -
             cur = project2d(S, vid, cu, cv)
             ab_tie = ab_tie or eq(qa[key], qb[key])
         ds = ds * (F(2.0) / F(nq - 1))
-    qold = float(ring_quality(kind, ring, x0))
+    qold = float(ring_quality(kind, ring, x0, S.corner8))
     margin = min(margin, rel_gap(qm[key], qold))
     written = gt(qm[key], qold)
     return Result(cur if written else x0.astype(np.float64), written, margin, {"uv": (F(cu), F(cv)), "ab_tie": ab_tie})
@@ -436,15 +441,15 @@ def search_feature1d(S: SurfaceState, vid: int, nq=8, depth=3, grid_scale=0.5) -
         umax, umin = ds * dist, -ds * dist
         u = cu + _grid(aff, i1, nq, umin, umax)
         pw = np.array([project1d(S, vid, nb, uu) for uu in u])
-        q = ring_quality(kind, ring, pw)
-        qlast = float(ring_quality(kind, ring, cur))
+        q = ring_quality(kind, ring, pw, S.corner8)
+        qlast = float(ring_quality(kind, ring, cur, S.corner8))
         key = _argmax_threads(q)
         margin = min(margin, _top_gap(q), rel_gap(q[key], qlast))
         if gt(q[key], qlast):
             cu = u[key]
             cur = project1d(S, vid, nb, cu)
         ds = ds * (F(2.0) / F(nq - 1))
-    qold = float(ring_quality(kind, ring, x0))
+    qold = float(ring_quality(kind, ring, x0, S.corner8))
     margin = min(margin, rel_gap(q[key], qold))
     written = gt(q[key], qold)
     return Result(cur if written else x0.astype(np.float64), written, margin, {"uv": F(cu)})
@@ -466,7 +471,7 @@ def nearest_same_class(pos, pinit, is_feature):
 
 
 # --------------------------------------------------- learned proposal window
-def learned_window(kind: str, x0, ring, c, depth_l, frac, grid_scale=0.5) -> Result:
+def learned_window(kind: str, x0, ring, c, depth_l, frac, grid_scale=0.5, corner8=False) -> Result:
     """The learned proposal's window search, restated from its specification
     (this build's own search; no reference counterpart): 4 x 4 x 4
     candidates i + 4j + 16k (the lowest index wins ties) on `frac` times the
@@ -484,7 +489,7 @@ def learned_window(kind: str, x0, ring, c, depth_l, frac, grid_scale=0.5) -> Res
     aff = F(1.0) / F(nl - 1)
     idx = np.arange(nl ** 3)
     ii, jj, kk = idx % nl, (idx // nl) % nl, idx // (nl * nl)
-    q0 = float(ring_quality(kind, pts, x0))
+    q0 = float(ring_quality(kind, pts, x0, corner8))
     cur = np.asarray(c, F).reshape(3).copy()
     qbest = q0
     moved = tie = False
@@ -495,7 +500,7 @@ def learned_window(kind: str, x0, ring, c, depth_l, frac, grid_scale=0.5) -> Res
         gmin = cur - ds * ext
         cand = np.stack([_grid(aff, ii, nl, gmin[0], gmax[0]), _grid(aff, jj, nl, gmin[1], gmax[1]),
                          _grid(aff, kk, nl, gmin[2], gmax[2])], axis=1)
-        q = ring_quality(kind, pts, cand)
+        q = ring_quality(kind, pts, cand, corner8)
         b = 0
         for k in range(1, len(q)):
             if gt(q[k], q[b]):

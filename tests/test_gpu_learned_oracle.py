@@ -51,12 +51,13 @@ def nets():
         m.close()
 
 
-def oracle_learned(oracle, kind, xyz, cells, mlp, mode, depth_l, frac, policy, iters):
+def oracle_learned(oracle, kind, xyz, cells, mlp, mode, depth_l, frac, policy, iters, hex_metric=0):
     """The learned iterations restated on the CPU; returns the positions after
     each iteration and the number of full searches run for fallbacks (under
     the deferred policy a deferral counts when its search runs, in the next
     iteration, as the optimiser counts it)."""
     om = oracle.OracleMesh(kind, xyz, cells)
+    om.hex_metric = hex_metric
     row, val = om.ring()
     stride = 3 if kind == "tet" else 7
     off = om.offsets("inner")
@@ -147,6 +148,37 @@ def test_learned_iterations_match_oracle(oracle, nets, kind, mode, depth, frac, 
             b.iterate(1)
             assert np.array_equal(b.positions(0), ref[it]), f"batch, iteration {it}"
         assert b.learned_stats()["fallbacks"] == ref_fb
+    finally:
+        b.close()
+        mb.close()
+
+
+def test_learned_hex_corner8_matches_oracle(oracle, nets):
+    """the learned hex iteration with the eight-corner metric (windows,
+    fallbacks and the deferred full searches all score on it), single mesh
+    and one-mesh batch"""
+    xyz, cells = _mesh("hex", 8, 18)
+    ref, ref_fb = oracle_learned(oracle, "hex", xyz, cells, nets["hex"], 1, 1, 2.0, "deferred", ITERS,
+                                 hex_metric=oracle.OR_HEX_CORNER8)
+    m = DMOMeshHexFactory.create(xyz, cells)
+    o = DMOHexClass(m, n_iter=ITERS, fixed_boundary=True, hex_metric=_lib.VSDMO_HEX_CORNER8)
+    try:
+        o.set_proposal(nets["hex"], mode=1, depth=1, window_frac=2.0, fallback="deferred")
+        for it in range(ITERS):
+            o.iterate(1)
+            assert np.array_equal(m.positions(original_order=False), ref[it]), it
+        assert o.stats()["fallbacks"] == ref_fb
+    finally:
+        o.close()
+        m.close()
+    mb = DMOMeshHexFactory.create(xyz, cells)
+    b = DMOBatch([mb], use_graph=True, hex_metric=_lib.VSDMO_HEX_CORNER8)
+    try:
+        b.set_fallback("deferred")
+        b.set_proposal(nets["hex"], mode=1, depth=1, window_frac=2.0, elem=_lib.VSDMO_HEX)
+        for it in range(ITERS):
+            b.iterate(1)
+            assert np.array_equal(b.positions(0), ref[it]), f"batch, iteration {it}"
     finally:
         b.close()
         mb.close()

@@ -32,6 +32,8 @@ MUTATIONS = {
     14: ("surface-mesh fan window from the nearest fan vertex", lambda o: P.check_fan_search(o, "tri", 2, "sphere")),
     15: ("flat boundary 1D pass: the open fan closed", P.check_flat1d_search),
     16: ("feature sectors: a cut mapped to the sector it opens", P.check_sectors),
+    17: ("2-ring fit: neighbours shared by two one-ring vertices taken twice", P.check_ring2_fits),
+    18: ("inner hex search: the nine sub-tets scored under the eight-corner metric", P.check_hex_corner8_rings),
 }
 
 

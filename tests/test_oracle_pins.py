@@ -116,3 +116,46 @@ def test_flat_boundary_1d_matches_restatement(oracle):
 @pytest.mark.parametrize("kind", ["tet", "hex"])
 def test_feature_sectors_follow_the_specification(oracle, kind):
     P.check_sectors(oracle, kind)
+
+
+def test_hex_corner8_metric(oracle):
+    """the eight-corner hex metric (vsdmo_hex_metric CORNER8): shapeMetricHex
+    (dmo/DMOQualityHex.h:54-71) without the principal-axes sub-tet, against
+    the restatement; it scores the twisted hexes above the nine-tet metric"""
+    P.check_hex_corner8(oracle)
+
+
+def test_hex_corner8_search_matches_restatement(oracle):
+    """the inner hex search (dmo/DMOHexImplCub.h:20-137) on the eight-corner
+    metric: a jittered cube, and single twisted hexes whose minimum is the
+    principal-axes sub-tet"""
+    r = P.check_volume_search(oracle, "hex", n=5, seed=7, jitter=0.2, min_clean=0.75, corner8=True)
+    assert r["vertices"] == 64
+    P.check_hex_corner8_rings(oracle)
+
+
+@pytest.mark.parametrize("dim", [2, 1])
+def test_hex_corner8_surface_search(oracle, dim):
+    """the surface 2D / feature 1D hex searches (dmo/DMOHexImplCub.h:141-489)
+    on the eight-corner metric"""
+    P.check_surface_search(oracle, "hex", dim, corner8=True)
+
+
+def test_hex_corner8_learned_window(oracle):
+    st = P.check_learned_window(oracle, "hex", 1, 2.0, corner8=True)
+    assert st[1] > 0 and st[-1] > 0, st
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_ring2_fits(oracle, kind):
+    """quadric fits over {v} + one-ring + 2-ring (vsdmo_params.surface_ring
+    2; the reference fits the one-ring, dmo/surfaceP/Surface.cu:11-105):
+    point sets from the face graph and FP64 least squares"""
+    P.check_ring2_fits(oracle, kind)
+
+
+@pytest.mark.parametrize("kind,dim", [("tet", 2), ("hex", 2), ("hex", 1)])
+def test_surface_search_on_ring2_fits(oracle, kind, dim):
+    """the surface searches on 2-ring fits (the restatement reads the fits
+    from the oracle, pinned by test_ring2_fits)"""
+    P.check_surface_search(oracle, kind, dim, ring=2)

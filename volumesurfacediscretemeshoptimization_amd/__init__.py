@@ -28,10 +28,13 @@ from . import _lib
 from ._lib import VsdmoError, check, load  # noqa: F401
 
 MEAN_RATIO = 0
+# hex metrics: the reference's 9 sub-tets, or the 8 corner tets alone
+HEX_SHAPE9, HEX_CORNER8 = _lib.VSDMO_HEX_SHAPE9, _lib.VSDMO_HEX_CORNER8
 
 __all__ = ["read_mesh_file", "write_mesh_file", "DMOMeshTriFactory", "DMOMeshQuadFactory", "DMOMeshTetFactory",
            "DMOMeshHexFactory", "DMOMesh", "DMOTetClass", "DMOHexClass", "DMOTriFlatClass", "DMOTriClass", "DMOQuadClass",
-           "laplaceSmoothing", "DMOBatch", "DMOMLP", "DMODataset", "VsdmoError", "MEAN_RATIO", "load"]
+           "laplaceSmoothing", "DMOBatch", "DMOMLP", "DMODataset", "VsdmoError", "MEAN_RATIO", "HEX_SHAPE9",
+           "HEX_CORNER8", "load"]
 
 
 def _fptr(a):
@@ -235,7 +238,7 @@ class _DMOBase:
                  n_iter: int = 30, nq: int = 8, depth: int = 3, shrink: float = 0.0,
                  extent_mode: int = _lib.VSDMO_EXTENT_AUTO, fixed_boundary: bool = False,
                  quality_guard: bool = True, guard_tolerance: float = 1e-5, waves_per_vertex: int = 1,
-                 surface_path: int = None):
+                 surface_path: int = None, hex_metric: int = _lib.VSDMO_HEX_SHAPE9, surface_ring: int = 1):
         lib = load()
         self._lib = lib
         self.mesh = mesh
@@ -245,6 +248,7 @@ class _DMOBase:
         p.fixed_boundary, p.quality_guard, p.guard_tolerance = int(fixed_boundary), int(quality_guard), guard_tolerance
         p.waves_per_vertex = int(waves_per_vertex)
         p.surface_path = self.SURFACE_PATH if surface_path is None else int(surface_path)
+        p.hex_metric, p.surface_ring = int(hex_metric), int(surface_ring)
         h = C.c_void_p()
         check(lib.vsdmo_opt_create(mesh.handle, qualityCriterium, gridScale, n_iter, C.byref(p), C.byref(h)),
               "vsdmo_opt_create")
@@ -507,7 +511,7 @@ class DMOBatch:
 
     def __init__(self, meshes, nq: int = 8, depth: int = 3, waves_per_vertex: int = 1, gridScale: float = 0.5,
                  use_graph: bool = True, shuffle_seed: int = 0, quality_guard: bool = True,
-                 guard_tolerance: float = 1e-5, persistent: bool = True):
+                 guard_tolerance: float = 1e-5, persistent: bool = True, hex_metric: int = _lib.VSDMO_HEX_SHAPE9):
         lib = load()
         self._lib = lib
         self.meshes = list(meshes)
@@ -517,6 +521,7 @@ class DMOBatch:
         p.use_graph, p.shuffle_seed = int(use_graph), int(shuffle_seed)
         p.quality_guard, p.guard_tolerance = int(quality_guard), guard_tolerance
         p.persistent = int(persistent)
+        p.hex_metric = int(hex_metric)
         arr = (C.c_void_p * len(self.meshes))(*[m.handle.value for m in self.meshes])
         h = C.c_void_p()
         check(lib.vsdmo_batch_create(arr, len(self.meshes), C.byref(p), C.byref(h)), "vsdmo_batch_create")

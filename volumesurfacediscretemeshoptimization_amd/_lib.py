@@ -23,6 +23,7 @@ VSDMO_TRI, VSDMO_QUAD, VSDMO_TET, VSDMO_HEX = 0, 1, 2, 3
 VSDMO_COLOR_GREEDY, VSDMO_COLOR_JPL = 0, 1
 VSDMO_EXTENT_AUTO, VSDMO_EXTENT_MAX, VSDMO_EXTENT_MIN = -1, 0, 1
 VSDMO_SURFACE_PATH_AUTO, VSDMO_SURFACE_PATH_SURFACE, VSDMO_SURFACE_PATH_FLAT = 0, 1, 2
+VSDMO_HEX_SHAPE9, VSDMO_HEX_CORNER8 = 0, 1
 
 
 class MeshParams(C.Structure):
@@ -32,7 +33,8 @@ class MeshParams(C.Structure):
 class Params(C.Structure):
     _fields_ = [("nq", C.c_int32), ("depth", C.c_int32), ("shrink", C.c_float), ("extent_mode", C.c_int32),
                 ("fixed_boundary", C.c_int32), ("quality_guard", C.c_int32), ("guard_tolerance", C.c_float),
-                ("waves_per_vertex", C.c_int32), ("surface_path", C.c_int32)]
+                ("waves_per_vertex", C.c_int32), ("surface_path", C.c_int32), ("hex_metric", C.c_int32),
+                ("surface_ring", C.c_int32)]
 
 
 class Counts(C.Structure):
@@ -51,7 +53,7 @@ class Stats(C.Structure):
 class BatchParams(C.Structure):
     _fields_ = [("nq", C.c_int32), ("depth", C.c_int32), ("waves_per_vertex", C.c_int32), ("grid_scale", C.c_float),
                 ("use_graph", C.c_int32), ("shuffle_seed", C.c_uint32), ("quality_guard", C.c_int32),
-                ("guard_tolerance", C.c_float), ("persistent", C.c_int32)]
+                ("guard_tolerance", C.c_float), ("persistent", C.c_int32), ("hex_metric", C.c_int32)]
 
 
 class BatchStats(C.Structure):

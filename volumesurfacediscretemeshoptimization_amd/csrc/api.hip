@@ -164,6 +164,8 @@ void vsdmo_params_default(vsdmo_params* p)
     p->guard_tolerance = 1e-5f;
     p->waves_per_vertex = 1;
     p->surface_path = VSDMO_SURFACE_PATH_AUTO;
+    p->hex_metric = VSDMO_HEX_SHAPE9;
+    p->surface_ring = 1;
 }
 
 const char* vsdmo_last_error(void) { return g_err.c_str(); }
@@ -405,7 +407,8 @@ vsdmo_status vsdmo_mesh_get_flags(const vsdmo_mesh* m, uint8_t* b1d, uint8_t* fe
 static void opt_quality(vsdmo_opt* o)
 {
     vsdmo_mesh* m = o->m;
-    QualityArgs qa{ m->x.p, m->y.p, m->z.p, m->cells.p, m->nE, m->npe, o->qelem, o->q.p };
+    QualityArgs qa{ m->x.p, m->y.p, m->z.p, m->cells.p, m->nE, m->npe, o->qelem, o->q.p,
+                    o->p.hex_metric == VSDMO_HEX_CORNER8 };
     launch_quality(qa, o->pmin.p, o->psum.p, reinterpret_cast<float*>(o->qout.p), o->qout.p + 1, m->stream);
     VSDMO_HIP(hipGetLastError());
     // one 16-byte copy into pinned memory (no staging, one synchronisation)
@@ -423,6 +426,9 @@ static void opt_quality(vsdmo_opt* o)
     if (err & kSurfErrNoSector)
         throw StatusError{ VSDMO_TOPOLOGY, "projection onto a feature neighbour's sectors found no sector for "
                                            "its halfedge nearest to the vertex" };
+    if (err & kSurfErrFitPoints)
+        throw StatusError{ VSDMO_NOT_SUPPORTED, "surface_ring 2: a surface vertex, its one-ring and its 2-ring "
+                                                "hold more than 64 points" };
 }
 
 // empty deferred lists and no deferral epochs (allocated on first use)
@@ -458,6 +464,8 @@ static SurfArgs surf_args(vsdmo_opt* o)
     a.pinit = o->pinit.p; a.fnorm = o->fnorm.p; a.vnorm = o->vnorm.p; a.ls = o->ls.p; a.ls1 = o->ls1.p; a.fs = o->fs.p;
     a.nn = o->nn.p; a.frow = o->frow.p; a.table = o->table.p; a.opt1d = o->opt1d.p;
     a.gridScale = o->grid_scale; a.shrink = o->shrink; a.aff = o->aff; a.depth = o->p.depth;
+    a.hexCorner8 = o->p.hex_metric == VSDMO_HEX_CORNER8;
+    a.ring2 = o->p.surface_ring == 2;
     a.err = reinterpret_cast<int*>(o->qout.p) + 1;
     return a;
 }
@@ -520,6 +528,9 @@ vsdmo_status vsdmo_opt_create(vsdmo_mesh* m, int32_t criterion, float grid_scale
             return fail(VSDMO_INVALID_ARG, "waves_per_vertex must be 0 (auto), 1, 2 or 4 and divide nq");
         if (p.surface_path < VSDMO_SURFACE_PATH_AUTO || p.surface_path > VSDMO_SURFACE_PATH_FLAT)
             return fail(VSDMO_INVALID_ARG, "unknown surface_path");
+        if (p.hex_metric != VSDMO_HEX_SHAPE9 && p.hex_metric != VSDMO_HEX_CORNER8)
+            return fail(VSDMO_INVALID_ARG, "unknown hex_metric");
+        if (p.surface_ring != 1 && p.surface_ring != 2) return fail(VSDMO_INVALID_ARG, "surface_ring must be 1 or 2");
         const bool surfaceMesh = (m->elem == VSDMO_TRI || m->elem == VSDMO_QUAD);
         bool surfMesh = false;   // the surface path of a tri / quad mesh
         if (surfaceMesh) {
@@ -634,6 +645,7 @@ vsdmo_status vsdmo_opt_iterate(vsdmo_opt* o, int32_t n)
         a.heRow = m->heRow.p; a.heTarget = m->heTarget.p;
         a.gridScale = o->grid_scale; a.shrink = o->shrink; a.aff = o->aff;
         a.depth = o->p.depth; a.extentMin = o->extent_min;
+        a.hexCorner8 = o->p.hex_metric == VSDMO_HEX_CORNER8;
         for (int it = 0; it < n; ++it) {
             if (m->elem == VSDMO_TET || m->elem == VSDMO_HEX) {
                 size_t lds = search_lds_bytes(m->elem, m->maxRing);

@@ -139,6 +139,7 @@ SearchArgs search_args(vsdmo_batch* b)
     a.heRow = b->heRow.p; a.heTarget = b->heTarget.p;
     a.gridScale = b->p.grid_scale; a.shrink = b->shrink; a.aff = b->aff;
     a.depth = b->p.depth;
+    a.hexCorner8 = b->p.hex_metric == VSDMO_HEX_CORNER8;
     return a;
 }
 
@@ -152,6 +153,7 @@ BatchQArgs quality_args(vsdmo_batch* b)
     q.cmin = b->cmin.p; q.csum = b->csum.p; q.mmin = b->mmin.p; q.msum = b->msum.p;
     q.hist = b->hist.p; q.hctr = b->hctr.p;
     q.nChunks = b->nChunks; q.nMeshes = (int)b->meshes.size(); q.hcap = b->hcap;
+    q.hexCorner8 = b->p.hex_metric == VSDMO_HEX_CORNER8;
     return q;
 }
 
@@ -427,6 +429,7 @@ std::string check_params(const vsdmo_batch_params& p)
     if (p.waves_per_vertex < 1 || !waves_supported(p.nq, p.waves_per_vertex))
         return "waves_per_vertex must be 1, 2 or 4 and divide nq";
     if (!(p.grid_scale > 0.0f)) return "grid_scale must be > 0";
+    if (p.hex_metric != VSDMO_HEX_SHAPE9 && p.hex_metric != VSDMO_HEX_CORNER8) return "unknown hex_metric";
     return "";
 }
 
@@ -473,6 +476,7 @@ void vsdmo_batch_params_default(vsdmo_batch_params* p)
     p->shuffle_seed = 0;
     p->quality_guard = 1;
     p->guard_tolerance = 1e-5f;
+    p->hex_metric = VSDMO_HEX_SHAPE9;
 }
 
 vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, const vsdmo_batch_params* params,

@@ -46,6 +46,7 @@ struct SearchArgs {
     float gridScale, shrink, aff;
     int depth;
     int extentMin;
+    int hexCorner8;                    // 1: hex metric over the 8 corner sub-tets (vsdmo_hex_metric)
 };
 
 // element kinds of the quality kernels beyond vsdmo_elem: triangles of a
@@ -59,6 +60,7 @@ struct QualityArgs {
     const int* cells;
     int nE, npe, elem;
     float* q;
+    int hexCorner8;
 };
 
 // surface-vertex state and mesh views for the surface kernels
@@ -96,6 +98,8 @@ struct SurfArgs {
     uint8_t* opt1d;   // nS
     float gridScale, shrink, aff;
     int depth;
+    int hexCorner8;   // hex ring scores over the 8 corner sub-tets
+    int ring2;        // quadric fits over {v}, one-ring and 2-ring
     // error bits (kSurfErr*), or-ed by the kernels; the host reads them with
     // the quality after every iteration
     int* err;
@@ -106,6 +110,10 @@ constexpr int kSurfErrNoSector = 1;
 // a feature vertex has neighbours after its last feature neighbour on an
 // open fan: they belong to no sector
 constexpr int kSurfErrUnsectored = 2;
+// a surface vertex's {v} + one-ring + 2-ring holds more than kMaxFitPoints
+// points (ring-2 fits only; reported by the init, which then fails)
+constexpr int kSurfErrFitPoints = 4;
+constexpr int kMaxFitPoints = 64;
 
 size_t surf_lds_bytes(int elem, int maxRing);
 void surf_init_launch(const SurfArgs& a, hipStream_t s);
@@ -143,6 +151,7 @@ struct BatchQArgs {
     float* hist;               // hcap rows of nMeshes minima
     int* hctr;
     int nChunks, nMeshes, hcap;
+    int hexCorner8;
 };
 
 size_t search_lds_bytes(int kind, int maxRing);

@@ -281,6 +281,7 @@ struct HexLds { float cd[4], cF[4]; HexP hp[5]; };
 struct HexFrame {
     float x0x, x0y, x0z, ex, ey, ez, sg, epsD, epsF;
     int n;
+    bool corner8;   // the principal-axes sub-tet (slot 4 of HexP) left out
     HexAcc acc0;
 };
 
@@ -293,6 +294,7 @@ __device__ HexFrame hex_stage(const SearchArgs& a, int vid, bool emin, HexLds* s
     const int r0 = a.ringRow[vid];
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
+    f.corner8 = a.hexCorner8 != 0;
     float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
     for (int e = tid; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 7;
@@ -349,6 +351,7 @@ __device__ __forceinline__ float hex_score(const HexFrame& f, const HexLds* s_hx
     for (int e = 0; e < f.n; ++e)
 #pragma unroll
         for (int sIdx = 0; sIdx < 5; ++sIdx) {
+            if (sIdx == 4 && f.corner8) break;
             const HexP h = s_hx[e].hp[sIdx];
             const bool a3 = (sIdx == 1 || sIdx == 4);
             float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
@@ -390,6 +393,7 @@ __device__ bool hex_levels(const SearchArgs& a, const HexFrame& f, const HexLds*
         for (int e = 0; e < f.n; ++e) {
 #pragma unroll
             for (int sIdx = 0; sIdx < 5; ++sIdx) {
+                if (sIdx == 4 && a.hexCorner8) break;   // uniform: a kernel argument
                 const HexP h = s_hx[e].hp[sIdx];
                 const bool a3 = (sIdx == 1 || sIdx == 4);
                 const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
@@ -1057,6 +1061,7 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
             for (int e = 0; e < f.n; ++e) {
     #pragma unroll
                 for (int sIdx = 0; sIdx < 5; ++sIdx) {
+                    if (sIdx == 4 && a.hexCorner8) break;
                     const HexP h = hx[e].hp[sIdx];
                     const bool a3 = (sIdx == 1 || sIdx == 4);
                     const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
@@ -1414,7 +1419,7 @@ __global__ __launch_bounds__(64 * W) void k_batch_round(SearchArgs a, const int*
 // ------------------------------------------------------------ qualities
 // element quality = ring score of the element seen from its vertex 0 at p'=0
 __device__ float elem_quality(const float* __restrict__ X, const float* __restrict__ Y, const float* __restrict__ Z,
-                              const int* c, int elem)
+                              const int* c, int elem, bool corner8)
 {
     const float x0x = X[c[0]], x0y = Y[c[0]], x0z = Z[c[0]];
     if (elem == VSDMO_TET) {
@@ -1462,7 +1467,7 @@ __device__ float elem_quality(const float* __restrict__ X, const float* __restri
         for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(acc, cd[sIdx], cF[sIdx]);
         const float px = 0.0f, py = 0.0f, pz = 0.0f;
         const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
-        for (int sIdx = 0; sIdx < 5; ++sIdx) {
+        for (int sIdx = 0; sIdx < (corner8 ? 4 : 5); ++sIdx) {
             const bool a3 = (sIdx == 1 || sIdx == 4);
             float Tyz = fmaf(hp[sIdx].gy, py, fmaf(hp[sIdx].gz, pz, hp[sIdx].D));
             float Ryz = fmaf(hp[sIdx].sy, py, fmaf(hp[sIdx].sz, pz, hp[sIdx].K + (a3 ? Q3 : Q1)));
@@ -1503,7 +1508,7 @@ __global__ __launch_bounds__(256) void k_elem_quality(QualityArgs a, float* __re
     float v = FLT_MAX;
     double sv = 0.0;
     if (e < a.nE) {
-        v = elem_quality(a.x, a.y, a.z, a.cells + (size_t)e * a.npe, a.elem);
+        v = elem_quality(a.x, a.y, a.z, a.cells + (size_t)e * a.npe, a.elem, a.hexCorner8 != 0);
         a.q[e] = v;
         sv = (double)v;
     }
@@ -1533,7 +1538,7 @@ __global__ void k_batch_quality(BatchQArgs a)
     float q = FLT_MAX;
     double s = 0.0;
     if ((int)threadIdx.x < ne) {
-        q = elem_quality(a.x, a.y, a.z, a.cells + a.chunkCell0[c] + (size_t)threadIdx.x * npe, elem);
+        q = elem_quality(a.x, a.y, a.z, a.cells + a.chunkCell0[c] + (size_t)threadIdx.x * npe, elem, a.hexCorner8 != 0);
         s = (double)q;
     }
     smin[threadIdx.x] = q;

@@ -189,11 +189,33 @@ __global__ void k_surf_vertex_init(SurfArgs a)
     } else {
         a.opt1d[v] = 0;
     }
-    // quadric fit over {v} and its one-ring (Surface.cu:11-105)
-    w3 pts[kMaxNeigh + 1];
+    // quadric fit over {v} and its one-ring (Surface.cu:11-105); ring 2 adds
+    // the one-ring's CCW targets not yet taken, neighbour by neighbour
+    w3 pts[kMaxFitPoints + 1];
     int np = 0;
     pts[np++] = pv;
     for (int h = a.heRow[v]; h < a.heRow[v + 1] && np < kMaxNeigh; ++h) pts[np++] = WP(a.x, a.y, a.z, a.heTarget[h]);
+    if (a.ring2) {
+        int ids[kMaxFitPoints];
+        int ni = 0;
+        ids[ni++] = v;
+        for (int h = a.heRow[v]; h < a.heRow[v + 1] && ni < np; ++h) ids[ni++] = a.heTarget[h];
+        const int n1 = ni;
+        bool over = false;
+        for (int k = 1; k < n1 && !over; ++k) {
+            const int d = ids[k];
+            for (int h = a.heRow[d]; h < a.heRow[d + 1]; ++h) {
+                const int t = a.heTarget[h];
+                bool seen = false;
+                for (int q = 0; q < ni && !seen; ++q) seen = ids[q] == t;
+                if (seen) continue;
+                if (ni == kMaxFitPoints) { over = true; break; }
+                ids[ni++] = t;
+                pts[np++] = WP(a.x, a.y, a.z, t);
+            }
+        }
+        if (over) atomicOr(a.err, kSurfErrFitPoints);
+    }
     if (np == 5) { pts[np] = wmul(0.5f, wadd(pts[2], pts[3])); ++np; }
     LSurf q;
     q.p0 = pts[0];
@@ -539,6 +561,7 @@ struct HexRingLds { float cd[4], cF[4]; HexP hp[5]; };
 struct RingFrame {
     float x0x, x0y, x0z, sg, is3, epsD, epsF, minDist, maxDist;
     int n;
+    int nsub;   // p-dependent hex sub-tets scored: 5, or 4 without the principal-axes tet
 };
 
 // stage the volume one-ring of vid in LDS (same constants as the inner
@@ -550,6 +573,7 @@ __device__ RingFrame stage_ring(const SurfArgs& a, int vid, bool tet, void* lds,
     const int r0 = a.ringRow[vid];
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
+    f.nsub = a.hexCorner8 ? 4 : 5;
     const int st = tet ? 3 : 7;
     float mabs = 0.0f, mn = FLT_MAX, mx = -FLT_MAX;
     for (int e = lane; e < f.n; e += 64) {
@@ -627,7 +651,7 @@ __device__ float ring_score(const RingFrame& f, bool tet, const void* lds, const
     const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
     HexAcc acc = hacc;
     for (int e = 0; e < f.n; ++e)
-        for (int s = 0; s < 5; ++s) {
+        for (int s = 0; s < f.nsub; ++s) {
             const HexP h = L[e].hp[s];
             const bool a3 = (s == 1 || s == 4);
             float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
