@@ -12,6 +12,7 @@
 // Every step is deterministic: sorts are counting sorts followed by a per-row
 // sort on the original index, so the output never depends on atomic order.
 #include <algorithm>
+#include <map>
 #include <numeric>
 
 #include "vsdmo_internal.h"
@@ -60,28 +61,89 @@ __global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ sums, 
     if (i < n) out[i] += sums[i / SCAN_CHUNK];
 }
 
+// Read-back slots of the build steps, per host thread and device: a few
+// device ints, the partial sums of the scans (grown on demand) and pinned
+// host ints, so that a scalar comes back with one small DMA and one
+// synchronisation.  Allocated on first use and kept for the life of the
+// process (freeing them at thread exit could run after the HIP runtime has
+// shut down).
+struct BuildSlots {
+    int* dev = nullptr;      // kSlots ints
+    int* host = nullptr;     // kSlots pinned ints
+    int* sums = nullptr;     // scan partial sums
+    size_t sumsCap = 0;
+};
+constexpr int kSlots = 16;
+
+static BuildSlots& build_slots(size_t sumsNeed = 0)
+{
+    thread_local std::map<int, BuildSlots> slots;
+    int d = 0;
+    VSDMO_HIP(hipGetDevice(&d));
+    BuildSlots& b = slots[d];
+    if (!b.dev) {
+        VSDMO_HIP(hipMalloc(&b.dev, kSlots * sizeof(int)));
+        VSDMO_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.host), kSlots * sizeof(int), hipHostMallocDefault));
+    }
+    if (sumsNeed > b.sumsCap) {
+        if (b.sums) VSDMO_HIP(hipFree(b.sums));
+        b.sums = nullptr;
+        const size_t cap = std::max<size_t>(sumsNeed, 4096);
+        VSDMO_HIP(hipMalloc(&b.sums, cap * sizeof(int)));
+        b.sumsCap = cap;
+    }
+    return b;
+}
+
+// one int of device memory, read back through the pinned slot
+int read_int(const int* d, hipStream_t s)
+{
+    BuildSlots& b = build_slots();
+    VSDMO_HIP(hipMemcpyAsync(b.host, d, sizeof(int), hipMemcpyDeviceToHost, s));
+    VSDMO_HIP(hipStreamSynchronize(s));
+    return b.host[0];
+}
+
+__global__ void k_copy_int(const int* __restrict__ src, int* __restrict__ dst) { *dst = *src; }
+__global__ void k_scan_total(const int* __restrict__ out, int n, const int* __restrict__ last, int* __restrict__ total)
+{
+    *total = out[n - 1] + *last;
+}
+
+// partial-sum ints the scan of n values needs over all its levels
+static size_t scan_sums_need(int n)
+{
+    size_t need = 0;
+    for (long long m = n; m > SCAN_CHUNK;) {
+        m = (m + SCAN_CHUNK - 1) / SCAN_CHUNK;
+        need += (size_t)m;
+    }
+    return need;
+}
+
+// exclusive scan on the device, partial sums of each level in sums
+static void scan_levels(const int* in, int* out, int n, int* sums, hipStream_t s)
+{
+    const int nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (nb == 1) {
+        k_scan_chunk<<<1, SCAN_TB, 0, s>>>(in, out, nullptr, n);
+        return;
+    }
+    k_scan_chunk<<<nb, SCAN_TB, 0, s>>>(in, out, sums, n);
+    scan_levels(sums, sums, nb, sums + nb, s);
+    k_scan_add<<<nblk(n), TB, 0, s>>>(out, sums, n);
+}
+
 // exclusive scan of n ints (out may alias in); returns the total
 int exclusive_scan(const int* in, int* out, int n, hipStream_t s)
 {
     if (n <= 0) return 0;
-    int last = 0;
-    VSDMO_HIP(hipMemcpyAsync(&last, in + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    VSDMO_HIP(hipStreamSynchronize(s));
-    int nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    if (nb == 1) {
-        k_scan_chunk<<<1, SCAN_TB, 0, s>>>(in, out, nullptr, n);
-    } else {
-        DevBuf<int> sums(nb);
-        k_scan_chunk<<<nb, SCAN_TB, 0, s>>>(in, out, sums.p, n);
-        exclusive_scan(sums.p, sums.p, nb, s);
-        k_scan_add<<<nblk(n), TB, 0, s>>>(out, sums.p, n);
-        VSDMO_HIP(hipStreamSynchronize(s));
-    }
+    BuildSlots& b = build_slots(scan_sums_need(n));
+    k_copy_int<<<1, 1, 0, s>>>(in + (n - 1), b.dev);   // before an in-place scan overwrites it
+    scan_levels(in, out, n, b.sums, s);
+    k_scan_total<<<1, 1, 0, s>>>(out, n, b.dev, b.dev + 1);
     VSDMO_HIP(hipGetLastError());
-    int lastOut = 0;
-    VSDMO_HIP(hipMemcpyAsync(&lastOut, out + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    VSDMO_HIP(hipStreamSynchronize(s));
-    return lastOut + last;
+    return read_int(b.dev + 1, s);
 }
 
 // ------------------------------------------------------ counting-sort CSR
@@ -308,41 +370,149 @@ struct Incidence {
 // sequential greedy colouring whatever the round in which a vertex is reached.
 __global__ void k_color_greedy_round(Incidence inc, int lo, int hi, int* __restrict__ col, int* __restrict__ left)
 {
-    int v = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= hi) return;
+    const int v = lo + blockIdx.x * blockDim.x + threadIdx.x;
     volatile int* vc = col;
-    if (vc[v - lo] >= 0) return;
-    int o = inc.oldOfCur[v];
-    uint64_t mask = 0;
-    bool big = false;
-    for (int k = inc.row[o]; k < inc.row[o + 1]; ++k) {
-        int e = inc.val[k] >> 3, c = inc.val[k] & 7;
-        for (int i = 0; i < inc.npe; ++i) {
-            if (i == c) continue;
-            int u = inc.cells[(size_t)e * inc.npe + i];
-            if (u < lo || u >= v) continue;
-            int cu = vc[u - lo];
-            if (cu < 0) { atomicAdd(left, 1); return; }
-            if (cu < 64) mask |= (1ull << cu); else big = true;
+    bool wait = false;
+    if (v < hi && vc[v - lo] < 0) {
+        const int o = inc.oldOfCur[v];
+        uint64_t mask = 0;
+        bool big = false;
+        for (int k = inc.row[o]; k < inc.row[o + 1] && !wait; ++k) {
+            const int e = inc.val[k] >> 3, c = inc.val[k] & 7;
+            for (int i = 0; i < inc.npe; ++i) {
+                if (i == c) continue;
+                const int u = inc.cells[(size_t)e * inc.npe + i];
+                if (u < lo || u >= v) continue;
+                const int cu = vc[u - lo];
+                if (cu < 0) { wait = true; break; }
+                if (cu < 64) mask |= (1ull << cu); else big = true;
+            }
         }
-    }
-    int cc = __builtin_ctzll(~mask);
-    if (big && cc == 64) {
-        // more than 64 colours in use around v: linear mex search
-        for (cc = 64;; ++cc) {
-            bool used = false;
-            for (int k = inc.row[o]; k < inc.row[o + 1] && !used; ++k) {
-                int e = inc.val[k] >> 3, c = inc.val[k] & 7;
-                for (int i = 0; i < inc.npe; ++i) {
-                    if (i == c) continue;
-                    int u = inc.cells[(size_t)e * inc.npe + i];
-                    if (u >= lo && u < v && vc[u - lo] == cc) { used = true; break; }
+        if (!wait) {
+            int cc = __builtin_ctzll(~mask);
+            if (big && cc == 64) {
+                // more than 64 colours in use around v: linear mex search
+                for (cc = 64;; ++cc) {
+                    bool used = false;
+                    for (int k = inc.row[o]; k < inc.row[o + 1] && !used; ++k) {
+                        const int e = inc.val[k] >> 3, c = inc.val[k] & 7;
+                        for (int i = 0; i < inc.npe; ++i) {
+                            if (i == c) continue;
+                            const int u = inc.cells[(size_t)e * inc.npe + i];
+                            if (u >= lo && u < v && vc[u - lo] == cc) { used = true; break; }
+                        }
+                    }
+                    if (!used) break;
                 }
             }
-            if (!used) break;
+            vc[v - lo] = cc;
         }
     }
-    vc[v - lo] = cc;
+    // one plain store per wave with a waiting vertex (no atomic per vertex)
+    if (__ballot(wait) != 0 && (threadIdx.x & 63) == 0) *left = 1;
+}
+
+// Lower neighbours of each vertex v of [lo, hi): the in-range ids u < v of
+// the corners of v's elements, as offsets from lo (an id appears once per
+// element shared with v), for the single-pass colouring below
+__global__ void k_lower_count(Incidence inc, int lo, int hi, int* __restrict__ cnt)
+{
+    const int v = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= hi) return;
+    const int o = inc.oldOfCur[v];
+    int c = 0;
+    for (int k = inc.row[o]; k < inc.row[o + 1]; ++k) {
+        const int e = inc.val[k] >> 3, cr = inc.val[k] & 7;
+        for (int i = 0; i < inc.npe; ++i) {
+            if (i == cr) continue;
+            const int u = inc.cells[(size_t)e * inc.npe + i];
+            c += (u >= lo && u < v) ? 1 : 0;
+        }
+    }
+    cnt[v - lo] = c;
+}
+
+__global__ void k_lower_fill(Incidence inc, int lo, int hi, const int* __restrict__ row, int* __restrict__ val)
+{
+    const int v = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= hi) return;
+    const int o = inc.oldOfCur[v];
+    int w = row[v - lo];
+    for (int k = inc.row[o]; k < inc.row[o + 1]; ++k) {
+        const int e = inc.val[k] >> 3, cr = inc.val[k] & 7;
+        for (int i = 0; i < inc.npe; ++i) {
+            if (i == cr) continue;
+            const int u = inc.cells[(size_t)e * inc.npe + i];
+            if (u >= lo && u < v) val[w++] = u - lo;
+        }
+    }
+}
+
+// The greedy first-fit colouring of k_color_greedy_round in one launch: each
+// vertex polls the colours of its lower neighbours until all are set, then
+// takes the smallest colour none of them uses.  A colour depends only on the
+// final colours of the lower neighbours, so the result is the sequential
+// greedy colouring.  A vertex waits only for lower vertices: those of its own
+// wave (the wave loops until all its lanes are done, so the lowest waiting
+// lane always finds its neighbours set) and those of lower workgroups, which
+// are dispatched before it.  The wait is bounded all the same: a vertex that
+// has polled spinLimit times stops uncoloured and raises *gaveUp, and the
+// round kernel completes the colouring with the same colours.  Colours are
+// loaded and stored at device scope, so that a poll sees a store made on
+// another XCD.
+__global__ __launch_bounds__(256) void k_color_greedy_sweep(const int* __restrict__ row, const int* __restrict__ val,
+                                                           int n, int* __restrict__ col, int spinLimit,
+                                                           int* __restrict__ gaveUp)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    bool todo = v < n;
+    const int r1 = todo ? row[v + 1] : 0;
+    int k = todo ? row[v] : 0;        // lower neighbours [row[v], k) are coloured and in mask
+    uint64_t mask = 0;
+    bool big = false;
+    int polls = 0;
+    constexpr int B = 8;   // list entries loaded together (their latencies overlap)
+    while (__ballot(todo) != 0) {
+        if (todo) {
+            bool blocked = false;
+            while (k < r1 && !blocked) {
+                int id[B], cu[B];
+                // entries past the list re-read its last one (k < r1), so every
+                // load is unconditional and in bounds; their values are ignored
+#pragma unroll
+                for (int j = 0; j < B; ++j) id[j] = val[min(k + j, r1 - 1)];
+#pragma unroll
+                for (int j = 0; j < B; ++j) cu[j] = __hip_atomic_load(col + id[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int done = 0;   // entries taken in order before the first uncoloured one
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    if (blocked || k + j >= r1) continue;
+                    if (cu[j] < 0) { blocked = true; continue; }
+                    if (cu[j] < 64) mask |= 1ull << cu[j]; else big = true;
+                    ++done;
+                }
+                k += done;
+            }
+            if (k == r1) {
+                int cc = (~mask == 0ull) ? 64 : __builtin_ctzll(~mask);
+                if (big && cc == 64) {
+                    // more than 64 colours around v: linear mex search
+                    for (;; ++cc) {
+                        bool used = false;
+                        for (int j = row[v]; j < r1 && !used; ++j)
+                            used = __hip_atomic_load(col + val[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == cc;
+                        if (!used) break;
+                    }
+                }
+                __hip_atomic_store(col + v, cc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                todo = false;
+            } else if (++polls >= spinLimit) {
+                __hip_atomic_store(gaveUp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                todo = false;
+            }
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
 }
 
 // Reference color_jpl round c (VertexColoring.cu:76-105): v takes colour c iff
@@ -350,24 +520,26 @@ __global__ void k_color_greedy_round(Incidence inc, int lo, int hi, int* __restr
 // a lower priority n-u, i.e. a larger index.
 __global__ void k_color_jpl_round(Incidence inc, int lo, int hi, int c, int* __restrict__ col, int* __restrict__ left)
 {
-    int v = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= hi) return;
+    const int v = lo + blockIdx.x * blockDim.x + threadIdx.x;
     volatile int* vc = col;
-    if (vc[v - lo] != -1) return;
-    int o = inc.oldOfCur[v];
-    bool ok = true;
-    for (int k = inc.row[o]; k < inc.row[o + 1] && ok; ++k) {
-        int e = inc.val[k] >> 3, cr = inc.val[k] & 7;
-        for (int i = 0; i < inc.npe; ++i) {
-            if (i == cr) continue;
-            int u = inc.cells[(size_t)e * inc.npe + i];
-            if (u < lo || u >= hi || u == v) continue;
-            int cu = vc[u - lo];
-            if ((cu != -1) && (cu != c)) continue;
-            if (!(v < u)) { ok = false; break; }
+    bool wait = false;
+    if (v < hi && vc[v - lo] == -1) {
+        const int o = inc.oldOfCur[v];
+        bool ok = true;
+        for (int k = inc.row[o]; k < inc.row[o + 1] && ok; ++k) {
+            const int e = inc.val[k] >> 3, cr = inc.val[k] & 7;
+            for (int i = 0; i < inc.npe; ++i) {
+                if (i == cr) continue;
+                const int u = inc.cells[(size_t)e * inc.npe + i];
+                if (u < lo || u >= hi || u == v) continue;
+                const int cu = vc[u - lo];
+                if ((cu != -1) && (cu != c)) continue;
+                if (!(v < u)) { ok = false; break; }
+            }
         }
+        if (ok) vc[v - lo] = c; else wait = true;
     }
-    if (ok) vc[v - lo] = c; else atomicAdd(left, 1);
+    if (__ballot(wait) != 0 && (threadIdx.x & 63) == 0) *left = 1;
 }
 
 __global__ void k_max_reduce(const int* __restrict__ a, int n, int* __restrict__ out)
@@ -376,29 +548,50 @@ __global__ void k_max_reduce(const int* __restrict__ a, int n, int* __restrict__
     if (i < n) atomicMax(out, a[i]);
 }
 
-__global__ void k_eq_flag(const int* __restrict__ a, int n, int val, int* __restrict__ flag)
+// colours of one sort at most (the per-block counts live in LDS)
+constexpr int kMaxSortColours = 16384;
+
+// per block of TB vertices of [0, n): the count of each colour, stored
+// colour-major (hist[c * nb + b]) so that one exclusive scan gives every
+// (colour, block) its first position in the colour-sorted order
+__global__ void k_colour_hist(const int* __restrict__ col, int n, int nc, int nb, int* __restrict__ hist)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = (a[i] == val) ? 1 : 0;
+    extern __shared__ int sh[];
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) sh[c] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&sh[col[i]], 1);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) hist[(size_t)c * nb + blockIdx.x] = sh[c];
 }
 
-__global__ void k_colour_newid(const int* __restrict__ col, const int* __restrict__ rank, int n, int val, int base,
-                               int* __restrict__ newid)
+// new id of vertex lo + i: lo + the scanned first position of (its colour,
+// its block) + the vertices of its colour before it in the block
+__global__ __launch_bounds__(TB) void k_colour_place(const int* __restrict__ col, int n, int nb,
+                                                     const int* __restrict__ pos, int lo, int* __restrict__ newid)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && col[i] == val) newid[base + i] = rank[i];
+    __shared__ int sc[TB];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (i < n) ? col[i] : -1;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    if (i >= n) return;
+    int r = 0;
+    for (int j = 0; j < (int)threadIdx.x; ++j) r += (sc[j] == c) ? 1 : 0;
+    newid[lo + i] = lo + pos[(size_t)c * nb + blockIdx.x] + r;
+}
+
+// first position of each colour: off[c] = pos[c * nb]
+__global__ void k_colour_offsets(const int* __restrict__ pos, int nb, int nc, int* __restrict__ off)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) off[c] = pos[(size_t)c * nb];
 }
 
 __global__ void k_iota(int* __restrict__ a, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = i;
-}
-
-__global__ void k_add_offset(int* __restrict__ a, int n, int off)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] += off;
 }
 
 __global__ void k_fill_range(int* __restrict__ a, int lo, int n, const int* __restrict__ col)
@@ -447,9 +640,7 @@ void check_classes(const vsdmo_mesh& m, const std::vector<int>& off)
     bad.upload(&none, 1, s);
     k_check_classes<<<nblk(m.nE), TB, 0, s>>>(m.cells.p, m.nE, m.npe, doff.p, nc, bad.p);
     VSDMO_HIP(hipGetLastError());
-    int h = none;
-    bad.download(&h, 1, s);
-    VSDMO_HIP(hipStreamSynchronize(s));
+    const int h = read_int(bad.p, s);
     if (h != none)
         throw StatusError{ VSDMO_TOPOLOGY, "colour class " + std::to_string(h) + " (vertices " +
                                                std::to_string(off[h]) + ".." + std::to_string(off[h + 1] - 1) +
@@ -458,6 +649,11 @@ void check_classes(const vsdmo_mesh& m, const std::vector<int>& off)
 
 // colour [lo,hi) of the current numbering and sort it stably by colour.
 // Returns absolute colour offsets, checked to be independent sets.
+// polls of a vertex in the single-pass greedy colouring before it leaves its
+// colour to the rounds (a wait of ~0.1 s at most; the chains of the meshes
+// here need a few hundred)
+constexpr int kSweepPolls = 1 << 16;
+
 std::vector<int> colour_and_sort(vsdmo_mesh& m, Renumber& rn, const int* incRow, const int* incVal, int lo, int hi,
                                  int mode, std::vector<DevBuf<uint8_t>*> u8)
 {
@@ -465,45 +661,60 @@ std::vector<int> colour_and_sort(vsdmo_mesh& m, Renumber& rn, const int* incRow,
     int n = hi - lo;
     if (n <= 0) return { lo };
     Incidence inc{ incRow, incVal, rn.oldOfCur.p, m.cells.p, m.npe };
-    DevBuf<int> col(n), left(1);
-    VSDMO_HIP(hipMemsetAsync(col.p, 0xff, sizeof(int) * n, s));
-    // rounds go out in batches of kRoundsPerCheck and the host reads the
-    // count of the batch's last round: a round after the colouring is
-    // complete finds every vertex coloured and changes nothing, so the colours
-    // are those of one check per round, with 1/kRoundsPerCheck of the syncs
+    // rounds go out in batches of kRoundsPerCheck, each round raising its own
+    // flag when a vertex still waits, and the host reads the flag of the
+    // batch's last round: a round after the colouring is complete finds every
+    // vertex coloured and changes nothing, so the colours are those of one
+    // check per round, with 1/kRoundsPerCheck of the syncs
     constexpr int kRoundsPerCheck = 16;
-    int h_left = n;
-    for (int round = 0; h_left > 0;) {
+    DevBuf<int> col(n), left(kRoundsPerCheck);
+    VSDMO_HIP(hipMemsetAsync(col.p, 0xff, sizeof(int) * n, s));
+    int h_left = 1;
+    if (mode != VSDMO_COLOR_JPL) {
+        // greedy: one sweep over lower-neighbour lists; the rounds below only
+        // run if a vertex ran out of polls
+        DevBuf<int> lrow((size_t)n + 1), gave(1);
+        k_lower_count<<<nblk(n), TB, 0, s>>>(inc, lo, hi, lrow.p);
+        VSDMO_HIP(hipMemsetAsync(lrow.p + n, 0, sizeof(int), s));
+        const int tot = exclusive_scan(lrow.p, lrow.p, n + 1, s);
+        DevBuf<int> lval((size_t)std::max(tot, 1));
+        k_lower_fill<<<nblk(n), TB, 0, s>>>(inc, lo, hi, lrow.p, lval.p);
+        VSDMO_HIP(hipMemsetAsync(gave.p, 0, sizeof(int), s));
+        k_color_greedy_sweep<<<nblk(n), TB, 0, s>>>(lrow.p, lval.p, n, col.p, kSweepPolls, gave.p);
+        VSDMO_HIP(hipGetLastError());
+        h_left = read_int(gave.p, s);
+    }
+    for (int round = 0; h_left != 0;) {
         if (round > n + 1) throw StatusError{ VSDMO_TOPOLOGY, "colouring did not converge" };
+        VSDMO_HIP(hipMemsetAsync(left.p, 0, sizeof(int) * kRoundsPerCheck, s));
         for (int k = 0; k < kRoundsPerCheck; ++k, ++round) {
-            VSDMO_HIP(hipMemsetAsync(left.p, 0, sizeof(int), s));
             if (mode == VSDMO_COLOR_JPL)
-                k_color_jpl_round<<<nblk(n), TB, 0, s>>>(inc, lo, hi, round, col.p, left.p);
+                k_color_jpl_round<<<nblk(n), TB, 0, s>>>(inc, lo, hi, round, col.p, left.p + k);
             else
-                k_color_greedy_round<<<nblk(n), TB, 0, s>>>(inc, lo, hi, col.p, left.p);
+                k_color_greedy_round<<<nblk(n), TB, 0, s>>>(inc, lo, hi, col.p, left.p + k);
         }
         VSDMO_HIP(hipGetLastError());
-        VSDMO_HIP(hipMemcpyAsync(&h_left, left.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        VSDMO_HIP(hipStreamSynchronize(s));
+        h_left = read_int(left.p + kRoundsPerCheck - 1, s);
     }
     DevBuf<int> mx(1);
     VSDMO_HIP(hipMemsetAsync(mx.p, 0, sizeof(int), s));
     k_max_reduce<<<nblk(n), TB, 0, s>>>(col.p, n, mx.p);
-    int nc = 0;
-    VSDMO_HIP(hipMemcpyAsync(&nc, mx.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    VSDMO_HIP(hipStreamSynchronize(s));
-    nc += 1;
-    // stable counting sort by colour: newid = lo + offset[c] + rank within c
-    DevBuf<int> newid(m.nV), flag(n), rank(n);
+    const int nc = read_int(mx.p, s) + 1;
+    if (nc > kMaxSortColours) throw StatusError{ VSDMO_NOT_SUPPORTED, "more than 16384 colours" };
+    // stable counting sort by colour: per block of TB vertices the count of
+    // each colour, scanned colour-major, gives each (colour, block) its first
+    // position; a vertex adds the same-coloured vertices before it in its block
+    const int nb = nblk(n);
+    DevBuf<int> newid(m.nV), pos((size_t)nc * nb), doff(nc);
     k_iota<<<nblk(m.nV), TB, 0, s>>>(newid.p, m.nV);
+    k_colour_hist<<<nb, TB, sizeof(int) * nc, s>>>(col.p, n, nc, nb, pos.p);
+    exclusive_scan(pos.p, pos.p, nc * nb, s);
+    k_colour_place<<<nb, TB, 0, s>>>(col.p, n, nb, pos.p, lo, newid.p);
+    k_colour_offsets<<<nblk(nc), TB, 0, s>>>(pos.p, nb, nc, doff.p);
     std::vector<int> off(nc + 1, 0);
-    for (int c = 0; c < nc; ++c) {
-        k_eq_flag<<<nblk(n), TB, 0, s>>>(col.p, n, c, flag.p);
-        int cnt = exclusive_scan(flag.p, rank.p, n, s);
-        k_add_offset<<<nblk(n), TB, 0, s>>>(rank.p, n, lo + off[c]);
-        k_colour_newid<<<nblk(n), TB, 0, s>>>(col.p, rank.p, n, c, lo, newid.p);
-        off[c + 1] = off[c] + cnt;
-    }
+    doff.download(off.data(), nc, s);
+    VSDMO_HIP(hipStreamSynchronize(s));
+    off[nc] = n;
     VSDMO_HIP(hipGetLastError());
     k_fill_range<<<nblk(n), TB, 0, s>>>(m.colorOf.p, lo, n, col.p);
     std::vector<DevBuf<int>*> i32 = { &m.colorOf };
@@ -629,9 +840,8 @@ void build_volume(vsdmo_mesh& m, const vsdmo_mesh_params& prm)
         DevBuf<int> mx(1);
         VSDMO_HIP(hipMemsetAsync(mx.p, 0, sizeof(int), s));
         k_max_reduce<<<nblk(nV), TB, 0, s>>>(cnt.p, nV, mx.p);
-        VSDMO_HIP(hipMemcpyAsync(&m.maxRing, mx.p, sizeof(int), hipMemcpyDeviceToHost, s));
         VSDMO_HIP(hipGetLastError());
-        VSDMO_HIP(hipStreamSynchronize(s));
+        m.maxRing = read_int(mx.p, s);
     }
 }
 
@@ -966,9 +1176,7 @@ void build_surface(vsdmo_mesh& m, Renumber& rn, const SurfaceInputs& si, const v
         k_ccw<<<nblk(nS), TB, 0, s>>>(m.heRow.p, nS, sTgt.p, sFace.p, sOpp.p, m.isB1D.p, m.isFeature.p, m.faces.p, k,
                                       order.p, err.p);
         {
-            int herr = 0;
-            VSDMO_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            VSDMO_HIP(hipStreamSynchronize(s));
+            const int herr = read_int(err.p, s);
             if (herr)
                 throw StatusError{ VSDMO_TOPOLOGY, "surface halfedge fan walk failed (non-manifold surface?) code " +
                                                        std::to_string(herr) };
@@ -980,8 +1188,7 @@ void build_surface(vsdmo_mesh& m, Renumber& rn, const SurfaceInputs& si, const v
         VSDMO_HIP(hipMemsetAsync(mx.p, 0, sizeof(int), s));
         k_row_sizes<<<nblk(nS), TB, 0, s>>>(m.heRow.p, nS, mx.p);
         VSDMO_HIP(hipGetLastError());
-        VSDMO_HIP(hipMemcpyAsync(&m.maxHE, mx.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        VSDMO_HIP(hipStreamSynchronize(s));
+        m.maxHE = read_int(mx.p, s);
         m.nHE = tot;
     }
 }
