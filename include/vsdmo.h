@@ -75,6 +75,9 @@ typedef struct vsdmo_opt vsdmo_opt;
 typedef struct vsdmo_mesh_params {
     int32_t coloring;          /* vsdmo_coloring */
     float feature_angle_deg;   /* 20 (reference mesh/MeshTriGPU.cu:506) */
+    int32_t coloring_polls;    /* greedy colouring: the polls of its lower neighbours' colours a
+                                  vertex makes in the one-launch colouring before colouring
+                                  rounds take over; 0 = 65536; the colours do not depend on it */
 } vsdmo_mesh_params;
 
 typedef struct vsdmo_params {

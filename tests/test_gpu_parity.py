@@ -66,6 +66,21 @@ def test_flat_tri_mesh_build_bitexact(oracle):
     _compare_structures(g, o, "tri")
 
 
+@pytest.mark.parametrize("polls", [1, 3])
+def test_colouring_independent_of_the_poll_limit(oracle, polls):
+    """the one-launch greedy colouring leaves the vertices that run out of
+    polls to the colouring rounds: with one or three polls per vertex (most
+    vertices handed over) the colours, the colour sort and every structure
+    equal the default build's and the oracle's"""
+    for kind, (xyz, cells) in (("tet", synth.tet_cube(10, seed=81)), ("hex", synth.hex_cube(7, seed=82)),
+                               ("tri", synth.tri_grid(30, seed=83))):
+        fac = {"tet": DMOMeshTetFactory, "hex": DMOMeshHexFactory, "tri": DMOMeshTriFactory}[kind]
+        g = fac.create(xyz, cells, coloring_polls=polls)
+        o = oracle.OracleMesh(kind, xyz, cells)
+        _compare_structures(g, o, kind)
+        g.close()
+
+
 @pytest.mark.parametrize("kind", ["tet", "hex", "tri"])
 def test_element_quality_bitexact(oracle, kind):
     gen = {"tet": lambda: synth.tet_cube(9, seed=31, jitter=0.45),

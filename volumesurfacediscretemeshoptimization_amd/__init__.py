@@ -50,7 +50,8 @@ class DMOMesh:
 
     NPE = {_lib.VSDMO_TRI: 3, _lib.VSDMO_QUAD: 4, _lib.VSDMO_TET: 4, _lib.VSDMO_HEX: 8}
 
-    def __init__(self, elem: int, xyz, cells, coloring: int = _lib.VSDMO_COLOR_GREEDY, feature_angle: float = 20.0):
+    def __init__(self, elem: int, xyz, cells, coloring: int = _lib.VSDMO_COLOR_GREEDY, feature_angle: float = 20.0,
+                 coloring_polls: int = 0):
         lib = load()
         self._lib = lib
         self.elem = elem
@@ -61,7 +62,7 @@ class DMOMesh:
         npe = self.NPE[elem]
         if cells.ndim != 2 or cells.shape[1] != npe:
             raise ValueError(f"cells must have shape (nE, {npe})")
-        prm = _lib.MeshParams(coloring, feature_angle)
+        prm = _lib.MeshParams(coloring, feature_angle, coloring_polls)
         h = C.c_void_p()
         check(lib.vsdmo_mesh_create(elem, _fptr(xyz), xyz.shape[0], _iptr(cells), cells.shape[0], None,
                                     C.byref(prm), C.byref(h)), "vsdmo_mesh_create")

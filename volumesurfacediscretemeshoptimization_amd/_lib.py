@@ -27,7 +27,7 @@ VSDMO_HEX_SHAPE9, VSDMO_HEX_CORNER8 = 0, 1
 
 
 class MeshParams(C.Structure):
-    _fields_ = [("coloring", C.c_int32), ("feature_angle_deg", C.c_float)]
+    _fields_ = [("coloring", C.c_int32), ("feature_angle_deg", C.c_float), ("coloring_polls", C.c_int32)]
 
 
 class Params(C.Structure):

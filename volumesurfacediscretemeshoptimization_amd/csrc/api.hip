@@ -150,6 +150,7 @@ void vsdmo_mesh_params_default(vsdmo_mesh_params* p)
     if (!p) return;
     p->coloring = VSDMO_COLOR_GREEDY;
     p->feature_angle_deg = 20.0f;
+    p->coloring_polls = 0;
 }
 
 void vsdmo_params_default(vsdmo_params* p)
@@ -196,6 +197,7 @@ vsdmo_status vsdmo_mesh_create(vsdmo_elem type, const float* xyz, int32_t nV, co
         if (params) prm = *params;
         if (prm.coloring != VSDMO_COLOR_GREEDY && prm.coloring != VSDMO_COLOR_JPL)
             return fail(VSDMO_INVALID_ARG, "unknown colouring mode");
+        if (prm.coloring_polls < 0) return fail(VSDMO_INVALID_ARG, "coloring_polls must be >= 0");
 
         auto* m = new vsdmo_mesh();
         try {

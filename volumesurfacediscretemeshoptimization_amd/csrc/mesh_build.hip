@@ -649,17 +649,19 @@ void check_classes(const vsdmo_mesh& m, const std::vector<int>& off)
 
 // colour [lo,hi) of the current numbering and sort it stably by colour.
 // Returns absolute colour offsets, checked to be independent sets.
-// polls of a vertex in the single-pass greedy colouring before it leaves its
-// colour to the rounds (a wait of ~0.1 s at most; the chains of the meshes
-// here need a few hundred)
+// default polls of a vertex in the single-pass greedy colouring before it
+// leaves its colour to the rounds (a wait of ~0.1 s at most; the chains of
+// the meshes here need a few hundred)
 constexpr int kSweepPolls = 1 << 16;
 
 std::vector<int> colour_and_sort(vsdmo_mesh& m, Renumber& rn, const int* incRow, const int* incVal, int lo, int hi,
-                                 int mode, std::vector<DevBuf<uint8_t>*> u8)
+                                 const vsdmo_mesh_params& prm, std::vector<DevBuf<uint8_t>*> u8)
 {
     hipStream_t s = m.stream;
     int n = hi - lo;
     if (n <= 0) return { lo };
+    const int mode = prm.coloring;
+    const int polls = prm.coloring_polls > 0 ? prm.coloring_polls : kSweepPolls;
     Incidence inc{ incRow, incVal, rn.oldOfCur.p, m.cells.p, m.npe };
     // rounds go out in batches of kRoundsPerCheck, each round raising its own
     // flag when a vertex still waits, and the host reads the flag of the
@@ -680,7 +682,7 @@ std::vector<int> colour_and_sort(vsdmo_mesh& m, Renumber& rn, const int* incRow,
         DevBuf<int> lval((size_t)std::max(tot, 1));
         k_lower_fill<<<nblk(n), TB, 0, s>>>(inc, lo, hi, lrow.p, lval.p);
         VSDMO_HIP(hipMemsetAsync(gave.p, 0, sizeof(int), s));
-        k_color_greedy_sweep<<<nblk(n), TB, 0, s>>>(lrow.p, lval.p, n, col.p, kSweepPolls, gave.p);
+        k_color_greedy_sweep<<<nblk(n), TB, 0, s>>>(lrow.p, lval.p, n, col.p, polls, gave.p);
         VSDMO_HIP(hipGetLastError());
         h_left = read_int(gave.p, s);
     }
@@ -820,7 +822,7 @@ void build_volume(vsdmo_mesh& m, const vsdmo_mesh_params& prm)
         m.nInner = nV - nS;
     }
     // inner colouring and colour sort
-    m.colOffInner = colour_and_sort(m, rn, incRow.p, incVal.p, m.nSurf, nV, prm.coloring, {});
+    m.colOffInner = colour_and_sort(m, rn, incRow.p, incVal.p, m.nSurf, nV, prm, {});
 
     // surface: features, [free | feature], colouring, halfedges
     SurfaceInputs si{ incRow.p, incVal.p };
@@ -1153,8 +1155,8 @@ void build_surface(vsdmo_mesh& m, Renumber& rn, const SurfaceInputs& si, const v
         m.nFree = nFree;
         m.nFeature = nFeat;
     }
-    m.colOffFree = colour_and_sort(m, rn, si.incRow, si.incVal, 0, m.nFree, prm.coloring, { &m.isB1D, &m.isFeature });
-    m.colOffFeature = colour_and_sort(m, rn, si.incRow, si.incVal, m.nFree, nS, prm.coloring, { &m.isB1D, &m.isFeature });
+    m.colOffFree = colour_and_sort(m, rn, si.incRow, si.incVal, 0, m.nFree, prm, { &m.isB1D, &m.isFeature });
+    m.colOffFeature = colour_and_sort(m, rn, si.incRow, si.incVal, m.nFree, nS, prm, { &m.isB1D, &m.isFeature });
     // halfedges on the final numbering
     {
         DevBuf<int> key(nRaw), row, items, opp(nRaw), noopp(nRaw), dpos(nRaw);
