@@ -729,8 +729,9 @@ def north_star_mesh(name, args):
         wo.close()
         wm.close()
     # plain, timed (no per-launch events), then its kernel time with events
+    tb = time.perf_counter()
     mp = fac.create(xyz, cells)
-    build_t0 = time.perf_counter()
+    build_s = time.perf_counter() - tb
     op = cls(mp, n_iter=K, **kw)
     q0 = op.getQuality()
     t0 = time.perf_counter()
@@ -741,7 +742,6 @@ def north_star_mesh(name, args):
     plain_updates = stp["searches_inner"] + stp["searches_surface"] + stp["searches_feature"]
     op.close()
     mp.close()
-    del build_t0
     mk = fac.create(xyz, cells)
     ok = cls(mk, n_iter=K, **kw)
     ok.set_timing(True)
@@ -788,7 +788,8 @@ def north_star_mesh(name, args):
         "plain": {"iterations": K, "ms_per_step": 1e3 * dt_plain / K, "value": plain_updates / dt_plain,
                   "min_quality_start": q0, "min_quality": qp, "mean_quality": qmp,
                   "kernel": wl["kernel"], "kernel_mean_launch_us": plain_kernel_us,
-                  "kernel_tflops_canonical": plain_flop_launch / (plain_kernel_us * 1e-6) / 1e12},
+                  "kernel_tflops_canonical": plain_flop_launch / (plain_kernel_us * 1e-6) / 1e12,
+                  "mesh_build_ms": 1e3 * build_s},
         "learned": {"mode": mode, "window": [depth, frac], "fallback_policy": policy,
                     "iterations": lrn["iterations"],
                     "ms_per_learned_iteration": 1e3 * lrn["seconds"] / lrn["iterations"],
