@@ -74,3 +74,31 @@ def test_invalid_arguments_rejected_without_gpu_work():
                                None, None, ctypes.byref(h))
     assert st == _lib.VSDMO_INVALID_ARG
     assert lib.vsdmo_opt_create(None, 0, 0.5, 1, None, ctypes.byref(h)) == _lib.VSDMO_INVALID_ARG
+
+
+def test_struct_layouts_match_the_header():
+    """the ctypes mirrors of the ABI structs have the C layout: the size and
+    every field offset, as a C compiler lays out include/vsdmo.h"""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    from volumesurfacediscretemeshoptimization_amd import _lib
+    pairs = {"vsdmo_mesh_params": _lib.MeshParams, "vsdmo_params": _lib.Params, "vsdmo_counts": _lib.Counts,
+             "vsdmo_stats": _lib.Stats, "vsdmo_batch_params": _lib.BatchParams,
+             "vsdmo_batch_stats": _lib.BatchStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vsdmo.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src, exe = "/tmp/vsdmo_layout.c", "/tmp/vsdmo_layout"
+    open(src, "w").write("\n".join(lines) + "\n")
+    r = subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), src, "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in pairs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
