@@ -123,6 +123,32 @@ def test_dmo_positions_parity(oracle, kind, n, iters):
     opt.close()
 
 
+def test_flat_four_vertices_per_wave_parity(oracle):
+    """a 160^2 grid: colour classes of ~6,200 vertices take the flat search
+    with 4 vertices per wave (classes of 4,096 and more); positions bit-exact
+    with the oracle after each iteration, and with one wave per vertex
+    (waves_per_vertex 2 runs the one-vertex-per-block search)"""
+    xyz, tris = synth.tri_grid(160, seed=43)
+    g = _build("tri", xyz, tris)
+    c = g.counts()
+    off = g.color_offsets("free")
+    assert np.diff(off).max() >= 4096, off
+    opt = DMOTriFlatClass(g, n_iter=2, fixed_boundary=True)
+    o = oracle.OracleMesh("tri", xyz, tris)
+    g2 = _build("tri", xyz, tris)
+    opt2 = DMOTriFlatClass(g2, n_iter=2, fixed_boundary=True, waves_per_vertex=2)
+    for it in range(2):
+        opt.doIteration()
+        opt2.doIteration()
+        o.iterate()
+        gp = g.positions(original_order=False)
+        assert np.array_equal(gp.view(np.uint32), o.positions().view(np.uint32)), it
+        assert np.array_equal(gp, g2.positions(original_order=False)), it
+    assert c["n_free"] > 20000
+    opt.close()
+    opt2.close()
+
+
 @pytest.mark.parametrize("w", [1, 4])
 def test_hex_pinned_vertices_parity(oracle, w):
     """Jitter 0.45h inverts hexes: a vertex whose ring is held at quality 0

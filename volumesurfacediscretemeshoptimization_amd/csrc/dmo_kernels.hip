@@ -673,6 +673,149 @@ __device__ __forceinline__ void group_argmax(float& best, int& idx)
     }
 }
 
+// The flat search of flat_vertex at nq = 8 for 4 vertices per wave.  The 64
+// candidate slots of a vertex are the lanes of the one-wave layout (thread
+// t = slot / 2 holds B = (t / 8 + 4, t % 8) at even and A = (t / 8, t % 8) at
+// odd slots); here they are spread 4 per lane over a group of 16 lanes.  Each
+// lane keeps its first best slot and the group argmax takes the larger
+// score, then the lower slot: the wave argmax of the one-wave layout, so
+// every decision and every result is the same.  The two slots of a thread
+// share the (y) part of each element.  vid < 0: an idle group (no writes);
+// s_tr: the group's cap TriC entries of LDS.
+constexpr int kFlatGroups = 16;   // vertices per 256-thread block
+
+__device__ void flat_group8(const SearchArgs& a, int vid, TriC* s_tr)
+{
+    constexpr int NQ = 8, G = 16;
+    const int s = (int)(threadIdx.x & (G - 1));
+    const bool live = vid >= 0;
+    const int h0 = live ? a.heRow[vid] : 0;
+    const int n = live ? a.heRow[vid + 1] - h0 : 0;
+    const float x0x = live ? a.x[vid] : 0.0f, x0y = live ? a.y[vid] : 0.0f;
+    float ex = -FLT_MAX, ey = -FLT_MAX, mabs = 0.0f;
+    for (int e = s; e < n; e += G) {
+        const int v = a.heTarget[h0 + e];
+        const float dx = fabsf(a.x[v] - x0x), dy = fabsf(a.y[v] - x0y);
+        ex = fmaxf(ex, dx);
+        ey = fmaxf(ey, dy);
+        mabs = fmaxf(mabs, fmaxf(dx, dy));
+    }
+    ex = group_max<G>(ex);
+    ey = group_max<G>(ey);
+    mabs = group_max<G>(mabs);
+    const float sg = pow2_scale(mabs);
+    const float isg = 1.0f / sg;
+    const float is2 = (1.0f * isg) * isg;
+    for (int e = s; e < n; e += G) {
+        const int va = a.heTarget[h0 + e];
+        const int vb = a.heTarget[h0 + (e + 1 == n ? 0 : e + 1)];
+        s_tr[e] = tri_consts((a.x[va] - x0x) * sg, (a.y[va] - x0y) * sg, (a.x[vb] - x0x) * sg, (a.y[vb] - x0y) * sg);
+    }
+    __syncthreads();
+
+    float qcur;
+    {
+        TriAcc acc = tri_acc0();
+        const float px = 0.0f, py = 0.0f, Q2 = 2.0f * (py * py), t2 = 2.0f * px;
+        for (int e = 0; e < n; ++e) {
+            const TriC c = s_tr[e];
+            const float Ty = fmaf(c.gy, py, c.C);
+            const float Ry = fmaf(c.sy, py, c.K + Q2);
+            tri_acc_add(acc, fmaf(c.gx, px, Ty), fmaf(px, t2 + c.sx, Ry));
+        }
+        qcur = tri_acc_score(acc, is2);
+    }
+    const float qold = qcur;
+    // slots 4s + i: threads t0 = 2s (i = 0 B, 1 A) and t1 = 2s + 1 (i = 2 B, 3 A)
+    const int t0 = 2 * s, t1 = 2 * s + 1;
+    const int cj[2] = { t0 % NQ, t1 % NQ };
+    const int ci[4] = { t0 / NQ + NQ / 2, t0 / NQ, t1 / NQ + NQ / 2, t1 / NQ };
+    float cx = x0x, cy = x0y;
+    float ds = a.gridScale;
+    float qlast = -FLT_MAX;
+    for (int d = 0; d < a.depth; ++d) {
+        const float gxM = cx + ds * ex, gxm = cx - ds * ex;
+        const float gyM = cy + ds * ey, gym = cy - ds * ey;
+        float py[2], Q2[2], px[4], t2[4];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            py[j] = (grid_coord(a.aff, cj[j], NQ, gym, gyM) - x0y) * sg;
+            Q2[j] = 2.0f * (py[j] * py[j]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            px[i] = (grid_coord(a.aff, ci[i], NQ, gxm, gxM) - x0x) * sg;
+            t2[i] = 2.0f * px[i];
+        }
+        TriAcc acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = tri_acc0();
+        for (int e = 0; e < n; ++e) {
+            const TriC c = s_tr[e];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const float Ty = fmaf(c.gy, py[j], c.C);
+                const float Ry = fmaf(c.sy, py[j], c.K + Q2[j]);
+#pragma unroll
+                for (int i = 2 * j; i < 2 * j + 2; ++i)
+                    tri_acc_add(acc[i], fmaf(c.gx, px[i], Ty), fmaf(px[i], t2[i] + c.sx, Ry));
+            }
+        }
+        float best = -FLT_MAX;
+        int bi = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float q = tri_acc_score(acc[i], is2);
+            if (q > best) { best = q; bi = i; }
+        }
+        int wl = 4 * s + bi;
+        group_argmax<G>(best, wl);
+        if (qcur < best) {
+            const int wt = wl >> 1;
+            const int wi = wt / NQ + ((wl & 1) ? 0 : NQ / 2), wj = wt % NQ;
+            cx = grid_coord(a.aff, wi, NQ, gxm, gxM);
+            cy = grid_coord(a.aff, wj, NQ, gym, gyM);
+            qcur = best;
+        }
+        qlast = best;
+        ds = ds * a.shrink;
+    }
+    if (live && qold < qlast && s == 0) { a.xw[vid] = cx; a.yw[vid] = cy; }
+}
+
+// 16 vertices of the class per 256-thread block; cap TriC per group
+__global__ __launch_bounds__(256) void k_search_flat8x4(SearchArgs a, int cap)
+{
+    extern __shared__ TriC s_trg[];
+    const int g = (int)(threadIdx.x >> 4);
+    const int i = (int)blockIdx.x * kFlatGroups + g;
+    const int vid = (a.cStart + i < a.cEnd) ? a.cStart + i : -1;
+    flat_group8(a, vid, s_trg + (size_t)g * cap);
+}
+
+// the flat triangle items [lo, hi) of a batch round, 16 per block
+__global__ __launch_bounds__(256) void k_batch_flat8x4(SearchArgs a, const int* __restrict__ items, int lo, int hi,
+                                                      int cap)
+{
+    extern __shared__ TriC s_trb[];
+    const int g = (int)(threadIdx.x >> 4);
+    const int i = lo + (int)blockIdx.x * kFlatGroups + g;
+    const int vid = (i < hi) ? (items[i] & 0x3fffffff) : -1;
+    flat_group8(a, vid, s_trb + (size_t)g * cap);
+}
+
+// whether the 4-per-wave flat search runs: nq 8, the block's fans fit 64 KB
+// of LDS, and at least kFlat4MinVertices vertices (256 blocks: a smaller
+// launch spreads over more CUs with one vertex per wave; measured on the 64^2
+// grid's ~960-vertex classes, 0.112 against 0.123 ms per iteration)
+constexpr int kFlat4MinVertices = 256 * kFlatGroups;
+static bool flat8x4_ok(int nq, size_t ldsPerVertex, int nVertices)
+{
+    return nq == 8 && ldsPerVertex >= sizeof(TriC) && kFlatGroups * ldsPerVertex <= 65536 &&
+           nVertices >= kFlat4MinVertices;
+}
+
+
 // proposal displacement of the group's vertex (class position ci, at (vx,
 // vy, vz)): c = v + l * y.  FUSED: the block's 16 vertices (class positions
 // ci0 + 0..15, group gb = threadIdx.x / 16) are one tile of the network, run
@@ -1783,6 +1926,11 @@ void launch_batch_typed(int nq, int w, const SearchArgs& a, const int* rowHex, c
                         int lo, int hi, int type, size_t lds, hipStream_t s)
 {
     if (hi <= lo) return;
+    if (type == 2 && w == 1 && flat8x4_ok(nq, lds, hi - lo)) {
+        const int groups = (hi - lo + kFlatGroups - 1) / kFlatGroups;
+        k_batch_flat8x4<<<groups, 256, kFlatGroups * lds, s>>>(a, items, lo, hi, (int)(lds / sizeof(TriC)));
+        return;
+    }
     switch (nq) {
     case 2: batch_typed_nq<2>(w, a, rowHex, valHex, items, lo, hi, type, lds, s); break;
     case 4: batch_typed_nq<4>(w, a, rowHex, valHex, items, lo, hi, type, lds, s); break;
@@ -1840,6 +1988,11 @@ void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hip
 {
     int nBlocks = a.cEnd - a.cStart;
     if (nBlocks <= 0) return;
+    if (kind == VSDMO_TRI && w == 1 && flat8x4_ok(nq, lds, nBlocks)) {
+        const int groups = (nBlocks + kFlatGroups - 1) / kFlatGroups;
+        launch_kernel(k_search_flat8x4, groups, 256, kFlatGroups * lds, s, a, (int)(lds / sizeof(TriC)));
+        return;
+    }
     switch (nq) {
     case 2: launch_nq<2>(kind, w, a, nBlocks, lds, s); break;
     case 4: launch_nq<4>(kind, w, a, nBlocks, lds, s); break;
