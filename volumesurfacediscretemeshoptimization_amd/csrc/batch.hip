@@ -612,15 +612,15 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
             }
             b->items.alloc(std::max<size_t>(1, b->itemsMeshOrder.size()));
             b->ctr.alloc((size_t)std::max(1, nRounds));
-            // quality chunks of <= 256 elements, never across meshes
+            // quality chunks of <= kQualityChunk elements, never across meshes
             std::vector<int> ce0{ 0 }, ctype, mc0{ 0 };
             std::vector<long long> cc0;
             long long cpos = 0;
             int epos = 0;
             for (int i = 0; i < nm; ++i) {
                 vsdmo_mesh* m = b->meshes[i];
-                for (int e = 0; e < m->nE; e += 256) {
-                    int ne = std::min(256, m->nE - e);
+                for (int e = 0; e < m->nE; e += kQualityChunk) {
+                    int ne = std::min(kQualityChunk, m->nE - e);
                     cc0.push_back(cpos + (long long)e * m->npe);
                     ctype.push_back(m->elem);
                     epos += ne;

@@ -135,6 +135,9 @@ void snapshot_launch(const SurfArgs& a, hipStream_t s);
 void laplace_launch(float* x, float* y, float* z, const int* heRow, const int* heTarget, int lo, int hi, hipStream_t s);
 
 // chunked per-mesh qualities of a batch
+// elements per quality chunk of a batch (one block of k_batch_quality)
+constexpr int kQualityChunk = 1024;
+
 struct BatchQArgs {
     const float* x;
     const float* y;

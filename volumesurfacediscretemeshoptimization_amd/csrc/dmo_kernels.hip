@@ -1640,61 +1640,73 @@ __device__ float elem_quality(const float* __restrict__ X, const float* __restri
     return tri_acc_score(acc, (1.0f * isg) * isg);
 }
 
-// the qualities of the block's 256 elements, written to q, and their min and
-// FP64 sum in a fixed tree order (per-block partials for k_reduce_final)
+// elements per block of the quality kernels: 4 per thread of 256
+constexpr int kQualPerBlock = 1024;
+
+// min and FP64 sum over the 256 threads of a block: a butterfly within each
+// wave (lane 0's value: the order of its additions is fixed), then the 4 waves
+// in order; the result is every thread's.  One call per kernel (static LDS).
+__device__ __forceinline__ void block_minsum256(float& m, double& s)
+{
+    __shared__ float wm[4];
+    __shared__ double ws[4];
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) {
+        m = fminf(m, __shfl_xor(m, k));
+        s = s + __shfl_xor(s, k);
+    }
+    const int w = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0) { wm[w] = m; ws[w] = s; }
+    __syncthreads();
+    m = fminf(fminf(wm[0], wm[1]), fminf(wm[2], wm[3]));
+    s = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+// the qualities of the block's kQualPerBlock elements (4 per thread, strided
+// by 256), written to q, and their min and FP64 sum in a fixed order
+// (per-block partials for k_reduce_final)
 __global__ __launch_bounds__(256) void k_elem_quality(QualityArgs a, float* __restrict__ pmin,
                                                       double* __restrict__ psum)
 {
-    __shared__ float smin[256];
-    __shared__ double ssum[256];
-    const int e = blockIdx.x * 256 + threadIdx.x;
     float v = FLT_MAX;
     double sv = 0.0;
-    if (e < a.nE) {
-        v = elem_quality(a.x, a.y, a.z, a.cells + (size_t)e * a.npe, a.elem, a.hexCorner8 != 0);
-        a.q[e] = v;
-        sv = (double)v;
-    }
-    smin[threadIdx.x] = v;
-    ssum[threadIdx.x] = sv;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) {
-            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + k]);
-            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + k];
+#pragma unroll
+    for (int k = 0; k < kQualPerBlock / 256; ++k) {
+        const int e = blockIdx.x * kQualPerBlock + k * 256 + (int)threadIdx.x;
+        if (e < a.nE) {
+            const float q = elem_quality(a.x, a.y, a.z, a.cells + (size_t)e * a.npe, a.elem, a.hexCorner8 != 0);
+            a.q[e] = q;
+            v = fminf(v, q);
+            sv = sv + (double)q;
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) { pmin[blockIdx.x] = smin[0]; psum[blockIdx.x] = ssum[0]; }
+    block_minsum256(v, sv);
+    if (threadIdx.x == 0) { pmin[blockIdx.x] = v; psum[blockIdx.x] = sv; }
 }
 
-// batch qualities: one block of 256 per chunk of at most 256 elements of one
-// mesh; per-chunk min and FP64 sum in a fixed tree order
-__global__ void k_batch_quality(BatchQArgs a)
+// batch qualities: one block of 256 per chunk of at most kQualityChunk
+// elements of one mesh; per-chunk min and FP64 sum in a fixed order
+static_assert(kQualityChunk % 256 == 0, "quality chunks are whole multiples of the block");
+__global__ __launch_bounds__(256) void k_batch_quality(BatchQArgs a)
 {
-    __shared__ float smin[256];
-    __shared__ double ssum[256];
     const int c = blockIdx.x;
     const int e0 = a.chunkElem0[c], ne = a.chunkElem0[c + 1] - e0;
     const int elem = a.chunkType[c];
     const int npe = elem == VSDMO_TET ? 4 : elem == VSDMO_HEX ? 8 : 3;
-    float q = FLT_MAX;
+    float m = FLT_MAX;
     double s = 0.0;
-    if ((int)threadIdx.x < ne) {
-        q = elem_quality(a.x, a.y, a.z, a.cells + a.chunkCell0[c] + (size_t)threadIdx.x * npe, elem, a.hexCorner8 != 0);
-        s = (double)q;
-    }
-    smin[threadIdx.x] = q;
-    ssum[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (threadIdx.x < k) {
-            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + k]);
-            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + k];
+#pragma unroll
+    for (int k = 0; k < kQualityChunk / 256; ++k) {
+        const int t = k * 256 + (int)threadIdx.x;
+        if (t < ne) {
+            const float q = elem_quality(a.x, a.y, a.z, a.cells + a.chunkCell0[c] + (size_t)t * npe, elem,
+                                         a.hexCorner8 != 0);
+            m = fminf(m, q);
+            s = s + (double)q;
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) { a.cmin[c] = smin[0]; a.csum[c] = ssum[0]; }
+    block_minsum256(m, s);
+    if (threadIdx.x == 0) { a.cmin[c] = m; a.csum[c] = s; }
 }
 
 // per-mesh min / FP64 sum over the mesh's chunks: one block per mesh,
@@ -2004,7 +2016,7 @@ void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hip
 
 void launch_quality(const QualityArgs& a, float* pmin, double* psum, float* omin, double* osum, hipStream_t s)
 {
-    int nb = (a.nE + 255) / 256;
+    int nb = (a.nE + kQualPerBlock - 1) / kQualPerBlock;
     k_elem_quality<<<nb, 256, 0, s>>>(a, pmin, psum);
     k_reduce_final<<<1, 256, 0, s>>>(pmin, psum, nb, omin, osum);
 }
