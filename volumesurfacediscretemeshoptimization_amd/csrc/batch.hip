@@ -76,6 +76,7 @@ struct vsdmo_batch {
     DevBuf<double> csum, msum;
     DevBuf<int> hctr;
     int nChunks = 0, hcap = 0, hrow = 0;
+    int qper = 1;          // quality elements per thread (chunks of 256 * qper)
     std::vector<float> lastMin;
     std::vector<int> nElem;
     // execution
@@ -154,6 +155,7 @@ BatchQArgs quality_args(vsdmo_batch* b)
     q.hist = b->hist.p; q.hctr = b->hctr.p;
     q.nChunks = b->nChunks; q.nMeshes = (int)b->meshes.size(); q.hcap = b->hcap;
     q.hexCorner8 = b->p.hex_metric == VSDMO_HEX_CORNER8;
+    q.per = b->qper;
     return q;
 }
 
@@ -612,15 +614,19 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
             }
             b->items.alloc(std::max<size_t>(1, b->itemsMeshOrder.size()));
             b->ctr.alloc((size_t)std::max(1, nRounds));
-            // quality chunks of <= kQualityChunk elements, never across meshes
+            // quality chunks of <= 256 * qper elements, never across meshes
+            long long nAll = 0;
+            for (int i = 0; i < nm; ++i) nAll += b->meshes[i]->nE;
+            b->qper = quality_per_thread(nAll);
+            const int chunk = 256 * b->qper;
             std::vector<int> ce0{ 0 }, ctype, mc0{ 0 };
             std::vector<long long> cc0;
             long long cpos = 0;
             int epos = 0;
             for (int i = 0; i < nm; ++i) {
                 vsdmo_mesh* m = b->meshes[i];
-                for (int e = 0; e < m->nE; e += kQualityChunk) {
-                    int ne = std::min(kQualityChunk, m->nE - e);
+                for (int e = 0; e < m->nE; e += chunk) {
+                    int ne = std::min(chunk, m->nE - e);
                     cc0.push_back(cpos + (long long)e * m->npe);
                     ctype.push_back(m->elem);
                     epos += ne;

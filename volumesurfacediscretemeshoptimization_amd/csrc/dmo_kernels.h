@@ -135,8 +135,9 @@ void snapshot_launch(const SurfArgs& a, hipStream_t s);
 void laplace_launch(float* x, float* y, float* z, const int* heRow, const int* heTarget, int lo, int hi, hipStream_t s);
 
 // chunked per-mesh qualities of a batch
-// elements per quality chunk of a batch (one block of k_batch_quality)
-constexpr int kQualityChunk = 1024;
+// elements per thread of the quality kernels for a launch over nElements
+// (chunks of 256 * that many elements per block)
+int quality_per_thread(long long nElements);
 
 struct BatchQArgs {
     const float* x;
@@ -155,6 +156,7 @@ struct BatchQArgs {
     int* hctr;
     int nChunks, nMeshes, hcap;
     int hexCorner8;
+    int per;                   // elements per thread: chunks of <= 256 * per elements
 };
 
 size_t search_lds_bytes(int kind, int maxRing);

@@ -344,21 +344,52 @@ __device__ __forceinline__ bool hex_pinned(const HexFrame& f)
     return f.acc0.dmin < f.epsD || f.acc0.fmin < f.epsF;
 }
 
-__device__ __forceinline__ float hex_score(const HexFrame& f, const HexLds* s_hx, float px, float py, float pz)
+// the p-dependent sub-tets of the ring at one point: NS = 5 (the nine-tet
+// metric) or 4 (eight corners).  The slot count is a template argument so
+// that the callers branch on the metric once, outside the element loop (a
+// branch inside it holds the LDS reads of the last slot back)
+template <int NS>
+__device__ __forceinline__ void hex_ring_add(HexAcc& acc, const HexLds* s_hx, int n, float px, float py, float pz)
 {
     const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
-    HexAcc acc = f.acc0;
-    for (int e = 0; e < f.n; ++e)
+    for (int e = 0; e < n; ++e)
 #pragma unroll
-        for (int sIdx = 0; sIdx < 5; ++sIdx) {
-            if (sIdx == 4 && f.corner8) break;
+        for (int sIdx = 0; sIdx < NS; ++sIdx) {
             const HexP h = s_hx[e].hp[sIdx];
             const bool a3 = (sIdx == 1 || sIdx == 4);
             float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
             float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
             hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
         }
+}
+
+__device__ __forceinline__ float hex_score(const HexFrame& f, const HexLds* s_hx, float px, float py, float pz)
+{
+    HexAcc acc = f.acc0;
+    if (f.corner8) hex_ring_add<4>(acc, s_hx, f.n, px, py, pz);
+    else hex_ring_add<5>(acc, s_hx, f.n, px, py, pz);
     return hex_acc_score(acc, f.epsD, f.epsF);
+}
+
+// the p-dependent sub-tets of the ring for the CPL candidates px[i] of a lane
+// (shared y, z): NS slots per element as in hex_ring_add
+template <int NS, int CPL>
+__device__ __forceinline__ void hex_ring_add_cpl(HexAcc* acc, const HexLds* s_hx, int n, const float* px,
+                                                 const float* t3, float py, float pz)
+{
+    const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
+    for (int e = 0; e < n; ++e) {
+#pragma unroll
+        for (int sIdx = 0; sIdx < NS; ++sIdx) {
+            const HexP h = s_hx[e].hp[sIdx];
+            const bool a3 = (sIdx == 1 || sIdx == 4);
+            const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+            const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+#pragma unroll
+            for (int i = 0; i < CPL; ++i)
+                hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
+        }
+    }
 }
 
 // the nested grid search of a hex vertex (lane layout as tet_levels)
@@ -386,23 +417,11 @@ __device__ bool hex_levels(const SearchArgs& a, const HexFrame& f, const HexLds*
             px[i] = (grid_coord(a.aff, i0 + i, NQ, gxm, gxM) - f.x0x) * f.sg;
             t3[i] = 3.0f * px[i];
         }
-        const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
         HexAcc acc[CPL];
 #pragma unroll
         for (int i = 0; i < CPL; ++i) acc[i] = f.acc0;
-        for (int e = 0; e < f.n; ++e) {
-#pragma unroll
-            for (int sIdx = 0; sIdx < 5; ++sIdx) {
-                if (sIdx == 4 && a.hexCorner8) break;   // uniform: a kernel argument
-                const HexP h = s_hx[e].hp[sIdx];
-                const bool a3 = (sIdx == 1 || sIdx == 4);
-                const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
-                const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
-#pragma unroll
-                for (int i = 0; i < CPL; ++i)
-                    hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
-            }
-        }
+        if (a.hexCorner8) hex_ring_add_cpl<4, CPL>(acc, s_hx, f.n, px, t3, py, pz);   // uniform: a kernel argument
+        else hex_ring_add_cpl<5, CPL>(acc, s_hx, f.n, px, t3, py, pz);
         float best = -FLT_MAX;
         int bi = 0;
 #pragma unroll
@@ -1084,7 +1103,10 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
     if (need && la.epoch) la.epoch[vid] = la.it;
 }
 
-template <int NQ, int LV, int WPB, bool FUSED>
+// C8: the eight-corner hex metric (a.hexCorner8) as a template argument: a
+// runtime branch around the principal-axes slot held its LDS reads back in
+// this loop (36.5 -> 38.1 us per 48^3 class)
+template <int NQ, int LV, int WPB, bool FUSED, bool C8>
 __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap, int blk, unsigned char* s_hxl)
 {
     constexpr int NL = 4, G = 64 / LV, CPL = NL * NL * NL / G;
@@ -1203,8 +1225,7 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
             for (int i = 0; i < CPL; ++i) acc[i] = f.acc0;
             for (int e = 0; e < f.n; ++e) {
     #pragma unroll
-                for (int sIdx = 0; sIdx < 5; ++sIdx) {
-                    if (sIdx == 4 && a.hexCorner8) break;
+                for (int sIdx = 0; sIdx < (C8 ? 4 : 5); ++sIdx) {
                     const HexP h = hx[e].hp[sIdx];
                     const bool a3 = (sIdx == 1 || sIdx == 4);
                     const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
@@ -1256,11 +1277,11 @@ __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_tet_learned(SearchA
     extern __shared__ __align__(16) unsigned char s_tcl[];
     tet_window<NQ, LV, FUSED ? 4 : 1, FUSED>(a, la, ringCap, blockIdx.x, s_tcl);
 }
-template <int NQ, int LV, bool FUSED>
+template <int NQ, int LV, bool FUSED, bool C8>
 __global__ __launch_bounds__(FUSED ? 256 : 64) void k_search_hex_learned(SearchArgs a, LearnArgs la, int ringCap)
 {
     extern __shared__ __align__(16) unsigned char s_hxl[];
-    hex_window<NQ, LV, FUSED ? 4 : 1, FUSED>(a, la, ringCap, blockIdx.x, s_hxl);
+    hex_window<NQ, LV, FUSED ? 4 : 1, FUSED, C8>(a, la, ringCap, blockIdx.x, s_hxl);
 }
 
 // deferred fallback: the first gridDim.x - nbList blocks take the windows of
@@ -1293,7 +1314,7 @@ void k_search_tet_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbLi
 }
 // 4 waves per SIMD (<= 128 registers): a 48^3 class (~13k vertices, 3.3k
 // waves) then fits the chip's wave slots in one round
-template <int NQ, bool FUSED>
+template <int NQ, bool FUSED, bool C8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_search_hex_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbList)
 {
@@ -1311,7 +1332,7 @@ void k_search_hex_learned_def(SearchArgs a, LearnArgs la0, int ringCap, int nbLi
         }
         return;
     }
-    hex_window<NQ, 4, 4, FUSED>(a, la, ringCap, (int)blockIdx.x, s_hxd);
+    hex_window<NQ, 4, 4, FUSED, C8>(a, la, ringCap, (int)blockIdx.x, s_hxd);
 }
 
 // plain search of the vertices list[0 .. *cnt) (the fallbacks of a class),
@@ -1357,7 +1378,8 @@ static void learned_nq_lv(int kind, const SearchArgs& a, const LearnArgs& la, in
 {
     constexpr int T = FUSED ? 256 : 64;
     if (kind == VSDMO_TET) launch_kernel(k_search_tet_learned<NQ, LV, FUSED>, nb, T, lds, s, a, la, ringCap);
-    else launch_kernel(k_search_hex_learned<NQ, LV, FUSED>, nb, T, lds, s, a, la, ringCap);
+    else if (a.hexCorner8) launch_kernel(k_search_hex_learned<NQ, LV, FUSED, true>, nb, T, lds, s, a, la, ringCap);
+    else launch_kernel(k_search_hex_learned<NQ, LV, FUSED, false>, nb, T, lds, s, a, la, ringCap);
 }
 
 static size_t learned_per(int kind) { return (kind == VSDMO_TET) ? sizeof(TetC) : sizeof(HexLds); }
@@ -1398,7 +1420,9 @@ static void learned_def_nq(int kind, const SearchArgs& a, const LearnArgs& la, i
                            size_t lds, hipStream_t s)
 {
     if (kind == VSDMO_TET) launch_kernel(k_search_tet_learned_def<NQ, FUSED>, nbList + nb, 256, lds, s, a, la, ringCap, nbList);
-    else launch_kernel(k_search_hex_learned_def<NQ, FUSED>, nbList + nb, 256, lds, s, a, la, ringCap, nbList);
+    else if (a.hexCorner8)
+        launch_kernel(k_search_hex_learned_def<NQ, FUSED, true>, nbList + nb, 256, lds, s, a, la, ringCap, nbList);
+    else launch_kernel(k_search_hex_learned_def<NQ, FUSED, false>, nbList + nb, 256, lds, s, a, la, ringCap, nbList);
 }
 
 void launch_search_learned(int kind, int nq, const SearchArgs& a, const LearnNet& net, int mode, int depthL,
@@ -1640,8 +1664,14 @@ __device__ float elem_quality(const float* __restrict__ X, const float* __restri
     return tri_acc_score(acc, (1.0f * isg) * isg);
 }
 
-// elements per block of the quality kernels: 4 per thread of 256
-constexpr int kQualPerBlock = 1024;
+// elements per thread of the quality kernels: up to 4 (strided by 256), as
+// long as the launch keeps at least 512 blocks (two per CU)
+int quality_per_thread(long long nElements)
+{
+    int per = 4;
+    while (per > 1 && nElements < (long long)per * 256 * 512) per >>= 1;
+    return per;
+}
 
 // min and FP64 sum over the 256 threads of a block: a butterfly within each
 // wave (lane 0's value: the order of its additions is fixed), then the 4 waves
@@ -1662,17 +1692,16 @@ __device__ __forceinline__ void block_minsum256(float& m, double& s)
     s = (ws[0] + ws[1]) + (ws[2] + ws[3]);
 }
 
-// the qualities of the block's kQualPerBlock elements (4 per thread, strided
+// the qualities of the block's 256 * per elements (per per thread, strided
 // by 256), written to q, and their min and FP64 sum in a fixed order
 // (per-block partials for k_reduce_final)
 __global__ __launch_bounds__(256) void k_elem_quality(QualityArgs a, float* __restrict__ pmin,
-                                                      double* __restrict__ psum)
+                                                      double* __restrict__ psum, int per)
 {
     float v = FLT_MAX;
     double sv = 0.0;
-#pragma unroll
-    for (int k = 0; k < kQualPerBlock / 256; ++k) {
-        const int e = blockIdx.x * kQualPerBlock + k * 256 + (int)threadIdx.x;
+    for (int k = 0; k < per; ++k) {
+        const int e = blockIdx.x * 256 * per + k * 256 + (int)threadIdx.x;
         if (e < a.nE) {
             const float q = elem_quality(a.x, a.y, a.z, a.cells + (size_t)e * a.npe, a.elem, a.hexCorner8 != 0);
             a.q[e] = q;
@@ -1684,9 +1713,8 @@ __global__ __launch_bounds__(256) void k_elem_quality(QualityArgs a, float* __re
     if (threadIdx.x == 0) { pmin[blockIdx.x] = v; psum[blockIdx.x] = sv; }
 }
 
-// batch qualities: one block of 256 per chunk of at most kQualityChunk
+// batch qualities: one block of 256 per chunk of at most 256 * a.per
 // elements of one mesh; per-chunk min and FP64 sum in a fixed order
-static_assert(kQualityChunk % 256 == 0, "quality chunks are whole multiples of the block");
 __global__ __launch_bounds__(256) void k_batch_quality(BatchQArgs a)
 {
     const int c = blockIdx.x;
@@ -1695,8 +1723,7 @@ __global__ __launch_bounds__(256) void k_batch_quality(BatchQArgs a)
     const int npe = elem == VSDMO_TET ? 4 : elem == VSDMO_HEX ? 8 : 3;
     float m = FLT_MAX;
     double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < kQualityChunk / 256; ++k) {
+    for (int k = 0; k < a.per; ++k) {
         const int t = k * 256 + (int)threadIdx.x;
         if (t < ne) {
             const float q = elem_quality(a.x, a.y, a.z, a.cells + a.chunkCell0[c] + (size_t)t * npe, elem,
@@ -2016,8 +2043,9 @@ void launch_search(int kind, int nq, int w, const SearchArgs& a, size_t lds, hip
 
 void launch_quality(const QualityArgs& a, float* pmin, double* psum, float* omin, double* osum, hipStream_t s)
 {
-    int nb = (a.nE + kQualPerBlock - 1) / kQualPerBlock;
-    k_elem_quality<<<nb, 256, 0, s>>>(a, pmin, psum);
+    const int per = quality_per_thread(a.nE);
+    const int nb = (a.nE + 256 * per - 1) / (256 * per);
+    k_elem_quality<<<nb, 256, 0, s>>>(a, pmin, psum, per);
     k_reduce_final<<<1, 256, 0, s>>>(pmin, psum, nb, omin, osum);
 }
 
