@@ -631,6 +631,24 @@ __device__ RingFrame stage_ring(const SurfArgs& a, int vid, bool tet, void* lds,
     return f;
 }
 
+// the p-dependent hex sub-tets of the ring at one point, NS per element (5:
+// the nine-tet metric, 4: eight corners; a compile-time count keeps the
+// element loop free of the metric branch)
+template <int NS>
+__device__ __forceinline__ void ring_hex_add(HexAcc& acc, const HexRingLds* L, int n, float px, float py, float pz)
+{
+    const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
+    for (int e = 0; e < n; ++e)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const HexP h = L[e].hp[s];
+            const bool a3 = (s == 1 || s == 4);
+            float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+            float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+            hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
+        }
+}
+
 __device__ float ring_score(const RingFrame& f, bool tet, const void* lds, const HexAcc& hacc, w3 p)
 {
     const float px = (p.x - f.x0x) * f.sg, py = (p.y - f.x0y) * f.sg, pz = (p.z - f.x0z) * f.sg;
@@ -648,16 +666,9 @@ __device__ float ring_score(const RingFrame& f, bool tet, const void* lds, const
         return tet_acc_score(acc, f.is3);
     }
     const HexRingLds* L = reinterpret_cast<const HexRingLds*>(lds);
-    const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
     HexAcc acc = hacc;
-    for (int e = 0; e < f.n; ++e)
-        for (int s = 0; s < f.nsub; ++s) {
-            const HexP h = L[e].hp[s];
-            const bool a3 = (s == 1 || s == 4);
-            float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
-            float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
-            hex_acc_add(acc, fmaf(h.gx, px, Tyz), fmaf(px, (a3 ? t3 : px) + h.sx, Ryz));
-        }
+    if (f.nsub == 4) ring_hex_add<4>(acc, L, f.n, px, py, pz);
+    else ring_hex_add<5>(acc, L, f.n, px, py, pz);
     return hex_acc_score(acc, f.epsD, f.epsF);
 }
 
