@@ -1634,7 +1634,9 @@ __device__ float elem_quality(const float* __restrict__ X, const float* __restri
         for (int sIdx = 0; sIdx < 4; ++sIdx) hex_acc_add(acc, cd[sIdx], cF[sIdx]);
         const float px = 0.0f, py = 0.0f, pz = 0.0f;
         const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1, t3 = 3.0f * px;
-        for (int sIdx = 0; sIdx < (corner8 ? 4 : 5); ++sIdx) {
+#pragma unroll
+        for (int sIdx = 0; sIdx < 5; ++sIdx) {   // unrolled: hp[] stays in registers
+            if (sIdx == 4 && corner8) break;
             const bool a3 = (sIdx == 1 || sIdx == 4);
             float Tyz = fmaf(hp[sIdx].gy, py, fmaf(hp[sIdx].gz, pz, hp[sIdx].D));
             float Ryz = fmaf(hp[sIdx].sy, py, fmaf(hp[sIdx].sz, pz, hp[sIdx].K + (a3 ? Q3 : Q1)));
