@@ -90,9 +90,9 @@ VALU_ISSUE_BASIS = ("SQ_INSTS_VALU / kernel-trace duration / CUs over the calibr
 SURF_FLOP1D = {"tri": 3 * 8 * (149 + 59 * 6), "quad": 3 * 8 * (149 + 104 * 4)}
 WORKLOADS = {
     "tet40": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet<8>",
-              "traffic": 9_159_429, "learned_traffic": 6_364_917},
+              "traffic": 9_295_621, "learned_traffic": 6_438_890},
     "hex48": {"kind": "hex", "flop": FLOP_HEX, "cbrt": FULL_EVALS, "kernel": "k_search_hex<8>",
-              "traffic": 16_070_763, "learned_traffic": 10_486_711},
+              "traffic": 16_096_351, "learned_traffic": 10_471_234},
     # the 48^3 hex cube at jitter 0.15h: a valid mesh, so the surface and
     # feature passes run every iteration (the roofline is the inner kernel's)
     "hex48_surface": {"kind": "hex", "flop": FLOP_HEX, "cbrt": FULL_EVALS, "kernel": "k_search_hex<8>",
