@@ -59,7 +59,7 @@ PEAK_FP32_TFLOPS = 157.3
 # HBM traffic per launch: rocprofv3 --pmc FETCH_SIZE x 1.99981 (the
 # copy-kernel scale of bench.py --calibrate) + WRITE_SIZE x 1.0, mean over
 # the dispatches of the default bench line's runs
-# (profiles/north_star_kernels.md; the fan kernels in
+# (profiles/learned_vs_plain_tet40_hex48.md; the fan kernels in
 # profiles/k_search_fan2d_surf.md).
 def canon_flop(kind: str, cand_per_lane: int) -> float:
     if kind == "tet":
@@ -82,11 +82,11 @@ SURF_FLOP = {"tri": 3 * 64 * (57 + 69 * 6 + 59 * 6), "quad": 3 * 64 * (57 + 69 *
 # the kernel trace's mean duration / 256 CUs) over the highest rate measured
 # for the search's own instruction mix (bench.py --calibrate's selection-step
 # probe, 3.81 wave-instructions per CU per ns), from the counter passes of
-# profiles/north_star_kernels.md
+# profiles/learned_vs_plain_tet40_hex48.md
 VALU_ISSUE_FRAC = {"k_search_tet": 0.80, "k_search_hex": 0.92, "k_search_tet_learned_def": 0.38,
                    "k_search_hex_learned_def": 0.47}
 VALU_ISSUE_BASIS = ("SQ_INSTS_VALU / kernel-trace duration / CUs over the calibrated mixed-instruction issue "
-                    "ceiling (profiles/north_star_kernels.md, profiles/valu_issue_calibration.md)")
+                    "ceiling (profiles/learned_vs_plain_tet40_hex48.md, profiles/valu_issue_calibration.md)")
 SURF_FLOP1D = {"tri": 3 * 8 * (149 + 59 * 6), "quad": 3 * 8 * (149 + 104 * 4)}
 WORKLOADS = {
     "tet40": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet<8>",
@@ -104,7 +104,7 @@ WORKLOADS = {
                   "kernel": "k_search_fan2d<true>", "traffic": 20_562_741},
     "batch256": {"kind": "batch", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_batch_round", "traffic": None},
     # the default line: learned (+ batched) vs plain on tet40 and hex48
-    "north_star": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet_learned_def<8>",
+    "learned_vs_plain": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet_learned_def<8>",
                    "traffic": None},
 }
 # algorithmic FLOP of one flat-triangle vertex update (8x8 candidates: 196 ring
@@ -706,7 +706,7 @@ def _to_quality(fac, cls, xyz, cells, kw, mlp, mode, depth, frac, policy, qp, qm
     return out
 
 
-def north_star_mesh(name, args):
+def learned_vs_plain_mesh(name, args):
     """plain vs learned vs learned + batched on one mesh at equal final quality"""
     from volumesurfacediscretemeshoptimization_amd import (DMOHexClass, DMOMeshHexFactory, DMOMeshTetFactory,
                                                            DMOTetClass)
@@ -881,10 +881,10 @@ def calibrate(copy_mib: int = 256, copy_reps: int = 5, iters: int = 20000, reps:
     return 0
 
 
-def bench_north_star(args):
+def bench_learned_vs_plain(args):
     t_start = time.perf_counter()
-    tet = north_star_mesh("tet40", args)
-    hexr = north_star_mesh("hex48", args) if args.with_hex else None
+    tet = learned_vs_plain_mesh("tet40", args)
+    hexr = learned_vs_plain_mesh("hex48", args) if args.with_hex else None
     lb = tet["learned_batched"]
     if "error" in lb or not lb.get("reached_plain_quality"):
         value = 0.0
@@ -921,7 +921,7 @@ def bench_north_star(args):
                          "(59,319 inner searches each) / the wall time of the learned + batched run to the plain "
                          "run's final quality (a one-mesh batch, one iteration replayed as a hipGraph); "
                          "ms_per_step = that time / K; the 48^3 hex cube beside it"),
-            "name": "north_star",
+            "name": "learned_vs_plain",
             "speedup_vs_plain": {"tet40_learned_batched": lb.get("speedup_vs_single_mesh_plain"),
                                  "tet40_learned": tet["learned"]["speedup_vs_plain"],
                                  "hex48_learned_batched": (hexr["learned_batched"].get("speedup_vs_single_mesh_plain")
@@ -942,10 +942,10 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="north_star",
-                    help="north_star (default): learned + batched vs plain at equal quality on tet40 and hex48; "
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="learned_vs_plain",
+                    help="learned_vs_plain (default): learned + batched vs plain at equal quality on tet40 and hex48; "
                          "the others: one plain (or --mode learned) line per workload")
-    ap.add_argument("--with-hex", type=int, default=1, help="north_star: also the 48^3 hex cube (0: skip)")
+    ap.add_argument("--with-hex", type=int, default=1, help="learned_vs_plain: also the 48^3 hex cube (0: skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="measurement calibration instead of a bench line: a 256 MiB device copy (HBM counter "
                          "scale) and a kernel with a known count of FP32 FMA instructions (VALU issue rate)")
@@ -997,11 +997,11 @@ def main(argv=None):
         return 2
     if args.calibrate:
         return calibrate()
-    if args.workload == "north_star":
-        return bench_north_star(args)
+    if args.workload == "learned_vs_plain":
+        return bench_learned_vs_plain(args)
 
     if args.compare_learned is None:
-        args.compare_learned = 0          # the north_star line carries the comparison
+        args.compare_learned = 0          # the learned_vs_plain line carries the comparison
     hexw = WORKLOADS[args.workload]["kind"] == "hex"
     if args.learn_depth is None:
         args.learn_depth = LEARN_WINDOW["hex" if hexw else "tet"][0]
