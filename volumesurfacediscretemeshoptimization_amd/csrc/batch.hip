@@ -515,7 +515,7 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
             const int nm = n_meshes;
             b->vOff.assign(nm + 1, 0);
             std::vector<long long> ringBase(nm + 1, 0), hexBase(nm + 1, 0), heBase(nm + 1, 0);
-            long long cellInts = 0, totalE = 0;
+            long long cellInts = 0;
             for (int i = 0; i < nm; ++i) {
                 vsdmo_mesh* m = b->meshes[i];
                 b->vOff[i + 1] = b->vOff[i] + m->nV;
@@ -527,7 +527,6 @@ vsdmo_status vsdmo_batch_create(vsdmo_mesh* const* meshes, int32_t n_meshes, con
                 if (m->elem == VSDMO_HEX) b->maxRingHex = std::max(b->maxRingHex, m->maxRing);
                 if (m->elem == VSDMO_TRI) b->maxHE = std::max(b->maxHE, m->maxHE);
                 cellInts += (long long)m->nE * m->npe;
-                totalE += m->nE;
             }
             if ((long long)b->vOff[nm] >= (1LL << 30) || ringBase[nm] >= (1LL << 31) / 3 ||
                 hexBase[nm] >= (1LL << 31) / 7 || heBase[nm] >= (1LL << 31) || cellInts >= (1LL << 31))

@@ -12,6 +12,7 @@
 // (VSDMO/src/dmo/DMOTetImplCub.h:20-137, DMOHexImplCub.h:20-137) and
 // k_optimizeHierarchical2DFlat (DMOTriFlatImplCub.h:23-152).
 #include <float.h>
+#include <type_traits>
 
 #include "vsdmo_internal.h"
 #include "vsdmo_numerics.h"
@@ -229,6 +230,10 @@ __device__ bool tet_levels(const SearchArgs& a, const TetFrame& f, const TetC* s
         TetAcc acc[CPL];
 #pragma unroll
         for (int i = 0; i < CPL; ++i) acc[i] = tet_acc0();
+        // two elements per iteration: their minima fold into v_min3 without
+        // canonicalising each accumulator per element (111 -> 97.5 VALU per
+        // element at 8 candidates per lane)
+#pragma unroll 2
         for (int e = 0; e < f.n; ++e) {
             const TetC c = s_tc[e];
             const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
@@ -378,6 +383,12 @@ __device__ __forceinline__ void hex_ring_add_cpl(HexAcc* acc, const HexLds* s_hx
                                                  const float* t3, float py, float pz)
 {
     const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
+    // two elements per iteration at 8 candidates per lane: the minima of
+    // consecutive elements fold into v_min3 without canonicalising the
+    // accumulator each time (533 -> 506 VALU per element); at fewer
+    // candidates per lane the register growth costs more than it saves
+    constexpr int kUnrollE = CPL >= 8 ? 2 : 1;
+#pragma unroll kUnrollE
     for (int e = 0; e < n; ++e) {
 #pragma unroll
         for (int sIdx = 0; sIdx < NS; ++sIdx) {
@@ -537,9 +548,11 @@ __device__ void flat_vertex(const SearchArgs& a, int vid, TriC* s_tr, RedLds<W>&
 }
 
 // one vertex per block, vertices [cStart, cEnd) of one colour class.  The
-// tet search asks for 7 waves per SIMD (72 VGPRs, no spill): a colour class of
-// the 40^3 cube (~7.4k vertices) then fits the 256 CUs in about one round
-// instead of 1.45 rounds at the unconstrained 95 VGPRs.
+// tet search asks for at least 5 waves per SIMD (launch bounds' second
+// argument; 89 VGPRs, no spill): a colour class of the 40^3 cube (~7.4k
+// vertices) runs in ~1.45 rounds of the 256 CUs' 5 x 4 wave slots.  6 or 7
+// waves per SIMD spill (7 / 18 registers) with the element loop taking two
+// elements per iteration.
 #ifndef VSDMO_TET_MIN_WAVES
 #define VSDMO_TET_MIN_WAVES 5
 #endif
@@ -1057,19 +1070,29 @@ __device__ void tet_window(const SearchArgs& a, const LearnArgs& la, int ringCap
                 t3[i] = 3.0f * px[i];
             }
             const float Qyz = 3.0f * fmaf(pz, pz, py * py);
-            TetAcc acc[CPL], acc0 = tet_acc0();
+            TetAcc acc[CPL];
     #pragma unroll
             for (int i = 0; i < CPL; ++i) acc[i] = tet_acc0();
-            for (int e = 0; e < f.n; ++e) {
-                const TetC c = tc[e];
-                const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
-                const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+            // one pass over the ring; the first level run also accumulates q0
+            // (later levels and mode 3's window skip it)
+            auto ring = [&](auto withQ0) {
+                constexpr bool WQ = decltype(withQ0)::value;
+                TetAcc acc0 = tet_acc0();
+                // two elements per iteration (the minima fold into v_min3)
+    #pragma unroll 2
+                for (int e = 0; e < f.n; ++e) {
+                    const TetC c = tc[e];
+                    const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+                    const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
     #pragma unroll
-                for (int i = 0; i < CPL; ++i)
-                    tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
-                tet_acc_add(acc0, c.D, c.K);
-            }
-            if (!haveQ0) { q0 = tet_acc_score(acc0, f.is3); qbest = q0; haveQ0 = true; }
+                    for (int i = 0; i < CPL; ++i)
+                        tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
+                    if constexpr (WQ) tet_acc_add(acc0, c.D, c.K);
+                }
+                if constexpr (WQ) { q0 = tet_acc_score(acc0, f.is3); qbest = q0; haveQ0 = true; }
+            };
+            if (!haveQ0) ring(std::true_type{});
+            else ring(std::false_type{});
             float best = -FLT_MAX;
             int bi = 0;
     #pragma unroll
@@ -1220,23 +1243,31 @@ __device__ void hex_window(const SearchArgs& a, const LearnArgs& la, int ringCap
                 t3[i] = 3.0f * px[i];
             }
             const float Q1 = fmaf(pz, pz, py * py), Q3 = 3.0f * Q1;
-            HexAcc acc[CPL], accO = f.acc0;
+            HexAcc acc[CPL];
     #pragma unroll
             for (int i = 0; i < CPL; ++i) acc[i] = f.acc0;
-            for (int e = 0; e < f.n; ++e) {
+            // the first level run also accumulates q0 (later levels and mode
+            // 3's window skip it)
+            auto ring = [&](auto withQ0) {
+                constexpr bool WQ = decltype(withQ0)::value;
+                HexAcc accO = f.acc0;
+                for (int e = 0; e < f.n; ++e) {
     #pragma unroll
-                for (int sIdx = 0; sIdx < (C8 ? 4 : 5); ++sIdx) {
-                    const HexP h = hx[e].hp[sIdx];
-                    const bool a3 = (sIdx == 1 || sIdx == 4);
-                    const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
-                    const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
+                    for (int sIdx = 0; sIdx < (C8 ? 4 : 5); ++sIdx) {
+                        const HexP h = hx[e].hp[sIdx];
+                        const bool a3 = (sIdx == 1 || sIdx == 4);
+                        const float Tyz = fmaf(h.gy, py, fmaf(h.gz, pz, h.D));
+                        const float Ryz = fmaf(h.sy, py, fmaf(h.sz, pz, h.K + (a3 ? Q3 : Q1)));
     #pragma unroll
-                    for (int i = 0; i < CPL; ++i)
-                        hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
-                    hex_acc_add(accO, h.D, h.K);
+                        for (int i = 0; i < CPL; ++i)
+                            hex_acc_add(acc[i], fmaf(h.gx, px[i], Tyz), fmaf(px[i], (a3 ? t3[i] : px[i]) + h.sx, Ryz));
+                        if constexpr (WQ) hex_acc_add(accO, h.D, h.K);
+                    }
                 }
-            }
-            if (!haveQ0) { q0 = hex_acc_score(accO, f.epsD, f.epsF); qbest = q0; haveQ0 = true; }
+                if constexpr (WQ) { q0 = hex_acc_score(accO, f.epsD, f.epsF); qbest = q0; haveQ0 = true; }
+            };
+            if (!haveQ0) ring(std::true_type{});
+            else ring(std::false_type{});
             float best = -FLT_MAX;
             int bi = 0;
     #pragma unroll
