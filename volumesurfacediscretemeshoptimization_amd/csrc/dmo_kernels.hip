@@ -151,16 +151,33 @@ __device__ TetFrame tet_stage(const SearchArgs& a, int vid, bool emin, TetC* s_t
     f.n = a.ringRow[vid + 1] - r0;
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
     float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    for (int e = tid; e < f.n; e += T) {
+    auto extent = [&](float dxs, float dys, float dzs) {
+        const float dx = fabsf(dxs), dy = fabsf(dys), dz = fabsf(dzs);
+        ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+        ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+        ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+        mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+    };
+    // the thread's first element (all of them when the ring has at most T)
+    // keeps its corner offsets in registers for the constants, so the
+    // positions are read from memory once
+    const bool own0 = tid < f.n;
+    float d0[3][3];
+    if (own0) {
+        const int* h = a.ringVal + (size_t)(r0 + tid) * 3;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int v = h[t];
+            d0[t][0] = a.x[v] - f.x0x; d0[t][1] = a.y[v] - f.x0y; d0[t][2] = a.z[v] - f.x0z;
+            extent(d0[t][0], d0[t][1], d0[t][2]);
+        }
+    }
+    for (int e = tid + T; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 3;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-            int v = h[t];
-            float dx = fabsf(a.x[v] - f.x0x), dy = fabsf(a.y[v] - f.x0y), dz = fabsf(a.z[v] - f.x0z);
-            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
-            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
-            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
-            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+            const int v = h[t];
+            extent(a.x[v] - f.x0x, a.y[v] - f.x0y, a.z[v] - f.x0z);
         }
     }
     block_extents<W>(emin, ex, ey, ez, mabs, rd);
@@ -168,7 +185,13 @@ __device__ TetFrame tet_stage(const SearchArgs& a, int vid, bool emin, TetC* s_t
     f.sg = pow2_scale(mabs);
     const float isg = 1.0f / f.sg;
     f.is3 = ((1.0f * isg) * isg) * isg;
-    for (int e = tid; e < f.n; e += T) {
+    if (own0) {
+        f3 p[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) p[t] = { d0[t][0] * f.sg, d0[t][1] * f.sg, d0[t][2] * f.sg };
+        s_tc[tid] = tet_consts(p[0], p[1], p[2]);
+    }
+    for (int e = tid + T; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 3;
         f3 p[3];
 #pragma unroll
@@ -301,15 +324,31 @@ __device__ HexFrame hex_stage(const SearchArgs& a, int vid, bool emin, HexLds* s
     f.x0x = a.x[vid]; f.x0y = a.y[vid]; f.x0z = a.z[vid];
     f.corner8 = a.hexCorner8 != 0;
     float ex = emin ? FLT_MAX : -FLT_MAX, ey = ex, ez = ex, mabs = 0.0f;
-    for (int e = tid; e < f.n; e += T) {
+    auto extent = [&](float dxs, float dys, float dzs) {
+        const float dx = fabsf(dxs), dy = fabsf(dys), dz = fabsf(dzs);
+        ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
+        ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
+        ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
+        mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+    };
+    // the thread's first element keeps its corner offsets in registers for
+    // the constants (the positions are read once; tet_stage)
+    const bool own0 = tid < f.n;
+    float d0[7][3];
+    if (own0) {
+        const int* h = a.ringVal + (size_t)(r0 + tid) * 7;
+#pragma unroll
+        for (int t = 0; t < 7; ++t) {
+            const int v = h[t];
+            d0[t][0] = a.x[v] - f.x0x; d0[t][1] = a.y[v] - f.x0y; d0[t][2] = a.z[v] - f.x0z;
+            extent(d0[t][0], d0[t][1], d0[t][2]);
+        }
+    }
+    for (int e = tid + T; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 7;
         for (int t = 0; t < 7; ++t) {
-            int v = h[t];
-            float dx = fabsf(a.x[v] - f.x0x), dy = fabsf(a.y[v] - f.x0y), dz = fabsf(a.z[v] - f.x0z);
-            ex = emin ? fminf(ex, dx) : fmaxf(ex, dx);
-            ey = emin ? fminf(ey, dy) : fmaxf(ey, dy);
-            ez = emin ? fminf(ez, dz) : fmaxf(ez, dz);
-            mabs = fmaxf(mabs, fmaxf(dx, fmaxf(dy, dz)));
+            const int v = h[t];
+            extent(a.x[v] - f.x0x, a.y[v] - f.x0y, a.z[v] - f.x0z);
         }
     }
     block_extents<W>(emin, ex, ey, ez, mabs, rd);
@@ -317,7 +356,16 @@ __device__ HexFrame hex_stage(const SearchArgs& a, int vid, bool emin, HexLds* s
     f.sg = pow2_scale(mabs);
     const float s2 = f.sg * f.sg, s3 = s2 * f.sg;
     f.epsD = kHexEps * s3; f.epsF = kHexEps * s2;
-    for (int e = tid; e < f.n; e += T) {
+    if (own0) {
+        f3 r[8];
+        r[0] = { 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int t = 0; t < 7; ++t) r[t + 1] = { d0[t][0] * f.sg, d0[t][1] * f.sg, d0[t][2] * f.sg };
+        HexLds L;
+        hex_consts(r, L.cd, L.cF, L.hp);
+        s_hx[tid] = L;
+    }
+    for (int e = tid + T; e < f.n; e += T) {
         const int* h = a.ringVal + (size_t)(r0 + e) * 7;
         f3 r[8];
         r[0] = { 0.0f, 0.0f, 0.0f };
