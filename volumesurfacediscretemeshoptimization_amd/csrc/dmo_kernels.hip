@@ -223,11 +223,13 @@ __device__ __forceinline__ float tet_score(const TetFrame& f, const TetC* s_tc, 
 // The nested grid search from the start position (depth levels of NQ^3
 // candidates) with W waves.  Lane l = j + nq*k of wave w owns the candidates
 // (i, j, k) with i in [w*nq/W, (w+1)*nq/W); candidate index i + nq*l, so the
-// lowest index wins ties as in the reference.  qcur: the start score.
-// Returns whether the vertex moved, and its position in (cx, cy, cz).
+// lowest index wins ties as in the reference.  The start score qcur is
+// accumulated in the first level's element loop (at p = 0, det = D and L =
+// K exactly: tet_score's terms, in ring order).  Returns whether the vertex
+// moved, and its position in (cx, cy, cz).
 template <int NQ, int W>
-__device__ bool tet_levels(const SearchArgs& a, const TetFrame& f, const TetC* s_tc, RedLds<W>& rd, float qcur,
-                           float& cx, float& cy, float& cz)
+__device__ bool tet_levels(const SearchArgs& a, const TetFrame& f, const TetC* s_tc, RedLds<W>& rd, float& cx,
+                           float& cy, float& cz)
 {
     constexpr int CPL = NQ / W;
     static_assert(NQ % W == 0, "nq must be a multiple of the waves per vertex");
@@ -237,6 +239,7 @@ __device__ bool tet_levels(const SearchArgs& a, const TetFrame& f, const TetC* s
     const bool active = lane < NQ * NQ;
     float ds = a.gridScale;
     bool moved = false;
+    float qcur = 0.0f;
     for (int d = 0; d < a.depth; ++d) {
         const float gxM = cx + ds * f.ex, gxm = cx - ds * f.ex;
         const float gyM = cy + ds * f.ey, gym = cy - ds * f.ey;
@@ -256,15 +259,23 @@ __device__ bool tet_levels(const SearchArgs& a, const TetFrame& f, const TetC* s
         // two elements per iteration: their minima fold into v_min3 without
         // canonicalising each accumulator per element (111 -> 97.5 VALU per
         // element at 8 candidates per lane)
+        auto ring = [&](auto withQ0) {
+            constexpr bool WQ = decltype(withQ0)::value;
+            TetAcc acc0 = tet_acc0();
 #pragma unroll 2
-        for (int e = 0; e < f.n; ++e) {
-            const TetC c = s_tc[e];
-            const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
-            const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
+            for (int e = 0; e < f.n; ++e) {
+                const TetC c = s_tc[e];
+                const float Tyz = fmaf(c.nny, py, fmaf(c.nnz, pz, c.D));
+                const float Ryz = fmaf(c.m2sy, py, fmaf(c.m2sz, pz, c.K + Qyz));
 #pragma unroll
-            for (int i = 0; i < CPL; ++i)
-                tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
-        }
+                for (int i = 0; i < CPL; ++i)
+                    tet_acc_add(acc[i], fmaf(c.nnx, px[i], Tyz), fmaf(px[i], t3[i] + c.m2sx, Ryz));
+                if constexpr (WQ) tet_acc_add(acc0, c.D, c.K);
+            }
+            if constexpr (WQ) qcur = tet_acc_score(acc0, f.is3);
+        };
+        if (d == 0) ring(std::true_type{});
+        else ring(std::false_type{});
         float best = -FLT_MAX;
         int bi = 0;
 #pragma unroll
@@ -294,9 +305,8 @@ __device__ void tet_vertex(const SearchArgs& a, int vid, bool emin, TetC* s_tc, 
 {
     const int tid = (W == 1) ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
     const TetFrame f = tet_stage<W>(a, vid, emin, s_tc, rd);
-    const float qcur = tet_score(f, s_tc, 0.0f, 0.0f, 0.0f);
     float cx = f.x0x, cy = f.x0y, cz = f.x0z;
-    if (tet_levels<NQ, W>(a, f, s_tc, rd, qcur, cx, cy, cz) && tid == 0) {
+    if (tet_levels<NQ, W>(a, f, s_tc, rd, cx, cy, cz) && tid == 0) {
         a.xw[vid] = cx; a.yw[vid] = cy; a.zw[vid] = cz;
     }
 }
