@@ -83,16 +83,16 @@ SURF_FLOP = {"tri": 3 * 64 * (57 + 69 * 6 + 59 * 6), "quad": 3 * 64 * (57 + 69 *
 # for the search's own instruction mix (bench.py --calibrate's selection-step
 # probe, 3.81 wave-instructions per CU per ns), from the counter passes of
 # profiles/learned_vs_plain_tet40_hex48.md
-VALU_ISSUE_FRAC = {"k_search_tet": 0.80, "k_search_hex": 0.92, "k_search_tet_learned_def": 0.38,
+VALU_ISSUE_FRAC = {"k_search_tet": 0.81, "k_search_hex": 0.93, "k_search_tet_learned_def": 0.37,
                    "k_search_hex_learned_def": 0.47}
 VALU_ISSUE_BASIS = ("SQ_INSTS_VALU / kernel-trace duration / CUs over the calibrated mixed-instruction issue "
                     "ceiling (profiles/learned_vs_plain_tet40_hex48.md, profiles/valu_issue_calibration.md)")
 SURF_FLOP1D = {"tri": 3 * 8 * (149 + 59 * 6), "quad": 3 * 8 * (149 + 104 * 4)}
 WORKLOADS = {
     "tet40": {"kind": "tet", "flop": FLOP_TET, "cbrt": FULL_EVALS, "kernel": "k_search_tet<8>",
-              "traffic": 9_295_621, "learned_traffic": 6_438_890},
+              "traffic": 9_346_689, "learned_traffic": 6_482_023},
     "hex48": {"kind": "hex", "flop": FLOP_HEX, "cbrt": FULL_EVALS, "kernel": "k_search_hex<8>",
-              "traffic": 16_096_351, "learned_traffic": 10_471_234},
+              "traffic": 16_172_174, "learned_traffic": 10_517_734},
     # the 48^3 hex cube at jitter 0.15h: a valid mesh, so the surface and
     # feature passes run every iteration (the roofline is the inner kernel's)
     "hex48_surface": {"kind": "hex", "flop": FLOP_HEX, "cbrt": FULL_EVALS, "kernel": "k_search_hex<8>",

@@ -1916,9 +1916,11 @@ static void launch_nq(int kind, int w, const SearchArgs& a, int nBlocks, size_t 
 // Non-persistent variant of a round: one block per item, scheduled by the
 // hardware dispatcher (no queue, no atomics), one launch per element type
 // (0 tet, 1 hex, 2 flat tri) with the search body of that type alone, so
-// that each keeps the registers and LDS of its own body.
+// that each keeps the registers and LDS of its own body.  The tet body asks
+// for the plain tet search's occupancy (VSDMO_TET_MIN_WAVES waves per SIMD).
 template <int NQ, int W, int TYPE>
-__global__ __launch_bounds__(64 * W) void k_batch_typed(SearchArgs a, const int* __restrict__ rowHex,
+__global__ __launch_bounds__(64 * W, TYPE == 0 ? VSDMO_TET_MIN_WAVES : 1) void k_batch_typed(SearchArgs a,
+                                                                                        const int* __restrict__ rowHex,
                                                        const int* __restrict__ valHex, const int* __restrict__ items,
                                                        int lo)
 {
