@@ -123,6 +123,39 @@ def test_dmo_positions_parity(oracle, kind, n, iters):
     opt.close()
 
 
+@pytest.mark.parametrize("w", [1, 2])
+def test_tet_ring_beyond_one_wave(oracle, w):
+    """an inner vertex with 80 ring tets (a bipyramid over a 40-gon): at one
+    wave per vertex the ring staging holds one element per lane in registers
+    and loads the 16 elements past the wave's 64 lanes in a second pass; the
+    search bit-exact against the oracle, at 1 and 2 waves per vertex"""
+    k = 40
+    t = 2.0 * np.pi * np.arange(k) / k
+    pts = [[0.07, -0.04, 0.03]] + [[np.cos(a), np.sin(a), 0.0] for a in t] + [[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]]
+    top, bot = k + 1, k + 2
+    cells = []
+    for i in range(k):
+        j = (i + 1) % k
+        cells.append([0, 1 + i, 1 + j, top])
+        cells.append([0, 1 + j, 1 + i, bot])
+    xyz, cells = np.array(pts, np.float32), np.array(cells, np.int32)
+    g = _build("tet", xyz, cells)
+    assert g.counts()["max_ring"] == 2 * k
+    opt = DMOTetClass(g, n_iter=3, fixed_boundary=True, waves_per_vertex=w)
+    o = oracle.OracleMesh("tet", xyz, cells)
+    try:
+        for it in range(3):
+            opt.doIteration()
+            o.iterate()
+            gp = g.positions(original_order=False)
+            assert np.array_equal(gp.view(np.uint32), o.positions().view(np.uint32)), it
+            assert opt.getQuality() == o.min_quality(), it
+        assert (g.positions()[0] != xyz[0]).any()   # the centre moved
+    finally:
+        opt.close()
+        g.close()
+
+
 def test_flat_four_vertices_per_wave_parity(oracle):
     """a 160^2 grid: colour classes of ~6,200 vertices take the flat search
     with 4 vertices per wave (classes of 4,096 and more); positions bit-exact
